@@ -1,0 +1,139 @@
+"""Benchmark: binned pair evaluations per second of the split-precision pair kernel against the plain
+FP64 kernel, on the 1M-galaxy wg+/w++ periodic box (L = 500 Mpc/h; 70% uniform + 30% in 5000
+clumps; shapes and lognormal weights; 20 log rp bins in [0.1, 100); 40 pi bins in [-100, 100);
+27 jackknife regions).  One step is one full DD pass of corr_pc's which_corr=2 run (the SD pass:
+shape_shape_corel, 1M outer x 1M inner), from catalogs resident on the device to the finished
+histogram in host memory: cell lists, both pair kernels (full sample + jackknife touch) and the
+assembly of the jackknife layout.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# algorithmic FP32-VALU flops per binned pair of the shape-shape kind (w++): distances and bins 12,
+# rotation/projection of one shape 21 (wg+), second shape, products and error squares 20
+FLOPS_PER_PAIR = {"DD": 12, "SD": 33, "SS": 53}
+PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-stride", type=int, default=250,
+                    help="CPU baseline: every n-th outer galaxy of the same pass")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is described "
+              f"in DESIGN.md and is not launched.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from corr_pc_amd import synth
+    from corr_pc_amd.engine import PREC_FP64, PREC_SPLIT, Engine
+    from corr_pc_amd.passes import config_datainfo, setup_pass
+
+    t = time.perf_counter()
+    cats = synth.catalogs(1)
+    cfg = cats["config"]
+    di = config_datainfo(cfg)
+    outer, inner, p_split, _ = setup_pass("SD", cats["D"], cats["S"], di, precision=PREC_SPLIT)
+    _, _, p_fp64, _ = setup_pass("SD", cats["D"], cats["S"], di, precision=PREC_FP64)
+    log(f"catalogs ready in {time.perf_counter() - t:.1f}s: outer {outer.n}, inner {inner.n}")
+
+    eng = Engine(0)
+    eng.upload(outer)
+    eng.upload(inner)
+
+    def run(p, steps, label):
+        stats = []
+        h = None
+        t0 = time.perf_counter()
+        for k in range(steps):
+            h, st = eng.count(outer, inner, p)
+            stats.append(st)
+            log(f"{label} step {k}: {st['ms_total']:.1f} ms (primary {st['ms_primary']:.1f} ms, touch "
+                f"{st['ms_touch']:.1f} ms, build {st['ms_build']:.1f} ms), binned {st['n_binned']}")
+        return time.perf_counter() - t0, stats, h
+
+    run(p_split, args.warmup, "warmup split")
+    dt, st_split, h_split = run(p_split, args.steps, "split")
+    ms_step = dt / args.steps * 1e3
+    n_binned = st_split[-1]["n_binned"]
+    value = n_binned / (dt / args.steps)
+    ms_primary = float(np.mean([s["ms_primary"] for s in st_split]))
+
+    fp64 = None
+    if not args.no_fp64:
+        run(p_fp64, 1, "warmup fp64")
+        dt64, st64, h64 = run(p_fp64, args.steps, "fp64")
+        v64 = st64[-1]["n_binned"] / (dt64 / args.steps)
+        fp64 = {"value": v64, "ms_per_step": dt64 / args.steps * 1e3,
+                "ms_primary": float(np.mean([s["ms_primary"] for s in st64])),
+                "counts_identical": bool(np.array_equal(h64.num, h_split.num))}
+
+    achieved = FLOPS_PER_PAIR["SS"] * n_binned / (ms_primary * 1e-3) / 1e12
+    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_TFLOPS, "traffic": None,
+                "kernel": "pair_kernel<shape_shape, split, primary>",
+                "flops_per_pair": FLOPS_PER_PAIR["SS"], "ms_per_launch": ms_primary}
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        sub = outer.subset(np.arange(0, outer.n, args.cpu_stride))
+        t0 = time.perf_counter()
+        _, cst = oracle.count(sub, inner, p_split, n_threads=threads)
+        cdt = time.perf_counter() - t0
+        cpu = {"value": cst["n_binned"] / cdt, "unit": "pair evals/s", "cores": threads, "kind": "port",
+               "sample": f"same SD pass, every {args.cpu_stride}th outer galaxy ({sub.n} of {outer.n}) against all "
+                         f"{inner.n} inner galaxies: {cst['n_binned']} binned pairs in {cdt:.1f} s"}
+        log(f"cpu baseline: {cpu['value']:.3e} pairs/s on {threads} threads")
+
+    out = {
+        "metric": "pair evals/sec: split-FP32 vs FP64 GPU kernel, wg+ 1M-gal box, bit-exact bins",
+        "value": value,
+        "unit": "pair evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "wg+/w++ periodic box L=500, 1M galaxies (30% in 5000 clumps) x 1M, SD pass "
+                               "(shape_shape, estimator 0), 20 log rp bins [0.1,100) x 40 pi bins [-100,100), "
+                               "27 jackknife regions",
+                   "global_batch": outer.n, "parallelism": "single GPU"},
+        "n_binned": n_binned,
+        "n_candidates": st_split[-1]["n_candidates"],
+        "n_fp64_reeval": st_split[-1]["n_fp64_reeval"],
+        "fp64": fp64,
+        "split_vs_fp64": (value / fp64["value"]) if fp64 else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,850 @@
+// cpc_api.cpp -- host side of the C ABI declared in include/cpc.h.
+//
+// Responsibilities: argument validation with the reference's preconditions, the data_info-derived
+// binning quantities (initialization.cpp:305-311), the rp thresholds calibrated against the host
+// libm, cell-list orchestration on the device, work lists, the pair-kernel launches and the final
+// assembly of the reference's histogram layout (full sample + jackknife regions).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/cpc.h"
+#include "cpc_internal.h"
+
+using namespace cpc;
+
+namespace {
+
+const char *kVersion = "corr_pc_amd 0.1 (gfx950)";
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t cap = 0;  // elements
+  hipError_t ensure(size_t n) {
+    if (n <= cap && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc((void **)&p, bytes);
+    if (e == hipSuccess) cap = std::max<size_t>(n, 1);
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct Status {
+  cpc_status code = CPC_OK;
+  std::string msg;
+};
+
+// One catalog ordered for a pass: records in (column, region, coin, z) order plus group tables.
+struct Side {
+  int64_t n_incl = 0;
+  DevBuf<PRec> rec;
+  DevBuf<FRec> fr;
+  DevBuf<Group> d_groups;
+  DevBuf<int> d_colbeg;
+  std::vector<Group> groups;
+  std::vector<int> colbeg;           // [ncol+1]
+  std::vector<int64_t> colcount;     // galaxies per column
+  void release() {
+    rec.release();
+    fr.release();
+    d_groups.release();
+    d_colbeg.release();
+  }
+};
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// ---------------------------------------------------------------------------------------------
+// rp bin thresholds.  bin_value restates corel_bin's rp index (corels.cpp:31-34) applied to
+// rp_calc_PB's sqrt (PB_calcs.cpp:97), kept as a double so out-of-range values stay ordered.
+
+double bin_value(double s, double rmin, double rsize, int lin) {
+  double rp = std::sqrt(s);
+  if (lin) return std::floor((rp - rmin) / rsize);
+  return std::floor(std::log10(rp / rmin) / rsize);
+}
+
+uint64_t dbits(double d) {
+  uint64_t u;
+  std::memcpy(&u, &d, 8);
+  return u;
+}
+double from_bits(uint64_t u) {
+  double d;
+  std::memcpy(&d, &u, 8);
+  return d;
+}
+
+// Smallest non-negative double s with bin_value(s) >= k, by bisection over the ordered bit
+// patterns of non-negative doubles.  Then a scan of `window` neighbours on each side confirms that
+// the bin index is monotone around the edge (it is a composition of correctly rounded sqrt and
+// division with the host log10).
+bool rp_threshold(double rmin, double rsize, int lin, int k, int window, double *out) {
+  uint64_t lo = 0, hi = dbits(INFINITY);
+  while (lo < hi) {
+    uint64_t mid = lo + (hi - lo) / 2;
+    if (bin_value(from_bits(mid), rmin, rsize, lin) >= k)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  for (int d = 1; d <= window; ++d) {
+    if (lo + d <= dbits(INFINITY) && !(bin_value(from_bits(lo + d), rmin, rsize, lin) >= k)) return false;
+    if (lo >= (uint64_t)d && (bin_value(from_bits(lo - d), rmin, rsize, lin) >= k)) return false;
+  }
+  *out = from_bits(lo);
+  return true;
+}
+
+struct Derived {
+  double psize, rsize;
+};
+
+Derived derive(const cpc_params &p) {
+  Derived d;
+  d.psize = (p.pi_max - p.pi_min) / (p.n_pi);  // initialization.cpp:305
+  if (p.lin_bin)
+    d.rsize = (p.rp_max - p.rp_min) / p.n_rp;   // initialization.cpp:310
+  else
+    d.rsize = (std::log10(p.rp_max) - std::log10(p.rp_min)) / p.n_rp;  // initialization.cpp:311
+  return d;
+}
+
+Status thresholds(const cpc_params &p, int window, std::vector<double> &thr) {
+  Derived d = derive(p);
+  thr.assign(p.n_rp + 1, 0.0);
+  for (int k = 0; k <= p.n_rp; ++k) {
+    if (!rp_threshold(p.rp_min, d.rsize, p.lin_bin, k, window, &thr[k]))
+      return {CPC_ERR_UNSUPPORTED, "rp bin index is not monotone near edge " + std::to_string(k) +
+                                       " with this host libm"};
+  }
+  return {};
+}
+
+float round_up_f(double v) {
+  float f = (float)v;
+  if ((double)f < v) f = std::nextafter(f, INFINITY);
+  return f;
+}
+float round_down_f(double v) {
+  float f = (float)v;
+  if ((double)f > v) f = std::nextafter(f, -INFINITY);
+  return f;
+}
+
+hipError_t check(hipError_t e, Status &st, const char *what) {
+  if (e != hipSuccess && st.code == CPC_OK) {
+    st.code = (e == hipErrorOutOfMemory) ? CPC_ERR_NOMEM : CPC_ERR_DEVICE;
+    st.msg = std::string(what) + ": " + hipGetErrorString(e);
+  }
+  return e;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+
+struct cpc_catalog {
+  int64_t n = 0;
+  double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_w = nullptr, *d_e1 = nullptr, *d_e2 = nullptr,
+         *d_phi = nullptr, *d_jkp = nullptr, *d_inc = nullptr;
+  int32_t *d_jk = nullptr;
+  double *d_zsorted = nullptr;  // every galaxy's z ascending (only when the upload order is not sorted)
+  bool z_sorted = true, has_e = false;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  int32_t jk_lo = 0, jk_hi = -1;
+  bool has_jk = false;
+  std::vector<double> h_z, h_inc;  // for the first-included-galaxy rule
+};
+
+struct cpc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  std::string err;
+  size_t max_lds = 65536;
+  int n_cu = 256;
+  void *sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  double col_frac = 0.0;
+  int item_size = 0, block_threads = 0;
+  // scratch
+  DevBuf<uint32_t> keys_a, keys_b;
+  DevBuf<int32_t> vals_a, vals_b, order;
+  DevBuf<double> zk_a, zk_b;
+  DevBuf<double> thr;
+  DevBuf<float> thr_lo, thr_hi;
+  DevBuf<int2> nbr;
+  DevBuf<Item> items;
+  DevBuf<double> accp, acct;
+  DevBuf<unsigned long long> accpn, acctn, counters;
+  Side side_o, side_i;
+  std::map<std::tuple<double, double, int, int>, std::vector<double>> thr_cache;
+};
+
+namespace {
+
+cpc_status fail(cpc_ctx *ctx, const Status &s) {
+  if (ctx) ctx->err = s.msg;
+  return s.code;
+}
+cpc_status fail(cpc_ctx *ctx, cpc_status code, const std::string &msg) { return fail(ctx, Status{code, msg}); }
+
+// Build one side: keys, stable sort, gather, group tables.  `zfirst_plus_L` feeds the PBz>0 flag.
+Status build_side(cpc_ctx *ctx, const cpc_catalog *c, int nc, double h, int nreg, bool jk_on,
+                  const cpc_params &p, bool has_zfirst, double zfirst_plus_L, Side &S) {
+  Status st;
+  hipStream_t s = ctx->stream;
+  const int64_t n = c->n;
+  S.groups.clear();
+  S.colbeg.assign((size_t)nc * nc + 1, 0);
+  S.colcount.assign((size_t)nc * nc, 0);
+  S.n_incl = 0;
+  if (n == 0) {
+    check(S.d_colbeg.ensure(S.colbeg.size()), st, "alloc");
+    check(hipMemcpyAsync(S.d_colbeg.p, S.colbeg.data(), S.colbeg.size() * sizeof(int), hipMemcpyHostToDevice, s),
+          st, "copy");
+    check(S.d_groups.ensure(1), st, "alloc");
+    return st;
+  }
+  if (check(ctx->keys_a.ensure(n), st, "alloc") || check(ctx->keys_b.ensure(n), st, "alloc") ||
+      check(ctx->vals_a.ensure(n), st, "alloc") || check(ctx->vals_b.ensure(n), st, "alloc"))
+    return st;
+  const int32_t *order = nullptr;
+  if (!c->z_sorted) {
+    // galaxies into ascending z first, so that the stable key sort keeps z order inside a group
+    if (check(ctx->order.ensure(n), st, "alloc") || check(ctx->zk_b.ensure(n), st, "alloc")) return st;
+    std::vector<int32_t> iota(n);
+    std::iota(iota.begin(), iota.end(), 0);
+    check(hipMemcpyAsync(ctx->vals_a.p, iota.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s), st, "copy");
+    check(sort_f64(ctx->sort_tmp, ctx->sort_tmp_bytes, c->d_z, ctx->zk_b.p, ctx->vals_a.p, ctx->order.p, n, s), st,
+          "sort z");
+    if (st.code) return st;
+    order = ctx->order.p;
+  }
+  KeyArgs ka{c->d_x, c->d_y, c->d_inc, c->d_jk, jk_on ? c->d_jkp : nullptr, n, nc, h, nreg, p.include_prob_max};
+  check(launch_keys(ka, order, ctx->keys_a.p, ctx->vals_a.p, s), st, "keys");
+  check(sort_u32(ctx->sort_tmp, ctx->sort_tmp_bytes, ctx->keys_a.p, ctx->keys_b.p, ctx->vals_a.p, ctx->vals_b.p, n, s),
+        st, "sort keys");
+  if (st.code) return st;
+  std::vector<uint32_t> hkeys(n);
+  std::vector<int32_t> hidx(n);
+  check(hipMemcpyAsync(hkeys.data(), ctx->keys_b.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s), st, "copy");
+  check(hipMemcpyAsync(hidx.data(), ctx->vals_b.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, s), st, "copy");
+  check(hipStreamSynchronize(s), st, "sync");
+  if (st.code) return st;
+  int64_t ni = std::lower_bound(hkeys.begin(), hkeys.end(), 0xffffffffu) - hkeys.begin();
+  S.n_incl = ni;
+  // groups: runs of equal keys
+  for (int64_t a = 0; a < ni;) {
+    int64_t b = a + 1;
+    while (b < ni && hkeys[b] == hkeys[a]) ++b;
+    uint32_t key = hkeys[a];
+    Group g{};
+    g.start = (int32_t)a;
+    g.end = (int32_t)b;
+    g.coin = key & 1u;
+    g.region = (int32_t)((key >> 1) % (uint32_t)nreg);
+    g.col = (int32_t)((key >> 1) / (uint32_t)nreg);
+    g.zlo = c->h_z[hidx[a]];
+    g.zhi = c->h_z[hidx[b - 1]];
+    S.groups.push_back(g);
+    S.colcount[g.col] += b - a;
+    a = b;
+  }
+  // first group of each column
+  {
+    size_t gi = 0;
+    for (int col = 0; col < nc * nc; ++col) {
+      S.colbeg[col] = (int)gi;
+      while (gi < S.groups.size() && S.groups[gi].col == col) ++gi;
+    }
+    S.colbeg[(size_t)nc * nc] = (int)S.groups.size();
+  }
+  if (check(S.rec.ensure(ni), st, "alloc") || check(S.fr.ensure(ni), st, "alloc") ||
+      check(S.d_groups.ensure(S.groups.size()), st, "alloc") || check(S.d_colbeg.ensure(S.colbeg.size()), st, "alloc"))
+    return st;
+  GatherArgs ga{c->d_x, c->d_y,  c->d_z, c->d_w, c->d_e1, c->d_e2, c->d_phi, ni, nc, h,
+                p.box_size, p.pi_min, p.pi_max, p.rp_max, zfirst_plus_L, has_zfirst ? 1 : 0};
+  check(launch_gather(ga, ctx->vals_b.p, S.rec.p, S.fr.p, s), st, "gather");
+  if (!S.groups.empty())
+    check(hipMemcpyAsync(S.d_groups.p, S.groups.data(), S.groups.size() * sizeof(Group), hipMemcpyHostToDevice, s),
+          st, "copy");
+  check(hipMemcpyAsync(S.d_colbeg.p, S.colbeg.data(), S.colbeg.size() * sizeof(int), hipMemcpyHostToDevice, s), st,
+        "copy");
+  check(hipStreamSynchronize(s), st, "sync");
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *cpc_version(void) { return kVersion; }
+
+const char *cpc_last_error(const cpc_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+cpc_status cpc_init(cpc_ctx **out, int device) {
+  if (!out) return CPC_ERR_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CPC_ERR_DEVICE;
+  if (device < 0 || device >= ndev) return CPC_ERR_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return CPC_ERR_DEVICE;
+  cpc_ctx *ctx = new cpc_ctx();
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    ctx->n_cu = prop.multiProcessorCount;
+    ctx->max_lds = prop.sharedMemPerBlock;
+    if (prop.maxSharedMemoryPerMultiProcessor > ctx->max_lds) ctx->max_lds = prop.maxSharedMemoryPerMultiProcessor;
+  }
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess || hipEventCreate(&ctx->ev2) != hipSuccess) {
+    delete ctx;
+    return CPC_ERR_DEVICE;
+  }
+  *out = ctx;
+  return CPC_OK;
+}
+
+void cpc_destroy(cpc_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->keys_a.release();
+  ctx->keys_b.release();
+  ctx->vals_a.release();
+  ctx->vals_b.release();
+  ctx->order.release();
+  ctx->zk_a.release();
+  ctx->zk_b.release();
+  ctx->thr.release();
+  ctx->thr_lo.release();
+  ctx->thr_hi.release();
+  ctx->nbr.release();
+  ctx->items.release();
+  ctx->accp.release();
+  ctx->acct.release();
+  ctx->accpn.release();
+  ctx->acctn.release();
+  ctx->counters.release();
+  ctx->side_o.release();
+  ctx->side_i.release();
+  if (ctx->sort_tmp) (void)hipFree(ctx->sort_tmp);
+  (void)hipEventDestroy(ctx->ev0);
+  (void)hipEventDestroy(ctx->ev1);
+  (void)hipEventDestroy(ctx->ev2);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+cpc_status cpc_set_tuning(cpc_ctx *ctx, double col_frac, int item_size, int block_threads) {
+  if (!ctx) return CPC_ERR_INVALID;
+  if (col_frac < 0 || item_size < 0 || block_threads < 0) return fail(ctx, CPC_ERR_INVALID, "negative tuning value");
+  if (block_threads && (block_threads % 64 || block_threads > 256))
+    return fail(ctx, CPC_ERR_INVALID, "block_threads must be a multiple of 64 and at most 256");
+  ctx->col_frac = col_frac;
+  ctx->item_size = item_size;
+  ctx->block_threads = block_threads;
+  return CPC_OK;
+}
+
+cpc_status cpc_catalog_upload(cpc_ctx *ctx, const cpc_catalog_desc *d, cpc_catalog **out) {
+  if (!ctx || !d || !out) return ctx ? fail(ctx, CPC_ERR_INVALID, "null argument") : CPC_ERR_INVALID;
+  *out = nullptr;
+  if (d->n < 0 || d->n > (int64_t)0x7fffffff - 1)
+    return fail(ctx, CPC_ERR_INVALID, "catalog size must be in [0, 2^31-1)");
+  if (d->n > 0 && (!d->x || !d->y || !d->z)) return fail(ctx, CPC_ERR_INVALID, "x, y and z are required");
+  if ((d->e1 == nullptr) != (d->e2 == nullptr)) return fail(ctx, CPC_ERR_INVALID, "e1 and e2 must both be given");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, CPC_ERR_DEVICE, "hipSetDevice failed");
+  const int64_t n = d->n;
+  cpc_catalog *c = new cpc_catalog();
+  c->n = n;
+  c->has_e = d->e1 != nullptr;
+  c->has_jk = d->jk != nullptr;
+  Status st;
+  auto up = [&](const double *src, double **dst) {
+    if (!src || n == 0) return;
+    if (check(hipMalloc((void **)dst, n * sizeof(double)), st, "alloc")) return;
+    check(hipMemcpyAsync(*dst, src, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream), st, "copy");
+  };
+  up(d->x, &c->d_x);
+  up(d->y, &c->d_y);
+  up(d->z, &c->d_z);
+  up(d->w, &c->d_w);
+  up(d->e1, &c->d_e1);
+  up(d->e2, &c->d_e2);
+  up(d->phi, &c->d_phi);
+  up(d->jk_prob, &c->d_jkp);
+  up(d->include_prob, &c->d_inc);
+  if (d->jk && n > 0) {
+    if (!check(hipMalloc((void **)&c->d_jk, n * sizeof(int32_t)), st, "alloc"))
+      check(hipMemcpyAsync(c->d_jk, d->jk, n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream), st, "copy");
+  }
+  if (n > 0) {
+    const double *v[3] = {d->x, d->y, d->z};
+    for (int a = 0; a < 3; ++a) {
+      double lo = v[a][0], hi = v[a][0];
+      for (int64_t i = 1; i < n; ++i) {
+        lo = std::min(lo, v[a][i]);
+        hi = std::max(hi, v[a][i]);
+      }
+      if (!(lo == lo) || !(hi == hi)) st = {CPC_ERR_INVALID, "NaN coordinate"};
+      c->lo[a] = lo;
+      c->hi[a] = hi;
+    }
+    for (int64_t i = 1; i < n; ++i)
+      if (d->z[i] < d->z[i - 1]) {
+        c->z_sorted = false;
+        break;
+      }
+    c->h_z.assign(d->z, d->z + n);
+    if (d->include_prob) c->h_inc.assign(d->include_prob, d->include_prob + n);
+    if (d->jk) {
+      c->jk_lo = *std::min_element(d->jk, d->jk + n);
+      c->jk_hi = *std::max_element(d->jk, d->jk + n);
+    }
+    if (!c->z_sorted && st.code == CPC_OK) {
+      std::vector<double> zs(d->z, d->z + n);
+      std::sort(zs.begin(), zs.end());
+      if (!check(hipMalloc((void **)&c->d_zsorted, n * sizeof(double)), st, "alloc"))
+        check(hipMemcpy(c->d_zsorted, zs.data(), n * sizeof(double), hipMemcpyHostToDevice), st, "copy");
+    }
+  }
+  check(hipStreamSynchronize(ctx->stream), st, "sync");
+  if (st.code != CPC_OK) {
+    cpc_catalog_free(ctx, c);
+    return fail(ctx, st);
+  }
+  *out = c;
+  return CPC_OK;
+}
+
+void cpc_catalog_free(cpc_ctx *ctx, cpc_catalog *c) {
+  if (!c) return;
+  if (ctx) (void)hipSetDevice(ctx->device);
+  double *bufs[] = {c->d_x, c->d_y, c->d_z, c->d_w, c->d_e1, c->d_e2, c->d_phi, c->d_jkp, c->d_inc, c->d_zsorted};
+  for (double *b : bufs)
+    if (b) (void)hipFree(b);
+  if (c->d_jk) (void)hipFree(c->d_jk);
+  delete c;
+}
+
+cpc_status cpc_rand_stream(uint32_t seed, int64_t skip, int64_t n, double *out) {
+  if (skip < 0 || n < 0 || (n > 0 && !out)) return CPC_ERR_INVALID;
+  srand(seed);
+  for (int64_t k = 0; k < skip; ++k) (void)rand();
+  for (int64_t k = 0; k < n; ++k) out[k] = rand() / (RAND_MAX + 1.);
+  return CPC_OK;
+}
+
+cpc_status cpc_rp_thresholds(const cpc_params *p, double *thr) {
+  if (!p || !thr || p->n_rp <= 0) return CPC_ERR_INVALID;
+  std::vector<double> t;
+  Status st = thresholds(*p, 64, t);
+  if (st.code) return st.code;
+  std::copy(t.begin(), t.end(), thr);
+  return CPC_OK;
+}
+
+cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci, const cpc_params *pp, cpc_hist *out,
+                     cpc_stats *stats) {
+  auto t_start = std::chrono::steady_clock::now();
+  if (!ctx) return CPC_ERR_INVALID;
+  if (!co || !ci || !pp || !out) return fail(ctx, CPC_ERR_INVALID, "null argument");
+  const cpc_params p = *pp;
+  // ---- validation (the reference's own preconditions and the engine's)
+  if (p.geometry != CPC_GEOM_BOX) return fail(ctx, CPC_ERR_UNSUPPORTED, "only the periodic-box geometry is built");
+  if (!(p.box_size > 0)) return fail(ctx, CPC_ERR_INVALID, "box_size must be > 0");
+  if (p.n_rp <= 0 || p.n_pi <= 0) return fail(ctx, CPC_ERR_INVALID, "n_rp and n_pi must be > 0");
+  if (!(p.rp_max > 0) || p.rp_min > p.rp_max)
+    return fail(ctx, CPC_ERR_INVALID, "Bad Bin limits");  // initialization.cpp:299-303
+  if (!(p.pi_max > p.pi_min)) return fail(ctx, CPC_ERR_INVALID, "pi_max must exceed pi_min");
+  if (!p.lin_bin && !(p.rp_min > 0)) return fail(ctx, CPC_ERR_INVALID, "log rp bins need rp_min > 0");
+  if (2 * p.rp_max >= p.box_size || p.pi_max - p.pi_min >= p.box_size)
+    return fail(ctx, CPC_ERR_UNSUPPORTED, "the rp and pi windows must be narrower than the box");
+  if (p.kind < CPC_KIND_DD || p.kind > CPC_KIND_SHAPE_SHAPE) return fail(ctx, CPC_ERR_INVALID, "unknown kind");
+  if (p.kind != CPC_KIND_DD && p.g_with_shape != 1 && p.g_with_shape != 2)
+    return fail(ctx, CPC_ERR_INVALID, "g_with_shape must be 1 or 2 for shape kinds");
+  if (p.do_jk != 0 && p.do_jk != 1)
+    return fail(ctx, CPC_ERR_UNSUPPORTED, "do_jk=2 (region pairs) is not supported");
+  if (p.precision != CPC_PREC_SPLIT && p.precision != CPC_PREC_FP64)
+    return fail(ctx, CPC_ERR_INVALID, "unknown precision");
+  const bool jk_on = p.do_jk == 1 && p.n_jk > 0;
+  const int nreg = jk_on ? p.n_jk : 1;
+  if (jk_on) {
+    for (const cpc_catalog *c : {co, ci}) {
+      if (c->n > 0 && !c->has_jk) return fail(ctx, CPC_ERR_INVALID, "jackknife on but a catalog has no jk array");
+      if (c->n > 0 && (c->jk_lo < 0 || c->jk_hi >= p.n_jk))
+        return fail(ctx, CPC_ERR_INVALID, "jk region outside [0, n_jk)");
+    }
+  }
+  if (p.kind == CPC_KIND_SHAPE_SHAPE && ((co->n && !co->has_e) || (ci->n && !ci->has_e)))
+    return fail(ctx, CPC_ERR_INVALID, "shape_shape needs e1/e2 on both catalogs");
+  if (p.kind == CPC_KIND_SHAPE_DENSITY) {
+    const cpc_catalog *sc = p.g_with_shape == 2 ? ci : co;
+    if (sc->n && !sc->has_e) return fail(ctx, CPC_ERR_INVALID, "shape_density needs e1/e2 on the shape side");
+  }
+  for (const cpc_catalog *c : {co, ci})
+    for (int a = 0; a < 3; ++a)
+      if (c->n > 0 && (c->lo[a] < 0 || c->hi[a] > p.box_size))
+        return fail(ctx, CPC_ERR_INVALID, "coordinates must lie in [0, box_size]");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, CPC_ERR_DEVICE, "hipSetDevice failed");
+
+  const int R = jk_on ? 1 + p.n_jk : 1;
+  const int ncell = p.n_rp * p.n_pi;
+  const Derived dv = derive(p);
+  Status st;
+  hipStream_t s = ctx->stream;
+
+  // ---- thresholds (cached per binning)
+  auto tkey = std::make_tuple(p.rp_min, p.rp_max, p.n_rp, p.lin_bin);
+  auto tit = ctx->thr_cache.find(tkey);
+  if (tit == ctx->thr_cache.end()) {
+    std::vector<double> thr;
+    st = thresholds(p, 256, thr);
+    if (st.code) return fail(ctx, st);
+    tit = ctx->thr_cache.emplace(tkey, thr).first;
+  }
+  const std::vector<double> &thr = tit->second;
+
+  // ---- kind, slots
+  int kind_code;  // kernel kind: 0 DD, 1 SD, 2 SS est 0, 3 SS est != 0
+  if (p.kind == CPC_KIND_DD)
+    kind_code = 0;
+  else if (p.kind == CPC_KIND_SHAPE_DENSITY)
+    kind_code = 1;
+  else
+    kind_code = p.estimator == 0 ? 2 : 3;
+  static const int kNV[4] = {0, 3, 5, 6};
+  static const int kValField[4][6] = {{0}, {1, 2, 6}, {1, 3, 4, 5, 6}, {1, 2, 3, 4, 5, 6}};
+  const int NV = kNV[kind_code];
+  const int NSP = 1 + 2 * NV, NST = 1 + NV;
+  const bool split = p.precision == CPC_PREC_SPLIT;
+  const int threads = ctx->block_threads ? ctx->block_threads : 256;
+  const size_t lds_p = pair_lds_bytes(ncell, NSP, p.n_rp);
+  if (lds_p > ctx->max_lds)
+    return fail(ctx, CPC_ERR_UNSUPPORTED,
+                "histogram of " + std::to_string(ncell) + " cells does not fit in LDS (" + std::to_string(lds_p) +
+                    " bytes)");
+
+  // ---- grid of columns
+  const double L = p.box_size;
+  const int64_t nbig = std::max(co->n, ci->n);
+  int nc;
+  {
+    double frac = ctx->col_frac > 0 ? ctx->col_frac : 0.25;
+    nc = std::max(1, (int)std::floor(L / (p.rp_max * frac)));
+    // keep enough galaxies per column for full waves
+    int nc_dense = std::max(1, (int)std::floor(std::sqrt((double)std::max<int64_t>(nbig, 1) / 256.0)));
+    if (ctx->col_frac <= 0) nc = std::min(nc, nc_dense);
+    nc = std::min(nc, 1024);
+  }
+  const double h = L / nc;
+
+  // ---- neighbour column offsets (circle cut at the largest binned separation)
+  const double rcut = std::sqrt(thr[p.n_rp]) * (1 + 1e-9) + 1e-9 * L;
+  const int Rn = (int)std::floor(p.rp_max / h) + 1;
+  std::vector<int2> nbr;
+  for (int dy = -Rn; dy <= Rn; ++dy)
+    for (int dx = -Rn; dx <= Rn; ++dx) {
+      double ex = std::max(0, std::abs(dx) - 1) * h, ey = std::max(0, std::abs(dy) - 1) * h;
+      if (ex * ex + ey * ey <= rcut * rcut) nbr.push_back(make_int2(dx, dy));
+    }
+
+  // ---- first included galaxy of the inner catalog in upload order (PBz>0 sweep, corr.cpp:407-416)
+  bool has_zfirst = false;
+  double zfirst_plus_L = 0;
+  for (int64_t j = 0; j < ci->n; ++j) {
+    double inc = ci->h_inc.empty() ? 0.0 : ci->h_inc[j];
+    if (inc > p.include_prob_max) continue;
+    has_zfirst = true;
+    zfirst_plus_L = ci->h_z[j] + L * 1;
+    break;
+  }
+
+  auto t_build = std::chrono::steady_clock::now();
+  // ---- sides
+  st = build_side(ctx, co, nc, h, nreg, jk_on, p, has_zfirst, zfirst_plus_L, ctx->side_o);
+  if (st.code) return fail(ctx, st);
+  const bool same = (co == ci);
+  if (!same) {
+    st = build_side(ctx, ci, nc, h, nreg, jk_on, p, has_zfirst, zfirst_plus_L, ctx->side_i);
+    if (st.code) return fail(ctx, st);
+  }
+  Side &SO = ctx->side_o;
+  Side &SI = same ? ctx->side_o : ctx->side_i;
+  const double ms_build = ms_since(t_build);
+
+  // ---- work items
+  const int item_size = ctx->item_size ? ctx->item_size : 512;
+  auto make_items = [&](const Side &B, const Side &T, bool touch, std::vector<Item> &items) {
+    std::vector<std::pair<double, Item>> tmp;
+    for (size_t g = 0; g < B.groups.size(); ++g) {
+      const Group &G = B.groups[g];
+      if (touch && G.coin == 0) continue;
+      const int bcx = G.col % nc, bcy = G.col / nc;
+      double ncol = 0;
+      for (const int2 &o : nbr) {
+        int ux = ((bcx + o.x) % nc + nc) % nc, uy = ((bcy + o.y) % nc + nc) % nc;
+        ncol += (double)T.colcount[(size_t)uy * nc + ux];
+      }
+      int target;
+      if (touch)
+        target = G.region;
+      else if (jk_on)
+        target = 2 * G.region + (G.coin ? 1 : 0);
+      else
+        target = 1;
+      for (int f = G.start; f < G.end; f += item_size) {
+        Item it{(int32_t)g, f, std::min(item_size, G.end - f), target};
+        tmp.push_back({(double)it.count * ncol, it});
+      }
+    }
+    std::stable_sort(tmp.begin(), tmp.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+    items.clear();
+    for (auto &t : tmp) items.push_back(t.second);
+  };
+  std::vector<Item> items_p, items_t;
+  make_items(SO, SI, false, items_p);
+  if (jk_on) make_items(SI, SO, true, items_t);
+
+  // ---- device tables
+  std::vector<float> tlo(p.n_rp + 1), thi(p.n_rp + 1);
+  const double two24 = std::ldexp(1.0, -24);
+  const double amax = 1.01 * h + 1e-9 * L;
+  const double omax = (Rn + 1) * h + 1e-9 * L;
+  const double eps_xy = 2.0 * (6 * amax + 2 * omax) * two24;
+  for (int k = 0; k <= p.n_rp; ++k) {
+    const double T = thr[k];
+    const double d = 2.0 * (2 * std::sqrt(2.0) * std::sqrt(T) * eps_xy + 2 * eps_xy * eps_xy + std::ldexp(T, -21));
+    tlo[k] = round_up_f(T + d);
+    thi[k] = round_down_f(T - d);
+  }
+  const double eps_z = 2.0 * 5 * L * two24;
+  const double pabs = std::max(std::fabs(p.pi_min), std::fabs(p.pi_max));
+  double gz = eps_z / dv.psize + 8.0 * (p.n_pi + 2 * pabs / dv.psize + 2) * two24;
+  const bool split_ok = gz < 0.25;
+
+  size_t nP = (size_t)2 * nreg, nT = (size_t)nreg;
+  if (check(ctx->thr.ensure(p.n_rp + 1), st, "alloc") || check(ctx->thr_lo.ensure(p.n_rp + 1), st, "alloc") ||
+      check(ctx->thr_hi.ensure(p.n_rp + 1), st, "alloc") || check(ctx->nbr.ensure(nbr.size()), st, "alloc") ||
+      check(ctx->items.ensure(items_p.size() + items_t.size()), st, "alloc") ||
+      check(ctx->accp.ensure(nP * NSP * ncell), st, "alloc") || check(ctx->accpn.ensure(nP * ncell), st, "alloc") ||
+      check(ctx->acct.ensure(nT * NST * ncell), st, "alloc") || check(ctx->acctn.ensure(nT * ncell), st, "alloc") ||
+      check(ctx->counters.ensure(4), st, "alloc"))
+    return fail(ctx, st);
+  check(hipMemcpyAsync(ctx->thr.p, thr.data(), thr.size() * 8, hipMemcpyHostToDevice, s), st, "copy");
+  check(hipMemcpyAsync(ctx->thr_lo.p, tlo.data(), tlo.size() * 4, hipMemcpyHostToDevice, s), st, "copy");
+  check(hipMemcpyAsync(ctx->thr_hi.p, thi.data(), thi.size() * 4, hipMemcpyHostToDevice, s), st, "copy");
+  check(hipMemcpyAsync(ctx->nbr.p, nbr.data(), nbr.size() * sizeof(int2), hipMemcpyHostToDevice, s), st, "copy");
+  if (!items_p.empty())
+    check(hipMemcpyAsync(ctx->items.p, items_p.data(), items_p.size() * sizeof(Item), hipMemcpyHostToDevice, s), st,
+          "copy");
+  if (!items_t.empty())
+    check(hipMemcpyAsync(ctx->items.p + items_p.size(), items_t.data(), items_t.size() * sizeof(Item),
+                         hipMemcpyHostToDevice, s),
+          st, "copy");
+  check(hipMemsetAsync(ctx->accp.p, 0, nP * NSP * ncell * 8, s), st, "memset");
+  check(hipMemsetAsync(ctx->accpn.p, 0, nP * ncell * 8, s), st, "memset");
+  check(hipMemsetAsync(ctx->acct.p, 0, nT * NST * ncell * 8, s), st, "memset");
+  check(hipMemsetAsync(ctx->acctn.p, 0, nT * ncell * 8, s), st, "memset");
+  check(hipMemsetAsync(ctx->counters.p, 0, 4 * 8, s), st, "memset");
+  if (st.code) return fail(ctx, st);
+
+  KArgs K{};
+  K.L = L;
+  K.pmin = p.pi_min;
+  K.pmax = p.pi_max;
+  K.psize = dv.psize;
+  K.rmax = p.rp_max;
+  K.h = h;
+  K.zmargin = 1e-7 * (L + pabs);
+  K.nc = nc;
+  K.n_rp = p.n_rp;
+  K.n_pi = p.n_pi;
+  K.ncell = ncell;
+  K.gws = p.g_with_shape;
+  K.n_nbr = (int)nbr.size();
+  K.lin = p.lin_bin;
+  K.pmin32 = (float)p.pi_min;
+  K.inv_psize32 = (float)(1.0 / dv.psize);
+  K.gz = (float)gz;
+  K.la = (float)(0.5 * std::log10(2.0) / dv.rsize);
+  K.lb = (float)(-std::log10(p.rp_min > 0 ? p.rp_min : 1.0) / dv.rsize);
+  K.lrmin = (float)p.rp_min;
+  K.linv = (float)(1.0 / dv.rsize);
+  K.thr = ctx->thr.p;
+  K.thr_lo = ctx->thr_lo.p;
+  K.thr_hi = ctx->thr_hi.p;
+  K.nbr = ctx->nbr.p;
+  K.counters = ctx->counters.p;
+  const bool use_split = split && split_ok;
+
+  check(hipEventRecord(ctx->ev0, s), st, "event");
+  if (!items_p.empty()) {
+    KArgs A = K;
+    A.n_items = (int)items_p.size();
+    A.phase_mode = jk_on ? 1 : 0;
+    A.nslots = NSP;
+    A.items = ctx->items.p;
+    A.bgrp = SO.d_groups.p;
+    A.tgrp = SI.d_groups.p;
+    A.tcol = SI.d_colbeg.p;
+    A.acc = ctx->accp.p;
+    A.accnum = ctx->accpn.p;
+    check(launch_pair(kind_code, use_split, false, A, threads, SO.rec.p, SO.fr.p, SI.rec.p, SI.fr.p, s), st,
+          "pair kernel");
+  }
+  check(hipEventRecord(ctx->ev2, s), st, "event");
+  if (!items_t.empty()) {
+    KArgs A = K;
+    A.n_items = (int)items_t.size();
+    A.phase_mode = 2;
+    A.nslots = NST;
+    A.items = ctx->items.p + items_p.size();
+    A.bgrp = SI.d_groups.p;
+    A.tgrp = SO.d_groups.p;
+    A.tcol = SO.d_colbeg.p;
+    A.acc = ctx->acct.p;
+    A.accnum = ctx->acctn.p;
+    check(launch_pair(kind_code, use_split, true, A, threads, SI.rec.p, SI.fr.p, SO.rec.p, SO.fr.p, s), st,
+          "touch kernel");
+  }
+  check(hipEventRecord(ctx->ev1, s), st, "event");
+  // sparse-window diagnostic
+  if (co->n > 0 && ci->n > 0) {
+    SparseArgs sa{co->d_z, co->d_inc, co->n, ci->z_sorted ? ci->d_z : ci->d_zsorted, ci->n,
+                  L, p.pi_min, p.pi_max, p.include_prob_max};
+    check(launch_sparse(sa, ctx->counters.p + 2, s), st, "sparse");
+  }
+  if (st.code) return fail(ctx, st);
+
+  // ---- results
+  std::vector<double> accp(nP * NSP * ncell), acct(nT * NST * ncell);
+  std::vector<unsigned long long> accpn(nP * ncell), acctn(nT * ncell), counters(4);
+  check(hipMemcpyAsync(accp.data(), ctx->accp.p, accp.size() * 8, hipMemcpyDeviceToHost, s), st, "copy");
+  check(hipMemcpyAsync(accpn.data(), ctx->accpn.p, accpn.size() * 8, hipMemcpyDeviceToHost, s), st, "copy");
+  check(hipMemcpyAsync(acct.data(), ctx->acct.p, acct.size() * 8, hipMemcpyDeviceToHost, s), st, "copy");
+  check(hipMemcpyAsync(acctn.data(), ctx->acctn.p, acctn.size() * 8, hipMemcpyDeviceToHost, s), st, "copy");
+  check(hipMemcpyAsync(counters.data(), ctx->counters.p, 4 * 8, hipMemcpyDeviceToHost, s), st, "copy");
+  check(hipStreamSynchronize(s), st, "sync");
+  if (st.code) return fail(ctx, st);
+  float kms = 0, kms_p = 0, kms_t = 0;
+  (void)hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1);
+  (void)hipEventElapsedTime(&kms_p, ctx->ev0, ctx->ev2);
+  (void)hipEventElapsedTime(&kms_t, ctx->ev2, ctx->ev1);
+
+  // ---- assemble the reference layout
+  const size_t cells_all = (size_t)R * ncell;
+  std::fill(out->num, out->num + cells_all, 0);
+  std::fill(out->wt_num, out->wt_num + cells_all, 0.0);
+  std::fill(out->val, out->val + cells_all * CPC_NFIELD, 0.0);
+  std::fill(out->err, out->err + cells_all * CPC_NFIELD, 0.0);
+  std::fill(out->num_rp, out->num_rp + (size_t)R * p.n_rp, 0);
+  auto add_target = [&](const double *acc, const unsigned long long *num, int NS, int t, size_t region, bool errs,
+                        double sign) {
+    for (int c = 0; c < ncell; ++c) {
+      const size_t o = region * ncell + c;
+      const unsigned long long n = num[(size_t)t * ncell + c];
+      if (!n) continue;
+      out->num[o] += sign > 0 ? (int64_t)n : -(int64_t)n;
+      out->wt_num[o] += sign * acc[((size_t)t * NS + 0) * ncell + c];
+      for (int v = 0; v < NV; ++v) {
+        const int f = kValField[kind_code][v];
+        out->val[o * CPC_NFIELD + f] += sign * acc[((size_t)t * NS + 1 + v) * ncell + c];
+        if (errs) out->err[o * CPC_NFIELD + f] += sign * acc[((size_t)t * NS + 1 + NV + v) * ncell + c];
+      }
+    }
+  };
+  // full sample = every primary target
+  for (int t = 0; t < 2 * nreg; ++t) add_target(accp.data(), accpn.data(), NSP, t, 0, true, 1.0);
+  if (jk_on) {
+    for (int r = 0; r < p.n_jk; ++r) {
+      // touch_r (jk.cpp:138-171): vals only; shape vals only when which_corr != 0
+      std::vector<double> tv(ncell * (NV + 1), 0.0);
+      std::vector<int64_t> tn(ncell, 0);
+      for (int src = 0; src < 2; ++src) {
+        const double *acc = src == 0 ? accp.data() : acct.data();
+        const unsigned long long *num = src == 0 ? accpn.data() : acctn.data();
+        const int NS = src == 0 ? NSP : NST;
+        const int t = src == 0 ? 2 * r + 1 : r;
+        for (int c = 0; c < ncell; ++c) {
+          const unsigned long long n = num[(size_t)t * ncell + c];
+          if (!n) continue;
+          tn[c] += (int64_t)n;
+          for (int v = 0; v <= NV; ++v) tv[(size_t)c * (NV + 1) + v] += acc[((size_t)t * NS + v) * ncell + c];
+        }
+      }
+      const size_t reg = (size_t)(r + 1);
+      for (int c = 0; c < ncell; ++c) {
+        const size_t o = reg * ncell + c;
+        const double *tvc = &tv[(size_t)c * (NV + 1)];
+        if (p.jk_mode == CPC_JK_TOUCH) {
+          out->num[o] = tn[c];
+          out->wt_num[o] = tvc[0];
+          if (p.which_corr != 0)
+            for (int v = 0; v < NV; ++v) out->val[o * CPC_NFIELD + kValField[kind_code][v]] = tvc[1 + v];
+        } else {
+          // final_jk_bins: add_bins(full, touch, jk, subtract) over every field (outp.cpp:242-288)
+          out->num[o] = out->num[c] + tn[c] * -1;
+          out->wt_num[o] = out->wt_num[c] + tvc[0] * -1;
+          for (int f = 0; f < CPC_NFIELD; ++f) {
+            out->val[o * CPC_NFIELD + f] = out->val[(size_t)c * CPC_NFIELD + f];
+            out->err[o * CPC_NFIELD + f] = out->err[(size_t)c * CPC_NFIELD + f];
+          }
+          if (p.which_corr != 0)
+            for (int v = 0; v < NV; ++v) {
+              const int f = kValField[kind_code][v];
+              out->val[o * CPC_NFIELD + f] = out->val[(size_t)c * CPC_NFIELD + f] + tvc[1 + v] * -1;
+            }
+        }
+      }
+    }
+  }
+  int64_t nbinned = 0;
+  for (int r = 0; r < R; ++r)
+    for (int k = 0; k < p.n_rp; ++k) {
+      int64_t sum = 0;
+      for (int q = 0; q < p.n_pi; ++q) sum += out->num[((size_t)r * p.n_rp + k) * p.n_pi + q];
+      out->num_rp[(size_t)r * p.n_rp + k] = sum;
+      if (r == 0) nbinned += sum;
+    }
+  if (stats) {
+    stats->n_binned = nbinned;
+    stats->n_candidates = (int64_t)counters[0];
+    stats->n_fp64_reeval = (int64_t)counters[1];
+    stats->n_sparse_windows = (int64_t)counters[2];
+    stats->ms_kernel = kms;
+    stats->ms_primary = kms_p;
+    stats->ms_touch = kms_t;
+    stats->ms_build = ms_build;
+    stats->n_items_primary = (int64_t)items_p.size();
+    stats->ms_total = ms_since(t_start);
+  }
+  ctx->err.clear();
+  return CPC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+// cpc_internal.h -- layouts shared by the host orchestration (cpc_api.cpp) and the HIP kernels
+// (cpc_kernels.hip) of the rp-pi pair engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cpc {
+
+// A galaxy as the pair kernel reads it: one 64-byte record, so that a wave fetches a whole inner
+// galaxy with a single scalar load.  Gathered into (column, region, coin, z) order per pass.
+struct alignas(64) PRec {
+  double x, y, z;      // gal.ra.val_deg, gal.dec.val_deg, gal.redshift
+  double w;            // gal.wt
+  double e1, e2;       // gal.e[0], gal.e[1] (0 when the catalog has no shapes)
+  double phi;          // gal.phi
+  int32_t meta;        // image flags of this galaxy as an outer galaxy (kMeta*)
+  int32_t idx;         // index in the uploaded catalog
+};
+
+// FP32 companion of PRec for the split-precision kernel: x, y relative to the galaxy's column
+// corner, z absolute.
+struct alignas(16) FRec {
+  float xl, yl, zf, pad;
+};
+
+// Image flags of an outer galaxy i, computed exactly as lim_calc_PB (PB_calcs.cpp:30-61) decides
+// which periodic images of the inner catalog it visits (corr.cpp:380-423).
+enum : int32_t {
+  kMetaXm = 1,   // ra_min < 0  -> PBx = -1 visited
+  kMetaXp = 2,   // ra_max > L  -> PBx = +1 visited
+  kMetaYm = 4,
+  kMetaYp = 8,
+  kMetaZm = 16,
+  kMetaZp = 32,  // z_max > L and the sweep from j = 0 does not stop at once (corr.cpp:389-408)
+};
+
+// One run of galaxies sharing (column, jackknife region, coin); z-sorted inside.
+struct alignas(8) Group {
+  int32_t start, end;  // record range
+  int32_t region, coin;
+  int32_t col, pad;    // column index cy*nc + cx
+  double zlo, zhi;     // z of the first and last record
+};
+
+// A unit of work: up to `count` galaxies of one block-side group, starting at record `first`.
+struct alignas(16) Item {
+  int32_t group, first, count, target;
+};
+
+// Histogram slots per pair kind.  The primary pass accumulates [wt, vals..., errs...], the touch
+// pass [wt, vals...].  Quantity codes: 0 = wt, 1..6 = data[f].val, 7..12 = data[f-6].err.
+constexpr int kMaxSlots = 13;
+
+// Kernel arguments (passed by value).
+struct KArgs {
+  double L, pmin, pmax, psize, rmax;
+  double h;            // column width
+  double zmargin;      // slack of the tile z-range search
+  int nc;              // columns per side
+  int n_rp, n_pi, ncell;
+  int gws, n_nbr, n_items;
+  int phase_mode;      // 0 primary pass without jackknife, 1 primary with jackknife, 2 touch pass
+  int nslots;          // f64 slots accumulated by this pass
+  int lin;
+  // split-precision classification
+  float pmin32, inv_psize32, gz;   // pi bin: u = (dz - pmin)*inv_psize, guard gz in bin units
+  float la, lb;                    // rp bin estimate from log2(s) (log bins) ...
+  float lrmin, linv;               // ... or from sqrt(s) (linear bins)
+  // tables (device)
+  const double *thr;   // n_rp+1 FP64 thresholds on s
+  const float *thr_lo; // n_rp+1: thr[k] + guard, rounded up
+  const float *thr_hi; // n_rp+1: thr[k] - guard, rounded down
+  const int2 *nbr;     // neighbour column offsets
+  const Item *items;
+  const Group *bgrp;   // groups of the block side
+  const Group *tgrp;   // groups of the tile side
+  const int *tcol;     // [nc*nc+1] first group of each tile-side column
+  double *acc;         // [n_target][nslots][ncell]
+  unsigned long long *accnum;  // [n_target][ncell]
+  unsigned long long *counters;  // [0] candidates, [1] FP64 re-evaluations
+};
+
+
+// Arguments of the cell-list construction kernels.
+struct KeyArgs {
+  const double *x, *y, *inc;
+  const int32_t *jk;
+  const double *jkp;
+  int64_t n;
+  int nc;
+  double h;
+  int nreg;          // regions in the key (1 when the jackknife is off)
+  double inc_max;
+};
+
+struct GatherArgs {
+  const double *x, *y, *z, *w, *e1, *e2, *phi;
+  int64_t n;  // included galaxies
+  int nc;
+  double h;
+  // image flags (lim_calc_PB, PB_calcs.cpp:30-61)
+  double L, pmin, pmax, rmax;
+  double zfirst_plus_L;  // fl(z_first + L) of the inner catalog's first included galaxy
+  int has_zfirst;
+};
+
+struct SparseArgs {
+  const double *zo, *inco;  // outer z, include_prob
+  int64_t no;
+  const double *zi;         // inner z, ascending, every galaxy
+  int64_t ni;
+  double L, pmin, pmax, inc_max;
+};
+
+// Launch helpers implemented in cpc_kernels.hip.
+hipError_t launch_keys(const KeyArgs &A, const int32_t *order, uint32_t *keys, int32_t *vals, hipStream_t st);
+hipError_t launch_gather(const GatherArgs &A, const int32_t *sidx, PRec *rec, FRec *fr, hipStream_t st);
+hipError_t launch_sparse(const SparseArgs &A, unsigned long long *count, hipStream_t st);
+hipError_t sort_u32(void *&tmp, size_t &tmp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
+                    int32_t *vout, int64_t n, hipStream_t st);
+hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *kout, const int32_t *vin,
+                    int32_t *vout, int64_t n, hipStream_t st);
+size_t pair_lds_bytes(int ncell, int nslots, int n_rp);
+hipError_t launch_pair(int kind, bool split, bool swap, const KArgs &P, int threads, const PRec *brec,
+                       const FRec *bfr, const PRec *trec, const FRec *tfr, hipStream_t st);
+
+}  // namespace cpc

@@ -1,0 +1,538 @@
+// cpc_kernels.hip -- CDNA4 (gfx950) kernels of the rp-pi pair engine.
+//
+// Work decomposition.  Both catalogs are gathered into 64-byte records ordered by
+// (xy column, jackknife region, coin, z).  A work item is a run of up to a few hundred block-side
+// galaxies of one (column, region, coin) group; one workgroup owns an item and keeps a private
+// (rp bin x pi bin) histogram in LDS.  Each wave holds 64 consecutive block galaxies (one per lane,
+// contiguous in z) and walks the neighbour columns; inside a neighbour group it binary-searches the
+// z range its lanes can reach and then streams the tile galaxies with wave-uniform (scalar) loads,
+// so every tile galaxy is fetched once per wave and evaluated against 64 block galaxies.
+//
+// Exactness.  The FP64 path evaluates corr_pc's predicates in the reference's operation order
+// (do_pair_sorted corr.cpp:67-86, pi_calc_PB/rp_calc_PB PB_calcs.cpp:87-98, corel_bin
+// corels.cpp:21-46) with contraction disabled, and bins rp by comparing s = t0*t0 + t1*t1 against
+// host-calibrated thresholds (the host libm's log10 decides them), so bin indices are bit-identical.
+// The split path classifies in FP32 on column-local coordinates and sends every candidate that lies
+// within a proven error bound of a bin edge or window edge to the FP64 path.
+//
+// Jackknife.  Every pair is accumulated once, into a histogram selected per (item, phase):
+//   primary pass (block = outer catalog):
+//     target 2a+1: pairs that touch region a through the outer galaxy (same region, or coin >= 0.5)
+//     target 2a  : pairs that do not (different region and outer coin < 0.5)
+//   touch pass (block = inner galaxies of region b with coin >= 0.5, tiles = outer galaxies of other
+//     regions): target b.
+// The host rebuilds full = sum of primary targets and touch_r = target 2r+1 + touch-pass target r
+// (jk.cpp:201-223), then leave-one-out jk_r = full - touch_r (jk.cpp:74-81).
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "cpc_internal.h"
+
+namespace cpc {
+
+enum { kDD = 0, kSD = 1, kSS0 = 2, kSS1 = 3 };
+
+template <int K>
+struct KindNV;
+template <>
+struct KindNV<kDD> { static constexpr int NV = 0; };
+template <>
+struct KindNV<kSD> { static constexpr int NV = 3; };  // data[1], data[2], data[6]
+template <>
+struct KindNV<kSS0> { static constexpr int NV = 5; };  // data[1], data[3..6]
+template <>
+struct KindNV<kSS1> { static constexpr int NV = 6; };  // data[1..6]
+
+struct Gal {
+  double x, y, w, e1, e2, phi;
+};
+
+// ephi_calc, calcs.cpp:97-105 (with absolute() of calcs.cpp:50-53)
+__device__ __forceinline__ double ephi(double phi, double theta) {
+  double t = phi - theta;
+  t = t < 0 ? -1 * t : t;
+  if (t > M_PI / 2.0) t = M_PI - t;
+  return t;
+}
+
+// Shape quantities of one binned pair, g1 = outer, g2 = inner (corels.cpp:122-244).
+// q = [wt, vals..., errs...] in the slot order of KindNV.
+template <int K, bool SPLIT>
+__device__ __forceinline__ void pair_values(const Gal &g1, const Gal &g2, int gws, double *q) {
+  const double wt = g1.w * g2.w;
+  q[0] = wt;
+  if constexpr (K == kDD) {
+    return;
+  } else {
+    // angle_orient_axis_PB, calcs.cpp:54-57: raw coordinates, no image shift
+    const double dxr = g2.x - g1.x, dyr = g2.y - g1.y;
+    const double theta = atan(dyr / dxr);
+    double c2, s2;
+    if constexpr (SPLIT) {
+      // cos 2theta and sin 2theta of theta = atan(dy/dx), algebraically
+      const double xx = dxr * dxr, yy = dyr * dyr;
+      const double inv = 1.0 / (xx + yy);
+      c2 = (xx - yy) * inv;
+      s2 = (2.0 * dxr * dyr) * inv;
+    } else {
+      c2 = cos(2.0 * theta);
+      s2 = sin(2.0 * theta);
+    }
+    if constexpr (K == kSD) {
+      // shape_density_corel: rotate the shape galaxy, theta column from g1's phi
+      const Gal &s = (gws == 2) ? g2 : g1;
+      const double ep = c2 * s.e1 - s2 * s.e2;
+      const double ex = s2 * s.e1 + c2 * s.e2;
+      const double a = wt * ep, b = wt * ex;
+      const double t = ephi(g1.phi, theta);
+      q[1] = a;
+      q[2] = b;
+      q[3] = t;
+      q[4] = a * a;
+      q[5] = b * b;
+      q[6] = t * t;
+    } else {
+      const double epi = c2 * g1.e1 - s2 * g1.e2, exi = s2 * g1.e1 + c2 * g1.e2;
+      const double epj = c2 * g2.e1 - s2 * g2.e2, exj = s2 * g2.e1 + c2 * g2.e2;
+      const double pp = wt * (epi * epj), xx2 = wt * (exi * exj);
+      const double px = wt * (exi * epj + epi * exj) / 2.0;
+      const double ea = wt * (exi * epj) / 2.0, eb = wt * epi * exj / 2.0;
+      if constexpr (K == kSS0) {
+        // estimator 0: e+ of the shape-side galaxy only, ex never assigned (data[2] += 0)
+        const double ep = (gws == 2) ? epj : epi;
+        const double a = wt * ep;
+        const double t = ephi((gws == 2) ? g2.phi : g1.phi, theta);
+        q[1] = a;
+        q[2] = pp;
+        q[3] = xx2;
+        q[4] = px;
+        q[5] = t;
+        q[6] = a * a;
+        q[7] = pp * pp;
+        q[8] = xx2 * xx2;
+        q[9] = ea * ea + eb * eb;
+        q[10] = t * t;
+      } else {
+        const double ep = wt * (epi + epj) / 2.0;
+        const double ex = wt * (exi + exj) / 2.0;
+        const double ai = wt * epi / 2.0, aj = wt * epj / 2.0;
+        const double bi = wt * exi / 2.0, bj = wt * exj / 2.0;
+        const double t1 = ephi(g1.phi, theta), t2 = ephi(g2.phi, theta);
+        q[1] = ep;
+        q[2] = ex;
+        q[3] = pp;
+        q[4] = xx2;
+        q[5] = px;
+        q[6] = t1 + t2;
+        q[7] = ai * ai + aj * aj;
+        q[8] = bi * bi + bj * bj;
+        q[9] = pp * pp;
+        q[10] = xx2 * xx2;
+        q[11] = ea * ea + eb * eb;
+        q[12] = t1 * t1 + t2 * t2;
+      }
+    }
+  }
+}
+
+// rp bin of a squared separation against the FP64 thresholds thr[0..n_rp]: bin k iff
+// thr[k] <= s < thr[k+1].  An FP32 logarithm gives the starting guess.
+__device__ __forceinline__ int rp_bin64(const KArgs &P, const double *thr, double s) {
+  const float sf = (float)s;
+  float kf = P.lin ? (sqrtf(sf) - P.lrmin) * P.linv : __log2f(sf) * P.la + P.lb;
+  kf = fminf(fmaxf(kf, -1.0f), (float)P.n_rp);
+  int k = (int)floorf(kf);
+  while (k >= 0 && s < thr[k]) --k;
+  while (k < P.n_rp && s >= thr[k + 1]) ++k;
+  return (k >= 0 && k < P.n_rp) ? k : -1;
+}
+
+// The reference's candidate test and binning for outer (x1,y1,z1), inner (x2,y2,z2) seen through
+// image (PBx, PBy, PBz): do_pair_sorted (corr.cpp:67-86) then corel_bin (corels.cpp:21-46).
+// Returns the histogram cell n1*n_pi + n2, or -1.
+__device__ __forceinline__ int exact_cell(const KArgs &P, const double *thr, double x1, double y1, double z1,
+                                          double x2, double y2, double z2, int PBx, int PBy, int PBz) {
+  const double L = P.L;
+  const double zz = z2 + L * PBz;
+  const double zmax = z1 + P.pmax, zmin = z1 + P.pmin;
+  if (zz > zmax || zz < zmin) return -1;
+  const double yy = y2 + L * PBy;
+  if (yy > y1 + P.rmax || yy < y1 - P.rmax) return -1;
+  const double xx = x2 + L * PBx;
+  if (xx < x1 - P.rmax || xx > x1 + P.rmax) return -1;
+  const double q2 = floor(((zz - z1) - P.pmin) / P.psize);
+  if (!(q2 >= 0.0 && q2 < (double)P.n_pi)) return -1;
+  const double t0 = x1 - xx, t1 = y1 - yy;
+  const double s = t0 * t0 + t1 * t1;
+  const int k = rp_bin64(P, thr, s);
+  if (k < 0) return -1;
+  return k * P.n_pi + (int)q2;
+}
+
+// Whether the outer galaxy with image flags `meta` visits image (PBx, PBy, PBz) (corr.cpp:380-423).
+__device__ __forceinline__ bool image_allowed(int meta, int PBx, int PBy, int PBz) {
+  bool ok = true;
+  ok &= (PBx == 0) || (PBx < 0 ? (meta & kMetaXm) : (meta & kMetaXp));
+  ok &= (PBy == 0) || (PBy < 0 ? (meta & kMetaYm) : (meta & kMetaYp));
+  ok &= (PBz == 0) || (PBz < 0 ? (meta & kMetaZm) : (meta & kMetaZp));
+  return ok;
+}
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// first index in [lo, hi) of the z-sorted records with z >= v (upper=false) or z > v (upper=true)
+template <bool UPPER>
+__device__ __forceinline__ int zsearch(const PRec *__restrict__ rec, int lo, int hi, double v) {
+  while (lo < hi) {
+    const int mid = uni(lo + ((hi - lo) >> 1));
+    const double z = rec[mid].z;
+    const bool go_right = UPPER ? (z <= v) : (z < v);
+    if (go_right)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <int K, bool SPLIT, bool SWAP>
+__global__ void __launch_bounds__(256) pair_kernel(KArgs P, const PRec *__restrict__ brec,
+                                                   const FRec *__restrict__ bfr, const PRec *__restrict__ trec,
+                                                   const FRec *__restrict__ tfr) {
+  constexpr int NV = KindNV<K>::NV;
+  constexpr int NS = SWAP ? 1 + NV : 1 + 2 * NV;  // f64 slots accumulated
+  constexpr int NQ = 1 + 2 * NV;                  // quantities computed
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ncell = P.ncell;
+  unsigned *lnum = reinterpret_cast<unsigned *>(smem);
+  double *lacc = reinterpret_cast<double *>(smem + ((ncell * 4 + 15) & ~15));
+  double *lthr = lacc + (size_t)NS * ncell;
+  float *llo = reinterpret_cast<float *>(lthr + P.n_rp + 1);
+  float *lhi = llo + P.n_rp + 1;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = uni(tid >> 6);
+  const int nwave = blockDim.x >> 6;
+
+  for (int k = tid; k <= P.n_rp; k += blockDim.x) {
+    lthr[k] = P.thr[k];
+    if (SPLIT) {
+      llo[k] = P.thr_lo[k];
+      lhi[k] = P.thr_hi[k];
+    }
+  }
+
+  const Item it = P.items[blockIdx.x];
+  const Group bg = P.bgrp[it.group];
+  const int bcx = bg.col % P.nc, bcy = bg.col / P.nc;
+  // phase_mode: 0 primary without jackknife, 1 primary with jackknife, 2 touch pass
+  const bool two_phase = (P.phase_mode == 1) && (bg.coin == 0);
+  const int nphase = two_phase ? 2 : 1;
+
+  unsigned long long ncand = 0, nre = 0;
+
+  for (int ph = 0; ph < nphase; ++ph) {
+    // filter on tile region: 0 all, 1 same region as block, 2 other regions
+    const int filt = (P.phase_mode == 2) ? 2 : (two_phase ? (ph == 0 ? 1 : 2) : 0);
+    const int tgt = (P.phase_mode == 2) ? it.target : (two_phase ? it.target + (ph == 0 ? 1 : 0) : it.target);
+    for (int c = tid; c < ncell; c += blockDim.x) {
+      lnum[c] = 0;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) lacc[s * ncell + c] = 0.0;
+    }
+    __syncthreads();
+
+    const int nchunk = (it.count + 63) >> 6;
+    for (int ch = wave; ch < nchunk; ch += nwave) {
+      const int base = uni(it.first + ch * 64);
+      const int cnt = uni(min(64, it.first + it.count - base));
+      const bool valid = lane < cnt;
+      const int p = base + (valid ? lane : 0);
+      const PRec me = brec[p];
+      FRec mf;
+      if (SPLIT) mf = bfr[p];
+      const double wzlo = brec[base].z;
+      const double wzhi = brec[base + cnt - 1].z;
+      const Gal gme{me.x, me.y, me.w, me.e1, me.e2, me.phi};
+
+      for (int q = 0; q < P.n_nbr; ++q) {
+        const int2 o = P.nbr[q];
+        const int ux = bcx + o.x, uy = bcy + o.y;
+        const int sx = ux >= 0 ? ux / P.nc : -((-ux + P.nc - 1) / P.nc);
+        const int sy = uy >= 0 ? uy / P.nc : -((-uy + P.nc - 1) / P.nc);
+        const int tcx = ux - sx * P.nc, tcy = uy - sy * P.nc;
+        const int tcoln = tcy * P.nc + tcx;
+        const int g0 = P.tcol[tcoln], g1 = P.tcol[tcoln + 1];
+        const float Ox = (float)((double)(tcx - bcx) * P.h + P.L * sx);
+        const float Oy = (float)((double)(tcy - bcy) * P.h + P.L * sy);
+        for (int g = g0; g < g1; ++g) {
+          const Group tg = P.tgrp[g];
+          if (filt == 1 && tg.region != bg.region) continue;
+          if (filt == 2 && tg.region == bg.region) continue;
+          for (int sz = -1; sz <= 1; ++sz) {
+            double lo, hi;
+            if (!SWAP) {
+              lo = wzlo + P.pmin - P.L * sz - P.zmargin;
+              hi = wzhi + P.pmax - P.L * sz + P.zmargin;
+            } else {
+              lo = wzlo - P.pmax - P.L * sz - P.zmargin;
+              hi = wzhi - P.pmin - P.L * sz + P.zmargin;
+            }
+            if (hi < tg.zlo || lo > tg.zhi) continue;
+            const int j0 = zsearch<false>(trec, tg.start, tg.end, lo);
+            const int j1 = zsearch<true>(trec, j0, tg.end, hi);
+            if (j0 >= j1) continue;
+            // image of the inner galaxy in the reference's terms
+            const int PBx = SWAP ? -sx : sx, PBy = SWAP ? -sy : sy, PBz = SWAP ? -sz : sz;
+            const bool lane_ok = valid && (SWAP || image_allowed(me.meta, PBx, PBy, PBz));
+            const float Oz = (float)(P.L * sz);
+            for (int j = j0; j < j1; ++j) {
+              bool ok = lane_ok;
+              if (SWAP) ok = ok && image_allowed(trec[j].meta, PBx, PBy, PBz);
+              ncand += ok;
+              int cell = -1;
+              bool unc = ok;
+              if (SPLIT && ok) {
+                const FRec tf = tfr[j];
+                const float dx = (tf.xl - mf.xl) + Ox;
+                const float dy = (tf.yl - mf.yl) + Oy;
+                const float dz = (tf.zf - mf.zf) + Oz;
+                const float pif = SWAP ? -dz : dz;
+                const float s32 = fmaf(dx, dx, dy * dy);
+                const float u = (pif - P.pmin32) * P.inv_psize32;
+                const float fu = floorf(u);
+                const float fr = u - fu;
+                const bool near_pi = (fr < P.gz) || (fr > 1.0f - P.gz);
+                const bool pi_in = (fu >= 0.0f) && (fu < (float)P.n_pi);
+                float kf = P.lin ? (sqrtf(s32) - P.lrmin) * P.linv : __log2f(s32) * P.la + P.lb;
+                kf = fminf(fmaxf(kf, -1.0f), (float)P.n_rp);
+                const int k = (int)floorf(kf);
+                const int kk = min(max(k, 0), P.n_rp - 1);
+                const bool rp_out = (s32 < lhi[0]) || (s32 >= llo[P.n_rp]);
+                const bool rp_in = (k == kk) && (s32 >= llo[kk]) && (s32 < lhi[kk + 1]);
+                if ((!near_pi && !pi_in) || rp_out) {
+                  unc = false;
+                } else if (!near_pi && rp_in) {
+                  unc = false;
+                  cell = k * P.n_pi + (int)fu;
+                }
+              }
+              if (unc) {
+                nre += SPLIT;
+                const PRec t = trec[j];
+                if (!SWAP)
+                  cell = exact_cell(P, lthr, me.x, me.y, me.z, t.x, t.y, t.z, PBx, PBy, PBz);
+                else
+                  cell = exact_cell(P, lthr, t.x, t.y, t.z, me.x, me.y, me.z, PBx, PBy, PBz);
+              }
+              if (cell >= 0) {
+                const PRec t = trec[j];
+                const Gal gt{t.x, t.y, t.w, t.e1, t.e2, t.phi};
+                double qv[NQ];
+                if (!SWAP)
+                  pair_values<K, SPLIT>(gme, gt, P.gws, qv);
+                else
+                  pair_values<K, SPLIT>(gt, gme, P.gws, qv);
+                atomicAdd(&lnum[cell], 1u);
+#pragma unroll
+                for (int s = 0; s < NS; ++s) atomicAdd(&lacc[s * ncell + cell], qv[s]);
+              }
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int c = tid; c < ncell; c += blockDim.x) {
+      const unsigned n = lnum[c];
+      if (n) {
+        atomicAdd(&P.accnum[(size_t)tgt * ncell + c], (unsigned long long)n);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) unsafeAtomicAdd(&P.acc[((size_t)tgt * NS + s) * ncell + c], lacc[s * ncell + c]);
+      }
+    }
+    __syncthreads();
+  }
+  if (ncand) atomicAdd(&P.counters[0], ncand);
+  if (nre) atomicAdd(&P.counters[1], nre);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Cell-list construction
+
+
+// key = ((column * nreg + region) * 2 + coin), or 0xffffffff for galaxies the pass skips
+// (include_prob > include_prob_max, corr.cpp:352,414)
+__global__ void k_keys(KeyArgs A, const int32_t *__restrict__ order, uint32_t *__restrict__ keys,
+                       int32_t *__restrict__ vals) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= A.n) return;
+  const int32_t i = order ? order[p] : (int32_t)p;
+  uint32_t key = 0xffffffffu;
+  const double inc = A.inc ? A.inc[i] : 0.0;
+  if (!(inc > A.inc_max)) {
+    int cx = (int)floor(A.x[i] / A.h), cy = (int)floor(A.y[i] / A.h);
+    cx = min(max(cx, 0), A.nc - 1);
+    cy = min(max(cy, 0), A.nc - 1);
+    const int reg = (A.nreg > 1) ? A.jk[i] : 0;
+    const int coin = A.jkp ? (A.jkp[i] >= 0.5) : 0;
+    key = ((uint32_t)(cy * A.nc + cx) * (uint32_t)A.nreg + (uint32_t)reg) * 2u + (uint32_t)coin;
+  }
+  keys[p] = key;
+  vals[p] = i;
+}
+
+
+__global__ void k_gather(GatherArgs A, const int32_t *__restrict__ sidx, PRec *__restrict__ rec,
+                         FRec *__restrict__ fr) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= A.n) return;
+  const int32_t i = sidx[p];
+  PRec r;
+  r.x = A.x[i];
+  r.y = A.y[i];
+  r.z = A.z[i];
+  r.w = A.w ? A.w[i] : 1.0;
+  r.e1 = A.e1 ? A.e1[i] : 0.0;
+  r.e2 = A.e2 ? A.e2[i] : 0.0;
+  r.phi = A.phi ? A.phi[i] : -10.0;
+  r.idx = i;
+  const double z_max = r.z + A.pmax, z_min = r.z + A.pmin;
+  const double ra_max = r.x + A.rmax, ra_min = r.x - A.rmax;
+  const double dec_max = r.y + A.rmax, dec_min = r.y - A.rmax;
+  int m = 0;
+  if (ra_min < 0) m |= kMetaXm;
+  if (ra_max > A.L) m |= kMetaXp;
+  if (dec_min < 0) m |= kMetaYm;
+  if (dec_max > A.L) m |= kMetaYp;
+  if (z_min < 0) m |= kMetaZm;
+  // PBz>0 sweeps from j = 0 and stops at the first included galaxy outside the window (corr.cpp:407-416)
+  if (z_max > A.L && A.has_zfirst && !(A.zfirst_plus_L < z_min)) m |= kMetaZp;
+  r.meta = m;
+  rec[p] = r;
+  int cx = (int)floor(r.x / A.h), cy = (int)floor(r.y / A.h);
+  cx = min(max(cx, 0), A.nc - 1);
+  cy = min(max(cy, 0), A.nc - 1);
+  FRec f;
+  f.xl = (float)(r.x - cx * A.h);
+  f.yl = (float)(r.y - cy * A.h);
+  f.zf = (float)r.z;
+  f.pad = 0.0f;
+  fr[p] = f;
+}
+
+// Outer galaxies whose PBz = 0 or PBz = -1 window holds no inner galaxy while later ones exist: the
+// reference's forward-only start pointer runs off in that case (corr.cpp:361-369, 395-404).
+
+__global__ void k_sparse(SparseArgs A, unsigned long long *count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned c = 0;
+  if (i < A.no && !((A.inco ? A.inco[i] : 0.0) > A.inc_max)) {
+    const double z_max = A.zo[i] + A.pmax, z_min = A.zo[i] + A.pmin;
+    for (int PBz = -1; PBz <= 0; ++PBz) {
+      if (PBz < 0 && !(z_min < 0)) continue;
+      const double sh = A.L * PBz;
+      int64_t lo = 0, hi = A.ni;
+      while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (A.zi[mid] + sh < z_min)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      if (lo < A.ni && A.zi[lo] + sh > z_max) ++c;
+    }
+  }
+  if (c) atomicAdd(count, (unsigned long long)c);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host-side launch helpers (called from cpc_api.cpp)
+
+hipError_t launch_keys(const KeyArgs &A, const int32_t *order, uint32_t *keys, int32_t *vals, hipStream_t st) {
+  if (A.n == 0) return hipSuccess;
+  const int bs = 256;
+  k_keys<<<(unsigned)((A.n + bs - 1) / bs), bs, 0, st>>>(A, order, keys, vals);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const GatherArgs &A, const int32_t *sidx, PRec *rec, FRec *fr, hipStream_t st) {
+  if (A.n == 0) return hipSuccess;
+  const int bs = 256;
+  k_gather<<<(unsigned)((A.n + bs - 1) / bs), bs, 0, st>>>(A, sidx, rec, fr);
+  return hipGetLastError();
+}
+
+hipError_t launch_sparse(const SparseArgs &A, unsigned long long *count, hipStream_t st) {
+  if (A.no == 0 || A.ni == 0) return hipSuccess;
+  const int bs = 256;
+  k_sparse<<<(unsigned)((A.no + bs - 1) / bs), bs, 0, st>>>(A, count);
+  return hipGetLastError();
+}
+
+// Stable sort of (key, value) pairs; temporary storage grows as needed.
+template <typename KeyT>
+static hipError_t sort_pairs_impl(void *&tmp, size_t &tmp_bytes, const KeyT *kin, KeyT *kout, const int32_t *vin,
+                                  int32_t *vout, int64_t n, int end_bit, hipStream_t st) {
+  size_t need = 0;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, need, kin, kout, vin, vout, (int)n, 0, end_bit, st);
+  if (e != hipSuccess) return e;
+  if (need > tmp_bytes) {
+    if (tmp) (void)hipFree(tmp);
+    tmp = nullptr;
+    tmp_bytes = 0;
+    e = hipMalloc(&tmp, need);
+    if (e != hipSuccess) return e;
+    tmp_bytes = need;
+  }
+  return hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kin, kout, vin, vout, (int)n, 0, end_bit, st);
+}
+
+hipError_t sort_u32(void *&tmp, size_t &tmp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
+                    int32_t *vout, int64_t n, hipStream_t st) {
+  return sort_pairs_impl<uint32_t>(tmp, tmp_bytes, kin, kout, vin, vout, n, 32, st);
+}
+
+hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *kout, const int32_t *vin,
+                    int32_t *vout, int64_t n, hipStream_t st) {
+  return sort_pairs_impl<double>(tmp, tmp_bytes, kin, kout, vin, vout, n, 64, st);
+}
+
+template <int K, bool SPLIT, bool SWAP>
+static hipError_t launch_pair_t(const KArgs &P, int threads, size_t lds, const PRec *brec, const FRec *bfr,
+                                const PRec *trec, const FRec *tfr, hipStream_t st) {
+  auto fn = pair_kernel<K, SPLIT, SWAP>;
+  hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  fn<<<P.n_items, threads, lds, st>>>(P, brec, bfr, trec, tfr);
+  return hipGetLastError();
+}
+
+// LDS bytes a pair kernel needs for histograms of `ncell` cells and `nslots` f64 slots.
+size_t pair_lds_bytes(int ncell, int nslots, int n_rp) {
+  return (size_t)((ncell * 4 + 15) & ~15) + (size_t)nslots * ncell * 8 + (size_t)(n_rp + 1) * 8 +
+         (size_t)2 * (n_rp + 1) * 4;
+}
+
+hipError_t launch_pair(int kind, bool split, bool swap, const KArgs &P, int threads, const PRec *brec,
+                       const FRec *bfr, const PRec *trec, const FRec *tfr, hipStream_t st) {
+  if (P.n_items == 0) return hipSuccess;
+  const size_t lds = pair_lds_bytes(P.ncell, P.nslots, P.n_rp);
+#define CPC_DISPATCH(KK)                                                                        \
+  if (kind == KK) {                                                                             \
+    if (split && swap) return launch_pair_t<KK, true, true>(P, threads, lds, brec, bfr, trec, tfr, st);   \
+    if (split) return launch_pair_t<KK, true, false>(P, threads, lds, brec, bfr, trec, tfr, st);          \
+    if (swap) return launch_pair_t<KK, false, true>(P, threads, lds, brec, bfr, trec, tfr, st);           \
+    return launch_pair_t<KK, false, false>(P, threads, lds, brec, bfr, trec, tfr, st);                    \
+  }
+  CPC_DISPATCH(kDD)
+  CPC_DISPATCH(kSD)
+  CPC_DISPATCH(kSS0)
+  CPC_DISPATCH(kSS1)
+#undef CPC_DISPATCH
+  return hipErrorInvalidValue;
+}
+
+}  // namespace cpc

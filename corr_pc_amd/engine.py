@@ -1,0 +1,275 @@
+"""ctypes bindings of the engine's C ABI (include/cpc.h) and numpy-facing helpers.
+
+The product path is libcpc.so (HIP kernels for gfx950).  Loading fails loudly when the library is
+missing; there is no CPU fallback.
+"""
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_int64, c_void_p
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcpc.so")
+
+CPC_OK = 0
+CPC_ERR_INVALID = -1
+CPC_ERR_DEVICE = -2
+CPC_ERR_NOMEM = -3
+CPC_ERR_UNSUPPORTED = -4
+
+GEOM_BOX = 0
+KIND_DD = 0
+KIND_SHAPE_DENSITY = 1
+KIND_SHAPE_SHAPE = 2
+PREC_SPLIT = 0
+PREC_FP64 = 1
+JK_LEAVE_ONE_OUT = 0
+JK_TOUCH = 1
+NFIELD = 7
+
+_dp = POINTER(c_double)
+_ip = POINTER(c_int32)
+_lp = POINTER(c_int64)
+
+
+class CatalogDesc(ctypes.Structure):
+    _fields_ = [("n", c_int64), ("x", _dp), ("y", _dp), ("z", _dp), ("w", _dp), ("e1", _dp), ("e2", _dp),
+                ("phi", _dp), ("jk", _ip), ("jk_prob", _dp), ("include_prob", _dp)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("geometry", c_int), ("box_size", c_double), ("kind", c_int), ("g_with_shape", c_int),
+                ("estimator", c_int), ("which_corr", c_int), ("n_rp", c_int), ("rp_min", c_double),
+                ("rp_max", c_double), ("lin_bin", c_int), ("n_pi", c_int), ("pi_min", c_double),
+                ("pi_max", c_double), ("do_jk", c_int), ("n_jk", c_int), ("include_prob_max", c_double),
+                ("precision", c_int), ("jk_mode", c_int)]
+
+
+class HistC(ctypes.Structure):
+    _fields_ = [("num", _lp), ("wt_num", _dp), ("val", _dp), ("err", _dp), ("num_rp", _lp)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("n_binned", c_int64), ("n_candidates", c_int64), ("n_fp64_reeval", c_int64),
+                ("n_sparse_windows", c_int64), ("ms_total", c_double), ("ms_kernel", c_double),
+                ("ms_primary", c_double), ("ms_touch", c_double), ("ms_build", c_double),
+                ("n_items_primary", c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class EngineError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"cpc status {status}: {msg}")
+        self.status = status
+
+
+@dataclass
+class Catalog:
+    """One catalog in SoA form; the fields of corr_pc's `gal` that the pair loop reads."""
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    w: Optional[np.ndarray] = None
+    e1: Optional[np.ndarray] = None
+    e2: Optional[np.ndarray] = None
+    phi: Optional[np.ndarray] = None
+    jk: Optional[np.ndarray] = None
+    jk_prob: Optional[np.ndarray] = None
+    include_prob: Optional[np.ndarray] = None
+    _keep: list = field(default_factory=list, repr=False)
+
+    def __post_init__(self):
+        for name in ("x", "y", "z", "w", "e1", "e2", "phi", "jk_prob", "include_prob"):
+            v = getattr(self, name)
+            if v is not None:
+                setattr(self, name, np.ascontiguousarray(v, dtype=np.float64))
+        if self.jk is not None:
+            self.jk = np.ascontiguousarray(self.jk, dtype=np.int32)
+        n = len(self.x)
+        for name in ("y", "z", "w", "e1", "e2", "phi", "jk", "jk_prob", "include_prob"):
+            v = getattr(self, name)
+            if v is not None and len(v) != n:
+                raise ValueError(f"catalog field {name} has length {len(v)} != {n}")
+
+    @property
+    def n(self):
+        return len(self.x)
+
+    def subset(self, idx):
+        """Catalog restricted to rows idx (keeps order)."""
+        kw = {}
+        for name in ("x", "y", "z", "w", "e1", "e2", "phi", "jk", "jk_prob", "include_prob"):
+            v = getattr(self, name)
+            kw[name] = None if v is None else v[idx]
+        return Catalog(**kw)
+
+    def desc(self):
+        def p(a, t=_dp):
+            return None if a is None else a.ctypes.data_as(t)
+        return CatalogDesc(self.n, p(self.x), p(self.y), p(self.z), p(self.w), p(self.e1), p(self.e2),
+                           p(self.phi), p(self.jk, _ip), p(self.jk_prob), p(self.include_prob))
+
+
+def make_params(box_size, n_rp, rp_min, rp_max, n_pi, pi_min, pi_max, kind=KIND_DD, g_with_shape=2,
+                estimator=0, which_corr=None, lin_bin=0, do_jk=0, n_jk=0, include_prob_max=1.0,
+                precision=PREC_SPLIT, jk_mode=JK_LEAVE_ONE_OUT):
+    if which_corr is None:
+        which_corr = {KIND_DD: 0, KIND_SHAPE_DENSITY: 1, KIND_SHAPE_SHAPE: 2}[kind]
+    return Params(GEOM_BOX, box_size, kind, g_with_shape, estimator, which_corr, n_rp, rp_min, rp_max, lin_bin,
+                  n_pi, pi_min, pi_max, do_jk, n_jk, include_prob_max, precision, jk_mode)
+
+
+class Histogram:
+    """The reference's bin_jk in array form: region 0 is the full sample, 1..n_jk the jackknife regions."""
+
+    def __init__(self, n_regions, n_rp, n_pi):
+        self.R, self.n_rp, self.n_pi = n_regions, n_rp, n_pi
+        self.num = np.zeros((n_regions, n_rp, n_pi), np.int64)
+        self.wt_num = np.zeros((n_regions, n_rp, n_pi), np.float64)
+        self.val = np.zeros((n_regions, n_rp, n_pi, NFIELD), np.float64)
+        self.err = np.zeros((n_regions, n_rp, n_pi, NFIELD), np.float64)
+        self.num_rp = np.zeros((n_regions, n_rp), np.int64)
+
+    @classmethod
+    def for_params(cls, p):
+        R = 1 + p.n_jk if (p.do_jk == 1 and p.n_jk > 0) else 1
+        return cls(R, p.n_rp, p.n_pi)
+
+    def c_struct(self):
+        return HistC(self.num.ctypes.data_as(_lp), self.wt_num.ctypes.data_as(_dp), self.val.ctypes.data_as(_dp),
+                     self.err.ctypes.data_as(_dp), self.num_rp.ctypes.data_as(_lp))
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libcpc.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: build it with `python -m corr_pc_amd.build`")
+    lib = ctypes.CDLL(path)
+    lib.cpc_init.argtypes = [POINTER(c_void_p), c_int]
+    lib.cpc_init.restype = c_int
+    lib.cpc_destroy.argtypes = [c_void_p]
+    lib.cpc_destroy.restype = None
+    lib.cpc_last_error.argtypes = [c_void_p]
+    lib.cpc_last_error.restype = c_char_p
+    lib.cpc_version.argtypes = []
+    lib.cpc_version.restype = c_char_p
+    lib.cpc_catalog_upload.argtypes = [c_void_p, POINTER(CatalogDesc), POINTER(c_void_p)]
+    lib.cpc_catalog_upload.restype = c_int
+    lib.cpc_catalog_free.argtypes = [c_void_p, c_void_p]
+    lib.cpc_catalog_free.restype = None
+    lib.cpc_count.argtypes = [c_void_p, c_void_p, c_void_p, POINTER(Params), POINTER(HistC), POINTER(Stats)]
+    lib.cpc_count.restype = c_int
+    lib.cpc_rp_thresholds.argtypes = [POINTER(Params), _dp]
+    lib.cpc_rp_thresholds.restype = c_int
+    lib.cpc_rand_stream.argtypes = [ctypes.c_uint32, c_int64, c_int64, _dp]
+    lib.cpc_rand_stream.restype = c_int
+    lib.cpc_set_tuning.argtypes = [c_void_p, c_double, c_int, c_int]
+    lib.cpc_set_tuning.restype = c_int
+    _lib = lib
+    return lib
+
+
+EXPORTED_SYMBOLS = ["cpc_init", "cpc_destroy", "cpc_last_error", "cpc_version", "cpc_catalog_upload",
+                    "cpc_catalog_free", "cpc_count", "cpc_rp_thresholds", "cpc_rand_stream", "cpc_set_tuning"]
+
+
+REFERENCE_SEED = 124693517  # seed(), jk.cpp:40-43
+
+
+def rand_stream(n, skip=0, seed=REFERENCE_SEED):
+    """Draws skip..skip+n-1 of rand()/(RAND_MAX+1.) after srand(seed), host C library."""
+    lib = load_library()
+    out = np.zeros(int(n), np.float64)
+    rc = lib.cpc_rand_stream(seed, int(skip), int(n), out.ctypes.data_as(_dp))
+    if rc != CPC_OK:
+        raise EngineError(rc, "cpc_rand_stream failed")
+    return out
+
+
+def rp_thresholds(params):
+    """Squared-separation edges thr[0..n_rp] of the rp bins (host libm), see include/cpc.h."""
+    lib = load_library()
+    out = np.zeros(params.n_rp + 1, np.float64)
+    rc = lib.cpc_rp_thresholds(ctypes.byref(params), out.ctypes.data_as(_dp))
+    if rc != CPC_OK:
+        raise EngineError(rc, "cpc_rp_thresholds failed")
+    return out
+
+
+class Engine:
+    """One device context.  Catalog handles are cached per Python Catalog object."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = c_void_p()
+        rc = self.lib.cpc_init(ctypes.byref(h), device)
+        if rc != CPC_OK:
+            raise EngineError(rc, f"cpc_init(device={device}) failed")
+        self.ctx = h
+        self._cats = {}
+
+    def _check(self, rc, what):
+        if rc != CPC_OK:
+            msg = self.lib.cpc_last_error(self.ctx).decode()
+            raise EngineError(rc, f"{what}: {msg}")
+
+    def upload(self, cat: Catalog):
+        key = id(cat)
+        if key in self._cats:
+            return self._cats[key][0]
+        h = c_void_p()
+        d = cat.desc()
+        self._check(self.lib.cpc_catalog_upload(self.ctx, ctypes.byref(d), ctypes.byref(h)), "upload")
+        self._cats[key] = (h, cat)
+        return h
+
+    def free(self, cat: Catalog):
+        ent = self._cats.pop(id(cat), None)
+        if ent is not None:
+            self.lib.cpc_catalog_free(self.ctx, ent[0])
+
+    def set_tuning(self, col_frac=0.0, item_size=0, block_threads=0):
+        self._check(self.lib.cpc_set_tuning(self.ctx, col_frac, item_size, block_threads), "set_tuning")
+
+    def count(self, outer: Catalog, inner: Catalog, params: Params, hist: Histogram = None):
+        """One cross pass (the reference's corels2 after the role swap).  Returns (Histogram, stats dict)."""
+        ho = self.upload(outer)
+        hi = ho if inner is outer else self.upload(inner)
+        if hist is None:
+            hist = Histogram.for_params(params)
+        hc = hist.c_struct()
+        st = Stats()
+        self._check(self.lib.cpc_count(self.ctx, ho, hi, ctypes.byref(params), ctypes.byref(hc), ctypes.byref(st)),
+                    "count")
+        return hist, st.as_dict()
+
+    def close(self):
+        if self.ctx is not None:
+            for h, _ in self._cats.values():
+                self.lib.cpc_catalog_free(self.ctx, h)
+            self._cats.clear()
+            self.lib.cpc_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,76 @@
+"""Host mirror of corels2's pass set-up (corr.cpp:219-296) and choose_corel_func
+(initialization.cpp:490-543): which catalog is outer, which carries the shape, which pair kind runs.
+
+corels2(g1, g2, bjk, data_inf) takes its catalog sizes from the do_* role flags, makes the larger
+catalog the outer one (ties keep g1), and flips g_with_shape when it swaps.  `setup_pass` returns
+the engine-level (outer, inner, params) of such a call, ready for Engine.count.
+"""
+from dataclasses import dataclass
+
+from .engine import (JK_LEAVE_ONE_OUT, KIND_DD, KIND_SHAPE_DENSITY, KIND_SHAPE_SHAPE, PREC_SPLIT, make_params)
+
+PASSES = ("SD", "SRd", "RsRd", "SRs", "DRs", "RsRs")
+
+
+@dataclass
+class DataInfo:
+    """The data_info fields a pass needs (data_def.h:46-96); box geometry."""
+    which_corr: int
+    estimator: int
+    box_size: float
+    n_rp: int
+    rp_min: float
+    rp_max: float
+    n_pi: int
+    pi_min: float
+    pi_max: float
+    lin_bin: int = 0
+    do_jk: int = 1
+    n_jk: int = 0
+    include_prob_RR: float = 1.0
+
+
+def pass_kind(which_corr, pass_name):
+    """choose_corel_func, initialization.cpp:490-543 (coordinates 6)."""
+    if pass_name in ("RsRs", "RsRd"):
+        return KIND_DD
+    if which_corr == 0:
+        return KIND_DD
+    if which_corr == 1:
+        return KIND_SHAPE_DENSITY
+    if which_corr == 2:
+        return KIND_SHAPE_SHAPE if pass_name == "SD" else KIND_SHAPE_DENSITY
+    raise ValueError(f"which_corr={which_corr} is outside the pair kinds this engine runs")
+
+
+def setup_pass(pass_name, g1, g2, di: DataInfo, precision=PREC_SPLIT, jk_mode=JK_LEAVE_ONE_OUT):
+    """(outer, inner, params, order_changed) of corels2(g1, g2, ...) for pass `pass_name`.
+
+    g1, g2 are the arguments in the order do_corr passes them (e.g. SD: corels2(gdensity, gshape),
+    do_corr.cpp:125; SRd: corels2(grand_density, gshape), :110; RsRd: corels2(grand_density,
+    grand_shape), :115)."""
+    if pass_name not in PASSES:
+        raise ValueError(pass_name)
+    # operand roles, corr.cpp:226-271: g_with_shape before the swap
+    gws = 2 if pass_name in ("SRd", "SD", "SRs") else 0
+    n1, n2 = g1.n, g2.n
+    if n1 == 0 or n2 == 0:
+        return None  # corels2 returns 0 pairs (corr.cpp:273-277)
+    outer, inner, swapped = g1, g2, False
+    if n2 > n1:  # corr.cpp:281-296
+        outer, inner, swapped = g2, g1, True
+        gws = {2: 1, 1: 2}.get(gws, gws)
+    kind = pass_kind(di.which_corr, pass_name)
+    inc_max = di.include_prob_RR if pass_name in ("RsRs", "RsRd") else 1.0  # initialization.cpp:597-599
+    p = make_params(di.box_size, di.n_rp, di.rp_min, di.rp_max, di.n_pi, di.pi_min, di.pi_max, kind=kind,
+                    g_with_shape=gws if gws else 2, estimator=di.estimator, which_corr=di.which_corr,
+                    lin_bin=di.lin_bin, do_jk=di.do_jk, n_jk=di.n_jk, include_prob_max=inc_max,
+                    precision=precision, jk_mode=jk_mode)
+    return outer, inner, p, swapped
+
+
+def config_datainfo(cfg, do_jk=1):
+    """DataInfo of a synthetic box configuration (synth.CONFIGS)."""
+    return DataInfo(which_corr=cfg.which_corr, estimator=0, box_size=cfg.box, n_rp=cfg.n_rp, rp_min=cfg.rp_min,
+                    rp_max=cfg.rp_max, n_pi=cfg.n_pi, pi_min=-cfg.pi_max, pi_max=cfg.pi_max, do_jk=do_jk,
+                    n_jk=cfg.jk_side ** 3 if do_jk else 0)

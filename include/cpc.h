@@ -1,0 +1,159 @@
+/*
+ * cpc.h -- C ABI of the MI355X rp-pi pair-counting engine.
+ *
+ * This header is the drop-in boundary for corr_pc's cross-pair loop.  The reference seam it replaces is
+ *     long corels2(gal g1[], gal g2[], bin_jk &bjk, data_info &data_inf);      (corr.h:30, corr.cpp:219-459)
+ * which callers reach from do_corr.cpp:70,78,110,115,125,131.  corels2 takes AoS `gal` records
+ * (data_def.h:23-44), a `bin_jk` histogram tree (data_def.h:98-132) and the `data_info` mode flags
+ * (data_def.h:46-96).  Here the same information crosses as plain pointers and sizes:
+ *
+ *   gal g1[], gal g2[]      -> cpc_catalog_desc (host SoA, one per catalog) uploaded once with
+ *                              cpc_catalog_upload; the library keeps the device copy.
+ *   data_info (hot fields)  -> cpc_params (geometry, kind, binning, jackknife, precision)
+ *   bin_jk &bjk             -> cpc_hist (caller-owned host arrays in the reference's logical layout
+ *                              [region][rp][pi], region 0 = full sample, 1..n_jk = jackknife regions)
+ *   return value n_in       -> cpc_stats.n_binned
+ *
+ * The host applies corels2's operand-role table and larger-catalog-outer swap (corr.cpp:226-296)
+ * before calling cpc_count, so `outer` is the reference's g1 after the swap and `inner` its g2.
+ *
+ * No function here calls exit(); every failure is a negative cpc_status with a message retrievable
+ * through cpc_last_error().  A context is bound to one device and is not thread safe.
+ */
+#ifndef CPC_H
+#define CPC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cpc_status {
+  CPC_OK = 0,
+  CPC_ERR_INVALID = -1,     /* an argument violates the documented contract */
+  CPC_ERR_DEVICE = -2,      /* the HIP runtime reported an error */
+  CPC_ERR_NOMEM = -3,       /* device or host allocation failed */
+  CPC_ERR_UNSUPPORTED = -4  /* a mode the engine does not implement */
+} cpc_status;
+
+/* geometry: periodic box with x, y, z (data_info.periodic_box=1, coordinates=6) */
+#define CPC_GEOM_BOX 0
+
+/* pair kind: which per-pair function of corels.cpp runs (choose_corel_func, initialization.cpp:490-543) */
+#define CPC_KIND_DD 0            /* density_density_corel, corels.cpp:48-77 */
+#define CPC_KIND_SHAPE_DENSITY 1 /* shape_density_corel,   corels.cpp:206-244 */
+#define CPC_KIND_SHAPE_SHAPE 2   /* shape_shape_corel,     corels.cpp:122-204 */
+
+/* arithmetic of the pair kernel */
+#define CPC_PREC_SPLIT 0 /* FP32 classification, FP64 re-evaluation inside a guard band */
+#define CPC_PREC_FP64 1  /* every candidate in FP64, shape angles through atan/cos/sin */
+
+/* what the jackknife part of cpc_hist holds */
+#define CPC_JK_LEAVE_ONE_OUT 0 /* full - touch_r, as after final_jk_bins (jk.cpp:74-81) */
+#define CPC_JK_TOUCH 1         /* the raw per-region touch sums before final_jk_bins */
+
+#define CPC_NFIELD 7 /* data[7]: 0=W, 1=e+, 2=ex, 3=e++, 4=exx, 5=e+x, 6=theta (data_def.h:105) */
+
+/* One catalog as host SoA arrays; caller-owned, read only during cpc_catalog_upload.
+ * Mirrors the fields of `gal` that the pair loop reads (data_def.h:23-44, filled in read_dat.cpp:216-325). */
+typedef struct cpc_catalog_desc {
+  int64_t n;                  /* number of galaxies (>= 0) */
+  const double *x;            /* gal.ra.val_deg: box x coordinate */
+  const double *y;            /* gal.dec.val_deg: box y coordinate */
+  const double *z;            /* gal.redshift: box z coordinate (line of sight) */
+  const double *w;            /* gal.wt; NULL means every weight is 1 */
+  const double *e1;           /* gal.e[0]; NULL for catalogs without shapes */
+  const double *e2;           /* gal.e[1]; NULL iff e1 is NULL */
+  const double *phi;          /* gal.phi (elip_angle, calcs.cpp:73-83); NULL means -10 (read_dat.cpp:269) */
+  const int32_t *jk;          /* gal.jk, 0 <= jk < n_jk; NULL allowed only when jackknife is off */
+  const double *jk_prob;      /* gal.jk_prob (read_dat.cpp:239-255); NULL means 0 */
+  const double *include_prob; /* gal.include_prob (read_dat.cpp:318-320); NULL means 0 */
+} cpc_catalog_desc;
+
+/* The data_info fields the pair loop reads (data_def.h:46-96) plus engine choices. */
+typedef struct cpc_params {
+  int geometry;            /* CPC_GEOM_BOX */
+  double box_size;         /* data_info.periodic_box_size */
+  int kind;                /* CPC_KIND_* */
+  int g_with_shape;        /* data_info.g_with_shape after the swap: 1 = outer, 2 = inner carries the shape */
+  int estimator;           /* data_info.estimator (selects the e+ branch of shape_shape_corel) */
+  int which_corr;          /* data_info.which_corr (0 disables the shape fields of corel_jk1, jk.cpp:150) */
+  int n_rp;                /* data_info.n_bins */
+  double rp_min, rp_max;   /* data_info.bin_r_min, bin_r_max */
+  int lin_bin;             /* data_info.lin_bin */
+  int n_pi;                /* data_info.n_p_bin */
+  double pi_min, pi_max;   /* data_info.p_min, p_max */
+  int do_jk;               /* data_info.do_jk: 0 or 1 */
+  int n_jk;                /* data_info.n_jk */
+  double include_prob_max; /* calc_temp.include_prob_max (initialization.cpp:597-599) */
+  int precision;           /* CPC_PREC_* */
+  int jk_mode;             /* CPC_JK_* */
+} cpc_params;
+
+/* Output histograms, caller-allocated host memory, overwritten by cpc_count.
+ * R = 1 + n_jk when jackknife is on, else 1.  Index (r, k, p) = (r*n_rp + k)*n_pi + p.
+ * Mirrors bin_jk: b[k].p_bin[p] is region 0, jk[r-1].b[k].p_bin[p] is region r (data_def.h:98-132). */
+typedef struct cpc_hist {
+  int64_t *num;    /* [R*n_rp*n_pi]   P_bin.num */
+  double *wt_num;  /* [R*n_rp*n_pi]   P_bin.wt_num */
+  double *val;     /* [R*n_rp*n_pi*7] P_bin.data[f].val */
+  double *err;     /* [R*n_rp*n_pi*7] P_bin.data[f].err */
+  int64_t *num_rp; /* [R*n_rp]        bin.num */
+} cpc_hist;
+
+typedef struct cpc_stats {
+  int64_t n_binned;         /* pairs that landed in a valid (rp, pi) bin: corels2's n_in (corr.cpp:455) */
+  int64_t n_candidates;     /* candidate pairs the kernel evaluated */
+  int64_t n_fp64_reeval;    /* split precision: candidates re-evaluated in FP64 */
+  int64_t n_sparse_windows; /* outer galaxies whose z window held no inner galaxy while later ones exist */
+  double ms_total;          /* wall time of cpc_count */
+  double ms_kernel;         /* device time of the pair kernels (HIP events on the engine's stream) */
+  double ms_primary;        /* device time of the primary pair kernel (block side = outer catalog) */
+  double ms_touch;          /* device time of the jackknife touch kernel (block side = inner catalog) */
+  double ms_build;          /* wall time of the cell-list construction */
+  int64_t n_items_primary;  /* work items (workgroups) of the primary kernel */
+} cpc_stats;
+
+typedef struct cpc_ctx cpc_ctx;
+typedef struct cpc_catalog cpc_catalog;
+
+/* Context on HIP device `device`. */
+cpc_status cpc_init(cpc_ctx **ctx, int device);
+void cpc_destroy(cpc_ctx *ctx);
+/* Message of the last failure on this context (empty string if none). */
+const char *cpc_last_error(const cpc_ctx *ctx);
+/* Library version string. */
+const char *cpc_version(void);
+
+/* Copy one catalog to the device.  Replaces the reference's `new gal[n]` + read_data
+ * (do_corr.cpp:41,47,85,94); the arrays are not referenced after the call returns. */
+cpc_status cpc_catalog_upload(cpc_ctx *ctx, const cpc_catalog_desc *desc, cpc_catalog **cat);
+void cpc_catalog_free(cpc_ctx *ctx, cpc_catalog *cat);
+
+/* One cross pass: replaces corels2(g1=outer, g2=inner, bjk, data_inf) (corr.cpp:219-459) after the
+ * host has resolved roles and the swap.  Writes `out` (overwrite, not accumulate) and, if non-NULL,
+ * `stats`.  outer and inner may be the same catalog handle. */
+cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *outer, const cpc_catalog *inner,
+                     const cpc_params *params, cpc_hist *out, cpc_stats *stats);
+
+/* The squared-separation thresholds the engine bins rp with: thr[k] (k = 0..n_rp) is the smallest
+ * double s >= 0 for which the reference's bin index of rp = sqrt(s) (corels.cpp:31-34, with the host
+ * libm's log10) is >= k.  Exposed so that callers and tests can check them against the host libm. */
+cpc_status cpc_rp_thresholds(const cpc_params *params, double *thr);
+
+/* The reference's per-galaxy random stream: out[k] = rand() / (RAND_MAX + 1.) for the draws numbered
+ * skip .. skip+n-1 after srand(seed), with the host C library's rand().  corr_pc seeds 124693517 in
+ * seed() (jk.cpp:40-43) and draws jk_prob (read_dat.cpp:243), a random phi (read_dat.cpp:307) and
+ * include_prob (read_dat.cpp:318-320) in catalog read order (do_corr.cpp:40-96). */
+cpc_status cpc_rand_stream(uint32_t seed, int64_t skip, int64_t n, double *out);
+
+/* Engine tuning knobs (column width in units of rp_max, outer galaxies per work item, threads per
+ * work group); 0 keeps the automatic choice.  Used by the autotuner. */
+cpc_status cpc_set_tuning(cpc_ctx *ctx, double col_frac, int item_size, int block_threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CPC_H */

@@ -1,0 +1,421 @@
+/*
+ * cpc_oracle.c -- TEST INFRASTRUCTURE ONLY: the parity oracle and CPU baseline.
+ *
+ * A plain-C restatement of corr_pc's cross-pair loop for the periodic box, written from the
+ * reference's behaviour (not its text).  It follows, step for step:
+ *   corels2 outer loop, image loops and inner sweep    corr.cpp:219-459 (do_pair_sorted corr.cpp:67-86)
+ *   per-outer window and image flags                   PB_calcs.cpp:12-61 (lim_calc_xyz, lim_calc_PB)
+ *   pi and rp of a pair                                PB_calcs.cpp:87-98 (pi_calc_PB, rp_calc_PB)
+ *   bin indices, pair weight                           corels.cpp:21-46 (corel_bin)
+ *   the three pair kinds                               corels.cpp:48-77, 122-204, 206-244
+ *   orientation angle, rotation, ephi                  calcs.cpp:54-64, 97-105
+ *   jackknife touch slots and leave-one-out            jk.cpp:53-81, 138-171, 201-223; outp.cpp:242-288
+ * in FP64 with the host libm (log10, sqrt, atan, cos, sin, floor) and no FMA contraction, like the
+ * reference's -O2 x86-64 build.  OpenMP threads own private histograms, merged in thread order.
+ *
+ * One deliberate difference: the reference finds the first inner galaxy of each outer z window with
+ * a per-thread pointer that only moves forward (corr.cpp:361-408).  When some window holds no inner
+ * galaxy but later galaxies exist, that pointer runs past the end and the reference's result depends
+ * on the OpenMP schedule.  Here every window starts at its binary-searched lower bound, which equals
+ * the reference whenever no such window occurs; such windows are counted in n_sparse_windows.  The
+ * PBz>0 image keeps the reference's start at j=0 and break at the first galaxy out of the window.
+ *
+ * Nothing in the product may load this file: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it, and only as the checker.
+ */
+#include <math.h>
+#include <omp.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+
+#include "../include/cpc.h"
+
+#define NF CPC_NFIELD
+
+typedef struct {
+  int n_rp, n_pi, R;
+  int64_t *num;
+  double *wt;
+  double *val;
+  double *err;
+  int64_t *num_rp;
+} ohist;
+
+static int hist_alloc(ohist *h, int n_rp, int n_pi, int R) {
+  size_t cells = (size_t)R * n_rp * n_pi;
+  h->n_rp = n_rp;
+  h->n_pi = n_pi;
+  h->R = R;
+  h->num = calloc(cells, sizeof(int64_t));
+  h->wt = calloc(cells, sizeof(double));
+  h->val = calloc(cells * NF, sizeof(double));
+  h->err = calloc(cells * NF, sizeof(double));
+  h->num_rp = calloc((size_t)R * n_rp, sizeof(int64_t));
+  return h->num && h->wt && h->val && h->err && h->num_rp;
+}
+
+static void hist_free(ohist *h) {
+  free(h->num);
+  free(h->wt);
+  free(h->val);
+  free(h->err);
+  free(h->num_rp);
+}
+
+/* absolute() of calcs.cpp:50-53: returns -n for n<0, n otherwise (so -0.0 stays -0.0) */
+static inline double absolute_(double n) { return n < 0 ? -1 * n : n; }
+
+/* ephi_calc, calcs.cpp:97-105 */
+static inline double ephi_(double phi, double theta) {
+  double t = absolute_(phi - theta);
+  if (t > M_PI / 2.0) t = M_PI - t;
+  return t;
+}
+
+/* elip_rotate, calcs.cpp:60-64 */
+static inline void rotate_(double e1, double e2, double theta, double *ep, double *ex) {
+  ep[0] = cos(2.0 * theta) * e1 - sin(2.0 * theta) * e2;
+  ex[0] = sin(2.0 * theta) * e1 + cos(2.0 * theta) * e2;
+}
+
+typedef struct {
+  const cpc_catalog_desc *c;
+} ocat;
+
+static inline double cw(const cpc_catalog_desc *c, int64_t i) { return c->w ? c->w[i] : 1.0; }
+static inline double cphi(const cpc_catalog_desc *c, int64_t i) { return c->phi ? c->phi[i] : -10.0; }
+static inline double cjp(const cpc_catalog_desc *c, int64_t i) { return c->jk_prob ? c->jk_prob[i] : 0.0; }
+static inline double cinc(const cpc_catalog_desc *c, int64_t i) {
+  return c->include_prob ? c->include_prob[i] : 0.0;
+}
+
+/* first index j of the z-sorted catalog with z[j] + shift >= zmin (all galaxies, as the reference's
+ * starts pointer scans every galaxy, corr.cpp:361-369, 395-404) */
+static int64_t lower_bound_(const double *z, int64_t n, double shift, double zmin) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = lo + (hi - lo) / 2;
+    if (z[mid] + shift < zmin)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+typedef struct {
+  double L, pmin, pmax, psize, rmin, rmax, rsize;
+  int n_rp, n_pi, lin, kind, gws, est, which_corr, jk_on;
+  double inc_max;
+} oparm;
+
+/* One pair that passed the window predicates: corel_jk (jk.cpp:201-223) with the kind's corel.
+ * Returns 1 if the pair was binned. */
+static inline int corel_pair(const oparm *P, const cpc_catalog_desc *g1, int64_t i, const cpc_catalog_desc *g2,
+                             int64_t j, int PBx, int PBy, int PBz, ohist *h) {
+  const double L = P->L;
+  /* pi_calc_PB, PB_calcs.cpp:87-91 */
+  double d_P = (g2->z[j] + L * PBz) - g1->z[i];
+  /* rp_calc_PB, PB_calcs.cpp:93-98 */
+  double t0 = g1->x[i] - (g2->x[j] + L * PBx);
+  double t1 = g1->y[i] - (g2->y[j] + L * PBy);
+  double da_R = sqrt(t0 * t0 + t1 * t1);
+  /* corel_bin, corels.cpp:26-38 */
+  double q2 = floor((d_P - P->pmin) / P->psize);
+  if (!(q2 >= 0 && q2 < P->n_pi)) return 0;
+  double q1;
+  if (P->lin)
+    q1 = floor((da_R - P->rmin) / P->rsize);
+  else
+    q1 = floor(log10(da_R / P->rmin) / P->rsize);
+  if (!(q1 >= 0 && q1 < P->n_rp)) return 0;
+  int n1 = (int)q1, n2 = (int)q2;
+  double wt = cw(g1, i) * cw(g2, j);
+
+  /* shape fields: ep, ex, epp, exx, epx, etheta (val, err); unset ones stay 0 as calc_temp_ini leaves them */
+  double v[NF] = {0}, e[NF] = {0};
+  if (P->kind == CPC_KIND_SHAPE_DENSITY) {
+    /* shape_density_corel, corels.cpp:206-244 */
+    double theta = atan((g2->y[j] - g1->y[i]) / (g2->x[j] - g1->x[i])); /* calcs.cpp:54-57 */
+    double ep, ex;
+    if (P->gws == 2)
+      rotate_(g2->e1[j], g2->e2[j], theta, &ep, &ex);
+    else
+      rotate_(g1->e1[i], g1->e2[i], theta, &ep, &ex);
+    v[1] = 1.0 * wt * ep;
+    e[1] = (wt * ep) * (wt * ep);
+    v[2] = 1.0 * wt * ex;
+    e[2] = (wt * ex) * (wt * ex);
+    v[6] = ephi_(cphi(g1, i), theta); /* always g1's phi, corels.cpp:229 */
+    e[6] = v[6] * v[6];
+  } else if (P->kind == CPC_KIND_SHAPE_SHAPE) {
+    /* shape_shape_corel, corels.cpp:122-204 */
+    double theta = atan((g2->y[j] - g1->y[i]) / (g2->x[j] - g1->x[i]));
+    double epi, exi, epj, exj;
+    rotate_(g1->e1[i], g1->e2[i], theta, &epi, &exi);
+    rotate_(g2->e1[j], g2->e2[j], theta, &epj, &exj);
+    if (P->est == 0) {
+      if (P->gws == 2) {
+        v[1] = 1.0 * wt * (epj);
+        e[1] = (wt * epj) * (wt * epj);
+        v[6] = ephi_(cphi(g2, j), theta);
+        e[6] = v[6] * v[6];
+      } else if (P->gws == 1) {
+        v[1] = 1.0 * wt * (epi);
+        e[1] = (wt * epi) * (wt * epi);
+        v[6] = ephi_(cphi(g1, i), theta);
+        e[6] = v[6] * v[6];
+      }
+      /* ex is never assigned on this branch: data[2] receives 0 */
+    } else {
+      v[1] = 1.0 * wt * (epi + epj) / 2.0;
+      e[1] = (wt * epi / 2.0) * (wt * epi / 2.0) + (wt * epj / 2.0) * (wt * epj / 2.0);
+      v[2] = 1.0 * wt * (exi + exj) / 2.0;
+      e[2] = (wt * exi / 2.0) * (wt * exi / 2.0) + (wt * exj / 2.0) * (wt * exj / 2.0);
+      double t1_ = ephi_(cphi(g1, i), theta);
+      double t2_ = ephi_(cphi(g2, j), theta);
+      e[6] = t1_ * t1_ + t2_ * t2_;
+      v[6] = t1_ + t2_;
+    }
+    v[3] = 1.0 * wt * (epi * epj);
+    e[3] = (wt * (epi * epj)) * (wt * (epi * epj));
+    v[4] = 1.0 * wt * (exi * exj);
+    e[4] = (wt * (exi * exj)) * (wt * (exi * exj));
+    v[5] = 1.0 * wt * (exi * epj + epi * exj) / 2.0;
+    {
+      double a = wt * (exi * epj) / 2.0, b = wt * epi * exj / 2.0;
+      e[5] = a * a + b * b;
+    }
+  }
+
+  const size_t nc = (size_t)P->n_rp * P->n_pi;
+  size_t c = (size_t)n1 * P->n_pi + n2;
+  /* density_density_corel, corels.cpp:67-73 */
+  h->num_rp[n1]++;
+  h->num[c]++;
+  h->wt[c] += wt;
+  if (P->kind != CPC_KIND_DD) {
+    for (int f = 1; f < NF; f++) {
+      h->val[c * NF + f] += v[f];
+      h->err[c * NF + f] += e[f];
+    }
+  }
+  if (P->jk_on) {
+    /* corel_jk + jk_check_PB + corel_jk1 (jk.cpp:64-71, 138-171, 201-223) */
+    int gjk[2] = {g1->jk[i], g2->jk[j]};
+    double jp[2] = {cjp(g1, i), cjp(g2, j)};
+    if (gjk[0] == gjk[1]) {
+      jp[0] = 1;
+      jp[1] = 0;
+    }
+    for (int s = 0; s < 2; s++) {
+      if (jp[s] < 0.5) continue;
+      int r = 1 + gjk[s];
+      size_t cr = (size_t)r * nc + c;
+      h->num_rp[(size_t)r * P->n_rp + n1]++;
+      h->num[cr]++;
+      h->wt[cr] += wt;
+      if (P->which_corr == 0) continue;
+      for (int f = 1; f < NF; f++) h->val[cr * NF + f] += v[f];
+    }
+  }
+  return 1;
+}
+
+static double now_ms(void) {
+  struct timeval tv;
+  gettimeofday(&tv, NULL);
+  return tv.tv_sec * 1e3 + tv.tv_usec * 1e-3;
+}
+
+/* Validate and derive the data_info quantities (initialization.cpp:305-311). */
+static int make_params(const cpc_params *p, oparm *P) {
+  if (p->geometry != CPC_GEOM_BOX) return CPC_ERR_UNSUPPORTED;
+  if (p->n_rp <= 0 || p->n_pi <= 0 || !(p->rp_max > 0) || p->rp_min > p->rp_max) return CPC_ERR_INVALID;
+  if (p->do_jk != 0 && p->do_jk != 1) return CPC_ERR_UNSUPPORTED;
+  P->L = p->box_size;
+  P->pmin = p->pi_min;
+  P->pmax = p->pi_max;
+  P->psize = (p->pi_max - p->pi_min) / (p->n_pi);
+  P->rmin = p->rp_min;
+  P->rmax = p->rp_max;
+  if (p->lin_bin)
+    P->rsize = (p->rp_max - p->rp_min) / p->n_rp;
+  else
+    P->rsize = (log10(p->rp_max) - log10(p->rp_min)) / p->n_rp;
+  P->n_rp = p->n_rp;
+  P->n_pi = p->n_pi;
+  P->lin = p->lin_bin;
+  P->kind = p->kind;
+  P->gws = p->g_with_shape;
+  P->est = p->estimator;
+  P->which_corr = p->which_corr;
+  P->jk_on = (p->do_jk == 1 && p->n_jk > 0);
+  P->inc_max = p->include_prob_max;
+  return CPC_OK;
+}
+
+/*
+ * orc_count: the cross pass of corels2 for outer = g1, inner = g2 (after the swap), box geometry.
+ * Writes `out` in cpc_hist layout; returns a cpc_status.
+ */
+int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_params *params, cpc_hist *out,
+              cpc_stats *st, int n_threads) {
+  oparm P;
+  int rc = make_params(params, &P);
+  if (rc != CPC_OK) return rc;
+  const int R = P.jk_on ? 1 + params->n_jk : 1;
+  const size_t nc = (size_t)P.n_rp * P.n_pi;
+  const int64_t n1 = g1->n, n2 = g2->n;
+  if (P.jk_on) {
+    for (int64_t i = 0; i < n1; i++)
+      if (g1->jk[i] < 0 || g1->jk[i] >= params->n_jk) return CPC_ERR_INVALID;
+    for (int64_t j = 0; j < n2; j++)
+      if (g2->jk[j] < 0 || g2->jk[j] >= params->n_jk) return CPC_ERR_INVALID;
+  }
+  if (n_threads <= 0) n_threads = omp_get_num_procs();
+  double t0 = now_ms();
+
+  ohist *th = calloc(n_threads, sizeof(ohist));
+  int64_t *tbinned = calloc(n_threads, sizeof(int64_t));
+  int64_t *tcand = calloc(n_threads, sizeof(int64_t));
+  int64_t *tsparse = calloc(n_threads, sizeof(int64_t));
+  int ok = th && tbinned && tcand && tsparse;
+  for (int t = 0; ok && t < n_threads; t++) ok = hist_alloc(&th[t], P.n_rp, P.n_pi, R);
+  if (!ok) return CPC_ERR_NOMEM;
+
+  const double L = P.L;
+  long chunk = (long)floor((double)n1 / (n_threads * 40)); /* schedule of corr.cpp:331 */
+  if (chunk < 1) chunk = 1;
+
+  if (n1 > 0 && n2 > 0) {
+#pragma omp parallel num_threads(n_threads)
+    {
+      int t = omp_get_thread_num();
+      ohist *h = &th[t];
+      int64_t binned = 0, cand = 0, sparse = 0;
+#pragma omp for schedule(dynamic, chunk)
+      for (int64_t i = 0; i < n1; i++) {
+        if (cinc(g1, i) > P.inc_max) continue;
+        /* lim_calc_xyz + lim_calc_PB, PB_calcs.cpp:12-61 */
+        double z_max = g1->z[i] + P.pmax, z_min = g1->z[i] + P.pmin;
+        double ra_max = g1->x[i] + P.rmax, ra_min = g1->x[i] - P.rmax;
+        double dec_max = g1->y[i] + P.rmax, dec_min = g1->y[i] - P.rmax;
+        int PBx = 0, PBy = 0, PBz = 0;
+        if (z_min < 0) PBz = -1;
+        if (z_max > L) PBz = abs(PBz) + 1;
+        if (ra_min < 0) PBx = -1;
+        if (ra_max > L) PBx = abs(PBx) + 1;
+        if (dec_min < 0) PBy = -1;
+        if (dec_max > L) PBy = abs(PBy) + 1;
+        int xt = PBx, yt = PBy, zt = PBz;
+        /* image loops, corr.cpp:380-423 */
+        int cy = yt > 0;
+        for (int yi = 0; yi <= abs(yt); yi++, cy--) {
+          int cx = xt > 0;
+          for (int xi = 0; xi <= abs(xt); xi++, cx--) {
+            int cz = zt > 0;
+            for (int zi = 0; zi <= abs(zt); zi++, cz--) {
+              const double sz = L * cz, sy = L * cy, sx = L * cx;
+              int64_t j;
+              if (cz > 0) {
+                j = 0;
+              } else {
+                j = lower_bound_(g2->z, n2, sz, z_min);
+                if (j < n2 && g2->z[j] + sz > z_max) sparse++;
+              }
+              for (; j < n2; j++) {
+                if (cinc(g2, j) > P.inc_max) continue;
+                double zz = g2->z[j] + sz;
+                if (zz > z_max || zz < z_min) break; /* do_pair_sorted, corr.cpp:71-73 */
+                double yy = g2->y[j] + sy;
+                if (yy > dec_max || yy < dec_min) continue;
+                double xx = g2->x[j] + sx;
+                if (xx < ra_min || xx > ra_max) continue;
+                cand++;
+                binned += corel_pair(&P, g1, i, g2, j, cx, cy, cz, h);
+              }
+            }
+          }
+        }
+      }
+      tbinned[t] = binned;
+      tcand[t] = cand;
+      tsparse[t] = sparse;
+    }
+  }
+
+  /* merge in thread order (comb_thread_bins, corr.cpp:88-100) */
+  ohist acc;
+  if (!hist_alloc(&acc, P.n_rp, P.n_pi, R)) return CPC_ERR_NOMEM;
+  int64_t binned = 0, cand = 0, sparse = 0;
+  for (int t = 0; t < n_threads; t++) {
+    for (size_t k = 0; k < (size_t)R * nc; k++) {
+      acc.num[k] += th[t].num[k];
+      acc.wt[k] += th[t].wt[k];
+      for (int f = 0; f < NF; f++) {
+        acc.val[k * NF + f] += th[t].val[k * NF + f];
+        acc.err[k * NF + f] += th[t].err[k * NF + f];
+      }
+    }
+    for (size_t k = 0; k < (size_t)R * P.n_rp; k++) acc.num_rp[k] += th[t].num_rp[k];
+    binned += tbinned[t];
+    cand += tcand[t];
+    sparse += tsparse[t];
+    hist_free(&th[t]);
+  }
+  free(th);
+  free(tbinned);
+  free(tcand);
+  free(tsparse);
+
+  /* final_jk_bins (jk.cpp:74-81): jk_r = full - touch_r over every field, add_bins(sub=true) */
+  if (P.jk_on && params->jk_mode == CPC_JK_LEAVE_ONE_OUT) {
+    for (int r = 1; r < R; r++) {
+      for (size_t k = 0; k < nc; k++) {
+        size_t a = (size_t)r * nc + k;
+        acc.num[a] = acc.num[k] + acc.num[a] * -1;
+        acc.wt[a] = acc.wt[k] + acc.wt[a] * -1;
+        for (int f = 0; f < NF; f++) {
+          acc.val[a * NF + f] = acc.val[k * NF + f] + acc.val[a * NF + f] * -1;
+          acc.err[a * NF + f] = acc.err[k * NF + f] + acc.err[a * NF + f] * -1;
+        }
+      }
+      for (int k = 0; k < P.n_rp; k++)
+        acc.num_rp[(size_t)r * P.n_rp + k] = acc.num_rp[k] + acc.num_rp[(size_t)r * P.n_rp + k] * -1;
+    }
+  }
+  memcpy(out->num, acc.num, (size_t)R * nc * sizeof(int64_t));
+  memcpy(out->wt_num, acc.wt, (size_t)R * nc * sizeof(double));
+  memcpy(out->val, acc.val, (size_t)R * nc * NF * sizeof(double));
+  memcpy(out->err, acc.err, (size_t)R * nc * NF * sizeof(double));
+  memcpy(out->num_rp, acc.num_rp, (size_t)R * P.n_rp * sizeof(int64_t));
+  hist_free(&acc);
+  if (st) {
+    memset(st, 0, sizeof(*st));
+    st->n_binned = binned;
+    st->n_candidates = cand;
+    st->n_sparse_windows = sparse;
+    st->ms_total = now_ms() - t0;
+  }
+  return CPC_OK;
+}
+
+/* The reference's rp bin index for a squared separation s (corels.cpp:31-34 after rp_calc_PB's sqrt),
+ * as a double so that out-of-range values stay comparable.  Used by tests to pin the threshold table. */
+double orc_rp_bin(const cpc_params *p, double s) {
+  oparm P;
+  if (make_params(p, &P) != CPC_OK) return NAN;
+  double rp = sqrt(s);
+  if (P.lin) return floor((rp - P.rmin) / P.rsize);
+  return floor(log10(rp / P.rmin) / P.rsize);
+}
+
+/* The reference's pi bin index (corels.cpp:26). */
+double orc_pi_bin(const cpc_params *p, double d_P) {
+  oparm P;
+  if (make_params(p, &P) != CPC_OK) return NAN;
+  return floor((d_P - P.pmin) / P.psize);
+}

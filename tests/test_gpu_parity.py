@@ -1,0 +1,152 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle on the same inputs.
+
+Counts must be bit-identical; weighted sums within helpers.RTOL (1e-10 of the bin's accumulated
+magnitude).  Every case runs both the split-precision and the plain FP64 kernel.
+"""
+import numpy as np
+import pytest
+
+from corr_pc_amd import synth
+from corr_pc_amd.engine import (JK_LEAVE_ONE_OUT, JK_TOUCH, KIND_DD, KIND_SHAPE_DENSITY, KIND_SHAPE_SHAPE, PREC_FP64,
+                                PREC_SPLIT, make_params)
+from corr_pc_amd.passes import config_datainfo, setup_pass
+from helpers import assert_hist_match, small_catalog
+
+pytestmark = pytest.mark.gpu
+
+PRECS = [PREC_SPLIT, PREC_FP64]
+
+
+def _run(engine, oracle_lib, outer, inner, p, what):
+    print(f"[case] {what}", flush=True)
+    got, st = engine.count(outer, inner, p)
+    ref, rst = oracle_lib.count(outer, inner, p)
+    assert_hist_match(got, ref, what=what)
+    assert st["n_binned"] == rst["n_binned"]
+    return st
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("kind,est,gws", [(KIND_DD, 0, 2), (KIND_SHAPE_DENSITY, 0, 2), (KIND_SHAPE_DENSITY, 0, 1),
+                                          (KIND_SHAPE_SHAPE, 0, 2), (KIND_SHAPE_SHAPE, 0, 1),
+                                          (KIND_SHAPE_SHAPE, 1, 2)])
+def test_small_box_kinds(engine, oracle_lib, prec, kind, est, gws):
+    L = 60.0
+    A = small_catalog(3000, L, 11, clumps=40)
+    B = small_catalog(2500, L, 12, clumps=30)
+    p = make_params(L, 8, 0.2, 12.0, 10, -15.0, 15.0, kind=kind, g_with_shape=gws, estimator=est, do_jk=1, n_jk=8,
+                    precision=prec)
+    _run(engine, oracle_lib, A, B, p, f"kind={kind} est={est} gws={gws} prec={prec}")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("jk_mode", [JK_LEAVE_ONE_OUT, JK_TOUCH])
+def test_jackknife_modes(engine, oracle_lib, prec, jk_mode):
+    L = 40.0
+    A = small_catalog(2000, L, 21, jk_side=3)
+    p = make_params(L, 6, 0.3, 8.0, 8, -10.0, 10.0, kind=KIND_SHAPE_SHAPE, do_jk=1, n_jk=27, precision=prec,
+                    jk_mode=jk_mode)
+    # same handle on both sides: self pairs are rejected by rp = 0
+    _run(engine, oracle_lib, A, A, p, f"jk_mode={jk_mode} prec={prec}")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_no_jackknife_linear_bins(engine, oracle_lib, prec):
+    L = 50.0
+    A = small_catalog(2500, L, 31, jk_side=1)
+    B = small_catalog(1500, L, 32, jk_side=1, shapes=False)
+    p = make_params(L, 7, 0.0, 9.0, 9, -9.0, 9.0, kind=KIND_SHAPE_DENSITY, g_with_shape=1, lin_bin=1, do_jk=0,
+                    precision=prec)
+    _run(engine, oracle_lib, A, B, p, f"lin prec={prec}")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_include_prob_subsample(engine, oracle_lib, prec):
+    L = 50.0
+    A = small_catalog(2500, L, 41, shapes=False, inc=True)
+    B = small_catalog(2000, L, 42, shapes=False, inc=True)
+    p = make_params(L, 6, 0.2, 10.0, 8, -12.0, 12.0, kind=KIND_DD, do_jk=1, n_jk=8, include_prob_max=0.4,
+                    precision=prec)
+    _run(engine, oracle_lib, A, B, p, f"include prec={prec}")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_edge_cases(engine, oracle_lib, prec):
+    """Galaxies on the box faces and at exact bin edges, duplicates, one-sided pi range."""
+    L = 32.0
+    r = np.random.default_rng(5)
+    n = 1200
+    pos = r.uniform(0, L, (n, 3))
+    pos[:50, 0] = 0.0
+    pos[50:100, 1] = 0.0
+    pos[100:150, 2] = 0.0
+    pos[150:200, 0] = np.nextafter(L, 0)
+    # exact separations on bin edges: partners at rp = 1, 2, 4 and pi = 0, +-2
+    for k, (dx, dz) in enumerate([(1.0, 0.0), (2.0, 2.0), (4.0, -2.0), (0.5, 4.0)]):
+        i = 200 + 2 * k
+        pos[i] = [10.0, 10.0, 10.0]
+        pos[i + 1] = [10.0 + dx, 10.0, 10.0 + dz]
+    pos[300:320] = pos[320:340]  # duplicates: rp = 0
+    pos = pos[np.argsort(pos[:, 2], kind="stable")]
+    from corr_pc_amd.engine import Catalog
+    A = Catalog(x=pos[:, 0], y=pos[:, 1], z=pos[:, 2], jk=np.zeros(n, np.int32), jk_prob=r.uniform(0, 1, n))
+    for pmin, pmax in [(-8.0, 8.0), (0.0, 8.0), (-6.0, 10.0)]:
+        p = make_params(L, 5, 1.0, 8.0, 8, pmin, pmax, kind=KIND_DD, lin_bin=0, do_jk=1, n_jk=1, precision=prec)
+        _run(engine, oracle_lib, A, A, p, f"edges pi=[{pmin},{pmax}) prec={prec}")
+        p = make_params(L, 8, 0.0, 8.0, 8, pmin, pmax, kind=KIND_DD, lin_bin=1, do_jk=0, precision=prec)
+        _run(engine, oracle_lib, A, A, p, f"edges lin pi=[{pmin},{pmax}) prec={prec}")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_empty_and_tiny(engine, oracle_lib, prec):
+    L = 20.0
+    A = small_catalog(1, L, 51)
+    B = small_catalog(3, L, 52)
+    p = make_params(L, 4, 0.1, 5.0, 4, -5.0, 5.0, kind=KIND_SHAPE_SHAPE, do_jk=1, n_jk=8, precision=prec)
+    _run(engine, oracle_lib, A, B, p, "tiny")
+    E = A.subset(np.arange(0))
+    got, st = engine.count(E, B, p)
+    assert got.num.sum() == 0 and st["n_binned"] == 0
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_config0_passes(engine, oracle_lib, prec):
+    """The CPU-runnable wgg box at full size: SD (DD), SRd (DR) and RsRd (RR) passes."""
+    cats = synth.catalogs(0)
+    di = config_datainfo(cats["config"])
+    for name, g1, g2 in (("SD", cats["D"], cats["S"]), ("SRd", cats["Rd"], cats["S"]),
+                         ("RsRd", cats["Rd"], cats["Rs"])):
+        outer, inner, p, _ = setup_pass(name, g1, g2, di, precision=prec)
+        _run(engine, oracle_lib, outer, inner, p, f"config0 {name} prec={prec}")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_config1_subsample(engine, oracle_lib, prec):
+    """The 1M-galaxy wg+/w++ box: the full inner catalog against every 500th outer galaxy (SD pass,
+    shape_shape) and every 1500th random (SRd pass, shape_density)."""
+    cats = synth.catalogs(1)
+    di = config_datainfo(cats["config"])
+    D, S, Rd = cats["D"], cats["S"], cats["Rd"]
+    outer, inner, p, _ = setup_pass("SD", D, S, di, precision=prec)
+    sub = outer.subset(np.arange(0, outer.n, 500))
+    _run(engine, oracle_lib, sub, inner, p, f"config1 SD/500 prec={prec}")
+    outer, inner, p, _ = setup_pass("SRd", Rd, S, di, precision=prec)
+    sub = outer.subset(np.arange(0, outer.n, 1500))
+    _run(engine, oracle_lib, sub, inner, p, f"config1 SRd/1500 prec={prec}")
+
+
+def test_split_matches_fp64_full_config1(engine):
+    """Full 1M x 1M SD pass: the split kernel against the plain FP64 kernel (size-independent check:
+    identical counts everywhere, weighted sums within tolerance)."""
+    cats = synth.catalogs(1)
+    di = config_datainfo(cats["config"])
+    outer, inner, p, _ = setup_pass("SD", cats["D"], cats["S"], di, precision=PREC_SPLIT)
+    h_split, st_s = engine.count(outer, inner, p)
+    p.precision = PREC_FP64
+    h_64, st_d = engine.count(outer, inner, p)
+    assert_hist_match(h_split, h_64, what="config1 full SD split vs fp64")
+    assert st_s["n_binned"] == st_d["n_binned"] > 4e10
+    # invariants of the jackknife layout: leave-one-out counts never exceed the full sample and the
+    # rp-level counts are the pi sums
+    assert np.all(h_split.num[1:] <= h_split.num[0][None])
+    assert np.array_equal(h_split.num_rp, h_split.num.sum(axis=2))
