@@ -187,7 +187,7 @@ struct cpc_ctx {
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   double col_frac = 0.0;
-  int item_size = 0, block_threads = 0;
+  int item_size = 0, block_threads = 0, acc_mode = -1;
   // scratch
   DevBuf<uint32_t> keys_a, keys_b;
   DevBuf<int32_t> vals_a, vals_b, order;
@@ -281,7 +281,7 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, int nc, double h, int nreg
     }
     S.colbeg[(size_t)nc * nc] = (int)S.groups.size();
   }
-  if (check(S.rec.ensure(ni), st, "alloc") || check(S.fr.ensure(ni), st, "alloc") ||
+  if (check(S.rec.ensure(ni + kRecPad), st, "alloc") || check(S.fr.ensure(ni + kRecPad), st, "alloc") ||
       check(S.d_groups.ensure(S.groups.size()), st, "alloc") || check(S.d_colbeg.ensure(S.colbeg.size()), st, "alloc"))
     return st;
   GatherArgs ga{c->d_x, c->d_y,  c->d_z, c->d_w, c->d_e1, c->d_e2, c->d_phi, ni, nc, h,
@@ -359,14 +359,16 @@ void cpc_destroy(cpc_ctx *ctx) {
   delete ctx;
 }
 
-cpc_status cpc_set_tuning(cpc_ctx *ctx, double col_frac, int item_size, int block_threads) {
+cpc_status cpc_set_tuning(cpc_ctx *ctx, double col_frac, int item_size, int block_threads, int acc_mode) {
   if (!ctx) return CPC_ERR_INVALID;
   if (col_frac < 0 || item_size < 0 || block_threads < 0) return fail(ctx, CPC_ERR_INVALID, "negative tuning value");
   if (block_threads && (block_threads % 64 || block_threads > 256))
     return fail(ctx, CPC_ERR_INVALID, "block_threads must be a multiple of 64 and at most 256");
+  if (acc_mode < -1 || acc_mode > 1) return fail(ctx, CPC_ERR_INVALID, "acc_mode must be -1 (automatic), 0 or 1");
   ctx->col_frac = col_frac;
   ctx->item_size = item_size;
   ctx->block_threads = block_threads;
+  ctx->acc_mode = acc_mode;
   return CPC_OK;
 }
 
@@ -543,7 +545,10 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
   const int NSP = 1 + 2 * NV, NST = 1 + NV;
   const bool split = p.precision == CPC_PREC_SPLIT;
   const int threads = ctx->block_threads ? ctx->block_threads : 256;
-  const size_t lds_p = pair_lds_bytes(ncell, NSP, p.n_rp);
+  int acc_mode = ctx->acc_mode >= 0 ? ctx->acc_mode : 0;
+  if (ncell > 65535) acc_mode = 0;  // the compaction ring packs the cell index in 16 bits
+  if (pair_lds_bytes(ncell, NSP, p.n_rp, threads / 64, acc_mode) > ctx->max_lds) acc_mode = 0;
+  const size_t lds_p = pair_lds_bytes(ncell, NSP, p.n_rp, threads / 64, acc_mode);
   if (lds_p > ctx->max_lds)
     return fail(ctx, CPC_ERR_UNSUPPORTED,
                 "histogram of " + std::to_string(ncell) + " cells does not fit in LDS (" + std::to_string(lds_p) +
@@ -700,6 +705,7 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
   K.thr_hi = ctx->thr_hi.p;
   K.nbr = ctx->nbr.p;
   K.counters = ctx->counters.p;
+  K.acc_mode = acc_mode;
   const bool use_split = split && split_ok;
 
   check(hipEventRecord(ctx->ev0, s), st, "event");

@@ -10,19 +10,25 @@ namespace cpc {
 // A galaxy as the pair kernel reads it: one 64-byte record, so that a wave fetches a whole inner
 // galaxy with a single scalar load.  Gathered into (column, region, coin, z) order per pass.
 struct alignas(64) PRec {
-  double x, y, z;      // gal.ra.val_deg, gal.dec.val_deg, gal.redshift
+  double x, y;         // gal.ra.val_deg, gal.dec.val_deg
   double w;            // gal.wt
   double e1, e2;       // gal.e[0], gal.e[1] (0 when the catalog has no shapes)
   double phi;          // gal.phi
+  double z;            // gal.redshift
   int32_t meta;        // image flags of this galaxy as an outer galaxy (kMeta*)
   int32_t idx;         // index in the uploaded catalog
 };
 
 // FP32 companion of PRec for the split-precision kernel: x, y relative to the galaxy's column
-// corner, z absolute.
+// corner, z absolute, and the image flags.
 struct alignas(16) FRec {
-  float xl, yl, zf, pad;
+  float xl, yl, zf;
+  int32_t meta;
 };
+
+// Sorted record arrays carry this many padding records so that kernels may load FP32 records four
+// at a time past the end of a range.
+constexpr int kRecPad = 4;
 
 // Image flags of an outer galaxy i, computed exactly as lim_calc_PB (PB_calcs.cpp:30-61) decides
 // which periodic images of the inner catalog it visits (corr.cpp:380-423).
@@ -63,6 +69,7 @@ struct KArgs {
   int phase_mode;      // 0 primary pass without jackknife, 1 primary with jackknife, 2 touch pass
   int nslots;          // f64 slots accumulated by this pass
   int lin;
+  int acc_mode;        // 0 run-length privatisation, 1 per-wave compaction of binned pairs
   // split-precision classification
   float pmin32, inv_psize32, gz;   // pi bin: u = (dz - pmin)*inv_psize, guard gz in bin units
   float la, lb;                    // rp bin estimate from log2(s) (log bins) ...
@@ -121,7 +128,7 @@ hipError_t sort_u32(void *&tmp, size_t &tmp_bytes, const uint32_t *kin, uint32_t
                     int32_t *vout, int64_t n, hipStream_t st);
 hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *kout, const int32_t *vin,
                     int32_t *vout, int64_t n, hipStream_t st);
-size_t pair_lds_bytes(int ncell, int nslots, int n_rp);
+size_t pair_lds_bytes(int ncell, int nslots, int n_rp, int nwave, int acc_mode);
 hipError_t launch_pair(int kind, bool split, bool swap, const KArgs &P, int threads, const PRec *brec,
                        const FRec *bfr, const PRec *trec, const FRec *tfr, hipStream_t st);
 

@@ -44,6 +44,15 @@ struct KindNV<kSS0> { static constexpr int NV = 5; };  // data[1], data[3..6]
 template <>
 struct KindNV<kSS1> { static constexpr int NV = 6; };  // data[1..6]
 
+// 1/x to about one ulp: hardware reciprocal estimate refined by two Newton steps (x finite, != 0).
+__device__ __forceinline__ double rcp_refined(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
 struct Gal {
   double x, y, w, e1, e2, phi;
 };
@@ -67,15 +76,17 @@ __device__ __forceinline__ void pair_values(const Gal &g1, const Gal &g2, int gw
   } else {
     // angle_orient_axis_PB, calcs.cpp:54-57: raw coordinates, no image shift
     const double dxr = g2.x - g1.x, dyr = g2.y - g1.y;
-    const double theta = atan(dyr / dxr);
-    double c2, s2;
+    double theta, c2, s2;
     if constexpr (SPLIT) {
-      // cos 2theta and sin 2theta of theta = atan(dy/dx), algebraically
+      // theta = atan(dy/dx) with a refined reciprocal; cos 2theta and sin 2theta algebraically
+      // (about one ulp each, far inside the weighted-sum tolerance)
+      theta = atan((dxr != 0.0) ? dyr * rcp_refined(dxr) : copysign(INFINITY, dyr));
       const double xx = dxr * dxr, yy = dyr * dyr;
-      const double inv = 1.0 / (xx + yy);
+      const double inv = rcp_refined(xx + yy);
       c2 = (xx - yy) * inv;
       s2 = (2.0 * dxr * dyr) * inv;
     } else {
+      theta = atan(dyr / dxr);
       c2 = cos(2.0 * theta);
       s2 = sin(2.0 * theta);
     }
@@ -196,13 +207,27 @@ __device__ __forceinline__ int zsearch(const PRec *__restrict__ rec, int lo, int
   return lo;
 }
 
-template <int K, bool SPLIT, bool SWAP>
+template <int NS>
+__device__ __forceinline__ void lds_add(unsigned *lnum, double *lacc, int ncell, int cell, unsigned n,
+                                        const double *v) {
+  atomicAdd(&lnum[cell], n);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) atomicAdd(&lacc[s * ncell + cell], v[s]);
+}
+
+struct F4 {
+  FRec r[4];
+};
+
+// ACC: 0 = run-length privatisation in registers, 1 = per-wave compaction of binned pairs.
+template <int K, bool SPLIT, bool SWAP, int ACC>
 __global__ void __launch_bounds__(256) pair_kernel(KArgs P, const PRec *__restrict__ brec,
                                                    const FRec *__restrict__ bfr, const PRec *__restrict__ trec,
                                                    const FRec *__restrict__ tfr) {
   constexpr int NV = KindNV<K>::NV;
   constexpr int NS = SWAP ? 1 + NV : 1 + 2 * NV;  // f64 slots accumulated
   constexpr int NQ = 1 + 2 * NV;                  // quantities computed
+  constexpr int QCAP = 128;                       // compaction ring per wave
   extern __shared__ __align__(16) unsigned char smem[];
   const int ncell = P.ncell;
   unsigned *lnum = reinterpret_cast<unsigned *>(smem);
@@ -210,11 +235,14 @@ __global__ void __launch_bounds__(256) pair_kernel(KArgs P, const PRec *__restri
   double *lthr = lacc + (size_t)NS * ncell;
   float *llo = reinterpret_cast<float *>(lthr + P.n_rp + 1);
   float *lhi = llo + P.n_rp + 1;
+  unsigned long long *lq =
+      reinterpret_cast<unsigned long long *>(reinterpret_cast<uintptr_t>(lhi + P.n_rp + 1 + 1) & ~uintptr_t(7));
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = uni(tid >> 6);
   const int nwave = blockDim.x >> 6;
+  unsigned long long *wq = lq + wave * QCAP;
 
   for (int k = tid; k <= P.n_rp; k += blockDim.x) {
     lthr[k] = P.thr[k];
@@ -224,6 +252,9 @@ __global__ void __launch_bounds__(256) pair_kernel(KArgs P, const PRec *__restri
     }
   }
 
+  // outermost split-precision edges, wave-uniform: s below thr[0] or above thr[n_rp] for sure
+  const float hi_bottom = SPLIT ? P.thr_hi[0] : 0.0f;
+  const float lo_top = SPLIT ? P.thr_lo[P.n_rp] : 0.0f;
   const Item it = P.items[blockIdx.x];
   const Group bg = P.bgrp[it.group];
   const int bcx = bg.col % P.nc, bcy = bg.col / P.nc;
@@ -255,7 +286,40 @@ __global__ void __launch_bounds__(256) pair_kernel(KArgs P, const PRec *__restri
       if (SPLIT) mf = bfr[p];
       const double wzlo = brec[base].z;
       const double wzhi = brec[base + cnt - 1].z;
-      const Gal gme{me.x, me.y, me.w, me.e1, me.e2, me.phi};
+
+      // ACC 0: a lane keeps the sums of its current histogram cell in registers and adds them to LDS
+      // only when its cell changes.  Lanes of a wave hold neighbouring galaxies, so their pairs crowd
+      // into few cells; flushing per run instead of per pair divides the same-address LDS atomics.
+      int run_cell = -1;
+      unsigned run_n = 0;
+      double run[ACC == 0 ? NS : 1];
+#pragma unroll
+      for (int s = 0; s < (ACC == 0 ? NS : 1); ++s) run[s] = 0.0;
+      // ACC 1: binned pairs (tile index, lane, cell) go to a per-wave ring in LDS; every 64 entries
+      // the wave evaluates them with all lanes busy, so the FP64 shape arithmetic runs on full waves.
+      int qn = 0, qr = 0;
+
+      auto run_batch = [&](int nvalid) {
+        const bool act = lane < nvalid;
+        const unsigned long long e = wq[(qr + lane) & (QCAP - 1)];
+        if (act) {
+          const int j = (int)(e >> 32);
+          const int src = (int)((e >> 16) & 63);
+          const int cell = (int)(e & 0xffff);
+          const PRec b = brec[base + src];
+          const PRec t = trec[j];
+          const Gal gb{b.x, b.y, b.w, b.e1, b.e2, b.phi};
+          const Gal gt{t.x, t.y, t.w, t.e1, t.e2, t.phi};
+          double qv[NQ];
+          if (!SWAP)
+            pair_values<K, SPLIT>(gb, gt, P.gws, qv);
+          else
+            pair_values<K, SPLIT>(gt, gb, P.gws, qv);
+          lds_add<NS>(lnum, lacc, ncell, cell, 1u, qv);
+        }
+        qr = (qr + nvalid) & (QCAP - 1);
+        qn -= nvalid;
+      };
 
       for (int q = 0; q < P.n_nbr; ++q) {
         const int2 o = P.nbr[q];
@@ -288,59 +352,108 @@ __global__ void __launch_bounds__(256) pair_kernel(KArgs P, const PRec *__restri
             const int PBx = SWAP ? -sx : sx, PBy = SWAP ? -sy : sy, PBz = SWAP ? -sz : sz;
             const bool lane_ok = valid && (SWAP || image_allowed(me.meta, PBx, PBy, PBz));
             const float Oz = (float)(P.L * sz);
-            for (int j = j0; j < j1; ++j) {
-              bool ok = lane_ok;
-              if (SWAP) ok = ok && image_allowed(trec[j].meta, PBx, PBy, PBz);
-              ncand += ok;
-              int cell = -1;
-              bool unc = ok;
-              if (SPLIT && ok) {
-                const FRec tf = tfr[j];
-                const float dx = (tf.xl - mf.xl) + Ox;
-                const float dy = (tf.yl - mf.yl) + Oy;
-                const float dz = (tf.zf - mf.zf) + Oz;
-                const float pif = SWAP ? -dz : dz;
-                const float s32 = fmaf(dx, dx, dy * dy);
-                const float u = (pif - P.pmin32) * P.inv_psize32;
-                const float fu = floorf(u);
-                const float fr = u - fu;
-                const bool near_pi = (fr < P.gz) || (fr > 1.0f - P.gz);
-                const bool pi_in = (fu >= 0.0f) && (fu < (float)P.n_pi);
-                float kf = P.lin ? (sqrtf(s32) - P.lrmin) * P.linv : __log2f(s32) * P.la + P.lb;
-                kf = fminf(fmaxf(kf, -1.0f), (float)P.n_rp);
-                const int k = (int)floorf(kf);
-                const int kk = min(max(k, 0), P.n_rp - 1);
-                const bool rp_out = (s32 < lhi[0]) || (s32 >= llo[P.n_rp]);
-                const bool rp_in = (k == kk) && (s32 >= llo[kk]) && (s32 < lhi[kk + 1]);
-                if ((!near_pi && !pi_in) || rp_out) {
-                  unc = false;
-                } else if (!near_pi && rp_in) {
-                  unc = false;
-                  cell = k * P.n_pi + (int)fu;
-                }
-              }
-              if (unc) {
-                nre += SPLIT;
-                const PRec t = trec[j];
-                if (!SWAP)
-                  cell = exact_cell(P, lthr, me.x, me.y, me.z, t.x, t.y, t.z, PBx, PBy, PBz);
-                else
-                  cell = exact_cell(P, lthr, t.x, t.y, t.z, me.x, me.y, me.z, PBx, PBy, PBz);
-              }
-              if (cell >= 0) {
-                const PRec t = trec[j];
-                const Gal gt{t.x, t.y, t.w, t.e1, t.e2, t.phi};
-                double qv[NQ];
-                if (!SWAP)
-                  pair_values<K, SPLIT>(gme, gt, P.gws, qv);
-                else
-                  pair_values<K, SPLIT>(gt, gme, P.gws, qv);
-                atomicAdd(&lnum[cell], 1u);
+            for (int jb = j0; jb < j1; jb += 4) {
+              F4 f4;
+              if (SPLIT) f4 = *reinterpret_cast<const F4 *>(tfr + jb);  // padded past the end
 #pragma unroll
-                for (int s = 0; s < NS; ++s) atomicAdd(&lacc[s * ncell + cell], qv[s]);
+              for (int u = 0; u < 4; ++u) {
+                const int j = jb + u;
+                if (j >= j1) break;
+                bool ok = lane_ok;
+                int cell = -1;
+                bool unc = ok;
+                if (SPLIT) {
+                  const FRec tf = f4.r[u];
+                  if (SWAP) ok = ok && image_allowed(tf.meta, PBx, PBy, PBz);
+                  unc = ok;
+                  if (ok) {
+                    const float dx = (tf.xl - mf.xl) + Ox;
+                    const float dy = (tf.yl - mf.yl) + Oy;
+                    const float dz = (tf.zf - mf.zf) + Oz;
+                    const float pif = SWAP ? -dz : dz;
+                    const float s32 = fmaf(dx, dx, dy * dy);
+                    const float uu = (pif - P.pmin32) * P.inv_psize32;
+                    const float fu = floorf(uu);
+                    const float fr = uu - fu;
+                    const bool near_pi = (fr < P.gz) || (fr > 1.0f - P.gz);
+                    const bool pi_in = (fu >= 0.0f) && (fu < (float)P.n_pi);
+                    float kf = P.lin ? (sqrtf(s32) - P.lrmin) * P.linv : __log2f(s32) * P.la + P.lb;
+                    kf = fminf(fmaxf(kf, -1.0f), (float)P.n_rp);
+                    const int k = (int)floorf(kf);
+                    const int kk = min(max(k, 0), P.n_rp - 1);
+                    const bool rp_out = (s32 < hi_bottom) || (s32 >= lo_top);
+                    const bool rp_in = (k == kk) && (s32 >= llo[kk]) && (s32 < lhi[kk + 1]);
+                    if ((!near_pi && !pi_in) || rp_out) {
+                      unc = false;
+                    } else if (!near_pi && rp_in) {
+                      unc = false;
+                      cell = k * P.n_pi + (int)fu;
+                    }
+                  }
+                } else if (SWAP) {
+                  ok = ok && image_allowed(trec[j].meta, PBx, PBy, PBz);
+                  unc = ok;
+                }
+                ncand += ok;
+                if (unc) {
+                  nre += SPLIT;
+                  const PRec t = trec[j];
+                  if (!SWAP)
+                    cell = exact_cell(P, lthr, me.x, me.y, me.z, t.x, t.y, t.z, PBx, PBy, PBz);
+                  else
+                    cell = exact_cell(P, lthr, t.x, t.y, t.z, me.x, me.y, me.z, PBx, PBy, PBz);
+                }
+                if constexpr (ACC == 0) {
+                  if (cell >= 0) {
+                    if (cell != run_cell) {
+                      if (run_cell >= 0) lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
+                      run_cell = cell;
+                      run_n = 0;
+#pragma unroll
+                      for (int s = 0; s < NS; ++s) run[s] = 0.0;
+                    }
+                    ++run_n;
+                    const PRec t = trec[j];
+                    const Gal gme{me.x, me.y, me.w, me.e1, me.e2, me.phi};
+                    const Gal gt{t.x, t.y, t.w, t.e1, t.e2, t.phi};
+                    double qv[NQ];
+                    if (!SWAP)
+                      pair_values<K, SPLIT>(gme, gt, P.gws, qv);
+                    else
+                      pair_values<K, SPLIT>(gt, gme, P.gws, qv);
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) run[s] += qv[s];
+                  }
+                } else {
+                  const unsigned long long m = __ballot(cell >= 0);
+                  if (m) {
+                    const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                    if (cell >= 0)
+                      wq[(qr + qn + pos) & (QCAP - 1)] =
+                          ((unsigned long long)(unsigned)j << 32) | ((unsigned long long)lane << 16) | (unsigned)cell;
+                    qn += __popcll(m);
+                    if (qn >= 64) {
+                      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                      __builtin_amdgcn_wave_barrier();
+                      run_batch(64);
+                      __builtin_amdgcn_wave_barrier();
+                    }
+                  }
+                }
               }
             }
           }
+        }
+      }
+      if constexpr (ACC == 0) {
+        if (run_cell >= 0) lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
+      } else {
+        if (qn > 0) {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          run_batch(qn);
+          __builtin_amdgcn_wave_barrier();
         }
       }
     }
@@ -419,7 +532,7 @@ __global__ void k_gather(GatherArgs A, const int32_t *__restrict__ sidx, PRec *_
   f.xl = (float)(r.x - cx * A.h);
   f.yl = (float)(r.y - cy * A.h);
   f.zf = (float)r.z;
-  f.pad = 0.0f;
+  f.meta = m;
   fr[p] = f;
 }
 
@@ -500,10 +613,10 @@ hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *ko
   return sort_pairs_impl<double>(tmp, tmp_bytes, kin, kout, vin, vout, n, 64, st);
 }
 
-template <int K, bool SPLIT, bool SWAP>
+template <int K, bool SPLIT, bool SWAP, int ACC>
 static hipError_t launch_pair_t(const KArgs &P, int threads, size_t lds, const PRec *brec, const FRec *bfr,
                                 const PRec *trec, const FRec *tfr, hipStream_t st) {
-  auto fn = pair_kernel<K, SPLIT, SWAP>;
+  auto fn = pair_kernel<K, SPLIT, SWAP, ACC>;
   hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   fn<<<P.n_items, threads, lds, st>>>(P, brec, bfr, trec, tfr);
@@ -511,27 +624,37 @@ static hipError_t launch_pair_t(const KArgs &P, int threads, size_t lds, const P
 }
 
 // LDS bytes a pair kernel needs for histograms of `ncell` cells and `nslots` f64 slots.
-size_t pair_lds_bytes(int ncell, int nslots, int n_rp) {
-  return (size_t)((ncell * 4 + 15) & ~15) + (size_t)nslots * ncell * 8 + (size_t)(n_rp + 1) * 8 +
-         (size_t)2 * (n_rp + 1) * 4;
+size_t pair_lds_bytes(int ncell, int nslots, int n_rp, int nwave, int acc_mode) {
+  size_t b = (size_t)((ncell * 4 + 15) & ~15) + (size_t)nslots * ncell * 8 + (size_t)(n_rp + 1) * 8 +
+             (size_t)2 * (n_rp + 1) * 4;
+  b = (b + 4 + 7) & ~(size_t)7;
+  if (acc_mode == 1) b += (size_t)nwave * 128 * 8;
+  return b;
 }
 
 hipError_t launch_pair(int kind, bool split, bool swap, const KArgs &P, int threads, const PRec *brec,
                        const FRec *bfr, const PRec *trec, const FRec *tfr, hipStream_t st) {
   if (P.n_items == 0) return hipSuccess;
-  const size_t lds = pair_lds_bytes(P.ncell, P.nslots, P.n_rp);
-#define CPC_DISPATCH(KK)                                                                        \
-  if (kind == KK) {                                                                             \
-    if (split && swap) return launch_pair_t<KK, true, true>(P, threads, lds, brec, bfr, trec, tfr, st);   \
-    if (split) return launch_pair_t<KK, true, false>(P, threads, lds, brec, bfr, trec, tfr, st);          \
-    if (swap) return launch_pair_t<KK, false, true>(P, threads, lds, brec, bfr, trec, tfr, st);           \
-    return launch_pair_t<KK, false, false>(P, threads, lds, brec, bfr, trec, tfr, st);                    \
+  const size_t lds = pair_lds_bytes(P.ncell, P.nslots, P.n_rp, threads / 64, P.acc_mode);
+#define CPC_DISPATCH2(KK, AA)                                                                          \
+  if (split && swap) return launch_pair_t<KK, true, true, AA>(P, threads, lds, brec, bfr, trec, tfr, st); \
+  if (split) return launch_pair_t<KK, true, false, AA>(P, threads, lds, brec, bfr, trec, tfr, st);        \
+  if (swap) return launch_pair_t<KK, false, true, AA>(P, threads, lds, brec, bfr, trec, tfr, st);         \
+  return launch_pair_t<KK, false, false, AA>(P, threads, lds, brec, bfr, trec, tfr, st);
+#define CPC_DISPATCH(KK)                    \
+  if (kind == KK) {                         \
+    if (P.acc_mode == 1) {                  \
+      CPC_DISPATCH2(KK, 1)                  \
+    } else {                                \
+      CPC_DISPATCH2(KK, 0)                  \
+    }                                       \
   }
   CPC_DISPATCH(kDD)
   CPC_DISPATCH(kSD)
   CPC_DISPATCH(kSS0)
   CPC_DISPATCH(kSS1)
 #undef CPC_DISPATCH
+#undef CPC_DISPATCH2
   return hipErrorInvalidValue;
 }
 

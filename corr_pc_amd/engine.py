@@ -21,6 +21,7 @@ CPC_ERR_NOMEM = -3
 CPC_ERR_UNSUPPORTED = -4
 
 GEOM_BOX = 0
+GEOM_SKY = 1
 KIND_DD = 0
 KIND_SHAPE_DENSITY = 1
 KIND_SHAPE_SHAPE = 2
@@ -37,7 +38,7 @@ _lp = POINTER(c_int64)
 
 class CatalogDesc(ctypes.Structure):
     _fields_ = [("n", c_int64), ("x", _dp), ("y", _dp), ("z", _dp), ("w", _dp), ("e1", _dp), ("e2", _dp),
-                ("phi", _dp), ("jk", _ip), ("jk_prob", _dp), ("include_prob", _dp)]
+                ("phi", _dp), ("jk", _ip), ("jk_prob", _dp), ("include_prob", _dp), ("dc", _dp), ("da", _dp)]
 
 
 class Params(ctypes.Structure):
@@ -45,7 +46,16 @@ class Params(ctypes.Structure):
                 ("estimator", c_int), ("which_corr", c_int), ("n_rp", c_int), ("rp_min", c_double),
                 ("rp_max", c_double), ("lin_bin", c_int), ("n_pi", c_int), ("pi_min", c_double),
                 ("pi_max", c_double), ("do_jk", c_int), ("n_jk", c_int), ("include_prob_max", c_double),
-                ("precision", c_int), ("jk_mode", c_int)]
+                ("precision", c_int), ("jk_mode", c_int), ("sky_H", _dp), ("sky_dA", _dp), ("sky_n", c_int64),
+                ("sky_dz", c_double), ("z_max", c_double), ("z_sep_max", c_double)]
+
+    def attach_table(self, H, dA, dz, z_max, z_sep_max):
+        """Sky geometry: keep references to the distance table arrays and point the struct at them."""
+        self._table = (np.ascontiguousarray(H, np.float64), np.ascontiguousarray(dA, np.float64))
+        self.sky_H = self._table[0].ctypes.data_as(_dp)
+        self.sky_dA = self._table[1].ctypes.data_as(_dp)
+        self.sky_n = len(self._table[0])
+        self.sky_dz, self.z_max, self.z_sep_max = dz, z_max, z_sep_max
 
 
 class HistC(ctypes.Structure):
@@ -81,17 +91,19 @@ class Catalog:
     jk: Optional[np.ndarray] = None
     jk_prob: Optional[np.ndarray] = None
     include_prob: Optional[np.ndarray] = None
+    dc: Optional[np.ndarray] = None
+    da: Optional[np.ndarray] = None
     _keep: list = field(default_factory=list, repr=False)
 
     def __post_init__(self):
-        for name in ("x", "y", "z", "w", "e1", "e2", "phi", "jk_prob", "include_prob"):
+        for name in ("x", "y", "z", "w", "e1", "e2", "phi", "jk_prob", "include_prob", "dc", "da"):
             v = getattr(self, name)
             if v is not None:
                 setattr(self, name, np.ascontiguousarray(v, dtype=np.float64))
         if self.jk is not None:
             self.jk = np.ascontiguousarray(self.jk, dtype=np.int32)
         n = len(self.x)
-        for name in ("y", "z", "w", "e1", "e2", "phi", "jk", "jk_prob", "include_prob"):
+        for name in ("y", "z", "w", "e1", "e2", "phi", "jk", "jk_prob", "include_prob", "dc", "da"):
             v = getattr(self, name)
             if v is not None and len(v) != n:
                 raise ValueError(f"catalog field {name} has length {len(v)} != {n}")
@@ -103,7 +115,7 @@ class Catalog:
     def subset(self, idx):
         """Catalog restricted to rows idx (keeps order)."""
         kw = {}
-        for name in ("x", "y", "z", "w", "e1", "e2", "phi", "jk", "jk_prob", "include_prob"):
+        for name in ("x", "y", "z", "w", "e1", "e2", "phi", "jk", "jk_prob", "include_prob", "dc", "da"):
             v = getattr(self, name)
             kw[name] = None if v is None else v[idx]
         return Catalog(**kw)
@@ -112,7 +124,8 @@ class Catalog:
         def p(a, t=_dp):
             return None if a is None else a.ctypes.data_as(t)
         return CatalogDesc(self.n, p(self.x), p(self.y), p(self.z), p(self.w), p(self.e1), p(self.e2),
-                           p(self.phi), p(self.jk, _ip), p(self.jk_prob), p(self.include_prob))
+                           p(self.phi), p(self.jk, _ip), p(self.jk_prob), p(self.include_prob), p(self.dc),
+                           p(self.da))
 
 
 def make_params(box_size, n_rp, rp_min, rp_max, n_pi, pi_min, pi_max, kind=KIND_DD, g_with_shape=2,
@@ -174,7 +187,7 @@ def load_library(path=LIB_PATH):
     lib.cpc_rp_thresholds.restype = c_int
     lib.cpc_rand_stream.argtypes = [ctypes.c_uint32, c_int64, c_int64, _dp]
     lib.cpc_rand_stream.restype = c_int
-    lib.cpc_set_tuning.argtypes = [c_void_p, c_double, c_int, c_int]
+    lib.cpc_set_tuning.argtypes = [c_void_p, c_double, c_int, c_int, c_int]
     lib.cpc_set_tuning.restype = c_int
     _lib = lib
     return lib
@@ -239,8 +252,8 @@ class Engine:
         if ent is not None:
             self.lib.cpc_catalog_free(self.ctx, ent[0])
 
-    def set_tuning(self, col_frac=0.0, item_size=0, block_threads=0):
-        self._check(self.lib.cpc_set_tuning(self.ctx, col_frac, item_size, block_threads), "set_tuning")
+    def set_tuning(self, col_frac=0.0, item_size=0, block_threads=0, acc_mode=-1):
+        self._check(self.lib.cpc_set_tuning(self.ctx, col_frac, item_size, block_threads, acc_mode), "set_tuning")
 
     def count(self, outer: Catalog, inner: Catalog, params: Params, hist: Histogram = None):
         """One cross pass (the reference's corels2 after the role swap).  Returns (Histogram, stats dict)."""

@@ -185,3 +185,116 @@ def catalogs(config, n_data=None, n_rand=None):
     Rs = cat(rand, rs_draw[0::2], rs_draw[1::2], False) if n_rs else None
     Rd = cat(rand, rd_draw[0::2], rd_draw[1::2], False)
     return {"S": S, "Rs": Rs, "D": D, "Rd": Rd, "config": c}
+
+
+# ---------------------------------------------------------------------------------------------
+# Sky survey (RA/Dec/redshift)
+
+C_LIGHT = 299792.458  # km/s
+
+
+@dataclass
+class SkyConfig:
+    name: str
+    n_data: int
+    n_rand: int
+    fields: tuple  # (ra_lo, ra_hi, dec_lo, dec_hi, n_jk_ra, n_jk_dec) in degrees
+    z_lo: float
+    z_hi: float
+    omega_m: float
+    dz: float
+    n_rp: int
+    rp_min: float
+    rp_max: float
+    n_pi: int
+    pi_max: float
+    which_corr: int
+
+
+# six 5x5 degree fields (about 150 deg^2 in total), |Dec| <= 30; 100 jackknife regions on an RA/Dec
+# grid (4x4 in each of the first four fields, 6x3 in the last two)
+SKY_FIELDS = ((20.0, 25.0, -20.0, -15.0, 4, 4), (80.0, 85.0, 0.0, 5.0, 4, 4), (140.0, 145.0, 10.0, 15.0, 4, 4),
+              (200.0, 205.0, -10.0, -5.0, 4, 4), (260.0, 265.0, 5.0, 10.0, 6, 3), (320.0, 325.0, -25.0, -20.0, 6, 3))
+
+SKY_CONFIGS = {
+    2: SkyConfig("sky survey, 2e6 D / 2e7 R, 6 fields ~150 deg^2", 2_000_000, 20_000_000, SKY_FIELDS, 0.2, 1.0, 0.3,
+                 1e-4, 20, 0.1, 100.0, 40, 100.0, 1),
+}
+
+
+def distance_table(omega_m=0.3, dz=1e-4, z_max=1.0):
+    """Flat LCDM distance table in corr_pc's 5-column layout with use_comoving=1
+    (initialization.cpp:37-44): z, H(z) [km/s/(Mpc/h)], unused, comoving DC, transverse comoving DA
+    (equal to DC when flat), rows k*dz for k = 0 .. ceil((z_max+1)/dz)+20 (the reference reads
+    ceil((z_max+1)/dz)+1 rows and allocates 20 more)."""
+    n = int(np.ceil((z_max + 1) / dz)) + 21
+    z = np.arange(n) * dz
+    H = 100.0 * np.sqrt(omega_m * (1 + z) ** 3 + (1 - omega_m))
+    # comoving distance by the trapezoid rule on a 10x finer grid
+    zf = np.linspace(0.0, z[-1], 10 * (n - 1) + 1)
+    inv = C_LIGHT / (100.0 * np.sqrt(omega_m * (1 + zf) ** 3 + (1 - omega_m)))
+    chi_f = np.concatenate([[0.0], np.cumsum(0.5 * (inv[1:] + inv[:-1]) * np.diff(zf))])
+    dc = chi_f[::10]
+    return {"z": z, "H": H, "dC": dc, "dA": dc.copy(), "dz": dz}
+
+
+def _sky_positions(rng, n, fields):
+    """Uniform on the sphere inside the union of RA/Dec rectangles (area-weighted)."""
+    areas = np.array([(f[1] - f[0]) * (np.sin(np.radians(f[3])) - np.sin(np.radians(f[2]))) for f in fields])
+    which = rng.choice(len(fields), size=n, p=areas / areas.sum())
+    ra = np.empty(n)
+    dec = np.empty(n)
+    jk = np.empty(n, np.int32)
+    base = 0
+    for k, f in enumerate(fields):
+        m = which == k
+        c = int(m.sum())
+        ra[m] = rng.uniform(f[0], f[1], c)
+        s = rng.uniform(np.sin(np.radians(f[2])), np.sin(np.radians(f[3])), c)
+        dec[m] = np.degrees(np.arcsin(s))
+        ia = np.minimum(((ra[m] - f[0]) / (f[1] - f[0]) * f[4]).astype(int), f[4] - 1)
+        ib = np.minimum(((dec[m] - f[2]) / (f[3] - f[2]) * f[5]).astype(int), f[5] - 1)
+        jk[m] = base + ia * f[5] + ib
+        base += f[4] * f[5]
+    return ra, dec, jk, base
+
+
+def _nz_draw(rng, n, z_lo, z_hi):
+    """n(z) proportional to z^2 exp(-(z/0.4)^1.5) on (z_lo, z_hi), by inverse CDF."""
+    zg = np.linspace(z_lo, z_hi, 200001)
+    pdf = zg ** 2 * np.exp(-(zg / 0.4) ** 1.5)
+    cdf = np.concatenate([[0.0], np.cumsum(0.5 * (pdf[1:] + pdf[:-1]) * np.diff(zg))])
+    cdf /= cdf[-1]
+    z = np.interp(rng.uniform(0.0, 1.0, n), cdf, zg)
+    return np.clip(z, z_lo, np.nextafter(z_hi, 0))
+
+
+def sky_catalog(config, which, n=None, table=None):
+    """Data (which='data') or randoms of a sky configuration, z-sorted, with the reader's derived
+    fields: DC = dC_z[floor(z/dz)], DA = dA_z[floor(z/dz)] (read_dat.cpp:249-251), jk_prob = 1
+    (read_dat.cpp:239) and, for data, shapes and lognormal weights."""
+    c = SKY_CONFIGS[config]
+    table = table or distance_table(c.omega_m, c.dz, c.z_hi)
+    n = (c.n_data if which == "data" else c.n_rand) if n is None else n
+    rng = _rng(config, 0 if which == "data" else 1)
+    ra, dec, jk, n_jk = _sky_positions(rng, n, c.fields)
+    z = _nz_draw(rng, n, c.z_lo, c.z_hi)
+    cols = {"x": ra, "y": dec, "z": z, "jk": jk, "w": None, "e1": None, "e2": None, "phi": None}
+    if which == "data":
+        e1, e2 = shapes(rng, np.zeros((n, 3)), np.full((n, 3), np.nan), 1.0)
+        cols.update(e1=e1, e2=e2, phi=elip_angle(e1, e2), w=lognormal_weights(rng, n))
+    cols = _sorted(cols)
+    it = np.floor(cols["z"] / table["dz"]).astype(np.int64)
+    cols["dc"] = table["dC"][it]
+    cols["da"] = table["dA"][it]
+    return Catalog(x=cols["x"], y=cols["y"], z=cols["z"], w=cols["w"], e1=cols["e1"], e2=cols["e2"],
+                   phi=cols["phi"], jk=cols["jk"], jk_prob=np.ones(n), dc=cols["dc"], da=cols["da"]), n_jk, table
+
+
+def sky_params_table(config, table):
+    """(H, dA, dz, z_max, z_sep_max) as data_ini derives them for coordinates 0
+    (initialization.cpp:223, 272-276): z_sep_max = p_max*H[ceil(z_max/dz)]/300000 + 0.001."""
+    c = SKY_CONFIGS[config]
+    z_idx = int(np.ceil(c.z_hi / table["dz"]))
+    z_sep_max = c.pi_max * table["H"][z_idx] / 300000.0 + 0.001
+    return table["H"], table["dA"], table["dz"], c.z_hi, z_sep_max

@@ -37,8 +37,9 @@ typedef enum cpc_status {
   CPC_ERR_UNSUPPORTED = -4  /* a mode the engine does not implement */
 } cpc_status;
 
-/* geometry: periodic box with x, y, z (data_info.periodic_box=1, coordinates=6) */
-#define CPC_GEOM_BOX 0
+/* geometry */
+#define CPC_GEOM_BOX 0 /* periodic box with x, y, z (data_info.periodic_box=1, coordinates=6) */
+#define CPC_GEOM_SKY 1 /* RA/Dec/redshift survey, rp-pi (periodic_box=0, coordinates=0) */
 
 /* pair kind: which per-pair function of corels.cpp runs (choose_corel_func, initialization.cpp:490-543) */
 #define CPC_KIND_DD 0            /* density_density_corel, corels.cpp:48-77 */
@@ -69,6 +70,10 @@ typedef struct cpc_catalog_desc {
   const int32_t *jk;          /* gal.jk, 0 <= jk < n_jk; NULL allowed only when jackknife is off */
   const double *jk_prob;      /* gal.jk_prob (read_dat.cpp:239-255); NULL means 0 */
   const double *include_prob; /* gal.include_prob (read_dat.cpp:318-320); NULL means 0 */
+  /* sky geometry only: x = RA and y = Dec in degrees, z = redshift, and the comoving distances the
+   * reader looked up, gal.DC = dC_z[floor(z/dz)] and gal.DA = dA_z[floor(z/dz)] (read_dat.cpp:249-251) */
+  const double *dc;
+  const double *da;
 } cpc_catalog_desc;
 
 /* The data_info fields the pair loop reads (data_def.h:46-96) plus engine choices. */
@@ -89,6 +94,15 @@ typedef struct cpc_params {
   double include_prob_max; /* calc_temp.include_prob_max (initialization.cpp:597-599) */
   int precision;           /* CPC_PREC_* */
   int jk_mode;             /* CPC_JK_* */
+  /* sky geometry only: the distance table of da_Z (initialization.cpp:23-60) that lim_calc_sky
+   * interpolates (sky_calcs.cpp:33-81, 105-190): H(z) and the angular-diameter column dA_z, sampled
+   * at z = k*dz for k < sky_n; data_info.z_max and data_info.z_sep_max (initialization.cpp:272-276) */
+  const double *sky_H;
+  const double *sky_dA;
+  int64_t sky_n;
+  double sky_dz;
+  double z_max;
+  double z_sep_max;
 } cpc_params;
 
 /* Output histograms, caller-allocated host memory, overwritten by cpc_count.
@@ -148,9 +162,11 @@ cpc_status cpc_rp_thresholds(const cpc_params *params, double *thr);
  * include_prob (read_dat.cpp:318-320) in catalog read order (do_corr.cpp:40-96). */
 cpc_status cpc_rand_stream(uint32_t seed, int64_t skip, int64_t n, double *out);
 
-/* Engine tuning knobs (column width in units of rp_max, outer galaxies per work item, threads per
- * work group); 0 keeps the automatic choice.  Used by the autotuner. */
-cpc_status cpc_set_tuning(cpc_ctx *ctx, double col_frac, int item_size, int block_threads);
+/* Engine tuning knobs: column width in units of rp_max, block galaxies per work item, threads per
+ * work group (0 keeps the automatic choice) and the histogram accumulation mode (-1 automatic,
+ * 0 run-length privatisation in registers, 1 per-wave compaction of binned pairs).  Used by the
+ * autotuner; results do not depend on them beyond floating-point summation order. */
+cpc_status cpc_set_tuning(cpc_ctx *ctx, double col_frac, int item_size, int block_threads, int acc_mode);
 
 #ifdef __cplusplus
 }

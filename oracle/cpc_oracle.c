@@ -1,17 +1,19 @@
 /*
  * cpc_oracle.c -- TEST INFRASTRUCTURE ONLY: the parity oracle and CPU baseline.
  *
- * A plain-C restatement of corr_pc's cross-pair loop for the periodic box, written from the
- * reference's behaviour (not its text).  It follows, step for step:
+ * A plain-C restatement of corr_pc's cross-pair loop, written from the reference's behaviour (not
+ * its text).  It follows, step for step:
  *   corels2 outer loop, image loops and inner sweep    corr.cpp:219-459 (do_pair_sorted corr.cpp:67-86)
- *   per-outer window and image flags                   PB_calcs.cpp:12-61 (lim_calc_xyz, lim_calc_PB)
- *   pi and rp of a pair                                PB_calcs.cpp:87-98 (pi_calc_PB, rp_calc_PB)
+ *   per-outer window and image flags (box)             PB_calcs.cpp:12-61 (lim_calc_xyz, lim_calc_PB)
+ *   per-outer window (sky)                             sky_calcs.cpp:33-81, 105-190 (lim_calc_sky)
+ *   pi and rp of a pair                                PB_calcs.cpp:87-98; calcs.cpp:126-132;
+ *                                                      sky_calcs.cpp:24-31, 91-96 (dA_calc, rp_calc_sky)
  *   bin indices, pair weight                           corels.cpp:21-46 (corel_bin)
  *   the three pair kinds                               corels.cpp:48-77, 122-204, 206-244
- *   orientation angle, rotation, ephi                  calcs.cpp:54-64, 97-105
+ *   orientation angle, rotation, ephi                  calcs.cpp:54-64, 97-105; sky_calcs.cpp:13-22
  *   jackknife touch slots and leave-one-out            jk.cpp:53-81, 138-171, 201-223; outp.cpp:242-288
- * in FP64 with the host libm (log10, sqrt, atan, cos, sin, floor) and no FMA contraction, like the
- * reference's -O2 x86-64 build.  OpenMP threads own private histograms, merged in thread order.
+ * in FP64 with the host libm (log10, sqrt, atan, sin, cos, acos, floor) and no FMA contraction, like
+ * the reference's -O2 x86-64 build.  OpenMP threads own private histograms, merged in thread order.
  *
  * One deliberate difference: the reference finds the first inner galaxy of each outer z window with
  * a per-thread pointer that only moves forward (corr.cpp:361-408).  When some window holds no inner
@@ -33,6 +35,8 @@
 #include "../include/cpc.h"
 
 #define NF CPC_NFIELD
+
+static const double R2D = 180.0 / M_PI; /* r2d of calcs.cpp:19, sky_calcs.cpp:10 */
 
 typedef struct {
   int n_rp, n_pi, R;
@@ -80,10 +84,6 @@ static inline void rotate_(double e1, double e2, double theta, double *ep, doubl
   ex[0] = sin(2.0 * theta) * e1 + cos(2.0 * theta) * e2;
 }
 
-typedef struct {
-  const cpc_catalog_desc *c;
-} ocat;
-
 static inline double cw(const cpc_catalog_desc *c, int64_t i) { return c->w ? c->w[i] : 1.0; }
 static inline double cphi(const cpc_catalog_desc *c, int64_t i) { return c->phi ? c->phi[i] : -10.0; }
 static inline double cjp(const cpc_catalog_desc *c, int64_t i) { return c->jk_prob ? c->jk_prob[i] : 0.0; }
@@ -106,39 +106,23 @@ static int64_t lower_bound_(const double *z, int64_t n, double shift, double zmi
 }
 
 typedef struct {
+  int geom;
   double L, pmin, pmax, psize, rmin, rmax, rsize;
   int n_rp, n_pi, lin, kind, gws, est, which_corr, jk_on;
   double inc_max;
+  /* sky */
+  const double *H, *dA;
+  int64_t ntab;
+  double dz, z_max, z_sep_max;
 } oparm;
 
-/* One pair that passed the window predicates: corel_jk (jk.cpp:201-223) with the kind's corel.
- * Returns 1 if the pair was binned. */
-static inline int corel_pair(const oparm *P, const cpc_catalog_desc *g1, int64_t i, const cpc_catalog_desc *g2,
-                             int64_t j, int PBx, int PBy, int PBz, ohist *h) {
-  const double L = P->L;
-  /* pi_calc_PB, PB_calcs.cpp:87-91 */
-  double d_P = (g2->z[j] + L * PBz) - g1->z[i];
-  /* rp_calc_PB, PB_calcs.cpp:93-98 */
-  double t0 = g1->x[i] - (g2->x[j] + L * PBx);
-  double t1 = g1->y[i] - (g2->y[j] + L * PBy);
-  double da_R = sqrt(t0 * t0 + t1 * t1);
-  /* corel_bin, corels.cpp:26-38 */
-  double q2 = floor((d_P - P->pmin) / P->psize);
-  if (!(q2 >= 0 && q2 < P->n_pi)) return 0;
-  double q1;
-  if (P->lin)
-    q1 = floor((da_R - P->rmin) / P->rsize);
-  else
-    q1 = floor(log10(da_R / P->rmin) / P->rsize);
-  if (!(q1 >= 0 && q1 < P->n_rp)) return 0;
-  int n1 = (int)q1, n2 = (int)q2;
-  double wt = cw(g1, i) * cw(g2, j);
-
-  /* shape fields: ep, ex, epp, exx, epx, etheta (val, err); unset ones stay 0 as calc_temp_ini leaves them */
-  double v[NF] = {0}, e[NF] = {0};
+/* Shape quantities of a binned pair (corels.cpp:122-244), theta = the orientation angle.
+ * v[f], e[f]: the values added to data[f].val / data[f].err; unset ones stay 0, as calc_temp_ini
+ * leaves the per-thread ep/ex/epp/exx/epx/etheta that a kind never assigns. */
+static inline void kind_values(const oparm *P, const cpc_catalog_desc *g1, int64_t i, const cpc_catalog_desc *g2,
+                               int64_t j, double wt, double theta, double *v, double *e) {
   if (P->kind == CPC_KIND_SHAPE_DENSITY) {
     /* shape_density_corel, corels.cpp:206-244 */
-    double theta = atan((g2->y[j] - g1->y[i]) / (g2->x[j] - g1->x[i])); /* calcs.cpp:54-57 */
     double ep, ex;
     if (P->gws == 2)
       rotate_(g2->e1[j], g2->e2[j], theta, &ep, &ex);
@@ -152,7 +136,6 @@ static inline int corel_pair(const oparm *P, const cpc_catalog_desc *g1, int64_t
     e[6] = v[6] * v[6];
   } else if (P->kind == CPC_KIND_SHAPE_SHAPE) {
     /* shape_shape_corel, corels.cpp:122-204 */
-    double theta = atan((g2->y[j] - g1->y[i]) / (g2->x[j] - g1->x[i]));
     double epi, exi, epj, exj;
     rotate_(g1->e1[i], g1->e2[i], theta, &epi, &exi);
     rotate_(g2->e1[j], g2->e2[j], theta, &epj, &exj);
@@ -189,10 +172,29 @@ static inline int corel_pair(const oparm *P, const cpc_catalog_desc *g1, int64_t
       e[5] = a * a + b * b;
     }
   }
+}
 
+/* corel_bin's index tests (corels.cpp:26-38) given pi (d_P) and rp (da_R); returns 0 if rejected. */
+static inline int bin_pair(const oparm *P, double d_P, double da_R, int *n1, int *n2) {
+  double q2 = floor((d_P - P->pmin) / P->psize);
+  if (!(q2 >= 0 && q2 < P->n_pi)) return 0;
+  double q1;
+  if (P->lin)
+    q1 = floor((da_R - P->rmin) / P->rsize);
+  else
+    q1 = floor(log10(da_R / P->rmin) / P->rsize);
+  if (!(q1 >= 0 && q1 < P->n_rp)) return 0;
+  *n1 = (int)q1;
+  *n2 = (int)q2;
+  return 1;
+}
+
+/* density_density_corel's counters (corels.cpp:67-73), the kind's data fields, then corel_jk with
+ * jk_check_PB or jk_check_sky and corel_jk1 (jk.cpp:53-71, 138-171, 201-223). */
+static inline void accumulate(const oparm *P, ohist *h, int n1, int n2, double wt, const double *v, const double *e,
+                              int jk1, int jk2, double jp1, double jp2) {
   const size_t nc = (size_t)P->n_rp * P->n_pi;
   size_t c = (size_t)n1 * P->n_pi + n2;
-  /* density_density_corel, corels.cpp:67-73 */
   h->num_rp[n1]++;
   h->num[c]++;
   h->wt[c] += wt;
@@ -202,25 +204,100 @@ static inline int corel_pair(const oparm *P, const cpc_catalog_desc *g1, int64_t
       h->err[c * NF + f] += e[f];
     }
   }
-  if (P->jk_on) {
-    /* corel_jk + jk_check_PB + corel_jk1 (jk.cpp:64-71, 138-171, 201-223) */
-    int gjk[2] = {g1->jk[i], g2->jk[j]};
-    double jp[2] = {cjp(g1, i), cjp(g2, j)};
-    if (gjk[0] == gjk[1]) {
-      jp[0] = 1;
-      jp[1] = 0;
-    }
-    for (int s = 0; s < 2; s++) {
-      if (jp[s] < 0.5) continue;
-      int r = 1 + gjk[s];
-      size_t cr = (size_t)r * nc + c;
-      h->num_rp[(size_t)r * P->n_rp + n1]++;
-      h->num[cr]++;
-      h->wt[cr] += wt;
-      if (P->which_corr == 0) continue;
-      for (int f = 1; f < NF; f++) h->val[cr * NF + f] += v[f];
-    }
+  if (!P->jk_on) return;
+  int gjk[2] = {jk1, jk2};
+  double jp[2] = {jp1, jp2};
+  if (gjk[0] == gjk[1]) {
+    if (P->geom == CPC_GEOM_BOX) jp[0] = 1; /* jk_check_PB */
+    jp[1] = 0;
   }
+  for (int s = 0; s < 2; s++) {
+    if (jp[s] < 0.5) continue;
+    int r = 1 + gjk[s];
+    size_t cr = (size_t)r * nc + c;
+    h->num_rp[(size_t)r * P->n_rp + n1]++;
+    h->num[cr]++;
+    h->wt[cr] += wt;
+    if (P->which_corr == 0) continue;
+    for (int f = 1; f < NF; f++) h->val[cr * NF + f] += v[f];
+  }
+}
+
+/* Box pair that passed the window predicates (pi_calc_PB, rp_calc_PB, corel, corel_jk). */
+static inline int corel_box(const oparm *P, const cpc_catalog_desc *g1, int64_t i, const cpc_catalog_desc *g2,
+                            int64_t j, int PBx, int PBy, int PBz, ohist *h) {
+  const double L = P->L;
+  double d_P = (g2->z[j] + L * PBz) - g1->z[i];
+  double t0 = g1->x[i] - (g2->x[j] + L * PBx);
+  double t1 = g1->y[i] - (g2->y[j] + L * PBy);
+  double da_R = sqrt(t0 * t0 + t1 * t1);
+  int n1, n2;
+  if (!bin_pair(P, d_P, da_R, &n1, &n2)) return 0;
+  double wt = cw(g1, i) * cw(g2, j);
+  double v[NF] = {0}, e[NF] = {0};
+  if (P->kind != CPC_KIND_DD) {
+    double theta = atan((g2->y[j] - g1->y[i]) / (g2->x[j] - g1->x[i])); /* calcs.cpp:54-57 */
+    kind_values(P, g1, i, g2, j, wt, theta, v, e);
+  }
+  accumulate(P, h, n1, n2, wt, v, e, P->jk_on ? g1->jk[i] : 0, P->jk_on ? g2->jk[j] : 0, cjp(g1, i), cjp(g2, j));
+  return 1;
+}
+
+/* Sky pair that passed the window predicates (pi_calc, rp_calc_sky/dA_calc, corel, corel_jk). */
+static inline int corel_sky(const oparm *P, const cpc_catalog_desc *g1, int64_t i, const cpc_catalog_desc *g2,
+                            int64_t j, ohist *h) {
+  double d_P = g2->dc[j] - g1->dc[i];
+  const double dec1 = g1->y[i] / R2D, dec2 = g2->y[j] / R2D, ra1 = g1->x[i] / R2D, ra2 = g2->x[j] / R2D;
+  double sum = sin(dec1) * sin(dec2) + cos(dec1) * cos(dec2) * cos((ra1 - ra2));
+  double da = (sum > 0.9999999999) ? acos(1.0) : acos(sum);
+  double da_R = da * (g1->da[i] + g2->da[j]) / 2.;
+  int n1, n2;
+  if (!bin_pair(P, d_P, da_R, &n1, &n2)) return 0;
+  double wt = cw(g1, i) * cw(g2, j);
+  double v[NF] = {0}, e[NF] = {0};
+  if (P->kind != CPC_KIND_DD) {
+    /* angle_orient_axis_sky, sky_calcs.cpp:13-22 */
+    double theta = atan((g2->y[j] - g1->y[i]) / ((g2->x[j] - g1->x[i]) * cos((dec1 + dec2) / 2.0)));
+    kind_values(P, g1, i, g2, j, wt, theta, v, e);
+  }
+  accumulate(P, h, n1, n2, wt, v, e, P->jk_on ? g1->jk[i] : 0, P->jk_on ? g2->jk[j] : 0, cjp(g1, i), cjp(g2, j));
+  return 1;
+}
+
+/* Linear interpolation of a distance-table column, as dAR_to_dA/p_to_dz (sky_calcs.cpp:71-81). */
+static inline double tab_interp(const double *tab, double dz, double z) {
+  int i_t = (int)floor(z / dz);
+  return tab[i_t] + ((tab[i_t + 1] - tab[i_t]) * (z - dz * i_t) / dz);
+}
+
+static inline int tab_ok(const oparm *P, double z) {
+  double f = floor(z / P->dz);
+  return f >= 0 && f + 1 < (double)P->ntab;
+}
+
+/* lim_calc_sky + lim_calc_ra_dec for coordinates 0 (sky_calcs.cpp:105-190).  Returns 0 on the
+ * conditions where the reference exits or reads outside its table. */
+static int sky_window(const oparm *P, double ra, double dec, double z, double *w) {
+  double z_max = z + P->z_sep_max;
+  if (!tab_ok(P, z_max) || !tab_ok(P, z)) return 0;
+  z_max = z + P->pmax * tab_interp(P->H, P->dz, z_max) / 300000.0;
+  double z_min = z + P->pmin * tab_interp(P->H, P->dz, z) / 300000.0;
+  if (z_min < 0.001) z_min = 0.001;
+  if (z_min > P->z_max) return 0; /* the reference calls exit(1) */
+  if (!tab_ok(P, z_min)) return 0;
+  double da_max = P->rmax / tab_interp(P->dA, P->dz, z_min);
+  double dec_rad = dec / R2D;
+  double sd = sin(dec_rad), cd = cos(dec_rad);
+  double ra_lim = acos((cos(da_max) - sd * sd) / (cd * cd)) * 1.02 * R2D;
+  double ra_max = ra + ra_lim, ra_min = ra - ra_lim;
+  if (ra_max > 360) ra_min = 0;
+  if (ra_min < 0) ra_max = 360;
+  w[0] = z_min;
+  w[1] = z_max;
+  w[2] = ra_min;
+  w[3] = ra_max;
+  w[4] = dec - da_max * R2D * 1.05;
+  w[5] = dec + da_max * R2D * 1.05;
   return 1;
 }
 
@@ -232,9 +309,10 @@ static double now_ms(void) {
 
 /* Validate and derive the data_info quantities (initialization.cpp:305-311). */
 static int make_params(const cpc_params *p, oparm *P) {
-  if (p->geometry != CPC_GEOM_BOX) return CPC_ERR_UNSUPPORTED;
+  if (p->geometry != CPC_GEOM_BOX && p->geometry != CPC_GEOM_SKY) return CPC_ERR_UNSUPPORTED;
   if (p->n_rp <= 0 || p->n_pi <= 0 || !(p->rp_max > 0) || p->rp_min > p->rp_max) return CPC_ERR_INVALID;
   if (p->do_jk != 0 && p->do_jk != 1) return CPC_ERR_UNSUPPORTED;
+  P->geom = p->geometry;
   P->L = p->box_size;
   P->pmin = p->pi_min;
   P->pmax = p->pi_max;
@@ -254,11 +332,18 @@ static int make_params(const cpc_params *p, oparm *P) {
   P->which_corr = p->which_corr;
   P->jk_on = (p->do_jk == 1 && p->n_jk > 0);
   P->inc_max = p->include_prob_max;
+  P->H = p->sky_H;
+  P->dA = p->sky_dA;
+  P->ntab = p->sky_n;
+  P->dz = p->sky_dz;
+  P->z_max = p->z_max;
+  P->z_sep_max = p->z_sep_max;
+  if (P->geom == CPC_GEOM_SKY && (!P->H || !P->dA || P->ntab < 2 || !(P->dz > 0))) return CPC_ERR_INVALID;
   return CPC_OK;
 }
 
 /*
- * orc_count: the cross pass of corels2 for outer = g1, inner = g2 (after the swap), box geometry.
+ * orc_count: the cross pass of corels2 for outer = g1, inner = g2 (after the swap).
  * Writes `out` in cpc_hist layout; returns a cpc_status.
  */
 int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_params *params, cpc_hist *out,
@@ -270,11 +355,21 @@ int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_
   const size_t nc = (size_t)P.n_rp * P.n_pi;
   const int64_t n1 = g1->n, n2 = g2->n;
   if (P.jk_on) {
+    if ((n1 && !g1->jk) || (n2 && !g2->jk)) return CPC_ERR_INVALID;
     for (int64_t i = 0; i < n1; i++)
       if (g1->jk[i] < 0 || g1->jk[i] >= params->n_jk) return CPC_ERR_INVALID;
     for (int64_t j = 0; j < n2; j++)
       if (g2->jk[j] < 0 || g2->jk[j] >= params->n_jk) return CPC_ERR_INVALID;
   }
+  const int sky = P.geom == CPC_GEOM_SKY;
+  if (sky && ((n1 && (!g1->dc || !g1->da)) || (n2 && (!g2->dc || !g2->da)))) return CPC_ERR_INVALID;
+  /* the shape kinds read e[] of the shape side (both sides for shape_shape); the reference would
+   * dereference an unallocated pointer here (corels.cpp:216-219, read_dat.cpp:126) */
+  if (P.kind == CPC_KIND_SHAPE_DENSITY) {
+    const cpc_catalog_desc *s = P.gws == 2 ? g2 : g1;
+    if (s->n && (!s->e1 || !s->e2)) return CPC_ERR_INVALID;
+  }
+  if (P.kind == CPC_KIND_SHAPE_SHAPE && ((n1 && !g1->e1) || (n2 && !g2->e1))) return CPC_ERR_INVALID;
   if (n_threads <= 0) n_threads = omp_get_num_procs();
   double t0 = now_ms();
 
@@ -282,7 +377,8 @@ int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_
   int64_t *tbinned = calloc(n_threads, sizeof(int64_t));
   int64_t *tcand = calloc(n_threads, sizeof(int64_t));
   int64_t *tsparse = calloc(n_threads, sizeof(int64_t));
-  int ok = th && tbinned && tcand && tsparse;
+  int *tbad = calloc(n_threads, sizeof(int));
+  int ok = th && tbinned && tcand && tsparse && tbad;
   for (int t = 0; ok && t < n_threads; t++) ok = hist_alloc(&th[t], P.n_rp, P.n_pi, R);
   if (!ok) return CPC_ERR_NOMEM;
 
@@ -299,6 +395,25 @@ int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_
 #pragma omp for schedule(dynamic, chunk)
       for (int64_t i = 0; i < n1; i++) {
         if (cinc(g1, i) > P.inc_max) continue;
+        if (sky) {
+          double w[6];
+          if (!sky_window(&P, g1->x[i], g1->y[i], g1->z[i], w)) {
+            tbad[t] = 1;
+            continue;
+          }
+          int64_t j = lower_bound_(g2->z, n2, 0.0, w[0]);
+          if (j < n2 && g2->z[j] > w[1]) sparse++;
+          for (; j < n2; j++) {
+            if (cinc(g2, j) > P.inc_max) continue;
+            double zz = g2->z[j];
+            if (zz > w[1] || zz < w[0]) break;
+            if (g2->y[j] > w[5] || g2->y[j] < w[4]) continue;
+            if (g2->x[j] < w[2] || g2->x[j] > w[3]) continue;
+            cand++;
+            binned += corel_sky(&P, g1, i, g2, j, h);
+          }
+          continue;
+        }
         /* lim_calc_xyz + lim_calc_PB, PB_calcs.cpp:12-61 */
         double z_max = g1->z[i] + P.pmax, z_min = g1->z[i] + P.pmin;
         double ra_max = g1->x[i] + P.rmax, ra_min = g1->x[i] - P.rmax;
@@ -335,7 +450,7 @@ int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_
                 double xx = g2->x[j] + sx;
                 if (xx < ra_min || xx > ra_max) continue;
                 cand++;
-                binned += corel_pair(&P, g1, i, g2, j, cx, cy, cz, h);
+                binned += corel_box(&P, g1, i, g2, j, cx, cy, cz, h);
               }
             }
           }
@@ -347,6 +462,8 @@ int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_
     }
   }
 
+  int bad = 0;
+  for (int t = 0; t < n_threads; t++) bad |= tbad[t];
   /* merge in thread order (comb_thread_bins, corr.cpp:88-100) */
   ohist acc;
   if (!hist_alloc(&acc, P.n_rp, P.n_pi, R)) return CPC_ERR_NOMEM;
@@ -370,6 +487,11 @@ int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_
   free(tbinned);
   free(tcand);
   free(tsparse);
+  free(tbad);
+  if (bad) {
+    hist_free(&acc);
+    return CPC_ERR_INVALID;
+  }
 
   /* final_jk_bins (jk.cpp:74-81): jk_r = full - touch_r over every field, add_bins(sub=true) */
   if (P.jk_on && params->jk_mode == CPC_JK_LEAVE_ONE_OUT) {
@@ -418,4 +540,12 @@ double orc_pi_bin(const cpc_params *p, double d_P) {
   oparm P;
   if (make_params(p, &P) != CPC_OK) return NAN;
   return floor((d_P - P.pmin) / P.psize);
+}
+
+/* The sky window of one outer galaxy (lim_calc_sky): w = z_min, z_max, ra_min, ra_max, dec_min,
+ * dec_max.  Returns 0 where the reference would exit. */
+int orc_sky_window(const cpc_params *p, double ra, double dec, double z, double *w) {
+  oparm P;
+  if (make_params(p, &P) != CPC_OK) return 0;
+  return sky_window(&P, ra, dec, z, w);
 }

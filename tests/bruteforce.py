@@ -158,3 +158,106 @@ def count(outer, inner, p):
             val[r] = val[0] - val[r]
             err[r] = err[0] - err[r]
     return {"num": num, "wt_num": wt, "val": val, "err": err, "num_rp": num.sum(axis=2)}
+
+
+R2D = 180.0 / math.pi
+
+
+def _interp(tab, dz, z):
+    i = int(math.floor(z / dz))
+    return tab[i] + ((tab[i + 1] - tab[i]) * (z - dz * i) / dz)
+
+
+def sky_window(p, ra, dec, z):
+    """lim_calc_sky + lim_calc_ra_dec for coordinates 0 (sky_calcs.cpp:105-190)."""
+    H, dA, dz = p._table[0], p._table[1], p.sky_dz
+    zmax = z + p.z_sep_max
+    zmax = z + p.pi_max * _interp(H, dz, zmax) / 300000.0
+    zmin = z + p.pi_min * _interp(H, dz, z) / 300000.0
+    if zmin < 0.001:
+        zmin = 0.001
+    da_max = p.rp_max / _interp(dA, dz, zmin)
+    dr = dec / R2D
+    arg = (math.cos(da_max) - math.sin(dr) ** 2) / math.cos(dr) ** 2
+    ra_lim = (math.acos(arg) if -1.0 <= arg <= 1.0 else float("nan")) * 1.02 * R2D
+    ra_max, ra_min = ra + ra_lim, ra - ra_lim
+    if ra_max > 360:
+        ra_min = 0
+    if ra_min < 0:
+        ra_max = 360
+    return zmin, zmax, ra_min, ra_max, dec - da_max * R2D * 1.05, dec + da_max * R2D * 1.05
+
+
+def count_sky(outer, inner, p):
+    """Every ordered (outer, inner) pair of the sky pass, no z-sorted sweep."""
+    pmin, pmax, rmin, rmax = p.pi_min, p.pi_max, p.rp_min, p.rp_max
+    psize = (pmax - pmin) / p.n_pi
+    rsize = (rmax - rmin) / p.n_rp if p.lin_bin else (math.log10(rmax) - math.log10(rmin)) / p.n_rp
+    jk_on = p.do_jk == 1 and p.n_jk > 0
+    R = 1 + p.n_jk if jk_on else 1
+    num = np.zeros((R, p.n_rp, p.n_pi), np.int64)
+    wt = np.zeros((R, p.n_rp, p.n_pi))
+    val = np.zeros((R, p.n_rp, p.n_pi, NF))
+    err = np.zeros((R, p.n_rp, p.n_pi, NF))
+
+    def g(c, name, i, default):
+        a = getattr(c, name)
+        return default if a is None else float(a[i])
+
+    for i in range(outer.n):
+        if g(outer, "include_prob", i, 0.0) > p.include_prob_max:
+            continue
+        rai, deci, zi = float(outer.x[i]), float(outer.y[i]), float(outer.z[i])
+        zmin, zmax, ramin, ramax, decmin, decmax = sky_window(p, rai, deci, zi)
+        for j in range(inner.n):
+            if g(inner, "include_prob", j, 0.0) > p.include_prob_max:
+                continue
+            raj, decj, zj = float(inner.x[j]), float(inner.y[j]), float(inner.z[j])
+            if zj > zmax or zj < zmin or decj > decmax or decj < decmin or raj < ramin or raj > ramax:
+                continue
+            dP = float(inner.dc[j]) - float(outer.dc[i])
+            d1, d2 = deci / R2D, decj / R2D
+            s = math.sin(d1) * math.sin(d2) + math.cos(d1) * math.cos(d2) * math.cos((rai / R2D - raj / R2D))
+            da = math.acos(1.0) if s > 0.9999999999 else math.acos(s)
+            rp = da * (float(outer.da[i]) + float(inner.da[j])) / 2.0
+            q2 = math.floor((dP - pmin) / psize)
+            if not (0 <= q2 < p.n_pi) or rp <= 0:
+                continue
+            q1 = math.floor((rp - rmin) / rsize) if p.lin_bin else math.floor(math.log10(rp / rmin) / rsize)
+            if not (0 <= q1 < p.n_rp):
+                continue
+            w = g(outer, "w", i, 1.0) * g(inner, "w", j, 1.0)
+            v = [0.0] * NF
+            e = [0.0] * NF
+            if p.kind == 1:
+                den = (raj - rai) * math.cos((d1 + d2) / 2.0)
+                th = math.atan((decj - deci) / den) if den != 0 else math.copysign(math.pi / 2, decj - deci)
+                src = (inner, j) if p.g_with_shape == 2 else (outer, i)
+                ep, ex = _rot(float(src[0].e1[src[1]]), float(src[0].e2[src[1]]), th)
+                v[1], e[1] = w * ep, (w * ep) ** 2
+                v[2], e[2] = w * ex, (w * ex) ** 2
+                v[6] = _ephi(g(outer, "phi", i, -10.0), th)
+                e[6] = v[6] ** 2
+            num[0, q1, q2] += 1
+            wt[0, q1, q2] += w
+            val[0, q1, q2] += v
+            err[0, q1, q2] += e
+            if jk_on:
+                a, b = int(outer.jk[i]), int(inner.jk[j])
+                pa, pb = g(outer, "jk_prob", i, 0.0), g(inner, "jk_prob", j, 0.0)
+                if a == b:
+                    pb = 0.0  # jk_check_sky
+                for r, pr in ((a, pa), (b, pb)):
+                    if pr < 0.5:
+                        continue
+                    num[1 + r, q1, q2] += 1
+                    wt[1 + r, q1, q2] += w
+                    if p.which_corr != 0:
+                        val[1 + r, q1, q2] += v
+    if jk_on and p.jk_mode == 0:
+        for r in range(1, R):
+            num[r] = num[0] - num[r]
+            wt[r] = wt[0] - wt[r]
+            val[r] = val[0] - val[r]
+            err[r] = err[0] - err[r]
+    return {"num": num, "wt_num": wt, "val": val, "err": err, "num_rp": num.sum(axis=2)}

@@ -18,11 +18,18 @@ PRECS = [PREC_SPLIT, PREC_FP64]
 
 
 def _run(engine, oracle_lib, outer, inner, p, what):
-    print(f"[case] {what}", flush=True)
-    got, st = engine.count(outer, inner, p)
+    """Both histogram accumulation modes of the engine against the oracle."""
     ref, rst = oracle_lib.count(outer, inner, p)
-    assert_hist_match(got, ref, what=what)
-    assert st["n_binned"] == rst["n_binned"]
+    st = None
+    for acc_mode in (0, 1):
+        print(f"[case] {what} acc_mode={acc_mode}", flush=True)
+        engine.set_tuning(acc_mode=acc_mode)
+        try:
+            got, st = engine.count(outer, inner, p)
+        finally:
+            engine.set_tuning()
+        assert_hist_match(got, ref, what=f"{what} acc_mode={acc_mode}")
+        assert st["n_binned"] == rst["n_binned"]
     return st
 
 
