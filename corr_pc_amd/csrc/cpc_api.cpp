@@ -2,8 +2,9 @@
 //
 // Responsibilities: argument validation with the reference's preconditions, the data_info-derived
 // binning quantities (initialization.cpp:305-311), the rp thresholds calibrated against the host
-// libm, cell-list orchestration on the device, work lists, the pair-kernel launches and the final
-// assembly of the reference's histogram layout (full sample + jackknife regions).
+// libm, the per-galaxy sky windows (lim_calc_sky), cell-list orchestration on the device, work
+// lists, the pair-kernel launches, the host binning of the few sky pairs the device cannot decide
+// bit-exactly, and the final assembly of the reference's histogram layout.
 
 #include <hip/hip_runtime.h>
 
@@ -24,7 +25,8 @@ using namespace cpc;
 
 namespace {
 
-const char *kVersion = "corr_pc_amd 0.1 (gfx950)";
+const char *kVersion = "corr_pc_amd 0.2 (gfx950)";
+const double kR2D = 180.0 / M_PI;  // r2d of calcs.cpp:19
 
 template <typename T>
 struct DevBuf {
@@ -52,19 +54,29 @@ struct Status {
   std::string msg;
 };
 
-// One catalog ordered for a pass: records in (column, region, coin, z) order plus group tables.
+// A 2-D cell grid: box columns (x, y) or sky cells (RA, Dec).
+struct Grid {
+  int ncx = 1, ncy = 1;
+  double x0 = 0, y0 = 0, hx = 1, hy = 1;
+};
+
+// One catalog ordered for a pass: records in (cell, region, coin, z) order plus group tables.
 struct Side {
   int64_t n_incl = 0;
   DevBuf<PRec> rec;
   DevBuf<FRec> fr;
+  DevBuf<SRec> sr;
+  DevBuf<Win> win;
   DevBuf<Group> d_groups;
   DevBuf<int> d_colbeg;
   std::vector<Group> groups;
-  std::vector<int> colbeg;           // [ncol+1]
-  std::vector<int64_t> colcount;     // galaxies per column
+  std::vector<int> colbeg;        // [ncells+1]
+  std::vector<int64_t> colcount;  // galaxies per cell
   void release() {
     rec.release();
     fr.release();
+    sr.release();
+    win.release();
     d_groups.release();
     d_colbeg.release();
   }
@@ -75,11 +87,12 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// rp bin thresholds.  bin_value restates corel_bin's rp index (corels.cpp:31-34) applied to
-// rp_calc_PB's sqrt (PB_calcs.cpp:97), kept as a double so out-of-range values stay ordered.
+// rp bin thresholds.  The reference's rp bin index (corels.cpp:31-34) kept as a double so that
+// out-of-range values stay ordered; as a function of the squared separation s (box: rp_calc_PB's
+// sqrt, PB_calcs.cpp:97) or of rp itself (sky: rp_calc_sky, sky_calcs.cpp:91-96).
 
-double bin_value(double s, double rmin, double rsize, int lin) {
-  double rp = std::sqrt(s);
+double bin_value(double v, bool squared, double rmin, double rsize, int lin) {
+  const double rp = squared ? std::sqrt(v) : v;
   if (lin) return std::floor((rp - rmin) / rsize);
   return std::floor(std::log10(rp / rmin) / rsize);
 }
@@ -95,22 +108,21 @@ double from_bits(uint64_t u) {
   return d;
 }
 
-// Smallest non-negative double s with bin_value(s) >= k, by bisection over the ordered bit
-// patterns of non-negative doubles.  Then a scan of `window` neighbours on each side confirms that
-// the bin index is monotone around the edge (it is a composition of correctly rounded sqrt and
-// division with the host log10).
-bool rp_threshold(double rmin, double rsize, int lin, int k, int window, double *out) {
+// Smallest non-negative double v with bin_value(v) >= k, by bisection over the ordered bit patterns
+// of non-negative doubles; then a scan of `window` neighbours on each side confirms that the index
+// is monotone around the edge (a composition of correctly rounded operations and the host log10).
+bool threshold(bool squared, double rmin, double rsize, int lin, int k, int window, double *out) {
   uint64_t lo = 0, hi = dbits(INFINITY);
   while (lo < hi) {
     uint64_t mid = lo + (hi - lo) / 2;
-    if (bin_value(from_bits(mid), rmin, rsize, lin) >= k)
+    if (bin_value(from_bits(mid), squared, rmin, rsize, lin) >= k)
       hi = mid;
     else
       lo = mid + 1;
   }
   for (int d = 1; d <= window; ++d) {
-    if (lo + d <= dbits(INFINITY) && !(bin_value(from_bits(lo + d), rmin, rsize, lin) >= k)) return false;
-    if (lo >= (uint64_t)d && (bin_value(from_bits(lo - d), rmin, rsize, lin) >= k)) return false;
+    if (lo + d <= dbits(INFINITY) && !(bin_value(from_bits(lo + d), squared, rmin, rsize, lin) >= k)) return false;
+    if (lo >= (uint64_t)d && (bin_value(from_bits(lo - d), squared, rmin, rsize, lin) >= k)) return false;
   }
   *out = from_bits(lo);
   return true;
@@ -124,19 +136,19 @@ Derived derive(const cpc_params &p) {
   Derived d;
   d.psize = (p.pi_max - p.pi_min) / (p.n_pi);  // initialization.cpp:305
   if (p.lin_bin)
-    d.rsize = (p.rp_max - p.rp_min) / p.n_rp;   // initialization.cpp:310
+    d.rsize = (p.rp_max - p.rp_min) / p.n_rp;  // initialization.cpp:310
   else
     d.rsize = (std::log10(p.rp_max) - std::log10(p.rp_min)) / p.n_rp;  // initialization.cpp:311
   return d;
 }
 
-Status thresholds(const cpc_params &p, int window, std::vector<double> &thr) {
+Status thresholds(const cpc_params &p, bool squared, int window, std::vector<double> &thr) {
   Derived d = derive(p);
   thr.assign(p.n_rp + 1, 0.0);
   for (int k = 0; k <= p.n_rp; ++k) {
-    if (!rp_threshold(p.rp_min, d.rsize, p.lin_bin, k, window, &thr[k]))
-      return {CPC_ERR_UNSUPPORTED, "rp bin index is not monotone near edge " + std::to_string(k) +
-                                       " with this host libm"};
+    if (!threshold(squared, p.rp_min, d.rsize, p.lin_bin, k, window, &thr[k]))
+      return {CPC_ERR_UNSUPPORTED,
+              "rp bin index is not monotone near edge " + std::to_string(k) + " with this host libm"};
   }
   return {};
 }
@@ -160,6 +172,52 @@ hipError_t check(hipError_t e, Status &st, const char *what) {
   return e;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Sky helpers: the distance-table interpolation and per-galaxy window of lim_calc_sky
+// (sky_calcs.cpp:33-81, 105-190), restated with the host libm.
+
+struct SkyTable {
+  const double *H, *dA;
+  int64_t n;
+  double dz, z_max, z_sep_max;
+};
+
+bool tab_ok(const SkyTable &T, double z) {
+  const double f = std::floor(z / T.dz);
+  return f >= 0 && f + 1 < (double)T.n;
+}
+
+double tab_interp(const double *tab, double dz, double z) {
+  const int i_t = (int)std::floor(z / dz);
+  return tab[i_t] + ((tab[i_t + 1] - tab[i_t]) * (z - dz * i_t) / dz);
+}
+
+// Window of an outer galaxy for coordinates 0; false where the reference exits (z_min > z_max of
+// the data, sky_calcs.cpp:109-113) or would read outside its table.
+bool sky_window(const SkyTable &T, double pmin, double pmax, double rmax, double ra, double dec, double z, Win &w) {
+  double z_max = z + T.z_sep_max;
+  if (!tab_ok(T, z_max) || !tab_ok(T, z)) return false;
+  z_max = z + pmax * tab_interp(T.H, T.dz, z_max) / 300000.0;  // p_to_dz(p_max, z + z_sep_max)
+  double z_min = z + pmin * tab_interp(T.H, T.dz, z) / 300000.0;  // p_to_dz(p_min, z)
+  if (z_min < 0.001) z_min = 0.001;
+  if (z_min > T.z_max || !tab_ok(T, z_min)) return false;
+  const double da_max = rmax / tab_interp(T.dA, T.dz, z_min);  // dAR_to_dA(bin_r_max, z_min)
+  const double dec_rad = dec / kR2D;
+  const double sd = std::sin(dec_rad), cd = std::cos(dec_rad);
+  const double ra_lim = std::acos((std::cos(da_max) - sd * sd) / (cd * cd)) * 1.02 * kR2D;
+  double ra_max = ra + ra_lim, ra_min = ra - ra_lim;
+  if (ra_max > 360) ra_min = 0;
+  if (ra_min < 0) ra_max = 360;
+  w.zmin = z_min;
+  w.zmax = z_max;
+  w.ramin = ra_min;
+  w.ramax = ra_max;
+  w.decmin = dec - da_max * kR2D * 1.05;
+  w.decmax = dec + da_max * kR2D * 1.05;
+  w.pad0 = w.pad1 = 0;
+  return true;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------------
@@ -170,11 +228,14 @@ struct cpc_catalog {
          *d_phi = nullptr, *d_jkp = nullptr, *d_inc = nullptr;
   int32_t *d_jk = nullptr;
   double *d_zsorted = nullptr;  // every galaxy's z ascending (only when the upload order is not sorted)
-  bool z_sorted = true, has_e = false;
+  SRec *d_sr = nullptr;         // sky quantities per galaxy (upload order), when DC/DA were given
+  bool z_sorted = true, has_e = false, has_jk = false, has_sky = false;
   double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   int32_t jk_lo = 0, jk_hi = -1;
-  bool has_jk = false;
-  std::vector<double> h_z, h_inc;  // for the first-included-galaxy rule
+  // host copies: z and include_prob (first-included-galaxy rule); for sky catalogs also what the
+  // host needs to bin guard-band pairs
+  std::vector<double> h_z, h_inc, h_x, h_y, h_w, h_e1, h_e2, h_phi, h_jkp, h_dc, h_da;
+  std::vector<int32_t> h_jk;
 };
 
 struct cpc_ctx {
@@ -191,15 +252,17 @@ struct cpc_ctx {
   // scratch
   DevBuf<uint32_t> keys_a, keys_b;
   DevBuf<int32_t> vals_a, vals_b, order;
-  DevBuf<double> zk_a, zk_b;
+  DevBuf<double> zk_b;
   DevBuf<double> thr;
   DevBuf<float> thr_lo, thr_hi;
   DevBuf<int2> nbr;
   DevBuf<Item> items;
   DevBuf<double> accp, acct;
   DevBuf<unsigned long long> accpn, acctn, counters;
+  DevBuf<Win> win_raw;
+  DevBuf<uint2> hq;
   Side side_o, side_i;
-  std::map<std::tuple<double, double, int, int>, std::vector<double>> thr_cache;
+  std::map<std::tuple<double, double, int, int, int>, std::vector<double>> thr_cache;
 };
 
 namespace {
@@ -210,21 +273,40 @@ cpc_status fail(cpc_ctx *ctx, const Status &s) {
 }
 cpc_status fail(cpc_ctx *ctx, cpc_status code, const std::string &msg) { return fail(ctx, Status{code, msg}); }
 
-// Build one side: keys, stable sort, gather, group tables.  `zfirst_plus_L` feeds the PBz>0 flag.
-Status build_side(cpc_ctx *ctx, const cpc_catalog *c, int nc, double h, int nreg, bool jk_on,
-                  const cpc_params &p, bool has_zfirst, double zfirst_plus_L, Side &S) {
+const std::vector<double> *cached_thresholds(cpc_ctx *ctx, const cpc_params &p, bool squared, Status &st) {
+  auto key = std::make_tuple(p.rp_min, p.rp_max, p.n_rp, p.lin_bin, squared ? 1 : 0);
+  auto it = ctx->thr_cache.find(key);
+  if (it == ctx->thr_cache.end()) {
+    std::vector<double> thr;
+    st = thresholds(p, squared, 256, thr);
+    if (st.code) return nullptr;
+    it = ctx->thr_cache.emplace(key, thr).first;
+  }
+  return &it->second;
+}
+
+// Build one side: keys, stable sort, gather, group tables.  `zfirst_plus_L` feeds the PBz>0 flag
+// (box only).  For the sky, `win_raw` (per original galaxy, outer side) is gathered as well.
+Status build_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, bool jk_on, const cpc_params &p,
+                  bool has_zfirst, double zfirst_plus_L, bool sky, const Win *win_raw, Side &S) {
   Status st;
   hipStream_t s = ctx->stream;
   const int64_t n = c->n;
+  const size_t ncells = (size_t)G.ncx * G.ncy;
   S.groups.clear();
-  S.colbeg.assign((size_t)nc * nc + 1, 0);
-  S.colcount.assign((size_t)nc * nc, 0);
+  S.colbeg.assign(ncells + 1, 0);
+  S.colcount.assign(ncells, 0);
   S.n_incl = 0;
   if (n == 0) {
     check(S.d_colbeg.ensure(S.colbeg.size()), st, "alloc");
     check(hipMemcpyAsync(S.d_colbeg.p, S.colbeg.data(), S.colbeg.size() * sizeof(int), hipMemcpyHostToDevice, s),
           st, "copy");
     check(S.d_groups.ensure(1), st, "alloc");
+    check(S.rec.ensure(kRecPad), st, "alloc");
+    check(S.fr.ensure(kRecPad), st, "alloc");
+    check(S.sr.ensure(1), st, "alloc");
+    check(S.win.ensure(1), st, "alloc");
+    check(hipStreamSynchronize(s), st, "sync");
     return st;
   }
   if (check(ctx->keys_a.ensure(n), st, "alloc") || check(ctx->keys_b.ensure(n), st, "alloc") ||
@@ -239,10 +321,12 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, int nc, double h, int nreg
     check(hipMemcpyAsync(ctx->vals_a.p, iota.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s), st, "copy");
     check(sort_f64(ctx->sort_tmp, ctx->sort_tmp_bytes, c->d_z, ctx->zk_b.p, ctx->vals_a.p, ctx->order.p, n, s), st,
           "sort z");
+    check(hipStreamSynchronize(s), st, "sync");
     if (st.code) return st;
     order = ctx->order.p;
   }
-  KeyArgs ka{c->d_x, c->d_y, c->d_inc, c->d_jk, jk_on ? c->d_jkp : nullptr, n, nc, h, nreg, p.include_prob_max};
+  KeyArgs ka{c->d_x, c->d_y, c->d_inc, c->d_jk, jk_on ? c->d_jkp : nullptr, n, G.ncx, G.ncy,
+             G.x0, G.y0, G.hx, G.hy, nreg, p.include_prob_max};
   check(launch_keys(ka, order, ctx->keys_a.p, ctx->vals_a.p, s), st, "keys");
   check(sort_u32(ctx->sort_tmp, ctx->sort_tmp_bytes, ctx->keys_a.p, ctx->keys_b.p, ctx->vals_a.p, ctx->vals_b.p, n, s),
         st, "sort keys");
@@ -253,13 +337,12 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, int nc, double h, int nreg
   check(hipMemcpyAsync(hidx.data(), ctx->vals_b.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, s), st, "copy");
   check(hipStreamSynchronize(s), st, "sync");
   if (st.code) return st;
-  int64_t ni = std::lower_bound(hkeys.begin(), hkeys.end(), 0xffffffffu) - hkeys.begin();
+  const int64_t ni = std::lower_bound(hkeys.begin(), hkeys.end(), 0xffffffffu) - hkeys.begin();
   S.n_incl = ni;
-  // groups: runs of equal keys
-  for (int64_t a = 0; a < ni;) {
+  for (int64_t a = 0; a < ni;) {  // groups: runs of equal keys
     int64_t b = a + 1;
     while (b < ni && hkeys[b] == hkeys[a]) ++b;
-    uint32_t key = hkeys[a];
+    const uint32_t key = hkeys[a];
     Group g{};
     g.start = (int32_t)a;
     g.end = (int32_t)b;
@@ -272,21 +355,24 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, int nc, double h, int nreg
     S.colcount[g.col] += b - a;
     a = b;
   }
-  // first group of each column
   {
     size_t gi = 0;
-    for (int col = 0; col < nc * nc; ++col) {
+    for (size_t col = 0; col < ncells; ++col) {
       S.colbeg[col] = (int)gi;
-      while (gi < S.groups.size() && S.groups[gi].col == col) ++gi;
+      while (gi < S.groups.size() && (size_t)S.groups[gi].col == col) ++gi;
     }
-    S.colbeg[(size_t)nc * nc] = (int)S.groups.size();
+    S.colbeg[ncells] = (int)S.groups.size();
   }
   if (check(S.rec.ensure(ni + kRecPad), st, "alloc") || check(S.fr.ensure(ni + kRecPad), st, "alloc") ||
       check(S.d_groups.ensure(S.groups.size()), st, "alloc") || check(S.d_colbeg.ensure(S.colbeg.size()), st, "alloc"))
     return st;
-  GatherArgs ga{c->d_x, c->d_y,  c->d_z, c->d_w, c->d_e1, c->d_e2, c->d_phi, ni, nc, h,
+  GatherArgs ga{c->d_x, c->d_y, c->d_z, c->d_w, c->d_e1, c->d_e2, c->d_phi, ni, G.ncx, G.ncy, G.x0, G.y0, G.hx, G.hy,
                 p.box_size, p.pi_min, p.pi_max, p.rp_max, zfirst_plus_L, has_zfirst ? 1 : 0};
   check(launch_gather(ga, ctx->vals_b.p, S.rec.p, S.fr.p, s), st, "gather");
+  if (sky) {
+    if (check(S.sr.ensure(ni), st, "alloc") || check(S.win.ensure(ni), st, "alloc")) return st;
+    check(launch_gather_sky(ctx->vals_b.p, ni, c->d_sr, win_raw, S.sr.p, S.win.p, s), st, "gather sky");
+  }
   if (!S.groups.empty())
     check(hipMemcpyAsync(S.d_groups.p, S.groups.data(), S.groups.size() * sizeof(Group), hipMemcpyHostToDevice, s),
           st, "copy");
@@ -294,6 +380,142 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, int nc, double h, int nreg
         "copy");
   check(hipStreamSynchronize(s), st, "sync");
   return st;
+}
+
+// Work items over the block side's groups, most expensive first.  `nbr_cost` gives, for a group,
+// the number of tile galaxies it may meet.
+template <typename Cost>
+void make_items(const Side &B, bool touch, bool jk_on, int item_size, Cost nbr_cost, std::vector<Item> &items) {
+  std::vector<std::pair<double, Item>> tmp;
+  for (size_t g = 0; g < B.groups.size(); ++g) {
+    const Group &G = B.groups[g];
+    if (touch && G.coin == 0) continue;
+    const double ncol = nbr_cost(G);
+    int target;
+    if (touch)
+      target = G.region;
+    else if (jk_on)
+      target = 2 * G.region + (G.coin ? 1 : 0);
+    else
+      target = 1;
+    for (int f = G.start; f < G.end; f += item_size) {
+      Item it{(int32_t)g, f, std::min(item_size, G.end - f), target};
+      tmp.push_back({(double)it.count * ncol, it});
+    }
+  }
+  std::stable_sort(tmp.begin(), tmp.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+  items.clear();
+  for (auto &t : tmp) items.push_back(t.second);
+}
+
+// Contributions binned on the host, in the reference layout: region 0 the full sample (all
+// fields), regions 1..n_jk the touch sums (num, wt_num, data[].val).
+struct HostHist {
+  int R = 0, ncell = 0;
+  std::vector<int64_t> num;
+  std::vector<double> wt, val, err;
+  void init(int R_, int ncell_) {
+    R = R_;
+    ncell = ncell_;
+    num.assign((size_t)R * ncell, 0);
+    wt.assign((size_t)R * ncell, 0.0);
+    val.assign((size_t)R * ncell * CPC_NFIELD, 0.0);
+    err.assign((size_t)R * ncell * CPC_NFIELD, 0.0);
+  }
+};
+
+double h_or(const std::vector<double> &v, int64_t i, double d) { return v.empty() ? d : v[i]; }
+
+// The reference's sky pair with the host libm (pi_calc, rp_calc_sky/dA_calc, corel_bin, the kind,
+// corel_jk with jk_check_sky): corels.cpp:21-244, calcs.cpp:60-64,97-105,126-132,
+// sky_calcs.cpp:13-31,91-96, jk.cpp:53-57,138-171,201-223.
+void host_sky_pair(const cpc_params &p, const Derived &dv, const cpc_catalog *g1, int64_t i, const cpc_catalog *g2,
+                   int64_t j, bool jk_on, HostHist &H) {
+  const double d_P = g2->h_dc[j] - g1->h_dc[i];
+  const double dec1 = g1->h_y[i] / kR2D, dec2 = g2->h_y[j] / kR2D;
+  const double ra1 = g1->h_x[i] / kR2D, ra2 = g2->h_x[j] / kR2D;
+  const double sum = std::sin(dec1) * std::sin(dec2) + std::cos(dec1) * std::cos(dec2) * std::cos((ra1 - ra2));
+  const double da = (sum > 0.9999999999) ? std::acos(1.0) : std::acos(sum);
+  const double da_R = da * (g1->h_da[i] + g2->h_da[j]) / 2.;
+  const double q2 = std::floor((d_P - p.pi_min) / dv.psize);
+  if (!(q2 >= 0 && q2 < p.n_pi)) return;
+  const double q1 = p.lin_bin ? std::floor((da_R - p.rp_min) / dv.rsize)
+                              : std::floor(std::log10(da_R / p.rp_min) / dv.rsize);
+  if (!(q1 >= 0 && q1 < p.n_rp)) return;
+  const int c = (int)q1 * p.n_pi + (int)q2;
+  const double wt = h_or(g1->h_w, i, 1.0) * h_or(g2->h_w, j, 1.0);
+  double v[CPC_NFIELD] = {0}, e[CPC_NFIELD] = {0};
+  if (p.kind != CPC_KIND_DD) {
+    const double theta =
+        std::atan((g2->h_y[j] - g1->h_y[i]) / ((g2->h_x[j] - g1->h_x[i]) * std::cos((dec1 + dec2) / 2.0)));
+    auto rot = [&](double e1, double e2, double &ep, double &ex) {
+      ep = std::cos(2.0 * theta) * e1 - std::sin(2.0 * theta) * e2;
+      ex = std::sin(2.0 * theta) * e1 + std::cos(2.0 * theta) * e2;
+    };
+    auto ephi = [&](double phi) {
+      double t = phi - theta;
+      t = t < 0 ? -1 * t : t;
+      if (t > M_PI / 2.0) t = M_PI - t;
+      return t;
+    };
+    if (p.kind == CPC_KIND_SHAPE_DENSITY) {
+      double ep, ex;
+      if (p.g_with_shape == 2)
+        rot(g2->h_e1[j], g2->h_e2[j], ep, ex);
+      else
+        rot(g1->h_e1[i], g1->h_e2[i], ep, ex);
+      v[1] = 1.0 * wt * ep;
+      e[1] = (wt * ep) * (wt * ep);
+      v[2] = 1.0 * wt * ex;
+      e[2] = (wt * ex) * (wt * ex);
+      v[6] = ephi(h_or(g1->h_phi, i, -10.0));
+      e[6] = v[6] * v[6];
+    } else {
+      double epi, exi, epj, exj;
+      rot(g1->h_e1[i], g1->h_e2[i], epi, exi);
+      rot(g2->h_e1[j], g2->h_e2[j], epj, exj);
+      if (p.estimator == 0) {
+        const double ep = p.g_with_shape == 2 ? epj : epi;
+        v[1] = 1.0 * wt * ep;
+        e[1] = (wt * ep) * (wt * ep);
+        v[6] = ephi(p.g_with_shape == 2 ? h_or(g2->h_phi, j, -10.0) : h_or(g1->h_phi, i, -10.0));
+        e[6] = v[6] * v[6];
+      } else {
+        v[1] = 1.0 * wt * (epi + epj) / 2.0;
+        e[1] = (wt * epi / 2.0) * (wt * epi / 2.0) + (wt * epj / 2.0) * (wt * epj / 2.0);
+        v[2] = 1.0 * wt * (exi + exj) / 2.0;
+        e[2] = (wt * exi / 2.0) * (wt * exi / 2.0) + (wt * exj / 2.0) * (wt * exj / 2.0);
+        const double t1 = ephi(h_or(g1->h_phi, i, -10.0)), t2 = ephi(h_or(g2->h_phi, j, -10.0));
+        e[6] = t1 * t1 + t2 * t2;
+        v[6] = t1 + t2;
+      }
+      v[3] = 1.0 * wt * (epi * epj);
+      e[3] = (wt * (epi * epj)) * (wt * (epi * epj));
+      v[4] = 1.0 * wt * (exi * exj);
+      e[4] = (wt * (exi * exj)) * (wt * (exi * exj));
+      v[5] = 1.0 * wt * (exi * epj + epi * exj) / 2.0;
+      const double a = wt * (exi * epj) / 2.0, b = wt * epi * exj / 2.0;
+      e[5] = a * a + b * b;
+    }
+  }
+  H.num[c]++;
+  H.wt[c] += wt;
+  for (int f = 1; f < CPC_NFIELD; ++f) {
+    H.val[(size_t)c * CPC_NFIELD + f] += v[f];
+    H.err[(size_t)c * CPC_NFIELD + f] += e[f];
+  }
+  if (!jk_on) return;
+  int gjk[2] = {g1->h_jk[i], g2->h_jk[j]};
+  double jp[2] = {h_or(g1->h_jkp, i, 0.0), h_or(g2->h_jkp, j, 0.0)};
+  if (gjk[0] == gjk[1]) jp[1] = 0;
+  for (int s = 0; s < 2; ++s) {
+    if (jp[s] < 0.5) continue;
+    const size_t o = (size_t)(1 + gjk[s]) * H.ncell + c;
+    H.num[o]++;
+    H.wt[o] += wt;
+    if (p.which_corr != 0)
+      for (int f = 1; f < CPC_NFIELD; ++f) H.val[o * CPC_NFIELD + f] += v[f];
+  }
 }
 
 }  // namespace
@@ -320,7 +542,8 @@ cpc_status cpc_init(cpc_ctx **out, int device) {
     if (prop.maxSharedMemoryPerMultiProcessor > ctx->max_lds) ctx->max_lds = prop.maxSharedMemoryPerMultiProcessor;
   }
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess || hipEventCreate(&ctx->ev2) != hipSuccess) {
+      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+      hipEventCreate(&ctx->ev2) != hipSuccess) {
     delete ctx;
     return CPC_ERR_DEVICE;
   }
@@ -337,7 +560,6 @@ void cpc_destroy(cpc_ctx *ctx) {
   ctx->vals_a.release();
   ctx->vals_b.release();
   ctx->order.release();
-  ctx->zk_a.release();
   ctx->zk_b.release();
   ctx->thr.release();
   ctx->thr_lo.release();
@@ -349,6 +571,8 @@ void cpc_destroy(cpc_ctx *ctx) {
   ctx->accpn.release();
   ctx->acctn.release();
   ctx->counters.release();
+  ctx->win_raw.release();
+  ctx->hq.release();
   ctx->side_o.release();
   ctx->side_i.release();
   if (ctx->sort_tmp) (void)hipFree(ctx->sort_tmp);
@@ -379,12 +603,14 @@ cpc_status cpc_catalog_upload(cpc_ctx *ctx, const cpc_catalog_desc *d, cpc_catal
     return fail(ctx, CPC_ERR_INVALID, "catalog size must be in [0, 2^31-1)");
   if (d->n > 0 && (!d->x || !d->y || !d->z)) return fail(ctx, CPC_ERR_INVALID, "x, y and z are required");
   if ((d->e1 == nullptr) != (d->e2 == nullptr)) return fail(ctx, CPC_ERR_INVALID, "e1 and e2 must both be given");
+  if ((d->dc == nullptr) != (d->da == nullptr)) return fail(ctx, CPC_ERR_INVALID, "dc and da must both be given");
   if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, CPC_ERR_DEVICE, "hipSetDevice failed");
   const int64_t n = d->n;
   cpc_catalog *c = new cpc_catalog();
   c->n = n;
   c->has_e = d->e1 != nullptr;
   c->has_jk = d->jk != nullptr;
+  c->has_sky = d->dc != nullptr;
   Status st;
   auto up = [&](const double *src, double **dst) {
     if (!src || n == 0) return;
@@ -404,6 +630,7 @@ cpc_status cpc_catalog_upload(cpc_ctx *ctx, const cpc_catalog_desc *d, cpc_catal
     if (!check(hipMalloc((void **)&c->d_jk, n * sizeof(int32_t)), st, "alloc"))
       check(hipMemcpyAsync(c->d_jk, d->jk, n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream), st, "copy");
   }
+  std::vector<SRec> sr;
   if (n > 0) {
     const double *v[3] = {d->x, d->y, d->z};
     for (int a = 0; a < 3; ++a) {
@@ -433,6 +660,35 @@ cpc_status cpc_catalog_upload(cpc_ctx *ctx, const cpc_catalog_desc *d, cpc_catal
       if (!check(hipMalloc((void **)&c->d_zsorted, n * sizeof(double)), st, "alloc"))
         check(hipMemcpy(c->d_zsorted, zs.data(), n * sizeof(double), hipMemcpyHostToDevice), st, "copy");
     }
+    if (c->has_sky && st.code == CPC_OK) {
+      // per-galaxy sky quantities with the host libm (read_dat.cpp:221-222, sky_calcs.cpp:26)
+      sr.resize(n);
+      for (int64_t i = 0; i < n; ++i) {
+        SRec &r = sr[i];
+        r.dc = d->dc[i];
+        r.da = d->da[i];
+        r.ra_rad = d->x[i] / kR2D;
+        r.dec_rad = d->y[i] / kR2D;
+        r.sd = std::sin(r.dec_rad);
+        r.cd = std::cos(r.dec_rad);
+        r.pad0 = r.pad1 = 0;
+      }
+      if (!check(hipMalloc((void **)&c->d_sr, n * sizeof(SRec)), st, "alloc"))
+        check(hipMemcpy(c->d_sr, sr.data(), n * sizeof(SRec), hipMemcpyHostToDevice), st, "copy");
+      auto keep = [&](const double *src, std::vector<double> &dst) {
+        if (src) dst.assign(src, src + n);
+      };
+      keep(d->x, c->h_x);
+      keep(d->y, c->h_y);
+      keep(d->w, c->h_w);
+      keep(d->e1, c->h_e1);
+      keep(d->e2, c->h_e2);
+      keep(d->phi, c->h_phi);
+      keep(d->jk_prob, c->h_jkp);
+      keep(d->dc, c->h_dc);
+      keep(d->da, c->h_da);
+      if (d->jk) c->h_jk.assign(d->jk, d->jk + n);
+    }
   }
   check(hipStreamSynchronize(ctx->stream), st, "sync");
   if (st.code != CPC_OK) {
@@ -450,6 +706,7 @@ void cpc_catalog_free(cpc_ctx *ctx, cpc_catalog *c) {
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (c->d_jk) (void)hipFree(c->d_jk);
+  if (c->d_sr) (void)hipFree(c->d_sr);
   delete c;
 }
 
@@ -464,7 +721,7 @@ cpc_status cpc_rand_stream(uint32_t seed, int64_t skip, int64_t n, double *out) 
 cpc_status cpc_rp_thresholds(const cpc_params *p, double *thr) {
   if (!p || !thr || p->n_rp <= 0) return CPC_ERR_INVALID;
   std::vector<double> t;
-  Status st = thresholds(*p, 64, t);
+  Status st = thresholds(*p, p->geometry != CPC_GEOM_SKY, 64, t);
   if (st.code) return st.code;
   std::copy(t.begin(), t.end(), thr);
   return CPC_OK;
@@ -476,21 +733,21 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
   if (!ctx) return CPC_ERR_INVALID;
   if (!co || !ci || !pp || !out) return fail(ctx, CPC_ERR_INVALID, "null argument");
   const cpc_params p = *pp;
+  const bool sky = p.geometry == CPC_GEOM_SKY;
   // ---- validation (the reference's own preconditions and the engine's)
-  if (p.geometry != CPC_GEOM_BOX) return fail(ctx, CPC_ERR_UNSUPPORTED, "only the periodic-box geometry is built");
-  if (!(p.box_size > 0)) return fail(ctx, CPC_ERR_INVALID, "box_size must be > 0");
+  if (p.geometry != CPC_GEOM_BOX && !sky) return fail(ctx, CPC_ERR_UNSUPPORTED, "unknown geometry");
+  if (!sky && !(p.box_size > 0)) return fail(ctx, CPC_ERR_INVALID, "box_size must be > 0");
   if (p.n_rp <= 0 || p.n_pi <= 0) return fail(ctx, CPC_ERR_INVALID, "n_rp and n_pi must be > 0");
   if (!(p.rp_max > 0) || p.rp_min > p.rp_max)
     return fail(ctx, CPC_ERR_INVALID, "Bad Bin limits");  // initialization.cpp:299-303
   if (!(p.pi_max > p.pi_min)) return fail(ctx, CPC_ERR_INVALID, "pi_max must exceed pi_min");
   if (!p.lin_bin && !(p.rp_min > 0)) return fail(ctx, CPC_ERR_INVALID, "log rp bins need rp_min > 0");
-  if (2 * p.rp_max >= p.box_size || p.pi_max - p.pi_min >= p.box_size)
+  if (!sky && (2 * p.rp_max >= p.box_size || p.pi_max - p.pi_min >= p.box_size))
     return fail(ctx, CPC_ERR_UNSUPPORTED, "the rp and pi windows must be narrower than the box");
   if (p.kind < CPC_KIND_DD || p.kind > CPC_KIND_SHAPE_SHAPE) return fail(ctx, CPC_ERR_INVALID, "unknown kind");
   if (p.kind != CPC_KIND_DD && p.g_with_shape != 1 && p.g_with_shape != 2)
     return fail(ctx, CPC_ERR_INVALID, "g_with_shape must be 1 or 2 for shape kinds");
-  if (p.do_jk != 0 && p.do_jk != 1)
-    return fail(ctx, CPC_ERR_UNSUPPORTED, "do_jk=2 (region pairs) is not supported");
+  if (p.do_jk != 0 && p.do_jk != 1) return fail(ctx, CPC_ERR_UNSUPPORTED, "do_jk=2 (region pairs) is not supported");
   if (p.precision != CPC_PREC_SPLIT && p.precision != CPC_PREC_FP64)
     return fail(ctx, CPC_ERR_INVALID, "unknown precision");
   const bool jk_on = p.do_jk == 1 && p.n_jk > 0;
@@ -506,12 +763,21 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
     return fail(ctx, CPC_ERR_INVALID, "shape_shape needs e1/e2 on both catalogs");
   if (p.kind == CPC_KIND_SHAPE_DENSITY) {
     const cpc_catalog *sc = p.g_with_shape == 2 ? ci : co;
-    if (sc->n && !sc->has_e) return fail(ctx, CPC_ERR_INVALID, "shape_density needs e1/e2 on the shape side");
+    if (sc->n && !sc->has_e)
+      return fail(ctx, CPC_ERR_INVALID, "shape_density needs e1/e2 on the shape side (the reference would "
+                                        "dereference an unallocated e pointer, corels.cpp:216-219)");
   }
-  for (const cpc_catalog *c : {co, ci})
-    for (int a = 0; a < 3; ++a)
-      if (c->n > 0 && (c->lo[a] < 0 || c->hi[a] > p.box_size))
-        return fail(ctx, CPC_ERR_INVALID, "coordinates must lie in [0, box_size]");
+  if (sky) {
+    if ((co->n && !co->has_sky) || (ci->n && !ci->has_sky))
+      return fail(ctx, CPC_ERR_INVALID, "the sky geometry needs dc and da on both catalogs");
+    if (!p.sky_H || !p.sky_dA || p.sky_n < 2 || !(p.sky_dz > 0))
+      return fail(ctx, CPC_ERR_INVALID, "the sky geometry needs the distance table");
+  } else {
+    for (const cpc_catalog *c : {co, ci})
+      for (int a = 0; a < 3; ++a)
+        if (c->n > 0 && (c->lo[a] < 0 || c->hi[a] > p.box_size))
+          return fail(ctx, CPC_ERR_INVALID, "coordinates must lie in [0, box_size]");
+  }
   if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, CPC_ERR_DEVICE, "hipSetDevice failed");
 
   const int R = jk_on ? 1 + p.n_jk : 1;
@@ -520,16 +786,9 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
   Status st;
   hipStream_t s = ctx->stream;
 
-  // ---- thresholds (cached per binning)
-  auto tkey = std::make_tuple(p.rp_min, p.rp_max, p.n_rp, p.lin_bin);
-  auto tit = ctx->thr_cache.find(tkey);
-  if (tit == ctx->thr_cache.end()) {
-    std::vector<double> thr;
-    st = thresholds(p, 256, thr);
-    if (st.code) return fail(ctx, st);
-    tit = ctx->thr_cache.emplace(tkey, thr).first;
-  }
-  const std::vector<double> &thr = tit->second;
+  const std::vector<double> *thrp = cached_thresholds(ctx, p, !sky, st);
+  if (!thrp) return fail(ctx, st);
+  const std::vector<double> &thr = *thrp;
 
   // ---- kind, slots
   int kind_code;  // kernel kind: 0 DD, 1 SD, 2 SS est 0, 3 SS est != 0
@@ -546,124 +805,189 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
   const bool split = p.precision == CPC_PREC_SPLIT;
   const int threads = ctx->block_threads ? ctx->block_threads : 256;
   int acc_mode = ctx->acc_mode >= 0 ? ctx->acc_mode : 0;
-  if (ncell > 65535) acc_mode = 0;  // the compaction ring packs the cell index in 16 bits
+  if (sky || ncell > 65535) acc_mode = 0;  // the compaction ring packs the cell index in 16 bits
   if (pair_lds_bytes(ncell, NSP, p.n_rp, threads / 64, acc_mode) > ctx->max_lds) acc_mode = 0;
   const size_t lds_p = pair_lds_bytes(ncell, NSP, p.n_rp, threads / 64, acc_mode);
   if (lds_p > ctx->max_lds)
     return fail(ctx, CPC_ERR_UNSUPPORTED,
                 "histogram of " + std::to_string(ncell) + " cells does not fit in LDS (" + std::to_string(lds_p) +
                     " bytes)");
-
-  // ---- grid of columns
-  const double L = p.box_size;
+  const int item_size = ctx->item_size ? ctx->item_size : 512;
   const int64_t nbig = std::max(co->n, ci->n);
-  int nc;
-  {
-    double frac = ctx->col_frac > 0 ? ctx->col_frac : 0.25;
-    nc = std::max(1, (int)std::floor(L / (p.rp_max * frac)));
-    // keep enough galaxies per column for full waves
-    int nc_dense = std::max(1, (int)std::floor(std::sqrt((double)std::max<int64_t>(nbig, 1) / 256.0)));
-    if (ctx->col_frac <= 0) nc = std::min(nc, nc_dense);
-    nc = std::min(nc, 1024);
-  }
-  const double h = L / nc;
 
-  // ---- neighbour column offsets (circle cut at the largest binned separation)
-  const double rcut = std::sqrt(thr[p.n_rp]) * (1 + 1e-9) + 1e-9 * L;
-  const int Rn = (int)std::floor(p.rp_max / h) + 1;
+  // ---- geometry-specific set-up: grid, neighbour cells, windows
+  Grid G;
   std::vector<int2> nbr;
-  for (int dy = -Rn; dy <= Rn; ++dy)
-    for (int dx = -Rn; dx <= Rn; ++dx) {
-      double ex = std::max(0, std::abs(dx) - 1) * h, ey = std::max(0, std::abs(dy) - 1) * h;
-      if (ex * ex + ey * ey <= rcut * rcut) nbr.push_back(make_int2(dx, dy));
-    }
-
-  // ---- first included galaxy of the inner catalog in upload order (PBz>0 sweep, corr.cpp:407-416)
+  int Rn = 0;
   bool has_zfirst = false;
   double zfirst_plus_L = 0;
-  for (int64_t j = 0; j < ci->n; ++j) {
-    double inc = ci->h_inc.empty() ? 0.0 : ci->h_inc[j];
-    if (inc > p.include_prob_max) continue;
-    has_zfirst = true;
-    zfirst_plus_L = ci->h_z[j] + L * 1;
-    break;
+  SkyArgs SA{};
+  std::vector<Win> wins;
+  auto t_build = std::chrono::steady_clock::now();
+  if (!sky) {
+    const double L = p.box_size;
+    double frac = ctx->col_frac > 0 ? ctx->col_frac : 0.25;
+    int nc = std::max(1, (int)std::floor(L / (p.rp_max * frac)));
+    // keep enough galaxies per column for full waves
+    const int nc_dense = std::max(1, (int)std::floor(std::sqrt((double)std::max<int64_t>(nbig, 1) / 256.0)));
+    if (ctx->col_frac <= 0) nc = std::min(nc, nc_dense);
+    nc = std::min(nc, 1024);
+    G.ncx = G.ncy = nc;
+    G.hx = G.hy = L / nc;
+    const double h = G.hx;
+    // neighbour column offsets (circle cut at the largest binned separation)
+    const double rcut = std::sqrt(thr[p.n_rp]) * (1 + 1e-9) + 1e-9 * L;
+    Rn = (int)std::floor(p.rp_max / h) + 1;
+    for (int dy = -Rn; dy <= Rn; ++dy)
+      for (int dx = -Rn; dx <= Rn; ++dx) {
+        const double ex = std::max(0, std::abs(dx) - 1) * h, ey = std::max(0, std::abs(dy) - 1) * h;
+        if (ex * ex + ey * ey <= rcut * rcut) nbr.push_back(make_int2(dx, dy));
+      }
+    // first included galaxy of the inner catalog in upload order (PBz>0 sweep, corr.cpp:407-416)
+    for (int64_t j = 0; j < ci->n; ++j) {
+      const double inc = ci->h_inc.empty() ? 0.0 : ci->h_inc[j];
+      if (inc > p.include_prob_max) continue;
+      has_zfirst = true;
+      zfirst_plus_L = ci->h_z[j] + L * 1;
+      break;
+    }
+  } else {
+    // per-galaxy windows of the outer catalog (lim_calc_sky), with the host libm
+    const SkyTable T{p.sky_H, p.sky_dA, p.sky_n, p.sky_dz, p.z_max, p.z_sep_max};
+    wins.resize(co->n);
+    double reach_ra = 0, reach_dec = 0, reach_zup = 0, reach_zdn = 0;
+    bool ra_full = false;
+    std::vector<double> dec_half;
+    dec_half.reserve(co->n);
+    for (int64_t i = 0; i < co->n; ++i) {
+      const double inc = co->h_inc.empty() ? 0.0 : co->h_inc[i];
+      Win &w = wins[i];
+      if (inc > p.include_prob_max) {
+        w = Win{1, 0, 0, 0, 0, 0, 0, 0};  // never used: the galaxy is not in the pass
+        continue;
+      }
+      if (!sky_window(T, p.pi_min, p.pi_max, p.rp_max, co->h_x[i], co->h_y[i], co->h_z[i], w))
+        return fail(ctx, CPC_ERR_INVALID,
+                    "galaxy " + std::to_string(i) + ": its redshift window leaves the distance table or exceeds z_max "
+                    "(the reference exits here, sky_calcs.cpp:109-113)");
+      if (!(w.ramin <= w.ramax) || w.ramin <= 0 || w.ramax >= 360)
+        ra_full = true;
+      else
+        reach_ra = std::max(reach_ra, std::max(co->h_x[i] - w.ramin, w.ramax - co->h_x[i]));
+      const double dh = std::max(co->h_y[i] - w.decmin, w.decmax - co->h_y[i]);
+      reach_dec = std::max(reach_dec, dh);
+      dec_half.push_back(dh);
+      reach_zup = std::max(reach_zup, w.zmax - co->h_z[i]);
+      reach_zdn = std::max(reach_zdn, co->h_z[i] - w.zmin);
+    }
+    // RA/Dec grid over both catalogs, cells about half the median window
+    double ralo = 360, rahi = 0, declo = 90, dechi = -90;
+    for (const cpc_catalog *c : {co, ci})
+      if (c->n) {
+        ralo = std::min(ralo, c->lo[0]);
+        rahi = std::max(rahi, c->hi[0]);
+        declo = std::min(declo, c->lo[1]);
+        dechi = std::max(dechi, c->hi[1]);
+      }
+    double hd = 1.0;
+    if (!dec_half.empty()) {
+      std::nth_element(dec_half.begin(), dec_half.begin() + dec_half.size() / 2, dec_half.end());
+      hd = dec_half[dec_half.size() / 2] * (ctx->col_frac > 0 ? ctx->col_frac : 0.5);
+    }
+    const double span_ra = std::max(rahi - ralo, 1e-6), span_dec = std::max(dechi - declo, 1e-6);
+    hd = std::max(hd, span_dec / 2048);
+    const double cdec = std::max(0.2, std::cos(0.5 * (declo + dechi) / kR2D));
+    const double hr = std::max(hd / cdec, span_ra / 2048);
+    G.ncx = std::max(1, (int)std::ceil(span_ra / hr));
+    G.ncy = std::max(1, (int)std::ceil(span_dec / hd));
+    G.hx = span_ra / G.ncx * (1 + 1e-12);
+    G.hy = span_dec / G.ncy * (1 + 1e-12);
+    G.x0 = ralo;
+    G.y0 = declo;
+    SA.reach_ra = reach_ra * (1 + 1e-12) + 1e-9;
+    SA.reach_dec = reach_dec * (1 + 1e-12) + 1e-9;
+    SA.reach_zup = reach_zup * (1 + 1e-12) + 1e-12;
+    SA.reach_zdn = reach_zdn * (1 + 1e-12) + 1e-12;
+    SA.ra_full = ra_full ? 1 : 0;
+    if (check(ctx->win_raw.ensure(std::max<int64_t>(co->n, 1)), st, "alloc")) return fail(ctx, st);
+    if (co->n)
+      check(hipMemcpyAsync(ctx->win_raw.p, wins.data(), co->n * sizeof(Win), hipMemcpyHostToDevice, s), st, "copy");
   }
 
-  auto t_build = std::chrono::steady_clock::now();
-  // ---- sides
-  st = build_side(ctx, co, nc, h, nreg, jk_on, p, has_zfirst, zfirst_plus_L, ctx->side_o);
+  // ---- sides (sort keys are (cell, region, coin) packed in 32 bits, all ones meaning excluded)
+  if ((double)G.ncx * G.ncy * nreg * 2.0 >= 4294967295.0)
+    return fail(ctx, CPC_ERR_UNSUPPORTED, "too many cells x jackknife regions for 32-bit sort keys");
+  st = build_side(ctx, co, G, nreg, jk_on, p, has_zfirst, zfirst_plus_L, sky, sky ? ctx->win_raw.p : nullptr,
+                  ctx->side_o);
   if (st.code) return fail(ctx, st);
   const bool same = (co == ci);
   if (!same) {
-    st = build_side(ctx, ci, nc, h, nreg, jk_on, p, has_zfirst, zfirst_plus_L, ctx->side_i);
+    st = build_side(ctx, ci, G, nreg, jk_on, p, has_zfirst, zfirst_plus_L, sky, nullptr, ctx->side_i);
     if (st.code) return fail(ctx, st);
   }
   Side &SO = ctx->side_o;
   Side &SI = same ? ctx->side_o : ctx->side_i;
   const double ms_build = ms_since(t_build);
 
-  // ---- work items
-  const int item_size = ctx->item_size ? ctx->item_size : 512;
-  auto make_items = [&](const Side &B, const Side &T, bool touch, std::vector<Item> &items) {
-    std::vector<std::pair<double, Item>> tmp;
-    for (size_t g = 0; g < B.groups.size(); ++g) {
-      const Group &G = B.groups[g];
-      if (touch && G.coin == 0) continue;
-      const int bcx = G.col % nc, bcy = G.col / nc;
-      double ncol = 0;
-      for (const int2 &o : nbr) {
-        int ux = ((bcx + o.x) % nc + nc) % nc, uy = ((bcy + o.y) % nc + nc) % nc;
-        ncol += (double)T.colcount[(size_t)uy * nc + ux];
-      }
-      int target;
-      if (touch)
-        target = G.region;
-      else if (jk_on)
-        target = 2 * G.region + (G.coin ? 1 : 0);
-      else
-        target = 1;
-      for (int f = G.start; f < G.end; f += item_size) {
-        Item it{(int32_t)g, f, std::min(item_size, G.end - f), target};
-        tmp.push_back({(double)it.count * ncol, it});
-      }
-    }
-    std::stable_sort(tmp.begin(), tmp.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
-    items.clear();
-    for (auto &t : tmp) items.push_back(t.second);
-  };
+  // ---- work items (cost: block galaxies x tile galaxies in reach)
   std::vector<Item> items_p, items_t;
-  make_items(SO, SI, false, items_p);
-  if (jk_on) make_items(SI, SO, true, items_t);
-
-  // ---- device tables
-  std::vector<float> tlo(p.n_rp + 1), thi(p.n_rp + 1);
-  const double two24 = std::ldexp(1.0, -24);
-  const double amax = 1.01 * h + 1e-9 * L;
-  const double omax = (Rn + 1) * h + 1e-9 * L;
-  const double eps_xy = 2.0 * (6 * amax + 2 * omax) * two24;
-  for (int k = 0; k <= p.n_rp; ++k) {
-    const double T = thr[k];
-    const double d = 2.0 * (2 * std::sqrt(2.0) * std::sqrt(T) * eps_xy + 2 * eps_xy * eps_xy + std::ldexp(T, -21));
-    tlo[k] = round_up_f(T + d);
-    thi[k] = round_down_f(T - d);
+  if (!sky) {
+    const int nc = G.ncx;
+    auto cost_of = [&](const Side &T) {
+      return [&T, &nbr, nc](const Group &g) {
+        const int bcx = g.col % nc, bcy = g.col / nc;
+        double c = 0;
+        for (const int2 &o : nbr) {
+          const int ux = ((bcx + o.x) % nc + nc) % nc, uy = ((bcy + o.y) % nc + nc) % nc;
+          c += (double)T.colcount[(size_t)uy * nc + ux];
+        }
+        return c;
+      };
+    };
+    make_items(SO, false, jk_on, item_size, cost_of(SI), items_p);
+    if (jk_on) make_items(SI, true, jk_on, item_size, cost_of(SO), items_t);
+  } else {
+    auto flat = [](const Group &) { return 1.0; };
+    make_items(SO, false, jk_on, item_size, flat, items_p);
+    if (jk_on) make_items(SI, true, jk_on, item_size, flat, items_t);
   }
-  const double eps_z = 2.0 * 5 * L * two24;
-  const double pabs = std::max(std::fabs(p.pi_min), std::fabs(p.pi_max));
-  double gz = eps_z / dv.psize + 8.0 * (p.n_pi + 2 * pabs / dv.psize + 2) * two24;
-  const bool split_ok = gz < 0.25;
 
-  size_t nP = (size_t)2 * nreg, nT = (size_t)nreg;
+  // ---- device tables and accumulators
+  const size_t nP = (size_t)2 * nreg, nT = (size_t)nreg;
   if (check(ctx->thr.ensure(p.n_rp + 1), st, "alloc") || check(ctx->thr_lo.ensure(p.n_rp + 1), st, "alloc") ||
-      check(ctx->thr_hi.ensure(p.n_rp + 1), st, "alloc") || check(ctx->nbr.ensure(nbr.size()), st, "alloc") ||
+      check(ctx->thr_hi.ensure(p.n_rp + 1), st, "alloc") ||
+      check(ctx->nbr.ensure(std::max<size_t>(nbr.size(), 1)), st, "alloc") ||
       check(ctx->items.ensure(items_p.size() + items_t.size()), st, "alloc") ||
       check(ctx->accp.ensure(nP * NSP * ncell), st, "alloc") || check(ctx->accpn.ensure(nP * ncell), st, "alloc") ||
       check(ctx->acct.ensure(nT * NST * ncell), st, "alloc") || check(ctx->acctn.ensure(nT * ncell), st, "alloc") ||
       check(ctx->counters.ensure(4), st, "alloc"))
     return fail(ctx, st);
+  std::vector<float> tlo(p.n_rp + 1, 0.0f), thi(p.n_rp + 1, 0.0f);
+  const double two24 = std::ldexp(1.0, -24);
+  double gz = 1.0;
+  if (!sky) {
+    // guard bands of the FP32 classification: bounds on the FP32 error of dx, dy (column-local
+    // coordinates plus a rounded column offset) and of dz (absolute coordinates)
+    const double L = p.box_size, h = G.hx;
+    const double amax = 1.01 * h + 1e-9 * L;
+    const double omax = (Rn + 1) * h + 1e-9 * L;
+    const double eps_xy = 2.0 * (6 * amax + 2 * omax) * two24;
+    for (int k = 0; k <= p.n_rp; ++k) {
+      const double T = thr[k];
+      const double d = 2.0 * (2 * std::sqrt(2.0) * std::sqrt(T) * eps_xy + 2 * eps_xy * eps_xy + std::ldexp(T, -21));
+      tlo[k] = round_up_f(T + d);
+      thi[k] = round_down_f(T - d);
+    }
+    const double eps_z = 2.0 * 5 * L * two24;
+    const double pabs = std::max(std::fabs(p.pi_min), std::fabs(p.pi_max));
+    gz = eps_z / dv.psize + 8.0 * (p.n_pi + 2 * pabs / dv.psize + 2) * two24;
+  }
+  const bool use_split = split && (sky || gz < 0.25);
   check(hipMemcpyAsync(ctx->thr.p, thr.data(), thr.size() * 8, hipMemcpyHostToDevice, s), st, "copy");
   check(hipMemcpyAsync(ctx->thr_lo.p, tlo.data(), tlo.size() * 4, hipMemcpyHostToDevice, s), st, "copy");
   check(hipMemcpyAsync(ctx->thr_hi.p, thi.data(), thi.size() * 4, hipMemcpyHostToDevice, s), st, "copy");
-  check(hipMemcpyAsync(ctx->nbr.p, nbr.data(), nbr.size() * sizeof(int2), hipMemcpyHostToDevice, s), st, "copy");
+  if (!nbr.empty())
+    check(hipMemcpyAsync(ctx->nbr.p, nbr.data(), nbr.size() * sizeof(int2), hipMemcpyHostToDevice, s), st, "copy");
   if (!items_p.empty())
     check(hipMemcpyAsync(ctx->items.p, items_p.data(), items_p.size() * sizeof(Item), hipMemcpyHostToDevice, s), st,
           "copy");
@@ -671,96 +995,182 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
     check(hipMemcpyAsync(ctx->items.p + items_p.size(), items_t.data(), items_t.size() * sizeof(Item),
                          hipMemcpyHostToDevice, s),
           st, "copy");
-  check(hipMemsetAsync(ctx->accp.p, 0, nP * NSP * ncell * 8, s), st, "memset");
-  check(hipMemsetAsync(ctx->accpn.p, 0, nP * ncell * 8, s), st, "memset");
-  check(hipMemsetAsync(ctx->acct.p, 0, nT * NST * ncell * 8, s), st, "memset");
-  check(hipMemsetAsync(ctx->acctn.p, 0, nT * ncell * 8, s), st, "memset");
-  check(hipMemsetAsync(ctx->counters.p, 0, 4 * 8, s), st, "memset");
   if (st.code) return fail(ctx, st);
 
-  KArgs K{};
-  K.L = L;
-  K.pmin = p.pi_min;
-  K.pmax = p.pi_max;
-  K.psize = dv.psize;
-  K.rmax = p.rp_max;
-  K.h = h;
-  K.zmargin = 1e-7 * (L + pabs);
-  K.nc = nc;
-  K.n_rp = p.n_rp;
-  K.n_pi = p.n_pi;
-  K.ncell = ncell;
-  K.gws = p.g_with_shape;
-  K.n_nbr = (int)nbr.size();
-  K.lin = p.lin_bin;
-  K.pmin32 = (float)p.pi_min;
-  K.inv_psize32 = (float)(1.0 / dv.psize);
-  K.gz = (float)gz;
-  K.la = (float)(0.5 * std::log10(2.0) / dv.rsize);
-  K.lb = (float)(-std::log10(p.rp_min > 0 ? p.rp_min : 1.0) / dv.rsize);
-  K.lrmin = (float)p.rp_min;
-  K.linv = (float)(1.0 / dv.rsize);
-  K.thr = ctx->thr.p;
-  K.thr_lo = ctx->thr_lo.p;
-  K.thr_hi = ctx->thr_hi.p;
-  K.nbr = ctx->nbr.p;
-  K.counters = ctx->counters.p;
-  K.acc_mode = acc_mode;
-  const bool use_split = split && split_ok;
-
-  check(hipEventRecord(ctx->ev0, s), st, "event");
-  if (!items_p.empty()) {
-    KArgs A = K;
-    A.n_items = (int)items_p.size();
-    A.phase_mode = jk_on ? 1 : 0;
-    A.nslots = NSP;
-    A.items = ctx->items.p;
-    A.bgrp = SO.d_groups.p;
-    A.tgrp = SI.d_groups.p;
-    A.tcol = SI.d_colbeg.p;
-    A.acc = ctx->accp.p;
-    A.accnum = ctx->accpn.p;
-    check(launch_pair(kind_code, use_split, false, A, threads, SO.rec.p, SO.fr.p, SI.rec.p, SI.fr.p, s), st,
-          "pair kernel");
-  }
-  check(hipEventRecord(ctx->ev2, s), st, "event");
-  if (!items_t.empty()) {
-    KArgs A = K;
-    A.n_items = (int)items_t.size();
-    A.phase_mode = 2;
-    A.nslots = NST;
-    A.items = ctx->items.p + items_p.size();
-    A.bgrp = SI.d_groups.p;
-    A.tgrp = SO.d_groups.p;
-    A.tcol = SO.d_colbeg.p;
-    A.acc = ctx->acct.p;
-    A.accnum = ctx->acctn.p;
-    check(launch_pair(kind_code, use_split, true, A, threads, SI.rec.p, SI.fr.p, SO.rec.p, SO.fr.p, s), st,
-          "touch kernel");
-  }
-  check(hipEventRecord(ctx->ev1, s), st, "event");
-  // sparse-window diagnostic
-  if (co->n > 0 && ci->n > 0) {
-    SparseArgs sa{co->d_z, co->d_inc, co->n, ci->z_sorted ? ci->d_z : ci->d_zsorted, ci->n,
-                  L, p.pi_min, p.pi_max, p.include_prob_max};
-    check(launch_sparse(sa, ctx->counters.p + 2, s), st, "sparse");
-  }
-  if (st.code) return fail(ctx, st);
-
-  // ---- results
+  // host-queue capacity for sky guard-band pairs; grown and the pass rerun if ever exceeded
+  int hq_cap = sky ? (int)std::min<int64_t>(std::max<int64_t>(1 << 16, nbig), 1 << 24) : 1;
+  std::vector<uint2> hq_pairs;
+  float kms = 0, kms_p = 0, kms_t = 0;
   std::vector<double> accp(nP * NSP * ncell), acct(nT * NST * ncell);
   std::vector<unsigned long long> accpn(nP * ncell), acctn(nT * ncell), counters(4);
+
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if (check(ctx->hq.ensure(hq_cap), st, "alloc")) return fail(ctx, st);
+    check(hipMemsetAsync(ctx->accp.p, 0, nP * NSP * ncell * 8, s), st, "memset");
+    check(hipMemsetAsync(ctx->accpn.p, 0, nP * ncell * 8, s), st, "memset");
+    check(hipMemsetAsync(ctx->acct.p, 0, nT * NST * ncell * 8, s), st, "memset");
+    check(hipMemsetAsync(ctx->acctn.p, 0, nT * ncell * 8, s), st, "memset");
+    check(hipMemsetAsync(ctx->counters.p, 0, 4 * 8, s), st, "memset");
+    check(hipEventRecord(ctx->ev0, s), st, "event");
+    if (!sky) {
+      KArgs K{};
+      K.L = p.box_size;
+      K.pmin = p.pi_min;
+      K.pmax = p.pi_max;
+      K.psize = dv.psize;
+      K.rmax = p.rp_max;
+      K.h = G.hx;
+      K.zmargin = 1e-7 * (p.box_size + std::max(std::fabs(p.pi_min), std::fabs(p.pi_max)));
+      K.nc = G.ncx;
+      K.n_rp = p.n_rp;
+      K.n_pi = p.n_pi;
+      K.ncell = ncell;
+      K.gws = p.g_with_shape;
+      K.n_nbr = (int)nbr.size();
+      K.lin = p.lin_bin;
+      K.pmin32 = (float)p.pi_min;
+      K.inv_psize32 = (float)(1.0 / dv.psize);
+      K.gz = (float)gz;
+      K.la = (float)(0.5 * std::log10(2.0) / dv.rsize);
+      K.lb = (float)(-std::log10(p.rp_min > 0 ? p.rp_min : 1.0) / dv.rsize);
+      K.lrmin = (float)p.rp_min;
+      K.linv = (float)(1.0 / dv.rsize);
+      K.thr = ctx->thr.p;
+      K.thr_lo = ctx->thr_lo.p;
+      K.thr_hi = ctx->thr_hi.p;
+      K.nbr = ctx->nbr.p;
+      K.counters = ctx->counters.p;
+      K.acc_mode = acc_mode;
+      if (!items_p.empty()) {
+        KArgs A = K;
+        A.n_items = (int)items_p.size();
+        A.phase_mode = jk_on ? 1 : 0;
+        A.nslots = NSP;
+        A.items = ctx->items.p;
+        A.bgrp = SO.d_groups.p;
+        A.tgrp = SI.d_groups.p;
+        A.tcol = SI.d_colbeg.p;
+        A.acc = ctx->accp.p;
+        A.accnum = ctx->accpn.p;
+        check(launch_pair(kind_code, use_split, false, A, threads, SO.rec.p, SO.fr.p, SI.rec.p, SI.fr.p, s), st,
+              "pair kernel");
+      }
+      check(hipEventRecord(ctx->ev2, s), st, "event");
+      if (!items_t.empty()) {
+        KArgs A = K;
+        A.n_items = (int)items_t.size();
+        A.phase_mode = 2;
+        A.nslots = NST;
+        A.items = ctx->items.p + items_p.size();
+        A.bgrp = SI.d_groups.p;
+        A.tgrp = SO.d_groups.p;
+        A.tcol = SO.d_colbeg.p;
+        A.acc = ctx->acct.p;
+        A.accnum = ctx->acctn.p;
+        check(launch_pair(kind_code, use_split, true, A, threads, SI.rec.p, SI.fr.p, SO.rec.p, SO.fr.p, s), st,
+              "touch kernel");
+      }
+    } else {
+      SkyArgs A = SA;
+      A.pmin = p.pi_min;
+      A.pmax = p.pi_max;
+      A.psize = dv.psize;
+      A.n_rp = p.n_rp;
+      A.n_pi = p.n_pi;
+      A.ncell = ncell;
+      A.gws = p.g_with_shape;
+      A.lin = p.lin_bin;
+      A.ra0 = G.x0;
+      A.dec0 = G.y0;
+      A.hra = G.hx;
+      A.hdec = G.hy;
+      A.nra = G.ncx;
+      A.ndec = G.ncy;
+      A.dsum = 4e-15;
+      A.thr = ctx->thr.p;
+      A.counters = ctx->counters.p;
+      A.hq = ctx->hq.p;
+      A.hq_cap = hq_cap;
+      if (!items_p.empty()) {
+        SkyArgs B = A;
+        B.n_items = (int)items_p.size();
+        B.phase_mode = jk_on ? 1 : 0;
+        B.nslots = NSP;
+        B.items = ctx->items.p;
+        B.bgrp = SO.d_groups.p;
+        B.tgrp = SI.d_groups.p;
+        B.tcol = SI.d_colbeg.p;
+        B.acc = ctx->accp.p;
+        B.accnum = ctx->accpn.p;
+        check(launch_sky(kind_code, use_split, false, B, threads, SO.rec.p, SO.sr.p, SO.win.p, SI.rec.p, SI.sr.p,
+                         SI.win.p, s),
+              st, "sky pair kernel");
+      }
+      check(hipEventRecord(ctx->ev2, s), st, "event");
+      if (!items_t.empty()) {
+        SkyArgs B = A;
+        B.n_items = (int)items_t.size();
+        B.phase_mode = 2;
+        B.nslots = NST;
+        B.items = ctx->items.p + items_p.size();
+        B.bgrp = SI.d_groups.p;
+        B.tgrp = SO.d_groups.p;
+        B.tcol = SO.d_colbeg.p;
+        B.acc = ctx->acct.p;
+        B.accnum = ctx->acctn.p;
+        check(launch_sky(kind_code, use_split, true, B, threads, SI.rec.p, SI.sr.p, SI.win.p, SO.rec.p, SO.sr.p,
+                         SO.win.p, s),
+              st, "sky touch kernel");
+      }
+    }
+    check(hipEventRecord(ctx->ev1, s), st, "event");
+    if (!sky && co->n > 0 && ci->n > 0) {  // sparse-window diagnostic
+      SparseArgs sa{co->d_z, co->d_inc, co->n, ci->z_sorted ? ci->d_z : ci->d_zsorted, ci->n,
+                    p.box_size, p.pi_min, p.pi_max, p.include_prob_max};
+      check(launch_sparse(sa, ctx->counters.p + 2, s), st, "sparse");
+    }
+    if (st.code) return fail(ctx, st);
+    check(hipMemcpyAsync(counters.data(), ctx->counters.p, 4 * 8, hipMemcpyDeviceToHost, s), st, "copy");
+    check(hipStreamSynchronize(s), st, "sync");
+    if (st.code) return fail(ctx, st);
+    if (sky && counters[3] > (unsigned long long)hq_cap) {
+      if (counters[3] > (unsigned long long)INT32_MAX) return fail(ctx, CPC_ERR_NOMEM, "guard-band queue overflow");
+      hq_cap = (int)counters[3];
+      continue;  // rerun with a queue large enough for every guard-band pair
+    }
+    break;
+  }
   check(hipMemcpyAsync(accp.data(), ctx->accp.p, accp.size() * 8, hipMemcpyDeviceToHost, s), st, "copy");
   check(hipMemcpyAsync(accpn.data(), ctx->accpn.p, accpn.size() * 8, hipMemcpyDeviceToHost, s), st, "copy");
   check(hipMemcpyAsync(acct.data(), ctx->acct.p, acct.size() * 8, hipMemcpyDeviceToHost, s), st, "copy");
   check(hipMemcpyAsync(acctn.data(), ctx->acctn.p, acctn.size() * 8, hipMemcpyDeviceToHost, s), st, "copy");
-  check(hipMemcpyAsync(counters.data(), ctx->counters.p, 4 * 8, hipMemcpyDeviceToHost, s), st, "copy");
+  if (sky && counters[3] > 0) {
+    hq_pairs.resize(counters[3]);
+    check(hipMemcpyAsync(hq_pairs.data(), ctx->hq.p, hq_pairs.size() * sizeof(uint2), hipMemcpyDeviceToHost, s), st,
+          "copy");
+  }
   check(hipStreamSynchronize(s), st, "sync");
   if (st.code) return fail(ctx, st);
-  float kms = 0, kms_p = 0, kms_t = 0;
   (void)hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1);
   (void)hipEventElapsedTime(&kms_p, ctx->ev0, ctx->ev2);
   (void)hipEventElapsedTime(&kms_t, ctx->ev2, ctx->ev1);
+
+  // ---- pairs the device could not bin bit-exactly: bin them with the host libm
+  HostHist HH;
+  const bool host_pairs = !hq_pairs.empty();
+  if (host_pairs) {
+    HH.init(R, ncell);
+    for (const uint2 &q : hq_pairs) host_sky_pair(p, dv, co, (int64_t)q.x, ci, (int64_t)q.y, jk_on, HH);
+  }
+  int64_t sparse = (int64_t)counters[2];
+  if (sky && ci->z_sorted && ci->n > 0) {  // sparse-window diagnostic on the host
+    for (int64_t i = 0; i < co->n; ++i) {
+      const double inc = co->h_inc.empty() ? 0.0 : co->h_inc[i];
+      if (inc > p.include_prob_max) continue;
+      auto it = std::lower_bound(ci->h_z.begin(), ci->h_z.end(), wins[i].zmin);
+      if (it != ci->h_z.end() && *it > wins[i].zmax) ++sparse;
+    }
+  }
 
   // ---- assemble the reference layout
   const size_t cells_all = (size_t)R * ncell;
@@ -769,28 +1179,36 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
   std::fill(out->val, out->val + cells_all * CPC_NFIELD, 0.0);
   std::fill(out->err, out->err + cells_all * CPC_NFIELD, 0.0);
   std::fill(out->num_rp, out->num_rp + (size_t)R * p.n_rp, 0);
-  auto add_target = [&](const double *acc, const unsigned long long *num, int NS, int t, size_t region, bool errs,
-                        double sign) {
+  // full sample = every primary target (+ host-binned pairs)
+  for (int t = 0; t < 2 * nreg; ++t)
     for (int c = 0; c < ncell; ++c) {
-      const size_t o = region * ncell + c;
-      const unsigned long long n = num[(size_t)t * ncell + c];
+      const unsigned long long n = accpn[(size_t)t * ncell + c];
       if (!n) continue;
-      out->num[o] += sign > 0 ? (int64_t)n : -(int64_t)n;
-      out->wt_num[o] += sign * acc[((size_t)t * NS + 0) * ncell + c];
+      out->num[c] += (int64_t)n;
+      out->wt_num[c] += accp[((size_t)t * NSP + 0) * ncell + c];
       for (int v = 0; v < NV; ++v) {
         const int f = kValField[kind_code][v];
-        out->val[o * CPC_NFIELD + f] += sign * acc[((size_t)t * NS + 1 + v) * ncell + c];
-        if (errs) out->err[o * CPC_NFIELD + f] += sign * acc[((size_t)t * NS + 1 + NV + v) * ncell + c];
+        out->val[(size_t)c * CPC_NFIELD + f] += accp[((size_t)t * NSP + 1 + v) * ncell + c];
+        out->err[(size_t)c * CPC_NFIELD + f] += accp[((size_t)t * NSP + 1 + NV + v) * ncell + c];
       }
     }
-  };
-  // full sample = every primary target
-  for (int t = 0; t < 2 * nreg; ++t) add_target(accp.data(), accpn.data(), NSP, t, 0, true, 1.0);
+  if (host_pairs)
+    for (int c = 0; c < ncell; ++c) {
+      out->num[c] += HH.num[c];
+      out->wt_num[c] += HH.wt[c];
+      for (int f = 0; f < CPC_NFIELD; ++f) {
+        out->val[(size_t)c * CPC_NFIELD + f] += HH.val[(size_t)c * CPC_NFIELD + f];
+        out->err[(size_t)c * CPC_NFIELD + f] += HH.err[(size_t)c * CPC_NFIELD + f];
+      }
+    }
   if (jk_on) {
+    std::vector<double> tv((size_t)ncell * CPC_NFIELD), tw(ncell);
+    std::vector<int64_t> tn(ncell);
     for (int r = 0; r < p.n_jk; ++r) {
-      // touch_r (jk.cpp:138-171): vals only; shape vals only when which_corr != 0
-      std::vector<double> tv(ncell * (NV + 1), 0.0);
-      std::vector<int64_t> tn(ncell, 0);
+      // touch_r (jk.cpp:138-171): num, wt_num and data[].val; shape vals only when which_corr != 0
+      std::fill(tv.begin(), tv.end(), 0.0);
+      std::fill(tw.begin(), tw.end(), 0.0);
+      std::fill(tn.begin(), tn.end(), 0);
       for (int src = 0; src < 2; ++src) {
         const double *acc = src == 0 ? accp.data() : acct.data();
         const unsigned long long *num = src == 0 ? accpn.data() : acctn.data();
@@ -800,31 +1218,34 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
           const unsigned long long n = num[(size_t)t * ncell + c];
           if (!n) continue;
           tn[c] += (int64_t)n;
-          for (int v = 0; v <= NV; ++v) tv[(size_t)c * (NV + 1) + v] += acc[((size_t)t * NS + v) * ncell + c];
+          tw[c] += acc[((size_t)t * NS + 0) * ncell + c];
+          for (int v = 0; v < NV; ++v)
+            tv[(size_t)c * CPC_NFIELD + kValField[kind_code][v]] += acc[((size_t)t * NS + 1 + v) * ncell + c];
         }
       }
+      if (host_pairs)
+        for (int c = 0; c < ncell; ++c) {
+          const size_t o = (size_t)(r + 1) * ncell + c;
+          tn[c] += HH.num[o];
+          tw[c] += HH.wt[o];
+          for (int f = 0; f < CPC_NFIELD; ++f) tv[(size_t)c * CPC_NFIELD + f] += HH.val[o * CPC_NFIELD + f];
+        }
+      if (p.which_corr == 0) std::fill(tv.begin(), tv.end(), 0.0);
       const size_t reg = (size_t)(r + 1);
       for (int c = 0; c < ncell; ++c) {
         const size_t o = reg * ncell + c;
-        const double *tvc = &tv[(size_t)c * (NV + 1)];
         if (p.jk_mode == CPC_JK_TOUCH) {
           out->num[o] = tn[c];
-          out->wt_num[o] = tvc[0];
-          if (p.which_corr != 0)
-            for (int v = 0; v < NV; ++v) out->val[o * CPC_NFIELD + kValField[kind_code][v]] = tvc[1 + v];
+          out->wt_num[o] = tw[c];
+          for (int f = 0; f < CPC_NFIELD; ++f) out->val[o * CPC_NFIELD + f] = tv[(size_t)c * CPC_NFIELD + f];
         } else {
           // final_jk_bins: add_bins(full, touch, jk, subtract) over every field (outp.cpp:242-288)
           out->num[o] = out->num[c] + tn[c] * -1;
-          out->wt_num[o] = out->wt_num[c] + tvc[0] * -1;
+          out->wt_num[o] = out->wt_num[c] + tw[c] * -1;
           for (int f = 0; f < CPC_NFIELD; ++f) {
-            out->val[o * CPC_NFIELD + f] = out->val[(size_t)c * CPC_NFIELD + f];
+            out->val[o * CPC_NFIELD + f] = out->val[(size_t)c * CPC_NFIELD + f] + tv[(size_t)c * CPC_NFIELD + f] * -1;
             out->err[o * CPC_NFIELD + f] = out->err[(size_t)c * CPC_NFIELD + f];
           }
-          if (p.which_corr != 0)
-            for (int v = 0; v < NV; ++v) {
-              const int f = kValField[kind_code][v];
-              out->val[o * CPC_NFIELD + f] = out->val[(size_t)c * CPC_NFIELD + f] + tvc[1 + v] * -1;
-            }
         }
       }
     }
@@ -841,12 +1262,13 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
     stats->n_binned = nbinned;
     stats->n_candidates = (int64_t)counters[0];
     stats->n_fp64_reeval = (int64_t)counters[1];
-    stats->n_sparse_windows = (int64_t)counters[2];
+    stats->n_sparse_windows = sparse;
     stats->ms_kernel = kms;
     stats->ms_primary = kms_p;
     stats->ms_touch = kms_t;
     stats->ms_build = ms_build;
     stats->n_items_primary = (int64_t)items_p.size();
+    stats->n_host_pairs = (int64_t)hq_pairs.size();
     stats->ms_total = ms_since(t_start);
   }
   ctx->err.clear();

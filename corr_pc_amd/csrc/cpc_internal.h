@@ -89,14 +89,52 @@ struct KArgs {
 };
 
 
+// Sky geometry: per-galaxy quantities the pair test needs besides PRec, all computed on the host
+// with the reference's libm so that the device reproduces its sums term by term.
+struct alignas(64) SRec {
+  double dc, da;          // gal.DC, gal.DA
+  double sd, cd;          // sin and cos of gal.dec.val_rad
+  double ra_rad, dec_rad; // gal.ra.val_rad, gal.dec.val_rad (= degrees / r2d)
+  double pad0, pad1;
+};
+
+// The window lim_calc_sky gives an outer galaxy (sky_calcs.cpp:105-190).
+struct alignas(64) Win {
+  double zmin, zmax, ramin, ramax, decmin, decmax;
+  double pad0, pad1;
+};
+
+struct SkyArgs {
+  double pmin, pmax, psize;
+  int n_rp, n_pi, ncell, gws, n_items, nslots, lin;
+  int phase_mode;          // 0 no jackknife, 1 jackknife (primary pass), 2 touch pass
+  // cell grid over RA/Dec (degrees)
+  double ra0, dec0, hra, hdec;
+  int nra, ndec;
+  // touch pass: how far an outer window can reach from its galaxy
+  double reach_ra, reach_dec, reach_zup, reach_zdn;
+  int ra_full;             // some outer window spans every RA
+  double dsum;             // bound on the device/host difference of the great-circle cosine sum
+  const double *thr;       // n_rp+1 thresholds on rp (host libm log10)
+  const Item *items;
+  const Group *bgrp;
+  const Group *tgrp;
+  const int *tcol;
+  double *acc;
+  unsigned long long *accnum;
+  unsigned long long *counters;  // [0] candidates, [1] FP64 re-evaluations, [3] pairs for the host
+  uint2 *hq;               // pairs (outer index, inner index) the host bins with its libm
+  int hq_cap;
+};
+
 // Arguments of the cell-list construction kernels.
 struct KeyArgs {
   const double *x, *y, *inc;
   const int32_t *jk;
   const double *jkp;
   int64_t n;
-  int nc;
-  double h;
+  int ncx, ncy;      // cell grid: cx = floor((x - x0)/hx) clamped to [0, ncx)
+  double x0, y0, hx, hy;
   int nreg;          // regions in the key (1 when the jackknife is off)
   double inc_max;
 };
@@ -104,8 +142,8 @@ struct KeyArgs {
 struct GatherArgs {
   const double *x, *y, *z, *w, *e1, *e2, *phi;
   int64_t n;  // included galaxies
-  int nc;
-  double h;
+  int ncx, ncy;
+  double x0, y0, hx, hy;
   // image flags (lim_calc_PB, PB_calcs.cpp:30-61)
   double L, pmin, pmax, rmax;
   double zfirst_plus_L;  // fl(z_first + L) of the inner catalog's first included galaxy
@@ -131,5 +169,10 @@ hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *ko
 size_t pair_lds_bytes(int ncell, int nslots, int n_rp, int nwave, int acc_mode);
 hipError_t launch_pair(int kind, bool split, bool swap, const KArgs &P, int threads, const PRec *brec,
                        const FRec *bfr, const PRec *trec, const FRec *tfr, hipStream_t st);
+hipError_t launch_sky(int kind, bool split, bool swap, const SkyArgs &P, int threads, const PRec *brec,
+                      const SRec *bsr, const Win *bwin, const PRec *trec, const SRec *tsr, const Win *twin,
+                      hipStream_t st);
+hipError_t launch_gather_sky(const int32_t *sidx, int64_t n, const SRec *src_sr, const Win *src_win, SRec *sr,
+                             Win *win, hipStream_t st);
 
 }  // namespace cpc

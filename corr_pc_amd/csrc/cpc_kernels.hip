@@ -27,199 +27,11 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include "cpc_device.h"
 #include "cpc_internal.h"
 
 namespace cpc {
 
-enum { kDD = 0, kSD = 1, kSS0 = 2, kSS1 = 3 };
-
-template <int K>
-struct KindNV;
-template <>
-struct KindNV<kDD> { static constexpr int NV = 0; };
-template <>
-struct KindNV<kSD> { static constexpr int NV = 3; };  // data[1], data[2], data[6]
-template <>
-struct KindNV<kSS0> { static constexpr int NV = 5; };  // data[1], data[3..6]
-template <>
-struct KindNV<kSS1> { static constexpr int NV = 6; };  // data[1..6]
-
-// 1/x to about one ulp: hardware reciprocal estimate refined by two Newton steps (x finite, != 0).
-__device__ __forceinline__ double rcp_refined(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-x, r, 1.0);
-  return fma(r, e, r);
-}
-
-struct Gal {
-  double x, y, w, e1, e2, phi;
-};
-
-// ephi_calc, calcs.cpp:97-105 (with absolute() of calcs.cpp:50-53)
-__device__ __forceinline__ double ephi(double phi, double theta) {
-  double t = phi - theta;
-  t = t < 0 ? -1 * t : t;
-  if (t > M_PI / 2.0) t = M_PI - t;
-  return t;
-}
-
-// Shape quantities of one binned pair, g1 = outer, g2 = inner (corels.cpp:122-244).
-// q = [wt, vals..., errs...] in the slot order of KindNV.
-template <int K, bool SPLIT>
-__device__ __forceinline__ void pair_values(const Gal &g1, const Gal &g2, int gws, double *q) {
-  const double wt = g1.w * g2.w;
-  q[0] = wt;
-  if constexpr (K == kDD) {
-    return;
-  } else {
-    // angle_orient_axis_PB, calcs.cpp:54-57: raw coordinates, no image shift
-    const double dxr = g2.x - g1.x, dyr = g2.y - g1.y;
-    double theta, c2, s2;
-    if constexpr (SPLIT) {
-      // theta = atan(dy/dx) with a refined reciprocal; cos 2theta and sin 2theta algebraically
-      // (about one ulp each, far inside the weighted-sum tolerance)
-      theta = atan((dxr != 0.0) ? dyr * rcp_refined(dxr) : copysign(INFINITY, dyr));
-      const double xx = dxr * dxr, yy = dyr * dyr;
-      const double inv = rcp_refined(xx + yy);
-      c2 = (xx - yy) * inv;
-      s2 = (2.0 * dxr * dyr) * inv;
-    } else {
-      theta = atan(dyr / dxr);
-      c2 = cos(2.0 * theta);
-      s2 = sin(2.0 * theta);
-    }
-    if constexpr (K == kSD) {
-      // shape_density_corel: rotate the shape galaxy, theta column from g1's phi
-      const Gal &s = (gws == 2) ? g2 : g1;
-      const double ep = c2 * s.e1 - s2 * s.e2;
-      const double ex = s2 * s.e1 + c2 * s.e2;
-      const double a = wt * ep, b = wt * ex;
-      const double t = ephi(g1.phi, theta);
-      q[1] = a;
-      q[2] = b;
-      q[3] = t;
-      q[4] = a * a;
-      q[5] = b * b;
-      q[6] = t * t;
-    } else {
-      const double epi = c2 * g1.e1 - s2 * g1.e2, exi = s2 * g1.e1 + c2 * g1.e2;
-      const double epj = c2 * g2.e1 - s2 * g2.e2, exj = s2 * g2.e1 + c2 * g2.e2;
-      const double pp = wt * (epi * epj), xx2 = wt * (exi * exj);
-      const double px = wt * (exi * epj + epi * exj) / 2.0;
-      const double ea = wt * (exi * epj) / 2.0, eb = wt * epi * exj / 2.0;
-      if constexpr (K == kSS0) {
-        // estimator 0: e+ of the shape-side galaxy only, ex never assigned (data[2] += 0)
-        const double ep = (gws == 2) ? epj : epi;
-        const double a = wt * ep;
-        const double t = ephi((gws == 2) ? g2.phi : g1.phi, theta);
-        q[1] = a;
-        q[2] = pp;
-        q[3] = xx2;
-        q[4] = px;
-        q[5] = t;
-        q[6] = a * a;
-        q[7] = pp * pp;
-        q[8] = xx2 * xx2;
-        q[9] = ea * ea + eb * eb;
-        q[10] = t * t;
-      } else {
-        const double ep = wt * (epi + epj) / 2.0;
-        const double ex = wt * (exi + exj) / 2.0;
-        const double ai = wt * epi / 2.0, aj = wt * epj / 2.0;
-        const double bi = wt * exi / 2.0, bj = wt * exj / 2.0;
-        const double t1 = ephi(g1.phi, theta), t2 = ephi(g2.phi, theta);
-        q[1] = ep;
-        q[2] = ex;
-        q[3] = pp;
-        q[4] = xx2;
-        q[5] = px;
-        q[6] = t1 + t2;
-        q[7] = ai * ai + aj * aj;
-        q[8] = bi * bi + bj * bj;
-        q[9] = pp * pp;
-        q[10] = xx2 * xx2;
-        q[11] = ea * ea + eb * eb;
-        q[12] = t1 * t1 + t2 * t2;
-      }
-    }
-  }
-}
-
-// rp bin of a squared separation against the FP64 thresholds thr[0..n_rp]: bin k iff
-// thr[k] <= s < thr[k+1].  An FP32 logarithm gives the starting guess.
-__device__ __forceinline__ int rp_bin64(const KArgs &P, const double *thr, double s) {
-  const float sf = (float)s;
-  float kf = P.lin ? (sqrtf(sf) - P.lrmin) * P.linv : __log2f(sf) * P.la + P.lb;
-  kf = fminf(fmaxf(kf, -1.0f), (float)P.n_rp);
-  int k = (int)floorf(kf);
-  while (k >= 0 && s < thr[k]) --k;
-  while (k < P.n_rp && s >= thr[k + 1]) ++k;
-  return (k >= 0 && k < P.n_rp) ? k : -1;
-}
-
-// The reference's candidate test and binning for outer (x1,y1,z1), inner (x2,y2,z2) seen through
-// image (PBx, PBy, PBz): do_pair_sorted (corr.cpp:67-86) then corel_bin (corels.cpp:21-46).
-// Returns the histogram cell n1*n_pi + n2, or -1.
-__device__ __forceinline__ int exact_cell(const KArgs &P, const double *thr, double x1, double y1, double z1,
-                                          double x2, double y2, double z2, int PBx, int PBy, int PBz) {
-  const double L = P.L;
-  const double zz = z2 + L * PBz;
-  const double zmax = z1 + P.pmax, zmin = z1 + P.pmin;
-  if (zz > zmax || zz < zmin) return -1;
-  const double yy = y2 + L * PBy;
-  if (yy > y1 + P.rmax || yy < y1 - P.rmax) return -1;
-  const double xx = x2 + L * PBx;
-  if (xx < x1 - P.rmax || xx > x1 + P.rmax) return -1;
-  const double q2 = floor(((zz - z1) - P.pmin) / P.psize);
-  if (!(q2 >= 0.0 && q2 < (double)P.n_pi)) return -1;
-  const double t0 = x1 - xx, t1 = y1 - yy;
-  const double s = t0 * t0 + t1 * t1;
-  const int k = rp_bin64(P, thr, s);
-  if (k < 0) return -1;
-  return k * P.n_pi + (int)q2;
-}
-
-// Whether the outer galaxy with image flags `meta` visits image (PBx, PBy, PBz) (corr.cpp:380-423).
-__device__ __forceinline__ bool image_allowed(int meta, int PBx, int PBy, int PBz) {
-  bool ok = true;
-  ok &= (PBx == 0) || (PBx < 0 ? (meta & kMetaXm) : (meta & kMetaXp));
-  ok &= (PBy == 0) || (PBy < 0 ? (meta & kMetaYm) : (meta & kMetaYp));
-  ok &= (PBz == 0) || (PBz < 0 ? (meta & kMetaZm) : (meta & kMetaZp));
-  return ok;
-}
-
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// first index in [lo, hi) of the z-sorted records with z >= v (upper=false) or z > v (upper=true)
-template <bool UPPER>
-__device__ __forceinline__ int zsearch(const PRec *__restrict__ rec, int lo, int hi, double v) {
-  while (lo < hi) {
-    const int mid = uni(lo + ((hi - lo) >> 1));
-    const double z = rec[mid].z;
-    const bool go_right = UPPER ? (z <= v) : (z < v);
-    if (go_right)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
-template <int NS>
-__device__ __forceinline__ void lds_add(unsigned *lnum, double *lacc, int ncell, int cell, unsigned n,
-                                        const double *v) {
-  atomicAdd(&lnum[cell], n);
-#pragma unroll
-  for (int s = 0; s < NS; ++s) atomicAdd(&lacc[s * ncell + cell], v[s]);
-}
-
-struct F4 {
-  FRec r[4];
-};
-
-// ACC: 0 = run-length privatisation in registers, 1 = per-wave compaction of binned pairs.
 template <int K, bool SPLIT, bool SWAP, int ACC>
 __global__ void __launch_bounds__(256) pair_kernel(KArgs P, const PRec *__restrict__ brec,
                                                    const FRec *__restrict__ bfr, const PRec *__restrict__ trec,
@@ -486,12 +298,12 @@ __global__ void k_keys(KeyArgs A, const int32_t *__restrict__ order, uint32_t *_
   uint32_t key = 0xffffffffu;
   const double inc = A.inc ? A.inc[i] : 0.0;
   if (!(inc > A.inc_max)) {
-    int cx = (int)floor(A.x[i] / A.h), cy = (int)floor(A.y[i] / A.h);
-    cx = min(max(cx, 0), A.nc - 1);
-    cy = min(max(cy, 0), A.nc - 1);
+    int cx = (int)floor((A.x[i] - A.x0) / A.hx), cy = (int)floor((A.y[i] - A.y0) / A.hy);
+    cx = min(max(cx, 0), A.ncx - 1);
+    cy = min(max(cy, 0), A.ncy - 1);
     const int reg = (A.nreg > 1) ? A.jk[i] : 0;
     const int coin = A.jkp ? (A.jkp[i] >= 0.5) : 0;
-    key = ((uint32_t)(cy * A.nc + cx) * (uint32_t)A.nreg + (uint32_t)reg) * 2u + (uint32_t)coin;
+    key = ((uint32_t)(cy * A.ncx + cx) * (uint32_t)A.nreg + (uint32_t)reg) * 2u + (uint32_t)coin;
   }
   keys[p] = key;
   vals[p] = i;
@@ -525,12 +337,12 @@ __global__ void k_gather(GatherArgs A, const int32_t *__restrict__ sidx, PRec *_
   if (z_max > A.L && A.has_zfirst && !(A.zfirst_plus_L < z_min)) m |= kMetaZp;
   r.meta = m;
   rec[p] = r;
-  int cx = (int)floor(r.x / A.h), cy = (int)floor(r.y / A.h);
-  cx = min(max(cx, 0), A.nc - 1);
-  cy = min(max(cy, 0), A.nc - 1);
+  int cx = (int)floor((r.x - A.x0) / A.hx), cy = (int)floor((r.y - A.y0) / A.hy);
+  cx = min(max(cx, 0), A.ncx - 1);
+  cy = min(max(cy, 0), A.ncy - 1);
   FRec f;
-  f.xl = (float)(r.x - cx * A.h);
-  f.yl = (float)(r.y - cy * A.h);
+  f.xl = (float)(r.x - (A.x0 + cx * A.hx));
+  f.yl = (float)(r.y - (A.y0 + cy * A.hy));
   f.zf = (float)r.z;
   f.meta = m;
   fr[p] = f;

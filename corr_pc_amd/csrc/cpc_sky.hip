@@ -1,0 +1,255 @@
+// cpc_sky.hip -- pair kernel of the sky geometry (RA/Dec/redshift, rp-pi, corr_pc coordinates 0).
+//
+// Same decomposition as the box kernel: one workgroup per run of block galaxies in one
+// (RA/Dec cell, region, coin) group, a private (rp, pi) histogram in LDS, lanes = 64 block galaxies
+// contiguous in redshift, tile galaxies streamed with wave-uniform loads.  Differences:
+//   * each outer galaxy carries the window lim_calc_sky computed on the host (sky_calcs.cpp:105-190);
+//     the candidate test compares against it exactly as do_pair_sorted does (corr.cpp:67-86);
+//   * pi = DC_j - DC_i (calcs.cpp:126-132) is exact; rp = acos(sum) (DA_i + DA_j)/2
+//     (sky_calcs.cpp:24-31, 91-96) uses sin/cos of Dec from the host and the device cos/acos, so a
+//     pair whose rp lies within a proven bound of a bin edge (or whose sum lies near the 0.9999999999
+//     clamp) is handed to the host, which bins it with the reference's libm;
+//   * jackknife: the outer galaxy's region is touched iff its jk_prob >= 0.5 whatever the inner region
+//     (jk_check_sky, jk.cpp:53-57), so there is a single phase.
+
+#include <hip/hip_runtime.h>
+
+#include "cpc_device.h"
+#include "cpc_internal.h"
+
+namespace cpc {
+
+__device__ __forceinline__ double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+__device__ __forceinline__ int rp_bin_sky(const SkyArgs &P, const double *thr, double rp, double band,
+                                          bool *unc) {
+  // bin k iff thr[k] <= rp < thr[k+1]; uncertain if rp +- band straddles an edge
+  int k = -1;
+  while (k < P.n_rp && rp >= thr[k + 1]) ++k;
+  const bool lo_ok = (k < 0) ? true : (rp - band >= thr[k]);
+  const bool hi_ok = (k >= P.n_rp) ? true : (rp + band < thr[k + 1]);
+  *unc = !(lo_ok && hi_ok);
+  return (k >= 0 && k < P.n_rp) ? k : -1;
+}
+
+template <int K, bool SPLIT, bool SWAP>
+__global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, const PRec *__restrict__ brec,
+                                                  const SRec *__restrict__ bsr, const Win *__restrict__ bwin,
+                                                  const PRec *__restrict__ trec, const SRec *__restrict__ tsr,
+                                                  const Win *__restrict__ twin) {
+  constexpr int NV = KindNV<K>::NV;
+  constexpr int NS = SWAP ? 1 + NV : 1 + 2 * NV;
+  constexpr int NQ = 1 + 2 * NV;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ncell = P.ncell;
+  unsigned *lnum = reinterpret_cast<unsigned *>(smem);
+  double *lacc = reinterpret_cast<double *>(smem + ((ncell * 4 + 15) & ~15));
+  double *lthr = lacc + (size_t)NS * ncell;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = uni(tid >> 6);
+  const int nwave = blockDim.x >> 6;
+  for (int k = tid; k <= P.n_rp; k += blockDim.x) lthr[k] = P.thr[k];
+
+  const Item it = P.items[blockIdx.x];
+  const Group bg = P.bgrp[it.group];
+  const int tgt = it.target;
+  unsigned long long ncand = 0;
+  for (int c = tid; c < ncell; c += blockDim.x) {
+    lnum[c] = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) lacc[s * ncell + c] = 0.0;
+  }
+  __syncthreads();
+
+  const int nchunk = (it.count + 63) >> 6;
+  for (int ch = wave; ch < nchunk; ch += nwave) {
+    const int base = uni(it.first + ch * 64);
+    const int cnt = uni(min(64, it.first + it.count - base));
+    const bool valid = lane < cnt;
+    const int p = base + (valid ? lane : 0);
+    const PRec me = brec[p];
+    const SRec ms = bsr[p];
+    Win mw{};
+    if (!SWAP) mw = bwin[p];
+    // search rectangle of the wave
+    double zlo, zhi, ralo, rahi, declo, dechi;
+    bool ra_all;
+    if (!SWAP) {
+      zlo = wave_min(valid ? mw.zmin : INFINITY);
+      zhi = wave_max(valid ? mw.zmax : -INFINITY);
+      declo = wave_min(valid ? mw.decmin : INFINITY);
+      dechi = wave_max(valid ? mw.decmax : -INFINITY);
+      const bool nan_ra = valid && !(mw.ramin <= mw.ramax);
+      ra_all = __ballot(nan_ra) != 0;
+      ralo = wave_min(valid ? mw.ramin : INFINITY);
+      rahi = wave_max(valid ? mw.ramax : -INFINITY);
+    } else {
+      zlo = wave_min(valid ? me.z : INFINITY) - P.reach_zup;
+      zhi = wave_max(valid ? me.z : -INFINITY) + P.reach_zdn;
+      declo = wave_min(valid ? me.y : INFINITY) - P.reach_dec;
+      dechi = wave_max(valid ? me.y : -INFINITY) + P.reach_dec;
+      ra_all = P.ra_full != 0;
+      ralo = wave_min(valid ? me.x : INFINITY) - P.reach_ra;
+      rahi = wave_max(valid ? me.x : -INFINITY) + P.reach_ra;
+    }
+    const double eps = 1e-9;
+    int cx0 = 0, cx1 = P.nra - 1;
+    if (!ra_all) {
+      cx0 = uni((int)fmax(0.0, fmin((double)(P.nra - 1), floor((ralo - eps - P.ra0) / P.hra))));
+      cx1 = uni((int)fmax(0.0, fmin((double)(P.nra - 1), floor((rahi + eps - P.ra0) / P.hra))));
+    }
+    const int cy0 = uni((int)fmax(0.0, fmin((double)(P.ndec - 1), floor((declo - eps - P.dec0) / P.hdec))));
+    const int cy1 = uni((int)fmax(0.0, fmin((double)(P.ndec - 1), floor((dechi + eps - P.dec0) / P.hdec))));
+    const double zsl = zlo - 1e-12, zsh = zhi + 1e-12;
+
+    int run_cell = -1;
+    unsigned run_n = 0;
+    double run[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) run[s] = 0.0;
+
+    for (int cy = cy0; cy <= cy1; ++cy) {
+      for (int cx = cx0; cx <= cx1; ++cx) {
+        const int col = cy * P.nra + cx;
+        const int g0 = P.tcol[col], g1 = P.tcol[col + 1];
+        for (int g = g0; g < g1; ++g) {
+          const Group tg = P.tgrp[g];
+          if (P.phase_mode == 2 && tg.region == bg.region) continue;
+          if (zsh < tg.zlo || zsl > tg.zhi) continue;
+          const int j0 = zsearch<false>(trec, tg.start, tg.end, zsl);
+          const int j1 = zsearch<true>(trec, j0, tg.end, zsh);
+          for (int j = j0; j < j1; ++j) {
+            const PRec t = trec[j];
+            const SRec ts = tsr[j];
+            // outer (o) and inner (i) roles of the pair
+            const PRec &po = SWAP ? t : me;
+            const PRec &pi = SWAP ? me : t;
+            const SRec &so = SWAP ? ts : ms;
+            const SRec &si = SWAP ? ms : ts;
+            Win w;
+            if (SWAP)
+              w = twin[j];
+            else
+              w = mw;
+            // do_pair_sorted's tests against the outer galaxy's window (PBz = PBy = PBx = 0)
+            bool ok = valid && !(pi.z > w.zmax || pi.z < w.zmin) && !(pi.y > w.decmax || pi.y < w.decmin) &&
+                      !(pi.x < w.ramin || pi.x > w.ramax);
+            ncand += ok;
+            int cell = -1;
+            if (ok) {
+              const double q2 = floor(((si.dc - so.dc) - P.pmin) / P.psize);  // pi_calc, corel_bin
+              if (q2 >= 0.0 && q2 < (double)P.n_pi) {
+                // dA_calc: sin d1 sin d2 + cos d1 cos d2 cos(a1 - a2), clamp, acos
+                const double sum = so.sd * si.sd + so.cd * si.cd * cos((so.ra_rad - si.ra_rad));
+                const double dA = (sum > 0.9999999999) ? acos(1.0) : acos(sum);
+                const double rp = dA * (so.da + si.da) / 2.;
+                const double one_m = fmax(1.0 - sum * sum, 1e-300);
+                const double dda = P.dsum / sqrt(one_m) + 1e-15 * dA;
+                const double band = 1.5 * (0.5 * (so.da + si.da)) * dda + 4e-16 * rp;
+                bool unc;
+                const int k = rp_bin_sky(P, lthr, rp, band, &unc);
+                unc = unc || fabs(sum - 0.9999999999) <= P.dsum;
+                if (unc) {
+                  if (!SWAP) {
+                    const unsigned long long slot = atomicAdd(&P.counters[3], 1ull);
+                    if (slot < (unsigned long long)P.hq_cap) P.hq[slot] = make_uint2(po.idx, pi.idx);
+                  }
+                } else if (k >= 0) {
+                  cell = k * P.n_pi + (int)q2;
+                }
+              }
+            }
+            if (cell >= 0) {
+              if (cell != run_cell) {
+                if (run_cell >= 0) lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
+                run_cell = cell;
+                run_n = 0;
+#pragma unroll
+                for (int s = 0; s < NS; ++s) run[s] = 0.0;
+              }
+              ++run_n;
+              const Gal go{po.x, po.y, po.w, po.e1, po.e2, po.phi};
+              const Gal gi{pi.x, pi.y, pi.w, pi.e1, pi.e2, pi.phi};
+              // angle_orient_axis_sky: atan((dec_i - dec_o) / ((ra_i - ra_o) cos((dec_o + dec_i)/2)))
+              const double X = (pi.x - po.x) * cos((so.dec_rad + si.dec_rad) / 2.0);
+              const double Y = pi.y - po.y;
+              double qv[NQ];
+              pair_values_v<K, SPLIT>(go, gi, P.gws, X, Y, qv);
+#pragma unroll
+              for (int s = 0; s < NS; ++s) run[s] += qv[s];
+            }
+          }
+        }
+      }
+    }
+    if (run_cell >= 0) lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
+  }
+  __syncthreads();
+  for (int c = tid; c < ncell; c += blockDim.x) {
+    const unsigned n = lnum[c];
+    if (n) {
+      atomicAdd(&P.accnum[(size_t)tgt * ncell + c], (unsigned long long)n);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) unsafeAtomicAdd(&P.acc[((size_t)tgt * NS + s) * ncell + c], lacc[s * ncell + c]);
+    }
+  }
+  if (ncand) atomicAdd(&P.counters[0], ncand);
+}
+
+__global__ void k_gather_sky(const int32_t *__restrict__ sidx, int64_t n, const SRec *__restrict__ src_sr,
+                             const Win *__restrict__ src_win, SRec *__restrict__ sr, Win *__restrict__ win) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int32_t i = sidx[p];
+  sr[p] = src_sr[i];
+  if (src_win) win[p] = src_win[i];
+}
+
+hipError_t launch_gather_sky(const int32_t *sidx, int64_t n, const SRec *src_sr, const Win *src_win, SRec *sr,
+                             Win *win, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  k_gather_sky<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(sidx, n, src_sr, src_win, sr, win);
+  return hipGetLastError();
+}
+
+template <int K, bool SPLIT, bool SWAP>
+static hipError_t launch_sky_t(const SkyArgs &P, int threads, size_t lds, const PRec *brec, const SRec *bsr,
+                               const Win *bwin, const PRec *trec, const SRec *tsr, const Win *twin,
+                               hipStream_t st) {
+  auto fn = sky_kernel<K, SPLIT, SWAP>;
+  hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  fn<<<P.n_items, threads, lds, st>>>(P, brec, bsr, bwin, trec, tsr, twin);
+  return hipGetLastError();
+}
+
+hipError_t launch_sky(int kind, bool split, bool swap, const SkyArgs &P, int threads, const PRec *brec,
+                      const SRec *bsr, const Win *bwin, const PRec *trec, const SRec *tsr, const Win *twin,
+                      hipStream_t st) {
+  if (P.n_items == 0) return hipSuccess;
+  const size_t lds = pair_lds_bytes(P.ncell, P.nslots, P.n_rp, threads / 64, 0);
+#define CPC_SKY(KK)                                                                                     \
+  if (kind == KK) {                                                                                     \
+    if (split && swap) return launch_sky_t<KK, true, true>(P, threads, lds, brec, bsr, bwin, trec, tsr, twin, st); \
+    if (split) return launch_sky_t<KK, true, false>(P, threads, lds, brec, bsr, bwin, trec, tsr, twin, st);        \
+    if (swap) return launch_sky_t<KK, false, true>(P, threads, lds, brec, bsr, bwin, trec, tsr, twin, st);         \
+    return launch_sky_t<KK, false, false>(P, threads, lds, brec, bsr, bwin, trec, tsr, twin, st);                  \
+  }
+  CPC_SKY(kDD)
+  CPC_SKY(kSD)
+  CPC_SKY(kSS0)
+  CPC_SKY(kSS1)
+#undef CPC_SKY
+  return hipErrorInvalidValue;
+}
+
+}  // namespace cpc

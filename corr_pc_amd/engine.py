@@ -66,7 +66,7 @@ class Stats(ctypes.Structure):
     _fields_ = [("n_binned", c_int64), ("n_candidates", c_int64), ("n_fp64_reeval", c_int64),
                 ("n_sparse_windows", c_int64), ("ms_total", c_double), ("ms_kernel", c_double),
                 ("ms_primary", c_double), ("ms_touch", c_double), ("ms_build", c_double),
-                ("n_items_primary", c_int64)]
+                ("n_items_primary", c_int64), ("n_host_pairs", c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -135,6 +135,16 @@ def make_params(box_size, n_rp, rp_min, rp_max, n_pi, pi_min, pi_max, kind=KIND_
         which_corr = {KIND_DD: 0, KIND_SHAPE_DENSITY: 1, KIND_SHAPE_SHAPE: 2}[kind]
     return Params(GEOM_BOX, box_size, kind, g_with_shape, estimator, which_corr, n_rp, rp_min, rp_max, lin_bin,
                   n_pi, pi_min, pi_max, do_jk, n_jk, include_prob_max, precision, jk_mode)
+
+
+def make_sky_params(table, n_rp, rp_min, rp_max, n_pi, pi_min, pi_max, **kw):
+    """Sky geometry (coordinates 0: RA, Dec in degrees and redshift): `table` is
+    (H, dA, dz, z_max, z_sep_max) as the reference's data_ini derives them (initialization.cpp:223,
+    272-276); the remaining keywords are those of make_params."""
+    p = make_params(0.0, n_rp, rp_min, rp_max, n_pi, pi_min, pi_max, **kw)
+    p.geometry = GEOM_SKY
+    p.attach_table(*table)
+    return p
 
 
 class Histogram:

@@ -127,6 +127,7 @@ typedef struct cpc_stats {
   double ms_touch;          /* device time of the jackknife touch kernel (block side = inner catalog) */
   double ms_build;          /* wall time of the cell-list construction */
   int64_t n_items_primary;  /* work items (workgroups) of the primary kernel */
+  int64_t n_host_pairs;     /* sky: guard-band pairs binned on the host with the host libm */
 } cpc_stats;
 
 typedef struct cpc_ctx cpc_ctx;

@@ -29,6 +29,46 @@ def _rot(e1, e2, th):
     return (math.cos(2.0 * th) * e1 - math.sin(2.0 * th) * e2, math.sin(2.0 * th) * e1 + math.cos(2.0 * th) * e2)
 
 
+def _kind_fields(p, outer, i, inner, j, w, th):
+    """The per-pair fields of the shape kinds (corels.cpp:171-244) for position angle th."""
+    def g(c, name, k, default):
+        a = getattr(c, name)
+        return default if a is None else float(a[k])
+    v = [0.0] * NF
+    e = [0.0] * NF
+    if p.kind == 1:
+        src = (inner, j) if p.g_with_shape == 2 else (outer, i)
+        ep, ex = _rot(float(src[0].e1[src[1]]), float(src[0].e2[src[1]]), th)
+        v[1], e[1] = w * ep, (w * ep) ** 2
+        v[2], e[2] = w * ex, (w * ex) ** 2
+        v[6] = _ephi(g(outer, "phi", i, -10.0), th)
+        e[6] = v[6] ** 2
+        return v, e
+    epi, exi = _rot(float(outer.e1[i]), float(outer.e2[i]), th)
+    epj, exj = _rot(float(inner.e1[j]), float(inner.e2[j]), th)
+    if p.estimator == 0:
+        if p.g_with_shape == 2:
+            v[1], e[1] = w * epj, (w * epj) ** 2
+            v[6] = _ephi(g(inner, "phi", j, -10.0), th)
+        else:
+            v[1], e[1] = w * epi, (w * epi) ** 2
+            v[6] = _ephi(g(outer, "phi", i, -10.0), th)
+        e[6] = v[6] ** 2
+    else:
+        v[1] = w * (epi + epj) / 2.0
+        e[1] = (w * epi / 2.0) ** 2 + (w * epj / 2.0) ** 2
+        v[2] = w * (exi + exj) / 2.0
+        e[2] = (w * exi / 2.0) ** 2 + (w * exj / 2.0) ** 2
+        a = _ephi(g(outer, "phi", i, -10.0), th)
+        b = _ephi(g(inner, "phi", j, -10.0), th)
+        v[6], e[6] = a + b, a * a + b * b
+    v[3], e[3] = w * (epi * epj), (w * (epi * epj)) ** 2
+    v[4], e[4] = w * (exi * exj), (w * (exi * exj)) ** 2
+    v[5] = w * (exi * epj + epi * exj) / 2.0
+    e[5] = (w * (exi * epj) / 2.0) ** 2 + (w * epi * exj / 2.0) ** 2
+    return v, e
+
+
 def images(zi, xi, yi, L, pmin, pmax, rmax):
     """Image shifts (PBx, PBy, PBz) visited for an outer galaxy: 0 always, -1 if the window starts
     below 0, +1 if it ends above L."""
@@ -100,41 +140,11 @@ def count(outer, inner, p):
                 if not (0 <= q1 < p.n_rp):
                     continue
                 w = g(outer, "w", i, 1.0) * g(inner, "w", j, 1.0)
-                v = [0.0] * NF
-                e = [0.0] * NF
+                v, e = [0.0] * NF, [0.0] * NF
                 if p.kind != 0:
                     dx, dy = xj - xi, yj - yi
                     th = math.atan(dy / dx) if dx != 0 else math.copysign(math.pi / 2, dy)
-                    if p.kind == 1:
-                        src = (inner, j) if p.g_with_shape == 2 else (outer, i)
-                        ep, ex = _rot(float(src[0].e1[src[1]]), float(src[0].e2[src[1]]), th)
-                        v[1], e[1] = w * ep, (w * ep) ** 2
-                        v[2], e[2] = w * ex, (w * ex) ** 2
-                        v[6] = _ephi(g(outer, "phi", i, -10.0), th)
-                        e[6] = v[6] ** 2
-                    else:
-                        epi, exi = _rot(float(outer.e1[i]), float(outer.e2[i]), th)
-                        epj, exj = _rot(float(inner.e1[j]), float(inner.e2[j]), th)
-                        if p.estimator == 0:
-                            if p.g_with_shape == 2:
-                                v[1], e[1] = w * epj, (w * epj) ** 2
-                                v[6] = _ephi(g(inner, "phi", j, -10.0), th)
-                            else:
-                                v[1], e[1] = w * epi, (w * epi) ** 2
-                                v[6] = _ephi(g(outer, "phi", i, -10.0), th)
-                            e[6] = v[6] ** 2
-                        else:
-                            v[1] = w * (epi + epj) / 2.0
-                            e[1] = (w * epi / 2.0) ** 2 + (w * epj / 2.0) ** 2
-                            v[2] = w * (exi + exj) / 2.0
-                            e[2] = (w * exi / 2.0) ** 2 + (w * exj / 2.0) ** 2
-                            a = _ephi(g(outer, "phi", i, -10.0), th)
-                            b = _ephi(g(inner, "phi", j, -10.0), th)
-                            v[6], e[6] = a + b, a * a + b * b
-                        v[3], e[3] = w * (epi * epj), (w * (epi * epj)) ** 2
-                        v[4], e[4] = w * (exi * exj), (w * (exi * exj)) ** 2
-                        v[5] = w * (exi * epj + epi * exj) / 2.0
-                        e[5] = (w * (exi * epj) / 2.0) ** 2 + (w * epi * exj / 2.0) ** 2
+                    v, e = _kind_fields(p, outer, i, inner, j, w, th)
                 num[0, q1, q2] += 1
                 wt[0, q1, q2] += w
                 val[0, q1, q2] += v
@@ -221,23 +231,22 @@ def count_sky(outer, inner, p):
             da = math.acos(1.0) if s > 0.9999999999 else math.acos(s)
             rp = da * (float(outer.da[i]) + float(inner.da[j])) / 2.0
             q2 = math.floor((dP - pmin) / psize)
-            if not (0 <= q2 < p.n_pi) or rp <= 0:
+            if not (0 <= q2 < p.n_pi):
                 continue
-            q1 = math.floor((rp - rmin) / rsize) if p.lin_bin else math.floor(math.log10(rp / rmin) / rsize)
+            if p.lin_bin:
+                q1 = math.floor((rp - rmin) / rsize)
+            else:
+                if rp == 0.0:
+                    continue
+                q1 = math.floor(math.log10(rp / rmin) / rsize)
             if not (0 <= q1 < p.n_rp):
                 continue
             w = g(outer, "w", i, 1.0) * g(inner, "w", j, 1.0)
-            v = [0.0] * NF
-            e = [0.0] * NF
-            if p.kind == 1:
+            v, e = [0.0] * NF, [0.0] * NF
+            if p.kind != 0:
                 den = (raj - rai) * math.cos((d1 + d2) / 2.0)
                 th = math.atan((decj - deci) / den) if den != 0 else math.copysign(math.pi / 2, decj - deci)
-                src = (inner, j) if p.g_with_shape == 2 else (outer, i)
-                ep, ex = _rot(float(src[0].e1[src[1]]), float(src[0].e2[src[1]]), th)
-                v[1], e[1] = w * ep, (w * ep) ** 2
-                v[2], e[2] = w * ex, (w * ex) ** 2
-                v[6] = _ephi(g(outer, "phi", i, -10.0), th)
-                e[6] = v[6] ** 2
+                v, e = _kind_fields(p, outer, i, inner, j, w, th)
             num[0, q1, q2] += 1
             wt[0, q1, q2] += w
             val[0, q1, q2] += v

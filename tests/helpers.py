@@ -57,3 +57,24 @@ def assert_hist_match(got, ref, rtol=RTOL, what=""):
     if not np.all(de <= rtol * scale_e + 1e-300):
         idx = np.unravel_index(np.argmax(de / (scale_e + 1e-300)), de.shape)
         raise AssertionError(f"{what}: err differs at {idx}: {ge[idx]!r} vs {re_[idx]!r}")
+
+
+def sky_catalogs(n_out, n_in, seed, shapes_in=False, coins=True, inc=False, table=None):
+    """Sky survey subsamples (six 5x5 degree fields, 0.2 < z < 1): data as the outer catalog, randoms
+    as the inner one.  Returns (outer, inner, n_jk, table).  Coins (jk_prob) are drawn uniformly so
+    that both jackknife slot rules are exercised; the reader itself would set jk_prob = 1."""
+    from corr_pc_amd import synth
+    A, n_jk, table = synth.sky_catalog(2, "data", n=n_out, table=table)
+    B, _, _ = synth.sky_catalog(2, "rand", n=n_in, table=table)
+    r = np.random.default_rng(seed)
+    if coins:
+        A.jk_prob = r.uniform(0, 1, A.n)
+        B.jk_prob = r.uniform(0, 1, B.n)
+    if inc:
+        A.include_prob = r.uniform(0, 1, A.n)
+        B.include_prob = r.uniform(0, 1, B.n)
+    if shapes_in:
+        B.e1 = r.normal(0, 0.3, B.n)
+        B.e2 = r.normal(0, 0.3, B.n)
+        B.phi = r.uniform(-1.5, 1.5, B.n)
+    return A, B, n_jk, table
