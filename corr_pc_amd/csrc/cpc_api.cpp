@@ -64,7 +64,8 @@ struct Grid {
 struct Side {
   int64_t n_incl = 0;
   DevBuf<PRec> rec;
-  DevBuf<FRec> fr;
+  DevBuf<double> zs;    // z in record order
+  DevBuf<int32_t> idx;  // catalog index in record order
   DevBuf<SRec> sr;
   DevBuf<Win> win;
   DevBuf<Group> d_groups;
@@ -74,13 +75,16 @@ struct Side {
   std::vector<int64_t> colcount;  // galaxies per cell
   void release() {
     rec.release();
-    fr.release();
+    zs.release();
+    idx.release();
     sr.release();
     win.release();
     d_groups.release();
     d_colbeg.release();
   }
 };
+
+SkySide sky_side(const Side &S) { return SkySide{S.rec.p, S.zs.p, S.idx.p, S.sr.p, S.win.p}; }
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -162,6 +166,60 @@ float round_down_f(double v) {
   float f = (float)v;
   if ((double)f > v) f = std::nextafter(f, -INFINITY);
   return f;
+}
+
+// Guard band of the split-precision rp classification.  The kernel computes, in FP32, s32 (the
+// squared separation from column-local coordinates, |s32 - s| <= E(s) with
+// E(s) = 2 (2 sqrt2 eps r + 2 eps^2 + s 2^-21), r = sqrt(s), eps the bound on the FP32 error of dx
+// and dy) and the bin index estimate kf (log2 of s32 through v_log_f32, or sqrt of s32 through
+// v_rsq_f32).  A pair is binned in FP32 only when the fractional part of kf is at least g from 0
+// and 1, g = gA + (gB + gC/r)/r + gD r bounding |kf - f(s)| with f the real-valued bin index whose
+// integer crossings are the FP64 thresholds; other pairs (and s32 below s_cert_lo, where the
+// bound's linearisation does not hold) go to the FP64 path.
+struct Guard {
+  float s_out_lo, s_out_hi, s_cert_lo, gA, gB, gC, gD;
+};
+
+Guard rp_guard(const cpc_params &p, const Derived &dv, const std::vector<double> &thr, double eps) {
+  Guard g{};
+  auto E = [&](double s) { return 2.0 * (2 * std::sqrt(2.0) * std::sqrt(s) * eps + 2 * eps * eps + std::ldexp(s, -21)); };
+  g.s_out_lo = round_down_f(thr[0] - E(thr[0]));
+  g.s_out_hi = round_up_f(thr[p.n_rp] + E(thr[p.n_rp]));
+  // below r = 300 eps the relative error E(s)/s exceeds 3%: never certain there
+  g.s_cert_lo = round_up_f(std::pow(300.0 * eps, 2));
+  const double t24 = std::ldexp(1.0, -24), t21 = std::ldexp(1.0, -21), t20 = std::ldexp(1.0, -20);
+  const double margin = 2.0;  // s32 in place of s in the bound, and slack
+  const double shi = std::max((double)g.s_out_hi, 1e-30);
+  double a, b, c, d;
+  if (p.lin_bin) {
+    // f(s) = (sqrt(s) - rmin)/rsize:  |f(s32) - f(s)| <= E(s)/(r rsize)
+    const double k = margin / dv.rsize;
+    a = k * 2 * 2 * std::sqrt(2.0) * eps;
+    b = k * 2 * 2 * eps * eps;
+    c = 0.0;
+    d = k * 2 * t21;
+    // evaluation of kf: sqrt through rsq (2^-21 relative), subtraction, product, constants
+    const double rhi = std::sqrt(shi);
+    a += margin * ((2 * rhi + std::fabs(p.rp_min)) * t20 / dv.rsize + (p.n_rp + 2) * t24 * 4);
+  } else {
+    // f(s) = 0.5 log10(s)/rsize - log10(rmin)/rsize:  |f(s32) - f(s)| <= 0.2172/rsize * 1.03 E(s)/s
+    const double k = margin * 0.5 * std::log10(std::exp(1.0)) * 1.03 / dv.rsize;
+    a = k * 2 * t21;
+    b = k * 2 * 2 * std::sqrt(2.0) * eps;
+    c = k * 2 * 2 * eps * eps;
+    d = 0.0;
+    // evaluation of kf = log2(s32) la + lb: v_log_f32 error (taken as 2^-21 max(1, |log2 s|)),
+    // rounding of la, lb, the product and the sum
+    const double la = 0.5 * std::log10(2.0) / dv.rsize, lb = -std::log10(p.rp_min) / dv.rsize;
+    const double l2 = std::max({1.0, std::fabs(std::log2(std::max((double)g.s_out_lo, 1e-30))), std::fabs(std::log2(shi))});
+    a += margin * (la * l2 * (t21 + 2 * t24) + std::fabs(lb) * t24 + (p.n_rp + 2) * t24 * 2);
+  }
+  a += 1e-12 * (p.n_rp + 1);  // the reference's own rounding of its bin index
+  g.gA = round_up_f(a);
+  g.gB = round_up_f(b);
+  g.gC = round_up_f(c);
+  g.gD = round_up_f(d);
+  return g;
 }
 
 hipError_t check(hipError_t e, Status &st, const char *what) {
@@ -248,13 +306,12 @@ struct cpc_ctx {
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   double col_frac = 0.0;
-  int item_size = 0, block_threads = 0, acc_mode = -1;
+  int item_size = 0, block_threads = 0;
   // scratch
   DevBuf<uint32_t> keys_a, keys_b;
   DevBuf<int32_t> vals_a, vals_b, order;
   DevBuf<double> zk_b;
   DevBuf<double> thr;
-  DevBuf<float> thr_lo, thr_hi;
   DevBuf<int2> nbr;
   DevBuf<Item> items;
   DevBuf<double> accp, acct;
@@ -303,7 +360,8 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, b
           st, "copy");
     check(S.d_groups.ensure(1), st, "alloc");
     check(S.rec.ensure(kRecPad), st, "alloc");
-    check(S.fr.ensure(kRecPad), st, "alloc");
+    check(S.zs.ensure(kRecPad), st, "alloc");
+    check(S.idx.ensure(kRecPad), st, "alloc");
     check(S.sr.ensure(1), st, "alloc");
     check(S.win.ensure(1), st, "alloc");
     check(hipStreamSynchronize(s), st, "sync");
@@ -363,12 +421,13 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, b
     }
     S.colbeg[ncells] = (int)S.groups.size();
   }
-  if (check(S.rec.ensure(ni + kRecPad), st, "alloc") || check(S.fr.ensure(ni + kRecPad), st, "alloc") ||
+  if (check(S.rec.ensure(ni + kRecPad), st, "alloc") || check(S.zs.ensure(ni + kRecPad), st, "alloc") ||
+      check(S.idx.ensure(ni + kRecPad), st, "alloc") ||
       check(S.d_groups.ensure(S.groups.size()), st, "alloc") || check(S.d_colbeg.ensure(S.colbeg.size()), st, "alloc"))
     return st;
   GatherArgs ga{c->d_x, c->d_y, c->d_z, c->d_w, c->d_e1, c->d_e2, c->d_phi, ni, G.ncx, G.ncy, G.x0, G.y0, G.hx, G.hy,
                 p.box_size, p.pi_min, p.pi_max, p.rp_max, zfirst_plus_L, has_zfirst ? 1 : 0};
-  check(launch_gather(ga, ctx->vals_b.p, S.rec.p, S.fr.p, s), st, "gather");
+  check(launch_gather(ga, ctx->vals_b.p, S.rec.p, S.zs.p, S.idx.p, s), st, "gather");
   if (sky) {
     if (check(S.sr.ensure(ni), st, "alloc") || check(S.win.ensure(ni), st, "alloc")) return st;
     check(launch_gather_sky(ctx->vals_b.p, ni, c->d_sr, win_raw, S.sr.p, S.win.p, s), st, "gather sky");
@@ -562,8 +621,6 @@ void cpc_destroy(cpc_ctx *ctx) {
   ctx->order.release();
   ctx->zk_b.release();
   ctx->thr.release();
-  ctx->thr_lo.release();
-  ctx->thr_hi.release();
   ctx->nbr.release();
   ctx->items.release();
   ctx->accp.release();
@@ -583,16 +640,14 @@ void cpc_destroy(cpc_ctx *ctx) {
   delete ctx;
 }
 
-cpc_status cpc_set_tuning(cpc_ctx *ctx, double col_frac, int item_size, int block_threads, int acc_mode) {
+cpc_status cpc_set_tuning(cpc_ctx *ctx, double col_frac, int item_size, int block_threads) {
   if (!ctx) return CPC_ERR_INVALID;
   if (col_frac < 0 || item_size < 0 || block_threads < 0) return fail(ctx, CPC_ERR_INVALID, "negative tuning value");
-  if (block_threads && (block_threads % 64 || block_threads > 256))
-    return fail(ctx, CPC_ERR_INVALID, "block_threads must be a multiple of 64 and at most 256");
-  if (acc_mode < -1 || acc_mode > 1) return fail(ctx, CPC_ERR_INVALID, "acc_mode must be -1 (automatic), 0 or 1");
+  if (block_threads && (block_threads % 64 || block_threads > 512))
+    return fail(ctx, CPC_ERR_INVALID, "block_threads must be a multiple of 64 and at most 512");
   ctx->col_frac = col_frac;
   ctx->item_size = item_size;
   ctx->block_threads = block_threads;
-  ctx->acc_mode = acc_mode;
   return CPC_OK;
 }
 
@@ -803,11 +858,11 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
   const int NV = kNV[kind_code];
   const int NSP = 1 + 2 * NV, NST = 1 + NV;
   const bool split = p.precision == CPC_PREC_SPLIT;
-  const int threads = ctx->block_threads ? ctx->block_threads : 256;
-  int acc_mode = ctx->acc_mode >= 0 ? ctx->acc_mode : 0;
-  if (sky || ncell > 65535) acc_mode = 0;  // the compaction ring packs the cell index in 16 bits
-  if (pair_lds_bytes(ncell, NSP, p.n_rp, threads / 64, acc_mode) > ctx->max_lds) acc_mode = 0;
-  const size_t lds_p = pair_lds_bytes(ncell, NSP, p.n_rp, threads / 64, acc_mode);
+  // box kernel: up to 512 threads (the split kernel keeps 4 waves per SIMD with 8 waves sharing a
+  // histogram; the FP64 kernel needs more registers and runs 4 waves per group); sky kernel: up to 256
+  const int threads = sky ? std::min(ctx->block_threads ? ctx->block_threads : 256, 256)
+                          : (ctx->block_threads ? ctx->block_threads : (split ? 512 : 256));
+  const size_t lds_p = pair_lds_bytes(ncell, NSP, p.n_rp);
   if (lds_p > ctx->max_lds)
     return fail(ctx, CPC_ERR_UNSUPPORTED,
                 "histogram of " + std::to_string(ncell) + " cells does not fit in LDS (" + std::to_string(lds_p) +
@@ -954,38 +1009,30 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
 
   // ---- device tables and accumulators
   const size_t nP = (size_t)2 * nreg, nT = (size_t)nreg;
-  if (check(ctx->thr.ensure(p.n_rp + 1), st, "alloc") || check(ctx->thr_lo.ensure(p.n_rp + 1), st, "alloc") ||
-      check(ctx->thr_hi.ensure(p.n_rp + 1), st, "alloc") ||
+  if (check(ctx->thr.ensure(p.n_rp + 1), st, "alloc") ||
       check(ctx->nbr.ensure(std::max<size_t>(nbr.size(), 1)), st, "alloc") ||
       check(ctx->items.ensure(items_p.size() + items_t.size()), st, "alloc") ||
       check(ctx->accp.ensure(nP * NSP * ncell), st, "alloc") || check(ctx->accpn.ensure(nP * ncell), st, "alloc") ||
       check(ctx->acct.ensure(nT * NST * ncell), st, "alloc") || check(ctx->acctn.ensure(nT * ncell), st, "alloc") ||
       check(ctx->counters.ensure(4), st, "alloc"))
     return fail(ctx, st);
-  std::vector<float> tlo(p.n_rp + 1, 0.0f), thi(p.n_rp + 1, 0.0f);
   const double two24 = std::ldexp(1.0, -24);
   double gz = 1.0;
+  Guard gr{};
   if (!sky) {
-    // guard bands of the FP32 classification: bounds on the FP32 error of dx, dy (column-local
+    // guard bands of the FP32 classification from bounds on the FP32 error of dx, dy (column-local
     // coordinates plus a rounded column offset) and of dz (absolute coordinates)
     const double L = p.box_size, h = G.hx;
     const double amax = 1.01 * h + 1e-9 * L;
     const double omax = (Rn + 1) * h + 1e-9 * L;
     const double eps_xy = 2.0 * (6 * amax + 2 * omax) * two24;
-    for (int k = 0; k <= p.n_rp; ++k) {
-      const double T = thr[k];
-      const double d = 2.0 * (2 * std::sqrt(2.0) * std::sqrt(T) * eps_xy + 2 * eps_xy * eps_xy + std::ldexp(T, -21));
-      tlo[k] = round_up_f(T + d);
-      thi[k] = round_down_f(T - d);
-    }
+    gr = rp_guard(p, dv, thr, eps_xy);
     const double eps_z = 2.0 * 5 * L * two24;
     const double pabs = std::max(std::fabs(p.pi_min), std::fabs(p.pi_max));
     gz = eps_z / dv.psize + 8.0 * (p.n_pi + 2 * pabs / dv.psize + 2) * two24;
   }
   const bool use_split = split && (sky || gz < 0.25);
   check(hipMemcpyAsync(ctx->thr.p, thr.data(), thr.size() * 8, hipMemcpyHostToDevice, s), st, "copy");
-  check(hipMemcpyAsync(ctx->thr_lo.p, tlo.data(), tlo.size() * 4, hipMemcpyHostToDevice, s), st, "copy");
-  check(hipMemcpyAsync(ctx->thr_hi.p, thi.data(), thi.size() * 4, hipMemcpyHostToDevice, s), st, "copy");
   if (!nbr.empty())
     check(hipMemcpyAsync(ctx->nbr.p, nbr.data(), nbr.size() * sizeof(int2), hipMemcpyHostToDevice, s), st, "copy");
   if (!items_p.empty())
@@ -1036,11 +1083,15 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
       K.lrmin = (float)p.rp_min;
       K.linv = (float)(1.0 / dv.rsize);
       K.thr = ctx->thr.p;
-      K.thr_lo = ctx->thr_lo.p;
-      K.thr_hi = ctx->thr_hi.p;
+      K.s_out_lo = gr.s_out_lo;
+      K.s_out_hi = gr.s_out_hi;
+      K.s_cert_lo = gr.s_cert_lo;
+      K.gA = gr.gA;
+      K.gB = gr.gB;
+      K.gC = gr.gC;
+      K.gD = gr.gD;
       K.nbr = ctx->nbr.p;
       K.counters = ctx->counters.p;
-      K.acc_mode = acc_mode;
       if (!items_p.empty()) {
         KArgs A = K;
         A.n_items = (int)items_p.size();
@@ -1052,7 +1103,7 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
         A.tcol = SI.d_colbeg.p;
         A.acc = ctx->accp.p;
         A.accnum = ctx->accpn.p;
-        check(launch_pair(kind_code, use_split, false, A, threads, SO.rec.p, SO.fr.p, SI.rec.p, SI.fr.p, s), st,
+        check(launch_pair(kind_code, use_split, false, A, threads, SO.rec.p, SO.zs.p, SI.rec.p, SI.zs.p, s), st,
               "pair kernel");
       }
       check(hipEventRecord(ctx->ev2, s), st, "event");
@@ -1067,7 +1118,7 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
         A.tcol = SO.d_colbeg.p;
         A.acc = ctx->acct.p;
         A.accnum = ctx->acctn.p;
-        check(launch_pair(kind_code, use_split, true, A, threads, SI.rec.p, SI.fr.p, SO.rec.p, SO.fr.p, s), st,
+        check(launch_pair(kind_code, use_split, true, A, threads, SI.rec.p, SI.zs.p, SO.rec.p, SO.zs.p, s), st,
               "touch kernel");
       }
     } else {
@@ -1102,8 +1153,7 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
         B.tcol = SI.d_colbeg.p;
         B.acc = ctx->accp.p;
         B.accnum = ctx->accpn.p;
-        check(launch_sky(kind_code, use_split, false, B, threads, SO.rec.p, SO.sr.p, SO.win.p, SI.rec.p, SI.sr.p,
-                         SI.win.p, s),
+        check(launch_sky(kind_code, use_split, false, B, threads, sky_side(SO), sky_side(SI), s),
               st, "sky pair kernel");
       }
       check(hipEventRecord(ctx->ev2, s), st, "event");
@@ -1118,8 +1168,7 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
         B.tcol = SO.d_colbeg.p;
         B.acc = ctx->acct.p;
         B.accnum = ctx->acctn.p;
-        check(launch_sky(kind_code, use_split, true, B, threads, SI.rec.p, SI.sr.p, SI.win.p, SO.rec.p, SO.sr.p,
-                         SO.win.p, s),
+        check(launch_sky(kind_code, use_split, true, B, threads, sky_side(SI), sky_side(SO), s),
               st, "sky touch kernel");
       }
     }

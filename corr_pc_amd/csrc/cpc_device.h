@@ -34,6 +34,38 @@ struct Gal {
   double x, y, w, e1, e2, phi;
 };
 
+// atan(y/x) to about 2 ulp, in registers only.  With a = min(|x|,|y|), b = max, the ratio r = a/b
+// is reduced about the nearest quarter c = k/4 (k from an FP32 estimate):
+//   atan(r) = atan(c) + atan(d),  d = (a - c b)/(b + c a),  |d| <= 0.126,
+// atan(d) is its odd Taylor series to d^17 (truncation below 1e-18 relative) and atan(k/4) is one of
+// five correctly rounded constants.  |y| > |x| uses pi/2 - atan(x/y); the sign is that of the IEEE
+// quotient y/x; x = y = 0 gives NaN like atan(0/0).
+__device__ __forceinline__ double atan_ratio(double y, double x) {
+  const double ax = fabs(x), ay = fabs(y);
+  const bool sw = ay > ax;
+  const double a = sw ? ax : ay, b = sw ? ay : ax;
+  const float rf = (float)a * __builtin_amdgcn_rcpf((float)b);
+  const int k = (int)rintf(fminf(fmaxf(rf * 4.0f, 0.0f), 4.0f));
+  const double c = (double)k * 0.25;
+  const double d = fma(-c, b, a) * rcp_refined(fma(c, a, b));
+  const double d2 = d * d;
+  double p = fma(d2, 1.0 / 17.0, -1.0 / 15.0);
+  p = fma(d2, p, 1.0 / 13.0);
+  p = fma(d2, p, -1.0 / 11.0);
+  p = fma(d2, p, 1.0 / 9.0);
+  p = fma(d2, p, -1.0 / 7.0);
+  p = fma(d2, p, 1.0 / 5.0);
+  p = fma(d2, p, -1.0 / 3.0);
+  const double base = k == 0 ? 0.0
+                      : k == 1 ? 0x1.f5b75f92c80ddp-3   // atan(1/4)
+                      : k == 2 ? 0x1.dac670561bb4fp-2   // atan(1/2)
+                      : k == 3 ? 0x1.4978fa3269ee1p-1   // atan(3/4)
+                               : 0x1.921fb54442d18p-1;  // atan(1)
+  double t = base + fma(d * d2, p, d);
+  if (sw) t = M_PI_2 - t;
+  return (__builtin_signbit(x) != __builtin_signbit(y)) ? -t : t;
+}
+
 // ephi_calc, calcs.cpp:97-105 (with absolute() of calcs.cpp:50-53)
 __device__ __forceinline__ double ephi(double phi, double theta) {
   double t = phi - theta;
@@ -57,9 +89,9 @@ __device__ __forceinline__ void pair_values_v(const Gal &g1, const Gal &g2, int 
   } else {
     double theta, c2, s2;
     if constexpr (SPLIT) {
-      // theta = atan(dy/dx) with a refined reciprocal; cos 2theta and sin 2theta algebraically
-      // (about one ulp each, far inside the weighted-sum tolerance)
-      theta = atan((dxr != 0.0) ? dyr * rcp_refined(dxr) : copysign(INFINITY, dyr));
+      // theta = atan(dy/dx) by argument reduction; cos 2theta and sin 2theta algebraically (about
+      // one ulp each, far inside the weighted-sum tolerance)
+      theta = atan_ratio(dyr, dxr);
       const double xx = dxr * dxr, yy = dyr * dyr;
       const double inv = rcp_refined(xx + yy);
       c2 = (xx - yy) * inv;
@@ -178,10 +210,10 @@ __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlan
 
 // first index in [lo, hi) of the z-sorted records with z >= v (upper=false) or z > v (upper=true)
 template <bool UPPER>
-__device__ __forceinline__ int zsearch(const PRec *__restrict__ rec, int lo, int hi, double v) {
+__device__ __forceinline__ int zsearch(const double *__restrict__ zs, int lo, int hi, double v) {
   while (lo < hi) {
     const int mid = uni(lo + ((hi - lo) >> 1));
-    const double z = rec[mid].z;
+    const double z = zs[mid];
     const bool go_right = UPPER ? (z <= v) : (z < v);
     if (go_right)
       lo = mid + 1;
@@ -199,10 +231,6 @@ __device__ __forceinline__ void lds_add(unsigned *lnum, double *lacc, int ncell,
   for (int s = 0; s < NS; ++s) atomicAdd(&lacc[s * ncell + cell], v[s]);
 }
 
-struct F4 {
-  FRec r[4];
-};
 
-// ACC: 0 = run-length privatisation in registers, 1 = per-wave compaction of binned pairs.
 
 }  // namespace cpc

@@ -7,28 +7,25 @@
 
 namespace cpc {
 
-// A galaxy as the pair kernel reads it: one 64-byte record, so that a wave fetches a whole inner
-// galaxy with a single scalar load.  Gathered into (column, region, coin, z) order per pass.
+// A galaxy as the pair kernels read it: one 64-byte record whose 16-byte quarters are {x, y},
+// {w, e1}, {e2, phi} and {xl, yl, zf, meta}, so that four lanes fetch a galaxy with one coalesced
+// load each.  Gathered into (cell, region, coin, z) order per pass; z and the catalog index live in
+// side arrays of the same order.
 struct alignas(64) PRec {
-  double x, y;         // gal.ra.val_deg, gal.dec.val_deg
+  double x, y;         // gal.ra.val_deg, gal.dec.val_deg (box: x, y)
   double w;            // gal.wt
-  double e1, e2;       // gal.e[0], gal.e[1] (0 when the catalog has no shapes)
+  double e1;           // gal.e[0] (0 when the catalog has no shapes)
+  double e2;           // gal.e[1]
   double phi;          // gal.phi
-  double z;            // gal.redshift
-  int32_t meta;        // image flags of this galaxy as an outer galaxy (kMeta*)
-  int32_t idx;         // index in the uploaded catalog
+  float xl, yl;        // box: x, y relative to the galaxy's column corner (FP32)
+  float zf;            // box: z (FP32)
+  int32_t meta;        // box: image flags of this galaxy as an outer galaxy (kMeta*)
 };
+static_assert(sizeof(PRec) == 64, "PRec is four 16-byte quarters");
 
-// FP32 companion of PRec for the split-precision kernel: x, y relative to the galaxy's column
-// corner, z absolute, and the image flags.
-struct alignas(16) FRec {
-  float xl, yl, zf;
-  int32_t meta;
-};
-
-// Sorted record arrays carry this many padding records so that kernels may load FP32 records four
-// at a time past the end of a range.
-constexpr int kRecPad = 4;
+// Sorted record arrays carry this many padding records so that kernels may load a whole batch of
+// records past the end of a range.
+constexpr int kRecPad = 16;
 
 // Image flags of an outer galaxy i, computed exactly as lim_calc_PB (PB_calcs.cpp:30-61) decides
 // which periodic images of the inner catalog it visits (corr.cpp:380-423).
@@ -69,15 +66,15 @@ struct KArgs {
   int phase_mode;      // 0 primary pass without jackknife, 1 primary with jackknife, 2 touch pass
   int nslots;          // f64 slots accumulated by this pass
   int lin;
-  int acc_mode;        // 0 run-length privatisation, 1 per-wave compaction of binned pairs
   // split-precision classification
   float pmin32, inv_psize32, gz;   // pi bin: u = (dz - pmin)*inv_psize, guard gz in bin units
   float la, lb;                    // rp bin estimate from log2(s) (log bins) ...
   float lrmin, linv;               // ... or from sqrt(s) (linear bins)
   // tables (device)
   const double *thr;   // n_rp+1 FP64 thresholds on s
-  const float *thr_lo; // n_rp+1: thr[k] + guard, rounded up
-  const float *thr_hi; // n_rp+1: thr[k] - guard, rounded down
+  float s_out_lo, s_out_hi;  // FP32 s below s_out_lo or from s_out_hi on is outside every rp bin
+  float s_cert_lo;           // FP32 s below this is never classified in FP32
+  float gA, gB, gC, gD;      // guard band of the rp bin index: g = gA + (gB + gC/r)/r + gD r, r = sqrt(s)
   const int2 *nbr;     // neighbour column offsets
   const Item *items;
   const Group *bgrp;   // groups of the block side
@@ -160,18 +157,26 @@ struct SparseArgs {
 
 // Launch helpers implemented in cpc_kernels.hip.
 hipError_t launch_keys(const KeyArgs &A, const int32_t *order, uint32_t *keys, int32_t *vals, hipStream_t st);
-hipError_t launch_gather(const GatherArgs &A, const int32_t *sidx, PRec *rec, FRec *fr, hipStream_t st);
+hipError_t launch_gather(const GatherArgs &A, const int32_t *sidx, PRec *rec, double *zs, int32_t *idx,
+                         hipStream_t st);
 hipError_t launch_sparse(const SparseArgs &A, unsigned long long *count, hipStream_t st);
 hipError_t sort_u32(void *&tmp, size_t &tmp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
                     int32_t *vout, int64_t n, hipStream_t st);
 hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *kout, const int32_t *vin,
                     int32_t *vout, int64_t n, hipStream_t st);
-size_t pair_lds_bytes(int ncell, int nslots, int n_rp, int nwave, int acc_mode);
+size_t pair_lds_bytes(int ncell, int nslots, int n_rp);
 hipError_t launch_pair(int kind, bool split, bool swap, const KArgs &P, int threads, const PRec *brec,
-                       const FRec *bfr, const PRec *trec, const FRec *tfr, hipStream_t st);
-hipError_t launch_sky(int kind, bool split, bool swap, const SkyArgs &P, int threads, const PRec *brec,
-                      const SRec *bsr, const Win *bwin, const PRec *trec, const SRec *tsr, const Win *twin,
-                      hipStream_t st);
+                       const double *bz, const PRec *trec, const double *tz, hipStream_t st);
+// Record arrays of one side of a sky pass.
+struct SkySide {
+  const PRec *rec;
+  const double *z;
+  const int32_t *idx;
+  const SRec *sr;
+  const Win *win;  // outer side only
+};
+hipError_t launch_sky(int kind, bool split, bool swap, const SkyArgs &P, int threads, const SkySide &B,
+                      const SkySide &T, hipStream_t st);
 hipError_t launch_gather_sky(const int32_t *sidx, int64_t n, const SRec *src_sr, const Win *src_win, SRec *sr,
                              Win *win, hipStream_t st);
 

@@ -32,49 +32,48 @@
 
 namespace cpc {
 
-template <int K, bool SPLIT, bool SWAP, int ACC>
-__global__ void __launch_bounds__(256) pair_kernel(KArgs P, const PRec *__restrict__ brec,
-                                                   const FRec *__restrict__ bfr, const PRec *__restrict__ trec,
-                                                   const FRec *__restrict__ tfr) {
+// Tiles reach the wave in batches of kBatch z-consecutive records, loaded with coalesced vector loads
+// one batch ahead of use: lanes 4t..4t+3 hold the four 16-byte quarters of tile t's record, and the
+// wave reads a tile's fields with v_readlane, so no tile load stands between two candidate tests.
+constexpr int kBatch = 16;
+static_assert(kRecPad >= kBatch, "record padding must cover a batch read past the end of a range");
+
+__device__ __forceinline__ float rl_f(unsigned v, int l) { return __uint_as_float(__builtin_amdgcn_readlane(v, l)); }
+__device__ __forceinline__ int rl_i(unsigned v, int l) { return (int)__builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ double rl_d(unsigned lo, unsigned hi, int l) {
+  return __hiloint2double((int)__builtin_amdgcn_readlane(hi, l), (int)__builtin_amdgcn_readlane(lo, l));
+}
+__device__ __forceinline__ unsigned long long popc_ballot(bool b) { return (unsigned long long)__popcll(__ballot(b)); }
+
+template <int K, bool SPLIT, bool SWAP>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT ? 4 : 2))) pair_kernel(KArgs P, const PRec *__restrict__ brec,
+                                                   const double *__restrict__ bz, const PRec *__restrict__ trec,
+                                                   const double *__restrict__ tz) {
   constexpr int NV = KindNV<K>::NV;
   constexpr int NS = SWAP ? 1 + NV : 1 + 2 * NV;  // f64 slots accumulated
   constexpr int NQ = 1 + 2 * NV;                  // quantities computed
-  constexpr int QCAP = 128;                       // compaction ring per wave
   extern __shared__ __align__(16) unsigned char smem[];
   const int ncell = P.ncell;
   unsigned *lnum = reinterpret_cast<unsigned *>(smem);
   double *lacc = reinterpret_cast<double *>(smem + ((ncell * 4 + 15) & ~15));
   double *lthr = lacc + (size_t)NS * ncell;
-  float *llo = reinterpret_cast<float *>(lthr + P.n_rp + 1);
-  float *lhi = llo + P.n_rp + 1;
-  unsigned long long *lq =
-      reinterpret_cast<unsigned long long *>(reinterpret_cast<uintptr_t>(lhi + P.n_rp + 1 + 1) & ~uintptr_t(7));
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = uni(tid >> 6);
   const int nwave = blockDim.x >> 6;
-  unsigned long long *wq = lq + wave * QCAP;
 
-  for (int k = tid; k <= P.n_rp; k += blockDim.x) {
-    lthr[k] = P.thr[k];
-    if (SPLIT) {
-      llo[k] = P.thr_lo[k];
-      lhi[k] = P.thr_hi[k];
-    }
-  }
+  for (int k = tid; k <= P.n_rp; k += blockDim.x) lthr[k] = P.thr[k];
 
-  // outermost split-precision edges, wave-uniform: s below thr[0] or above thr[n_rp] for sure
-  const float hi_bottom = SPLIT ? P.thr_hi[0] : 0.0f;
-  const float lo_top = SPLIT ? P.thr_lo[P.n_rp] : 0.0f;
   const Item it = P.items[blockIdx.x];
   const Group bg = P.bgrp[it.group];
   const int bcx = bg.col % P.nc, bcy = bg.col / P.nc;
   // phase_mode: 0 primary without jackknife, 1 primary with jackknife, 2 touch pass
   const bool two_phase = (P.phase_mode == 1) && (bg.coin == 0);
   const int nphase = two_phase ? 2 : 1;
+  const uint4 *tq = reinterpret_cast<const uint4 *>(trec);  // records as 16-byte quarters
 
-  unsigned long long ncand = 0, nre = 0;
+  unsigned long long ncand = 0, nre = 0;  // wave totals
 
   for (int ph = 0; ph < nphase; ++ph) {
     // filter on tile region: 0 all, 1 same region as block, 2 other regions
@@ -94,44 +93,18 @@ __global__ void __launch_bounds__(256) pair_kernel(KArgs P, const PRec *__restri
       const bool valid = lane < cnt;
       const int p = base + (valid ? lane : 0);
       const PRec me = brec[p];
-      FRec mf;
-      if (SPLIT) mf = bfr[p];
-      const double wzlo = brec[base].z;
-      const double wzhi = brec[base + cnt - 1].z;
+      const double mz = bz[p];
+      const double wzlo = bz[base];
+      const double wzhi = bz[base + cnt - 1];
 
-      // ACC 0: a lane keeps the sums of its current histogram cell in registers and adds them to LDS
-      // only when its cell changes.  Lanes of a wave hold neighbouring galaxies, so their pairs crowd
-      // into few cells; flushing per run instead of per pair divides the same-address LDS atomics.
+      // A lane keeps the sums of its current histogram cell in registers and adds them to LDS only
+      // when its cell changes: lanes of a wave hold neighbouring galaxies, so their pairs crowd into
+      // few cells, and flushing per run instead of per pair divides the same-address LDS atomics.
       int run_cell = -1;
       unsigned run_n = 0;
-      double run[ACC == 0 ? NS : 1];
+      double run[NS];
 #pragma unroll
-      for (int s = 0; s < (ACC == 0 ? NS : 1); ++s) run[s] = 0.0;
-      // ACC 1: binned pairs (tile index, lane, cell) go to a per-wave ring in LDS; every 64 entries
-      // the wave evaluates them with all lanes busy, so the FP64 shape arithmetic runs on full waves.
-      int qn = 0, qr = 0;
-
-      auto run_batch = [&](int nvalid) {
-        const bool act = lane < nvalid;
-        const unsigned long long e = wq[(qr + lane) & (QCAP - 1)];
-        if (act) {
-          const int j = (int)(e >> 32);
-          const int src = (int)((e >> 16) & 63);
-          const int cell = (int)(e & 0xffff);
-          const PRec b = brec[base + src];
-          const PRec t = trec[j];
-          const Gal gb{b.x, b.y, b.w, b.e1, b.e2, b.phi};
-          const Gal gt{t.x, t.y, t.w, t.e1, t.e2, t.phi};
-          double qv[NQ];
-          if (!SWAP)
-            pair_values<K, SPLIT>(gb, gt, P.gws, qv);
-          else
-            pair_values<K, SPLIT>(gt, gb, P.gws, qv);
-          lds_add<NS>(lnum, lacc, ncell, cell, 1u, qv);
-        }
-        qr = (qr + nvalid) & (QCAP - 1);
-        qn -= nvalid;
-      };
+      for (int s = 0; s < NS; ++s) run[s] = 0.0;
 
       for (int q = 0; q < P.n_nbr; ++q) {
         const int2 o = P.nbr[q];
@@ -157,65 +130,77 @@ __global__ void __launch_bounds__(256) pair_kernel(KArgs P, const PRec *__restri
               hi = wzhi - P.pmin - P.L * sz + P.zmargin;
             }
             if (hi < tg.zlo || lo > tg.zhi) continue;
-            const int j0 = zsearch<false>(trec, tg.start, tg.end, lo);
-            const int j1 = zsearch<true>(trec, j0, tg.end, hi);
+            const int j0 = zsearch<false>(tz, tg.start, tg.end, lo);
+            const int j1 = zsearch<true>(tz, j0, tg.end, hi);
             if (j0 >= j1) continue;
             // image of the inner galaxy in the reference's terms
             const int PBx = SWAP ? -sx : sx, PBy = SWAP ? -sy : sy, PBz = SWAP ? -sz : sz;
             const bool lane_ok = valid && (SWAP || image_allowed(me.meta, PBx, PBy, PBz));
             const float Oz = (float)(P.L * sz);
-            for (int jb = j0; jb < j1; jb += 4) {
-              F4 f4;
-              if (SPLIT) f4 = *reinterpret_cast<const F4 *>(tfr + jb);  // padded past the end
-#pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                const int j = jb + u;
-                if (j >= j1) break;
+
+            uint4 cq = tq[(size_t)j0 * 4 + lane];
+            double cz = tz[j0 + (lane & (kBatch - 1))];  // z of tile t in lane t
+            for (int jb = j0; jb < j1; jb += kBatch) {
+              uint4 nq = cq;
+              double nz = cz;
+              if (jb + kBatch < j1) {
+                nq = tq[(size_t)(jb + kBatch) * 4 + lane];
+                nz = tz[jb + kBatch + (lane & (kBatch - 1))];
+              }
+              const int nt = min(kBatch, j1 - jb);
+              for (int t = 0; t < nt; ++t) {
+                const int l3 = 4 * t + 3;  // lane of the quarter {xl, yl, zf, meta}
                 bool ok = lane_ok;
+                if (SWAP) ok = ok && image_allowed(rl_i(cq.w, l3), PBx, PBy, PBz);
                 int cell = -1;
                 bool unc = ok;
-                if (SPLIT) {
-                  const FRec tf = f4.r[u];
-                  if (SWAP) ok = ok && image_allowed(tf.meta, PBx, PBy, PBz);
-                  unc = ok;
-                  if (ok) {
-                    const float dx = (tf.xl - mf.xl) + Ox;
-                    const float dy = (tf.yl - mf.yl) + Oy;
-                    const float dz = (tf.zf - mf.zf) + Oz;
-                    const float pif = SWAP ? -dz : dz;
-                    const float s32 = fmaf(dx, dx, dy * dy);
-                    const float uu = (pif - P.pmin32) * P.inv_psize32;
-                    const float fu = floorf(uu);
-                    const float fr = uu - fu;
-                    const bool near_pi = (fr < P.gz) || (fr > 1.0f - P.gz);
-                    const bool pi_in = (fu >= 0.0f) && (fu < (float)P.n_pi);
-                    float kf = P.lin ? (sqrtf(s32) - P.lrmin) * P.linv : __log2f(s32) * P.la + P.lb;
+                if (SPLIT && ok) {
+                  const float dx = (rl_f(cq.x, l3) - me.xl) + Ox;
+                  const float dy = (rl_f(cq.y, l3) - me.yl) + Oy;
+                  const float dz = (rl_f(cq.z, l3) - me.zf) + Oz;
+                  const float pif = SWAP ? -dz : dz;
+                  const float s32 = fmaf(dx, dx, dy * dy);
+                  const float uu = (pif - P.pmin32) * P.inv_psize32;
+                  const float fu = floorf(uu);
+                  const float fr = uu - fu;
+                  const bool near_pi = (fr < P.gz) || (fr > 1.0f - P.gz);
+                  const bool pi_in = (fu >= 0.0f) && (fu < (float)P.n_pi);
+                  const bool rp_out = (s32 < P.s_out_lo) || (s32 >= P.s_out_hi);
+                  if ((!near_pi && !pi_in) || rp_out) {
+                    unc = false;
+                  } else if (!near_pi && s32 >= P.s_cert_lo) {
+                    // rp bin index in FP32 and its guard band; outside the band the bin is that of
+                    // the FP64 squared separation
+                    const float rinv = __builtin_amdgcn_rsqf(s32);
+                    float kf = P.lin ? (s32 * rinv - P.lrmin) * P.linv : __log2f(s32) * P.la + P.lb;
                     kf = fminf(fmaxf(kf, -1.0f), (float)P.n_rp);
-                    const int k = (int)floorf(kf);
-                    const int kk = min(max(k, 0), P.n_rp - 1);
-                    const bool rp_out = (s32 < hi_bottom) || (s32 >= lo_top);
-                    const bool rp_in = (k == kk) && (s32 >= llo[kk]) && (s32 < lhi[kk + 1]);
-                    if ((!near_pi && !pi_in) || rp_out) {
+                    const float fk = floorf(kf);
+                    const float frk = kf - fk;
+                    const float gk = fmaf(P.gD, s32 * rinv, fmaf(fmaf(P.gC, rinv, P.gB), rinv, P.gA));
+                    if (frk >= gk && frk <= 1.0f - gk) {
                       unc = false;
-                    } else if (!near_pi && rp_in) {
-                      unc = false;
-                      cell = k * P.n_pi + (int)fu;
+                      const int k = (int)fk;
+                      if (k >= 0 && k < P.n_rp) cell = k * P.n_pi + (int)fu;
                     }
                   }
-                } else if (SWAP) {
-                  ok = ok && image_allowed(trec[j].meta, PBx, PBy, PBz);
-                  unc = ok;
                 }
-                ncand += ok;
-                if (unc) {
-                  nre += SPLIT;
-                  const PRec t = trec[j];
-                  if (!SWAP)
-                    cell = exact_cell(P, lthr, me.x, me.y, me.z, t.x, t.y, t.z, PBx, PBy, PBz);
-                  else
-                    cell = exact_cell(P, lthr, t.x, t.y, t.z, me.x, me.y, me.z, PBx, PBy, PBz);
+                ncand += popc_ballot(ok);
+                if (__ballot(unc) != 0ull) {
+                  nre += SPLIT ? popc_ballot(unc) : 0ull;
+                  const double tx = rl_d(cq.x, cq.y, 4 * t), ty = rl_d(cq.z, cq.w, 4 * t);
+                  const double tzz = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(cz), t),
+                                                      __builtin_amdgcn_readlane(__double2loint(cz), t));
+                  if (unc) {
+                    if (!SWAP)
+                      cell = exact_cell(P, lthr, me.x, me.y, mz, tx, ty, tzz, PBx, PBy, PBz);
+                    else
+                      cell = exact_cell(P, lthr, tx, ty, tzz, me.x, me.y, mz, PBx, PBy, PBz);
+                  }
                 }
-                if constexpr (ACC == 0) {
+                if (__ballot(cell >= 0) != 0ull) {
+                  const Gal gt{rl_d(cq.x, cq.y, 4 * t),     rl_d(cq.z, cq.w, 4 * t),
+                               rl_d(cq.x, cq.y, 4 * t + 1), rl_d(cq.z, cq.w, 4 * t + 1),
+                               rl_d(cq.x, cq.y, 4 * t + 2), rl_d(cq.z, cq.w, 4 * t + 2)};
                   if (cell >= 0) {
                     if (cell != run_cell) {
                       if (run_cell >= 0) lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
@@ -225,9 +210,7 @@ __global__ void __launch_bounds__(256) pair_kernel(KArgs P, const PRec *__restri
                       for (int s = 0; s < NS; ++s) run[s] = 0.0;
                     }
                     ++run_n;
-                    const PRec t = trec[j];
                     const Gal gme{me.x, me.y, me.w, me.e1, me.e2, me.phi};
-                    const Gal gt{t.x, t.y, t.w, t.e1, t.e2, t.phi};
                     double qv[NQ];
                     if (!SWAP)
                       pair_values<K, SPLIT>(gme, gt, P.gws, qv);
@@ -236,38 +219,15 @@ __global__ void __launch_bounds__(256) pair_kernel(KArgs P, const PRec *__restri
 #pragma unroll
                     for (int s = 0; s < NS; ++s) run[s] += qv[s];
                   }
-                } else {
-                  const unsigned long long m = __ballot(cell >= 0);
-                  if (m) {
-                    const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                              __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                    if (cell >= 0)
-                      wq[(qr + qn + pos) & (QCAP - 1)] =
-                          ((unsigned long long)(unsigned)j << 32) | ((unsigned long long)lane << 16) | (unsigned)cell;
-                    qn += __popcll(m);
-                    if (qn >= 64) {
-                      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                      __builtin_amdgcn_wave_barrier();
-                      run_batch(64);
-                      __builtin_amdgcn_wave_barrier();
-                    }
-                  }
                 }
               }
+              cq = nq;
+              cz = nz;
             }
           }
         }
       }
-      if constexpr (ACC == 0) {
-        if (run_cell >= 0) lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
-      } else {
-        if (qn > 0) {
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          run_batch(qn);
-          __builtin_amdgcn_wave_barrier();
-        }
-      }
+      if (run_cell >= 0) lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
     }
     __syncthreads();
     for (int c = tid; c < ncell; c += blockDim.x) {
@@ -280,8 +240,10 @@ __global__ void __launch_bounds__(256) pair_kernel(KArgs P, const PRec *__restri
     }
     __syncthreads();
   }
-  if (ncand) atomicAdd(&P.counters[0], ncand);
-  if (nre) atomicAdd(&P.counters[1], nre);
+  if (lane == 0) {
+    if (ncand) atomicAdd(&P.counters[0], ncand);
+    if (nre) atomicAdd(&P.counters[1], nre);
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -311,20 +273,19 @@ __global__ void k_keys(KeyArgs A, const int32_t *__restrict__ order, uint32_t *_
 
 
 __global__ void k_gather(GatherArgs A, const int32_t *__restrict__ sidx, PRec *__restrict__ rec,
-                         FRec *__restrict__ fr) {
+                         double *__restrict__ zs, int32_t *__restrict__ idx) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= A.n) return;
   const int32_t i = sidx[p];
   PRec r;
   r.x = A.x[i];
   r.y = A.y[i];
-  r.z = A.z[i];
+  const double z = A.z[i];
   r.w = A.w ? A.w[i] : 1.0;
   r.e1 = A.e1 ? A.e1[i] : 0.0;
   r.e2 = A.e2 ? A.e2[i] : 0.0;
   r.phi = A.phi ? A.phi[i] : -10.0;
-  r.idx = i;
-  const double z_max = r.z + A.pmax, z_min = r.z + A.pmin;
+  const double z_max = z + A.pmax, z_min = z + A.pmin;
   const double ra_max = r.x + A.rmax, ra_min = r.x - A.rmax;
   const double dec_max = r.y + A.rmax, dec_min = r.y - A.rmax;
   int m = 0;
@@ -336,16 +297,15 @@ __global__ void k_gather(GatherArgs A, const int32_t *__restrict__ sidx, PRec *_
   // PBz>0 sweeps from j = 0 and stops at the first included galaxy outside the window (corr.cpp:407-416)
   if (z_max > A.L && A.has_zfirst && !(A.zfirst_plus_L < z_min)) m |= kMetaZp;
   r.meta = m;
-  rec[p] = r;
   int cx = (int)floor((r.x - A.x0) / A.hx), cy = (int)floor((r.y - A.y0) / A.hy);
   cx = min(max(cx, 0), A.ncx - 1);
   cy = min(max(cy, 0), A.ncy - 1);
-  FRec f;
-  f.xl = (float)(r.x - (A.x0 + cx * A.hx));
-  f.yl = (float)(r.y - (A.y0 + cy * A.hy));
-  f.zf = (float)r.z;
-  f.meta = m;
-  fr[p] = f;
+  r.xl = (float)(r.x - (A.x0 + cx * A.hx));
+  r.yl = (float)(r.y - (A.y0 + cy * A.hy));
+  r.zf = (float)z;
+  rec[p] = r;
+  zs[p] = z;
+  idx[p] = i;
 }
 
 // Outer galaxies whose PBz = 0 or PBz = -1 window holds no inner galaxy while later ones exist: the
@@ -383,10 +343,11 @@ hipError_t launch_keys(const KeyArgs &A, const int32_t *order, uint32_t *keys, i
   return hipGetLastError();
 }
 
-hipError_t launch_gather(const GatherArgs &A, const int32_t *sidx, PRec *rec, FRec *fr, hipStream_t st) {
+hipError_t launch_gather(const GatherArgs &A, const int32_t *sidx, PRec *rec, double *zs, int32_t *idx,
+                         hipStream_t st) {
   if (A.n == 0) return hipSuccess;
   const int bs = 256;
-  k_gather<<<(unsigned)((A.n + bs - 1) / bs), bs, 0, st>>>(A, sidx, rec, fr);
+  k_gather<<<(unsigned)((A.n + bs - 1) / bs), bs, 0, st>>>(A, sidx, rec, zs, idx);
   return hipGetLastError();
 }
 
@@ -425,48 +386,37 @@ hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *ko
   return sort_pairs_impl<double>(tmp, tmp_bytes, kin, kout, vin, vout, n, 64, st);
 }
 
-template <int K, bool SPLIT, bool SWAP, int ACC>
-static hipError_t launch_pair_t(const KArgs &P, int threads, size_t lds, const PRec *brec, const FRec *bfr,
-                                const PRec *trec, const FRec *tfr, hipStream_t st) {
-  auto fn = pair_kernel<K, SPLIT, SWAP, ACC>;
+template <int K, bool SPLIT, bool SWAP>
+static hipError_t launch_pair_t(const KArgs &P, int threads, size_t lds, const PRec *brec, const double *bz,
+                                const PRec *trec, const double *tz, hipStream_t st) {
+  auto fn = pair_kernel<K, SPLIT, SWAP>;
   hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  fn<<<P.n_items, threads, lds, st>>>(P, brec, bfr, trec, tfr);
+  fn<<<P.n_items, threads, lds, st>>>(P, brec, bz, trec, tz);
   return hipGetLastError();
 }
 
 // LDS bytes a pair kernel needs for histograms of `ncell` cells and `nslots` f64 slots.
-size_t pair_lds_bytes(int ncell, int nslots, int n_rp, int nwave, int acc_mode) {
-  size_t b = (size_t)((ncell * 4 + 15) & ~15) + (size_t)nslots * ncell * 8 + (size_t)(n_rp + 1) * 8 +
-             (size_t)2 * (n_rp + 1) * 4;
-  b = (b + 4 + 7) & ~(size_t)7;
-  if (acc_mode == 1) b += (size_t)nwave * 128 * 8;
-  return b;
+size_t pair_lds_bytes(int ncell, int nslots, int n_rp) {
+  return (size_t)((ncell * 4 + 15) & ~15) + (size_t)nslots * ncell * 8 + (size_t)(n_rp + 1) * 8;
 }
 
 hipError_t launch_pair(int kind, bool split, bool swap, const KArgs &P, int threads, const PRec *brec,
-                       const FRec *bfr, const PRec *trec, const FRec *tfr, hipStream_t st) {
+                       const double *bz, const PRec *trec, const double *tz, hipStream_t st) {
   if (P.n_items == 0) return hipSuccess;
-  const size_t lds = pair_lds_bytes(P.ncell, P.nslots, P.n_rp, threads / 64, P.acc_mode);
-#define CPC_DISPATCH2(KK, AA)                                                                          \
-  if (split && swap) return launch_pair_t<KK, true, true, AA>(P, threads, lds, brec, bfr, trec, tfr, st); \
-  if (split) return launch_pair_t<KK, true, false, AA>(P, threads, lds, brec, bfr, trec, tfr, st);        \
-  if (swap) return launch_pair_t<KK, false, true, AA>(P, threads, lds, brec, bfr, trec, tfr, st);         \
-  return launch_pair_t<KK, false, false, AA>(P, threads, lds, brec, bfr, trec, tfr, st);
-#define CPC_DISPATCH(KK)                    \
-  if (kind == KK) {                         \
-    if (P.acc_mode == 1) {                  \
-      CPC_DISPATCH2(KK, 1)                  \
-    } else {                                \
-      CPC_DISPATCH2(KK, 0)                  \
-    }                                       \
+  const size_t lds = pair_lds_bytes(P.ncell, P.nslots, P.n_rp);
+#define CPC_DISPATCH(KK)                                                                                  \
+  if (kind == KK) {                                                                                       \
+    if (split && swap) return launch_pair_t<KK, true, true>(P, threads, lds, brec, bz, trec, tz, st);   \
+    if (split) return launch_pair_t<KK, true, false>(P, threads, lds, brec, bz, trec, tz, st);          \
+    if (swap) return launch_pair_t<KK, false, true>(P, threads, lds, brec, bz, trec, tz, st);           \
+    return launch_pair_t<KK, false, false>(P, threads, lds, brec, bz, trec, tz, st);                    \
   }
   CPC_DISPATCH(kDD)
   CPC_DISPATCH(kSD)
   CPC_DISPATCH(kSS0)
   CPC_DISPATCH(kSS1)
 #undef CPC_DISPATCH
-#undef CPC_DISPATCH2
   return hipErrorInvalidValue;
 }
 

@@ -40,10 +40,13 @@ __device__ __forceinline__ int rp_bin_sky(const SkyArgs &P, const double *thr, d
 }
 
 template <int K, bool SPLIT, bool SWAP>
-__global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, const PRec *__restrict__ brec,
-                                                  const SRec *__restrict__ bsr, const Win *__restrict__ bwin,
-                                                  const PRec *__restrict__ trec, const SRec *__restrict__ tsr,
-                                                  const Win *__restrict__ twin) {
+__global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide T) {
+  const PRec *__restrict__ brec = B.rec;
+  const SRec *__restrict__ bsr = B.sr;
+  const Win *__restrict__ bwin = B.win;
+  const PRec *__restrict__ trec = T.rec;
+  const SRec *__restrict__ tsr = T.sr;
+  const Win *__restrict__ twin = T.win;
   constexpr int NV = KindNV<K>::NV;
   constexpr int NS = SWAP ? 1 + NV : 1 + 2 * NV;
   constexpr int NQ = 1 + 2 * NV;
@@ -77,6 +80,7 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, const PRec *__restr
     const bool valid = lane < cnt;
     const int p = base + (valid ? lane : 0);
     const PRec me = brec[p];
+    const double mez = B.z[p];
     const SRec ms = bsr[p];
     Win mw{};
     if (!SWAP) mw = bwin[p];
@@ -93,8 +97,8 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, const PRec *__restr
       ralo = wave_min(valid ? mw.ramin : INFINITY);
       rahi = wave_max(valid ? mw.ramax : -INFINITY);
     } else {
-      zlo = wave_min(valid ? me.z : INFINITY) - P.reach_zup;
-      zhi = wave_max(valid ? me.z : -INFINITY) + P.reach_zdn;
+      zlo = wave_min(valid ? mez : INFINITY) - P.reach_zup;
+      zhi = wave_max(valid ? mez : -INFINITY) + P.reach_zdn;
       declo = wave_min(valid ? me.y : INFINITY) - P.reach_dec;
       dechi = wave_max(valid ? me.y : -INFINITY) + P.reach_dec;
       ra_all = P.ra_full != 0;
@@ -125,10 +129,11 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, const PRec *__restr
           const Group tg = P.tgrp[g];
           if (P.phase_mode == 2 && tg.region == bg.region) continue;
           if (zsh < tg.zlo || zsl > tg.zhi) continue;
-          const int j0 = zsearch<false>(trec, tg.start, tg.end, zsl);
-          const int j1 = zsearch<true>(trec, j0, tg.end, zsh);
+          const int j0 = zsearch<false>(T.z, tg.start, tg.end, zsl);
+          const int j1 = zsearch<true>(T.z, j0, tg.end, zsh);
           for (int j = j0; j < j1; ++j) {
             const PRec t = trec[j];
+            const double tzj = T.z[j];
             const SRec ts = tsr[j];
             // outer (o) and inner (i) roles of the pair
             const PRec &po = SWAP ? t : me;
@@ -141,7 +146,8 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, const PRec *__restr
             else
               w = mw;
             // do_pair_sorted's tests against the outer galaxy's window (PBz = PBy = PBx = 0)
-            bool ok = valid && !(pi.z > w.zmax || pi.z < w.zmin) && !(pi.y > w.decmax || pi.y < w.decmin) &&
+            const double piz = SWAP ? mez : tzj;
+            bool ok = valid && !(piz > w.zmax || piz < w.zmin) && !(pi.y > w.decmax || pi.y < w.decmin) &&
                       !(pi.x < w.ramin || pi.x > w.ramax);
             ncand += ok;
             int cell = -1;
@@ -161,7 +167,7 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, const PRec *__restr
                 if (unc) {
                   if (!SWAP) {
                     const unsigned long long slot = atomicAdd(&P.counters[3], 1ull);
-                    if (slot < (unsigned long long)P.hq_cap) P.hq[slot] = make_uint2(po.idx, pi.idx);
+                    if (slot < (unsigned long long)P.hq_cap) P.hq[slot] = make_uint2(B.idx[p], T.idx[j]);
                   }
                 } else if (k >= 0) {
                   cell = k * P.n_pi + (int)q2;
@@ -222,27 +228,25 @@ hipError_t launch_gather_sky(const int32_t *sidx, int64_t n, const SRec *src_sr,
 }
 
 template <int K, bool SPLIT, bool SWAP>
-static hipError_t launch_sky_t(const SkyArgs &P, int threads, size_t lds, const PRec *brec, const SRec *bsr,
-                               const Win *bwin, const PRec *trec, const SRec *tsr, const Win *twin,
+static hipError_t launch_sky_t(const SkyArgs &P, int threads, size_t lds, const SkySide &B, const SkySide &T,
                                hipStream_t st) {
   auto fn = sky_kernel<K, SPLIT, SWAP>;
   hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  fn<<<P.n_items, threads, lds, st>>>(P, brec, bsr, bwin, trec, tsr, twin);
+  fn<<<P.n_items, threads, lds, st>>>(P, B, T);
   return hipGetLastError();
 }
 
-hipError_t launch_sky(int kind, bool split, bool swap, const SkyArgs &P, int threads, const PRec *brec,
-                      const SRec *bsr, const Win *bwin, const PRec *trec, const SRec *tsr, const Win *twin,
-                      hipStream_t st) {
+hipError_t launch_sky(int kind, bool split, bool swap, const SkyArgs &P, int threads, const SkySide &B,
+                      const SkySide &T, hipStream_t st) {
   if (P.n_items == 0) return hipSuccess;
-  const size_t lds = pair_lds_bytes(P.ncell, P.nslots, P.n_rp, threads / 64, 0);
+  const size_t lds = pair_lds_bytes(P.ncell, P.nslots, P.n_rp);
 #define CPC_SKY(KK)                                                                                     \
   if (kind == KK) {                                                                                     \
-    if (split && swap) return launch_sky_t<KK, true, true>(P, threads, lds, brec, bsr, bwin, trec, tsr, twin, st); \
-    if (split) return launch_sky_t<KK, true, false>(P, threads, lds, brec, bsr, bwin, trec, tsr, twin, st);        \
-    if (swap) return launch_sky_t<KK, false, true>(P, threads, lds, brec, bsr, bwin, trec, tsr, twin, st);         \
-    return launch_sky_t<KK, false, false>(P, threads, lds, brec, bsr, bwin, trec, tsr, twin, st);                  \
+    if (split && swap) return launch_sky_t<KK, true, true>(P, threads, lds, B, T, st); \
+    if (split) return launch_sky_t<KK, true, false>(P, threads, lds, B, T, st);        \
+    if (swap) return launch_sky_t<KK, false, true>(P, threads, lds, B, T, st);         \
+    return launch_sky_t<KK, false, false>(P, threads, lds, B, T, st);                  \
   }
   CPC_SKY(kDD)
   CPC_SKY(kSD)

@@ -197,7 +197,7 @@ def load_library(path=LIB_PATH):
     lib.cpc_rp_thresholds.restype = c_int
     lib.cpc_rand_stream.argtypes = [ctypes.c_uint32, c_int64, c_int64, _dp]
     lib.cpc_rand_stream.restype = c_int
-    lib.cpc_set_tuning.argtypes = [c_void_p, c_double, c_int, c_int, c_int]
+    lib.cpc_set_tuning.argtypes = [c_void_p, c_double, c_int, c_int]
     lib.cpc_set_tuning.restype = c_int
     _lib = lib
     return lib
@@ -262,8 +262,8 @@ class Engine:
         if ent is not None:
             self.lib.cpc_catalog_free(self.ctx, ent[0])
 
-    def set_tuning(self, col_frac=0.0, item_size=0, block_threads=0, acc_mode=-1):
-        self._check(self.lib.cpc_set_tuning(self.ctx, col_frac, item_size, block_threads, acc_mode), "set_tuning")
+    def set_tuning(self, col_frac=0.0, item_size=0, block_threads=0):
+        self._check(self.lib.cpc_set_tuning(self.ctx, col_frac, item_size, block_threads), "set_tuning")
 
     def count(self, outer: Catalog, inner: Catalog, params: Params, hist: Histogram = None):
         """One cross pass (the reference's corels2 after the role swap).  Returns (Histogram, stats dict)."""

@@ -163,11 +163,11 @@ cpc_status cpc_rp_thresholds(const cpc_params *params, double *thr);
  * include_prob (read_dat.cpp:318-320) in catalog read order (do_corr.cpp:40-96). */
 cpc_status cpc_rand_stream(uint32_t seed, int64_t skip, int64_t n, double *out);
 
-/* Engine tuning knobs: column width in units of rp_max, block galaxies per work item, threads per
- * work group (0 keeps the automatic choice) and the histogram accumulation mode (-1 automatic,
- * 0 run-length privatisation in registers, 1 per-wave compaction of binned pairs).  Used by the
- * autotuner; results do not depend on them beyond floating-point summation order. */
-cpc_status cpc_set_tuning(cpc_ctx *ctx, double col_frac, int item_size, int block_threads, int acc_mode);
+/* Engine tuning knobs: column width in units of rp_max, block galaxies per work item and threads per
+ * work group (a multiple of 64, at most 512 for the box and 256 for the sky kernel; 0 keeps the
+ * automatic choice).  Used by the autotuner; results do not depend on them beyond floating-point
+ * summation order. */
+cpc_status cpc_set_tuning(cpc_ctx *ctx, double col_frac, int item_size, int block_threads);
 
 #ifdef __cplusplus
 }

@@ -17,18 +17,23 @@ pytestmark = pytest.mark.gpu
 PRECS = [PREC_SPLIT, PREC_FP64]
 
 
+# the automatic decomposition and a different one (narrower work groups, short items, wider columns):
+# results must not depend on how the pairs are scheduled
+TUNINGS = [{}, {"col_frac": 0.5, "item_size": 200, "block_threads": 256}]
+
+
 def _run(engine, oracle_lib, outer, inner, p, what):
-    """Both histogram accumulation modes of the engine against the oracle."""
+    """The engine under each tuning against the oracle."""
     ref, rst = oracle_lib.count(outer, inner, p)
     st = None
-    for acc_mode in (0, 1):
-        print(f"[case] {what} acc_mode={acc_mode}", flush=True)
-        engine.set_tuning(acc_mode=acc_mode)
+    for tun in TUNINGS:
+        print(f"[case] {what} tuning={tun}", flush=True)
+        engine.set_tuning(**tun)
         try:
             got, st = engine.count(outer, inner, p)
         finally:
             engine.set_tuning()
-        assert_hist_match(got, ref, what=f"{what} acc_mode={acc_mode}")
+        assert_hist_match(got, ref, what=f"{what} tuning={tun}")
         assert st["n_binned"] == rst["n_binned"]
     return st
 
