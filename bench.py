@@ -23,8 +23,24 @@ FLOPS_PER_PAIR = {"DD": 12, "SD": 33, "SS": 53}
 PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "pair_kernel_ss_split_primary_box1m_traffic.md")
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per launch of the primary split kernel, from the committed rocprofv3 FETCH_SIZE and
+    WRITE_SIZE summary of this workload (None when the summary is absent)."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -101,7 +117,7 @@ def main():
 
     achieved = FLOPS_PER_PAIR["SS"] * n_binned / (ms_primary * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_TFLOPS, "traffic": None,
+                "frac": achieved / PEAK_TFLOPS, "traffic": measured_traffic(),
                 "kernel": "pair_kernel<shape_shape, split, primary>",
                 "flops_per_pair": FLOPS_PER_PAIR["SS"], "ms_per_launch": ms_primary}
 

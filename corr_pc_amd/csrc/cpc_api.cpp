@@ -1075,7 +1075,7 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
       K.gws = p.g_with_shape;
       K.n_nbr = (int)nbr.size();
       K.lin = p.lin_bin;
-      K.pmin32 = (float)p.pi_min;
+      K.pioff32 = (float)(-p.pi_min / dv.psize);
       K.inv_psize32 = (float)(1.0 / dv.psize);
       K.gz = (float)gz;
       K.la = (float)(0.5 * std::log10(2.0) / dv.rsize);

@@ -67,7 +67,7 @@ struct KArgs {
   int nslots;          // f64 slots accumulated by this pass
   int lin;
   // split-precision classification
-  float pmin32, inv_psize32, gz;   // pi bin: u = (dz - pmin)*inv_psize, guard gz in bin units
+  float pioff32, inv_psize32, gz;  // pi bin: u = dz*inv_psize + pioff (pioff = -pmin/psize), guard gz in bin units
   float la, lb;                    // rp bin estimate from log2(s) (log bins) ...
   float lrmin, linv;               // ... or from sqrt(s) (linear bins)
   // tables (device)

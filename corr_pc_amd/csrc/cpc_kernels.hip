@@ -136,7 +136,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
             // image of the inner galaxy in the reference's terms
             const int PBx = SWAP ? -sx : sx, PBy = SWAP ? -sy : sy, PBz = SWAP ? -sz : sz;
             const bool lane_ok = valid && (SWAP || image_allowed(me.meta, PBx, PBy, PBz));
-            const float Oz = (float)(P.L * sz);
+            // FP32 offsets of this lane to the tile column: d = tile local + offset
+            const float ax = Ox - me.xl, ay = Oy - me.yl, az = (float)(P.L * sz) - me.zf;
+            const float pinv = SWAP ? -P.inv_psize32 : P.inv_psize32;  // pi = -dz in the touch pass
 
             uint4 cq = tq[(size_t)j0 * 4 + lane];
             double cz = tz[j0 + (lane & (kBatch - 1))];  // z of tile t in lane t
@@ -155,12 +157,11 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
                 int cell = -1;
                 bool unc = ok;
                 if (SPLIT && ok) {
-                  const float dx = (rl_f(cq.x, l3) - me.xl) + Ox;
-                  const float dy = (rl_f(cq.y, l3) - me.yl) + Oy;
-                  const float dz = (rl_f(cq.z, l3) - me.zf) + Oz;
-                  const float pif = SWAP ? -dz : dz;
+                  const float dx = rl_f(cq.x, l3) + ax;
+                  const float dy = rl_f(cq.y, l3) + ay;
+                  const float dz = rl_f(cq.z, l3) + az;
                   const float s32 = fmaf(dx, dx, dy * dy);
-                  const float uu = (pif - P.pmin32) * P.inv_psize32;
+                  const float uu = fmaf(dz, pinv, P.pioff32);
                   const float fu = floorf(uu);
                   const float fr = uu - fu;
                   const bool near_pi = (fr < P.gz) || (fr > 1.0f - P.gz);
@@ -171,16 +172,15 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
                   } else if (!near_pi && s32 >= P.s_cert_lo) {
                     // rp bin index in FP32 and its guard band; outside the band the bin is that of
                     // the FP64 squared separation
+                    // (s32 is finite and at least s_cert_lo here, so kf is finite)
                     const float rinv = __builtin_amdgcn_rsqf(s32);
-                    float kf = P.lin ? (s32 * rinv - P.lrmin) * P.linv : __log2f(s32) * P.la + P.lb;
-                    kf = fminf(fmaxf(kf, -1.0f), (float)P.n_rp);
+                    const float kf = P.lin ? (s32 * rinv - P.lrmin) * P.linv : __log2f(s32) * P.la + P.lb;
                     const float fk = floorf(kf);
                     const float frk = kf - fk;
                     const float gk = fmaf(P.gD, s32 * rinv, fmaf(fmaf(P.gC, rinv, P.gB), rinv, P.gA));
                     if (frk >= gk && frk <= 1.0f - gk) {
                       unc = false;
-                      const int k = (int)fk;
-                      if (k >= 0 && k < P.n_rp) cell = k * P.n_pi + (int)fu;
+                      if (fk >= 0.0f && fk < (float)P.n_rp) cell = (int)fmaf(fk, (float)P.n_pi, fu);
                     }
                   }
                 }

@@ -78,7 +78,7 @@ typedef struct cpc_catalog_desc {
 
 /* The data_info fields the pair loop reads (data_def.h:46-96) plus engine choices. */
 typedef struct cpc_params {
-  int geometry;            /* CPC_GEOM_BOX */
+  int geometry;            /* CPC_GEOM_BOX or CPC_GEOM_SKY */
   double box_size;         /* data_info.periodic_box_size */
   int kind;                /* CPC_KIND_* */
   int g_with_shape;        /* data_info.g_with_shape after the swap: 1 = outer, 2 = inner carries the shape */
@@ -152,9 +152,10 @@ void cpc_catalog_free(cpc_ctx *ctx, cpc_catalog *cat);
 cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *outer, const cpc_catalog *inner,
                      const cpc_params *params, cpc_hist *out, cpc_stats *stats);
 
-/* The squared-separation thresholds the engine bins rp with: thr[k] (k = 0..n_rp) is the smallest
- * double s >= 0 for which the reference's bin index of rp = sqrt(s) (corels.cpp:31-34, with the host
- * libm's log10) is >= k.  Exposed so that callers and tests can check them against the host libm. */
+/* The thresholds the engine bins rp with.  Box: thr[k] (k = 0..n_rp) is the smallest double s >= 0
+ * for which the reference's bin index of rp = sqrt(s) (corels.cpp:31-34, with the host libm's log10)
+ * is >= k.  Sky: the same on rp itself (rp_calc_sky, sky_calcs.cpp:91-96).  Exposed so that callers
+ * and tests can check them against the host libm. */
 cpc_status cpc_rp_thresholds(const cpc_params *params, double *thr);
 
 /* The reference's per-galaxy random stream: out[k] = rand() / (RAND_MAX + 1.) for the draws numbered
