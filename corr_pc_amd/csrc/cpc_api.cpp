@@ -314,6 +314,7 @@ struct cpc_ctx {
   DevBuf<double> thr;
   DevBuf<int2> nbr;
   DevBuf<Item> items;
+  DevBuf<Chunk> chunks;
   DevBuf<double> accp, acct;
   DevBuf<unsigned long long> accpn, acctn, counters;
   DevBuf<Win> win_raw;
@@ -441,11 +442,14 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, b
   return st;
 }
 
-// Work items over the block side's groups, most expensive first.  `nbr_cost` gives, for a group,
-// the number of tile galaxies it may meet.
+// Work items over the block side: every group is cut into chunks of up to 64 galaxies (one wave
+// each); chunks sharing a histogram target are sorted by cost and packed `per_item` at a time, and
+// the items run most expensive first.  `nbr_cost` gives, for a group, the number of tile galaxies
+// it may meet.  Chunks are appended to `chunks` in item order.
 template <typename Cost>
-void make_items(const Side &B, bool touch, bool jk_on, int item_size, Cost nbr_cost, std::vector<Item> &items) {
-  std::vector<std::pair<double, Item>> tmp;
+void make_items(const Side &B, bool touch, bool jk_on, int per_item, Cost nbr_cost, std::vector<Item> &items,
+                std::vector<Chunk> &chunks) {
+  std::map<int, std::vector<std::pair<double, Chunk>>> by_target;
   for (size_t g = 0; g < B.groups.size(); ++g) {
     const Group &G = B.groups[g];
     if (touch && G.coin == 0) continue;
@@ -457,14 +461,36 @@ void make_items(const Side &B, bool touch, bool jk_on, int item_size, Cost nbr_c
       target = 2 * G.region + (G.coin ? 1 : 0);
     else
       target = 1;
-    for (int f = G.start; f < G.end; f += item_size) {
-      Item it{(int32_t)g, f, std::min(item_size, G.end - f), target};
-      tmp.push_back({(double)it.count * ncol, it});
+    auto &v = by_target[target];
+    for (int f = G.start; f < G.end; f += 64) {
+      const Chunk c{(int32_t)g, f, std::min(64, G.end - f), 0};
+      v.push_back({(double)c.count * ncol, c});
     }
   }
-  std::stable_sort(tmp.begin(), tmp.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+  struct Packed {
+    double cost;
+    int target;
+    std::vector<Chunk> cs;
+  };
+  std::vector<Packed> packed;
+  for (auto &kv : by_target) {
+    auto &v = kv.second;
+    std::stable_sort(v.begin(), v.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+    for (size_t a = 0; a < v.size(); a += per_item) {
+      Packed pk{0.0, kv.first, {}};
+      for (size_t b = a; b < std::min(v.size(), a + per_item); ++b) {
+        pk.cost += v[b].first;
+        pk.cs.push_back(v[b].second);
+      }
+      packed.push_back(std::move(pk));
+    }
+  }
+  std::stable_sort(packed.begin(), packed.end(), [](const Packed &a, const Packed &b) { return a.cost > b.cost; });
   items.clear();
-  for (auto &t : tmp) items.push_back(t.second);
+  for (const Packed &pk : packed) {
+    items.push_back(Item{(int32_t)chunks.size(), (int32_t)pk.cs.size(), pk.target, pk.cs[0].group});
+    chunks.insert(chunks.end(), pk.cs.begin(), pk.cs.end());
+  }
 }
 
 // Contributions binned on the host, in the reference layout: region 0 the full sample (all
@@ -623,6 +649,7 @@ void cpc_destroy(cpc_ctx *ctx) {
   ctx->thr.release();
   ctx->nbr.release();
   ctx->items.release();
+  ctx->chunks.release();
   ctx->accp.release();
   ctx->acct.release();
   ctx->accpn.release();
@@ -986,6 +1013,8 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
 
   // ---- work items (cost: block galaxies x tile galaxies in reach)
   std::vector<Item> items_p, items_t;
+  std::vector<Chunk> chunks;
+  const int per_item = std::max(1, item_size / 64);
   if (!sky) {
     const int nc = G.ncx;
     auto cost_of = [&](const Side &T) {
@@ -999,12 +1028,12 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
         return c;
       };
     };
-    make_items(SO, false, jk_on, item_size, cost_of(SI), items_p);
-    if (jk_on) make_items(SI, true, jk_on, item_size, cost_of(SO), items_t);
+    make_items(SO, false, jk_on, per_item, cost_of(SI), items_p, chunks);
+    if (jk_on) make_items(SI, true, jk_on, per_item, cost_of(SO), items_t, chunks);
   } else {
     auto flat = [](const Group &) { return 1.0; };
-    make_items(SO, false, jk_on, item_size, flat, items_p);
-    if (jk_on) make_items(SI, true, jk_on, item_size, flat, items_t);
+    make_items(SO, false, jk_on, per_item, flat, items_p, chunks);
+    if (jk_on) make_items(SI, true, jk_on, per_item, flat, items_t, chunks);
   }
 
   // ---- device tables and accumulators
@@ -1012,6 +1041,7 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
   if (check(ctx->thr.ensure(p.n_rp + 1), st, "alloc") ||
       check(ctx->nbr.ensure(std::max<size_t>(nbr.size(), 1)), st, "alloc") ||
       check(ctx->items.ensure(items_p.size() + items_t.size()), st, "alloc") ||
+      check(ctx->chunks.ensure(chunks.size()), st, "alloc") ||
       check(ctx->accp.ensure(nP * NSP * ncell), st, "alloc") || check(ctx->accpn.ensure(nP * ncell), st, "alloc") ||
       check(ctx->acct.ensure(nT * NST * ncell), st, "alloc") || check(ctx->acctn.ensure(nT * ncell), st, "alloc") ||
       check(ctx->counters.ensure(4), st, "alloc"))
@@ -1035,6 +1065,9 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
   check(hipMemcpyAsync(ctx->thr.p, thr.data(), thr.size() * 8, hipMemcpyHostToDevice, s), st, "copy");
   if (!nbr.empty())
     check(hipMemcpyAsync(ctx->nbr.p, nbr.data(), nbr.size() * sizeof(int2), hipMemcpyHostToDevice, s), st, "copy");
+  if (!chunks.empty())
+    check(hipMemcpyAsync(ctx->chunks.p, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, s), st,
+          "copy");
   if (!items_p.empty())
     check(hipMemcpyAsync(ctx->items.p, items_p.data(), items_p.size() * sizeof(Item), hipMemcpyHostToDevice, s), st,
           "copy");
@@ -1098,6 +1131,7 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
         A.phase_mode = jk_on ? 1 : 0;
         A.nslots = NSP;
         A.items = ctx->items.p;
+        A.chunks = ctx->chunks.p;
         A.bgrp = SO.d_groups.p;
         A.tgrp = SI.d_groups.p;
         A.tcol = SI.d_colbeg.p;
@@ -1113,6 +1147,7 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
         A.phase_mode = 2;
         A.nslots = NST;
         A.items = ctx->items.p + items_p.size();
+        A.chunks = ctx->chunks.p;
         A.bgrp = SI.d_groups.p;
         A.tgrp = SO.d_groups.p;
         A.tcol = SO.d_colbeg.p;
@@ -1148,6 +1183,7 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
         B.phase_mode = jk_on ? 1 : 0;
         B.nslots = NSP;
         B.items = ctx->items.p;
+        B.chunks = ctx->chunks.p;
         B.bgrp = SO.d_groups.p;
         B.tgrp = SI.d_groups.p;
         B.tcol = SI.d_colbeg.p;
@@ -1163,6 +1199,7 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
         B.phase_mode = 2;
         B.nslots = NST;
         B.items = ctx->items.p + items_p.size();
+        B.chunks = ctx->chunks.p;
         B.bgrp = SI.d_groups.p;
         B.tgrp = SO.d_groups.p;
         B.tcol = SO.d_colbeg.p;

@@ -46,9 +46,16 @@ struct alignas(8) Group {
   double zlo, zhi;     // z of the first and last record
 };
 
-// A unit of work: up to `count` galaxies of one block-side group, starting at record `first`.
+// One wave's block galaxies: `count` <= 64 consecutive records of one block-side group from `first`.
+struct alignas(16) Chunk {
+  int32_t group, first, count, pad;
+};
+
+// A unit of work (one workgroup): chunks [begin, begin + n) of the chunk table, all from groups of
+// the same jackknife region and coin, so that they share one histogram target; group0 is one of
+// those groups.
 struct alignas(16) Item {
-  int32_t group, first, count, target;
+  int32_t begin, n, target, group0;
 };
 
 // Histogram slots per pair kind.  The primary pass accumulates [wt, vals..., errs...], the touch
@@ -77,6 +84,7 @@ struct KArgs {
   float gA, gB, gC, gD;      // guard band of the rp bin index: g = gA + (gB + gC/r)/r + gD r, r = sqrt(s)
   const int2 *nbr;     // neighbour column offsets
   const Item *items;
+  const Chunk *chunks;
   const Group *bgrp;   // groups of the block side
   const Group *tgrp;   // groups of the tile side
   const int *tcol;     // [nc*nc+1] first group of each tile-side column
@@ -114,6 +122,7 @@ struct SkyArgs {
   double dsum;             // bound on the device/host difference of the great-circle cosine sum
   const double *thr;       // n_rp+1 thresholds on rp (host libm log10)
   const Item *items;
+  const Chunk *chunks;
   const Group *bgrp;
   const Group *tgrp;
   const int *tcol;

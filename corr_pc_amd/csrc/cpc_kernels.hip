@@ -66,8 +66,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
   for (int k = tid; k <= P.n_rp; k += blockDim.x) lthr[k] = P.thr[k];
 
   const Item it = P.items[blockIdx.x];
-  const Group bg = P.bgrp[it.group];
-  const int bcx = bg.col % P.nc, bcy = bg.col / P.nc;
+  const Group bg = P.bgrp[it.group0];  // region and coin of every chunk of the item
   // phase_mode: 0 primary without jackknife, 1 primary with jackknife, 2 touch pass
   const bool two_phase = (P.phase_mode == 1) && (bg.coin == 0);
   const int nphase = two_phase ? 2 : 1;
@@ -86,10 +85,12 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
     }
     __syncthreads();
 
-    const int nchunk = (it.count + 63) >> 6;
-    for (int ch = wave; ch < nchunk; ch += nwave) {
-      const int base = uni(it.first + ch * 64);
-      const int cnt = uni(min(64, it.first + it.count - base));
+    for (int ch = wave; ch < it.n; ch += nwave) {
+      const Chunk ck = P.chunks[it.begin + ch];
+      const int col = uni(P.bgrp[ck.group].col);
+      const int bcx = col % P.nc, bcy = col / P.nc;
+      const int base = uni(ck.first);
+      const int cnt = uni(ck.count);
       const bool valid = lane < cnt;
       const int p = base + (valid ? lane : 0);
       const PRec me = brec[p];

@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
   for (int k = tid; k <= P.n_rp; k += blockDim.x) lthr[k] = P.thr[k];
 
   const Item it = P.items[blockIdx.x];
-  const Group bg = P.bgrp[it.group];
+  const Group bg = P.bgrp[it.group0];  // region of every chunk of the item
   const int tgt = it.target;
   unsigned long long ncand = 0;
   for (int c = tid; c < ncell; c += blockDim.x) {
@@ -73,10 +73,10 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
   }
   __syncthreads();
 
-  const int nchunk = (it.count + 63) >> 6;
-  for (int ch = wave; ch < nchunk; ch += nwave) {
-    const int base = uni(it.first + ch * 64);
-    const int cnt = uni(min(64, it.first + it.count - base));
+  for (int ch = wave; ch < it.n; ch += nwave) {
+    const Chunk ck = P.chunks[it.begin + ch];
+    const int base = uni(ck.first);
+    const int cnt = uni(ck.count);
     const bool valid = lane < cnt;
     const int p = base + (valid ? lane : 0);
     const PRec me = brec[p];
