@@ -318,6 +318,7 @@ struct cpc_ctx {
   DevBuf<double> accp, acct;
   DevBuf<unsigned long long> accpn, acctn, counters;
   DevBuf<Win> win_raw;
+  DevBuf<double2> reach;
   DevBuf<uint2> hq;
   Side side_o, side_i;
   std::map<std::tuple<double, double, int, int, int>, std::vector<double>> thr_cache;
@@ -656,6 +657,7 @@ void cpc_destroy(cpc_ctx *ctx) {
   ctx->acctn.release();
   ctx->counters.release();
   ctx->win_raw.release();
+  ctx->reach.release();
   ctx->hq.release();
   ctx->side_o.release();
   ctx->side_i.release();
@@ -937,9 +939,19 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
     // per-galaxy windows of the outer catalog (lim_calc_sky), with the host libm
     const SkyTable T{p.sky_H, p.sky_dA, p.sky_n, p.sky_dz, p.z_max, p.z_sep_max};
     wins.resize(co->n);
-    double reach_ra = 0, reach_dec = 0, reach_zup = 0, reach_zdn = 0;
-    bool ra_full = false;
+    double reach_zup = 0, reach_zdn = 0;
     std::vector<double> dec_half;
+    // RA/Dec reach per bin of the outer redshift
+    double ozlo = INFINITY, ozhi = -INFINITY;
+    for (int64_t i = 0; i < co->n; ++i) {
+      if (!co->h_inc.empty() && co->h_inc[i] > p.include_prob_max) continue;
+      ozlo = std::min(ozlo, co->h_z[i]);
+      ozhi = std::max(ozhi, co->h_z[i]);
+    }
+    const int nb = 512;
+    if (!(ozlo <= ozhi)) ozlo = ozhi = 0.0;
+    const double rz0 = ozlo, rinv = nb / std::max(ozhi - ozlo, 1e-9);
+    std::vector<double2> reach(nb, make_double2(0.0, 0.0));
     dec_half.reserve(co->n);
     for (int64_t i = 0; i < co->n; ++i) {
       const double inc = co->h_inc.empty() ? 0.0 : co->h_inc[i];
@@ -952,12 +964,14 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
         return fail(ctx, CPC_ERR_INVALID,
                     "galaxy " + std::to_string(i) + ": its redshift window leaves the distance table or exceeds z_max "
                     "(the reference exits here, sky_calcs.cpp:109-113)");
+      const int b = (int)std::fmax(0.0, std::fmin((double)(nb - 1), std::floor((co->h_z[i] - rz0) * rinv)));
+      double2 &rb = reach[b];
       if (!(w.ramin <= w.ramax) || w.ramin <= 0 || w.ramax >= 360)
-        ra_full = true;
+        rb.x = INFINITY;
       else
-        reach_ra = std::max(reach_ra, std::max(co->h_x[i] - w.ramin, w.ramax - co->h_x[i]));
+        rb.x = std::max(rb.x, std::max(co->h_x[i] - w.ramin, w.ramax - co->h_x[i]) * (1 + 1e-12) + 1e-9);
       const double dh = std::max(co->h_y[i] - w.decmin, w.decmax - co->h_y[i]);
-      reach_dec = std::max(reach_dec, dh);
+      rb.y = std::max(rb.y, dh * (1 + 1e-12) + 1e-9);
       dec_half.push_back(dh);
       reach_zup = std::max(reach_zup, w.zmax - co->h_z[i]);
       reach_zdn = std::max(reach_zdn, co->h_z[i] - w.zmin);
@@ -986,11 +1000,14 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
     G.hy = span_dec / G.ncy * (1 + 1e-12);
     G.x0 = ralo;
     G.y0 = declo;
-    SA.reach_ra = reach_ra * (1 + 1e-12) + 1e-9;
-    SA.reach_dec = reach_dec * (1 + 1e-12) + 1e-9;
     SA.reach_zup = reach_zup * (1 + 1e-12) + 1e-12;
     SA.reach_zdn = reach_zdn * (1 + 1e-12) + 1e-12;
-    SA.ra_full = ra_full ? 1 : 0;
+    SA.reach_nb = nb;
+    SA.reach_z0 = rz0;
+    SA.reach_inv = rinv;
+    if (check(ctx->reach.ensure(nb), st, "alloc")) return fail(ctx, st);
+    check(hipMemcpyAsync(ctx->reach.p, reach.data(), nb * sizeof(double2), hipMemcpyHostToDevice, s), st, "copy");
+    SA.reach_tab = ctx->reach.p;
     if (check(ctx->win_raw.ensure(std::max<int64_t>(co->n, 1)), st, "alloc")) return fail(ctx, st);
     if (co->n)
       check(hipMemcpyAsync(ctx->win_raw.p, wins.data(), co->n * sizeof(Win), hipMemcpyHostToDevice, s), st, "copy");

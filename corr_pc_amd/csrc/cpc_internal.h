@@ -116,9 +116,13 @@ struct SkyArgs {
   // cell grid over RA/Dec (degrees)
   double ra0, dec0, hra, hdec;
   int nra, ndec;
-  // touch pass: how far an outer window can reach from its galaxy
-  double reach_ra, reach_dec, reach_zup, reach_zdn;
-  int ra_full;             // some outer window spans every RA
+  // touch pass: how far an outer window can reach from its galaxy: in z over all outer galaxies, in
+  // RA and Dec per bin of the outer galaxy's z ({ra, dec} half-widths, ra = inf for a full-circle
+  // window), bin b = floor((z - reach_z0) * reach_inv) clamped to [0, reach_nb)
+  double reach_zup, reach_zdn;
+  const double2 *reach_tab;
+  int reach_nb;
+  double reach_z0, reach_inv;
   double dsum;             // bound on the device/host difference of the great-circle cosine sum
   const double *thr;       // n_rp+1 thresholds on rp (host libm log10)
   const Item *items;

@@ -99,11 +99,20 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
     } else {
       zlo = wave_min(valid ? mez : INFINITY) - P.reach_zup;
       zhi = wave_max(valid ? mez : -INFINITY) + P.reach_zdn;
-      declo = wave_min(valid ? me.y : INFINITY) - P.reach_dec;
-      dechi = wave_max(valid ? me.y : -INFINITY) + P.reach_dec;
-      ra_all = P.ra_full != 0;
-      ralo = wave_min(valid ? me.x : INFINITY) - P.reach_ra;
-      rahi = wave_max(valid ? me.x : -INFINITY) + P.reach_ra;
+      // the widest RA/Dec reach of outer galaxies whose redshift lies in [zlo, zhi]
+      const int b0 = uni((int)fmax(0.0, fmin((double)(P.reach_nb - 1), floor((zlo - P.reach_z0) * P.reach_inv))));
+      const int b1 = uni((int)fmax(0.0, fmin((double)(P.reach_nb - 1), floor((zhi - P.reach_z0) * P.reach_inv))));
+      double rra = 0.0, rdec = 0.0;
+      for (int b = b0; b <= b1; ++b) {
+        const double2 t = P.reach_tab[b];
+        rra = fmax(rra, t.x);
+        rdec = fmax(rdec, t.y);
+      }
+      declo = wave_min(valid ? me.y : INFINITY) - rdec;
+      dechi = wave_max(valid ? me.y : -INFINITY) + rdec;
+      ra_all = __builtin_isinf(rra);
+      ralo = wave_min(valid ? me.x : INFINITY) - rra;
+      rahi = wave_max(valid ? me.x : -INFINITY) + rra;
     }
     const double eps = 1e-9;
     int cx0 = 0, cx1 = P.nra - 1;
