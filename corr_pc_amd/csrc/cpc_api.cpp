@@ -802,6 +802,61 @@ cpc_status cpc_rand_stream(uint32_t seed, int64_t skip, int64_t n, double *out) 
   return CPC_OK;
 }
 
+cpc_status cpc_bin_edges(const cpc_params *p, double *rp_edges, double *pi_edges) {
+  if (!p || p->n_rp <= 0 || p->n_pi <= 0) return CPC_ERR_INVALID;
+  if (!(p->rp_max > 0) || p->rp_min > p->rp_max) return CPC_ERR_INVALID;  // initialization.cpp:299-303
+  if (rp_edges) {
+    if (p->lin_bin) {
+      // bin_lin: b_size = float((rmax - rmin) / (float)n), running FP32 sum from rmin
+      const float b_size = (float)((p->rp_max - p->rp_min) / (float)p->n_rp);
+      float temp = (float)p->rp_min;
+      rp_edges[0] = temp;
+      for (int i = 0; i < p->n_rp; ++i) {
+        temp += b_size;
+        rp_edges[i + 1] = temp;
+      }
+    } else {
+      // bin_log: FP32 log10 limits and step, edges pow(10, running FP32 log)
+      if (!(p->rp_min > 0)) return CPC_ERR_INVALID;
+      float d = (float)std::log10(p->rp_min);
+      const float d_max = (float)std::log10(p->rp_max);
+      const float b_size = (d_max - d) / (float)p->n_rp;
+      rp_edges[0] = std::pow(10.0, (double)d);
+      for (int i = 0; i < p->n_rp; ++i) {
+        d += b_size;
+        rp_edges[i + 1] = std::pow(10.0, (double)d);
+      }
+    }
+  }
+  if (pi_edges) {
+    // P_bin_ini: p_size = float((p_max - p_min) / n_p_bin), running FP32 sum from p_min
+    const float p_size = (float)((p->pi_max - p->pi_min) / (p->n_pi));
+    float temp = (float)p->pi_min;
+    pi_edges[0] = temp;
+    for (int i = 0; i < p->n_pi; ++i) {
+      temp += p_size;
+      pi_edges[i + 1] = temp;
+    }
+  }
+  return CPC_OK;
+}
+
+cpc_status cpc_box_rr(const cpc_params *p, double *wt_num) {
+  if (!p || !wt_num || !(p->box_size > 0)) return CPC_ERR_INVALID;
+  std::vector<double> re(p->n_rp + 1), pe(p->n_pi + 1);
+  const cpc_status st = cpc_bin_edges(p, re.data(), pe.data());
+  if (st != CPC_OK) return st;
+  const double total_volume = std::pow(p->box_size, 3);
+  for (int i = 0; i < p->n_rp; ++i) {
+    const double area = M_PI * (re[i + 1] * re[i + 1] - re[i] * re[i]);
+    for (int k = 0; k < p->n_pi; ++k) {
+      const double volume = area * (pe[k + 1] - pe[k]);
+      wt_num[(size_t)i * p->n_pi + k] = volume / total_volume;
+    }
+  }
+  return CPC_OK;
+}
+
 cpc_status cpc_rp_thresholds(const cpc_params *p, double *thr) {
   if (!p || !thr || p->n_rp <= 0) return CPC_ERR_INVALID;
   std::vector<double> t;

@@ -199,12 +199,17 @@ def load_library(path=LIB_PATH):
     lib.cpc_rand_stream.restype = c_int
     lib.cpc_set_tuning.argtypes = [c_void_p, c_double, c_int, c_int]
     lib.cpc_set_tuning.restype = c_int
+    lib.cpc_bin_edges.argtypes = [POINTER(Params), _dp, _dp]
+    lib.cpc_bin_edges.restype = c_int
+    lib.cpc_box_rr.argtypes = [POINTER(Params), _dp]
+    lib.cpc_box_rr.restype = c_int
     _lib = lib
     return lib
 
 
 EXPORTED_SYMBOLS = ["cpc_init", "cpc_destroy", "cpc_last_error", "cpc_version", "cpc_catalog_upload",
-                    "cpc_catalog_free", "cpc_count", "cpc_rp_thresholds", "cpc_rand_stream", "cpc_set_tuning"]
+                    "cpc_catalog_free", "cpc_count", "cpc_rp_thresholds", "cpc_rand_stream", "cpc_set_tuning",
+                    "cpc_bin_edges", "cpc_box_rr"]
 
 
 REFERENCE_SEED = 124693517  # seed(), jk.cpp:40-43
@@ -217,6 +222,27 @@ def rand_stream(n, skip=0, seed=REFERENCE_SEED):
     rc = lib.cpc_rand_stream(seed, int(skip), int(n), out.ctypes.data_as(_dp))
     if rc != CPC_OK:
         raise EngineError(rc, "cpc_rand_stream failed")
+    return out
+
+
+def bin_edges(params):
+    """(rp_edges[n_rp+1], pi_edges[n_pi+1]): the reference's FP32-stepped output edges, see include/cpc.h."""
+    lib = load_library()
+    re = np.zeros(params.n_rp + 1, np.float64)
+    pe = np.zeros(params.n_pi + 1, np.float64)
+    rc = lib.cpc_bin_edges(ctypes.byref(params), re.ctypes.data_as(_dp), pe.ctypes.data_as(_dp))
+    if rc != CPC_OK:
+        raise EngineError(rc, "cpc_bin_edges failed")
+    return re, pe
+
+
+def box_rr(params):
+    """Analytic periodic-box RR wt_num[n_rp, n_pi] (RR_PB_calc); the reference sets num = 1."""
+    lib = load_library()
+    out = np.zeros((params.n_rp, params.n_pi), np.float64)
+    rc = lib.cpc_box_rr(ctypes.byref(params), out.ctypes.data_as(_dp))
+    if rc != CPC_OK:
+        raise EngineError(rc, "cpc_box_rr failed")
     return out
 
 

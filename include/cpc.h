@@ -158,6 +158,19 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *outer, const cpc_catalog *
  * and tests can check them against the host libm. */
 cpc_status cpc_rp_thresholds(const cpc_params *params, double *thr);
 
+/* The output bin edges corr_pc reports and uses for its analytic RR: bin_log / bin_lin and P_bin_ini
+ * (initialization.cpp:357-444) step the edges as FP32 running sums (log edges as pow(10, running FP32
+ * log10)); they are returned here as the doubles the reference stores.  rp_edges[n_rp + 1] and
+ * pi_edges[n_pi + 1]; bin k spans [edges[k], edges[k+1]].  These edges label the output only: the
+ * pair kernels bin with the FP64 arithmetic of corel_bin. */
+cpc_status cpc_bin_edges(const cpc_params *params, double *rp_edges, double *pi_edges);
+
+/* Analytic RR of a periodic box in rp-pi (RR_PB_calc, PB_calcs.cpp:63-85, coordinates 6), which the
+ * reference puts in place of the counted RR in box mode (bins_calcs.cpp:44-49, 71-77):
+ * wt_num[k*n_pi + q] = pi (b_max^2 - b_min^2) (p_max - p_min) / L^3 over the edges of cpc_bin_edges;
+ * the reference sets num = 1 in every bin. */
+cpc_status cpc_box_rr(const cpc_params *params, double *wt_num);
+
 /* The reference's per-galaxy random stream: out[k] = rand() / (RAND_MAX + 1.) for the draws numbered
  * skip .. skip+n-1 after srand(seed), with the host C library's rand().  corr_pc seeds 124693517 in
  * seed() (jk.cpp:40-43) and draws jk_prob (read_dat.cpp:243), a random phi (read_dat.cpp:307) and

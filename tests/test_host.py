@@ -12,7 +12,7 @@ import pytest
 
 from corr_pc_amd import synth
 from corr_pc_amd.engine import (EXPORTED_SYMBOLS, KIND_DD, KIND_SHAPE_DENSITY, KIND_SHAPE_SHAPE, LIB_PATH, Catalog,
-                                make_params, rand_stream, rp_thresholds)
+                                bin_edges, box_rr, make_params, rand_stream, rp_thresholds)
 from corr_pc_amd.passes import DataInfo, config_datainfo, setup_pass
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -133,3 +133,50 @@ def test_synthetic_config1_shapes_small():
     assert np.all(np.diff(data["z"]) >= 0)
     di = config_datainfo(synth.CONFIGS[1])
     assert (di.n_jk, di.pi_min, di.pi_max, di.n_rp, di.n_pi) == (27, -100.0, 100.0, 20, 40)
+
+
+@pytest.mark.parametrize("lin", [0, 1])
+def test_bin_edges_fp32_stepping(lin):
+    """Output edges as bin_log / bin_lin / P_bin_ini step them (initialization.cpp:357-444): FP32 running
+    sums, log edges as pow(10, running FP32 log10); restated here with numpy float32."""
+    p = make_params(250.0, 10, 0.0 if lin else 0.1, 30.0, 20, -60.0, 60.0, lin_bin=lin)
+    re_, pe = bin_edges(p)
+    f32 = np.float32
+    if lin:
+        b = f32((30.0 - 0.0) / float(f32(10)))
+        t = f32(0.0)
+        expect = [float(t)]
+        for _ in range(10):
+            t = f32(t + b)
+            expect.append(float(t))
+    else:
+        d = f32(math.log10(0.1))
+        b = f32((f32(math.log10(30.0)) - d) / f32(10))
+        expect = [10.0 ** float(d)]
+        for _ in range(10):
+            d = f32(d + b)
+            expect.append(10.0 ** float(d))
+    assert re_.tolist() == expect
+    ps = f32(120.0 / 20)
+    t = f32(-60.0)
+    pexp = [float(t)]
+    for _ in range(20):
+        t = f32(t + ps)
+        pexp.append(float(t))
+    assert pe.tolist() == pexp
+    # the edges are close to the exact ones: log edges within FP32 rounding of the exponent
+    assert np.allclose(re_, np.linspace(0, 30, 11) if lin else np.logspace(-1, math.log10(30), 11), rtol=1e-5)
+
+
+def test_box_rr_analytic():
+    """RR_PB_calc (PB_calcs.cpp:63-85): cylinder-shell volume over the box volume; the sum over all cells
+    is pi (rmax^2 - rmin^2)(pmax - pmin)/L^3 on the same edges."""
+    L = 250.0
+    p = make_params(L, 10, 0.1, 30.0, 20, -60.0, 60.0)
+    re_, pe = bin_edges(p)
+    rr = box_rr(p)
+    assert rr.shape == (10, 20) and np.all(rr > 0)
+    k, q = 7, 3
+    assert rr[k, q] == math.pi * (re_[k + 1] ** 2 - re_[k] ** 2) * (pe[q + 1] - pe[q]) / L ** 3
+    total = math.pi * (re_[-1] ** 2 - re_[0] ** 2) * (pe[-1] - pe[0]) / L ** 3
+    assert rr.sum() == pytest.approx(total, rel=1e-12)
