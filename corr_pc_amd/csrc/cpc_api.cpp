@@ -319,6 +319,7 @@ struct cpc_ctx {
   DevBuf<unsigned long long> accpn, acctn, counters;
   DevBuf<Win> win_raw;
   DevBuf<double2> reach;
+  DevBuf<uint8_t> aflags;
   DevBuf<uint2> hq;
   Side side_o, side_i;
   std::map<std::tuple<double, double, int, int, int>, std::vector<double>> thr_cache;
@@ -347,7 +348,8 @@ const std::vector<double> *cached_thresholds(cpc_ctx *ctx, const cpc_params &p, 
 // Build one side: keys, stable sort, gather, group tables.  `zfirst_plus_L` feeds the PBz>0 flag
 // (box only).  For the sky, `win_raw` (per original galaxy, outer side) is gathered as well.
 Status build_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, bool jk_on, const cpc_params &p,
-                  bool has_zfirst, double zfirst_plus_L, bool sky, const Win *win_raw, Side &S) {
+                  bool has_zfirst, double zfirst_plus_L, bool sky, const Win *win_raw, const uint8_t *aflags,
+                  Side &S) {
   Status st;
   hipStream_t s = ctx->stream;
   const int64_t n = c->n;
@@ -428,7 +430,7 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, b
       check(S.d_groups.ensure(S.groups.size()), st, "alloc") || check(S.d_colbeg.ensure(S.colbeg.size()), st, "alloc"))
     return st;
   GatherArgs ga{c->d_x, c->d_y, c->d_z, c->d_w, c->d_e1, c->d_e2, c->d_phi, ni, G.ncx, G.ncy, G.x0, G.y0, G.hx, G.hy,
-                p.box_size, p.pi_min, p.pi_max, p.rp_max, zfirst_plus_L, has_zfirst ? 1 : 0};
+                p.box_size, p.pi_min, p.pi_max, p.rp_max, zfirst_plus_L, has_zfirst ? 1 : 0, aflags};
   check(launch_gather(ga, ctx->vals_b.p, S.rec.p, S.zs.p, S.idx.p, s), st, "gather");
   if (sky) {
     if (check(S.sr.ensure(ni), st, "alloc") || check(S.win.ensure(ni), st, "alloc")) return st;
@@ -658,6 +660,7 @@ void cpc_destroy(cpc_ctx *ctx) {
   ctx->counters.release();
   ctx->win_raw.release();
   ctx->reach.release();
+  ctx->aflags.release();
   ctx->hq.release();
   ctx->side_o.release();
   ctx->side_i.release();
@@ -887,6 +890,11 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
   if (p.kind != CPC_KIND_DD && p.g_with_shape != 1 && p.g_with_shape != 2)
     return fail(ctx, CPC_ERR_INVALID, "g_with_shape must be 1 or 2 for shape kinds");
   if (p.do_jk != 0 && p.do_jk != 1) return fail(ctx, CPC_ERR_UNSUPPORTED, "do_jk=2 (region pairs) is not supported");
+  if (p.auto_pairs != 0 && p.auto_pairs != 1) return fail(ctx, CPC_ERR_INVALID, "auto_pairs must be 0 or 1");
+  const bool autop = p.auto_pairs == 1;
+  if (autop && co != ci) return fail(ctx, CPC_ERR_INVALID, "the auto pass (corels1) takes one catalog on both sides");
+  if (autop && co->n > 0 && !co->z_sorted)
+    return fail(ctx, CPC_ERR_INVALID, "the auto pass needs the catalog sorted by z (data_sorted=1, corr.cpp:129-132)");
   if (p.precision != CPC_PREC_SPLIT && p.precision != CPC_PREC_FP64)
     return fail(ctx, CPC_ERR_INVALID, "unknown precision");
   const bool jk_on = p.do_jk == 1 && p.n_jk > 0;
@@ -990,6 +998,24 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
       zfirst_plus_L = ci->h_z[j] + L * 1;
       break;
     }
+    if (autop && co->n > 0) {
+      // corels1 scans for PBz = 0 and PBz = -1 start at j = i+1 and stop at once when that next
+      // included galaxy lies below the window (do_pair_sorted, corr.cpp:71-73): bit 0 / bit 1
+      std::vector<uint8_t> af(co->n, 0);
+      int64_t nxt = -1;
+      for (int64_t i = co->n - 1; i >= 0; --i) {
+        if (nxt >= 0) {
+          const double z_min = co->h_z[i] + p.pi_min;
+          if (!(co->h_z[nxt] + L * 0 < z_min)) af[i] |= 1;
+          if (!(co->h_z[nxt] + L * -1 < z_min)) af[i] |= 2;
+        }
+        if (!(!co->h_inc.empty() && co->h_inc[i] > p.include_prob_max)) nxt = i;
+      }
+      if (check(ctx->aflags.ensure(co->n), st, "alloc")) return fail(ctx, st);
+      check(hipMemcpyAsync(ctx->aflags.p, af.data(), co->n, hipMemcpyHostToDevice, s), st, "copy");
+      check(hipStreamSynchronize(s), st, "sync");
+      if (st.code) return fail(ctx, st);
+    }
   } else {
     // per-galaxy windows of the outer catalog (lim_calc_sky), with the host libm
     const SkyTable T{p.sky_H, p.sky_dA, p.sky_n, p.sky_dz, p.z_max, p.z_sep_max};
@@ -1063,6 +1089,19 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
     if (check(ctx->reach.ensure(nb), st, "alloc")) return fail(ctx, st);
     check(hipMemcpyAsync(ctx->reach.p, reach.data(), nb * sizeof(double2), hipMemcpyHostToDevice, s), st, "copy");
     SA.reach_tab = ctx->reach.p;
+    if (autop) {
+      // corels1's scan from j = i+1 stops at once when that next included galaxy lies below the
+      // window (corr.cpp:168-186): such outer galaxies take no pairs
+      int64_t nxt = -1;
+      for (int64_t i = co->n - 1; i >= 0; --i) {
+        const bool inc_i = !(!co->h_inc.empty() && co->h_inc[i] > p.include_prob_max);
+        if (inc_i && (nxt < 0 || co->h_z[nxt] < wins[i].zmin)) {
+          wins[i].zmin = 1.0;
+          wins[i].zmax = 0.0;
+        }
+        if (inc_i) nxt = i;
+      }
+    }
     if (check(ctx->win_raw.ensure(std::max<int64_t>(co->n, 1)), st, "alloc")) return fail(ctx, st);
     if (co->n)
       check(hipMemcpyAsync(ctx->win_raw.p, wins.data(), co->n * sizeof(Win), hipMemcpyHostToDevice, s), st, "copy");
@@ -1072,11 +1111,11 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
   if ((double)G.ncx * G.ncy * nreg * 2.0 >= 4294967295.0)
     return fail(ctx, CPC_ERR_UNSUPPORTED, "too many cells x jackknife regions for 32-bit sort keys");
   st = build_side(ctx, co, G, nreg, jk_on, p, has_zfirst, zfirst_plus_L, sky, sky ? ctx->win_raw.p : nullptr,
-                  ctx->side_o);
+                  (autop && !sky) ? ctx->aflags.p : nullptr, ctx->side_o);
   if (st.code) return fail(ctx, st);
   const bool same = (co == ci);
   if (!same) {
-    st = build_side(ctx, ci, G, nreg, jk_on, p, has_zfirst, zfirst_plus_L, sky, nullptr, ctx->side_i);
+    st = build_side(ctx, ci, G, nreg, jk_on, p, has_zfirst, zfirst_plus_L, sky, nullptr, nullptr, ctx->side_i);
     if (st.code) return fail(ctx, st);
   }
   Side &SO = ctx->side_o;
@@ -1197,6 +1236,7 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
       K.gD = gr.gD;
       K.nbr = ctx->nbr.p;
       K.counters = ctx->counters.p;
+      K.auto_mode = autop ? 1 : 0;
       if (!items_p.empty()) {
         KArgs A = K;
         A.n_items = (int)items_p.size();
@@ -1207,6 +1247,8 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
         A.bgrp = SO.d_groups.p;
         A.tgrp = SI.d_groups.p;
         A.tcol = SI.d_colbeg.p;
+        A.bidx = SO.idx.p;
+        A.tidx = SI.idx.p;
         A.acc = ctx->accp.p;
         A.accnum = ctx->accpn.p;
         check(launch_pair(kind_code, use_split, false, A, threads, SO.rec.p, SO.zs.p, SI.rec.p, SI.zs.p, s), st,
@@ -1223,6 +1265,8 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
         A.bgrp = SI.d_groups.p;
         A.tgrp = SO.d_groups.p;
         A.tcol = SO.d_colbeg.p;
+        A.bidx = SI.idx.p;
+        A.tidx = SO.idx.p;
         A.acc = ctx->acct.p;
         A.accnum = ctx->acctn.p;
         check(launch_pair(kind_code, use_split, true, A, threads, SI.rec.p, SI.zs.p, SO.rec.p, SO.zs.p, s), st,
@@ -1245,6 +1289,7 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
       A.nra = G.ncx;
       A.ndec = G.ncy;
       A.dsum = 4e-15;
+      A.auto_mode = autop ? 1 : 0;
       A.thr = ctx->thr.p;
       A.counters = ctx->counters.p;
       A.hq = ctx->hq.p;
@@ -1282,7 +1327,7 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
       }
     }
     check(hipEventRecord(ctx->ev1, s), st, "event");
-    if (!sky && co->n > 0 && ci->n > 0) {  // sparse-window diagnostic
+    if (!sky && !autop && co->n > 0 && ci->n > 0) {  // sparse-window diagnostic (corels2's starts pointer)
       SparseArgs sa{co->d_z, co->d_inc, co->n, ci->z_sorted ? ci->d_z : ci->d_zsorted, ci->n,
                     p.box_size, p.pi_min, p.pi_max, p.include_prob_max};
       check(launch_sparse(sa, ctx->counters.p + 2, s), st, "sparse");

@@ -201,7 +201,7 @@ __device__ __forceinline__ bool image_allowed(int meta, int PBx, int PBy, int PB
   bool ok = true;
   ok &= (PBx == 0) || (PBx < 0 ? (meta & kMetaXm) : (meta & kMetaXp));
   ok &= (PBy == 0) || (PBy < 0 ? (meta & kMetaYm) : (meta & kMetaYp));
-  ok &= (PBz == 0) || (PBz < 0 ? (meta & kMetaZm) : (meta & kMetaZp));
+  ok &= ((PBz == 0 ? kMetaZ0 : PBz < 0 ? kMetaZm : kMetaZp) & meta) != 0;
   return ok;
 }
 

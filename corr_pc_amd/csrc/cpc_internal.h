@@ -36,6 +36,8 @@ enum : int32_t {
   kMetaYp = 8,
   kMetaZm = 16,
   kMetaZp = 32,  // z_max > L and the sweep from j = 0 does not stop at once (corr.cpp:389-408)
+  kMetaZ0 = 64,  // PBz = 0 visited: always in a cross pass; in an auto pass when the scan from
+                 // j = i+1 does not stop at once (corr.cpp:168-186)
 };
 
 // One run of galaxies sharing (column, jackknife region, coin); z-sorted inside.
@@ -72,6 +74,9 @@ struct KArgs {
   int gws, n_nbr, n_items;
   int phase_mode;      // 0 primary pass without jackknife, 1 primary with jackknife, 2 touch pass
   int nslots;          // f64 slots accumulated by this pass
+  int auto_mode;       // corels1 pass: pairs through PBz <= 0 images only with the inner galaxy after
+                       // the outer one (z, then catalog index)
+  const int32_t *bidx, *tidx;  // catalog index of the block-side and tile-side records
   int lin;
   // split-precision classification
   float pioff32, inv_psize32, gz;  // pi bin: u = dz*inv_psize + pioff (pioff = -pmin/psize), guard gz in bin units
@@ -124,6 +129,7 @@ struct SkyArgs {
   int reach_nb;
   double reach_z0, reach_inv;
   double dsum;             // bound on the device/host difference of the great-circle cosine sum
+  int auto_mode;           // corels1 pass: pairs only with the inner galaxy after the outer one
   const double *thr;       // n_rp+1 thresholds on rp (host libm log10)
   const Item *items;
   const Chunk *chunks;
@@ -158,6 +164,9 @@ struct GatherArgs {
   double L, pmin, pmax, rmax;
   double zfirst_plus_L;  // fl(z_first + L) of the inner catalog's first included galaxy
   int has_zfirst;
+  // auto pass: per catalog galaxy, bit 0 = the PBz = 0 scan from j = i+1 does not stop at once,
+  // bit 1 = nor does the PBz = -1 scan (nullptr in a cross pass)
+  const uint8_t *aflags;
 };
 
 struct SparseArgs {

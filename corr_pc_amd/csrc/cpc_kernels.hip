@@ -95,6 +95,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
       const int p = base + (valid ? lane : 0);
       const PRec me = brec[p];
       const double mz = bz[p];
+      const int midx = P.auto_mode ? P.bidx[p] : 0;
       const double wzlo = bz[base];
       const double wzhi = bz[base + cnt - 1];
 
@@ -137,6 +138,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
             // image of the inner galaxy in the reference's terms
             const int PBx = SWAP ? -sx : sx, PBy = SWAP ? -sy : sy, PBz = SWAP ? -sz : sz;
             const bool lane_ok = valid && (SWAP || image_allowed(me.meta, PBx, PBy, PBz));
+            const bool ordered = P.auto_mode && PBz <= 0;  // the inner galaxy must come after the outer
             // FP32 offsets of this lane to the tile column: d = tile local + offset
             const float ax = Ox - me.xl, ay = Oy - me.yl, az = (float)(P.L * sz) - me.zf;
             const float pinv = SWAP ? -P.inv_psize32 : P.inv_psize32;  // pi = -dz in the touch pass
@@ -155,6 +157,19 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
                 const int l3 = 4 * t + 3;  // lane of the quarter {xl, yl, zf, meta}
                 bool ok = lane_ok;
                 if (SWAP) ok = ok && image_allowed(rl_i(cq.w, l3), PBx, PBy, PBz);
+                if (ordered) {
+                  // corels1 scans j > i in catalog order, which is (z, index) order for its z-sorted input
+                  const double tzo = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(cz), t),
+                                                      __builtin_amdgcn_readlane(__double2loint(cz), t));
+                  const double zo = SWAP ? tzo : mz, zi = SWAP ? mz : tzo;
+                  bool after = zi > zo;
+                  const bool tie = zi == zo;
+                  if (__ballot(ok && tie) != 0ull) {
+                    const int tix = P.tidx[jb + t];
+                    after = after || (tie && (SWAP ? midx > tix : tix > midx));
+                  }
+                  ok = ok && after;
+                }
                 int cell = -1;
                 bool unc = ok;
                 if (SPLIT && ok) {
@@ -289,12 +304,14 @@ __global__ void k_gather(GatherArgs A, const int32_t *__restrict__ sidx, PRec *_
   const double z_max = z + A.pmax, z_min = z + A.pmin;
   const double ra_max = r.x + A.rmax, ra_min = r.x - A.rmax;
   const double dec_max = r.y + A.rmax, dec_min = r.y - A.rmax;
+  const int af = A.aflags ? A.aflags[i] : 3;
   int m = 0;
   if (ra_min < 0) m |= kMetaXm;
   if (ra_max > A.L) m |= kMetaXp;
   if (dec_min < 0) m |= kMetaYm;
   if (dec_max > A.L) m |= kMetaYp;
-  if (z_min < 0) m |= kMetaZm;
+  if (z_min < 0 && (af & 2)) m |= kMetaZm;
+  if (af & 1) m |= kMetaZ0;
   // PBz>0 sweeps from j = 0 and stops at the first included galaxy outside the window (corr.cpp:407-416)
   if (z_max > A.L && A.has_zfirst && !(A.zfirst_plus_L < z_min)) m |= kMetaZp;
   r.meta = m;

@@ -158,6 +158,12 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
             const double piz = SWAP ? mez : tzj;
             bool ok = valid && !(piz > w.zmax || piz < w.zmin) && !(pi.y > w.decmax || pi.y < w.decmin) &&
                       !(pi.x < w.ramin || pi.x > w.ramax);
+            if (P.auto_mode) {
+              // corels1 pairs: the inner galaxy after the outer one in catalog ((z, index)) order
+              const double zo = SWAP ? tzj : mez;
+              const int io = SWAP ? T.idx[j] : B.idx[p], ii = SWAP ? B.idx[p] : T.idx[j];
+              ok = ok && (piz > zo || (piz == zo && ii > io));
+            }
             ncand += ok;
             int cell = -1;
             if (ok) {

@@ -103,6 +103,11 @@ typedef struct cpc_params {
   double sky_dz;
   double z_max;
   double z_sep_max;
+  /* 0: the cross pass of corels2 (corr.cpp:219-459).  1: the auto pass of corels1 (corr.cpp:103-216)
+   * over one z-sorted catalog (outer and inner the same handle): each unordered pair once, found from
+   * the lower galaxy i through images with PBz <= 0 (scan from j = i+1) or from any galaxy through
+   * PBz = +1 (scan from j = 0), each scan ending at its first included galaxy outside the z window. */
+  int auto_pairs;
 } cpc_params;
 
 /* Output histograms, caller-allocated host memory, overwritten by cpc_count.

@@ -4,6 +4,9 @@
  * A plain-C restatement of corr_pc's cross-pair loop, written from the reference's behaviour (not
  * its text).  It follows, step for step:
  *   corels2 outer loop, image loops and inner sweep    corr.cpp:219-459 (do_pair_sorted corr.cpp:67-86)
+ *   corels1 (auto pairs, params->auto_pairs)           corr.cpp:103-216: scans from j = i+1, or j = 0
+ *                                                      for PBz > 0, each ending at the first included
+ *                                                      galaxy outside the z window
  *   per-outer window and image flags (box)             PB_calcs.cpp:12-61 (lim_calc_xyz, lim_calc_PB)
  *   per-outer window (sky)                             sky_calcs.cpp:33-81, 105-190 (lim_calc_sky)
  *   pi and rp of a pair                                PB_calcs.cpp:87-98; calcs.cpp:126-132;
@@ -109,6 +112,7 @@ typedef struct {
   int geom;
   double L, pmin, pmax, psize, rmin, rmax, rsize;
   int n_rp, n_pi, lin, kind, gws, est, which_corr, jk_on;
+  int autop; /* corels1 auto pass */
   double inc_max;
   /* sky */
   const double *H, *dA;
@@ -338,6 +342,7 @@ static int make_params(const cpc_params *p, oparm *P) {
   P->dz = p->sky_dz;
   P->z_max = p->z_max;
   P->z_sep_max = p->z_sep_max;
+  P->autop = p->auto_pairs != 0;
   if (P->geom == CPC_GEOM_SKY && (!P->H || !P->dA || P->ntab < 2 || !(P->dz > 0))) return CPC_ERR_INVALID;
   return CPC_OK;
 }
@@ -362,6 +367,7 @@ int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_
       if (g2->jk[j] < 0 || g2->jk[j] >= params->n_jk) return CPC_ERR_INVALID;
   }
   const int sky = P.geom == CPC_GEOM_SKY;
+  if (P.autop && (g1->n != g2->n || g1->z != g2->z)) return CPC_ERR_INVALID; /* one catalog */
   if (sky && ((n1 && (!g1->dc || !g1->da)) || (n2 && (!g2->dc || !g2->da)))) return CPC_ERR_INVALID;
   /* the shape kinds read e[] of the shape side (both sides for shape_shape); the reference would
    * dereference an unallocated pointer here (corels.cpp:216-219, read_dat.cpp:126) */
@@ -401,8 +407,13 @@ int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_
             tbad[t] = 1;
             continue;
           }
-          int64_t j = lower_bound_(g2->z, n2, 0.0, w[0]);
-          if (j < n2 && g2->z[j] > w[1]) sparse++;
+          int64_t j;
+          if (P.autop) {
+            j = i + 1; /* corels1: scan from j = i+1 (corr.cpp:168) */
+          } else {
+            j = lower_bound_(g2->z, n2, 0.0, w[0]);
+            if (j < n2 && g2->z[j] > w[1]) sparse++;
+          }
           for (; j < n2; j++) {
             if (cinc(g2, j) > P.inc_max) continue;
             double zz = g2->z[j];
@@ -437,6 +448,8 @@ int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_
               int64_t j;
               if (cz > 0) {
                 j = 0;
+              } else if (P.autop) {
+                j = i + 1; /* corels1: PBz <= 0 scans from j = i+1 (corr.cpp:168-176) */
               } else {
                 j = lower_bound_(g2->z, n2, sz, z_min);
                 if (j < n2 && g2->z[j] + sz > z_max) sparse++;

@@ -84,6 +84,14 @@ def images(zi, xi, yi, L, pmin, pmax, rmax):
 
 
 def count(outer, inner, p):
+    """corels2(outer, inner) after the swap; with p.auto_pairs, corels1 over `outer` (inner ignored):
+    each pair (i, j) through an image with PBz <= 0 only for j after i, and an image only when its
+    scan would not stop at once, i.e. when the first included galaxy the scan meets (the next one
+    after i for PBz <= 0, the first of the catalog for PBz > 0) lies at or above the window's lower
+    edge (corr.cpp:159-190 with do_pair_sorted's break, corr.cpp:71-73)."""
+    auto = bool(getattr(p, "auto_pairs", 0))
+    if auto:
+        inner = outer
     L = p.box_size
     pmin, pmax, rmin, rmax = p.pi_min, p.pi_max, p.rp_min, p.rp_max
     psize = (pmax - pmin) / p.n_pi
@@ -109,11 +117,16 @@ def count(outer, inner, p):
             continue
         xi, yi, zi = float(outer.x[i]), float(outer.y[i]), float(outer.z[i])
         zmin, zmax = zi + pmin, zi + pmax
+        nxt = next((j for j in range(i + 1, inner.n) if not inc_i[j] > p.include_prob_max), None) if auto else None
         for (PBx, PBy, PBz) in images(zi, xi, yi, L, pmin, pmax, rmax):
             if PBz == 1 and (first is None or float(inner.z[first]) + L * 1 < zmin):
                 continue
+            if auto and PBz <= 0 and (nxt is None or float(inner.z[nxt]) + L * PBz < zmin):
+                continue
             for j in range(inner.n):
                 if inc_i[j] > p.include_prob_max:
+                    continue
+                if auto and PBz <= 0 and j <= i:
                     continue
                 xj, yj, zj = float(inner.x[j]), float(inner.y[j]), float(inner.z[j])
                 zz = zj + L * PBz
@@ -199,7 +212,11 @@ def sky_window(p, ra, dec, z):
 
 
 def count_sky(outer, inner, p):
-    """Every ordered (outer, inner) pair of the sky pass, no z-sorted sweep."""
+    """Every ordered (outer, inner) pair of the sky pass, no z-sorted sweep; with p.auto_pairs, the
+    corels1 pairs (i, j > i) of `outer`, for i whose next included galaxy is not below its window."""
+    auto = bool(getattr(p, "auto_pairs", 0))
+    if auto:
+        inner = outer
     pmin, pmax, rmin, rmax = p.pi_min, p.pi_max, p.rp_min, p.rp_max
     psize = (pmax - pmin) / p.n_pi
     rsize = (rmax - rmin) / p.n_rp if p.lin_bin else (math.log10(rmax) - math.log10(rmin)) / p.n_rp
@@ -219,8 +236,15 @@ def count_sky(outer, inner, p):
             continue
         rai, deci, zi = float(outer.x[i]), float(outer.y[i]), float(outer.z[i])
         zmin, zmax, ramin, ramax, decmin, decmax = sky_window(p, rai, deci, zi)
+        if auto:
+            nxt = next((j for j in range(i + 1, inner.n) if not g(inner, "include_prob", j, 0.0) > p.include_prob_max),
+                       None)
+            if nxt is None or float(inner.z[nxt]) < zmin:
+                continue
         for j in range(inner.n):
             if g(inner, "include_prob", j, 0.0) > p.include_prob_max:
+                continue
+            if auto and j <= i:
                 continue
             raj, decj, zj = float(inner.x[j]), float(inner.y[j]), float(inner.z[j])
             if zj > zmax or zj < zmin or decj > decmax or decj < decmin or raj < ramin or raj > ramax:
