@@ -43,6 +43,93 @@ def measured_traffic():
     return None
 
 
+def bench_mocks(args):
+    """The batched-mocks workload: independent wgg boxes (L = 250, data sizes log-uniform in
+    [1e4, 5e5], clustered, 2x randoms, 2x2x2 jackknife; 10 log rp bins in [0.1, 30), 20 pi bins in
+    [-60, 60)), each with its DD, DR and RR passes (SD, SRd, RsRd), all in one cpc_count_batch call
+    with the autotuned decomposition, against the same passes as sequential cpc_count calls."""
+    import numpy as np
+
+    from corr_pc_amd import synth
+    from corr_pc_amd.engine import PREC_SPLIT, Engine
+    from corr_pc_amd.passes import config_datainfo, setup_pass
+
+    t = time.perf_counter()
+    sizes = synth.mock_sizes(args.mocks)
+    di = config_datainfo(synth.MOCK_BINNING)
+    jobs, tune_jobs = [], []
+    for k, n in enumerate(sizes):
+        c = synth.mock_catalogs(k, int(n))
+        mine = []
+        for name, g1, g2 in (("SD", c["D"], c["S"]), ("SRd", c["Rd"], c["S"]), ("RsRd", c["Rd"], c["Rs"])):
+            outer, inner, p, _ = setup_pass(name, g1, g2, di, precision=PREC_SPLIT)
+            mine.append((outer, inner, p))
+        jobs += mine
+        if k % args.tune_stride == 0:
+            tune_jobs += mine
+        if (k + 1) % 32 == 0:
+            log(f"mocks generated: {k + 1}/{len(sizes)} ({time.perf_counter() - t:.1f}s)")
+    log(f"{len(sizes)} mocks, {int(sizes.sum())} data galaxies, {len(jobs)} passes ready in "
+        f"{time.perf_counter() - t:.1f}s")
+    eng = Engine(0)
+    for o, i, _ in jobs:
+        eng.upload(o)
+        eng.upload(i)
+    t0 = time.perf_counter()
+    tun, tried, _ = eng.autotune(tune_jobs)
+    log(f"autotune: {tun} after {tried} batches of {len(tune_jobs)} passes in {time.perf_counter() - t0:.1f}s")
+
+    for k in range(args.warmup):
+        _, tot = eng.count_batch(jobs)
+        log(f"warmup batch {k}: {tot['ms_total']:.0f} ms (device {tot['ms_kernel']:.0f} ms)")
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        res, tot = eng.count_batch(jobs)
+        log(f"batch step {k}: {tot['ms_total']:.0f} ms (device {tot['ms_kernel']:.0f} ms, build "
+            f"{tot['ms_build']:.0f} ms), binned {tot['n_binned']}")
+    dt = (time.perf_counter() - t0) / args.steps
+    n_binned = tot["n_binned"]
+
+    # the same passes one call at a time
+    t0 = time.perf_counter()
+    seq_dev = 0.0
+    identical = True
+    for (o, i, p), (h, _) in zip(jobs, res):
+        hs, st = eng.count(o, i, p)
+        seq_dev += st["ms_kernel"]
+        identical = identical and bool(np.array_equal(hs.num, h.num))
+    dt_seq = time.perf_counter() - t0
+    log(f"sequential: {dt_seq * 1e3:.0f} ms (device {seq_dev:.0f} ms), counts identical: {identical}")
+
+    out = {
+        "metric": "pair evals/sec: batched wgg mocks in one call vs sequential calls",
+        "value": n_binned / dt,
+        "unit": "pair evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(sizes)} wgg boxes L=250, data log-uniform in [1e4, 5e5] (clustered), 2x "
+                               f"randoms, 8 jackknife regions, DD+DR+RR passes, 10 log rp bins [0.1,30) x 20 pi "
+                               f"bins [-60,60)",
+                   "global_batch": int(sizes.sum()), "parallelism": "single GPU"},
+        "n_binned": n_binned,
+        "n_passes": len(jobs),
+        "tuning": tun,
+        "batch_device_ms": tot["ms_kernel"],
+        "sequential": {"value": n_binned / dt_seq, "ms": dt_seq * 1e3, "device_ms": seq_dev,
+                       "counts_identical": identical},
+        "batch_vs_sequential": dt_seq / dt,
+    }
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +139,10 @@ def main():
                     help="CPU baseline: every n-th outer galaxy of the same pass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--workload", choices=("wgplus", "mocks"), default="wgplus",
+                    help="wgplus: the 1M-galaxy wg+/w++ box (default); mocks: batched wgg mocks")
+    ap.add_argument("--mocks", type=int, default=256, help="number of mocks of the mocks workload")
+    ap.add_argument("--tune-stride", type=int, default=16, help="mocks workload: autotune on every n-th mock")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -69,6 +160,10 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is described "
               f"in DESIGN.md and is not launched.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "mocks":
+        bench_mocks(args)
+        return
 
     import numpy as np
 

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -296,6 +297,30 @@ struct cpc_catalog {
   std::vector<int32_t> h_jk;
 };
 
+// Device state of one prepared pass, kept in the context's pool (entry i serves job i of a call) and
+// reused by later calls.
+struct JobBufs {
+  Side so, si;  // outer and inner side (si unused when both are one catalog)
+  DevBuf<double> thr;
+  DevBuf<int2> nbr;
+  DevBuf<Chunk> chunks;
+  DevBuf<uint8_t> aflags;
+  DevBuf<Win> win_raw;
+  DevBuf<double2> reach;
+  DevBuf<uint2> hq;
+  void release() {
+    so.release();
+    si.release();
+    thr.release();
+    nbr.release();
+    chunks.release();
+    aflags.release();
+    win_raw.release();
+    reach.release();
+    hq.release();
+  }
+};
+
 struct cpc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -307,21 +332,17 @@ struct cpc_ctx {
   size_t sort_tmp_bytes = 0;
   double col_frac = 0.0;
   int item_size = 0, block_threads = 0;
-  // scratch
+  // scratch of the cell-list construction (sides are built one after another)
   DevBuf<uint32_t> keys_a, keys_b;
   DevBuf<int32_t> vals_a, vals_b, order;
   DevBuf<double> zk_b;
-  DevBuf<double> thr;
-  DevBuf<int2> nbr;
+  // per call: work items of every launch, box pass arguments (two per job), and the accumulator
+  // arenas (f64 histogram slots; counts and counters)
   DevBuf<Item> items;
-  DevBuf<Chunk> chunks;
-  DevBuf<double> accp, acct;
-  DevBuf<unsigned long long> accpn, acctn, counters;
-  DevBuf<Win> win_raw;
-  DevBuf<double2> reach;
-  DevBuf<uint8_t> aflags;
-  DevBuf<uint2> hq;
-  Side side_o, side_i;
+  DevBuf<KArgs> jobs;
+  DevBuf<double> acc;
+  DevBuf<unsigned long long> accn;
+  std::vector<std::unique_ptr<JobBufs>> pool;
   std::map<std::tuple<double, double, int, int, int>, std::vector<double>> thr_cache;
 };
 
@@ -448,10 +469,10 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, b
 // Work items over the block side: every group is cut into chunks of up to 64 galaxies (one wave
 // each); chunks sharing a histogram target are sorted by cost and packed `per_item` at a time, and
 // the items run most expensive first.  `nbr_cost` gives, for a group, the number of tile galaxies
-// it may meet.  Chunks are appended to `chunks` in item order.
+// it may meet.  Chunks are appended to `chunks` in item order; `costs` receives each item's cost.
 template <typename Cost>
 void make_items(const Side &B, bool touch, bool jk_on, int per_item, Cost nbr_cost, std::vector<Item> &items,
-                std::vector<Chunk> &chunks) {
+                std::vector<double> &costs, std::vector<Chunk> &chunks) {
   std::map<int, std::vector<std::pair<double, Chunk>>> by_target;
   for (size_t g = 0; g < B.groups.size(); ++g) {
     const Group &G = B.groups[g];
@@ -490,8 +511,10 @@ void make_items(const Side &B, bool touch, bool jk_on, int per_item, Cost nbr_co
   }
   std::stable_sort(packed.begin(), packed.end(), [](const Packed &a, const Packed &b) { return a.cost > b.cost; });
   items.clear();
+  costs.clear();
   for (const Packed &pk : packed) {
-    items.push_back(Item{(int32_t)chunks.size(), (int32_t)pk.cs.size(), pk.target, pk.cs[0].group});
+    items.push_back(Item{(int32_t)chunks.size(), (int32_t)pk.cs.size(), pk.target, pk.cs[0].group, 0, 0, 0, 0});
+    costs.push_back(pk.cost);
     chunks.insert(chunks.end(), pk.cs.begin(), pk.cs.end());
   }
 }
@@ -606,6 +629,853 @@ void host_sky_pair(const cpc_params &p, const Derived &dv, const cpc_catalog *g1
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Passes.  One pass (a cpc_count call, or one job of cpc_count_batch) is checked and prepared on
+// the host into a Plan: the derived binning, rp thresholds, cell grid, both sides' records on the
+// device, the work items and the kernel arguments.  Box plans then run together, one launch per
+// (kind, precision, pass direction) over the items of all of them; sky plans run one at a time.
+// Each plan's histograms are assembled into the reference layout last.
+
+struct Plan {
+  // arguments
+  const cpc_catalog *co = nullptr, *ci = nullptr;
+  cpc_params p{};
+  cpc_hist *out = nullptr;
+  cpc_stats *stats = nullptr;
+  // derived
+  bool sky = false, jk_on = false, autop = false, split = false, use_split = false;
+  int R = 1, nreg = 1, ncell = 0, kind_code = 0, NV = 0, NSP = 1, NST = 1, threads = 256;
+  Derived dv{};
+  const std::vector<double> *thr = nullptr;
+  // device state and the host arrays copied into it (alive until the call's last synchronisation)
+  JobBufs *B = nullptr;
+  Side *SO = nullptr, *SI = nullptr;
+  std::vector<int2> nbr;
+  std::vector<Chunk> chunks;
+  std::vector<uint8_t> aflags;
+  std::vector<Win> wins;
+  std::vector<double2> reach;
+  // work
+  std::vector<Item> items_p, items_t;
+  std::vector<double> cost_p, cost_t;
+  KArgs kp{}, kt{};  // box: primary and touch pass (accumulator pointers set at launch)
+  SkyArgs sa{};      // sky: arguments shared by both passes
+  int hq_cap = 1;
+  // accumulators: element offsets into the context's arenas
+  size_t nP = 0, nT = 0;
+  size_t off_p = 0, off_t = 0, off_pn = 0, off_tn = 0, off_c = 0;
+  // results
+  double ms_build = 0;
+  float kms = 0, kms_p = 0, kms_t = 0;
+  std::vector<uint2> hq_pairs;
+};
+
+// Pair-kind code of the kernels and the data[] fields each kind fills (corels.cpp:48-244):
+// 0 DD, 1 shape-density, 2 shape-shape estimator 0, 3 shape-shape other estimators.
+const int kNV[4] = {0, 3, 5, 6};
+const int kValField[4][6] = {{0}, {1, 2, 6}, {1, 3, 4, 5, 6}, {1, 2, 3, 4, 5, 6}};
+
+// The reference's own preconditions and the engine's, checked before any device work.
+Status validate(const cpc_job &job, Plan &P) {
+  P.co = job.outer;
+  P.ci = job.inner;
+  P.out = job.out;
+  P.stats = job.stats;
+  if (!P.co || !P.ci || !job.params || !P.out) return {CPC_ERR_INVALID, "null argument"};
+  P.p = *job.params;
+  const cpc_params &p = P.p;
+  const cpc_catalog *co = P.co, *ci = P.ci;
+  const bool sky = p.geometry == CPC_GEOM_SKY;
+  P.sky = sky;
+  if (p.geometry != CPC_GEOM_BOX && !sky) return {CPC_ERR_UNSUPPORTED, "unknown geometry"};
+  if (!sky && !(p.box_size > 0)) return {CPC_ERR_INVALID, "box_size must be > 0"};
+  if (p.n_rp <= 0 || p.n_pi <= 0) return {CPC_ERR_INVALID, "n_rp and n_pi must be > 0"};
+  if (!(p.rp_max > 0) || p.rp_min > p.rp_max) return {CPC_ERR_INVALID, "Bad Bin limits"};  // initialization.cpp:299-303
+  if (!(p.pi_max > p.pi_min)) return {CPC_ERR_INVALID, "pi_max must exceed pi_min"};
+  if (!p.lin_bin && !(p.rp_min > 0)) return {CPC_ERR_INVALID, "log rp bins need rp_min > 0"};
+  if (!sky && (2 * p.rp_max >= p.box_size || p.pi_max - p.pi_min >= p.box_size))
+    return {CPC_ERR_UNSUPPORTED, "the rp and pi windows must be narrower than the box"};
+  if (p.kind < CPC_KIND_DD || p.kind > CPC_KIND_SHAPE_SHAPE) return {CPC_ERR_INVALID, "unknown kind"};
+  if (p.kind != CPC_KIND_DD && p.g_with_shape != 1 && p.g_with_shape != 2)
+    return {CPC_ERR_INVALID, "g_with_shape must be 1 or 2 for shape kinds"};
+  if (p.do_jk != 0 && p.do_jk != 1) return {CPC_ERR_UNSUPPORTED, "do_jk=2 (region pairs) is not supported"};
+  if (p.auto_pairs != 0 && p.auto_pairs != 1) return {CPC_ERR_INVALID, "auto_pairs must be 0 or 1"};
+  P.autop = p.auto_pairs == 1;
+  if (P.autop && co != ci) return {CPC_ERR_INVALID, "the auto pass (corels1) takes one catalog on both sides"};
+  if (P.autop && co->n > 0 && !co->z_sorted)
+    return {CPC_ERR_INVALID, "the auto pass needs the catalog sorted by z (data_sorted=1, corr.cpp:129-132)"};
+  if (p.precision != CPC_PREC_SPLIT && p.precision != CPC_PREC_FP64) return {CPC_ERR_INVALID, "unknown precision"};
+  P.jk_on = p.do_jk == 1 && p.n_jk > 0;
+  if (P.jk_on) {
+    for (const cpc_catalog *c : {co, ci}) {
+      if (c->n > 0 && !c->has_jk) return {CPC_ERR_INVALID, "jackknife on but a catalog has no jk array"};
+      if (c->n > 0 && (c->jk_lo < 0 || c->jk_hi >= p.n_jk)) return {CPC_ERR_INVALID, "jk region outside [0, n_jk)"};
+    }
+  }
+  if (p.kind == CPC_KIND_SHAPE_SHAPE && ((co->n && !co->has_e) || (ci->n && !ci->has_e)))
+    return {CPC_ERR_INVALID, "shape_shape needs e1/e2 on both catalogs"};
+  if (p.kind == CPC_KIND_SHAPE_DENSITY) {
+    const cpc_catalog *sc = p.g_with_shape == 2 ? ci : co;
+    if (sc->n && !sc->has_e)
+      return {CPC_ERR_INVALID, "shape_density needs e1/e2 on the shape side (the reference would "
+                               "dereference an unallocated e pointer, corels.cpp:216-219)"};
+  }
+  if (sky) {
+    if ((co->n && !co->has_sky) || (ci->n && !ci->has_sky))
+      return {CPC_ERR_INVALID, "the sky geometry needs dc and da on both catalogs"};
+    if (!p.sky_H || !p.sky_dA || p.sky_n < 2 || !(p.sky_dz > 0))
+      return {CPC_ERR_INVALID, "the sky geometry needs the distance table"};
+  } else {
+    for (const cpc_catalog *c : {co, ci})
+      for (int a = 0; a < 3; ++a)
+        if (c->n > 0 && (c->lo[a] < 0 || c->hi[a] > p.box_size))
+          return {CPC_ERR_INVALID, "coordinates must lie in [0, box_size]"};
+  }
+  P.nreg = P.jk_on ? p.n_jk : 1;
+  P.R = P.jk_on ? 1 + p.n_jk : 1;
+  P.ncell = p.n_rp * p.n_pi;
+  P.dv = derive(p);
+  if (p.kind == CPC_KIND_DD)
+    P.kind_code = 0;
+  else if (p.kind == CPC_KIND_SHAPE_DENSITY)
+    P.kind_code = 1;
+  else
+    P.kind_code = p.estimator == 0 ? 2 : 3;
+  P.NV = kNV[P.kind_code];
+  P.NSP = 1 + 2 * P.NV;
+  P.NST = 1 + P.NV;
+  P.nP = (size_t)2 * P.nreg;
+  P.nT = (size_t)P.nreg;
+  P.split = p.precision == CPC_PREC_SPLIT;
+  return {};
+}
+
+// Thresholds, grid, windows, both sides, work items, device tables and kernel arguments.
+Status setup(cpc_ctx *ctx, Plan &P) {
+  const cpc_params &p = P.p;
+  const cpc_catalog *co = P.co, *ci = P.ci;
+  const bool sky = P.sky, jk_on = P.jk_on, autop = P.autop;
+  const int nreg = P.nreg, ncell = P.ncell;
+  const Derived &dv = P.dv;
+  JobBufs &B = *P.B;
+  Status st;
+  hipStream_t s = ctx->stream;
+
+  P.thr = cached_thresholds(ctx, p, !sky, st);
+  if (!P.thr) return st;
+  const std::vector<double> &thr = *P.thr;
+  // box kernel: up to 512 threads (the split kernel keeps 4 waves per SIMD with 8 waves sharing a
+  // histogram; the FP64 kernel needs more registers and runs 4 waves per group); sky kernel: up to 256
+  P.threads = sky ? std::min(ctx->block_threads ? ctx->block_threads : 256, 256)
+                  : (ctx->block_threads ? ctx->block_threads : (P.split ? 512 : 256));
+  const size_t lds_p = pair_lds_bytes(ncell, P.NSP, p.n_rp);
+  if (lds_p > ctx->max_lds)
+    return {CPC_ERR_UNSUPPORTED, "histogram of " + std::to_string(ncell) + " cells does not fit in LDS (" +
+                                     std::to_string(lds_p) + " bytes)"};
+  const int item_size = ctx->item_size ? ctx->item_size : 512;
+  const int64_t nbig = std::max(co->n, ci->n);
+
+  // ---- geometry-specific set-up: grid, neighbour cells, windows
+  Grid G;
+  int Rn = 0;
+  bool has_zfirst = false;
+  double zfirst_plus_L = 0;
+  SkyArgs SA{};
+  auto t_build = std::chrono::steady_clock::now();
+  if (!sky) {
+    const double L = p.box_size;
+    double frac = ctx->col_frac > 0 ? ctx->col_frac : 0.25;
+    int nc = std::max(1, (int)std::floor(L / (p.rp_max * frac)));
+    // keep enough galaxies per column for full waves
+    const int nc_dense = std::max(1, (int)std::floor(std::sqrt((double)std::max<int64_t>(nbig, 1) / 256.0)));
+    if (ctx->col_frac <= 0) nc = std::min(nc, nc_dense);
+    nc = std::min(nc, 1024);
+    G.ncx = G.ncy = nc;
+    G.hx = G.hy = L / nc;
+    const double h = G.hx;
+    // neighbour column offsets (circle cut at the largest binned separation)
+    const double rcut = std::sqrt(thr[p.n_rp]) * (1 + 1e-9) + 1e-9 * L;
+    Rn = (int)std::floor(p.rp_max / h) + 1;
+    for (int dy = -Rn; dy <= Rn; ++dy)
+      for (int dx = -Rn; dx <= Rn; ++dx) {
+        const double ex = std::max(0, std::abs(dx) - 1) * h, ey = std::max(0, std::abs(dy) - 1) * h;
+        if (ex * ex + ey * ey <= rcut * rcut) P.nbr.push_back(make_int2(dx, dy));
+      }
+    // first included galaxy of the inner catalog in upload order (PBz>0 sweep, corr.cpp:407-416)
+    for (int64_t j = 0; j < ci->n; ++j) {
+      const double inc = ci->h_inc.empty() ? 0.0 : ci->h_inc[j];
+      if (inc > p.include_prob_max) continue;
+      has_zfirst = true;
+      zfirst_plus_L = ci->h_z[j] + L * 1;
+      break;
+    }
+    if (autop && co->n > 0) {
+      // corels1 scans for PBz = 0 and PBz = -1 start at j = i+1 and stop at once when that next
+      // included galaxy lies below the window (do_pair_sorted, corr.cpp:71-73): bit 0 / bit 1
+      std::vector<uint8_t> &af = P.aflags;
+      af.assign(co->n, 0);
+      int64_t nxt = -1;
+      for (int64_t i = co->n - 1; i >= 0; --i) {
+        if (nxt >= 0) {
+          const double z_min = co->h_z[i] + p.pi_min;
+          if (!(co->h_z[nxt] + L * 0 < z_min)) af[i] |= 1;
+          if (!(co->h_z[nxt] + L * -1 < z_min)) af[i] |= 2;
+        }
+        if (!(!co->h_inc.empty() && co->h_inc[i] > p.include_prob_max)) nxt = i;
+      }
+      if (check(B.aflags.ensure(co->n), st, "alloc")) return st;
+      check(hipMemcpyAsync(B.aflags.p, af.data(), co->n, hipMemcpyHostToDevice, s), st, "copy");
+      if (st.code) return st;
+    }
+  } else {
+    // per-galaxy windows of the outer catalog (lim_calc_sky), with the host libm
+    const SkyTable T{p.sky_H, p.sky_dA, p.sky_n, p.sky_dz, p.z_max, p.z_sep_max};
+    std::vector<Win> &wins = P.wins;
+    wins.resize(co->n);
+    double reach_zup = 0, reach_zdn = 0;
+    std::vector<double> dec_half;
+    // RA/Dec reach per bin of the outer redshift
+    double ozlo = INFINITY, ozhi = -INFINITY;
+    for (int64_t i = 0; i < co->n; ++i) {
+      if (!co->h_inc.empty() && co->h_inc[i] > p.include_prob_max) continue;
+      ozlo = std::min(ozlo, co->h_z[i]);
+      ozhi = std::max(ozhi, co->h_z[i]);
+    }
+    const int nb = 512;
+    if (!(ozlo <= ozhi)) ozlo = ozhi = 0.0;
+    const double rz0 = ozlo, rinv = nb / std::max(ozhi - ozlo, 1e-9);
+    std::vector<double2> &reach = P.reach;
+    reach.assign(nb, make_double2(0.0, 0.0));
+    dec_half.reserve(co->n);
+    for (int64_t i = 0; i < co->n; ++i) {
+      const double inc = co->h_inc.empty() ? 0.0 : co->h_inc[i];
+      Win &w = wins[i];
+      if (inc > p.include_prob_max) {
+        w = Win{1, 0, 0, 0, 0, 0, 0, 0};  // never used: the galaxy is not in the pass
+        continue;
+      }
+      if (!sky_window(T, p.pi_min, p.pi_max, p.rp_max, co->h_x[i], co->h_y[i], co->h_z[i], w))
+        return {CPC_ERR_INVALID, "galaxy " + std::to_string(i) +
+                                     ": its redshift window leaves the distance table or exceeds z_max "
+                                     "(the reference exits here, sky_calcs.cpp:109-113)"};
+      const int b = (int)std::fmax(0.0, std::fmin((double)(nb - 1), std::floor((co->h_z[i] - rz0) * rinv)));
+      double2 &rb = reach[b];
+      if (!(w.ramin <= w.ramax) || w.ramin <= 0 || w.ramax >= 360)
+        rb.x = INFINITY;
+      else
+        rb.x = std::max(rb.x, std::max(co->h_x[i] - w.ramin, w.ramax - co->h_x[i]) * (1 + 1e-12) + 1e-9);
+      const double dh = std::max(co->h_y[i] - w.decmin, w.decmax - co->h_y[i]);
+      rb.y = std::max(rb.y, dh * (1 + 1e-12) + 1e-9);
+      dec_half.push_back(dh);
+      reach_zup = std::max(reach_zup, w.zmax - co->h_z[i]);
+      reach_zdn = std::max(reach_zdn, co->h_z[i] - w.zmin);
+    }
+    // RA/Dec grid over both catalogs, cells about half the median window
+    double ralo = 360, rahi = 0, declo = 90, dechi = -90;
+    for (const cpc_catalog *c : {co, ci})
+      if (c->n) {
+        ralo = std::min(ralo, c->lo[0]);
+        rahi = std::max(rahi, c->hi[0]);
+        declo = std::min(declo, c->lo[1]);
+        dechi = std::max(dechi, c->hi[1]);
+      }
+    double hd = 1.0;
+    if (!dec_half.empty()) {
+      std::nth_element(dec_half.begin(), dec_half.begin() + dec_half.size() / 2, dec_half.end());
+      hd = dec_half[dec_half.size() / 2] * (ctx->col_frac > 0 ? ctx->col_frac : 0.5);
+    }
+    const double span_ra = std::max(rahi - ralo, 1e-6), span_dec = std::max(dechi - declo, 1e-6);
+    hd = std::max(hd, span_dec / 2048);
+    const double cdec = std::max(0.2, std::cos(0.5 * (declo + dechi) / kR2D));
+    const double hr = std::max(hd / cdec, span_ra / 2048);
+    G.ncx = std::max(1, (int)std::ceil(span_ra / hr));
+    G.ncy = std::max(1, (int)std::ceil(span_dec / hd));
+    G.hx = span_ra / G.ncx * (1 + 1e-12);
+    G.hy = span_dec / G.ncy * (1 + 1e-12);
+    G.x0 = ralo;
+    G.y0 = declo;
+    SA.reach_zup = reach_zup * (1 + 1e-12) + 1e-12;
+    SA.reach_zdn = reach_zdn * (1 + 1e-12) + 1e-12;
+    SA.reach_nb = nb;
+    SA.reach_z0 = rz0;
+    SA.reach_inv = rinv;
+    if (check(B.reach.ensure(nb), st, "alloc")) return st;
+    check(hipMemcpyAsync(B.reach.p, reach.data(), nb * sizeof(double2), hipMemcpyHostToDevice, s), st, "copy");
+    SA.reach_tab = B.reach.p;
+    if (autop) {
+      // corels1's scan from j = i+1 stops at once when that next included galaxy lies below the
+      // window (corr.cpp:168-186): such outer galaxies take no pairs
+      int64_t nxt = -1;
+      for (int64_t i = co->n - 1; i >= 0; --i) {
+        const bool inc_i = !(!co->h_inc.empty() && co->h_inc[i] > p.include_prob_max);
+        if (inc_i && (nxt < 0 || co->h_z[nxt] < wins[i].zmin)) {
+          wins[i].zmin = 1.0;
+          wins[i].zmax = 0.0;
+        }
+        if (inc_i) nxt = i;
+      }
+    }
+    if (check(B.win_raw.ensure(std::max<int64_t>(co->n, 1)), st, "alloc")) return st;
+    if (co->n)
+      check(hipMemcpyAsync(B.win_raw.p, wins.data(), co->n * sizeof(Win), hipMemcpyHostToDevice, s), st, "copy");
+    if (st.code) return st;
+  }
+
+  // ---- sides (sort keys are (cell, region, coin) packed in 32 bits, all ones meaning excluded)
+  if ((double)G.ncx * G.ncy * nreg * 2.0 >= 4294967295.0)
+    return {CPC_ERR_UNSUPPORTED, "too many cells x jackknife regions for 32-bit sort keys"};
+  st = build_side(ctx, co, G, nreg, jk_on, p, has_zfirst, zfirst_plus_L, sky, sky ? B.win_raw.p : nullptr,
+                  (autop && !sky) ? B.aflags.p : nullptr, B.so);
+  if (st.code) return st;
+  const bool same = (co == ci);
+  if (!same) {
+    st = build_side(ctx, ci, G, nreg, jk_on, p, has_zfirst, zfirst_plus_L, sky, nullptr, nullptr, B.si);
+    if (st.code) return st;
+  }
+  P.SO = &B.so;
+  P.SI = same ? &B.so : &B.si;
+  Side &SO = *P.SO, &SI = *P.SI;
+  P.ms_build = ms_since(t_build);
+
+  // ---- work items (cost: block galaxies x tile galaxies in reach)
+  const int per_item = std::max(1, item_size / 64);
+  if (!sky) {
+    const int nc = G.ncx;
+    const std::vector<int2> &nbr = P.nbr;
+    auto cost_of = [&](const Side &T) {
+      return [&T, &nbr, nc](const Group &g) {
+        const int bcx = g.col % nc, bcy = g.col / nc;
+        double c = 0;
+        for (const int2 &o : nbr) {
+          const int ux = ((bcx + o.x) % nc + nc) % nc, uy = ((bcy + o.y) % nc + nc) % nc;
+          c += (double)T.colcount[(size_t)uy * nc + ux];
+        }
+        return c;
+      };
+    };
+    make_items(SO, false, jk_on, per_item, cost_of(SI), P.items_p, P.cost_p, P.chunks);
+    if (jk_on) make_items(SI, true, jk_on, per_item, cost_of(SO), P.items_t, P.cost_t, P.chunks);
+  } else {
+    auto flat = [](const Group &) { return 1.0; };
+    make_items(SO, false, jk_on, per_item, flat, P.items_p, P.cost_p, P.chunks);
+    if (jk_on) make_items(SI, true, jk_on, per_item, flat, P.items_t, P.cost_t, P.chunks);
+  }
+
+  // ---- device tables
+  if (check(B.thr.ensure(p.n_rp + 1), st, "alloc") ||
+      check(B.nbr.ensure(std::max<size_t>(P.nbr.size(), 1)), st, "alloc") ||
+      check(B.chunks.ensure(P.chunks.size()), st, "alloc"))
+    return st;
+  check(hipMemcpyAsync(B.thr.p, thr.data(), thr.size() * 8, hipMemcpyHostToDevice, s), st, "copy");
+  if (!P.nbr.empty())
+    check(hipMemcpyAsync(B.nbr.p, P.nbr.data(), P.nbr.size() * sizeof(int2), hipMemcpyHostToDevice, s), st, "copy");
+  if (!P.chunks.empty())
+    check(hipMemcpyAsync(B.chunks.p, P.chunks.data(), P.chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, s),
+          st, "copy");
+  if (st.code) return st;
+
+  const double two24 = std::ldexp(1.0, -24);
+  double gz = 1.0;
+  Guard gr{};
+  if (!sky) {
+    // guard bands of the FP32 classification from bounds on the FP32 error of dx, dy (column-local
+    // coordinates plus a rounded column offset) and of dz (absolute coordinates)
+    const double L = p.box_size, h = G.hx;
+    const double amax = 1.01 * h + 1e-9 * L;
+    const double omax = (Rn + 1) * h + 1e-9 * L;
+    const double eps_xy = 2.0 * (6 * amax + 2 * omax) * two24;
+    gr = rp_guard(p, dv, thr, eps_xy);
+    const double eps_z = 2.0 * 5 * L * two24;
+    const double pabs = std::max(std::fabs(p.pi_min), std::fabs(p.pi_max));
+    gz = eps_z / dv.psize + 8.0 * (p.n_pi + 2 * pabs / dv.psize + 2) * two24;
+  }
+  P.use_split = P.split && (sky || gz < 0.25);
+
+  if (!sky) {
+    KArgs K{};
+    K.L = p.box_size;
+    K.pmin = p.pi_min;
+    K.pmax = p.pi_max;
+    K.psize = dv.psize;
+    K.rmax = p.rp_max;
+    K.h = G.hx;
+    K.zmargin = 1e-7 * (p.box_size + std::max(std::fabs(p.pi_min), std::fabs(p.pi_max)));
+    K.nc = G.ncx;
+    K.n_rp = p.n_rp;
+    K.n_pi = p.n_pi;
+    K.ncell = ncell;
+    K.gws = p.g_with_shape;
+    K.n_nbr = (int)P.nbr.size();
+    K.lin = p.lin_bin;
+    K.pioff32 = (float)(-p.pi_min / dv.psize);
+    K.inv_psize32 = (float)(1.0 / dv.psize);
+    K.gz = (float)gz;
+    K.la = (float)(0.5 * std::log10(2.0) / dv.rsize);
+    K.lb = (float)(-std::log10(p.rp_min > 0 ? p.rp_min : 1.0) / dv.rsize);
+    K.lrmin = (float)p.rp_min;
+    K.linv = (float)(1.0 / dv.rsize);
+    K.thr = B.thr.p;
+    K.s_out_lo = gr.s_out_lo;
+    K.s_out_hi = gr.s_out_hi;
+    K.s_cert_lo = gr.s_cert_lo;
+    K.gA = gr.gA;
+    K.gB = gr.gB;
+    K.gC = gr.gC;
+    K.gD = gr.gD;
+    K.nbr = B.nbr.p;
+    K.auto_mode = autop ? 1 : 0;
+    K.chunks = B.chunks.p;
+    // primary pass: blocks = outer galaxies, tiles = inner galaxies
+    P.kp = K;
+    P.kp.brec = SO.rec.p;
+    P.kp.bz = SO.zs.p;
+    P.kp.trec = SI.rec.p;
+    P.kp.tz = SI.zs.p;
+    P.kp.phase_mode = jk_on ? 1 : 0;
+    P.kp.nslots = P.NSP;
+    P.kp.bgrp = SO.d_groups.p;
+    P.kp.tgrp = SI.d_groups.p;
+    P.kp.tcol = SI.d_colbeg.p;
+    P.kp.bidx = SO.idx.p;
+    P.kp.tidx = SI.idx.p;
+    // touch pass: blocks = inner galaxies with coin >= 0.5, tiles = outer galaxies
+    P.kt = K;
+    P.kt.brec = SI.rec.p;
+    P.kt.bz = SI.zs.p;
+    P.kt.trec = SO.rec.p;
+    P.kt.tz = SO.zs.p;
+    P.kt.phase_mode = 2;
+    P.kt.nslots = P.NST;
+    P.kt.bgrp = SI.d_groups.p;
+    P.kt.tgrp = SO.d_groups.p;
+    P.kt.tcol = SO.d_colbeg.p;
+    P.kt.bidx = SI.idx.p;
+    P.kt.tidx = SO.idx.p;
+  } else {
+    SkyArgs &A = SA;
+    A.pmin = p.pi_min;
+    A.pmax = p.pi_max;
+    A.psize = dv.psize;
+    A.n_rp = p.n_rp;
+    A.n_pi = p.n_pi;
+    A.ncell = ncell;
+    A.gws = p.g_with_shape;
+    A.lin = p.lin_bin;
+    A.ra0 = G.x0;
+    A.dec0 = G.y0;
+    A.hra = G.hx;
+    A.hdec = G.hy;
+    A.nra = G.ncx;
+    A.ndec = G.ncy;
+    A.dsum = 4e-15;
+    A.auto_mode = autop ? 1 : 0;
+    A.thr = B.thr.p;
+    A.chunks = B.chunks.p;
+    P.sa = A;
+    // host-queue capacity for sky guard-band pairs; grown and the pass rerun if ever exceeded
+    P.hq_cap = (int)std::min<int64_t>(std::max<int64_t>(1 << 16, nbig), 1 << 24);
+  }
+  return st;
+}
+
+// All box plans: one launch per (kind, precision, pass direction), each over the items of every
+// plan it serves ordered by decreasing cost, so that the device starts the longest items first
+// whichever job they belong to.  Kernel times are those of the whole group of launches.
+Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
+  Status st;
+  if (plans.empty()) return st;
+  hipStream_t s = ctx->stream;
+  // job table: entry 2i is plan i's primary pass, 2i+1 its touch pass
+  std::vector<KArgs> table(2 * plans.size());
+  for (size_t i = 0; i < plans.size(); ++i) {
+    Plan &P = *plans[i];
+    KArgs &a = table[2 * i];
+    a = P.kp;
+    a.acc = ctx->acc.p + P.off_p;
+    a.accnum = ctx->accn.p + P.off_pn;
+    a.counters = ctx->accn.p + P.off_c;
+    KArgs &b = table[2 * i + 1];
+    b = P.kt;
+    b.acc = ctx->acc.p + P.off_t;
+    b.accnum = ctx->accn.p + P.off_tn;
+    b.counters = ctx->accn.p + P.off_c;
+  }
+  struct Launch {
+    int kind;
+    bool split, swap;
+    int threads;
+    size_t lds, first, n;
+  };
+  std::map<std::pair<int, int>, std::vector<size_t>> groups;  // (kind, split) -> plans
+  for (size_t i = 0; i < plans.size(); ++i) groups[{plans[i]->kind_code, plans[i]->use_split ? 1 : 0}].push_back(i);
+  std::vector<Item> all;
+  std::vector<Launch> launches;
+  for (int swap = 0; swap < 2; ++swap)
+    for (const auto &g : groups) {
+      std::vector<std::pair<double, Item>> v;
+      size_t lds = 0;
+      int threads = 0;
+      for (size_t i : g.second) {
+        const Plan &P = *plans[i];
+        const std::vector<Item> &its = swap ? P.items_t : P.items_p;
+        const std::vector<double> &cs = swap ? P.cost_t : P.cost_p;
+        for (size_t k = 0; k < its.size(); ++k) {
+          Item it = its[k];
+          it.job = (int32_t)(2 * i + swap);
+          v.push_back({cs[k], it});
+        }
+        lds = std::max(lds, pair_lds_bytes(P.ncell, swap ? P.NST : P.NSP, P.p.n_rp));
+        threads = P.threads;
+      }
+      if (v.empty()) continue;
+      std::stable_sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+      launches.push_back(Launch{g.first.first, g.first.second != 0, swap != 0, threads, lds, all.size(), v.size()});
+      for (const auto &x : v) all.push_back(x.second);
+    }
+  if (check(ctx->jobs.ensure(table.size()), st, "alloc") || check(ctx->items.ensure(all.size()), st, "alloc"))
+    return st;
+  check(hipMemcpyAsync(ctx->jobs.p, table.data(), table.size() * sizeof(KArgs), hipMemcpyHostToDevice, s), st,
+        "copy");
+  if (!all.empty())
+    check(hipMemcpyAsync(ctx->items.p, all.data(), all.size() * sizeof(Item), hipMemcpyHostToDevice, s), st, "copy");
+  check(hipEventRecord(ctx->ev0, s), st, "event");
+  for (const Launch &l : launches)
+    if (!l.swap)
+      check(launch_pair(l.kind, l.split, false, ctx->jobs.p, ctx->items.p + l.first, (int)l.n, l.lds, l.threads, s),
+            st, "pair kernel");
+  check(hipEventRecord(ctx->ev2, s), st, "event");
+  for (const Launch &l : launches)
+    if (l.swap)
+      check(launch_pair(l.kind, l.split, true, ctx->jobs.p, ctx->items.p + l.first, (int)l.n, l.lds, l.threads, s),
+            st, "touch kernel");
+  check(hipEventRecord(ctx->ev1, s), st, "event");
+  for (Plan *Pp : plans) {  // sparse-window diagnostic (corels2's starts pointer)
+    const Plan &P = *Pp;
+    if (P.autop || P.co->n == 0 || P.ci->n == 0) continue;
+    SparseArgs sa{P.co->d_z, P.co->d_inc, P.co->n, P.ci->z_sorted ? P.ci->d_z : P.ci->d_zsorted, P.ci->n,
+                  P.p.box_size, P.p.pi_min, P.p.pi_max, P.p.include_prob_max};
+    check(launch_sparse(sa, ctx->accn.p + P.off_c + 2, s), st, "sparse");
+  }
+  check(hipStreamSynchronize(s), st, "sync");
+  if (st.code) return st;
+  float kms = 0, kms_p = 0, kms_t = 0;
+  (void)hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1);
+  (void)hipEventElapsedTime(&kms_p, ctx->ev0, ctx->ev2);
+  (void)hipEventElapsedTime(&kms_t, ctx->ev2, ctx->ev1);
+  for (Plan *P : plans) {
+    P->kms = kms;
+    P->kms_p = kms_p;
+    P->kms_t = kms_t;
+  }
+  return st;
+}
+
+// One sky plan: primary and touch launches, rerun with a larger host queue if guard-band pairs
+// overflowed it; the queued pairs are copied back for host binning.
+Status run_sky(cpc_ctx *ctx, Plan &P) {
+  Status st;
+  hipStream_t s = ctx->stream;
+  JobBufs &B = *P.B;
+  const size_t np = P.nP * P.NSP * P.ncell, nt = P.nT * P.NST * P.ncell;
+  const size_t npn = P.nP * P.ncell, ntn = P.nT * P.ncell;
+  if (check(ctx->items.ensure(P.items_p.size() + P.items_t.size()), st, "alloc")) return st;
+  if (!P.items_p.empty())
+    check(hipMemcpyAsync(ctx->items.p, P.items_p.data(), P.items_p.size() * sizeof(Item), hipMemcpyHostToDevice, s),
+          st, "copy");
+  if (!P.items_t.empty())
+    check(hipMemcpyAsync(ctx->items.p + P.items_p.size(), P.items_t.data(), P.items_t.size() * sizeof(Item),
+                         hipMemcpyHostToDevice, s),
+          st, "copy");
+  unsigned long long counters[4] = {0, 0, 0, 0};
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if (check(B.hq.ensure(P.hq_cap), st, "alloc")) return st;
+    check(hipMemsetAsync(ctx->acc.p + P.off_p, 0, np * 8, s), st, "memset");
+    check(hipMemsetAsync(ctx->acc.p + P.off_t, 0, nt * 8, s), st, "memset");
+    check(hipMemsetAsync(ctx->accn.p + P.off_pn, 0, npn * 8, s), st, "memset");
+    check(hipMemsetAsync(ctx->accn.p + P.off_tn, 0, ntn * 8, s), st, "memset");
+    check(hipMemsetAsync(ctx->accn.p + P.off_c, 0, 4 * 8, s), st, "memset");
+    SkyArgs A = P.sa;
+    A.counters = ctx->accn.p + P.off_c;
+    A.hq = B.hq.p;
+    A.hq_cap = P.hq_cap;
+    check(hipEventRecord(ctx->ev0, s), st, "event");
+    if (!P.items_p.empty()) {
+      SkyArgs C = A;
+      C.n_items = (int)P.items_p.size();
+      C.phase_mode = P.jk_on ? 1 : 0;
+      C.nslots = P.NSP;
+      C.items = ctx->items.p;
+      C.bgrp = P.SO->d_groups.p;
+      C.tgrp = P.SI->d_groups.p;
+      C.tcol = P.SI->d_colbeg.p;
+      C.acc = ctx->acc.p + P.off_p;
+      C.accnum = ctx->accn.p + P.off_pn;
+      check(launch_sky(P.kind_code, P.use_split, false, C, P.threads, sky_side(*P.SO), sky_side(*P.SI), s), st,
+            "sky pair kernel");
+    }
+    check(hipEventRecord(ctx->ev2, s), st, "event");
+    if (!P.items_t.empty()) {
+      SkyArgs C = A;
+      C.n_items = (int)P.items_t.size();
+      C.phase_mode = 2;
+      C.nslots = P.NST;
+      C.items = ctx->items.p + P.items_p.size();
+      C.bgrp = P.SI->d_groups.p;
+      C.tgrp = P.SO->d_groups.p;
+      C.tcol = P.SO->d_colbeg.p;
+      C.acc = ctx->acc.p + P.off_t;
+      C.accnum = ctx->accn.p + P.off_tn;
+      check(launch_sky(P.kind_code, P.use_split, true, C, P.threads, sky_side(*P.SI), sky_side(*P.SO), s), st,
+            "sky touch kernel");
+    }
+    check(hipEventRecord(ctx->ev1, s), st, "event");
+    if (st.code) return st;
+    check(hipMemcpyAsync(counters, ctx->accn.p + P.off_c, 4 * 8, hipMemcpyDeviceToHost, s), st, "copy");
+    check(hipStreamSynchronize(s), st, "sync");
+    if (st.code) return st;
+    if (counters[3] > (unsigned long long)P.hq_cap) {
+      if (counters[3] > (unsigned long long)INT32_MAX) return {CPC_ERR_NOMEM, "guard-band queue overflow"};
+      P.hq_cap = (int)counters[3];
+      continue;  // rerun with a queue large enough for every guard-band pair
+    }
+    break;
+  }
+  (void)hipEventElapsedTime(&P.kms, ctx->ev0, ctx->ev1);
+  (void)hipEventElapsedTime(&P.kms_p, ctx->ev0, ctx->ev2);
+  (void)hipEventElapsedTime(&P.kms_t, ctx->ev2, ctx->ev1);
+  if (counters[3] > 0) {
+    P.hq_pairs.resize(counters[3]);
+    check(hipMemcpyAsync(P.hq_pairs.data(), B.hq.p, P.hq_pairs.size() * sizeof(uint2), hipMemcpyDeviceToHost, s), st,
+          "copy");
+    check(hipStreamSynchronize(s), st, "sync");
+  }
+  return st;
+}
+
+// The reference layout from a plan's device histograms (host copies of the arenas): the full
+// sample, the touch or leave-one-out jackknife regions, num_rp, and the plan's stats.
+// Returns the plan's sparse-window count.
+int64_t assemble(const Plan &P, const std::vector<double> &acc, const std::vector<unsigned long long> &accn,
+                 double ms_total) {
+  const cpc_params &p = P.p;
+  const int ncell = P.ncell, nreg = P.nreg, R = P.R, NV = P.NV, NSP = P.NSP, NST = P.NST, kc = P.kind_code;
+  const double *accp = acc.data() + P.off_p, *acct = acc.data() + P.off_t;
+  const unsigned long long *accpn = accn.data() + P.off_pn, *acctn = accn.data() + P.off_tn;
+  const unsigned long long *counters = accn.data() + P.off_c;
+  cpc_hist *out = P.out;
+  const cpc_catalog *co = P.co, *ci = P.ci;
+
+  // ---- pairs the device could not bin bit-exactly: bin them with the host libm
+  HostHist HH;
+  const bool host_pairs = !P.hq_pairs.empty();
+  if (host_pairs) {
+    HH.init(R, ncell);
+    for (const uint2 &q : P.hq_pairs) host_sky_pair(p, P.dv, co, (int64_t)q.x, ci, (int64_t)q.y, P.jk_on, HH);
+  }
+  int64_t sparse = (int64_t)counters[2];
+  if (P.sky && ci->z_sorted && ci->n > 0) {  // sparse-window diagnostic on the host
+    for (int64_t i = 0; i < co->n; ++i) {
+      const double inc = co->h_inc.empty() ? 0.0 : co->h_inc[i];
+      if (inc > p.include_prob_max) continue;
+      auto it = std::lower_bound(ci->h_z.begin(), ci->h_z.end(), P.wins[i].zmin);
+      if (it != ci->h_z.end() && *it > P.wins[i].zmax) ++sparse;
+    }
+  }
+
+  const size_t cells_all = (size_t)R * ncell;
+  std::fill(out->num, out->num + cells_all, 0);
+  std::fill(out->wt_num, out->wt_num + cells_all, 0.0);
+  std::fill(out->val, out->val + cells_all * CPC_NFIELD, 0.0);
+  std::fill(out->err, out->err + cells_all * CPC_NFIELD, 0.0);
+  std::fill(out->num_rp, out->num_rp + (size_t)R * p.n_rp, 0);
+  // full sample = every primary target (+ host-binned pairs)
+  for (int t = 0; t < 2 * nreg; ++t)
+    for (int c = 0; c < ncell; ++c) {
+      const unsigned long long n = accpn[(size_t)t * ncell + c];
+      if (!n) continue;
+      out->num[c] += (int64_t)n;
+      out->wt_num[c] += accp[((size_t)t * NSP + 0) * ncell + c];
+      for (int v = 0; v < NV; ++v) {
+        const int f = kValField[kc][v];
+        out->val[(size_t)c * CPC_NFIELD + f] += accp[((size_t)t * NSP + 1 + v) * ncell + c];
+        out->err[(size_t)c * CPC_NFIELD + f] += accp[((size_t)t * NSP + 1 + NV + v) * ncell + c];
+      }
+    }
+  if (host_pairs)
+    for (int c = 0; c < ncell; ++c) {
+      out->num[c] += HH.num[c];
+      out->wt_num[c] += HH.wt[c];
+      for (int f = 0; f < CPC_NFIELD; ++f) {
+        out->val[(size_t)c * CPC_NFIELD + f] += HH.val[(size_t)c * CPC_NFIELD + f];
+        out->err[(size_t)c * CPC_NFIELD + f] += HH.err[(size_t)c * CPC_NFIELD + f];
+      }
+    }
+  if (P.jk_on) {
+    std::vector<double> tv((size_t)ncell * CPC_NFIELD), tw(ncell);
+    std::vector<int64_t> tn(ncell);
+    for (int r = 0; r < p.n_jk; ++r) {
+      // touch_r (jk.cpp:138-171): num, wt_num and data[].val; shape vals only when which_corr != 0
+      std::fill(tv.begin(), tv.end(), 0.0);
+      std::fill(tw.begin(), tw.end(), 0.0);
+      std::fill(tn.begin(), tn.end(), 0);
+      for (int src = 0; src < 2; ++src) {
+        const double *a = src == 0 ? accp : acct;
+        const unsigned long long *num = src == 0 ? accpn : acctn;
+        const int NS = src == 0 ? NSP : NST;
+        const int t = src == 0 ? 2 * r + 1 : r;
+        for (int c = 0; c < ncell; ++c) {
+          const unsigned long long n = num[(size_t)t * ncell + c];
+          if (!n) continue;
+          tn[c] += (int64_t)n;
+          tw[c] += a[((size_t)t * NS + 0) * ncell + c];
+          for (int v = 0; v < NV; ++v)
+            tv[(size_t)c * CPC_NFIELD + kValField[kc][v]] += a[((size_t)t * NS + 1 + v) * ncell + c];
+        }
+      }
+      if (host_pairs)
+        for (int c = 0; c < ncell; ++c) {
+          const size_t o = (size_t)(r + 1) * ncell + c;
+          tn[c] += HH.num[o];
+          tw[c] += HH.wt[o];
+          for (int f = 0; f < CPC_NFIELD; ++f) tv[(size_t)c * CPC_NFIELD + f] += HH.val[o * CPC_NFIELD + f];
+        }
+      if (p.which_corr == 0) std::fill(tv.begin(), tv.end(), 0.0);
+      const size_t reg = (size_t)(r + 1);
+      for (int c = 0; c < ncell; ++c) {
+        const size_t o = reg * ncell + c;
+        if (p.jk_mode == CPC_JK_TOUCH) {
+          out->num[o] = tn[c];
+          out->wt_num[o] = tw[c];
+          for (int f = 0; f < CPC_NFIELD; ++f) out->val[o * CPC_NFIELD + f] = tv[(size_t)c * CPC_NFIELD + f];
+        } else {
+          // final_jk_bins: add_bins(full, touch, jk, subtract) over every field (outp.cpp:242-288)
+          out->num[o] = out->num[c] + tn[c] * -1;
+          out->wt_num[o] = out->wt_num[c] + tw[c] * -1;
+          for (int f = 0; f < CPC_NFIELD; ++f) {
+            out->val[o * CPC_NFIELD + f] = out->val[(size_t)c * CPC_NFIELD + f] + tv[(size_t)c * CPC_NFIELD + f] * -1;
+            out->err[o * CPC_NFIELD + f] = out->err[(size_t)c * CPC_NFIELD + f];
+          }
+        }
+      }
+    }
+  }
+  int64_t nbinned = 0;
+  for (int r = 0; r < R; ++r)
+    for (int k = 0; k < p.n_rp; ++k) {
+      int64_t sum = 0;
+      for (int q = 0; q < p.n_pi; ++q) sum += out->num[((size_t)r * p.n_rp + k) * p.n_pi + q];
+      out->num_rp[(size_t)r * p.n_rp + k] = sum;
+      if (r == 0) nbinned += sum;
+    }
+  if (P.stats) {
+    cpc_stats *st = P.stats;
+    st->n_binned = nbinned;
+    st->n_candidates = (int64_t)counters[0];
+    st->n_fp64_reeval = (int64_t)counters[1];
+    st->n_sparse_windows = sparse;
+    st->ms_kernel = P.kms;
+    st->ms_primary = P.kms_p;
+    st->ms_touch = P.kms_t;
+    st->ms_build = P.ms_build;
+    st->n_items_primary = (int64_t)P.items_p.size();
+    st->n_host_pairs = (int64_t)P.hq_pairs.size();
+    st->ms_total = ms_total;
+  }
+  return sparse;
+}
+
+// cpc_count and cpc_count_batch: check every job, prepare each, run box plans together and sky
+// plans in turn, then copy the accumulators back once and assemble every output.
+cpc_status run_jobs(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *total, bool batch) {
+  auto t_start = std::chrono::steady_clock::now();
+  if (!ctx) return CPC_ERR_INVALID;
+  if (n_jobs < 0 || (n_jobs > 0 && !jobs)) return fail(ctx, CPC_ERR_INVALID, "bad job list");
+  auto tag = [&](int i, Status st) {
+    if (batch) st.msg = "job " + std::to_string(i) + ": " + st.msg;
+    return st;
+  };
+  std::vector<Plan> plans(n_jobs);
+  for (int i = 0; i < n_jobs; ++i) {
+    Status st = validate(jobs[i], plans[i]);
+    if (st.code) return fail(ctx, tag(i, st));
+  }
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, CPC_ERR_DEVICE, "hipSetDevice failed");
+  while (ctx->pool.size() < (size_t)n_jobs) ctx->pool.push_back(std::unique_ptr<JobBufs>(new JobBufs()));
+  for (int i = 0; i < n_jobs; ++i) {
+    plans[i].B = ctx->pool[i].get();
+    Status st = setup(ctx, plans[i]);
+    if (st.code) return fail(ctx, tag(i, st));
+  }
+  // accumulator arenas: per plan [primary f64 | touch f64] and [primary counts | touch counts | 4 counters]
+  size_t nd = 0, nu = 0;
+  for (Plan &P : plans) {
+    P.off_p = nd;
+    nd += P.nP * P.NSP * P.ncell;
+    P.off_t = nd;
+    nd += P.nT * P.NST * P.ncell;
+    P.off_pn = nu;
+    nu += P.nP * P.ncell;
+    P.off_tn = nu;
+    nu += P.nT * P.ncell;
+    P.off_c = nu;
+    nu += 4;
+  }
+  Status st;
+  hipStream_t s = ctx->stream;
+  if (check(ctx->acc.ensure(std::max<size_t>(nd, 1)), st, "alloc") ||
+      check(ctx->accn.ensure(std::max<size_t>(nu, 1)), st, "alloc"))
+    return fail(ctx, st);
+  if (nd) check(hipMemsetAsync(ctx->acc.p, 0, nd * 8, s), st, "memset");
+  if (nu) check(hipMemsetAsync(ctx->accn.p, 0, nu * 8, s), st, "memset");
+  if (st.code) return fail(ctx, st);
+  std::vector<Plan *> box;
+  for (Plan &P : plans)
+    if (!P.sky) box.push_back(&P);
+  st = run_box(ctx, box);
+  if (st.code) return fail(ctx, st);
+  for (int i = 0; i < n_jobs; ++i)
+    if (plans[i].sky) {
+      st = run_sky(ctx, plans[i]);
+      if (st.code) return fail(ctx, tag(i, st));
+    }
+  std::vector<double> acc(nd);
+  std::vector<unsigned long long> accn(nu);
+  if (nd) check(hipMemcpyAsync(acc.data(), ctx->acc.p, nd * 8, hipMemcpyDeviceToHost, s), st, "copy");
+  if (nu) check(hipMemcpyAsync(accn.data(), ctx->accn.p, nu * 8, hipMemcpyDeviceToHost, s), st, "copy");
+  check(hipStreamSynchronize(s), st, "sync");
+  if (st.code) return fail(ctx, st);
+  const double ms_total = ms_since(t_start);
+  int64_t sparse = 0;
+  for (const Plan &P : plans) sparse += assemble(P, acc, accn, ms_total);
+  if (total) {
+    cpc_stats t{};
+    for (const Plan &P : plans) {
+      const unsigned long long *c = accn.data() + P.off_c;
+      for (int k = 0; k < P.p.n_rp; ++k) t.n_binned += P.out->num_rp[k];
+      t.n_candidates += (int64_t)c[0];
+      t.n_fp64_reeval += (int64_t)c[1];
+      t.n_items_primary += (int64_t)P.items_p.size();
+      t.n_host_pairs += (int64_t)P.hq_pairs.size();
+      t.ms_build += P.ms_build;
+      if (P.sky) {
+        t.ms_kernel += P.kms;
+        t.ms_primary += P.kms_p;
+        t.ms_touch += P.kms_t;
+      }
+    }
+    if (!box.empty()) {  // box plans share their launches
+      t.ms_kernel += box[0]->kms;
+      t.ms_primary += box[0]->kms_p;
+      t.ms_touch += box[0]->kms_t;
+    }
+    t.n_sparse_windows = sparse;
+    t.ms_total = ms_since(t_start);
+    *total = t;
+  }
+  ctx->err.clear();
+  return CPC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -649,21 +1519,12 @@ void cpc_destroy(cpc_ctx *ctx) {
   ctx->vals_b.release();
   ctx->order.release();
   ctx->zk_b.release();
-  ctx->thr.release();
-  ctx->nbr.release();
   ctx->items.release();
-  ctx->chunks.release();
-  ctx->accp.release();
-  ctx->acct.release();
-  ctx->accpn.release();
-  ctx->acctn.release();
-  ctx->counters.release();
-  ctx->win_raw.release();
-  ctx->reach.release();
-  ctx->aflags.release();
-  ctx->hq.release();
-  ctx->side_o.release();
-  ctx->side_i.release();
+  ctx->jobs.release();
+  ctx->acc.release();
+  ctx->accn.release();
+  for (auto &b : ctx->pool) b->release();
+  ctx->pool.clear();
   if (ctx->sort_tmp) (void)hipFree(ctx->sort_tmp);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
@@ -871,611 +1732,70 @@ cpc_status cpc_rp_thresholds(const cpc_params *p, double *thr) {
 
 cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci, const cpc_params *pp, cpc_hist *out,
                      cpc_stats *stats) {
-  auto t_start = std::chrono::steady_clock::now();
+  const cpc_job job{co, ci, pp, out, stats};
+  return run_jobs(ctx, 1, &job, nullptr, false);
+}
+
+cpc_status cpc_count_batch(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *total) {
+  return run_jobs(ctx, n_jobs, jobs, total, true);
+}
+
+cpc_status cpc_autotune(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_tuning *best, int *tried) {
   if (!ctx) return CPC_ERR_INVALID;
-  if (!co || !ci || !pp || !out) return fail(ctx, CPC_ERR_INVALID, "null argument");
-  const cpc_params p = *pp;
-  const bool sky = p.geometry == CPC_GEOM_SKY;
-  // ---- validation (the reference's own preconditions and the engine's)
-  if (p.geometry != CPC_GEOM_BOX && !sky) return fail(ctx, CPC_ERR_UNSUPPORTED, "unknown geometry");
-  if (!sky && !(p.box_size > 0)) return fail(ctx, CPC_ERR_INVALID, "box_size must be > 0");
-  if (p.n_rp <= 0 || p.n_pi <= 0) return fail(ctx, CPC_ERR_INVALID, "n_rp and n_pi must be > 0");
-  if (!(p.rp_max > 0) || p.rp_min > p.rp_max)
-    return fail(ctx, CPC_ERR_INVALID, "Bad Bin limits");  // initialization.cpp:299-303
-  if (!(p.pi_max > p.pi_min)) return fail(ctx, CPC_ERR_INVALID, "pi_max must exceed pi_min");
-  if (!p.lin_bin && !(p.rp_min > 0)) return fail(ctx, CPC_ERR_INVALID, "log rp bins need rp_min > 0");
-  if (!sky && (2 * p.rp_max >= p.box_size || p.pi_max - p.pi_min >= p.box_size))
-    return fail(ctx, CPC_ERR_UNSUPPORTED, "the rp and pi windows must be narrower than the box");
-  if (p.kind < CPC_KIND_DD || p.kind > CPC_KIND_SHAPE_SHAPE) return fail(ctx, CPC_ERR_INVALID, "unknown kind");
-  if (p.kind != CPC_KIND_DD && p.g_with_shape != 1 && p.g_with_shape != 2)
-    return fail(ctx, CPC_ERR_INVALID, "g_with_shape must be 1 or 2 for shape kinds");
-  if (p.do_jk != 0 && p.do_jk != 1) return fail(ctx, CPC_ERR_UNSUPPORTED, "do_jk=2 (region pairs) is not supported");
-  if (p.auto_pairs != 0 && p.auto_pairs != 1) return fail(ctx, CPC_ERR_INVALID, "auto_pairs must be 0 or 1");
-  const bool autop = p.auto_pairs == 1;
-  if (autop && co != ci) return fail(ctx, CPC_ERR_INVALID, "the auto pass (corels1) takes one catalog on both sides");
-  if (autop && co->n > 0 && !co->z_sorted)
-    return fail(ctx, CPC_ERR_INVALID, "the auto pass needs the catalog sorted by z (data_sorted=1, corr.cpp:129-132)");
-  if (p.precision != CPC_PREC_SPLIT && p.precision != CPC_PREC_FP64)
-    return fail(ctx, CPC_ERR_INVALID, "unknown precision");
-  const bool jk_on = p.do_jk == 1 && p.n_jk > 0;
-  const int nreg = jk_on ? p.n_jk : 1;
-  if (jk_on) {
-    for (const cpc_catalog *c : {co, ci}) {
-      if (c->n > 0 && !c->has_jk) return fail(ctx, CPC_ERR_INVALID, "jackknife on but a catalog has no jk array");
-      if (c->n > 0 && (c->jk_lo < 0 || c->jk_hi >= p.n_jk))
-        return fail(ctx, CPC_ERR_INVALID, "jk region outside [0, n_jk)");
-    }
-  }
-  if (p.kind == CPC_KIND_SHAPE_SHAPE && ((co->n && !co->has_e) || (ci->n && !ci->has_e)))
-    return fail(ctx, CPC_ERR_INVALID, "shape_shape needs e1/e2 on both catalogs");
-  if (p.kind == CPC_KIND_SHAPE_DENSITY) {
-    const cpc_catalog *sc = p.g_with_shape == 2 ? ci : co;
-    if (sc->n && !sc->has_e)
-      return fail(ctx, CPC_ERR_INVALID, "shape_density needs e1/e2 on the shape side (the reference would "
-                                        "dereference an unallocated e pointer, corels.cpp:216-219)");
-  }
-  if (sky) {
-    if ((co->n && !co->has_sky) || (ci->n && !ci->has_sky))
-      return fail(ctx, CPC_ERR_INVALID, "the sky geometry needs dc and da on both catalogs");
-    if (!p.sky_H || !p.sky_dA || p.sky_n < 2 || !(p.sky_dz > 0))
-      return fail(ctx, CPC_ERR_INVALID, "the sky geometry needs the distance table");
-  } else {
-    for (const cpc_catalog *c : {co, ci})
-      for (int a = 0; a < 3; ++a)
-        if (c->n > 0 && (c->lo[a] < 0 || c->hi[a] > p.box_size))
-          return fail(ctx, CPC_ERR_INVALID, "coordinates must lie in [0, box_size]");
-  }
-  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, CPC_ERR_DEVICE, "hipSetDevice failed");
-
-  const int R = jk_on ? 1 + p.n_jk : 1;
-  const int ncell = p.n_rp * p.n_pi;
-  const Derived dv = derive(p);
-  Status st;
-  hipStream_t s = ctx->stream;
-
-  const std::vector<double> *thrp = cached_thresholds(ctx, p, !sky, st);
-  if (!thrp) return fail(ctx, st);
-  const std::vector<double> &thr = *thrp;
-
-  // ---- kind, slots
-  int kind_code;  // kernel kind: 0 DD, 1 SD, 2 SS est 0, 3 SS est != 0
-  if (p.kind == CPC_KIND_DD)
-    kind_code = 0;
-  else if (p.kind == CPC_KIND_SHAPE_DENSITY)
-    kind_code = 1;
-  else
-    kind_code = p.estimator == 0 ? 2 : 3;
-  static const int kNV[4] = {0, 3, 5, 6};
-  static const int kValField[4][6] = {{0}, {1, 2, 6}, {1, 3, 4, 5, 6}, {1, 2, 3, 4, 5, 6}};
-  const int NV = kNV[kind_code];
-  const int NSP = 1 + 2 * NV, NST = 1 + NV;
-  const bool split = p.precision == CPC_PREC_SPLIT;
-  // box kernel: up to 512 threads (the split kernel keeps 4 waves per SIMD with 8 waves sharing a
-  // histogram; the FP64 kernel needs more registers and runs 4 waves per group); sky kernel: up to 256
-  const int threads = sky ? std::min(ctx->block_threads ? ctx->block_threads : 256, 256)
-                          : (ctx->block_threads ? ctx->block_threads : (split ? 512 : 256));
-  const size_t lds_p = pair_lds_bytes(ncell, NSP, p.n_rp);
-  if (lds_p > ctx->max_lds)
-    return fail(ctx, CPC_ERR_UNSUPPORTED,
-                "histogram of " + std::to_string(ncell) + " cells does not fit in LDS (" + std::to_string(lds_p) +
-                    " bytes)");
-  const int item_size = ctx->item_size ? ctx->item_size : 512;
-  const int64_t nbig = std::max(co->n, ci->n);
-
-  // ---- geometry-specific set-up: grid, neighbour cells, windows
-  Grid G;
-  std::vector<int2> nbr;
-  int Rn = 0;
-  bool has_zfirst = false;
-  double zfirst_plus_L = 0;
-  SkyArgs SA{};
-  std::vector<Win> wins;
-  auto t_build = std::chrono::steady_clock::now();
-  if (!sky) {
-    const double L = p.box_size;
-    double frac = ctx->col_frac > 0 ? ctx->col_frac : 0.25;
-    int nc = std::max(1, (int)std::floor(L / (p.rp_max * frac)));
-    // keep enough galaxies per column for full waves
-    const int nc_dense = std::max(1, (int)std::floor(std::sqrt((double)std::max<int64_t>(nbig, 1) / 256.0)));
-    if (ctx->col_frac <= 0) nc = std::min(nc, nc_dense);
-    nc = std::min(nc, 1024);
-    G.ncx = G.ncy = nc;
-    G.hx = G.hy = L / nc;
-    const double h = G.hx;
-    // neighbour column offsets (circle cut at the largest binned separation)
-    const double rcut = std::sqrt(thr[p.n_rp]) * (1 + 1e-9) + 1e-9 * L;
-    Rn = (int)std::floor(p.rp_max / h) + 1;
-    for (int dy = -Rn; dy <= Rn; ++dy)
-      for (int dx = -Rn; dx <= Rn; ++dx) {
-        const double ex = std::max(0, std::abs(dx) - 1) * h, ey = std::max(0, std::abs(dy) - 1) * h;
-        if (ex * ex + ey * ey <= rcut * rcut) nbr.push_back(make_int2(dx, dy));
-      }
-    // first included galaxy of the inner catalog in upload order (PBz>0 sweep, corr.cpp:407-416)
-    for (int64_t j = 0; j < ci->n; ++j) {
-      const double inc = ci->h_inc.empty() ? 0.0 : ci->h_inc[j];
-      if (inc > p.include_prob_max) continue;
-      has_zfirst = true;
-      zfirst_plus_L = ci->h_z[j] + L * 1;
-      break;
-    }
-    if (autop && co->n > 0) {
-      // corels1 scans for PBz = 0 and PBz = -1 start at j = i+1 and stop at once when that next
-      // included galaxy lies below the window (do_pair_sorted, corr.cpp:71-73): bit 0 / bit 1
-      std::vector<uint8_t> af(co->n, 0);
-      int64_t nxt = -1;
-      for (int64_t i = co->n - 1; i >= 0; --i) {
-        if (nxt >= 0) {
-          const double z_min = co->h_z[i] + p.pi_min;
-          if (!(co->h_z[nxt] + L * 0 < z_min)) af[i] |= 1;
-          if (!(co->h_z[nxt] + L * -1 < z_min)) af[i] |= 2;
-        }
-        if (!(!co->h_inc.empty() && co->h_inc[i] > p.include_prob_max)) nxt = i;
-      }
-      if (check(ctx->aflags.ensure(co->n), st, "alloc")) return fail(ctx, st);
-      check(hipMemcpyAsync(ctx->aflags.p, af.data(), co->n, hipMemcpyHostToDevice, s), st, "copy");
-      check(hipStreamSynchronize(s), st, "sync");
-      if (st.code) return fail(ctx, st);
-    }
-  } else {
-    // per-galaxy windows of the outer catalog (lim_calc_sky), with the host libm
-    const SkyTable T{p.sky_H, p.sky_dA, p.sky_n, p.sky_dz, p.z_max, p.z_sep_max};
-    wins.resize(co->n);
-    double reach_zup = 0, reach_zdn = 0;
-    std::vector<double> dec_half;
-    // RA/Dec reach per bin of the outer redshift
-    double ozlo = INFINITY, ozhi = -INFINITY;
-    for (int64_t i = 0; i < co->n; ++i) {
-      if (!co->h_inc.empty() && co->h_inc[i] > p.include_prob_max) continue;
-      ozlo = std::min(ozlo, co->h_z[i]);
-      ozhi = std::max(ozhi, co->h_z[i]);
-    }
-    const int nb = 512;
-    if (!(ozlo <= ozhi)) ozlo = ozhi = 0.0;
-    const double rz0 = ozlo, rinv = nb / std::max(ozhi - ozlo, 1e-9);
-    std::vector<double2> reach(nb, make_double2(0.0, 0.0));
-    dec_half.reserve(co->n);
-    for (int64_t i = 0; i < co->n; ++i) {
-      const double inc = co->h_inc.empty() ? 0.0 : co->h_inc[i];
-      Win &w = wins[i];
-      if (inc > p.include_prob_max) {
-        w = Win{1, 0, 0, 0, 0, 0, 0, 0};  // never used: the galaxy is not in the pass
-        continue;
-      }
-      if (!sky_window(T, p.pi_min, p.pi_max, p.rp_max, co->h_x[i], co->h_y[i], co->h_z[i], w))
-        return fail(ctx, CPC_ERR_INVALID,
-                    "galaxy " + std::to_string(i) + ": its redshift window leaves the distance table or exceeds z_max "
-                    "(the reference exits here, sky_calcs.cpp:109-113)");
-      const int b = (int)std::fmax(0.0, std::fmin((double)(nb - 1), std::floor((co->h_z[i] - rz0) * rinv)));
-      double2 &rb = reach[b];
-      if (!(w.ramin <= w.ramax) || w.ramin <= 0 || w.ramax >= 360)
-        rb.x = INFINITY;
-      else
-        rb.x = std::max(rb.x, std::max(co->h_x[i] - w.ramin, w.ramax - co->h_x[i]) * (1 + 1e-12) + 1e-9);
-      const double dh = std::max(co->h_y[i] - w.decmin, w.decmax - co->h_y[i]);
-      rb.y = std::max(rb.y, dh * (1 + 1e-12) + 1e-9);
-      dec_half.push_back(dh);
-      reach_zup = std::max(reach_zup, w.zmax - co->h_z[i]);
-      reach_zdn = std::max(reach_zdn, co->h_z[i] - w.zmin);
-    }
-    // RA/Dec grid over both catalogs, cells about half the median window
-    double ralo = 360, rahi = 0, declo = 90, dechi = -90;
-    for (const cpc_catalog *c : {co, ci})
-      if (c->n) {
-        ralo = std::min(ralo, c->lo[0]);
-        rahi = std::max(rahi, c->hi[0]);
-        declo = std::min(declo, c->lo[1]);
-        dechi = std::max(dechi, c->hi[1]);
-      }
-    double hd = 1.0;
-    if (!dec_half.empty()) {
-      std::nth_element(dec_half.begin(), dec_half.begin() + dec_half.size() / 2, dec_half.end());
-      hd = dec_half[dec_half.size() / 2] * (ctx->col_frac > 0 ? ctx->col_frac : 0.5);
-    }
-    const double span_ra = std::max(rahi - ralo, 1e-6), span_dec = std::max(dechi - declo, 1e-6);
-    hd = std::max(hd, span_dec / 2048);
-    const double cdec = std::max(0.2, std::cos(0.5 * (declo + dechi) / kR2D));
-    const double hr = std::max(hd / cdec, span_ra / 2048);
-    G.ncx = std::max(1, (int)std::ceil(span_ra / hr));
-    G.ncy = std::max(1, (int)std::ceil(span_dec / hd));
-    G.hx = span_ra / G.ncx * (1 + 1e-12);
-    G.hy = span_dec / G.ncy * (1 + 1e-12);
-    G.x0 = ralo;
-    G.y0 = declo;
-    SA.reach_zup = reach_zup * (1 + 1e-12) + 1e-12;
-    SA.reach_zdn = reach_zdn * (1 + 1e-12) + 1e-12;
-    SA.reach_nb = nb;
-    SA.reach_z0 = rz0;
-    SA.reach_inv = rinv;
-    if (check(ctx->reach.ensure(nb), st, "alloc")) return fail(ctx, st);
-    check(hipMemcpyAsync(ctx->reach.p, reach.data(), nb * sizeof(double2), hipMemcpyHostToDevice, s), st, "copy");
-    SA.reach_tab = ctx->reach.p;
-    if (autop) {
-      // corels1's scan from j = i+1 stops at once when that next included galaxy lies below the
-      // window (corr.cpp:168-186): such outer galaxies take no pairs
-      int64_t nxt = -1;
-      for (int64_t i = co->n - 1; i >= 0; --i) {
-        const bool inc_i = !(!co->h_inc.empty() && co->h_inc[i] > p.include_prob_max);
-        if (inc_i && (nxt < 0 || co->h_z[nxt] < wins[i].zmin)) {
-          wins[i].zmin = 1.0;
-          wins[i].zmax = 0.0;
-        }
-        if (inc_i) nxt = i;
-      }
-    }
-    if (check(ctx->win_raw.ensure(std::max<int64_t>(co->n, 1)), st, "alloc")) return fail(ctx, st);
-    if (co->n)
-      check(hipMemcpyAsync(ctx->win_raw.p, wins.data(), co->n * sizeof(Win), hipMemcpyHostToDevice, s), st, "copy");
-  }
-
-  // ---- sides (sort keys are (cell, region, coin) packed in 32 bits, all ones meaning excluded)
-  if ((double)G.ncx * G.ncy * nreg * 2.0 >= 4294967295.0)
-    return fail(ctx, CPC_ERR_UNSUPPORTED, "too many cells x jackknife regions for 32-bit sort keys");
-  st = build_side(ctx, co, G, nreg, jk_on, p, has_zfirst, zfirst_plus_L, sky, sky ? ctx->win_raw.p : nullptr,
-                  (autop && !sky) ? ctx->aflags.p : nullptr, ctx->side_o);
-  if (st.code) return fail(ctx, st);
-  const bool same = (co == ci);
-  if (!same) {
-    st = build_side(ctx, ci, G, nreg, jk_on, p, has_zfirst, zfirst_plus_L, sky, nullptr, nullptr, ctx->side_i);
-    if (st.code) return fail(ctx, st);
-  }
-  Side &SO = ctx->side_o;
-  Side &SI = same ? ctx->side_o : ctx->side_i;
-  const double ms_build = ms_since(t_build);
-
-  // ---- work items (cost: block galaxies x tile galaxies in reach)
-  std::vector<Item> items_p, items_t;
-  std::vector<Chunk> chunks;
-  const int per_item = std::max(1, item_size / 64);
-  if (!sky) {
-    const int nc = G.ncx;
-    auto cost_of = [&](const Side &T) {
-      return [&T, &nbr, nc](const Group &g) {
-        const int bcx = g.col % nc, bcy = g.col / nc;
-        double c = 0;
-        for (const int2 &o : nbr) {
-          const int ux = ((bcx + o.x) % nc + nc) % nc, uy = ((bcy + o.y) % nc + nc) % nc;
-          c += (double)T.colcount[(size_t)uy * nc + ux];
-        }
-        return c;
-      };
-    };
-    make_items(SO, false, jk_on, per_item, cost_of(SI), items_p, chunks);
-    if (jk_on) make_items(SI, true, jk_on, per_item, cost_of(SO), items_t, chunks);
-  } else {
-    auto flat = [](const Group &) { return 1.0; };
-    make_items(SO, false, jk_on, per_item, flat, items_p, chunks);
-    if (jk_on) make_items(SI, true, jk_on, per_item, flat, items_t, chunks);
-  }
-
-  // ---- device tables and accumulators
-  const size_t nP = (size_t)2 * nreg, nT = (size_t)nreg;
-  if (check(ctx->thr.ensure(p.n_rp + 1), st, "alloc") ||
-      check(ctx->nbr.ensure(std::max<size_t>(nbr.size(), 1)), st, "alloc") ||
-      check(ctx->items.ensure(items_p.size() + items_t.size()), st, "alloc") ||
-      check(ctx->chunks.ensure(chunks.size()), st, "alloc") ||
-      check(ctx->accp.ensure(nP * NSP * ncell), st, "alloc") || check(ctx->accpn.ensure(nP * ncell), st, "alloc") ||
-      check(ctx->acct.ensure(nT * NST * ncell), st, "alloc") || check(ctx->acctn.ensure(nT * ncell), st, "alloc") ||
-      check(ctx->counters.ensure(4), st, "alloc"))
-    return fail(ctx, st);
-  const double two24 = std::ldexp(1.0, -24);
-  double gz = 1.0;
-  Guard gr{};
-  if (!sky) {
-    // guard bands of the FP32 classification from bounds on the FP32 error of dx, dy (column-local
-    // coordinates plus a rounded column offset) and of dz (absolute coordinates)
-    const double L = p.box_size, h = G.hx;
-    const double amax = 1.01 * h + 1e-9 * L;
-    const double omax = (Rn + 1) * h + 1e-9 * L;
-    const double eps_xy = 2.0 * (6 * amax + 2 * omax) * two24;
-    gr = rp_guard(p, dv, thr, eps_xy);
-    const double eps_z = 2.0 * 5 * L * two24;
-    const double pabs = std::max(std::fabs(p.pi_min), std::fabs(p.pi_max));
-    gz = eps_z / dv.psize + 8.0 * (p.n_pi + 2 * pabs / dv.psize + 2) * two24;
-  }
-  const bool use_split = split && (sky || gz < 0.25);
-  check(hipMemcpyAsync(ctx->thr.p, thr.data(), thr.size() * 8, hipMemcpyHostToDevice, s), st, "copy");
-  if (!nbr.empty())
-    check(hipMemcpyAsync(ctx->nbr.p, nbr.data(), nbr.size() * sizeof(int2), hipMemcpyHostToDevice, s), st, "copy");
-  if (!chunks.empty())
-    check(hipMemcpyAsync(ctx->chunks.p, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, s), st,
-          "copy");
-  if (!items_p.empty())
-    check(hipMemcpyAsync(ctx->items.p, items_p.data(), items_p.size() * sizeof(Item), hipMemcpyHostToDevice, s), st,
-          "copy");
-  if (!items_t.empty())
-    check(hipMemcpyAsync(ctx->items.p + items_p.size(), items_t.data(), items_t.size() * sizeof(Item),
-                         hipMemcpyHostToDevice, s),
-          st, "copy");
-  if (st.code) return fail(ctx, st);
-
-  // host-queue capacity for sky guard-band pairs; grown and the pass rerun if ever exceeded
-  int hq_cap = sky ? (int)std::min<int64_t>(std::max<int64_t>(1 << 16, nbig), 1 << 24) : 1;
-  std::vector<uint2> hq_pairs;
-  float kms = 0, kms_p = 0, kms_t = 0;
-  std::vector<double> accp(nP * NSP * ncell), acct(nT * NST * ncell);
-  std::vector<unsigned long long> accpn(nP * ncell), acctn(nT * ncell), counters(4);
-
-  for (int attempt = 0; attempt < 2; ++attempt) {
-    if (check(ctx->hq.ensure(hq_cap), st, "alloc")) return fail(ctx, st);
-    check(hipMemsetAsync(ctx->accp.p, 0, nP * NSP * ncell * 8, s), st, "memset");
-    check(hipMemsetAsync(ctx->accpn.p, 0, nP * ncell * 8, s), st, "memset");
-    check(hipMemsetAsync(ctx->acct.p, 0, nT * NST * ncell * 8, s), st, "memset");
-    check(hipMemsetAsync(ctx->acctn.p, 0, nT * ncell * 8, s), st, "memset");
-    check(hipMemsetAsync(ctx->counters.p, 0, 4 * 8, s), st, "memset");
-    check(hipEventRecord(ctx->ev0, s), st, "event");
-    if (!sky) {
-      KArgs K{};
-      K.L = p.box_size;
-      K.pmin = p.pi_min;
-      K.pmax = p.pi_max;
-      K.psize = dv.psize;
-      K.rmax = p.rp_max;
-      K.h = G.hx;
-      K.zmargin = 1e-7 * (p.box_size + std::max(std::fabs(p.pi_min), std::fabs(p.pi_max)));
-      K.nc = G.ncx;
-      K.n_rp = p.n_rp;
-      K.n_pi = p.n_pi;
-      K.ncell = ncell;
-      K.gws = p.g_with_shape;
-      K.n_nbr = (int)nbr.size();
-      K.lin = p.lin_bin;
-      K.pioff32 = (float)(-p.pi_min / dv.psize);
-      K.inv_psize32 = (float)(1.0 / dv.psize);
-      K.gz = (float)gz;
-      K.la = (float)(0.5 * std::log10(2.0) / dv.rsize);
-      K.lb = (float)(-std::log10(p.rp_min > 0 ? p.rp_min : 1.0) / dv.rsize);
-      K.lrmin = (float)p.rp_min;
-      K.linv = (float)(1.0 / dv.rsize);
-      K.thr = ctx->thr.p;
-      K.s_out_lo = gr.s_out_lo;
-      K.s_out_hi = gr.s_out_hi;
-      K.s_cert_lo = gr.s_cert_lo;
-      K.gA = gr.gA;
-      K.gB = gr.gB;
-      K.gC = gr.gC;
-      K.gD = gr.gD;
-      K.nbr = ctx->nbr.p;
-      K.counters = ctx->counters.p;
-      K.auto_mode = autop ? 1 : 0;
-      if (!items_p.empty()) {
-        KArgs A = K;
-        A.n_items = (int)items_p.size();
-        A.phase_mode = jk_on ? 1 : 0;
-        A.nslots = NSP;
-        A.items = ctx->items.p;
-        A.chunks = ctx->chunks.p;
-        A.bgrp = SO.d_groups.p;
-        A.tgrp = SI.d_groups.p;
-        A.tcol = SI.d_colbeg.p;
-        A.bidx = SO.idx.p;
-        A.tidx = SI.idx.p;
-        A.acc = ctx->accp.p;
-        A.accnum = ctx->accpn.p;
-        check(launch_pair(kind_code, use_split, false, A, threads, SO.rec.p, SO.zs.p, SI.rec.p, SI.zs.p, s), st,
-              "pair kernel");
-      }
-      check(hipEventRecord(ctx->ev2, s), st, "event");
-      if (!items_t.empty()) {
-        KArgs A = K;
-        A.n_items = (int)items_t.size();
-        A.phase_mode = 2;
-        A.nslots = NST;
-        A.items = ctx->items.p + items_p.size();
-        A.chunks = ctx->chunks.p;
-        A.bgrp = SI.d_groups.p;
-        A.tgrp = SO.d_groups.p;
-        A.tcol = SO.d_colbeg.p;
-        A.bidx = SI.idx.p;
-        A.tidx = SO.idx.p;
-        A.acc = ctx->acct.p;
-        A.accnum = ctx->acctn.p;
-        check(launch_pair(kind_code, use_split, true, A, threads, SI.rec.p, SI.zs.p, SO.rec.p, SO.zs.p, s), st,
-              "touch kernel");
-      }
-    } else {
-      SkyArgs A = SA;
-      A.pmin = p.pi_min;
-      A.pmax = p.pi_max;
-      A.psize = dv.psize;
-      A.n_rp = p.n_rp;
-      A.n_pi = p.n_pi;
-      A.ncell = ncell;
-      A.gws = p.g_with_shape;
-      A.lin = p.lin_bin;
-      A.ra0 = G.x0;
-      A.dec0 = G.y0;
-      A.hra = G.hx;
-      A.hdec = G.hy;
-      A.nra = G.ncx;
-      A.ndec = G.ncy;
-      A.dsum = 4e-15;
-      A.auto_mode = autop ? 1 : 0;
-      A.thr = ctx->thr.p;
-      A.counters = ctx->counters.p;
-      A.hq = ctx->hq.p;
-      A.hq_cap = hq_cap;
-      if (!items_p.empty()) {
-        SkyArgs B = A;
-        B.n_items = (int)items_p.size();
-        B.phase_mode = jk_on ? 1 : 0;
-        B.nslots = NSP;
-        B.items = ctx->items.p;
-        B.chunks = ctx->chunks.p;
-        B.bgrp = SO.d_groups.p;
-        B.tgrp = SI.d_groups.p;
-        B.tcol = SI.d_colbeg.p;
-        B.acc = ctx->accp.p;
-        B.accnum = ctx->accpn.p;
-        check(launch_sky(kind_code, use_split, false, B, threads, sky_side(SO), sky_side(SI), s),
-              st, "sky pair kernel");
-      }
-      check(hipEventRecord(ctx->ev2, s), st, "event");
-      if (!items_t.empty()) {
-        SkyArgs B = A;
-        B.n_items = (int)items_t.size();
-        B.phase_mode = 2;
-        B.nslots = NST;
-        B.items = ctx->items.p + items_p.size();
-        B.chunks = ctx->chunks.p;
-        B.bgrp = SI.d_groups.p;
-        B.tgrp = SO.d_groups.p;
-        B.tcol = SO.d_colbeg.p;
-        B.acc = ctx->acct.p;
-        B.accnum = ctx->acctn.p;
-        check(launch_sky(kind_code, use_split, true, B, threads, sky_side(SI), sky_side(SO), s),
-              st, "sky touch kernel");
-      }
-    }
-    check(hipEventRecord(ctx->ev1, s), st, "event");
-    if (!sky && !autop && co->n > 0 && ci->n > 0) {  // sparse-window diagnostic (corels2's starts pointer)
-      SparseArgs sa{co->d_z, co->d_inc, co->n, ci->z_sorted ? ci->d_z : ci->d_zsorted, ci->n,
-                    p.box_size, p.pi_min, p.pi_max, p.include_prob_max};
-      check(launch_sparse(sa, ctx->counters.p + 2, s), st, "sparse");
-    }
-    if (st.code) return fail(ctx, st);
-    check(hipMemcpyAsync(counters.data(), ctx->counters.p, 4 * 8, hipMemcpyDeviceToHost, s), st, "copy");
-    check(hipStreamSynchronize(s), st, "sync");
-    if (st.code) return fail(ctx, st);
-    if (sky && counters[3] > (unsigned long long)hq_cap) {
-      if (counters[3] > (unsigned long long)INT32_MAX) return fail(ctx, CPC_ERR_NOMEM, "guard-band queue overflow");
-      hq_cap = (int)counters[3];
-      continue;  // rerun with a queue large enough for every guard-band pair
-    }
-    break;
-  }
-  check(hipMemcpyAsync(accp.data(), ctx->accp.p, accp.size() * 8, hipMemcpyDeviceToHost, s), st, "copy");
-  check(hipMemcpyAsync(accpn.data(), ctx->accpn.p, accpn.size() * 8, hipMemcpyDeviceToHost, s), st, "copy");
-  check(hipMemcpyAsync(acct.data(), ctx->acct.p, acct.size() * 8, hipMemcpyDeviceToHost, s), st, "copy");
-  check(hipMemcpyAsync(acctn.data(), ctx->acctn.p, acctn.size() * 8, hipMemcpyDeviceToHost, s), st, "copy");
-  if (sky && counters[3] > 0) {
-    hq_pairs.resize(counters[3]);
-    check(hipMemcpyAsync(hq_pairs.data(), ctx->hq.p, hq_pairs.size() * sizeof(uint2), hipMemcpyDeviceToHost, s), st,
-          "copy");
-  }
-  check(hipStreamSynchronize(s), st, "sync");
-  if (st.code) return fail(ctx, st);
-  (void)hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1);
-  (void)hipEventElapsedTime(&kms_p, ctx->ev0, ctx->ev2);
-  (void)hipEventElapsedTime(&kms_t, ctx->ev2, ctx->ev1);
-
-  // ---- pairs the device could not bin bit-exactly: bin them with the host libm
-  HostHist HH;
-  const bool host_pairs = !hq_pairs.empty();
-  if (host_pairs) {
-    HH.init(R, ncell);
-    for (const uint2 &q : hq_pairs) host_sky_pair(p, dv, co, (int64_t)q.x, ci, (int64_t)q.y, jk_on, HH);
-  }
-  int64_t sparse = (int64_t)counters[2];
-  if (sky && ci->z_sorted && ci->n > 0) {  // sparse-window diagnostic on the host
-    for (int64_t i = 0; i < co->n; ++i) {
-      const double inc = co->h_inc.empty() ? 0.0 : co->h_inc[i];
-      if (inc > p.include_prob_max) continue;
-      auto it = std::lower_bound(ci->h_z.begin(), ci->h_z.end(), wins[i].zmin);
-      if (it != ci->h_z.end() && *it > wins[i].zmax) ++sparse;
-    }
-  }
-
-  // ---- assemble the reference layout
-  const size_t cells_all = (size_t)R * ncell;
-  std::fill(out->num, out->num + cells_all, 0);
-  std::fill(out->wt_num, out->wt_num + cells_all, 0.0);
-  std::fill(out->val, out->val + cells_all * CPC_NFIELD, 0.0);
-  std::fill(out->err, out->err + cells_all * CPC_NFIELD, 0.0);
-  std::fill(out->num_rp, out->num_rp + (size_t)R * p.n_rp, 0);
-  // full sample = every primary target (+ host-binned pairs)
-  for (int t = 0; t < 2 * nreg; ++t)
-    for (int c = 0; c < ncell; ++c) {
-      const unsigned long long n = accpn[(size_t)t * ncell + c];
-      if (!n) continue;
-      out->num[c] += (int64_t)n;
-      out->wt_num[c] += accp[((size_t)t * NSP + 0) * ncell + c];
-      for (int v = 0; v < NV; ++v) {
-        const int f = kValField[kind_code][v];
-        out->val[(size_t)c * CPC_NFIELD + f] += accp[((size_t)t * NSP + 1 + v) * ncell + c];
-        out->err[(size_t)c * CPC_NFIELD + f] += accp[((size_t)t * NSP + 1 + NV + v) * ncell + c];
-      }
-    }
-  if (host_pairs)
-    for (int c = 0; c < ncell; ++c) {
-      out->num[c] += HH.num[c];
-      out->wt_num[c] += HH.wt[c];
-      for (int f = 0; f < CPC_NFIELD; ++f) {
-        out->val[(size_t)c * CPC_NFIELD + f] += HH.val[(size_t)c * CPC_NFIELD + f];
-        out->err[(size_t)c * CPC_NFIELD + f] += HH.err[(size_t)c * CPC_NFIELD + f];
-      }
-    }
-  if (jk_on) {
-    std::vector<double> tv((size_t)ncell * CPC_NFIELD), tw(ncell);
-    std::vector<int64_t> tn(ncell);
-    for (int r = 0; r < p.n_jk; ++r) {
-      // touch_r (jk.cpp:138-171): num, wt_num and data[].val; shape vals only when which_corr != 0
-      std::fill(tv.begin(), tv.end(), 0.0);
-      std::fill(tw.begin(), tw.end(), 0.0);
-      std::fill(tn.begin(), tn.end(), 0);
-      for (int src = 0; src < 2; ++src) {
-        const double *acc = src == 0 ? accp.data() : acct.data();
-        const unsigned long long *num = src == 0 ? accpn.data() : acctn.data();
-        const int NS = src == 0 ? NSP : NST;
-        const int t = src == 0 ? 2 * r + 1 : r;
-        for (int c = 0; c < ncell; ++c) {
-          const unsigned long long n = num[(size_t)t * ncell + c];
-          if (!n) continue;
-          tn[c] += (int64_t)n;
-          tw[c] += acc[((size_t)t * NS + 0) * ncell + c];
-          for (int v = 0; v < NV; ++v)
-            tv[(size_t)c * CPC_NFIELD + kValField[kind_code][v]] += acc[((size_t)t * NS + 1 + v) * ncell + c];
-        }
-      }
-      if (host_pairs)
-        for (int c = 0; c < ncell; ++c) {
-          const size_t o = (size_t)(r + 1) * ncell + c;
-          tn[c] += HH.num[o];
-          tw[c] += HH.wt[o];
-          for (int f = 0; f < CPC_NFIELD; ++f) tv[(size_t)c * CPC_NFIELD + f] += HH.val[o * CPC_NFIELD + f];
-        }
-      if (p.which_corr == 0) std::fill(tv.begin(), tv.end(), 0.0);
-      const size_t reg = (size_t)(r + 1);
-      for (int c = 0; c < ncell; ++c) {
-        const size_t o = reg * ncell + c;
-        if (p.jk_mode == CPC_JK_TOUCH) {
-          out->num[o] = tn[c];
-          out->wt_num[o] = tw[c];
-          for (int f = 0; f < CPC_NFIELD; ++f) out->val[o * CPC_NFIELD + f] = tv[(size_t)c * CPC_NFIELD + f];
-        } else {
-          // final_jk_bins: add_bins(full, touch, jk, subtract) over every field (outp.cpp:242-288)
-          out->num[o] = out->num[c] + tn[c] * -1;
-          out->wt_num[o] = out->wt_num[c] + tw[c] * -1;
-          for (int f = 0; f < CPC_NFIELD; ++f) {
-            out->val[o * CPC_NFIELD + f] = out->val[(size_t)c * CPC_NFIELD + f] + tv[(size_t)c * CPC_NFIELD + f] * -1;
-            out->err[o * CPC_NFIELD + f] = out->err[(size_t)c * CPC_NFIELD + f];
-          }
-        }
+  if (!best) return fail(ctx, CPC_ERR_INVALID, "null argument");
+  bool any_box = false, any_sky = false;
+  for (int i = 0; i < n_jobs && jobs; ++i)
+    if (jobs[i].params) (jobs[i].params->geometry == CPC_GEOM_SKY ? any_sky : any_box) = true;
+  // candidate values per knob; the sky kernel runs at most 256 threads per group
+  const std::vector<double> fracs = any_sky && !any_box ? std::vector<double>{0.0, 0.35, 0.75}
+                                                        : std::vector<double>{0.0, 0.35, 0.5};
+  const std::vector<int> items = {0, 256, 1024};
+  const std::vector<int> threads = any_sky ? std::vector<int>{0, 128} : std::vector<int>{0, 256};
+  cpc_tuning cur{0.0, 0, 0};
+  double cur_ms = 0;
+  int runs = 0;
+  auto time_of = [&](const cpc_tuning &t, double &ms) -> cpc_status {
+    ctx->col_frac = t.col_frac;
+    ctx->item_size = t.item_size;
+    ctx->block_threads = t.block_threads;
+    cpc_stats tot{};
+    const cpc_status rc = run_jobs(ctx, n_jobs, jobs, &tot, true);
+    ++runs;
+    ms = tot.ms_kernel;
+    return rc;
+  };
+  cpc_status rc = time_of(cur, cur_ms);
+  for (int knob = 0; knob < 3 && rc == CPC_OK; ++knob) {
+    const size_t nv = knob == 0 ? fracs.size() : knob == 1 ? items.size() : threads.size();
+    cpc_tuning base = cur;
+    for (size_t v = 1; v < nv && rc == CPC_OK; ++v) {
+      cpc_tuning t = base;
+      if (knob == 0) t.col_frac = fracs[v];
+      if (knob == 1) t.item_size = items[v];
+      if (knob == 2) t.block_threads = threads[v];
+      double ms = 0;
+      rc = time_of(t, ms);
+      if (rc == CPC_OK && ms < 0.97 * cur_ms) {
+        cur = t;
+        cur_ms = ms;
       }
     }
   }
-  int64_t nbinned = 0;
-  for (int r = 0; r < R; ++r)
-    for (int k = 0; k < p.n_rp; ++k) {
-      int64_t sum = 0;
-      for (int q = 0; q < p.n_pi; ++q) sum += out->num[((size_t)r * p.n_rp + k) * p.n_pi + q];
-      out->num_rp[(size_t)r * p.n_rp + k] = sum;
-      if (r == 0) nbinned += sum;
-    }
-  if (stats) {
-    stats->n_binned = nbinned;
-    stats->n_candidates = (int64_t)counters[0];
-    stats->n_fp64_reeval = (int64_t)counters[1];
-    stats->n_sparse_windows = sparse;
-    stats->ms_kernel = kms;
-    stats->ms_primary = kms_p;
-    stats->ms_touch = kms_t;
-    stats->ms_build = ms_build;
-    stats->n_items_primary = (int64_t)items_p.size();
-    stats->n_host_pairs = (int64_t)hq_pairs.size();
-    stats->ms_total = ms_since(t_start);
+  if (rc != CPC_OK) {
+    ctx->col_frac = 0.0;
+    ctx->item_size = 0;
+    ctx->block_threads = 0;
+    return rc;
   }
-  ctx->err.clear();
-  return CPC_OK;
+  // leave the outputs as the chosen tuning computes them
+  ctx->col_frac = cur.col_frac;
+  ctx->item_size = cur.item_size;
+  ctx->block_threads = cur.block_threads;
+  double ms = 0;
+  rc = time_of(cur, ms);
+  *best = cur;
+  if (tried) *tried = runs;
+  return rc;
 }
 
 }  // extern "C"

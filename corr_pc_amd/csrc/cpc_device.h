@@ -162,9 +162,17 @@ __device__ __forceinline__ void pair_values(const Gal &g1, const Gal &g2, int gw
   pair_values_v<K, SPLIT>(g1, g2, gws, g2.x - g1.x, g2.y - g1.y, q);
 }
 
+// The KArgs fields the candidate loop of the box kernel reads, copied into registers once per work
+// item (the kernel reads KArgs from a table in constant memory).
+struct HotArgs {
+  double L, pmin, pmax, rmax, psize;
+  float gz, pioff32, s_out_lo, s_out_hi, s_cert_lo, linv, lrmin, la, lb, gA, gB, gC, gD, inv_psize32;
+  int n_pi, n_rp, lin, gws, auto_mode;
+};
+
 // rp bin of a squared separation against the FP64 thresholds thr[0..n_rp]: bin k iff
 // thr[k] <= s < thr[k+1].  An FP32 logarithm gives the starting guess.
-__device__ __forceinline__ int rp_bin64(const KArgs &P, const double *thr, double s) {
+__device__ __forceinline__ int rp_bin64(const HotArgs &P, const double *thr, double s) {
   const float sf = (float)s;
   float kf = P.lin ? (sqrtf(sf) - P.lrmin) * P.linv : __log2f(sf) * P.la + P.lb;
   kf = fminf(fmaxf(kf, -1.0f), (float)P.n_rp);
@@ -177,7 +185,7 @@ __device__ __forceinline__ int rp_bin64(const KArgs &P, const double *thr, doubl
 // The reference's candidate test and binning for outer (x1,y1,z1), inner (x2,y2,z2) seen through
 // image (PBx, PBy, PBz): do_pair_sorted (corr.cpp:67-86) then corel_bin (corels.cpp:21-46).
 // Returns the histogram cell n1*n_pi + n2, or -1.
-__device__ __forceinline__ int exact_cell(const KArgs &P, const double *thr, double x1, double y1, double z1,
+__device__ __forceinline__ int exact_cell(const HotArgs &P, const double *thr, double x1, double y1, double z1,
                                           double x2, double y2, double z2, int PBx, int PBy, int PBz) {
   const double L = P.L;
   const double zz = z2 + L * PBz;

@@ -53,25 +53,31 @@ struct alignas(16) Chunk {
   int32_t group, first, count, pad;
 };
 
-// A unit of work (one workgroup): chunks [begin, begin + n) of the chunk table, all from groups of
-// the same jackknife region and coin, so that they share one histogram target; group0 is one of
-// those groups.
+// A unit of work (one workgroup): chunks [begin, begin + n) of its job's chunk table, all from
+// groups of the same jackknife region and coin, so that they share one histogram target; group0 is
+// one of those groups.  `job` selects the pass arguments (KArgs) in the launch's job table.
 struct alignas(16) Item {
   int32_t begin, n, target, group0;
+  int32_t job, pad0, pad1, pad2;
 };
 
 // Histogram slots per pair kind.  The primary pass accumulates [wt, vals..., errs...], the touch
 // pass [wt, vals...].  Quantity codes: 0 = wt, 1..6 = data[f].val, 7..12 = data[f-6].err.
 constexpr int kMaxSlots = 13;
 
-// Kernel arguments (passed by value).
+// Arguments of one pass of one job.  A launch carries a device table of them and every work item
+// names its entry, so one launch serves the items of many jobs (catalogs, boxes, binnings).
 struct KArgs {
+  const PRec *brec;    // block-side records and their z
+  const double *bz;
+  const PRec *trec;    // tile-side records and their z
+  const double *tz;
   double L, pmin, pmax, psize, rmax;
   double h;            // column width
   double zmargin;      // slack of the tile z-range search
   int nc;              // columns per side
   int n_rp, n_pi, ncell;
-  int gws, n_nbr, n_items;
+  int gws, n_nbr;
   int phase_mode;      // 0 primary pass without jackknife, 1 primary with jackknife, 2 touch pass
   int nslots;          // f64 slots accumulated by this pass
   int auto_mode;       // corels1 pass: pairs through PBz <= 0 images only with the inner galaxy after
@@ -88,7 +94,6 @@ struct KArgs {
   float s_cert_lo;           // FP32 s below this is never classified in FP32
   float gA, gB, gC, gD;      // guard band of the rp bin index: g = gA + (gB + gC/r)/r + gD r, r = sqrt(s)
   const int2 *nbr;     // neighbour column offsets
-  const Item *items;
   const Chunk *chunks;
   const Group *bgrp;   // groups of the block side
   const Group *tgrp;   // groups of the tile side
@@ -187,8 +192,10 @@ hipError_t sort_u32(void *&tmp, size_t &tmp_bytes, const uint32_t *kin, uint32_t
 hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *kout, const int32_t *vin,
                     int32_t *vout, int64_t n, hipStream_t st);
 size_t pair_lds_bytes(int ncell, int nslots, int n_rp);
-hipError_t launch_pair(int kind, bool split, bool swap, const KArgs &P, int threads, const PRec *brec,
-                       const double *bz, const PRec *trec, const double *tz, hipStream_t st);
+// One launch over `n_items` items whose KArgs live in the device table `jobs` (all with the same
+// kind, precision and pass direction); `lds` covers the largest histogram among them.
+hipError_t launch_pair(int kind, bool split, bool swap, const KArgs *jobs, const Item *items, int n_items,
+                       size_t lds, int threads, hipStream_t st);
 // Record arrays of one side of a sky pass.
 struct SkySide {
   const PRec *rec;

@@ -45,14 +45,27 @@ __device__ __forceinline__ double rl_d(unsigned lo, unsigned hi, int l) {
 }
 __device__ __forceinline__ unsigned long long popc_ballot(bool b) { return (unsigned long long)__popcll(__ballot(b)); }
 
+using CKArgs = __attribute__((address_space(4))) const KArgs;
+
 template <int K, bool SPLIT, bool SWAP>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT ? 4 : 2))) pair_kernel(KArgs P, const PRec *__restrict__ brec,
-                                                   const double *__restrict__ bz, const PRec *__restrict__ trec,
-                                                   const double *__restrict__ tz) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT ? 4 : 2))) pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
   constexpr int NV = KindNV<K>::NV;
   constexpr int NS = SWAP ? 1 + NV : 1 + 2 * NV;  // f64 slots accumulated
   constexpr int NQ = 1 + 2 * NV;                  // quantities computed
   extern __shared__ __align__(16) unsigned char smem[];
+  const Item it = items[blockIdx.x];
+  // the pass arguments of the item's job, read as constant memory: wave-uniform scalar loads the
+  // compiler may repeat instead of holding them in registers, as it does with kernel arguments
+  const CKArgs &P = ((const CKArgs *)jobs)[it.job];
+  // the arguments of the inner loops, read once: the compiler keeps them in registers instead of
+  // reloading them inside the loop, where each reload would also wait for the LDS atomics in flight
+  const HotArgs H{P.L,         P.pmin, P.pmax,  P.rmax, P.psize, P.gz, P.pioff32, P.s_out_lo, P.s_out_hi,
+                  P.s_cert_lo, P.linv, P.lrmin, P.la, P.lb, P.gA, P.gB, P.gC, P.gD, P.inv_psize32, P.n_pi, P.n_rp,
+                  P.lin, P.gws, P.auto_mode};
+  const PRec *__restrict__ brec = P.brec;
+  const double *__restrict__ bz = P.bz;
+  const PRec *__restrict__ trec = P.trec;
+  const double *__restrict__ tz = P.tz;
   const int ncell = P.ncell;
   unsigned *lnum = reinterpret_cast<unsigned *>(smem);
   double *lacc = reinterpret_cast<double *>(smem + ((ncell * 4 + 15) & ~15));
@@ -63,9 +76,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
   const int wave = uni(tid >> 6);
   const int nwave = blockDim.x >> 6;
 
-  for (int k = tid; k <= P.n_rp; k += blockDim.x) lthr[k] = P.thr[k];
+  for (int k = tid; k <= H.n_rp; k += blockDim.x) lthr[k] = P.thr[k];
 
-  const Item it = P.items[blockIdx.x];
   const Group bg = P.bgrp[it.group0];  // region and coin of every chunk of the item
   // phase_mode: 0 primary without jackknife, 1 primary with jackknife, 2 touch pass
   const bool two_phase = (P.phase_mode == 1) && (bg.coin == 0);
@@ -95,7 +107,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
       const int p = base + (valid ? lane : 0);
       const PRec me = brec[p];
       const double mz = bz[p];
-      const int midx = P.auto_mode ? P.bidx[p] : 0;
+      const int midx = H.auto_mode ? P.bidx[p] : 0;
       const double wzlo = bz[base];
       const double wzhi = bz[base + cnt - 1];
 
@@ -116,8 +128,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
         const int tcx = ux - sx * P.nc, tcy = uy - sy * P.nc;
         const int tcoln = tcy * P.nc + tcx;
         const int g0 = P.tcol[tcoln], g1 = P.tcol[tcoln + 1];
-        const float Ox = (float)((double)(tcx - bcx) * P.h + P.L * sx);
-        const float Oy = (float)((double)(tcy - bcy) * P.h + P.L * sy);
+        const float Ox = (float)((double)(tcx - bcx) * P.h + H.L * sx);
+        const float Oy = (float)((double)(tcy - bcy) * P.h + H.L * sy);
         for (int g = g0; g < g1; ++g) {
           const Group tg = P.tgrp[g];
           if (filt == 1 && tg.region != bg.region) continue;
@@ -125,11 +137,11 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
           for (int sz = -1; sz <= 1; ++sz) {
             double lo, hi;
             if (!SWAP) {
-              lo = wzlo + P.pmin - P.L * sz - P.zmargin;
-              hi = wzhi + P.pmax - P.L * sz + P.zmargin;
+              lo = wzlo + H.pmin - H.L * sz - P.zmargin;
+              hi = wzhi + H.pmax - H.L * sz + P.zmargin;
             } else {
-              lo = wzlo - P.pmax - P.L * sz - P.zmargin;
-              hi = wzhi - P.pmin - P.L * sz + P.zmargin;
+              lo = wzlo - H.pmax - H.L * sz - P.zmargin;
+              hi = wzhi - H.pmin - H.L * sz + P.zmargin;
             }
             if (hi < tg.zlo || lo > tg.zhi) continue;
             const int j0 = zsearch<false>(tz, tg.start, tg.end, lo);
@@ -138,10 +150,10 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
             // image of the inner galaxy in the reference's terms
             const int PBx = SWAP ? -sx : sx, PBy = SWAP ? -sy : sy, PBz = SWAP ? -sz : sz;
             const bool lane_ok = valid && (SWAP || image_allowed(me.meta, PBx, PBy, PBz));
-            const bool ordered = P.auto_mode && PBz <= 0;  // the inner galaxy must come after the outer
+            const bool ordered = H.auto_mode && PBz <= 0;  // the inner galaxy must come after the outer
             // FP32 offsets of this lane to the tile column: d = tile local + offset
-            const float ax = Ox - me.xl, ay = Oy - me.yl, az = (float)(P.L * sz) - me.zf;
-            const float pinv = SWAP ? -P.inv_psize32 : P.inv_psize32;  // pi = -dz in the touch pass
+            const float ax = Ox - me.xl, ay = Oy - me.yl, az = (float)(H.L * sz) - me.zf;
+            const float pinv = SWAP ? -H.inv_psize32 : H.inv_psize32;  // pi = -dz in the touch pass
 
             uint4 cq = tq[(size_t)j0 * 4 + lane];
             double cz = tz[j0 + (lane & (kBatch - 1))];  // z of tile t in lane t
@@ -177,26 +189,26 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
                   const float dy = rl_f(cq.y, l3) + ay;
                   const float dz = rl_f(cq.z, l3) + az;
                   const float s32 = fmaf(dx, dx, dy * dy);
-                  const float uu = fmaf(dz, pinv, P.pioff32);
+                  const float uu = fmaf(dz, pinv, H.pioff32);
                   const float fu = floorf(uu);
                   const float fr = uu - fu;
-                  const bool near_pi = (fr < P.gz) || (fr > 1.0f - P.gz);
-                  const bool pi_in = (fu >= 0.0f) && (fu < (float)P.n_pi);
-                  const bool rp_out = (s32 < P.s_out_lo) || (s32 >= P.s_out_hi);
+                  const bool near_pi = (fr < H.gz) || (fr > 1.0f - H.gz);
+                  const bool pi_in = (fu >= 0.0f) && (fu < (float)H.n_pi);
+                  const bool rp_out = (s32 < H.s_out_lo) || (s32 >= H.s_out_hi);
                   if ((!near_pi && !pi_in) || rp_out) {
                     unc = false;
-                  } else if (!near_pi && s32 >= P.s_cert_lo) {
+                  } else if (!near_pi && s32 >= H.s_cert_lo) {
                     // rp bin index in FP32 and its guard band; outside the band the bin is that of
                     // the FP64 squared separation
                     // (s32 is finite and at least s_cert_lo here, so kf is finite)
                     const float rinv = __builtin_amdgcn_rsqf(s32);
-                    const float kf = P.lin ? (s32 * rinv - P.lrmin) * P.linv : __log2f(s32) * P.la + P.lb;
+                    const float kf = H.lin ? (s32 * rinv - H.lrmin) * H.linv : __log2f(s32) * H.la + H.lb;
                     const float fk = floorf(kf);
                     const float frk = kf - fk;
-                    const float gk = fmaf(P.gD, s32 * rinv, fmaf(fmaf(P.gC, rinv, P.gB), rinv, P.gA));
+                    const float gk = fmaf(H.gD, s32 * rinv, fmaf(fmaf(H.gC, rinv, H.gB), rinv, H.gA));
                     if (frk >= gk && frk <= 1.0f - gk) {
                       unc = false;
-                      if (fk >= 0.0f && fk < (float)P.n_rp) cell = (int)fmaf(fk, (float)P.n_pi, fu);
+                      if (fk >= 0.0f && fk < (float)H.n_rp) cell = (int)fmaf(fk, (float)H.n_pi, fu);
                     }
                   }
                 }
@@ -208,9 +220,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
                                                       __builtin_amdgcn_readlane(__double2loint(cz), t));
                   if (unc) {
                     if (!SWAP)
-                      cell = exact_cell(P, lthr, me.x, me.y, mz, tx, ty, tzz, PBx, PBy, PBz);
+                      cell = exact_cell(H, lthr, me.x, me.y, mz, tx, ty, tzz, PBx, PBy, PBz);
                     else
-                      cell = exact_cell(P, lthr, tx, ty, tzz, me.x, me.y, mz, PBx, PBy, PBz);
+                      cell = exact_cell(H, lthr, tx, ty, tzz, me.x, me.y, mz, PBx, PBy, PBz);
                   }
                 }
                 if (__ballot(cell >= 0) != 0ull) {
@@ -229,9 +241,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
                     const Gal gme{me.x, me.y, me.w, me.e1, me.e2, me.phi};
                     double qv[NQ];
                     if (!SWAP)
-                      pair_values<K, SPLIT>(gme, gt, P.gws, qv);
+                      pair_values<K, SPLIT>(gme, gt, H.gws, qv);
                     else
-                      pair_values<K, SPLIT>(gt, gme, P.gws, qv);
+                      pair_values<K, SPLIT>(gt, gme, H.gws, qv);
 #pragma unroll
                     for (int s = 0; s < NS; ++s) run[s] += qv[s];
                   }
@@ -405,12 +417,12 @@ hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *ko
 }
 
 template <int K, bool SPLIT, bool SWAP>
-static hipError_t launch_pair_t(const KArgs &P, int threads, size_t lds, const PRec *brec, const double *bz,
-                                const PRec *trec, const double *tz, hipStream_t st) {
+static hipError_t launch_pair_t(const KArgs *jobs, const Item *items, int n_items, int threads, size_t lds,
+                                hipStream_t st) {
   auto fn = pair_kernel<K, SPLIT, SWAP>;
   hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  fn<<<P.n_items, threads, lds, st>>>(P, brec, bz, trec, tz);
+  fn<<<n_items, threads, lds, st>>>(jobs, items);
   return hipGetLastError();
 }
 
@@ -419,16 +431,15 @@ size_t pair_lds_bytes(int ncell, int nslots, int n_rp) {
   return (size_t)((ncell * 4 + 15) & ~15) + (size_t)nslots * ncell * 8 + (size_t)(n_rp + 1) * 8;
 }
 
-hipError_t launch_pair(int kind, bool split, bool swap, const KArgs &P, int threads, const PRec *brec,
-                       const double *bz, const PRec *trec, const double *tz, hipStream_t st) {
-  if (P.n_items == 0) return hipSuccess;
-  const size_t lds = pair_lds_bytes(P.ncell, P.nslots, P.n_rp);
-#define CPC_DISPATCH(KK)                                                                                  \
-  if (kind == KK) {                                                                                       \
-    if (split && swap) return launch_pair_t<KK, true, true>(P, threads, lds, brec, bz, trec, tz, st);   \
-    if (split) return launch_pair_t<KK, true, false>(P, threads, lds, brec, bz, trec, tz, st);          \
-    if (swap) return launch_pair_t<KK, false, true>(P, threads, lds, brec, bz, trec, tz, st);           \
-    return launch_pair_t<KK, false, false>(P, threads, lds, brec, bz, trec, tz, st);                    \
+hipError_t launch_pair(int kind, bool split, bool swap, const KArgs *jobs, const Item *items, int n_items,
+                       size_t lds, int threads, hipStream_t st) {
+  if (n_items == 0) return hipSuccess;
+#define CPC_DISPATCH(KK)                                                                          \
+  if (kind == KK) {                                                                               \
+    if (split && swap) return launch_pair_t<KK, true, true>(jobs, items, n_items, threads, lds, st); \
+    if (split) return launch_pair_t<KK, true, false>(jobs, items, n_items, threads, lds, st);        \
+    if (swap) return launch_pair_t<KK, false, true>(jobs, items, n_items, threads, lds, st);         \
+    return launch_pair_t<KK, false, false>(jobs, items, n_items, threads, lds, st);                  \
   }
   CPC_DISPATCH(kDD)
   CPC_DISPATCH(kSD)

@@ -72,6 +72,16 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TuningC(ctypes.Structure):
+    _fields_ = [("col_frac", c_double), ("item_size", c_int), ("block_threads", c_int)]
+
+
+class JobC(ctypes.Structure):
+    """cpc_job: one pass of a cpc_count_batch call."""
+    _fields_ = [("outer", c_void_p), ("inner", c_void_p), ("params", POINTER(Params)), ("out", POINTER(HistC)),
+                ("stats", POINTER(Stats))]
+
+
 class EngineError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"cpc status {status}: {msg}")
@@ -193,6 +203,10 @@ def load_library(path=LIB_PATH):
     lib.cpc_catalog_free.restype = None
     lib.cpc_count.argtypes = [c_void_p, c_void_p, c_void_p, POINTER(Params), POINTER(HistC), POINTER(Stats)]
     lib.cpc_count.restype = c_int
+    lib.cpc_count_batch.argtypes = [c_void_p, c_int, POINTER(JobC), POINTER(Stats)]
+    lib.cpc_count_batch.restype = c_int
+    lib.cpc_autotune.argtypes = [c_void_p, c_int, POINTER(JobC), POINTER(TuningC), POINTER(c_int)]
+    lib.cpc_autotune.restype = c_int
     lib.cpc_rp_thresholds.argtypes = [POINTER(Params), _dp]
     lib.cpc_rp_thresholds.restype = c_int
     lib.cpc_rand_stream.argtypes = [ctypes.c_uint32, c_int64, c_int64, _dp]
@@ -208,8 +222,8 @@ def load_library(path=LIB_PATH):
 
 
 EXPORTED_SYMBOLS = ["cpc_init", "cpc_destroy", "cpc_last_error", "cpc_version", "cpc_catalog_upload",
-                    "cpc_catalog_free", "cpc_count", "cpc_rp_thresholds", "cpc_rand_stream", "cpc_set_tuning",
-                    "cpc_bin_edges", "cpc_box_rr"]
+                    "cpc_catalog_free", "cpc_count", "cpc_count_batch", "cpc_rp_thresholds", "cpc_rand_stream",
+                    "cpc_set_tuning", "cpc_autotune", "cpc_bin_edges", "cpc_box_rr"]
 
 
 REFERENCE_SEED = 124693517  # seed(), jk.cpp:40-43
@@ -302,6 +316,38 @@ class Engine:
         self._check(self.lib.cpc_count(self.ctx, ho, hi, ctypes.byref(params), ctypes.byref(hc), ctypes.byref(st)),
                     "count")
         return hist, st.as_dict()
+
+    def _jobs(self, jobs):
+        n = len(jobs)
+        arr = (JobC * max(n, 1))()
+        keep = []
+        for k, (outer, inner, params) in enumerate(jobs):
+            ho = self.upload(outer)
+            hi = ho if inner is outer else self.upload(inner)
+            hist = Histogram.for_params(params)
+            hc = hist.c_struct()
+            st = Stats()
+            keep.append((params, hist, hc, st))
+            arr[k] = JobC(ho, hi, ctypes.pointer(params), ctypes.pointer(hc), ctypes.pointer(st))
+        return n, arr, keep
+
+    def count_batch(self, jobs):
+        """Many passes in one call (cpc_count_batch): `jobs` is a list of (outer, inner, params).
+        Returns ([(Histogram, stats dict)] in job order, total stats dict)."""
+        n, arr, keep = self._jobs(jobs)
+        tot = Stats()
+        self._check(self.lib.cpc_count_batch(self.ctx, n, arr, ctypes.byref(tot)), "count_batch")
+        return [(h, st.as_dict()) for (_, h, _, st) in keep], tot.as_dict()
+
+    def autotune(self, jobs):
+        """cpc_autotune over (outer, inner, params) jobs; the context keeps the chosen tuning.
+        Returns (tuning dict, number of batches run, [(Histogram, stats dict)])."""
+        n, arr, keep = self._jobs(jobs)
+        best = TuningC()
+        tried = c_int(0)
+        self._check(self.lib.cpc_autotune(self.ctx, n, arr, ctypes.byref(best), ctypes.byref(tried)), "autotune")
+        tun = {"col_frac": best.col_frac, "item_size": best.item_size, "block_threads": best.block_threads}
+        return tun, tried.value, [(h, st.as_dict()) for (_, h, _, st) in keep]
 
     def close(self):
         if self.ctx is not None:

@@ -188,6 +188,52 @@ def catalogs(config, n_data=None, n_rand=None):
 
 
 # ---------------------------------------------------------------------------------------------
+# Batched mocks: many independent wgg boxes of uneven size
+
+MOCK_CONFIG = 3
+MOCK_BOX = 250.0
+MOCK_N_MIN, MOCK_N_MAX = 10_000, 500_000
+MOCK_PER_CLUMP = 60
+# binning of the plain wgg box (10 log rp bins in [0.1, 30), 20 pi bins in [-60, 60)), 2x2x2 jackknife
+MOCK_BINNING = CONFIGS[0]
+
+
+def mock_sizes(n_mocks=256, n_min=MOCK_N_MIN, n_max=MOCK_N_MAX):
+    """Data sizes of the mocks, log-uniform in [n_min, n_max] (catalog id 0 of the mock configuration)."""
+    rng = _rng(MOCK_CONFIG, 0)
+    return np.exp(rng.uniform(np.log(n_min), np.log(n_max), n_mocks)).astype(np.int64)
+
+
+def mock_catalogs(k, n):
+    """Catalogs S, Rs, D, Rd of mock k with n data galaxies: clustered as the wg+ box (30% of the
+    galaxies in Gaussian clumps of sigma 1, MOCK_PER_CLUMP members per clump), 2n uniform randoms,
+    unit weights, 2x2x2 jackknife; positions from catalog id 2 + k.  Every mock is its own corr_pc
+    run, so each draws its coins from the start of the reference's stream (read order S, Rs, D, Rd)."""
+    L = MOCK_BOX
+    rng = _rng(MOCK_CONFIG, 2 + k)
+    n_clumps = max(1, int(round(0.3 * n / MOCK_PER_CLUMP)))
+    pos, _ = positions(rng, n, L, True, n_clumps)
+    rpos, _ = positions(rng, 2 * n, L)
+
+    def cols(p):
+        c = {"x": p[:, 0], "y": p[:, 1], "z": p[:, 2], "w": None, "e1": None, "e2": None, "phi": None,
+             "jk": jk_grid(p[:, 0], p[:, 1], p[:, 2], L, 2)}
+        return _sorted(c)
+
+    data, rand = cols(pos), cols(rpos)
+    nr = 2 * n
+    stream = rand_stream(n + 2 * nr + n + 2 * nr)
+    s_jkp, rs_draw = stream[:n], stream[n:n + 2 * nr]
+    d_jkp, rd_draw = stream[n + 2 * nr:2 * n + 2 * nr], stream[2 * n + 2 * nr:]
+
+    def cat(c, jkp, inc):
+        return Catalog(x=c["x"], y=c["y"], z=c["z"], jk=c["jk"], jk_prob=jkp, include_prob=inc)
+
+    return {"S": cat(data, s_jkp, None), "Rs": cat(rand, rs_draw[0::2], rs_draw[1::2]),
+            "D": cat(data, d_jkp, None), "Rd": cat(rand, rd_draw[0::2], rd_draw[1::2]), "config": MOCK_BINNING}
+
+
+# ---------------------------------------------------------------------------------------------
 # Sky survey (RA/Dec/redshift)
 
 C_LIGHT = 299792.458  # km/s

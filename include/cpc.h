@@ -157,6 +157,27 @@ void cpc_catalog_free(cpc_ctx *ctx, cpc_catalog *cat);
 cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *outer, const cpc_catalog *inner,
                      const cpc_params *params, cpc_hist *out, cpc_stats *stats);
 
+/* One pass of a batch: the arguments of one cpc_count call. */
+typedef struct cpc_job {
+  const cpc_catalog *outer;
+  const cpc_catalog *inner;
+  const cpc_params *params;
+  cpc_hist *out;
+  cpc_stats *stats; /* may be NULL */
+} cpc_job;
+
+/* Many passes at once: every job's `out` and `stats` as cpc_count(job) would write them (integer
+ * counts identical; FP64 sums equal up to summation order).  The reference runs its passes one after
+ * another (do_corr.cpp:63-131, once per catalog pair); here the work items of all box jobs are merged
+ * into one launch per pass direction (the primary pass, then the jackknife touch pass; one more
+ * launch per further pair kind or precision in the batch), ordered by cost across jobs, so small
+ * catalogs fill the device together and only one tail remains.  Sky jobs run one after another
+ * within the call.  Jobs are validated before any work starts: on failure no output is written and
+ * the message names the job.  The records of all jobs are resident on the device together.
+ * `total`, if non-NULL, receives summed counters; its times are those of the whole call.  In each
+ * job's stats the kernel times are those of the launches that served it (shared by the batch). */
+cpc_status cpc_count_batch(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *total);
+
 /* The thresholds the engine bins rp with.  Box: thr[k] (k = 0..n_rp) is the smallest double s >= 0
  * for which the reference's bin index of rp = sqrt(s) (corels.cpp:31-34, with the host libm's log10)
  * is >= k.  Sky: the same on rp itself (rp_calc_sky, sky_calcs.cpp:91-96).  Exposed so that callers
@@ -187,6 +208,22 @@ cpc_status cpc_rand_stream(uint32_t seed, int64_t skip, int64_t n, double *out);
  * automatic choice).  Used by the autotuner; results do not depend on them beyond floating-point
  * summation order. */
 cpc_status cpc_set_tuning(cpc_ctx *ctx, double col_frac, int item_size, int block_threads);
+
+/* A choice of the cpc_set_tuning knobs (0 = automatic). */
+typedef struct cpc_tuning {
+  double col_frac;
+  int item_size;
+  int block_threads;
+} cpc_tuning;
+
+/* Autotuner of the decomposition (no counterpart in the reference, whose OpenMP loop has no such
+ * knobs): coordinate descent from the automatic choice over column width, block galaxies per work
+ * item and threads per work group, each candidate timed by the device time of cpc_count_batch on
+ * `jobs` (pass a representative subset to tune quickly).  A candidate replaces the current best
+ * only when at least 3% faster.  On return the context uses `best` (as after cpc_set_tuning), and
+ * every job's output and stats hold a valid result of the jobs.  `tried` (may be NULL) receives
+ * the number of batches run. */
+cpc_status cpc_autotune(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_tuning *best, int *tried);
 
 #ifdef __cplusplus
 }
