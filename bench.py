@@ -130,6 +130,54 @@ def bench_mocks(args):
     eng.close()
 
 
+def bench_large(args):
+    """The large box at its real shape: L = 1000, 2e7 galaxies (30% in 40,000 clumps) with shapes and
+    weights, 25 log rp bins in [0.1, 200), 60 pi bins in [-150, 150), 64 jackknife regions; the SD
+    pass of every `--large-stride`-th outer galaxy against all 2e7 inner galaxies (1/8 of the pass's
+    work by default)."""
+    import numpy as np
+
+    from corr_pc_amd import synth
+    from corr_pc_amd.engine import PREC_SPLIT, Engine
+    from corr_pc_amd.passes import config_datainfo, setup_pass
+
+    t = time.perf_counter()
+    cats = synth.catalogs(4, n_rand=10)
+    di = config_datainfo(cats["config"])
+    outer, inner, p, _ = setup_pass("SD", cats["D"], cats["S"], di, precision=PREC_SPLIT)
+    outer = outer.subset(np.arange(0, outer.n, args.large_stride))
+    log(f"catalogs ready in {time.perf_counter() - t:.1f}s: outer {outer.n}, inner {inner.n}")
+    eng = Engine(0)
+    eng.upload(outer)
+    eng.upload(inner)
+    for k in range(args.warmup):
+        _, st = eng.count(outer, inner, p)
+        log(f"warmup {k}: {st['ms_total']:.0f} ms")
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        _, st = eng.count(outer, inner, p)
+        log(f"step {k}: {st['ms_total']:.0f} ms (primary {st['ms_primary']:.0f} ms, touch {st['ms_touch']:.0f} ms, "
+            f"build {st['ms_build']:.0f} ms), binned {st['n_binned']}")
+    dt = (time.perf_counter() - t0) / args.steps
+    achieved = FLOPS_PER_PAIR["SS"] * st["n_binned"] / (st["ms_primary"] * 1e-3) / 1e12
+    out = {
+        "metric": "pair evals/sec: split-FP32 kernel, large wg+/w++ box at real shape",
+        "value": st["n_binned"] / dt, "unit": "pair evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"wg+/w++ periodic box L=1000, 2e7 galaxies (40,000 clumps), SD pass, every "
+                               f"{args.large_stride}th outer galaxy x all inner, 25 log rp bins [0.1,200) x 60 pi "
+                               f"bins [-150,150), 64 jackknife regions",
+                   "global_batch": outer.n, "parallelism": "single GPU"},
+        "n_binned": st["n_binned"], "n_candidates": st["n_candidates"], "n_fp64_reeval": st["n_fp64_reeval"],
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_TFLOPS, "traffic": None,
+                     "kernel": "pair_kernel<shape_shape, split, primary>", "ms_per_launch": st["ms_primary"]},
+    }
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,8 +187,10 @@ def main():
                     help="CPU baseline: every n-th outer galaxy of the same pass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--workload", choices=("wgplus", "mocks"), default="wgplus",
-                    help="wgplus: the 1M-galaxy wg+/w++ box (default); mocks: batched wgg mocks")
+    ap.add_argument("--workload", choices=("wgplus", "mocks", "large"), default="wgplus",
+                    help="wgplus: the 1M-galaxy wg+/w++ box (default); mocks: batched wgg mocks; large: the "
+                         "2e7-galaxy box on a sample of outer galaxies")
+    ap.add_argument("--large-stride", type=int, default=8, help="large workload: every n-th outer galaxy")
     ap.add_argument("--mocks", type=int, default=256, help="number of mocks of the mocks workload")
     ap.add_argument("--tune-stride", type=int, default=16, help="mocks workload: autotune on every n-th mock")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -163,6 +213,9 @@ def main():
 
     if args.workload == "mocks":
         bench_mocks(args)
+        return
+    if args.workload == "large":
+        bench_large(args)
         return
 
     import numpy as np

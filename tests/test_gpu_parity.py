@@ -133,6 +133,18 @@ def test_config0_passes(engine, oracle_lib, prec):
 
 
 @pytest.mark.parametrize("prec", PRECS)
+def test_config4_binning(engine, oracle_lib, prec):
+    """The large box's binning (25 log rp bins in [0.1, 200), 60 pi bins in [-150, 150), 4x4x4
+    jackknife: 1500 cells, a shape-shape histogram of 138 KB that takes most of a CU's LDS) on a
+    40k-galaxy sample of that L = 1000 box."""
+    cats = synth.catalogs(4, n_data=40_000, n_rand=10)
+    di = config_datainfo(cats["config"])
+    outer, inner, p, _ = setup_pass("SD", cats["D"], cats["S"], di, precision=prec)
+    st = _run(engine, oracle_lib, outer, inner, p, f"config4 binning prec={prec}")
+    assert st["n_binned"] > 1e7
+
+
+@pytest.mark.parametrize("prec", PRECS)
 def test_config1_subsample(engine, oracle_lib, prec):
     """The 1M-galaxy wg+/w++ box: the full inner catalog against every 500th outer galaxy (SD pass,
     shape_shape) and every 1500th random (SRd pass, shape_density)."""
