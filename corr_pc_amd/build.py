@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcpc.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["cpc_kernels.hip", "cpc_sky.hip", "cpc_api.cpp"]
+SOURCES = ["cpc_kernels.hip", "cpc_sky.hip", "cpc_api.cpp", "cpc_out.cpp"]
 
 COMMON = [
     "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

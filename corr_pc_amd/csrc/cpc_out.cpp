@@ -1,0 +1,322 @@
+// cpc_out.cpp -- the output side of corr_pc's cross correlation, on the histograms cpc_count returns.
+//
+// Restates, for estimator 0 (every configuration here):
+//   final_calc / final_calc_bins / int_calc    bins_calcs.cpp:30-52, 94-256, 331-371
+//   RR_PB_calc (analytic periodic-box RR)      PB_calcs.cpp:63-85
+//   jk_final / jk_final2D                      bins_calcs.cpp:393-462
+//   outp_bins / outp_2Dbins                    outp.cpp:45-74, 291-321
+//   the cross branch of do_bining              do_corr.cpp:326-431 (file names and order)
+// with the reference's operation order and its C++ stream formatting (default precision: six
+// significant digits), so that the same histograms give the same files.  Host code only.
+
+#include <cmath>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/cpc.h"
+
+namespace {
+
+constexpr int kF = 7;  // data[] fields: 0 = W, 1 = e+, 2 = ex, 3 = e++, 4 = exx, 5 = e+x, 6 = theta
+
+struct PBin {
+  double p_min = 0, p_max = 0;
+  long num = 0;
+  double wt_num = 0;
+  double val[kF] = {0}, err[kF] = {0};
+  double ls_err = 0;
+};
+
+// The reference's `bin` (data_def.h:109-121) with its P_bin row.
+struct Bin {
+  double b_min = 0, b_max = 0;
+  long num = 0;
+  double wt_num = 0, sig_crit = 0;
+  std::vector<PBin> p;
+  double val[kF] = {0}, err[kF] = {0};
+  double ls_err_int = 0;
+};
+
+using Bins = std::vector<Bin>;
+
+struct Wt {
+  double val = 0, err = 0;
+};
+
+// Empty bins with the reference's FP32-stepped edges (bin_log / bin_lin / P_bin_ini).
+Bins empty_bins(const cpc_params &p) {
+  std::vector<double> re(p.n_rp + 1), pe(p.n_pi + 1);
+  cpc_bin_edges(&p, re.data(), pe.data());
+  Bins b(p.n_rp);
+  for (int i = 0; i < p.n_rp; ++i) {
+    b[i].b_min = re[i];
+    b[i].b_max = re[i + 1];
+    b[i].p.resize(p.n_pi);
+    for (int k = 0; k < p.n_pi; ++k) {
+      b[i].p[k].p_min = pe[k];
+      b[i].p[k].p_max = pe[k + 1];
+    }
+  }
+  return b;
+}
+
+// Region r of a cpc_hist as the reference holds it after corels2: P_bin num, wt_num, data[] and the
+// per-rp pair count (its other per-rp fields are still zero at that point).
+Bins load_region(const cpc_params &p, const cpc_hist *h, int r) {
+  Bins b = empty_bins(p);
+  for (int i = 0; i < p.n_rp; ++i) {
+    b[i].num = (long)h->num_rp[(size_t)r * p.n_rp + i];
+    for (int k = 0; k < p.n_pi; ++k) {
+      const size_t c = ((size_t)r * p.n_rp + i) * p.n_pi + k;
+      PBin &q = b[i].p[k];
+      q.num = (long)h->num[c];
+      q.wt_num = h->wt_num[c];
+      for (int j = 0; j < kF; ++j) {
+        q.val[j] = h->val[c * kF + j];
+        q.err[j] = h->err[c * kF + j];
+      }
+    }
+  }
+  return b;
+}
+
+// RR_PB_calc (PB_calcs.cpp:63-85), coordinates 6: the expected fraction of pairs per (rp, pi) bin.
+void rr_periodic(Bins &rr, const cpc_params &p) {
+  const double total_volume = std::pow(p.box_size, 3);
+  for (Bin &b : rr) {
+    const double area = M_PI * (b.b_max * b.b_max - b.b_min * b.b_min);
+    b.num = 1;
+    for (PBin &q : b.p) {
+      const double volume = area * (q.p_max - q.p_min);
+      q.wt_num = volume / total_volume;
+      q.num = 1;
+    }
+  }
+}
+
+struct Norm {
+  Wt SD, SR, DR, RR;
+};
+
+// final_calc_bins (bins_calcs.cpp:94-256) for estimator 0 and the kinds of which_corr 0-2.
+void final_calc_bins(Bins &sd, Bins &sr, Bins &dr, Bins &rr, Bins &fin, const Norm &N, const cpc_params &p,
+                     bool periodic) {
+  const double SD_wt = N.SD.val, SR_wt = N.SR.val, DR_wt = N.DR.val, RR_wt = N.RR.val;
+  for (int i = 0; i < p.n_rp; ++i) {
+    for (int k = 0; k < p.n_pi; ++k) {
+      fin[i].num += sd[i].p[k].num;
+      const double sd_num = (double)sd[i].p[k].num;
+      const double sr_num = (double)sr[i].p[k].num;
+      const double rr_num = (double)rr[i].p[k].num;
+      const double sd_wt_num = sd[i].p[k].wt_num;
+      const double sr_wt_num = sr[i].p[k].wt_num;
+      const double dr_wt_num = dr[i].p[k].wt_num;
+      const double rr_wt_num = rr[i].p[k].wt_num;
+      if (rr_num == 0) continue;
+      const double sd_num_rel = (sd_wt_num / rr_wt_num) * (RR_wt / SD_wt);
+      const double sr_num_rel = (sr_wt_num / rr_wt_num) * (RR_wt / SR_wt);
+      const double dr_num_rel = (dr_wt_num / rr_wt_num) * (RR_wt / DR_wt);
+      double w_dp = sd_num_rel - sr_num_rel - dr_num_rel + 1.0;
+      if (periodic) w_dp = sd_num_rel - 1.0;
+      fin[i].p[k].val[0] = w_dp;
+      sd[i].p[k].val[0] = sd_num_rel;
+      sr[i].p[k].val[0] = sr_num_rel;
+      dr[i].p[k].val[0] = dr_num_rel;
+      rr[i].p[k].val[0] = 1.0;
+      if (p.which_corr == 0) continue;
+      for (int j = 1; j < kF; ++j) {
+        const double sd_val = sd[i].p[k].val[j];
+        const double sr_val = sr[i].p[k].val[j];
+        const double sd_val_norm = (sd_val / rr_wt_num) * (RR_wt / SD_wt);
+        const double sr_val_norm = (sr_val / rr_wt_num) * (RR_wt / SR_wt);
+        double val_f = sd_val_norm - sr_val_norm;
+        if (sd_num == 0 || sr_num == 0 || rr_num == 0) {
+          val_f = 0;
+          if (periodic) val_f = sd_val_norm;
+        }
+        fin[i].p[k].val[j] = val_f;
+        sd[i].p[k].val[j] = sd_val;  // the raw sum, not the mean (bins_calcs.cpp:189,235)
+        sr[i].p[k].val[j] = sr_val_norm;
+        if (j == 6) {
+          sd[i].p[k].val[j] = sd_num == 0 ? 0 : sd_val;
+          sr[i].p[k].val[j] = sr_num == 0 ? 0 : sr_val;
+        }
+      }
+    }
+  }
+}
+
+// int_calc (bins_calcs.cpp:331-358): pi integration of one bin array.
+void int_calc(Bins &dat, const cpc_params &p) {
+  const double dC = (p.pi_max - p.pi_min) / p.n_pi;  // data_inf.p_size
+  for (Bin &b : dat) {
+    for (const PBin &q : b.p) {
+      b.wt_num += q.wt_num;
+      for (int j = 0; j < kF; ++j) {
+        b.val[j] += q.val[j] * dC;
+        b.err[j] += std::pow(q.err[j] * dC, 2);
+      }
+      b.ls_err_int += std::pow(q.ls_err * dC, 2);
+    }
+    for (int j = 0; j < kF; ++j) b.err[j] = std::sqrt(b.err[j]);
+    b.ls_err_int = std::sqrt(b.ls_err_int);
+  }
+}
+
+// final_calc with four bin arrays (bins_calcs.cpp:30-52): pair-count normalisations from the
+// catalogs' weight sums, the analytic RR in a periodic box, the estimator, the pi integration.
+void final_calc(Bins &sd, Bins &sr, Bins &dr, Bins &rr, Bins &fin, const cpc_norm &n, const cpc_params &p) {
+  const bool periodic = p.geometry == CPC_GEOM_BOX;
+  Norm N;
+  N.SD.val = n.S.val * n.D.val;
+  N.SD.err = N.SD.val * std::sqrt(n.S.err / std::pow(n.S.val, 2) + n.D.err / std::pow(n.D.val, 2));
+  N.DR.val = n.Rs.val * n.D.val;
+  N.DR.err = N.DR.val * std::sqrt(n.Rs.err / std::pow(n.Rs.val, 2) + n.D.err / std::pow(n.D.val, 2));
+  N.SR.val = n.S.val * n.Rd.val;
+  N.SR.err = N.SR.val * std::sqrt(n.S.err / std::pow(n.S.val, 2) + n.Rd.err / std::pow(n.Rd.val, 2));
+  N.RR.val = n.Rs.val * n.Rd.val * n.include_prob_RR * n.include_prob_RR;
+  N.RR.err = N.RR.val * std::sqrt(n.Rs.err / std::pow(n.Rs.val, 2) + n.Rd.err / std::pow(n.Rd.val, 2));
+  if (periodic) {
+    N.RR.val = 1;
+    N.RR.err = 0;
+    rr_periodic(rr, p);
+  }
+  final_calc_bins(sd, sr, dr, rr, fin, N, p, periodic);
+  // int_calc(sd, sr, dr, rr, final) (bins_calcs.cpp:360-371)
+  int_calc(sd, p);
+  int_calc(sr, p);
+  int_calc(dr, p);
+  int_calc(rr, p);
+  int_calc(fin, p);
+}
+
+// jk_final and jk_final2D (bins_calcs.cpp:393-462): value of the full sample, error
+// sqrt((n-1)/n * sum over regions of (x_r - mean)^2).
+void jk_final(Bins &out, const Bins &fin, const std::vector<Bins> &jk, const cpc_params &p) {
+  const int n = (int)jk.size();
+  for (int i = 0; i < p.n_rp; ++i) {
+    for (int j = 0; j < kF; ++j) {  // data_inf.n_bin_var = 7 (initialization.cpp:141)
+      double var = 0, mean = 0;
+      for (int k = 0; k < n; ++k) mean += jk[k][i].val[j];
+      mean /= n;
+      for (int k = 0; k < n; ++k) var += std::pow(jk[k][i].val[j] - mean, 2.0);
+      var *= ((n - 1.0) / (n));
+      out[i].val[j] = fin[i].val[j];
+      out[i].err[j] = std::sqrt(var);
+    }
+    for (int l = 0; l < p.n_pi; ++l)
+      for (int j = 0; j < kF; ++j) {
+        double var = 0, mean = 0;
+        for (int k = 0; k < n; ++k) mean += jk[k][i].p[l].val[j];
+        mean /= n;
+        for (int k = 0; k < n; ++k) var += std::pow(jk[k][i].p[l].val[j] - mean, 2.0);
+        var *= ((n - 1.0) / (n));
+        out[i].p[l].val[j] = fin[i].p[l].val[j];
+        out[i].p[l].err[j] = std::sqrt(var);
+      }
+  }
+}
+
+// outp_bins (outp.cpp:45-74): one line per rp bin.
+bool write_bins(const Bins &b, const std::string &name) {
+  std::ofstream w(name.c_str());
+  if (!w.is_open()) return false;
+  w << "# rp\tnpairs\twgg\twgg_err\twgp\twgp_err\twgx wgx_err\twpp \twpp_err\t\twxx\twxx_err\t\twpx  wpx_err \ttheta   "
+       "theta_err   LS_err     wt_npair    rp_low   rp_high sig_crit"
+    << std::endl;
+  for (const Bin &x : b) {
+    w << (x.b_max + x.b_min) / 2 << "\t" << x.num;
+    for (int j = 0; j < kF; ++j) w << "\t" << x.val[j] << "\t" << x.err[j];
+    w << "\t" << x.ls_err_int << "   " << x.wt_num << "      " << x.b_min << "    " << x.b_max << "    " << x.sig_crit
+      << std::endl;
+  }
+  return true;
+}
+
+// outp_2Dbins (outp.cpp:291-321): one line per (rp, pi) bin.
+bool write_bins2d(const Bins &b, const std::string &name) {
+  std::ofstream w(name.c_str());
+  if (!w.is_open()) return false;
+  w << "# rp     pi   npairs     wgg      wgg_err  wgp     wgp_err  wgx  wgx_err     wpp  wpp_err    wxx   wxx_err  "
+       "wpx   wpx_err    theta    theta_err  wt_npairs "
+    << std::endl;
+  for (const Bin &x : b)
+    for (const PBin &q : x.p) {
+      w << (x.b_max + x.b_min) / 2 << "        " << (q.p_min + q.p_max) / 2.0 << "         " << q.num;
+      for (int j = 0; j < kF; ++j) w << "       " << q.val[j] << "      " << q.err[j];
+      w << "   " << q.wt_num << std::endl;
+    }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+cpc_status cpc_weight_sum(const double *w, int64_t n, cpc_wsum *out) {
+  if (!out || n < 0) return CPC_ERR_INVALID;
+  double s = 0, s2 = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const double x = w ? w[i] : 1.0;
+    s += x;
+    s2 += x * x;
+  }
+  out->val = s;
+  out->err = s2;
+  return CPC_OK;
+}
+
+cpc_status cpc_write_cross(const cpc_params *pp, const cpc_norm *norm, const cpc_hist *sd, const cpc_hist *sr,
+                           const cpc_hist *dr, const cpc_hist *rr, const char *out_file, int *n_files) {
+  if (!pp || !norm || !sd || !sr || !dr || !rr || !out_file) return CPC_ERR_INVALID;
+  const cpc_params &p = *pp;
+  if (p.estimator != 0) return CPC_ERR_UNSUPPORTED;  // estimator 4 (final_calc_mean_projected) not restated
+  if (p.which_corr < 0 || p.which_corr > 2) return CPC_ERR_UNSUPPORTED;
+  if (p.n_rp <= 0 || p.n_pi <= 0 || !(p.pi_max > p.pi_min)) return CPC_ERR_INVALID;
+  if (p.do_jk == 1 && p.n_jk > 0 && p.jk_mode != CPC_JK_LEAVE_ONE_OUT) return CPC_ERR_INVALID;
+  const std::string of(out_file);
+  int files = 0;
+  auto put = [&](const Bins &b, const std::string &name, bool two_d) {
+    const bool ok = two_d ? write_bins2d(b, of + name) : write_bins(b, of + name);
+    if (ok) ++files;
+    return ok;
+  };
+  // full sample (region 0)
+  Bins SD = load_region(p, sd, 0), SR = load_region(p, sr, 0), DR = load_region(p, dr, 0),
+       RR = load_region(p, rr, 0), F = empty_bins(p);
+  final_calc(SD, SR, DR, RR, F, *norm, p);
+  bool ok = put(F, "bins_cross_final.dat", false) && put(F, "bins2D_cross_final.dat", true) &&
+            put(SD, "bins_cross_SD.dat", false) && put(SD, "bins2D_cross_SD.dat", true) &&
+            put(SR, "bins_cross_SR.dat", false) && put(SR, "bins2D_cross_SR.dat", true) &&
+            put(DR, "bins_cross_DR.dat", false) && put(DR, "bins2D_cross_DR.dat", true) &&
+            put(RR, "bins_cross_RR.dat", false) && put(RR, "bins2D_cross_RR.dat", true);
+  if (ok && p.do_jk == 1 && p.n_jk > 0) {
+    std::vector<Bins> jk;
+    jk.reserve(p.n_jk);
+    for (int r = 0; r < p.n_jk && ok; ++r) {
+      Bins jSD = load_region(p, sd, r + 1), jSR = load_region(p, sr, r + 1), jDR = load_region(p, dr, r + 1),
+           jRR = load_region(p, rr, r + 1), jF = empty_bins(p);
+      final_calc(jSD, jSR, jDR, jRR, jF, *norm, p);
+      std::ostringstream s;
+      s << r;
+      const std::string t = s.str();
+      ok = put(jSD, "bins_cross_jk" + t + "_SD.dat", false) && put(jRR, "bins_cross_jk" + t + "_RR.dat", false) &&
+           put(jDR, "bins_cross_jk" + t + "_DR.dat", false) && put(jSR, "bins_cross_jk" + t + "_SR.dat", false) &&
+           put(jF, "bins_cross_jk" + t + "_final.dat", false) && put(jF, "bins2D_cross_jk" + t + "_final.dat", true) &&
+           put(jSD, "bins2D_cross_jk" + t + "_SD.dat", true) && put(jRR, "bins2D_cross_jk" + t + "_RR.dat", true) &&
+           put(jDR, "bins2D_cross_jk" + t + "_DR.dat", true) && put(jSR, "bins2D_cross_jk" + t + "_SR.dat", true);
+      jk.push_back(std::move(jF));
+    }
+    if (ok) {
+      Bins J = empty_bins(p);
+      jk_final(J, F, jk, p);
+      ok = put(J, "bins_cross_jk_final.dat", false) && put(J, "bins2D_cross_jk_final.dat", true);
+    }
+  }
+  if (n_files) *n_files = files;
+  return ok ? CPC_OK : CPC_ERR_INVALID;
+}
+
+}  // extern "C"

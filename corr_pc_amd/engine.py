@@ -72,6 +72,16 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class WSum(ctypes.Structure):
+    """cpc_wsum: sum of weights and of their squares."""
+    _fields_ = [("val", c_double), ("err", c_double)]
+
+
+class Norm(ctypes.Structure):
+    """cpc_norm: the weight sums of the S, D, Rs, Rd catalogs and include_prob_RR."""
+    _fields_ = [("S", WSum), ("D", WSum), ("Rs", WSum), ("Rd", WSum), ("include_prob_RR", c_double)]
+
+
 class TuningC(ctypes.Structure):
     _fields_ = [("col_frac", c_double), ("item_size", c_int), ("block_threads", c_int)]
 
@@ -217,13 +227,19 @@ def load_library(path=LIB_PATH):
     lib.cpc_bin_edges.restype = c_int
     lib.cpc_box_rr.argtypes = [POINTER(Params), _dp]
     lib.cpc_box_rr.restype = c_int
+    lib.cpc_weight_sum.argtypes = [_dp, c_int64, POINTER(WSum)]
+    lib.cpc_weight_sum.restype = c_int
+    lib.cpc_write_cross.argtypes = [POINTER(Params), POINTER(Norm), POINTER(HistC), POINTER(HistC), POINTER(HistC),
+                                    POINTER(HistC), c_char_p, POINTER(c_int)]
+    lib.cpc_write_cross.restype = c_int
     _lib = lib
     return lib
 
 
 EXPORTED_SYMBOLS = ["cpc_init", "cpc_destroy", "cpc_last_error", "cpc_version", "cpc_catalog_upload",
                     "cpc_catalog_free", "cpc_count", "cpc_count_batch", "cpc_rp_thresholds", "cpc_rand_stream",
-                    "cpc_set_tuning", "cpc_autotune", "cpc_bin_edges", "cpc_box_rr"]
+                    "cpc_set_tuning", "cpc_autotune", "cpc_bin_edges", "cpc_box_rr", "cpc_weight_sum",
+                    "cpc_write_cross"]
 
 
 REFERENCE_SEED = 124693517  # seed(), jk.cpp:40-43
@@ -258,6 +274,30 @@ def box_rr(params):
     if rc != CPC_OK:
         raise EngineError(rc, "cpc_box_rr failed")
     return out
+
+
+def weight_sum(cat):
+    """cpc_weight_sum over a Catalog's weights (unit weights when it has none)."""
+    lib = load_library()
+    out = WSum()
+    w = None if cat.w is None else cat.w.ctypes.data_as(_dp)
+    rc = lib.cpc_weight_sum(w, cat.n, ctypes.byref(out))
+    if rc != CPC_OK:
+        raise EngineError(rc, "cpc_weight_sum failed")
+    return out
+
+
+def write_cross(params, norm, sd, sr, dr, rr, out_prefix):
+    """cpc_write_cross: the cross branch's estimator and every .dat file under `out_prefix`, from the
+    four pair-set Histograms (leave-one-out layout).  Returns the number of files written."""
+    lib = load_library()
+    hs = [h.c_struct() for h in (sd, sr, dr, rr)]
+    n = c_int(0)
+    rc = lib.cpc_write_cross(ctypes.byref(params), ctypes.byref(norm), *[ctypes.byref(h) for h in hs],
+                             out_prefix.encode(), ctypes.byref(n))
+    if rc != CPC_OK:
+        raise EngineError(rc, "cpc_write_cross failed")
+    return n.value
 
 
 def rp_thresholds(params):

@@ -197,6 +197,36 @@ cpc_status cpc_bin_edges(const cpc_params *params, double *rp_edges, double *pi_
  * the reference sets num = 1 in every bin. */
 cpc_status cpc_box_rr(const cpc_params *params, double *wt_num);
 
+/* ---- Output side of the cross correlation (estimator 0) ---- */
+
+/* Sum of a catalog's weights and of their squares, in read order, as read_data accumulates them
+ * into data_info.{S,D,Rs,Rd}_wt (read_dat.cpp:322-323); w == NULL means unit weights. */
+typedef struct cpc_wsum {
+  double val;
+  double err;
+} cpc_wsum;
+cpc_status cpc_weight_sum(const double *w, int64_t n, cpc_wsum *out);
+
+/* The data_info inputs of final_calc besides the binning in cpc_params (bins_calcs.cpp:30-52). */
+typedef struct cpc_norm {
+  cpc_wsum S, D, Rs, Rd;  /* shape, density, shape-random and density-random catalogs */
+  double include_prob_RR; /* data_info.include_prob_RR (1 unless the randoms are subsampled) */
+} cpc_norm;
+
+/* The cross branch of do_bining (do_corr.cpp:326-431) on the four pair-set histograms that
+ * cpc_count returns in CPC_JK_LEAVE_ONE_OUT layout (SD, SRd, DRs, RsRd): final_calc with
+ * final_calc_bins and int_calc (bins_calcs.cpp:30-52, 94-256, 331-371; in the box geometry the
+ * analytic RR of RR_PB_calc, PB_calcs.cpp:63-85, replaces rr as the reference does), then every
+ * file of that branch under the prefix `out_file` (data_info.out_file): bins_cross_{final,SD,SR,
+ * DR,RR}.dat and their bins2D_ versions (outp_bins / outp_2Dbins, outp.cpp:45-74, 291-321), per
+ * jackknife region bins_cross_jk<r>_*.dat and bins2D_cross_jk<r>_*.dat, and the jackknife summary
+ * bins_cross_jk_final.dat / bins2D_cross_jk_final.dat (jk_final / jk_final2D,
+ * bins_calcs.cpp:393-462).  Numbers are written as the reference's streams write them (six
+ * significant digits).  Estimator 0 and which_corr 0-2 only (CPC_ERR_UNSUPPORTED otherwise).
+ * `n_files` (may be NULL) receives the number of files written. */
+cpc_status cpc_write_cross(const cpc_params *params, const cpc_norm *norm, const cpc_hist *sd, const cpc_hist *sr,
+                           const cpc_hist *dr, const cpc_hist *rr, const char *out_file, int *n_files);
+
 /* The reference's per-galaxy random stream: out[k] = rand() / (RAND_MAX + 1.) for the draws numbered
  * skip .. skip+n-1 after srand(seed), with the host C library's rand().  corr_pc seeds 124693517 in
  * seed() (jk.cpp:40-43) and draws jk_prob (read_dat.cpp:243), a random phi (read_dat.cpp:307) and
