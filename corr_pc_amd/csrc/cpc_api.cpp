@@ -645,6 +645,7 @@ struct Plan {
   // derived
   bool sky = false, jk_on = false, autop = false, split = false, use_split = false;
   int R = 1, nreg = 1, ncell = 0, kind_code = 0, NV = 0, NSP = 1, NST = 1, threads = 256;
+  int row0_p = 0, ncopy_p = 1, row0_t = 0, ncopy_t = 1;  // box: LDS rows and copies per pass
   Derived dv{};
   const std::vector<double> *thr = nullptr;
   // device state and the host arrays copied into it (alive until the call's last synchronisation)
@@ -772,6 +773,34 @@ Status setup(cpc_ctx *ctx, Plan &P) {
   if (lds_p > ctx->max_lds)
     return {CPC_ERR_UNSUPPORTED, "histogram of " + std::to_string(ncell) + " cells does not fit in LDS (" +
                                      std::to_string(lds_p) + " bytes)"};
+  if (!sky) {
+    // LDS histogram per pass: passes with 7 or more slots per cell (shape kinds, primary pass),
+    // whose run flushes dominate, get as many copies (at most 4) of the top rp rows as fit in the
+    // group's share of the CU's LDS, keeping in LDS every row from rp >= rp_max / 10 up (rows below
+    // hold about 1% of the pairs of a homogeneous sample and go to the global histogram directly);
+    // other passes keep the whole histogram in one copy
+    const int waves_cu = P.split ? 16 : 8;  // 4 or 2 waves per SIMD
+    const int groups_cu = std::max(1, waves_cu / std::max(1, P.threads / 64));
+    const size_t share = ctx->max_lds / groups_cu;
+    auto choose = [&](int ns, int &row0, int &ncopy) {
+      row0 = 0;
+      ncopy = 1;
+      if (ns < 7) return;  // the kernel keeps the whole histogram in one copy (compiled that way)
+      for (int c = ns >= 7 ? 4 : 1; c >= 1; --c) {
+        int rows = p.n_rp;
+        while (rows > 0 && pair_lds_bytes(rows * p.n_pi, ns, p.n_rp, c) > share) --rows;
+        if (rows == 0) continue;
+        const int r0 = p.n_rp - rows;
+        if (r0 == 0 || std::sqrt(thr[r0]) <= 0.1 * p.rp_max) {
+          row0 = r0;
+          ncopy = c;
+          return;
+        }
+      }
+    };
+    choose(P.NSP, P.row0_p, P.ncopy_p);
+    choose(P.NST, P.row0_t, P.ncopy_t);
+  }
   const int item_size = ctx->item_size ? ctx->item_size : 512;
   const int64_t nbig = std::max(co->n, ci->n);
 
@@ -1033,6 +1062,8 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     P.kp.tz = SI.zs.p;
     P.kp.phase_mode = jk_on ? 1 : 0;
     P.kp.nslots = P.NSP;
+    P.kp.row0 = P.row0_p;
+    P.kp.ncopy = P.ncopy_p;
     P.kp.bgrp = SO.d_groups.p;
     P.kp.tgrp = SI.d_groups.p;
     P.kp.tcol = SI.d_colbeg.p;
@@ -1046,6 +1077,8 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     P.kt.tz = SO.zs.p;
     P.kt.phase_mode = 2;
     P.kt.nslots = P.NST;
+    P.kt.row0 = P.row0_t;
+    P.kt.ncopy = P.ncopy_t;
     P.kt.bgrp = SI.d_groups.p;
     P.kt.tgrp = SO.d_groups.p;
     P.kt.tcol = SO.d_colbeg.p;
@@ -1124,7 +1157,8 @@ Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
           it.job = (int32_t)(2 * i + swap);
           v.push_back({cs[k], it});
         }
-        lds = std::max(lds, pair_lds_bytes(P.ncell, swap ? P.NST : P.NSP, P.p.n_rp));
+        const int row0 = swap ? P.row0_t : P.row0_p, ncopy = swap ? P.ncopy_t : P.ncopy_p;
+        lds = std::max(lds, pair_lds_bytes((P.p.n_rp - row0) * P.p.n_pi, swap ? P.NST : P.NSP, P.p.n_rp, ncopy));
         threads = P.threads;
       }
       if (v.empty()) continue;

@@ -168,6 +168,7 @@ struct HotArgs {
   double L, pmin, pmax, rmax, psize;
   float gz, pioff32, s_out_lo, s_out_hi, s_cert_lo, linv, lrmin, la, lb, gA, gB, gC, gD, inv_psize32;
   int n_pi, n_rp, lin, gws, auto_mode;
+  int lbase, lcell;  // first LDS cell (row0 * n_pi) and LDS cells per copy
 };
 
 // rp bin of a squared separation against the FP64 thresholds thr[0..n_rp]: bin k iff
@@ -236,6 +237,28 @@ __device__ __forceinline__ void lds_add(unsigned *lnum, double *lacc, int ncell,
   atomicAdd(&lnum[cell], n);
 #pragma unroll
   for (int s = 0; s < NS; ++s) atomicAdd(&lacc[s * ncell + cell], v[s]);
+}
+
+// A closed run of one lane: into its LDS histogram copy when the cell is held there, else straight
+// into the pass target's global histogram (gnum, gacc: target base, [NS][ncell] slots).  The LDS
+// side uses explicit LDS pointers, so the two sides stay distinct instructions.
+using lds_u32 = __attribute__((address_space(3))) unsigned;
+using lds_f64 = __attribute__((address_space(3))) double;
+template <int NS>
+__device__ __forceinline__ void flush_run(lds_u32 *mynum, lds_f64 *myacc, int lcell, int lbase,
+                                          unsigned long long *gnum, double *gacc, int ncell, int cell, unsigned n,
+                                          const double *v) {
+  const int lc = cell - lbase;
+  if (lc >= 0) {
+    __hip_atomic_fetch_add(&mynum[lc], n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      __hip_atomic_fetch_add(&myacc[s * lcell + lc], v[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  } else {
+    atomicAdd(&gnum[cell], (unsigned long long)n);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) unsafeAtomicAdd(&gacc[(size_t)s * ncell + cell], v[s]);
+  }
 }
 
 

@@ -77,6 +77,9 @@ struct KArgs {
   double zmargin;      // slack of the tile z-range search
   int nc;              // columns per side
   int n_rp, n_pi, ncell;
+  // LDS histogram: cells from row0 * n_pi on (the rp bins that hold nearly all pairs), in ncopy
+  // copies (lane l accumulates into copy l % ncopy); cells below go to the global histogram directly
+  int row0, ncopy;
   int gws, n_nbr;
   int phase_mode;      // 0 primary pass without jackknife, 1 primary with jackknife, 2 touch pass
   int nslots;          // f64 slots accumulated by this pass
@@ -191,7 +194,7 @@ hipError_t sort_u32(void *&tmp, size_t &tmp_bytes, const uint32_t *kin, uint32_t
                     int32_t *vout, int64_t n, hipStream_t st);
 hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *kout, const int32_t *vin,
                     int32_t *vout, int64_t n, hipStream_t st);
-size_t pair_lds_bytes(int ncell, int nslots, int n_rp);
+size_t pair_lds_bytes(int ncell, int nslots, int n_rp, int ncopy = 1);
 // One launch over `n_items` items whose KArgs live in the device table `jobs` (all with the same
 // kind, precision and pass direction); `lds` covers the largest histogram among them.
 hipError_t launch_pair(int kind, bool split, bool swap, const KArgs *jobs, const Item *items, int n_items,

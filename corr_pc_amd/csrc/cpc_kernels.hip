@@ -61,20 +61,29 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
   // reloading them inside the loop, where each reload would also wait for the LDS atomics in flight
   const HotArgs H{P.L,         P.pmin, P.pmax,  P.rmax, P.psize, P.gz, P.pioff32, P.s_out_lo, P.s_out_hi,
                   P.s_cert_lo, P.linv, P.lrmin, P.la, P.lb, P.gA, P.gB, P.gC, P.gD, P.inv_psize32, P.n_pi, P.n_rp,
-                  P.lin, P.gws, P.auto_mode};
+                  P.lin, P.gws, P.auto_mode, P.row0 * P.n_pi, (P.n_rp - P.row0) * P.n_pi};
   const PRec *__restrict__ brec = P.brec;
   const double *__restrict__ bz = P.bz;
   const PRec *__restrict__ trec = P.trec;
   const double *__restrict__ tz = P.tz;
-  const int ncell = P.ncell;
+  // LDS: ncopy histogram copies of the cells from lbase on, lnum[ncopy][lcell] and
+  // lacc[ncopy][NS][lcell], then the thresholds.  Neighbouring lanes often close runs of the same cell
+  // in the same instruction, and LDS atomics to one address serialise (about 3 cycles per lane,
+  // against about 6 cycles for a whole instruction without collisions); copies spread those lanes.
+  // Row cut and copies are compiled in only where the flush dominates: 7 or more slots per cell.
+  constexpr bool CUT = NS >= 7;
+  const int ncell = P.ncell, ncopy = CUT ? P.ncopy : 1, lcell = CUT ? H.lcell : P.ncell, lbase = CUT ? H.lbase : 0;
   unsigned *lnum = reinterpret_cast<unsigned *>(smem);
-  double *lacc = reinterpret_cast<double *>(smem + ((ncell * 4 + 15) & ~15));
-  double *lthr = lacc + (size_t)NS * ncell;
+  double *lacc = reinterpret_cast<double *>(smem + ((ncopy * lcell * 4 + 15) & ~15));
+  double *lthr = lacc + (size_t)ncopy * NS * lcell;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = uni(tid >> 6);
   const int nwave = blockDim.x >> 6;
+  const int cp = lane % ncopy;
+  lds_u32 *const mynum = (lds_u32 *)(lnum + cp * lcell);
+  lds_f64 *const myacc = (lds_f64 *)(lacc + (size_t)cp * NS * lcell);
 
   for (int k = tid; k <= H.n_rp; k += blockDim.x) lthr[k] = P.thr[k];
 
@@ -90,10 +99,12 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
     // filter on tile region: 0 all, 1 same region as block, 2 other regions
     const int filt = (P.phase_mode == 2) ? 2 : (two_phase ? (ph == 0 ? 1 : 2) : 0);
     const int tgt = (P.phase_mode == 2) ? it.target : (two_phase ? it.target + (ph == 0 ? 1 : 0) : it.target);
-    for (int c = tid; c < ncell; c += blockDim.x) {
+    unsigned long long *const gnum = P.accnum + (size_t)tgt * ncell;
+    double *const gacc = P.acc + (size_t)tgt * NS * ncell;
+    for (int c = tid; c < ncopy * lcell; c += blockDim.x) {
       lnum[c] = 0;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) lacc[s * ncell + c] = 0.0;
+      for (int s = 0; s < NS; ++s) lacc[s * ncopy * lcell + c] = 0.0;
     }
     __syncthreads();
 
@@ -231,7 +242,12 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
                                rl_d(cq.x, cq.y, 4 * t + 2), rl_d(cq.z, cq.w, 4 * t + 2)};
                   if (cell >= 0) {
                     if (cell != run_cell) {
-                      if (run_cell >= 0) lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
+                      if (run_cell >= 0) {
+                        if constexpr (CUT)
+                          flush_run<NS>(mynum, myacc, lcell, lbase, gnum, gacc, ncell, run_cell, run_n, run);
+                        else
+                          lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
+                      }
                       run_cell = cell;
                       run_n = 0;
 #pragma unroll
@@ -255,15 +271,25 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
           }
         }
       }
-      if (run_cell >= 0) lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
+      if (run_cell >= 0) {
+        if constexpr (CUT)
+          flush_run<NS>(mynum, myacc, lcell, lbase, gnum, gacc, ncell, run_cell, run_n, run);
+        else
+          lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
+      }
     }
     __syncthreads();
-    for (int c = tid; c < ncell; c += blockDim.x) {
-      const unsigned n = lnum[c];
+    for (int c = tid; c < lcell; c += blockDim.x) {
+      unsigned n = 0;
+      for (int k = 0; k < ncopy; ++k) n += lnum[k * lcell + c];
       if (n) {
-        atomicAdd(&P.accnum[(size_t)tgt * ncell + c], (unsigned long long)n);
+        atomicAdd(&gnum[lbase + c], (unsigned long long)n);
 #pragma unroll
-        for (int s = 0; s < NS; ++s) unsafeAtomicAdd(&P.acc[((size_t)tgt * NS + s) * ncell + c], lacc[s * ncell + c]);
+        for (int s = 0; s < NS; ++s) {
+          double v = 0.0;
+          for (int k = 0; k < ncopy; ++k) v += lacc[((size_t)k * NS + s) * lcell + c];
+          unsafeAtomicAdd(&gacc[(size_t)s * ncell + lbase + c], v);
+        }
       }
     }
     __syncthreads();
@@ -426,9 +452,9 @@ static hipError_t launch_pair_t(const KArgs *jobs, const Item *items, int n_item
   return hipGetLastError();
 }
 
-// LDS bytes a pair kernel needs for histograms of `ncell` cells and `nslots` f64 slots.
-size_t pair_lds_bytes(int ncell, int nslots, int n_rp) {
-  return (size_t)((ncell * 4 + 15) & ~15) + (size_t)nslots * ncell * 8 + (size_t)(n_rp + 1) * 8;
+// LDS bytes a pair kernel needs for `ncopy` histograms of `ncell` cells and `nslots` f64 slots.
+size_t pair_lds_bytes(int ncell, int nslots, int n_rp, int ncopy) {
+  return (size_t)((ncopy * ncell * 4 + 15) & ~15) + (size_t)ncopy * nslots * ncell * 8 + (size_t)(n_rp + 1) * 8;
 }
 
 hipError_t launch_pair(int kind, bool split, bool swap, const KArgs *jobs, const Item *items, int n_items,
