@@ -776,8 +776,10 @@ Status setup(cpc_ctx *ctx, Plan &P) {
   if (!sky) {
     // LDS histogram per pass: passes with 7 or more slots per cell (shape kinds, primary pass),
     // whose run flushes dominate, get as many copies (at most 4) of the top rp rows as fit in the
-    // group's share of the CU's LDS, keeping in LDS every row from rp >= rp_max / 10 up (rows below
-    // hold about 1% of the pairs of a homogeneous sample and go to the global histogram directly);
+    // group's share of the CU's LDS, keeping in LDS every row from rp >= rp_max / 20 up (rows below
+    // hold about 0.25% of the pairs of a homogeneous sample and go to the global histogram directly:
+    // device-wide atomics cost far more than LDS ones, 3% of the runs there cost as much as the
+    // whole LDS flush);
     // other passes keep the whole histogram in one copy
     const int waves_cu = P.split ? 16 : 8;  // 4 or 2 waves per SIMD
     const int groups_cu = std::max(1, waves_cu / std::max(1, P.threads / 64));
@@ -791,7 +793,7 @@ Status setup(cpc_ctx *ctx, Plan &P) {
         while (rows > 0 && pair_lds_bytes(rows * p.n_pi, ns, p.n_rp, c) > share) --rows;
         if (rows == 0) continue;
         const int r0 = p.n_rp - rows;
-        if (r0 == 0 || std::sqrt(thr[r0]) <= 0.1 * p.rp_max) {
+        if (r0 == 0 || std::sqrt(thr[r0]) <= 0.05 * p.rp_max) {
           row0 = r0;
           ncopy = c;
           return;
