@@ -178,6 +178,58 @@ def bench_large(args):
     eng.close()
 
 
+def bench_sky(args):
+    """The sky survey at its real shape: 2e6 galaxies in six RA/Dec fields (about 150 deg^2),
+    0.2 < z < 1 with n(z) ~ z^2 exp(-(z/0.4)^1.5), flat LCDM distances, 100 jackknife regions, 20 log
+    rp bins in [0.1, 100), 40 pi bins in [-100, 100); the SD pass of which_corr = 1 (shape_density,
+    wg+), 2e6 x 2e6, split precision and plain FP64."""
+    from corr_pc_amd import synth
+    from corr_pc_amd.engine import KIND_SHAPE_DENSITY, PREC_FP64, PREC_SPLIT, Catalog, Engine, make_sky_params
+
+    t = time.perf_counter()
+    table = synth.distance_table(0.3, 1e-4, 1.0)
+    S, n_jk, _ = synth.sky_catalog(2, "data", table=table)
+    D = Catalog(x=S.x, y=S.y, z=S.z, w=S.w, jk=S.jk, jk_prob=S.jk_prob, dc=S.dc, da=S.da)
+    pt = synth.sky_params_table(2, table)
+    params = {prec: make_sky_params(pt, 20, 0.1, 100.0, 40, -100.0, 100.0, kind=KIND_SHAPE_DENSITY, g_with_shape=2,
+                                    which_corr=1, do_jk=1, n_jk=n_jk, precision=prec)
+              for prec in (PREC_SPLIT, PREC_FP64)}
+    log(f"sky catalogs ready in {time.perf_counter() - t:.1f}s: {S.n} galaxies, {n_jk} jackknife regions")
+    eng = Engine(0)
+    eng.upload(D)
+    eng.upload(S)
+    res = {}
+    for prec, label in ((PREC_SPLIT, "split"), (PREC_FP64, "fp64")):
+        for k in range(args.warmup):
+            eng.count(D, S, params[prec])
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            h, st = eng.count(D, S, params[prec])
+            log(f"{label} step {k}: {st['ms_total']:.0f} ms (primary {st['ms_primary']:.0f} ms, touch {st['ms_touch']:.0f} "
+                f"ms, build {st['ms_build']:.0f} ms), binned {st['n_binned']}, host-binned {st['n_host_pairs']}")
+        res[label] = ((time.perf_counter() - t0) / args.steps, st, h)
+    dt, st, h = res["split"]
+    dt64, st64, h64 = res["fp64"]
+    import numpy as np
+    out = {
+        "metric": "pair evals/sec: split-FP32 vs FP64 sky kernel, wg+ 2M-galaxy survey",
+        "value": st["n_binned"] / dt, "unit": "pair evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"sky survey, 2e6 galaxies in 6 RA/Dec fields (~150 deg^2), 0.2<z<1, SD pass "
+                               f"(shape_density) 2e6 x 2e6, 20 log rp bins [0.1,100) x 40 pi bins [-100,100), "
+                               f"{n_jk} jackknife regions",
+                   "global_batch": S.n, "parallelism": "single GPU"},
+        "n_binned": st["n_binned"], "n_candidates": st["n_candidates"], "n_host_pairs": st["n_host_pairs"],
+        "ms_primary": st["ms_primary"], "ms_touch": st["ms_touch"],
+        "fp64": {"value": st64["n_binned"] / dt64, "ms_per_step": dt64 * 1e3, "ms_primary": st64["ms_primary"],
+                 "counts_identical": bool(np.array_equal(h.num, h64.num))},
+        "split_vs_fp64": dt64 / dt,
+    }
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,9 +239,9 @@ def main():
                     help="CPU baseline: every n-th outer galaxy of the same pass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--workload", choices=("wgplus", "mocks", "large"), default="wgplus",
+    ap.add_argument("--workload", choices=("wgplus", "mocks", "large", "sky"), default="wgplus",
                     help="wgplus: the 1M-galaxy wg+/w++ box (default); mocks: batched wgg mocks; large: the "
-                         "2e7-galaxy box on a sample of outer galaxies")
+                         "2e7-galaxy box on a sample of outer galaxies; sky: the 2e6-galaxy survey")
     ap.add_argument("--large-stride", type=int, default=8, help="large workload: every n-th outer galaxy")
     ap.add_argument("--mocks", type=int, default=256, help="number of mocks of the mocks workload")
     ap.add_argument("--tune-stride", type=int, default=16, help="mocks workload: autotune on every n-th mock")
@@ -216,6 +268,9 @@ def main():
         return
     if args.workload == "large":
         bench_large(args)
+        return
+    if args.workload == "sky":
+        bench_sky(args)
         return
 
     import numpy as np

@@ -1104,6 +1104,34 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     A.ndec = G.ncy;
     A.dsum = 4e-15;
     A.auto_mode = autop ? 1 : 0;
+    // Split classification.  The estimate rp^2 = theta^2 dbar^2 (theta^2 = 4 asin^2(c/2) from the
+    // chord c of FP64 unit vectors, a series with relative error below 1e-12) differs from the
+    // reference's acos(sum) dbar, squared, by at most 2 (5e-16 / theta^2) relative (the rounding of
+    // sum near 1, about 4 ulp, propagated through acos) plus 1e-12; theta >= rp_min / DA_max for pairs that can reach
+    // a bin.  Rounding s to FP32 adds 2^-24.  In bin units: la / ln 2 per unit relative error, plus
+    // v_log_f32 (2^-21 max(1, |log2 s|)) times la, the rounding of la and lb, and a margin of 2.
+    A.split_on = 0;
+    if (P.split && !p.lin_bin && p.rp_min > 0) {
+      double da_max = 0;
+      for (const cpc_catalog *c : {co, ci})
+        for (double v : c->h_da) da_max = std::max(da_max, v);
+      const double th_min = p.rp_min / std::max(da_max, 1e-300);
+      const double rel = 2 * 5e-16 / (th_min * th_min) + 1e-12 + std::ldexp(1.0, -24);
+      const double la = 0.5 * std::log10(2.0) / dv.rsize, lb = -std::log10(p.rp_min) / dv.rsize;
+      const double s_lo = thr[0] * thr[0] * (1 - 4 * rel), s_hi = thr[p.n_rp] * thr[p.n_rp] * (1 + 4 * rel);
+      const double l2 = std::max({1.0, std::fabs(std::log2(std::max(s_lo, 1e-300))), std::fabs(std::log2(s_hi))});
+      const double t24 = std::ldexp(1.0, -24), t21 = std::ldexp(1.0, -21);
+      const double g = 2.0 * (la * rel / std::log(2.0) + la * l2 * (t21 + 2 * t24) + std::fabs(lb) * t24 +
+                              (p.n_rp + 2) * t24 * 2) + 1e-12 * (p.n_rp + 1);
+      if (g < 0.1) {
+        A.split_on = 1;
+        A.la = (float)la;
+        A.lb = (float)lb;
+        A.g = round_up_f(g);
+        A.s_out_lo = round_down_f(s_lo);
+        A.s_out_hi = round_up_f(s_hi);
+      }
+    }
     A.thr = B.thr.p;
     A.chunks = B.chunks.p;
     P.sa = A;
@@ -1655,7 +1683,8 @@ cpc_status cpc_catalog_upload(cpc_ctx *ctx, const cpc_catalog_desc *d, cpc_catal
         r.dec_rad = d->y[i] / kR2D;
         r.sd = std::sin(r.dec_rad);
         r.cd = std::cos(r.dec_rad);
-        r.pad0 = r.pad1 = 0;
+        r.ux = r.cd * std::cos(r.ra_rad);
+        r.uy = r.cd * std::sin(r.ra_rad);
       }
       if (!check(hipMalloc((void **)&c->d_sr, n * sizeof(SRec)), st, "alloc"))
         check(hipMemcpy(c->d_sr, sr.data(), n * sizeof(SRec), hipMemcpyHostToDevice), st, "copy");

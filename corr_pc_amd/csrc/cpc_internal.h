@@ -113,7 +113,7 @@ struct alignas(64) SRec {
   double dc, da;          // gal.DC, gal.DA
   double sd, cd;          // sin and cos of gal.dec.val_rad
   double ra_rad, dec_rad; // gal.ra.val_rad, gal.dec.val_rad (= degrees / r2d)
-  double pad0, pad1;
+  double ux, uy;          // cd cos(ra), cd sin(ra): with sd, the unit vector of the direction
 };
 
 // The window lim_calc_sky gives an outer galaxy (sky_calcs.cpp:105-190).
@@ -137,6 +137,12 @@ struct SkyArgs {
   int reach_nb;
   double reach_z0, reach_inv;
   double dsum;             // bound on the device/host difference of the great-circle cosine sum
+  // split classification (log rp bins): rp^2 from the chord of the unit vectors, its FP32 log bin
+  // estimate kf = log2(s32) la + lb, certain when frac(kf) is at least g from 0 and 1; s32 below
+  // s_out_lo or from s_out_hi on is outside every bin; split_on = 0 sends every candidate to the
+  // reference's formula
+  int split_on;
+  float la, lb, g, s_out_lo, s_out_hi;
   int auto_mode;           // corels1 pass: pairs only with the inner galaxy after the outer one
   const double *thr;       // n_rp+1 thresholds on rp (host libm log10)
   const Item *items;

@@ -28,6 +28,21 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+// cos(x) for |x| <= pi/2 by its Taylor series to x^18 (truncation below 4e-15 there); feeds
+// only the orientation angle of a binned pair, whose weighted sums have a 1e-10 tolerance.
+__device__ __forceinline__ double cos_poly(double x) {
+  const double t = x * x;
+  double p = fma(t, -1.0 / 6402373705728000.0, 1.0 / 20922789888000.0);
+  p = fma(t, p, -1.0 / 87178291200.0);
+  p = fma(t, p, 1.0 / 479001600.0);
+  p = fma(t, p, -1.0 / 3628800.0);
+  p = fma(t, p, 1.0 / 40320.0);
+  p = fma(t, p, -1.0 / 720.0);
+  p = fma(t, p, 1.0 / 24.0);
+  p = fma(t, p, -0.5);
+  return fma(t, p, 1.0);
+}
+
 __device__ __forceinline__ int rp_bin_sky(const SkyArgs &P, const double *thr, double rp, double band,
                                           bool *unc) {
   // bin k iff thr[k] <= rp < thr[k+1]; uncertain if rp +- band straddles an edge
@@ -168,7 +183,39 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
             int cell = -1;
             if (ok) {
               const double q2 = floor(((si.dc - so.dc) - P.pmin) / P.psize);  // pi_calc, corel_bin
-              if (q2 >= 0.0 && q2 < (double)P.n_pi) {
+              bool exact = true;
+              if (SPLIT && P.split_on && q2 >= 0.0 && q2 < (double)P.n_pi) {
+                // rp^2 from the chord between the unit vectors: theta^2 = 4 asin^2(c/2), with
+                // asin^2(x) = x^2 (1 + x^2/3 + 8x^4/45 + 4x^6/35 + 128x^8/1575 + 128x^10/2079 + ...),
+                // truncated below 5e-14 relative for c < 0.2 (x^2 < 0.01)
+                const double ex = so.ux - si.ux, ey = so.uy - si.uy, ez = so.sd - si.sd;
+                const double c2 = fma(ex, ex, fma(ey, ey, ez * ez));
+                const double x2 = 0.25 * c2;
+                double poly = fma(x2, 128.0 / 2079.0, 128.0 / 1575.0);
+                poly = fma(x2, poly, 4.0 / 35.0);
+                poly = fma(x2, poly, 8.0 / 45.0);
+                poly = fma(x2, poly, 1.0 / 3.0);
+                poly = fma(x2, poly, 1.0);
+                const double th2 = c2 * poly;
+                const double dbar = (so.da + si.da) * 0.5;
+                const double s64 = th2 * dbar * dbar;
+                // below theta^2 = 2.2e-10 the reference's clamp (sum > 0.9999999999) may apply
+                if (th2 > 2.2e-10 && x2 < 0.01) {
+                  const float s32 = (float)s64;
+                  if (s32 < P.s_out_lo || s32 >= P.s_out_hi) {
+                    exact = false;
+                  } else {
+                    const float kf = __log2f(s32) * P.la + P.lb;
+                    const float fk = floorf(kf);
+                    const float fr = kf - fk;
+                    if (fr >= P.g && fr <= 1.0f - P.g) {
+                      exact = false;
+                      if (fk >= 0.0f && fk < (float)P.n_rp) cell = (int)fk * P.n_pi + (int)q2;
+                    }
+                  }
+                }
+              }
+              if (exact && q2 >= 0.0 && q2 < (double)P.n_pi) {
                 // dA_calc: sin d1 sin d2 + cos d1 cos d2 cos(a1 - a2), clamp, acos
                 const double sum = so.sd * si.sd + so.cd * si.cd * cos((so.ra_rad - si.ra_rad));
                 const double dA = (sum > 0.9999999999) ? acos(1.0) : acos(sum);
@@ -201,7 +248,8 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
               const Gal go{po.x, po.y, po.w, po.e1, po.e2, po.phi};
               const Gal gi{pi.x, pi.y, pi.w, pi.e1, pi.e2, pi.phi};
               // angle_orient_axis_sky: atan((dec_i - dec_o) / ((ra_i - ra_o) cos((dec_o + dec_i)/2)))
-              const double X = (pi.x - po.x) * cos((so.dec_rad + si.dec_rad) / 2.0);
+              const double X = (pi.x - po.x) * (SPLIT ? cos_poly((so.dec_rad + si.dec_rad) / 2.0)
+                                                      : cos((so.dec_rad + si.dec_rad) / 2.0));
               const double Y = pi.y - po.y;
               double qv[NQ];
               pair_values_v<K, SPLIT>(go, gi, P.gws, X, Y, qv);
