@@ -1,16 +1,19 @@
-// cpc_out.cpp -- the output side of corr_pc's cross correlation, on the histograms cpc_count returns.
+// cpc_out.cpp -- the output side of corr_pc (do_bining), on the histograms cpc_count returns.
 //
-// Restates, for estimator 0 (every configuration here):
-//   final_calc / final_calc_bins / int_calc    bins_calcs.cpp:30-52, 94-256, 331-371
+// Restates, for which_corr 0-2 and estimators 0-4:
+//   final_calc / final_calc_bins / int_calc    bins_calcs.cpp:30-52, 94-256, 331-383
+//   final_calc_auto / re_bin                   bins_calcs.cpp:54-91, 465-527
+//   final_calc_mean_projected (estimator 4)    bins_calcs.cpp:258-329
 //   RR_PB_calc (analytic periodic-box RR)      PB_calcs.cpp:63-85
 //   jk_final / jk_final2D                      bins_calcs.cpp:393-462
 //   outp_bins / outp_2Dbins                    outp.cpp:45-74, 291-321
-//   the cross branch of do_bining              do_corr.cpp:326-431 (file names and order)
+//   do_bining, both branches                   do_corr.cpp:203-433 (file names and order)
 // with the reference's operation order and its C++ stream formatting (default precision: six
 // significant digits), so that the same histograms give the same files.  Host code only.
 
 #include <cmath>
 #include <fstream>
+#include <iostream>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -167,9 +170,9 @@ void int_calc(Bins &dat, const cpc_params &p) {
 
 // final_calc with four bin arrays (bins_calcs.cpp:30-52): pair-count normalisations from the
 // catalogs' weight sums, the analytic RR in a periodic box, the estimator, the pi integration.
-void final_calc(Bins &sd, Bins &sr, Bins &dr, Bins &rr, Bins &fin, const cpc_norm &n, const cpc_params &p) {
+// N receives the normalisations, which the reference keeps in data_info.{SD,DRs,SRd,RsRd}_wt.
+void final_calc(Bins &sd, Bins &sr, Bins &dr, Bins &rr, Bins &fin, const cpc_norm &n, const cpc_params &p, Norm &N) {
   const bool periodic = p.geometry == CPC_GEOM_BOX;
-  Norm N;
   N.SD.val = n.S.val * n.D.val;
   N.SD.err = N.SD.val * std::sqrt(n.S.err / std::pow(n.S.val, 2) + n.D.err / std::pow(n.D.val, 2));
   N.DR.val = n.Rs.val * n.D.val;
@@ -219,6 +222,121 @@ void jk_final(Bins &out, const Bins &fin, const std::vector<Bins> &jk, const cpc
   }
 }
 
+// re_bin (bins_calcs.cpp:465-527): folds every negative-pi bin of each rp row onto the positive bin
+// that holds its mirrored centre, the bins' stored (FP32-stepped) edges deciding which one.  A pair
+// of bins whose edges do not mirror to within p_size/100 is reported and left alone, as there.
+void re_bin(Bins &ss, const cpc_params &p) {
+  if (p.n_pi <= 1) return;
+  const double p_size = (p.pi_max - p.pi_min) / p.n_pi;
+  for (int i = 0; i < p.n_rp; ++i) {
+    std::vector<PBin> &q = ss[i].p;
+    for (int k = 0; k < p.n_pi; ++k) {
+      if (q[k].p_min >= 0) break;  // only the negative side moves
+      const double p_val = 0.0 - (q[k].p_min + q[k].p_max) / 2.0;
+      int t = (int)std::floor((p_val - p.pi_min) / p_size);
+      // walk to the bin whose edges hold p_val; the edges are contiguous, so the walk is monotone
+      // and ends within n_pi steps (the bound only guards against a malformed edge table)
+      bool found = false;
+      for (int step = 0; step <= p.n_pi + 1; ++step) {
+        if (t < 0 || t >= p.n_pi) break;
+        if (p_val < q[t].p_min) {
+          --t;
+        } else if (p_val > q[t].p_max) {
+          ++t;
+        } else {
+          found = true;
+          break;
+        }
+      }
+      if (!found) break;  // the reference leaves the row here (bins_calcs.cpp:483-486)
+      if (std::fabs(q[k].p_min + q[t].p_max) > (p_size / 100.) || std::fabs(q[k].p_max + q[t].p_min) > (p_size / 100.) ||
+          q[t].p_max < 0) {
+        std::cout << "rebinning problem:" << i << "   " << t << "   " << k << "   " << p_val << "   " << q[t].p_min
+                  << std::endl;
+        continue;
+      }
+      q[t].num += q[k].num;
+      q[k].num = 0;
+      q[t].wt_num += q[k].wt_num;
+      q[k].wt_num = 0;
+      q[t].ls_err += q[k].ls_err;
+      q[k].ls_err = 0;
+      for (int j = 0; j < kF; ++j) {
+        q[t].val[j] = q[t].val[j] + q[k].val[j];
+        q[k].val[j] = 0;
+        q[t].err[j] = q[t].err[j] + q[k].err[j];
+        q[k].err[j] = 0;
+      }
+    }
+  }
+}
+
+// final_calc_auto (bins_calcs.cpp:54-91): the auto correlation's normalisations (SS pairs counted
+// once, RR halved unless it came from the ordered RsRd-same pass), the analytic RR of a periodic box
+// (folded like the data when re-binning), then the estimator with SR standing in for DR and the pi
+// integration of ss, sr, rr and the result.
+void final_calc_auto(Bins &ss, Bins &sr, Bins &rr, Bins &fin, const cpc_norm &n, const cpc_params &p, bool rsrd_same,
+                     bool rebin, Norm &out) {
+  const bool periodic = p.geometry == CPC_GEOM_BOX;
+  const double SS_wt = n.S.val * n.S.val / 2.0;
+  const double SS_err = SS_wt * std::sqrt(n.S.err / std::pow(n.S.val, 2) + n.S.err / std::pow(n.S.val, 2));
+  const double SR_wt = n.S.val * n.Rs.val;
+  const double SR_err = SR_wt * std::sqrt(n.S.err / std::pow(n.S.val, 2) + n.Rs.err / std::pow(n.Rs.val, 2));
+  double RR_wt = n.Rs.val * n.Rs.val * n.include_prob_RR * n.include_prob_RR;
+  double RR_err = RR_wt * std::sqrt(n.Rs.err / std::pow(n.Rs.val, 2) + n.Rs.err / std::pow(n.Rs.val, 2));
+  if (!rsrd_same) {
+    RR_wt /= 2.0;
+    RR_err /= 2.0;
+  }
+  if (periodic) {
+    RR_wt = 1;
+    RR_err = 0;
+    rr_periodic(rr, p);
+    if (rebin) re_bin(rr, p);
+  }
+  out.SD = {SS_wt, SS_err};
+  out.SR = {SR_wt, SR_err};
+  out.RR = {RR_wt, RR_err};
+  out.DR = {SR_wt, SR_err};
+  final_calc_bins(ss, sr, sr, rr, fin, out, p, periodic);
+  // int_calc(ss, rr, sr, final) (bins_calcs.cpp:373-383): each array integrated once
+  int_calc(ss, p);
+  int_calc(rr, p);
+  int_calc(sr, p);
+  int_calc(fin, p);
+}
+
+// final_calc_mean_projected (bins_calcs.cpp:258-329), estimator 4: per rp bin the pi-integrated
+// mean of each field over the SD pairs minus the same over the SR pairs (the boost factor applies
+// only to the lensing kinds, which_corr >= 7, which this engine does not count).  Only the rp rows
+// of the result are set; its (rp, pi) cells stay zero.  The error column is val * sqrt(0), as there.
+void final_calc_mean_projected(Bins &sd, Bins &sr, Bins &fin, const cpc_params &p) {
+  int_calc(sd, p);
+  int_calc(sr, p);
+  const double final_var = 0;
+  for (int i = 0; i < p.n_rp; ++i) {
+    if (sd[i].num > 0 || sr[i].num > 0) {
+      const double sd_num = (double)sd[i].num;
+      const double sr_num = (double)sr[i].num;
+      if (sd_num > 0) fin[i].sig_crit = sd[i].sig_crit / sd[i].wt_num;
+      for (int j = 0; j < kF; ++j) {
+        fin[i].val[j] = 0;
+        if (sd_num > 0) fin[i].val[j] = sd[i].val[j] / sd[i].wt_num;
+        fin[i].err[j] = fin[i].val[j] * std::sqrt(final_var);
+        if (sr_num > 0) fin[i].val[j] -= sr[i].val[j] / sr[i].wt_num;
+      }
+    } else {
+      for (int j = 0; j < kF; ++j) {
+        fin[i].val[j] = 0;
+        fin[i].err[j] = 0;
+      }
+    }
+    fin[i].num = sd[i].num;
+    fin[i].wt_num = sd[i].wt_num;
+    if (fin[i].val[0] != fin[i].val[0]) std::cout << "NaN Problem in bin " << i << "  " << std::endl;
+  }
+}
+
 // outp_bins (outp.cpp:45-74): one line per rp bin.
 bool write_bins(const Bins &b, const std::string &name) {
   std::ofstream w(name.c_str());
@@ -251,6 +369,145 @@ bool write_bins2d(const Bins &b, const std::string &name) {
   return true;
 }
 
+// Files of one do_bining call under the prefix `of`, counted; the first failure to open a file
+// stops the writing (the reference prints "<name> not open" and goes on, outp.cpp:50-54).
+struct Writer {
+  std::string of;
+  int files = 0;
+  bool ok = true;
+  void put(const Bins &b, const std::string &name, bool two_d) {
+    if (!ok) return;
+    ok = two_d ? write_bins2d(b, of + name) : write_bins(b, of + name);
+    if (ok) ++files;
+  }
+};
+
+// A pair set of do_bining: region r of the caller's histogram, or zero bins for a set that was not
+// counted (bins_ini allocates every set, initialization.cpp:472-488).
+Bins region_or_empty(const cpc_params &p, const cpc_hist *h, int r) {
+  return h ? load_region(p, h, r) : empty_bins(p);
+}
+
+std::string jk_tag(int r) {
+  std::ostringstream s;
+  s << r;  // do_jk == 1 (do_corr.cpp:278-284)
+  return s.str();
+}
+
+Wt to_wt(const cpc_wsum &w) { return {w.val, w.err}; }
+cpc_wsum to_wsum(const Wt &w) { return {w.val, w.err}; }
+
+// The auto branch of do_bining (do_corr.cpp:220-324).
+void auto_branch(Writer &W, const cpc_params &p, const cpc_norm &n, const cpc_bining &b, Norm &last) {
+  const bool jk_on = p.do_jk == 1 && p.n_jk > 0;
+  Bins SS = region_or_empty(p, b.SS, 0), SR = region_or_empty(p, b.SRs, 0), RR = region_or_empty(p, b.RsRs, 0),
+       F = empty_bins(p);
+  if (b.do_auto_rebin) {
+    re_bin(SS, p);
+    re_bin(SR, p);
+    re_bin(RR, p);
+    W.put(SR, "bins_auto_SR2.temp", true);
+  }
+  final_calc_auto(SS, SR, RR, F, n, p, b.rsrd_same != 0, b.do_auto_rebin != 0, last);
+  W.put(F, "bins_auto_final.dat", false);
+  W.put(SS, "bins_auto_SS.dat", false);
+  W.put(SR, "bins_auto_SR.dat", false);
+  W.put(RR, "bins_auto_RR.dat", false);
+  W.put(F, "bins2D_auto_final.dat", true);
+  W.put(SS, "bins2D_auto_SS.dat", true);
+  W.put(SR, "bins2D_auto_SR.dat", true);
+  W.put(RR, "bins2D_auto_RR.dat", true);
+  if (!jk_on) return;
+  std::vector<Bins> jk;
+  jk.reserve(p.n_jk);
+  for (int r = 0; r < p.n_jk && W.ok; ++r) {
+    Bins jSS = region_or_empty(p, b.SS, r + 1), jSR = region_or_empty(p, b.SRs, r + 1),
+         jRR = region_or_empty(p, b.RsRs, r + 1), jF = empty_bins(p);
+    if (b.do_auto_rebin) {
+      re_bin(jSS, p);
+      re_bin(jRR, p);
+      re_bin(jSR, p);
+    }
+    final_calc_auto(jSS, jSR, jRR, jF, n, p, b.rsrd_same != 0, b.do_auto_rebin != 0, last);
+    const std::string t = jk_tag(r);
+    W.put(jSS, "bins_auto_jk" + t + "_SS.dat", false);
+    W.put(jRR, "bins_auto_jk" + t + "_RR.dat", false);
+    W.put(jSR, "bins_auto_jk" + t + "_SR.dat", false);
+    W.put(jF, "bins_auto_jk" + t + "_final.dat", false);
+    W.put(jF, "bins2D_auto_jk" + t + "_final.dat", true);
+    W.put(jSS, "bins2D_auto_jk" + t + "_SS.dat", true);
+    W.put(jRR, "bins2D_auto_jk" + t + "_RR.dat", true);
+    W.put(jSR, "bins2D_auto_jk" + t + "_SR.dat", true);
+    jk.push_back(std::move(jF));
+  }
+  if (!W.ok) return;
+  Bins J = empty_bins(p);
+  jk_final(J, F, jk, p);
+  W.put(J, "bins_auto_jk_final.dat", false);
+  W.put(J, "bins2D_auto_jk_final.dat", true);
+}
+
+// One region of the cross estimator: final_calc_mean_projected for estimator 4, else final_calc.
+void cross_calc(Bins &sd, Bins &sr, Bins &dr, Bins &rr, Bins &fin, const cpc_norm &n, const cpc_params &p,
+                Norm &last) {
+  if (p.estimator == 4)
+    final_calc_mean_projected(sd, sr, fin, p);
+  else
+    final_calc(sd, sr, dr, rr, fin, n, p, last);
+}
+
+// The cross branch of do_bining (do_corr.cpp:326-431).
+void cross_branch(Writer &W, const cpc_params &p, const cpc_norm &n, const cpc_bining &b, Norm &last) {
+  const bool jk_on = p.do_jk == 1 && p.n_jk > 0;
+  Bins SD = region_or_empty(p, b.SD, 0), SR = region_or_empty(p, b.SRd, 0), DR = region_or_empty(p, b.DRs, 0),
+       RR = region_or_empty(p, b.RsRd, 0), F = empty_bins(p);
+  cross_calc(SD, SR, DR, RR, F, n, p, last);
+  W.put(F, "bins_cross_final.dat", false);
+  W.put(F, "bins2D_cross_final.dat", true);
+  W.put(SD, "bins_cross_SD.dat", false);
+  W.put(SD, "bins2D_cross_SD.dat", true);
+  W.put(SR, "bins_cross_SR.dat", false);
+  W.put(SR, "bins2D_cross_SR.dat", true);
+  W.put(DR, "bins_cross_DR.dat", false);
+  W.put(DR, "bins2D_cross_DR.dat", true);
+  W.put(RR, "bins_cross_RR.dat", false);
+  W.put(RR, "bins2D_cross_RR.dat", true);
+  if (!jk_on) return;
+  std::vector<Bins> jk;
+  jk.reserve(p.n_jk);
+  for (int r = 0; r < p.n_jk && W.ok; ++r) {
+    Bins jSD = region_or_empty(p, b.SD, r + 1), jSR = region_or_empty(p, b.SRd, r + 1),
+         jDR = region_or_empty(p, b.DRs, r + 1), jRR = region_or_empty(p, b.RsRd, r + 1), jF = empty_bins(p);
+    cross_calc(jSD, jSR, jDR, jRR, jF, n, p, last);
+    const std::string t = jk_tag(r);
+    W.put(jSD, "bins_cross_jk" + t + "_SD.dat", false);
+    W.put(jRR, "bins_cross_jk" + t + "_RR.dat", false);
+    W.put(jDR, "bins_cross_jk" + t + "_DR.dat", false);
+    W.put(jSR, "bins_cross_jk" + t + "_SR.dat", false);
+    W.put(jF, "bins_cross_jk" + t + "_final.dat", false);
+    W.put(jF, "bins2D_cross_jk" + t + "_final.dat", true);
+    W.put(jSD, "bins2D_cross_jk" + t + "_SD.dat", true);
+    W.put(jRR, "bins2D_cross_jk" + t + "_RR.dat", true);
+    W.put(jDR, "bins2D_cross_jk" + t + "_DR.dat", true);
+    W.put(jSR, "bins2D_cross_jk" + t + "_SR.dat", true);
+    jk.push_back(std::move(jF));
+  }
+  if (!W.ok) return;
+  Bins J = empty_bins(p);
+  jk_final(J, F, jk, p);
+  W.put(J, "bins_cross_jk_final.dat", false);
+  W.put(J, "bins2D_cross_jk_final.dat", true);
+}
+
+const char *check_output_params(const cpc_params &p) {
+  if (p.estimator < 0 || p.estimator > 4) return "estimator must be 0-4";
+  if (p.which_corr < 0 || p.which_corr > 2) return "which_corr must be 0-2";
+  if (p.n_rp <= 0 || p.n_pi <= 0 || !(p.pi_max > p.pi_min)) return "bad binning";
+  if (p.do_jk != 0 && p.do_jk != 1) return "do_jk must be 0 or 1";
+  if (p.do_jk == 1 && p.n_jk > 0 && p.jk_mode != CPC_JK_LEAVE_ONE_OUT) return "histograms must be leave-one-out";
+  return nullptr;
+}
+
 }  // namespace
 
 extern "C" {
@@ -268,55 +525,46 @@ cpc_status cpc_weight_sum(const double *w, int64_t n, cpc_wsum *out) {
   return CPC_OK;
 }
 
+cpc_status cpc_do_bining(const cpc_params *pp, const cpc_norm *norm, const cpc_bining *sets, const char *out_file,
+                         cpc_pair_norm *pair_norm, int *n_files) {
+  if (n_files) *n_files = 0;
+  if (!pp || !norm || !sets || !out_file) return CPC_ERR_INVALID;
+  const cpc_params &p = *pp;
+  if (p.estimator < 0 || p.estimator > 4 || p.which_corr < 0 || p.which_corr > 2) return CPC_ERR_UNSUPPORTED;
+  if (check_output_params(p)) return CPC_ERR_INVALID;
+  const cpc_bining &b = *sets;
+  Writer W;
+  W.of = out_file;
+  Norm last;  // data_info.{SD,SRd,DRs,RsRd}_wt start at zero (initialization.cpp:322-338)
+  W.put(region_or_empty(p, b.SRs, 0), "bins_auto_SR.temp", true);
+  W.put(region_or_empty(p, b.SRd, 0), "bins_cross_SR.temp", true);
+  if (b.do_auto_corr) auto_branch(W, p, *norm, b, last);
+  if (b.do_cross_corr) cross_branch(W, p, *norm, b, last);
+  if (pair_norm) *pair_norm = {to_wsum(last.SD), to_wsum(last.SR), to_wsum(last.DR), to_wsum(last.RR)};
+  if (n_files) *n_files = W.files;
+  return W.ok ? CPC_OK : CPC_ERR_INVALID;
+}
+
 cpc_status cpc_write_cross(const cpc_params *pp, const cpc_norm *norm, const cpc_hist *sd, const cpc_hist *sr,
                            const cpc_hist *dr, const cpc_hist *rr, const char *out_file, int *n_files) {
+  if (n_files) *n_files = 0;
   if (!pp || !norm || !sd || !sr || !dr || !rr || !out_file) return CPC_ERR_INVALID;
   const cpc_params &p = *pp;
-  if (p.estimator != 0) return CPC_ERR_UNSUPPORTED;  // estimator 4 (final_calc_mean_projected) not restated
+  if (p.estimator != 0 && p.estimator != 4) return CPC_ERR_UNSUPPORTED;
   if (p.which_corr < 0 || p.which_corr > 2) return CPC_ERR_UNSUPPORTED;
-  if (p.n_rp <= 0 || p.n_pi <= 0 || !(p.pi_max > p.pi_min)) return CPC_ERR_INVALID;
-  if (p.do_jk == 1 && p.n_jk > 0 && p.jk_mode != CPC_JK_LEAVE_ONE_OUT) return CPC_ERR_INVALID;
-  const std::string of(out_file);
-  int files = 0;
-  auto put = [&](const Bins &b, const std::string &name, bool two_d) {
-    const bool ok = two_d ? write_bins2d(b, of + name) : write_bins(b, of + name);
-    if (ok) ++files;
-    return ok;
-  };
-  // full sample (region 0)
-  Bins SD = load_region(p, sd, 0), SR = load_region(p, sr, 0), DR = load_region(p, dr, 0),
-       RR = load_region(p, rr, 0), F = empty_bins(p);
-  final_calc(SD, SR, DR, RR, F, *norm, p);
-  bool ok = put(F, "bins_cross_final.dat", false) && put(F, "bins2D_cross_final.dat", true) &&
-            put(SD, "bins_cross_SD.dat", false) && put(SD, "bins2D_cross_SD.dat", true) &&
-            put(SR, "bins_cross_SR.dat", false) && put(SR, "bins2D_cross_SR.dat", true) &&
-            put(DR, "bins_cross_DR.dat", false) && put(DR, "bins2D_cross_DR.dat", true) &&
-            put(RR, "bins_cross_RR.dat", false) && put(RR, "bins2D_cross_RR.dat", true);
-  if (ok && p.do_jk == 1 && p.n_jk > 0) {
-    std::vector<Bins> jk;
-    jk.reserve(p.n_jk);
-    for (int r = 0; r < p.n_jk && ok; ++r) {
-      Bins jSD = load_region(p, sd, r + 1), jSR = load_region(p, sr, r + 1), jDR = load_region(p, dr, r + 1),
-           jRR = load_region(p, rr, r + 1), jF = empty_bins(p);
-      final_calc(jSD, jSR, jDR, jRR, jF, *norm, p);
-      std::ostringstream s;
-      s << r;
-      const std::string t = s.str();
-      ok = put(jSD, "bins_cross_jk" + t + "_SD.dat", false) && put(jRR, "bins_cross_jk" + t + "_RR.dat", false) &&
-           put(jDR, "bins_cross_jk" + t + "_DR.dat", false) && put(jSR, "bins_cross_jk" + t + "_SR.dat", false) &&
-           put(jF, "bins_cross_jk" + t + "_final.dat", false) && put(jF, "bins2D_cross_jk" + t + "_final.dat", true) &&
-           put(jSD, "bins2D_cross_jk" + t + "_SD.dat", true) && put(jRR, "bins2D_cross_jk" + t + "_RR.dat", true) &&
-           put(jDR, "bins2D_cross_jk" + t + "_DR.dat", true) && put(jSR, "bins2D_cross_jk" + t + "_SR.dat", true);
-      jk.push_back(std::move(jF));
-    }
-    if (ok) {
-      Bins J = empty_bins(p);
-      jk_final(J, F, jk, p);
-      ok = put(J, "bins_cross_jk_final.dat", false) && put(J, "bins2D_cross_jk_final.dat", true);
-    }
-  }
-  if (n_files) *n_files = files;
-  return ok ? CPC_OK : CPC_ERR_INVALID;
+  if (check_output_params(p)) return CPC_ERR_INVALID;
+  cpc_bining b{};
+  b.do_cross_corr = 1;
+  b.SD = sd;
+  b.SRd = sr;
+  b.DRs = dr;
+  b.RsRd = rr;
+  Writer W;
+  W.of = out_file;
+  Norm last;
+  cross_branch(W, p, *norm, b, last);
+  if (n_files) *n_files = W.files;
+  return W.ok ? CPC_OK : CPC_ERR_INVALID;
 }
 
 }  // extern "C"

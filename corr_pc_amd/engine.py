@@ -82,6 +82,18 @@ class Norm(ctypes.Structure):
     _fields_ = [("S", WSum), ("D", WSum), ("Rs", WSum), ("Rd", WSum), ("include_prob_RR", c_double)]
 
 
+class Bining(ctypes.Structure):
+    """cpc_bining: do_bining's mode flags and its seven pair-set histograms (NULL = not counted)."""
+    _fields_ = [("do_auto_corr", c_int), ("do_cross_corr", c_int), ("do_auto_rebin", c_int), ("rsrd_same", c_int),
+                ("SS", POINTER(HistC)), ("SRs", POINTER(HistC)), ("RsRs", POINTER(HistC)), ("SD", POINTER(HistC)),
+                ("SRd", POINTER(HistC)), ("DRs", POINTER(HistC)), ("RsRd", POINTER(HistC))]
+
+
+class PairNorm(ctypes.Structure):
+    """cpc_pair_norm: data_info.{SD,SRd,DRs,RsRd}_wt after do_bining."""
+    _fields_ = [("SD", WSum), ("SRd", WSum), ("DRs", WSum), ("RsRd", WSum)]
+
+
 class TuningC(ctypes.Structure):
     _fields_ = [("col_frac", c_double), ("item_size", c_int), ("block_threads", c_int)]
 
@@ -232,6 +244,9 @@ def load_library(path=LIB_PATH):
     lib.cpc_write_cross.argtypes = [POINTER(Params), POINTER(Norm), POINTER(HistC), POINTER(HistC), POINTER(HistC),
                                     POINTER(HistC), c_char_p, POINTER(c_int)]
     lib.cpc_write_cross.restype = c_int
+    lib.cpc_do_bining.argtypes = [POINTER(Params), POINTER(Norm), POINTER(Bining), c_char_p, POINTER(PairNorm),
+                                  POINTER(c_int)]
+    lib.cpc_do_bining.restype = c_int
     _lib = lib
     return lib
 
@@ -239,7 +254,7 @@ def load_library(path=LIB_PATH):
 EXPORTED_SYMBOLS = ["cpc_init", "cpc_destroy", "cpc_last_error", "cpc_version", "cpc_catalog_upload",
                     "cpc_catalog_free", "cpc_count", "cpc_count_batch", "cpc_rp_thresholds", "cpc_rand_stream",
                     "cpc_set_tuning", "cpc_autotune", "cpc_bin_edges", "cpc_box_rr", "cpc_weight_sum",
-                    "cpc_write_cross"]
+                    "cpc_write_cross", "cpc_do_bining"]
 
 
 REFERENCE_SEED = 124693517  # seed(), jk.cpp:40-43
@@ -298,6 +313,34 @@ def write_cross(params, norm, sd, sr, dr, rr, out_prefix):
     if rc != CPC_OK:
         raise EngineError(rc, "cpc_write_cross failed")
     return n.value
+
+
+AUTO_SETS = ("SS", "SRs", "RsRs")
+CROSS_SETS = ("SD", "SRd", "DRs", "RsRd")
+
+
+def do_bining(params, norm, hists, out_prefix, do_auto_corr=0, do_cross_corr=1, do_auto_rebin=0, rsrd_same=0):
+    """cpc_do_bining: do_bining's estimators and every output file under `out_prefix`.
+
+    `hists` maps pair-set names (AUTO_SETS, CROSS_SETS) to Histograms in leave-one-out layout; a
+    missing set is one that was not counted.  Returns (number of files written, PairNorm)."""
+    lib = load_library()
+    unknown = set(hists) - set(AUTO_SETS + CROSS_SETS)
+    if unknown:
+        raise ValueError(f"unknown pair sets {sorted(unknown)}")
+    b = Bining(do_auto_corr, do_cross_corr, do_auto_rebin, rsrd_same)
+    keep = {}
+    for name, h in hists.items():
+        if h is not None:
+            keep[name] = h.c_struct()
+            setattr(b, name, ctypes.pointer(keep[name]))
+    pn = PairNorm()
+    n = c_int(0)
+    rc = lib.cpc_do_bining(ctypes.byref(params), ctypes.byref(norm), ctypes.byref(b), out_prefix.encode(),
+                           ctypes.byref(pn), ctypes.byref(n))
+    if rc != CPC_OK:
+        raise EngineError(rc, "cpc_do_bining failed")
+    return n.value, pn
 
 
 def rp_thresholds(params):

@@ -222,10 +222,39 @@ typedef struct cpc_norm {
  * jackknife region bins_cross_jk<r>_*.dat and bins2D_cross_jk<r>_*.dat, and the jackknife summary
  * bins_cross_jk_final.dat / bins2D_cross_jk_final.dat (jk_final / jk_final2D,
  * bins_calcs.cpp:393-462).  Numbers are written as the reference's streams write them (six
- * significant digits).  Estimator 0 and which_corr 0-2 only (CPC_ERR_UNSUPPORTED otherwise).
- * `n_files` (may be NULL) receives the number of files written. */
+ * significant digits).  Estimator 0, or 4 (final_calc_mean_projected, bins_calcs.cpp:258-329), and
+ * which_corr 0-2 only (CPC_ERR_UNSUPPORTED otherwise).  `n_files` (may be NULL) receives the number
+ * of files written. */
 cpc_status cpc_write_cross(const cpc_params *params, const cpc_norm *norm, const cpc_hist *sd, const cpc_hist *sr,
                            const cpc_hist *dr, const cpc_hist *rr, const char *out_file, int *n_files);
+
+/* The pair-set histograms and mode flags that do_bining reads from `bins B` and data_info
+ * (do_corr.cpp:203-433).  Each histogram is as cpc_count returns it (CPC_JK_LEAVE_ONE_OUT), after the
+ * caller's RsRd_same merge (do_corr.cpp:135-154: RsRs += RsRd and SRd += SRs, elementwise over every
+ * array); NULL stands for a set that was not counted, which the reference holds as zero bins. */
+typedef struct cpc_bining {
+  int do_auto_corr;  /* data_info.do_auto_corr: estimators 1, 2, 3 (initialization.cpp:163-185) */
+  int do_cross_corr; /* data_info.do_cross_corr: estimators 0, 3, 4 */
+  int do_auto_rebin; /* data_info.do_auto_rebin: estimators 2, 3 */
+  int rsrd_same;     /* data_info.RsRd_same: no density randoms (initialization.cpp:187-192) */
+  const cpc_hist *SS, *SRs, *RsRs; /* auto pair sets */
+  const cpc_hist *SD, *SRd, *DRs, *RsRd; /* cross pair sets */
+} cpc_bining;
+
+/* The pair normalisations data_info holds after do_bining: {SD,SRd,DRs,RsRd}_wt as the last
+ * final_calc / final_calc_auto call left them (zero when none ran), which outp_data_info reports. */
+typedef struct cpc_pair_norm {
+  cpc_wsum SD, SRd, DRs, RsRd;
+} cpc_pair_norm;
+
+/* do_bining (do_corr.cpp:203-433): the raw bins_auto_SR.temp / bins_cross_SR.temp (SRs, SRd), then
+ * the auto branch (re_bin, bins_calcs.cpp:465-527, when do_auto_rebin; final_calc_auto,
+ * bins_calcs.cpp:54-91; bins_auto_*.dat, bins2D_auto_*.dat, per-region and jackknife-summary files)
+ * and the cross branch (as cpc_write_cross), under the prefix `out_file`, with the reference's file
+ * names and formatting.  which_corr 0-2, estimators 0-4, do_jk 0 or 1.  `pair_norm` and `n_files`
+ * may be NULL. */
+cpc_status cpc_do_bining(const cpc_params *params, const cpc_norm *norm, const cpc_bining *sets,
+                         const char *out_file, cpc_pair_norm *pair_norm, int *n_files);
 
 /* The reference's per-galaxy random stream: out[k] = rand() / (RAND_MAX + 1.) for the draws numbered
  * skip .. skip+n-1 after srand(seed), with the host C library's rand().  corr_pc seeds 124693517 in

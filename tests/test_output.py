@@ -214,7 +214,8 @@ def test_formatting_six_significant_digits(tmp_path):
 
 
 def test_unsupported_estimator(tmp_path):
-    p = make_params(40.0, 2, 1.0, 8.0, 2, -4.0, 4.0, kind=KIND_DD, estimator=4)
+    """cpc_write_cross is the cross branch only: the auto estimators 1-3 go through cpc_do_bining."""
+    p = make_params(40.0, 2, 1.0, 8.0, 2, -4.0, 4.0, kind=KIND_DD, estimator=1)
     h = _hists(p, 1.0, 1.0, 1.0, 1.0)
     with pytest.raises(EngineError):
         write_cross(p, _norm(), *h, str(tmp_path) + "/u_")
