@@ -1,4 +1,5 @@
-"""Build recipe of the engine's shared library (corr_pc_amd/libcpc.so) for gfx950.
+"""Build recipe of the engine's shared library (corr_pc_amd/libcpc.so) for gfx950 and of the corr_pc
+program over it (corr_pc_amd/corr_pc).
 
 Run as ``python -m corr_pc_amd.build`` or through ``__graft_entry__.build()``.  Compiles with hipcc:
 contraction off (the FP64 candidate test must round like the reference's x86-64 build) and packed
@@ -14,6 +15,10 @@ LIB = os.path.join(HERE, "libcpc.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["cpc_kernels.hip", "cpc_sky.hip", "cpc_api.cpp", "cpc_out.cpp"]
+# the corr_pc program: host code over the library's C ABI, built with the host compiler
+PROGRAM = os.path.join(HERE, "corr_pc")
+PROGRAM_SOURCE = "cpc_main.cpp"
+CXX = os.environ.get("CXX", "g++")
 
 COMMON = [
     "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
@@ -53,7 +58,20 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_program(verbose)
     return LIB
+
+
+def build_program(verbose=False):
+    """Compile and link corr_pc_amd/corr_pc against libcpc.so (found beside it at run time)."""
+    tmp = PROGRAM + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-o", tmp, PROGRAM_SOURCE, "-L" + HERE, "-lcpc",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(tmp, PROGRAM)
+    return PROGRAM
 
 
 if __name__ == "__main__":

@@ -1,5 +1,6 @@
-"""Host mirror of corels2's pass set-up (corr.cpp:219-296) and choose_corel_func
-(initialization.cpp:490-543): which catalog is outer, which carries the shape, which pair kind runs.
+"""Host mirror of the pass set-up of corels2 (corr.cpp:219-296) and corels1 (corr.cpp:103-111) and of
+choose_corel_func (initialization.cpp:490-543): which catalog is outer, which carries the shape, which
+pair kind runs.
 
 corels2(g1, g2, bjk, data_inf) takes its catalog sizes from the do_* role flags, makes the larger
 catalog the outer one (ties keep g1), and flips g_with_shape when it swaps.  `setup_pass` returns
@@ -9,7 +10,7 @@ from dataclasses import dataclass
 
 from .engine import (JK_LEAVE_ONE_OUT, KIND_DD, KIND_SHAPE_DENSITY, KIND_SHAPE_SHAPE, PREC_SPLIT, make_params)
 
-PASSES = ("SD", "SRd", "RsRd", "SRs", "DRs", "RsRs")
+PASSES = ("SD", "SRd", "RsRd", "SRs", "DRs", "RsRs", "SS")
 
 
 @dataclass
@@ -31,26 +32,42 @@ class DataInfo:
 
 
 def pass_kind(which_corr, pass_name):
-    """choose_corel_func, initialization.cpp:490-543 (coordinates 6)."""
+    """choose_corel_func, initialization.cpp:490-543 (coordinates 0 and 6); pass_name is the do_* flag."""
     if pass_name in ("RsRs", "RsRd"):
         return KIND_DD
     if which_corr == 0:
         return KIND_DD
     if which_corr == 1:
-        return KIND_SHAPE_DENSITY
+        return KIND_SHAPE_SHAPE if pass_name == "SS" else KIND_SHAPE_DENSITY
     if which_corr == 2:
-        return KIND_SHAPE_SHAPE if pass_name == "SD" else KIND_SHAPE_DENSITY
+        return KIND_SHAPE_SHAPE if pass_name in ("SD", "SS") else KIND_SHAPE_DENSITY
     raise ValueError(f"which_corr={which_corr} is outside the pair kinds this engine runs")
 
 
-def setup_pass(pass_name, g1, g2, di: DataInfo, precision=PREC_SPLIT, jk_mode=JK_LEAVE_ONE_OUT):
-    """(outer, inner, params, order_changed) of corels2(g1, g2, ...) for pass `pass_name`.
+def setup_pass(pass_name, g1, g2, di: DataInfo, precision=PREC_SPLIT, jk_mode=JK_LEAVE_ONE_OUT, auto=False):
+    """(outer, inner, params, order_changed) of corels2(g1, g2, ...) for pass `pass_name` (the do_*
+    flag set around the call), or of corels1(g1, ...) when `auto`; None when a catalog is empty.
 
     g1, g2 are the arguments in the order do_corr passes them (e.g. SD: corels2(gdensity, gshape),
     do_corr.cpp:125; SRd: corels2(grand_density, gshape), :110; RsRd: corels2(grand_density,
     grand_shape), :115)."""
     if pass_name not in PASSES:
         raise ValueError(pass_name)
+    kind = pass_kind(di.which_corr, pass_name)
+    inc_max = di.include_prob_RR if pass_name in ("RsRs", "RsRd") else 1.0  # initialization.cpp:597-599
+
+    def params(kind, gws):
+        return make_params(di.box_size, di.n_rp, di.rp_min, di.rp_max, di.n_pi, di.pi_min, di.pi_max, kind=kind,
+                           g_with_shape=gws, estimator=di.estimator, which_corr=di.which_corr,
+                           lin_bin=di.lin_bin, do_jk=di.do_jk, n_jk=di.n_jk, include_prob_max=inc_max,
+                           precision=precision, jk_mode=jk_mode)
+
+    if auto:  # corels1: one catalog, g_with_shape = 1 (corr.cpp:111)
+        if g1.n == 0:
+            return None
+        p = params(kind, 1)
+        p.auto_pairs = 1
+        return g1, g1, p, False
     # operand roles, corr.cpp:226-271: g_with_shape before the swap
     gws = 2 if pass_name in ("SRd", "SD", "SRs") else 0
     n1, n2 = g1.n, g2.n
@@ -60,13 +77,13 @@ def setup_pass(pass_name, g1, g2, di: DataInfo, precision=PREC_SPLIT, jk_mode=JK
     if n2 > n1:  # corr.cpp:281-296
         outer, inner, swapped = g2, g1, True
         gws = {2: 1, 1: 2}.get(gws, gws)
-    kind = pass_kind(di.which_corr, pass_name)
-    inc_max = di.include_prob_RR if pass_name in ("RsRs", "RsRd") else 1.0  # initialization.cpp:597-599
-    p = make_params(di.box_size, di.n_rp, di.rp_min, di.rp_max, di.n_pi, di.pi_min, di.pi_max, kind=kind,
-                    g_with_shape=gws if gws else 2, estimator=di.estimator, which_corr=di.which_corr,
-                    lin_bin=di.lin_bin, do_jk=di.do_jk, n_jk=di.n_jk, include_prob_max=inc_max,
-                    precision=precision, jk_mode=jk_mode)
-    return outer, inner, p, swapped
+    if kind == KIND_SHAPE_DENSITY and gws == 0:
+        # shape_density_corel rotates g1's shape unless g_with_shape is 2 (corels.cpp:216-219); with no
+        # shapes on g1 the reference reads an unset pointer, and such columns feed no estimator
+        gws = 1
+        if outer.e1 is None:
+            kind = KIND_DD
+    return outer, inner, params(kind, gws if gws else 2), swapped
 
 
 def config_datainfo(cfg, do_jk=1):
