@@ -7,7 +7,7 @@
 // GPU inside cpc_count_batch.
 //
 // Restates, for which_corr 0-2 on a periodic box (periodic_box 1, coordinates 6) or a survey
-// (periodic_box 0, coordinates 0), sorted input (data_sorted 1), do_jk 0 or 1:
+// (periodic_box 0, coordinates 0), data_sorted 1 (or 0 without auto pairs), do_jk 0 or 1:
 //   main                                   main.cpp:27-82
 //   data_ini: the positional parser and    initialization.cpp:62-350
 //             the derived flags
@@ -174,7 +174,8 @@ bool data_ini(DataInfo &d, const std::string &name) {
     return fail("coordinates " + std::to_string(d.coordinates) +
                 " is not supported with periodic_box " + std::to_string((int)d.periodic_box) +
                 " (supported: 6 with a periodic box, 0 without)");
-  if (d.data_sorted != 1) return fail("data_sorted must be 1 (catalogs sorted by redshift)");
+  if (d.periodic_box && d.data_sorted == 2) return fail("data_sorted==2 doesnot work with periodic box.. use 1");
+  if (d.data_sorted != 0 && d.data_sorted != 1) return fail("data_sorted must be 0 or 1");
   if (d.do_jk != 0 && d.do_jk != 1) return fail("do_jk must be 0 or 1");
   if (d.estimator < 0 || d.estimator > 4) return fail("estimator must be 0-4");
 
@@ -211,6 +212,12 @@ bool data_ini(DataInfo &d, const std::string &name) {
       d.do_auto_rebin = false;
       break;
   }
+  // The unsorted scan (do_pair_unsorted, corr.cpp:36-52) visits every inner galaxy, so a cross pass
+  // counts every pair inside the window whatever the order, as the engine does.  The auto pass of
+  // unsorted mode starts every image at j = i+1 (corr.cpp:167-182), which gives the pairs across the
+  // z boundary the other pi sign than sorted mode; the engine implements sorted mode only.
+  if (d.data_sorted == 0 && d.do_auto_corr)
+    return fail("estimators 1-3 (auto pairs) need data_sorted 1 and catalogs sorted by redshift");
   d.RsRd_same = false;
   if (d.n_density_rand == 0 && d.which_corr <= 2) {
     d.RsRd_same = true;
@@ -454,7 +461,7 @@ bool read_data(DataInfo &d, Role role, Cat &c) {
     std::cout << "gal data reading ended unexpectedly: " << i << "   " << n_gal << std::endl;
     return false;
   }
-  for (size_t k = 1; k < c.z.size(); ++k)
+  for (size_t k = 1; d.data_sorted == 1 && k < c.z.size(); ++k)
     if (c.z[k] < c.z[k - 1])
       return fail("data_sorted=1 but " + z_f + " is not sorted by redshift (row " + std::to_string(k) + ")");
   return true;

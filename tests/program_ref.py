@@ -4,6 +4,8 @@ corr_pc_amd/csrc/cpc_main.cpp so the two can be compared: the derived flags of d
 (read_dat.cpp:216-325, do_corr.cpp:40-96), do_corr's pass list and RsRd_same merge (do_corr.cpp:59-154),
 then do_bining through cpc_do_bining.  The pair counts come from the `count` callable (the oracle in
 the tests)."""
+import math
+
 import numpy as np
 
 from corr_pc_amd.engine import (GEOM_SKY, Catalog, Histogram, Norm, WSum, do_bining, make_params, rand_stream,
@@ -25,7 +27,11 @@ def derive(inp):
     rsrd_same = n_dr == 0 and int(inp["which_corr"]) <= 2
     if rsrd_same:
         n_dr = int(inp["n_shape_rand"])
-    return {"auto": auto, "cross": cross, "rebin": rebin, "rsrd_same": rsrd_same, "n_density_rand": n_dr}
+    ip_rr = 1.0
+    if int(inp.get("rand_subsample", 0)):
+        ip_rr = math.sqrt(math.sqrt(int(inp["n_density"]) / n_dr * int(inp["n_shape"]) / int(inp["n_shape_rand"])))
+    return {"auto": auto, "cross": cross, "rebin": rebin, "rsrd_same": rsrd_same, "n_density_rand": n_dr,
+            "include_prob_RR": ip_rr}
 
 
 def read_order(inp, flags):
@@ -72,9 +78,14 @@ def build_catalogs(inp, cols, table=None):
             it = np.floor(c["z"][:n] / table["dz"]).astype(np.int64)
             kw = {"dc": table["dC"][it], "da": table["dA"][it]}
         jk = c["jk"][:n].astype(np.int32) if (do_jk and c.get("jk") is not None) else np.full(n, -1, np.int32)
-        cats[role] = Catalog(x=c["x"][:n], y=c["y"][:n], z=c["z"][:n], w=None if c.get("w") is None else c["w"][:n],
-                             e1=e1, e2=e2, phi=elip_angle(e1, e2) if with_e else None, jk=jk, jk_prob=jk_prob,
-                             include_prob=inc, **kw)
+        cat = Catalog(x=c["x"][:n], y=c["y"][:n], z=c["z"][:n], w=None if c.get("w") is None else c["w"][:n],
+                      e1=e1, e2=e2, phi=elip_angle(e1, e2) if with_e else None, jk=jk, jk_prob=jk_prob,
+                      include_prob=inc, **kw)
+        if int(inp.get("data_sorted", 1)) == 0:
+            # the unsorted scan counts every pair inside the window, whatever the order; the oracle
+            # scans in z order, so hand it the same galaxies sorted by z
+            cat = cat.subset(np.argsort(cat.z, kind="stable"))
+        cats[role] = cat
     return cats, flags
 
 
@@ -87,7 +98,8 @@ def run(inp, cols, count, out_prefix, table=None, precision=0):
     di = DataInfo(which_corr=int(inp["which_corr"]), estimator=int(inp["estimator"]),
                   box_size=float(inp["periodic_box_size"]), n_rp=int(inp["n_bins"]), rp_min=float(inp["bin_r_min"]),
                   rp_max=float(inp["bin_r_max"]), n_pi=int(inp["n_p_bin"]), pi_min=float(inp["p_min"]),
-                  pi_max=float(inp["p_max"]), lin_bin=int(inp["lin_bin"]), do_jk=int(inp["do_jk"]), n_jk=n_jk)
+                  pi_max=float(inp["p_max"]), lin_bin=int(inp["lin_bin"]), do_jk=int(inp["do_jk"]), n_jk=n_jk,
+                  include_prob_RR=flags["include_prob_RR"])
     empty = Catalog(x=np.zeros(0), y=np.zeros(0), z=np.zeros(0))
     S = cats.get("shape", empty)
     Rs = cats.get("shape_rand", empty)
@@ -148,7 +160,7 @@ def run(inp, cols, count, out_prefix, table=None, precision=0):
         bining["RsRs"] = add(hists.get("RsRs"), hists.get("RsRd"))
         bining["SRd"] = add(hists.get("SRd"), hists.get("SRs"))
         Rd_w = Rs_w
-    norm = Norm(S_w, D_w, Rs_w, Rd_w, 1.0)
+    norm = Norm(S_w, D_w, Rs_w, Rd_w, flags["include_prob_RR"])
     po = make_params(di.box_size, di.n_rp, di.rp_min, di.rp_max, di.n_pi, di.pi_min, di.pi_max,
                      estimator=di.estimator, which_corr=di.which_corr, lin_bin=di.lin_bin, do_jk=di.do_jk,
                      n_jk=di.n_jk)

@@ -51,23 +51,27 @@ def _run_program(inp_path, precision):
 
 
 CASES = [
-    # (geometry, which_corr, estimator, precision)
-    ("box", 2, 0, PREC_SPLIT),
-    ("box", 1, 3, PREC_SPLIT),
-    ("box", 0, 0, PREC_FP64),
-    ("sky", 1, 0, PREC_SPLIT),
-    ("sky", 0, 2, PREC_SPLIT),
-    ("sky", 2, 4, PREC_SPLIT),
+    # (geometry, which_corr, estimator, precision, extra .inp settings)
+    ("box", 2, 0, PREC_SPLIT, {}),
+    ("box", 1, 3, PREC_SPLIT, {}),
+    ("box", 0, 0, PREC_FP64, {}),
+    ("box", 1, 0, PREC_SPLIT, {"data_sorted": 0, "lin_bin": 1, "bin_r_min": 1.0}),
+    ("sky", 1, 0, PREC_SPLIT, {}),
+    ("sky", 0, 2, PREC_SPLIT, {}),
+    ("sky", 2, 4, PREC_SPLIT, {}),
+    ("sky", 0, 0, PREC_SPLIT, {"rand_subsample": 1}),
 ]
 
 
-@pytest.mark.parametrize("geom,which,estimator,precision", CASES)
-def test_program_matches_restatement(tmp_path, oracle_lib, geom, which, estimator, precision):
+@pytest.mark.parametrize("geom,which,estimator,precision,extra", CASES)
+def test_program_matches_restatement(tmp_path, oracle_lib, geom, which, estimator, precision, extra):
     table = None
     if geom == "box":
-        path, inp, cols = box_case(tmp_path, which=which, estimator=estimator)
+        path, inp, cols = box_case(tmp_path, which=which, estimator=estimator,
+                                   sorted_z=extra.get("data_sorted", 1) == 1, **extra)
     else:
-        path, inp, cols, table = sky_case(tmp_path, which=which, estimator=estimator, n_rd=1800 if estimator == 4 else 0)
+        n_rd = 1800 if (estimator == 4 or extra.get("rand_subsample")) else 0
+        path, inp, cols, table = sky_case(tmp_path, which=which, estimator=estimator, n_rd=n_rd, **extra)
     _run_program(path, precision)
     ref_prefix = str(tmp_path / "ref" / "out_")
     os.makedirs(tmp_path / "ref")

@@ -50,7 +50,7 @@ def parse_plan(out):
 
 
 def box_case(tmp_path, which=2, estimator=0, n_s=1500, n_d=1200, n_rs=2500, n_rd=2000, do_jk=1, jk_side=2, seed=3,
-             sorted_z=True):
+             sorted_z=True, **extra):
     """A small periodic box in the reference's file format: shape and density samples with shapes,
     weights and jk files, shape and density randoms."""
     rng = np.random.default_rng(seed)
@@ -74,11 +74,12 @@ def box_case(tmp_path, which=2, estimator=0, n_s=1500, n_d=1200, n_rs=2500, n_rd
                p_max=16.0, z_min=0, z_max=L, periodic_box=1, periodic_box_size=L)
     for role, n in (("shape", n_s), ("density", n_d), ("shape_rand", n_rs), ("density_rand", n_rd)):
         inp.update(inputs.catalog_params(role, files[role], n))
+    inp.update(extra)
     path = inputs.write_inp(str(tmp_path / "run.inp"), **inp)
     return path, inp, cols
 
 
-def sky_case(tmp_path, which=1, estimator=0, n_s=1200, n_d=1000, n_rs=2400, n_rd=0, seed=5):
+def sky_case(tmp_path, which=1, estimator=0, n_s=1200, n_d=1000, n_rs=2400, n_rd=0, seed=5, **extra):
     """A small survey field: RA 10-14 deg, Dec -2..2 deg, z in (0.2, 0.5), with a flat LCDM table."""
     rng = np.random.default_rng(seed)
     table = synth.distance_table(0.3, 1e-4, 0.5)
@@ -103,6 +104,7 @@ def sky_case(tmp_path, which=1, estimator=0, n_s=1200, n_d=1000, n_rs=2400, n_rd
                p_max=30.0, z_min=0, z_max=0.5, dz=0.0001, periodic_box=0, periodic_box_size=0, distance_file=dist)
     for role, n in (("shape", n_s), ("density", n_d), ("shape_rand", n_rs), ("density_rand", n_rd)):
         inp.update(inputs.catalog_params(role, files[role], n))
+    inp.update(extra)
     path = inputs.write_inp(str(tmp_path / "run.inp"), **inp)
     return path, inp, cols, table
 
@@ -227,3 +229,26 @@ def test_unsorted_catalog_rejected(tmp_path):
     path, inp, cols = box_case(tmp_path, which=0, sorted_z=False)
     r = run_prog(path, "--plan")
     assert r.returncode == 1 and "not sorted by redshift" in r.stdout
+
+
+def test_unsorted_input_cross_only(tmp_path):
+    """data_sorted 0: cross passes take catalogs in any order; the auto estimators refuse it."""
+    path, inp, cols = box_case(tmp_path, which=0, sorted_z=False, data_sorted=0)
+    r = run_prog(path, "--plan")
+    assert r.returncode == 0, r.stdout
+    inputs.write_inp(path, **dict(inp, estimator=1))
+    r = run_prog(path, "--plan")
+    assert r.returncode == 1 and "need data_sorted 1" in r.stdout
+
+
+def test_rand_subsample_include_prob(tmp_path):
+    """rand_subsample 1: include_prob_RR = (nD/nRd * nS/nRs)^(1/4) (initialization.cpp:195-196) caps the
+    randoms of the RsRd pass and no other."""
+    path, inp, cols, table = sky_case(tmp_path, which=0, n_rd=1800, rand_subsample=1)
+    r = run_prog(path, "--plan")
+    assert r.returncode == 0, r.stdout
+    cats, passes, info = parse_plan(r.stdout)
+    expect = ((1000 / 1800 * 1200 / 2400) ** 0.5) ** 0.5
+    assert float(info["flags"]["include_prob_RR"]) == pytest.approx(expect, rel=1e-15)
+    for p in passes:
+        assert float(p["include_prob_max"]) == (float(info["flags"]["include_prob_RR"]) if p["flag"] == "RsRd" else 1.0)
