@@ -56,6 +56,8 @@ CASES = [
     ("box", 1, 3, PREC_SPLIT, {}),
     ("box", 0, 0, PREC_FP64, {}),
     ("box", 1, 0, PREC_SPLIT, {"data_sorted": 0, "lin_bin": 1, "bin_r_min": 1.0}),
+    # the generator notebook's box run: shape and density samples only, no randoms
+    ("box", 2, 0, PREC_SPLIT, {"n_shape_rand": 0, "n_density_rand": 0}),
     ("sky", 1, 0, PREC_SPLIT, {}),
     ("sky", 0, 2, PREC_SPLIT, {}),
     ("sky", 2, 4, PREC_SPLIT, {}),

@@ -252,3 +252,16 @@ def test_rand_subsample_include_prob(tmp_path):
     assert float(info["flags"]["include_prob_RR"]) == pytest.approx(expect, rel=1e-15)
     for p in passes:
         assert float(p["include_prob_max"]) == (float(info["flags"]["include_prob_RR"]) if p["flag"] == "RsRd" else 1.0)
+
+
+def test_box_without_randoms_plan(tmp_path):
+    """The generator notebook's box run (no randoms): RsRd_same, the shape randoms read as empty, and
+    only the SD pass counted; the analytic RR stands in for the randoms."""
+    path, inp, cols = box_case(tmp_path, which=2, n_shape_rand=0, n_density_rand=0)
+    r = run_prog(path, "--plan")
+    assert r.returncode == 0, r.stdout
+    cats, passes, info = parse_plan(r.stdout)
+    assert info["flags"]["rsrd_same"] == "1" and cats["Rs"]["n"] == "0" and cats["Rd"]["read"] == "0"
+    assert [p["name"] for p in passes] == ["SD"]
+    s = rand_stream(1500 + 1200, seed=program_ref.BOX_SEED)
+    assert float(cats["D"]["jk_prob0"]) == s[1500]  # no draws for the empty shape randoms
