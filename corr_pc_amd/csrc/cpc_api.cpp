@@ -645,7 +645,7 @@ struct Plan {
   // derived
   bool sky = false, jk_on = false, autop = false, split = false, use_split = false;
   int R = 1, nreg = 1, ncell = 0, kind_code = 0, NV = 0, NSP = 1, NST = 1, threads = 256;
-  int row0_p = 0, ncopy_p = 1, row0_t = 0, ncopy_t = 1;  // box: LDS rows and copies per pass
+  int row0_p = 0, ncopy_p = 1, row0_t = 0, ncopy_t = 1;  // LDS rows and copies per pass
   Derived dv{};
   const std::vector<double> *thr = nullptr;
   // device state and the host arrays copied into it (alive until the call's last synchronisation)
@@ -773,27 +773,28 @@ Status setup(cpc_ctx *ctx, Plan &P) {
   if (lds_p > ctx->max_lds)
     return {CPC_ERR_UNSUPPORTED, "histogram of " + std::to_string(ncell) + " cells does not fit in LDS (" +
                                      std::to_string(lds_p) + " bytes)"};
-  if (!sky) {
-    // LDS histogram per pass: passes with 7 or more slots per cell (shape kinds, primary pass),
-    // whose run flushes dominate, get as many copies (at most 4) of the top rp rows as fit in the
-    // group's share of the CU's LDS, keeping in LDS every row from rp >= rp_max / 20 up (rows below
-    // hold about 0.25% of the pairs of a homogeneous sample and go to the global histogram directly:
-    // device-wide atomics cost far more than LDS ones, 3% of the runs there cost as much as the
-    // whole LDS flush);
-    // other passes keep the whole histogram in one copy
-    const int waves_cu = P.split ? 16 : 8;  // 4 or 2 waves per SIMD
+  {
+    // LDS histogram per pass: passes with 7 or more slots per cell (shape kinds), whose run flushes
+    // dominate, get as many copies (at most 4) of the top rp rows as fit in the group's share of the
+    // CU's LDS, keeping in LDS every row from rp >= rp_max / 20 up (rows below hold about 0.25% of
+    // the pairs of a homogeneous sample and go to the global histogram directly: device-wide atomics
+    // cost far more than LDS ones, 3% of the runs there cost as much as the whole LDS flush);
+    // other passes keep the whole histogram in one copy.  Waves per CU: the box split kernel keeps
+    // 4 per SIMD, the box FP64 and the sky kernels 2.
+    const int waves_cu = (!sky && P.split) ? 16 : 8;
     const int groups_cu = std::max(1, waves_cu / std::max(1, P.threads / 64));
     const size_t share = ctx->max_lds / groups_cu;
     auto choose = [&](int ns, int &row0, int &ncopy) {
       row0 = 0;
       ncopy = 1;
       if (ns < 7) return;  // the kernel keeps the whole histogram in one copy (compiled that way)
-      for (int c = ns >= 7 ? 4 : 1; c >= 1; --c) {
+      for (int c = 4; c >= 1; --c) {
         int rows = p.n_rp;
         while (rows > 0 && pair_lds_bytes(rows * p.n_pi, ns, p.n_rp, c) > share) --rows;
         if (rows == 0) continue;
         const int r0 = p.n_rp - rows;
-        if (r0 == 0 || std::sqrt(thr[r0]) <= 0.05 * p.rp_max) {
+        const double rp_edge = sky ? thr[r0] : std::sqrt(thr[r0]);  // sky thresholds are on rp, box on rp^2
+        if (r0 == 0 || rp_edge <= 0.05 * p.rp_max) {
           row0 = r0;
           ncopy = c;
           return;
@@ -1274,6 +1275,8 @@ Status run_sky(cpc_ctx *ctx, Plan &P) {
       C.tcol = P.SI->d_colbeg.p;
       C.acc = ctx->acc.p + P.off_p;
       C.accnum = ctx->accn.p + P.off_pn;
+      C.row0 = P.row0_p;
+      C.ncopy = P.ncopy_p;
       check(launch_sky(P.kind_code, P.use_split, false, C, P.threads, sky_side(*P.SO), sky_side(*P.SI), s), st,
             "sky pair kernel");
     }
@@ -1289,6 +1292,8 @@ Status run_sky(cpc_ctx *ctx, Plan &P) {
       C.tcol = P.SO->d_colbeg.p;
       C.acc = ctx->acc.p + P.off_t;
       C.accnum = ctx->accn.p + P.off_tn;
+      C.row0 = P.row0_t;
+      C.ncopy = P.ncopy_t;
       check(launch_sky(P.kind_code, P.use_split, true, C, P.threads, sky_side(*P.SI), sky_side(*P.SO), s), st,
             "sky touch kernel");
     }

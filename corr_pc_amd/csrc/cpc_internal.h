@@ -144,6 +144,7 @@ struct SkyArgs {
   int split_on;
   float la, lb, g, s_out_lo, s_out_hi;
   int auto_mode;           // corels1 pass: pairs only with the inner galaxy after the outer one
+  int row0, ncopy;         // LDS holds rp rows row0.. in ncopy lane-selected copies (7+ slots per cell)
   const double *thr;       // n_rp+1 thresholds on rp (host libm log10)
   const Item *items;
   const Chunk *chunks;

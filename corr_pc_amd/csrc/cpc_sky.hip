@@ -66,25 +66,35 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
   constexpr int NS = SWAP ? 1 + NV : 1 + 2 * NV;
   constexpr int NQ = 1 + 2 * NV;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int ncell = P.ncell;
+  // LDS as in the box kernel: ncopy lane-selected copies of the rp rows from row0 up, lnum[ncopy][lcell]
+  // and lacc[ncopy][NS][lcell], then the thresholds; runs of the rows below go to the global histogram.
+  // Compiled in where the flush dominates (7 or more slots per cell).
+  constexpr bool CUT = NS >= 7;
+  const int ncell = P.ncell, ncopy = CUT ? P.ncopy : 1;
+  const int lbase = CUT ? P.row0 * P.n_pi : 0, lcell = CUT ? ncell - lbase : ncell;
   unsigned *lnum = reinterpret_cast<unsigned *>(smem);
-  double *lacc = reinterpret_cast<double *>(smem + ((ncell * 4 + 15) & ~15));
-  double *lthr = lacc + (size_t)NS * ncell;
+  double *lacc = reinterpret_cast<double *>(smem + ((ncopy * lcell * 4 + 15) & ~15));
+  double *lthr = lacc + (size_t)ncopy * NS * lcell;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = uni(tid >> 6);
   const int nwave = blockDim.x >> 6;
+  const int cp = lane % ncopy;
+  lds_u32 *const mynum = (lds_u32 *)(lnum + cp * lcell);
+  lds_f64 *const myacc = (lds_f64 *)(lacc + (size_t)cp * NS * lcell);
   for (int k = tid; k <= P.n_rp; k += blockDim.x) lthr[k] = P.thr[k];
 
   const Item it = P.items[blockIdx.x];
   const Group bg = P.bgrp[it.group0];  // region of every chunk of the item
   const int tgt = it.target;
+  unsigned long long *const gnum = P.accnum + (size_t)tgt * ncell;
+  double *const gacc = P.acc + (size_t)tgt * NS * ncell;
   unsigned long long ncand = 0;
-  for (int c = tid; c < ncell; c += blockDim.x) {
+  for (int c = tid; c < ncopy * lcell; c += blockDim.x) {
     lnum[c] = 0;
 #pragma unroll
-    for (int s = 0; s < NS; ++s) lacc[s * ncell + c] = 0.0;
+    for (int s = 0; s < NS; ++s) lacc[s * ncopy * lcell + c] = 0.0;
   }
   __syncthreads();
 
@@ -238,7 +248,12 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
             }
             if (cell >= 0) {
               if (cell != run_cell) {
-                if (run_cell >= 0) lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
+                if (run_cell >= 0) {
+                  if constexpr (CUT)
+                    flush_run<NS>(mynum, myacc, lcell, lbase, gnum, gacc, ncell, run_cell, run_n, run);
+                  else
+                    lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
+                }
                 run_cell = cell;
                 run_n = 0;
 #pragma unroll
@@ -260,15 +275,25 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
         }
       }
     }
-    if (run_cell >= 0) lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
+    if (run_cell >= 0) {
+      if constexpr (CUT)
+        flush_run<NS>(mynum, myacc, lcell, lbase, gnum, gacc, ncell, run_cell, run_n, run);
+      else
+        lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
+    }
   }
   __syncthreads();
-  for (int c = tid; c < ncell; c += blockDim.x) {
-    const unsigned n = lnum[c];
+  for (int c = tid; c < lcell; c += blockDim.x) {
+    unsigned n = 0;
+    for (int k = 0; k < ncopy; ++k) n += lnum[k * lcell + c];
     if (n) {
-      atomicAdd(&P.accnum[(size_t)tgt * ncell + c], (unsigned long long)n);
+      atomicAdd(&gnum[lbase + c], (unsigned long long)n);
 #pragma unroll
-      for (int s = 0; s < NS; ++s) unsafeAtomicAdd(&P.acc[((size_t)tgt * NS + s) * ncell + c], lacc[s * ncell + c]);
+      for (int s = 0; s < NS; ++s) {
+        double v = 0.0;
+        for (int k = 0; k < ncopy; ++k) v += lacc[((size_t)k * NS + s) * lcell + c];
+        unsafeAtomicAdd(&gacc[(size_t)s * ncell + lbase + c], v);
+      }
     }
   }
   if (ncand) atomicAdd(&P.counters[0], ncand);
@@ -303,7 +328,7 @@ static hipError_t launch_sky_t(const SkyArgs &P, int threads, size_t lds, const 
 hipError_t launch_sky(int kind, bool split, bool swap, const SkyArgs &P, int threads, const SkySide &B,
                       const SkySide &T, hipStream_t st) {
   if (P.n_items == 0) return hipSuccess;
-  const size_t lds = pair_lds_bytes(P.ncell, P.nslots, P.n_rp);
+  const size_t lds = pair_lds_bytes(P.ncell - P.row0 * P.n_pi, P.nslots, P.n_rp, P.ncopy);
 #define CPC_SKY(KK)                                                                                     \
   if (kind == KK) {                                                                                     \
     if (split && swap) return launch_sky_t<KK, true, true>(P, threads, lds, B, T, st); \
