@@ -569,7 +569,24 @@ void host_sky_pair(const cpc_params &p, const Derived &dv, const cpc_catalog *g1
       if (t > M_PI / 2.0) t = M_PI - t;
       return t;
     };
-    if (p.kind == CPC_KIND_SHAPE_DENSITY) {
+    if (p.kind == CPC_KIND_DENSITY_CONVERGENCE) {
+      // density_convergence_corel (corels.cpp:79-91): kappa in e[0] of the g_with_shape galaxy
+      if (p.g_with_shape == 1) v[1] = wt * g1->h_e1[i] * 1.0;
+      else if (p.g_with_shape == 2) v[1] = wt * g2->h_e1[j] * 1.0;
+    } else if (p.kind == CPC_KIND_SHAPE_CONVERGENCE) {
+      // shape_convergence_corel (corels.cpp:93-120)
+      double ep = 0, epr = 0, exr = 0;
+      if (p.g_with_shape == 2) {
+        rot(g2->h_e1[j], g2->h_e2[j], epr, exr);
+        ep = wt * g1->h_e1[i] * 1.0;
+      } else if (p.g_with_shape == 1) {
+        rot(g1->h_e1[i], g1->h_e2[i], epr, exr);
+        ep = wt * g2->h_e1[j] * 1.0;
+      }
+      v[1] = ep;
+      v[3] = ep * epr;
+      v[5] = ep * exr;
+    } else if (p.kind == CPC_KIND_SHAPE_DENSITY) {
       double ep, ex;
       if (p.g_with_shape == 2)
         rot(g2->h_e1[j], g2->h_e2[j], ep, ex);
@@ -644,7 +661,7 @@ struct Plan {
   cpc_stats *stats = nullptr;
   // derived
   bool sky = false, jk_on = false, autop = false, split = false, use_split = false;
-  int R = 1, nreg = 1, ncell = 0, kind_code = 0, NV = 0, NSP = 1, NST = 1, threads = 256;
+  int R = 1, nreg = 1, ncell = 0, kind_code = 0, NV = 0, NE = 0, NSP = 1, NST = 1, threads = 256;
   int row0_p = 0, ncopy_p = 1, row0_t = 0, ncopy_t = 1;  // LDS rows and copies per pass
   Derived dv{};
   const std::vector<double> *thr = nullptr;
@@ -672,9 +689,12 @@ struct Plan {
 };
 
 // Pair-kind code of the kernels and the data[] fields each kind fills (corels.cpp:48-244):
-// 0 DD, 1 shape-density, 2 shape-shape estimator 0, 3 shape-shape other estimators.
-const int kNV[4] = {0, 3, 5, 6};
-const int kValField[4][6] = {{0}, {1, 2, 6}, {1, 3, 4, 5, 6}, {1, 2, 3, 4, 5, 6}};
+// 0 DD, 1 shape-density, 2 shape-shape estimator 0, 3 shape-shape other estimators,
+// 4 density-convergence, 5 shape-convergence.  kNE: how many of the fields (the first ones) also
+// carry their err sums.
+const int kNV[6] = {0, 3, 5, 6, 1, 3};
+const int kNE[6] = {0, 3, 5, 6, 0, 0};
+const int kValField[6][6] = {{0}, {1, 2, 6}, {1, 3, 4, 5, 6}, {1, 2, 3, 4, 5, 6}, {1}, {1, 3, 5}};
 
 // The reference's own preconditions and the engine's, checked before any device work.
 Status validate(const cpc_job &job, Plan &P) {
@@ -696,9 +716,12 @@ Status validate(const cpc_job &job, Plan &P) {
   if (!p.lin_bin && !(p.rp_min > 0)) return {CPC_ERR_INVALID, "log rp bins need rp_min > 0"};
   if (!sky && (2 * p.rp_max >= p.box_size || p.pi_max - p.pi_min >= p.box_size))
     return {CPC_ERR_UNSUPPORTED, "the rp and pi windows must be narrower than the box"};
-  if (p.kind < CPC_KIND_DD || p.kind > CPC_KIND_SHAPE_SHAPE) return {CPC_ERR_INVALID, "unknown kind"};
-  if (p.kind != CPC_KIND_DD && p.g_with_shape != 1 && p.g_with_shape != 2)
+  if (p.kind < CPC_KIND_DD || p.kind > CPC_KIND_SHAPE_CONVERGENCE) return {CPC_ERR_INVALID, "unknown kind"};
+  const bool conv = p.kind == CPC_KIND_DENSITY_CONVERGENCE || p.kind == CPC_KIND_SHAPE_CONVERGENCE;
+  if (p.kind != CPC_KIND_DD && !conv && p.g_with_shape != 1 && p.g_with_shape != 2)
     return {CPC_ERR_INVALID, "g_with_shape must be 1 or 2 for shape kinds"};
+  if (conv && (p.g_with_shape < 0 || p.g_with_shape > 2))
+    return {CPC_ERR_INVALID, "g_with_shape must be 0, 1 or 2 for the convergence kinds"};
   if (p.do_jk != 0 && p.do_jk != 1) return {CPC_ERR_UNSUPPORTED, "do_jk=2 (region pairs) is not supported"};
   if (p.auto_pairs != 0 && p.auto_pairs != 1) return {CPC_ERR_INVALID, "auto_pairs must be 0 or 1"};
   P.autop = p.auto_pairs == 1;
@@ -721,6 +744,16 @@ Status validate(const cpc_job &job, Plan &P) {
       return {CPC_ERR_INVALID, "shape_density needs e1/e2 on the shape side (the reference would "
                                "dereference an unallocated e pointer, corels.cpp:216-219)"};
   }
+  if (conv && p.g_with_shape != 0) {
+    // the galaxy read for kappa (e[0]) and, for shape-convergence, the one rotated (corels.cpp:79-120)
+    const cpc_catalog *ws = p.g_with_shape == 2 ? ci : co, *other = p.g_with_shape == 2 ? co : ci;
+    const cpc_catalog *kap = p.kind == CPC_KIND_DENSITY_CONVERGENCE ? ws : other;
+    if (kap->n && !kap->has_e)
+      return {CPC_ERR_INVALID, "the convergence kinds need kappa (e1) on the catalog that carries it (the "
+                               "reference would dereference an unallocated e pointer, corels.cpp:79-120)"};
+    if (p.kind == CPC_KIND_SHAPE_CONVERGENCE && ws->n && !ws->has_e)
+      return {CPC_ERR_INVALID, "shape_convergence needs e1/e2 on the shape side"};
+  }
   if (sky) {
     if ((co->n && !co->has_sky) || (ci->n && !ci->has_sky))
       return {CPC_ERR_INVALID, "the sky geometry needs dc and da on both catalogs"};
@@ -740,10 +773,13 @@ Status validate(const cpc_job &job, Plan &P) {
     P.kind_code = 0;
   else if (p.kind == CPC_KIND_SHAPE_DENSITY)
     P.kind_code = 1;
-  else
+  else if (p.kind == CPC_KIND_SHAPE_SHAPE)
     P.kind_code = p.estimator == 0 ? 2 : 3;
+  else
+    P.kind_code = p.kind == CPC_KIND_DENSITY_CONVERGENCE ? 4 : 5;
   P.NV = kNV[P.kind_code];
-  P.NSP = 1 + 2 * P.NV;
+  P.NE = kNE[P.kind_code];
+  P.NSP = 1 + P.NV + P.NE;
   P.NST = 1 + P.NV;
   P.nP = (size_t)2 * P.nreg;
   P.nT = (size_t)P.nreg;
@@ -1327,7 +1363,8 @@ Status run_sky(cpc_ctx *ctx, Plan &P) {
 int64_t assemble(const Plan &P, const std::vector<double> &acc, const std::vector<unsigned long long> &accn,
                  double ms_total) {
   const cpc_params &p = P.p;
-  const int ncell = P.ncell, nreg = P.nreg, R = P.R, NV = P.NV, NSP = P.NSP, NST = P.NST, kc = P.kind_code;
+  const int ncell = P.ncell, nreg = P.nreg, R = P.R, NV = P.NV, NE = P.NE, NSP = P.NSP, NST = P.NST,
+            kc = P.kind_code;
   const double *accp = acc.data() + P.off_p, *acct = acc.data() + P.off_t;
   const unsigned long long *accpn = accn.data() + P.off_pn, *acctn = accn.data() + P.off_tn;
   const unsigned long long *counters = accn.data() + P.off_c;
@@ -1367,7 +1404,7 @@ int64_t assemble(const Plan &P, const std::vector<double> &acc, const std::vecto
       for (int v = 0; v < NV; ++v) {
         const int f = kValField[kc][v];
         out->val[(size_t)c * CPC_NFIELD + f] += accp[((size_t)t * NSP + 1 + v) * ncell + c];
-        out->err[(size_t)c * CPC_NFIELD + f] += accp[((size_t)t * NSP + 1 + NV + v) * ncell + c];
+        if (v < NE) out->err[(size_t)c * CPC_NFIELD + f] += accp[((size_t)t * NSP + 1 + NV + v) * ncell + c];
       }
     }
   if (host_pairs)

@@ -8,18 +8,23 @@
 
 namespace cpc {
 
-enum { kDD = 0, kSD = 1, kSS0 = 2, kSS1 = 3 };
+enum { kDD = 0, kSD = 1, kSS0 = 2, kSS1 = 3, kDK = 4, kSK = 5 };
 
+// Per pair kind: NV data[] values accumulated, and NE of them with their err sums (the first NE).
 template <int K>
 struct KindNV;
 template <>
-struct KindNV<kDD> { static constexpr int NV = 0; };
+struct KindNV<kDD> { static constexpr int NV = 0, NE = 0; };
 template <>
-struct KindNV<kSD> { static constexpr int NV = 3; };  // data[1], data[2], data[6]
+struct KindNV<kSD> { static constexpr int NV = 3, NE = 3; };  // data[1], data[2], data[6]
 template <>
-struct KindNV<kSS0> { static constexpr int NV = 5; };  // data[1], data[3..6]
+struct KindNV<kSS0> { static constexpr int NV = 5, NE = 5; };  // data[1], data[3..6]
 template <>
-struct KindNV<kSS1> { static constexpr int NV = 6; };  // data[1..6]
+struct KindNV<kSS1> { static constexpr int NV = 6, NE = 6; };  // data[1..6]
+template <>
+struct KindNV<kDK> { static constexpr int NV = 1, NE = 0; };  // data[1]; no err (corels.cpp:79-91)
+template <>
+struct KindNV<kSK> { static constexpr int NV = 3, NE = 0; };  // data[1], data[3], data[5]; no err (:93-120)
 
 // 1/x from the hardware reciprocal estimate and one Newton step: the relative error squares, to
 // below 1e-13 even from a 2^-22 estimate (x finite, != 0).  Used only where the result feeds a
@@ -71,8 +76,9 @@ __device__ __forceinline__ double ephi(double phi, double theta) {
   return t;
 }
 
-// Shape quantities of one binned pair, g1 = outer, g2 = inner (corels.cpp:122-244).
-// q = [wt, vals..., errs...] in the slot order of KindNV.
+// Shape quantities of one binned pair, g1 = outer, g2 = inner (corels.cpp:79-244).
+// q = [wt, vals..., errs...] in the slot order of KindNV.  The convergence kinds read the convergence
+// kappa where the reference keeps it, in e[0] (e1 here) of the galaxy that carries it.
 // (dxr, dyr) is the orientation vector whose atan(dyr/dxr) is the reference's angle: raw coordinate
 // differences in the box (angle_orient_axis_PB, calcs.cpp:54-57), the cos(mean Dec)-scaled RA/Dec
 // differences on the sky (angle_orient_axis_sky, sky_calcs.cpp:13-22).
@@ -83,6 +89,10 @@ __device__ __forceinline__ void pair_values_v(const Gal &g1, const Gal &g2, int 
   q[0] = wt;
   if constexpr (K == kDD) {
     return;
+  } else if constexpr (K == kDK) {
+    // density_convergence_corel: w kappa of the g_with_shape galaxy (sig_crit = 1); g_with_shape 0
+    // assigns nothing and adds the initial 0
+    q[1] = gws == 1 ? wt * g1.e1 : (gws == 2 ? wt * g2.e1 : 0.0);
   } else {
     double theta, c2, s2;
     if constexpr (SPLIT) {
@@ -98,7 +108,21 @@ __device__ __forceinline__ void pair_values_v(const Gal &g1, const Gal &g2, int 
       c2 = cos(2.0 * theta);
       s2 = sin(2.0 * theta);
     }
-    if constexpr (K == kSD) {
+    if constexpr (K == kSK) {
+      // shape_convergence_corel: rotate the g_with_shape galaxy's shape, kappa from the other one;
+      // ep = w kappa, e++ = ep e+, e+x = ep ex; g_with_shape 0 adds the initial zeros
+      if (gws == 1 || gws == 2) {
+        const Gal &s = (gws == 2) ? g2 : g1;
+        const double kap = (gws == 2) ? g1.e1 : g2.e1;
+        const double ep = wt * kap;
+        q[1] = ep;
+        q[2] = ep * (c2 * s.e1 - s2 * s.e2);
+        q[3] = ep * (s2 * s.e1 + c2 * s.e2);
+      } else {
+        q[1] = q[2] = q[3] = 0.0;
+      }
+      (void)theta;
+    } else if constexpr (K == kSD) {
       // shape_density_corel: rotate the shape galaxy, theta column from g1's phi
       const Gal &s = (gws == 2) ? g2 : g1;
       const double ep = c2 * s.e1 - s2 * s.e2;

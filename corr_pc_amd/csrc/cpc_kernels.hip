@@ -50,8 +50,9 @@ using CKArgs = __attribute__((address_space(4))) const KArgs;
 template <int K, bool SPLIT, bool SWAP>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT ? 4 : 2))) pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
   constexpr int NV = KindNV<K>::NV;
-  constexpr int NS = SWAP ? 1 + NV : 1 + 2 * NV;  // f64 slots accumulated
-  constexpr int NQ = 1 + 2 * NV;                  // quantities computed
+  constexpr int NE = KindNV<K>::NE;
+  constexpr int NS = SWAP ? 1 + NV : 1 + NV + NE;  // f64 slots accumulated
+  constexpr int NQ = 1 + NV + NE;                  // quantities computed
   extern __shared__ __align__(16) unsigned char smem[];
   const Item it = items[blockIdx.x];
   // the pass arguments of the item's job, read as constant memory: wave-uniform scalar loads the
@@ -471,6 +472,8 @@ hipError_t launch_pair(int kind, bool split, bool swap, const KArgs *jobs, const
   CPC_DISPATCH(kSD)
   CPC_DISPATCH(kSS0)
   CPC_DISPATCH(kSS1)
+  CPC_DISPATCH(kDK)
+  CPC_DISPATCH(kSK)
 #undef CPC_DISPATCH
   return hipErrorInvalidValue;
 }

@@ -6,7 +6,7 @@
 // bins2D_*.dat / *.temp file of do_bining, and inp2.info.  Host side only; the pair loops run on the
 // GPU inside cpc_count_batch.
 //
-// Restates, for which_corr 0-2 on a periodic box (periodic_box 1, coordinates 6) or a survey
+// Restates, for which_corr 0-4 on a periodic box (periodic_box 1, coordinates 6) or a survey
 // (periodic_box 0, coordinates 0), data_sorted 1 (or 0 without auto pairs), do_jk 0 or 1:
 //   main                                   main.cpp:27-82
 //   data_ini: the positional parser and    initialization.cpp:62-350
@@ -163,8 +163,9 @@ bool data_ini(DataInfo &d, const std::string &name) {
   if (in.fail()) return fail("data_ini:: could not read every parameter of " + name);
 
   // modes this engine runs
-  if (d.which_corr < 0 || d.which_corr > 2)
-    return fail("which_corr " + std::to_string(d.which_corr) + " is not supported (0-2: density and IA pairs)");
+  if (d.which_corr < 0 || d.which_corr > 4)
+    return fail("which_corr " + std::to_string(d.which_corr) +
+                " is not supported (0-2: density and IA pairs, 3-4: convergence pairs)");
   if (d.periodic_box && d.coordinates != 6 && d.coordinates != 7) {
     std::cout << " wrong coordinates choice for periodic box:" << d.periodic_box_size << " " << d.coordinates
               << std::endl;
@@ -321,7 +322,8 @@ struct Cat {
   bool read = false;  // false: never read (an empty catalog for every pass)
   std::vector<double> x, y, z, w, e1, e2, phi, jkp, inc, dc, da;
   std::vector<int32_t> jk;
-  bool has_e = false;
+  bool has_e = false;  // e1 (and e2) carry values: shapes, or a convergence in e1
+  bool kappa = false;  // e1 is a convergence kappa read from a one-column file, e2 is zero
   int64_t n() const { return (int64_t)x.size(); }
 };
 
@@ -342,13 +344,32 @@ double elip_angle(double e0, double e1_in) {
   return phi;
 }
 
-// read_data (read_dat.cpp:18-347) for coordinates 0 and 6, which_corr 0-2.
+// read_data (read_dat.cpp:18-347) for coordinates 0 and 6, which_corr 0-4.
 bool read_data(DataInfo &d, Role role, Cat &c) {
   std::string p_f, z_f, e_f, w_f, jk_f;
   int n_gal = 0;
   VarSt *wt = nullptr;
-  bool read_e = d.which_corr != 0, read_jk = true;
-  const bool cross_shape_shape = d.which_corr == 2;
+  // what the e file holds, per which_corr (read_dat.cpp:28-68): shapes (e1 e2), or a convergence
+  bool read_e = true, read_kappa = false, read_phi = false, cross_shape_shape = false, read_jk = true;
+  switch (d.which_corr) {
+    case 0:
+      read_e = false;
+      break;
+    case 2:
+      cross_shape_shape = true;
+      break;
+    case 3:
+      read_e = false;
+      read_kappa = true;
+      break;
+    case 4:
+      read_e = role == kShape;  // the shape sample has e, the density sample a convergence
+      read_kappa = role != kShape;
+      cross_shape_shape = true;
+      break;
+    default:
+      break;
+  }
   const bool is_rand = role == kShapeRand || role == kDensityRand;
   switch (role) {
     case kShape:
@@ -361,15 +382,15 @@ bool read_data(DataInfo &d, Role role, Cat &c) {
       if (cross_shape_shape)
         e_f = d.density_e_file;
       else
-        read_e = false;
+        read_e = read_kappa = false;
       break;
     case kShapeRand:
       p_f = d.shape_rand_pos_file, z_f = d.shape_rand_z_file, w_f = d.shape_rand_wt_file,
-      jk_f = d.shape_rand_jk_file, n_gal = d.n_shape_rand, wt = &d.Rs_wt, read_e = false;
+      jk_f = d.shape_rand_jk_file, n_gal = d.n_shape_rand, wt = &d.Rs_wt, read_e = read_kappa = false;
       break;
     case kDensityRand:
       p_f = d.density_rand_pos_file, z_f = d.density_rand_z_file, w_f = d.density_rand_wt_file,
-      jk_f = d.density_rand_jk_file, n_gal = d.n_density_rand, wt = &d.Rd_wt, read_e = false;
+      jk_f = d.density_rand_jk_file, n_gal = d.n_density_rand, wt = &d.Rd_wt, read_e = read_kappa = false;
       break;
   }
   c.read = true;
@@ -393,15 +414,21 @@ bool read_data(DataInfo &d, Role role, Cat &c) {
   if (!data_p.is_open()) return fail("initialisation::pos File not open: " + p_f);
   if (!data_z.is_open()) return fail("initialisation::z File not open:" + z_f);
   if (d.do_jk && !data_jk.is_open() && read_jk) return fail("initialisation::jk File not open: " + jk_f);
-  if (read_e) {
+  if (read_e || read_kappa) {
     data_e.open(e_f.c_str());
     if (!data_e.is_open()) return fail("initialisation:: e File not open:" + e_f);
-    // The reference reads a file whose name holds "phi" or "kappa" as one column (phi or a
-    // convergence) and leaves the shapes unset, which its shape pair functions then read.
-    if (e_f.find("phi") != std::string::npos || e_f.find("kappa") != std::string::npos)
-      return fail("e file " + e_f + ": one-column phi / kappa inputs are not supported");
+    // the file's name can switch it to one column of phi or of kappa (read_dat.cpp:189-204)
+    if (e_f.find("phi") != std::string::npos) read_phi = true, read_e = read_kappa = false;
+    if (e_f.find("kappa") != std::string::npos || read_kappa) read_kappa = true, read_e = read_phi = false;
+    // One column of phi leaves the shapes unset, and so does a convergence where the pair kind
+    // rotates shapes (which_corr 1-2); the reference's pair functions would read them anyway.
+    if (read_phi) return fail("e file " + e_f + ": one-column phi inputs are not supported");
+    if (read_kappa && d.which_corr < 3)
+      return fail("e file " + e_f + ": read as one column of kappa, which leaves the shapes of which_corr " +
+                  std::to_string(d.which_corr) + " unset");
   }
-  c.has_e = read_e;
+  c.has_e = read_e || read_kappa;
+  c.kappa = read_kappa;
   c.x.reserve(n_gal), c.y.reserve(n_gal), c.z.reserve(n_gal), c.w.reserve(n_gal), c.jkp.reserve(n_gal);
   int i = 0;
   while (data_p.good() && data_z.good() && (data_jk.good() || !read_jk) && (data_wt.good() || w_f == "0") &&
@@ -440,6 +467,11 @@ bool read_data(DataInfo &d, Role role, Cat &c) {
       c.e1.push_back(a);
       c.e2.push_back(b);
       phi = elip_angle(a, b);
+    } else if (read_kappa) {
+      double k = 0;
+      data_e >> k;
+      c.e1.push_back(k);
+      c.e2.push_back(0.0);
     }
     double include_prob = 0;
     if (is_rand) include_prob = rand() / (RAND_MAX + 1.);
@@ -492,8 +524,10 @@ Setup setup_pass(const DataInfo &d, const Pass &ps) {
     s.kind = CPC_KIND_DD;
   } else if (d.which_corr == 1) {
     s.kind = f == "SS" ? CPC_KIND_SHAPE_SHAPE : CPC_KIND_SHAPE_DENSITY;
-  } else {
+  } else if (d.which_corr == 2) {
     s.kind = (f == "SD" || f == "SS") ? CPC_KIND_SHAPE_SHAPE : CPC_KIND_SHAPE_DENSITY;
+  } else {
+    s.kind = d.which_corr == 3 ? CPC_KIND_DENSITY_CONVERGENCE : CPC_KIND_SHAPE_CONVERGENCE;
   }
   s.include_prob_max = rr ? d.include_prob_RR : 1.0;
   if (ps.auto_pass) {
@@ -523,6 +557,17 @@ Setup setup_pass(const DataInfo &d, const Pass &ps) {
     if (!s.outer->has_e) {
       // g1 carries no shape: the reference reads an unset pointer there; these shape columns feed
       // no estimator, so the pass runs as density pairs
+      s.kind = CPC_KIND_DD;
+      s.shapes_undefined = true;
+    }
+  }
+  if ((s.kind == CPC_KIND_DENSITY_CONVERGENCE || s.kind == CPC_KIND_SHAPE_CONVERGENCE) && s.gws != 0) {
+    // kappa from the g_with_shape galaxy (density-convergence) or from the other one (shape-
+    // convergence, which rotates the g_with_shape galaxy's shape) (corels.cpp:79-120); a random
+    // catalog there carries none and the reference reads an unset pointer: density pairs instead
+    const Cat *ws = s.gws == 2 ? s.inner : s.outer, *ot = s.gws == 2 ? s.outer : s.inner;
+    const Cat *kap = s.kind == CPC_KIND_DENSITY_CONVERGENCE ? ws : ot;
+    if (!kap->has_e || (s.kind == CPC_KIND_SHAPE_CONVERGENCE && (!ws->has_e || ws->kappa))) {
       s.kind = CPC_KIND_DD;
       s.shapes_undefined = true;
     }
@@ -602,7 +647,13 @@ cpc_catalog_desc describe(const Cat &c) {
 }
 
 const char *kind_name(int k) {
-  return k == CPC_KIND_DD ? "dd" : (k == CPC_KIND_SHAPE_DENSITY ? "shape_density" : "shape_shape");
+  switch (k) {
+    case CPC_KIND_DD: return "dd";
+    case CPC_KIND_SHAPE_DENSITY: return "shape_density";
+    case CPC_KIND_SHAPE_SHAPE: return "shape_shape";
+    case CPC_KIND_DENSITY_CONVERGENCE: return "density_convergence";
+    default: return "shape_convergence";
+  }
 }
 
 struct Options {

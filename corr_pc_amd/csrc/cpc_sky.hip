@@ -63,8 +63,9 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
   const SRec *__restrict__ tsr = T.sr;
   const Win *__restrict__ twin = T.win;
   constexpr int NV = KindNV<K>::NV;
-  constexpr int NS = SWAP ? 1 + NV : 1 + 2 * NV;
-  constexpr int NQ = 1 + 2 * NV;
+  constexpr int NE = KindNV<K>::NE;
+  constexpr int NS = SWAP ? 1 + NV : 1 + NV + NE;
+  constexpr int NQ = 1 + NV + NE;
   extern __shared__ __align__(16) unsigned char smem[];
   // LDS as in the box kernel: ncopy lane-selected copies of the rp rows from row0 up, lnum[ncopy][lcell]
   // and lacc[ncopy][NS][lcell], then the thresholds; runs of the rows below go to the global histogram.
@@ -340,6 +341,8 @@ hipError_t launch_sky(int kind, bool split, bool swap, const SkyArgs &P, int thr
   CPC_SKY(kSD)
   CPC_SKY(kSS0)
   CPC_SKY(kSS1)
+  CPC_SKY(kDK)
+  CPC_SKY(kSK)
 #undef CPC_SKY
   return hipErrorInvalidValue;
 }

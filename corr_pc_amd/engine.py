@@ -25,6 +25,8 @@ GEOM_SKY = 1
 KIND_DD = 0
 KIND_SHAPE_DENSITY = 1
 KIND_SHAPE_SHAPE = 2
+KIND_DENSITY_CONVERGENCE = 3
+KIND_SHAPE_CONVERGENCE = 4
 PREC_SPLIT = 0
 PREC_FP64 = 1
 JK_LEAVE_ONE_OUT = 0
@@ -164,7 +166,8 @@ def make_params(box_size, n_rp, rp_min, rp_max, n_pi, pi_min, pi_max, kind=KIND_
                 estimator=0, which_corr=None, lin_bin=0, do_jk=0, n_jk=0, include_prob_max=1.0,
                 precision=PREC_SPLIT, jk_mode=JK_LEAVE_ONE_OUT):
     if which_corr is None:
-        which_corr = {KIND_DD: 0, KIND_SHAPE_DENSITY: 1, KIND_SHAPE_SHAPE: 2}[kind]
+        which_corr = {KIND_DD: 0, KIND_SHAPE_DENSITY: 1, KIND_SHAPE_SHAPE: 2, KIND_DENSITY_CONVERGENCE: 3,
+                      KIND_SHAPE_CONVERGENCE: 4}[kind]
     return Params(GEOM_BOX, box_size, kind, g_with_shape, estimator, which_corr, n_rp, rp_min, rp_max, lin_bin,
                   n_pi, pi_min, pi_max, do_jk, n_jk, include_prob_max, precision, jk_mode)
 

@@ -62,9 +62,10 @@ def write_inp(path, **params):
     return path
 
 
-def write_catalog(prefix, x, y, z, e1=None, e2=None, w=None, jk=None):
-    """Write <prefix>_pos.dat, _z.dat and, when given, _e.dat, _wt.dat, _jk.dat (write_data's layout).
-    Returns the .inp file entries (pos, z, e, wt, jk), "0" for the files not written."""
+def write_catalog(prefix, x, y, z, e1=None, e2=None, w=None, jk=None, kappa=None):
+    """Write <prefix>_pos.dat, _z.dat and, when given, _e.dat (e1 e2) or _kappa.dat (one column of
+    convergence, which_corr 3-4), _wt.dat, _jk.dat (write_data's layout).  Returns the .inp file
+    entries (pos, z, e, wt, jk), "0" for the files not written."""
     os.makedirs(os.path.dirname(prefix) or ".", exist_ok=True)
     np.savetxt(prefix + "_pos.dat", np.column_stack((x, y)))
     np.savetxt(prefix + "_z.dat", np.asarray(z))
@@ -72,6 +73,9 @@ def write_catalog(prefix, x, y, z, e1=None, e2=None, w=None, jk=None):
     if e1 is not None:
         np.savetxt(prefix + "_e.dat", np.column_stack((e1, e2)))
         files[2] = prefix + "_e.dat"
+    elif kappa is not None:
+        np.savetxt(prefix + "_kappa.dat", np.asarray(kappa))
+        files[2] = prefix + "_kappa.dat"
     if w is not None:
         np.savetxt(prefix + "_wt.dat", np.asarray(w))
         files[3] = prefix + "_wt.dat"

@@ -8,7 +8,8 @@ the engine-level (outer, inner, params) of such a call, ready for Engine.count.
 """
 from dataclasses import dataclass
 
-from .engine import (JK_LEAVE_ONE_OUT, KIND_DD, KIND_SHAPE_DENSITY, KIND_SHAPE_SHAPE, PREC_SPLIT, make_params)
+from .engine import (JK_LEAVE_ONE_OUT, KIND_DD, KIND_DENSITY_CONVERGENCE, KIND_SHAPE_CONVERGENCE, KIND_SHAPE_DENSITY,
+                     KIND_SHAPE_SHAPE, PREC_SPLIT, make_params)
 
 PASSES = ("SD", "SRd", "RsRd", "SRs", "DRs", "RsRs", "SS")
 
@@ -41,6 +42,10 @@ def pass_kind(which_corr, pass_name):
         return KIND_SHAPE_SHAPE if pass_name == "SS" else KIND_SHAPE_DENSITY
     if which_corr == 2:
         return KIND_SHAPE_SHAPE if pass_name in ("SD", "SS") else KIND_SHAPE_DENSITY
+    if which_corr == 3:
+        return KIND_DENSITY_CONVERGENCE
+    if which_corr == 4:
+        return KIND_SHAPE_CONVERGENCE
     raise ValueError(f"which_corr={which_corr} is outside the pair kinds this engine runs")
 
 
@@ -82,6 +87,15 @@ def setup_pass(pass_name, g1, g2, di: DataInfo, precision=PREC_SPLIT, jk_mode=JK
         # shapes on g1 the reference reads an unset pointer, and such columns feed no estimator
         gws = 1
         if outer.e1 is None:
+            kind = KIND_DD
+    if kind in (KIND_DENSITY_CONVERGENCE, KIND_SHAPE_CONVERGENCE):
+        if gws == 0:
+            return outer, inner, params(kind, 0), swapped  # assigns nothing: zero values (corels.cpp:79-120)
+        # kappa (e1) from the g_with_shape side (density-convergence) or the other side (shape-
+        # convergence); a catalog without it is a reference read of an unset pointer: density pairs
+        ws, ot = (inner, outer) if gws == 2 else (outer, inner)
+        kap = ws if kind == KIND_DENSITY_CONVERGENCE else ot
+        if kap.e1 is None or (kind == KIND_SHAPE_CONVERGENCE and (ws.e1 is None or getattr(ws, "kappa", False))):
             kind = KIND_DD
     return outer, inner, params(kind, gws if gws else 2), swapped
 

@@ -45,6 +45,11 @@ typedef enum cpc_status {
 #define CPC_KIND_DD 0            /* density_density_corel, corels.cpp:48-77 */
 #define CPC_KIND_SHAPE_DENSITY 1 /* shape_density_corel,   corels.cpp:206-244 */
 #define CPC_KIND_SHAPE_SHAPE 2   /* shape_shape_corel,     corels.cpp:122-204 */
+/* convergence kinds (which_corr 3 and 4): the convergence kappa travels in e1 (the reference's e[0]);
+ * a catalog that carries only kappa passes e2 as zeros.  g_with_shape may be 0 for these kinds: the
+ * reference then assigns nothing and adds the zero values its per-pass state starts with. */
+#define CPC_KIND_DENSITY_CONVERGENCE 3 /* density_convergence_corel, corels.cpp:79-91 */
+#define CPC_KIND_SHAPE_CONVERGENCE 4   /* shape_convergence_corel,   corels.cpp:93-120 */
 
 /* arithmetic of the pair kernel */
 #define CPC_PREC_SPLIT 0 /* FP32 classification, FP64 re-evaluation inside a guard band */

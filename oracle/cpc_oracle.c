@@ -12,7 +12,7 @@
  *   pi and rp of a pair                                PB_calcs.cpp:87-98; calcs.cpp:126-132;
  *                                                      sky_calcs.cpp:24-31, 91-96 (dA_calc, rp_calc_sky)
  *   bin indices, pair weight                           corels.cpp:21-46 (corel_bin)
- *   the three pair kinds                               corels.cpp:48-77, 122-204, 206-244
+ *   the pair kinds                                     corels.cpp:48-77, 79-120, 122-204, 206-244
  *   orientation angle, rotation, ephi                  calcs.cpp:54-64, 97-105; sky_calcs.cpp:13-22
  *   jackknife touch slots and leave-one-out            jk.cpp:53-81, 138-171, 201-223; outp.cpp:242-288
  * in FP64 with the host libm (log10, sqrt, atan, sin, cos, acos, floor) and no FMA contraction, like
@@ -125,7 +125,29 @@ typedef struct {
  * leaves the per-thread ep/ex/epp/exx/epx/etheta that a kind never assigns. */
 static inline void kind_values(const oparm *P, const cpc_catalog_desc *g1, int64_t i, const cpc_catalog_desc *g2,
                                int64_t j, double wt, double theta, double *v, double *e) {
-  if (P->kind == CPC_KIND_SHAPE_DENSITY) {
+  const double sig_crit = 1.0; /* calc_temp_ini, initialization.cpp:596 */
+  if (P->kind == CPC_KIND_DENSITY_CONVERGENCE) {
+    /* density_convergence_corel, corels.cpp:79-91: kappa sits in e[0]; no err */
+    double ep = 0;
+    if (P->gws == 1)
+      ep = wt * g1->e1[i] * sig_crit;
+    else if (P->gws == 2)
+      ep = wt * g2->e1[j] * sig_crit;
+    v[1] = ep;
+  } else if (P->kind == CPC_KIND_SHAPE_CONVERGENCE) {
+    /* shape_convergence_corel, corels.cpp:93-120: rotate the g_with_shape side, kappa from the other */
+    double ep = 0, r0 = 0, r1 = 0;
+    if (P->gws == 2) {
+      rotate_(g2->e1[j], g2->e2[j], theta, &r0, &r1);
+      ep = wt * g1->e1[i] * sig_crit;
+    } else if (P->gws == 1) {
+      rotate_(g1->e1[i], g1->e2[i], theta, &r0, &r1);
+      ep = wt * g2->e1[j] * sig_crit;
+    }
+    v[1] = ep;
+    v[3] = ep * r0;
+    v[5] = ep * r1;
+  } else if (P->kind == CPC_KIND_SHAPE_DENSITY) {
     /* shape_density_corel, corels.cpp:206-244 */
     double ep, ex;
     if (P->gws == 2)
@@ -376,6 +398,12 @@ int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_
     if (s->n && (!s->e1 || !s->e2)) return CPC_ERR_INVALID;
   }
   if (P.kind == CPC_KIND_SHAPE_SHAPE && ((n1 && !g1->e1) || (n2 && !g2->e1))) return CPC_ERR_INVALID;
+  if ((P.kind == CPC_KIND_DENSITY_CONVERGENCE || P.kind == CPC_KIND_SHAPE_CONVERGENCE) && P.gws != 0) {
+    const cpc_catalog_desc *ws = P.gws == 2 ? g2 : g1, *ot = P.gws == 2 ? g1 : g2;
+    const cpc_catalog_desc *kap = P.kind == CPC_KIND_DENSITY_CONVERGENCE ? ws : ot;
+    if (kap->n && !kap->e1) return CPC_ERR_INVALID;
+    if (P.kind == CPC_KIND_SHAPE_CONVERGENCE && ws->n && (!ws->e1 || !ws->e2)) return CPC_ERR_INVALID;
+  }
   if (n_threads <= 0) n_threads = omp_get_num_procs();
   double t0 = now_ms();
 

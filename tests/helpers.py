@@ -31,8 +31,10 @@ def _get(h, name):
     return h[name] if isinstance(h, dict) else getattr(h, name)
 
 
-def assert_hist_match(got, ref, rtol=RTOL, what=""):
-    """Bit-identical integer counts; weighted sums within rtol of the bin's accumulated magnitude."""
+def assert_hist_match(got, ref, rtol=RTOL, what="", mag=None):
+    """Bit-identical integer counts; weighted sums within rtol of the bin's accumulated magnitude.
+    That magnitude is sqrt(num * err) of the full sample for the fields that carry err sums; kinds
+    without them pass `mag` ([rp, pi, field], the full sample's sum of |terms|) instead."""
     gn, rn = _get(got, "num"), _get(ref, "num")
     if not np.array_equal(gn, rn):
         bad = np.argwhere(gn != rn)
@@ -46,7 +48,8 @@ def assert_hist_match(got, ref, rtol=RTOL, what=""):
     assert np.all(np.abs(gw - rw) <= rtol * scale_w + 1e-300), f"{what}: wt_num differs beyond {rtol}"
     gv, rv = _get(got, "val"), _get(ref, "val")
     ge, re_ = _get(got, "err"), _get(ref, "err")
-    mag = np.sqrt(num0[..., None] * np.abs(re_[0]))  # [rp, pi, field]
+    if mag is None:
+        mag = np.sqrt(num0[..., None] * np.abs(re_[0]))  # [rp, pi, field]
     scale_v = np.abs(rv) + mag[None]
     dv = np.abs(gv - rv)
     if not np.all(dv <= rtol * scale_v + 1e-300):

@@ -70,9 +70,10 @@ def build_catalogs(inp, cols, table=None):
         if do_jk == 0 or n_jk == 0:
             jk_prob[:] = 0
         inc = draws[:, -1].copy() if role.endswith("rand") else None
-        with_e = (role == "shape" and which != 0) or (role == "density" and which == 2)
-        e1 = c["e1"][:n] if with_e else None
-        e2 = c["e2"][:n] if with_e else None
+        kap = (which == 3 and role == "shape") or (which == 4 and role == "density")
+        with_e = not kap and ((role == "shape" and which != 0) or (role == "density" and which == 2))
+        e1 = c["e1"][:n] if with_e else (c["kappa"][:n] if kap else None)
+        e2 = c["e2"][:n] if with_e else (np.zeros(n) if kap else None)
         kw = {}
         if not box:
             it = np.floor(c["z"][:n] / table["dz"]).astype(np.int64)

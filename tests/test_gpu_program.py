@@ -62,6 +62,11 @@ CASES = [
     ("sky", 0, 2, PREC_SPLIT, {}),
     ("sky", 2, 4, PREC_SPLIT, {}),
     ("sky", 0, 0, PREC_SPLIT, {"rand_subsample": 1}),
+    # convergence kinds: which_corr 3 (kappa on the shape sample), 4 (kappa on the density sample)
+    ("box", 3, 0, PREC_SPLIT, {}),
+    ("box", 4, 0, PREC_SPLIT, {"n_shape_rand": 0, "n_density_rand": 0}),
+    ("sky", 3, 0, PREC_SPLIT, {}),
+    ("sky", 4, 0, PREC_FP64, {}),
 ]
 
 
@@ -72,7 +77,9 @@ def test_program_matches_restatement(tmp_path, oracle_lib, geom, which, estimato
         path, inp, cols = box_case(tmp_path, which=which, estimator=estimator,
                                    sorted_z=extra.get("data_sorted", 1) == 1, **extra)
     else:
-        n_rd = 1800 if (estimator == 4 or extra.get("rand_subsample")) else 0
+        # which_corr 3-4 never share the shape randoms as density randoms (RsRd_same needs
+        # which_corr <= 2, initialization.cpp:189-192): give them density randoms
+        n_rd = 1800 if (estimator == 4 or extra.get("rand_subsample") or which >= 3) else 0
         path, inp, cols, table = sky_case(tmp_path, which=which, estimator=estimator, n_rd=n_rd, **extra)
     _run_program(path, precision)
     ref_prefix = str(tmp_path / "ref" / "out_")

@@ -49,6 +49,11 @@ def parse_plan(out):
     return cats, passes, info
 
 
+def kappa_role(which, role):
+    """The catalogs whose e file is one column of convergence (read_dat.cpp:36-46, 97-99)."""
+    return (which == 3 and role == "shape") or (which == 4 and role == "density")
+
+
 def box_case(tmp_path, which=2, estimator=0, n_s=1500, n_d=1200, n_rs=2500, n_rd=2000, do_jk=1, jk_side=2, seed=3,
              sorted_z=True, **extra):
     """A small periodic box in the reference's file format: shape and density samples with shapes,
@@ -64,11 +69,13 @@ def box_case(tmp_path, which=2, estimator=0, n_s=1500, n_d=1200, n_rs=2500, n_rd
         jk = synth.jk_grid(pos[:, 0], pos[:, 1], pos[:, 2], L, jk_side)
         c = {"x": pos[:, 0], "y": pos[:, 1], "z": pos[:, 2], "jk": jk,
              "w": rng.lognormal(0, 0.3, n) if with_w else None}
-        if with_e:
+        if with_e and kappa_role(which, role):
+            c["kappa"] = rng.normal(0, 0.05, n)
+        elif with_e:
             c["e1"], c["e2"] = rng.normal(0, 0.25, n), rng.normal(0, 0.25, n)
         cols[role] = c
         files[role] = inputs.write_catalog(str(tmp_path / role), c["x"], c["y"], c["z"], c.get("e1"), c.get("e2"),
-                                           c["w"], jk)
+                                           c["w"], jk, kappa=c.get("kappa"))
     inp = dict(which_corr=which, coordinates=6, estimator=estimator, do_jk=do_jk, out_file=str(tmp_path / "out_"),
                n_jk_regions=jk_side ** 3, bin_r_min=0.5, bin_r_max=12.0, n_bins=6, lin_bin=0, n_p_bin=8, p_min=-16.0,
                p_max=16.0, z_min=0, z_max=L, periodic_box=1, periodic_box_size=L)
@@ -84,18 +91,21 @@ def sky_case(tmp_path, which=1, estimator=0, n_s=1200, n_d=1000, n_rs=2400, n_rd
     rng = np.random.default_rng(seed)
     table = synth.distance_table(0.3, 1e-4, 0.5)
     cols, files = {}, {}
-    for role, n, with_e, with_w in (("shape", n_s, True, True), ("density", n_d, which == 2, True),
+    for role, n, with_e, with_w in (("shape", n_s, True, True), ("density", n_d, which in (2, 4), True),
                                     ("shape_rand", n_rs, False, False), ("density_rand", n_rd, False, False)):
         ra = rng.uniform(10.0, 14.0, n)
         dec = rng.uniform(-2.0, 2.0, n)
         z = np.sort(rng.uniform(0.2, 0.5, n))
         jk = (np.minimum(((ra - 10.0) / 2.0).astype(int), 1) * 2 + np.minimum(((dec + 2.0) / 2.0).astype(int), 1))
         c = {"x": ra, "y": dec, "z": z, "jk": jk, "w": rng.lognormal(0, 0.3, n) if with_w else None}
-        if with_e:
+        if with_e and kappa_role(which, role):
+            c["kappa"] = rng.normal(0, 0.05, n)
+        elif with_e:
             c["e1"], c["e2"] = rng.normal(0, 0.25, n), rng.normal(0, 0.25, n)
         cols[role] = c
         if n:
-            files[role] = inputs.write_catalog(str(tmp_path / role), ra, dec, z, c.get("e1"), c.get("e2"), c["w"], jk)
+            files[role] = inputs.write_catalog(str(tmp_path / role), ra, dec, z, c.get("e1"), c.get("e2"), c["w"], jk,
+                                               kappa=c.get("kappa"))
         else:
             files[role] = ["0"] * 5
     dist = inputs.write_distance_table(str(tmp_path / "dist.dat"), table)
@@ -210,9 +220,9 @@ def test_sky_plan(tmp_path, estimator):
 
 def test_errors(tmp_path):
     path, inp, cols = box_case(tmp_path, which=0)
-    bad = inputs.write_inp(str(tmp_path / "w3.inp"), **dict(inp, which_corr=3))
+    bad = inputs.write_inp(str(tmp_path / "w5.inp"), **dict(inp, which_corr=5))
     r = run_prog(bad, "--plan")
-    assert r.returncode == 1 and "which_corr 3 is not supported" in r.stdout
+    assert r.returncode == 1 and "which_corr 5 is not supported" in r.stdout
     lines = open(path).read().splitlines()
     short = tmp_path / "short.inp"
     short.write_text("\n".join(lines[:-3]) + "\n")
@@ -265,3 +275,21 @@ def test_box_without_randoms_plan(tmp_path):
     assert [p["name"] for p in passes] == ["SD"]
     s = rand_stream(1500 + 1200, seed=program_ref.BOX_SEED)
     assert float(cats["D"]["jk_prob0"]) == s[1500]  # no draws for the empty shape randoms
+
+
+@pytest.mark.parametrize("which", [3, 4])
+def test_convergence_plan(tmp_path, which):
+    """which_corr 3: the shape sample's e file is one column of kappa and every non-random pass is
+    density-convergence; which_corr 4: the density sample carries kappa, the shape sample e, and the
+    passes whose kappa side is a random catalog run as density pairs."""
+    path, inp, cols = box_case(tmp_path, which=which)
+    r = run_prog(path, "--plan")
+    assert r.returncode == 0, r.stdout
+    cats, passes, _ = parse_plan(r.stdout)
+    assert cats["S"]["has_e"] == "1" and cats["D"]["has_e"] == str(int(which == 4))
+    assert float(cats["S"]["phi0"]) == (-10.0 if which == 3 else float(cats["S"]["phi0"]))
+    kinds = {p["name"]: p["kind"] for p in passes}
+    if which == 3:
+        assert kinds == {"SRd": "density_convergence", "RsRd": "dd", "SD": "density_convergence"}
+    else:
+        assert kinds == {"SRd": "dd", "RsRd": "dd", "SD": "shape_convergence"}
