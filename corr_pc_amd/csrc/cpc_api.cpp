@@ -137,6 +137,11 @@ struct Derived {
   double psize, rsize;
 };
 
+// z window of an outer galaxy, [z + lo, z + hi]: the pi range in rp-pi, +- rp_max in r-mu
+// (lim_calc_xyz, PB_calcs.cpp:14-23).
+double window_lo(const cpc_params &p) { return p.r_mu ? -p.rp_max : p.pi_min; }
+double window_hi(const cpc_params &p) { return p.r_mu ? p.rp_max : p.pi_max; }
+
 Derived derive(const cpc_params &p) {
   Derived d;
   d.psize = (p.pi_max - p.pi_min) / (p.n_pi);  // initialization.cpp:305
@@ -451,7 +456,7 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, b
       check(S.d_groups.ensure(S.groups.size()), st, "alloc") || check(S.d_colbeg.ensure(S.colbeg.size()), st, "alloc"))
     return st;
   GatherArgs ga{c->d_x, c->d_y, c->d_z, c->d_w, c->d_e1, c->d_e2, c->d_phi, ni, G.ncx, G.ncy, G.x0, G.y0, G.hx, G.hy,
-                p.box_size, p.pi_min, p.pi_max, p.rp_max, zfirst_plus_L, has_zfirst ? 1 : 0, aflags};
+                p.box_size, window_lo(p), window_hi(p), p.rp_max, zfirst_plus_L, has_zfirst ? 1 : 0, aflags};
   check(launch_gather(ga, ctx->vals_b.p, S.rec.p, S.zs.p, S.idx.p, s), st, "gather");
   if (sky) {
     if (check(S.sr.ensure(ni), st, "alloc") || check(S.win.ensure(ni), st, "alloc")) return st;
@@ -714,7 +719,9 @@ Status validate(const cpc_job &job, Plan &P) {
   if (!(p.rp_max > 0) || p.rp_min > p.rp_max) return {CPC_ERR_INVALID, "Bad Bin limits"};  // initialization.cpp:299-303
   if (!(p.pi_max > p.pi_min)) return {CPC_ERR_INVALID, "pi_max must exceed pi_min"};
   if (!p.lin_bin && !(p.rp_min > 0)) return {CPC_ERR_INVALID, "log rp bins need rp_min > 0"};
-  if (!sky && (2 * p.rp_max >= p.box_size || p.pi_max - p.pi_min >= p.box_size))
+  if (p.r_mu != 0 && p.r_mu != 1) return {CPC_ERR_INVALID, "r_mu must be 0 or 1"};
+  if (sky && p.r_mu) return {CPC_ERR_UNSUPPORTED, "r-mu bins are implemented for the periodic box only"};
+  if (!sky && (2 * p.rp_max >= p.box_size || window_hi(p) - window_lo(p) >= p.box_size))
     return {CPC_ERR_UNSUPPORTED, "the rp and pi windows must be narrower than the box"};
   if (p.kind < CPC_KIND_DD || p.kind > CPC_KIND_SHAPE_CONVERGENCE) return {CPC_ERR_INVALID, "unknown kind"};
   const bool conv = p.kind == CPC_KIND_DENSITY_CONVERGENCE || p.kind == CPC_KIND_SHAPE_CONVERGENCE;
@@ -885,7 +892,7 @@ Status setup(cpc_ctx *ctx, Plan &P) {
       int64_t nxt = -1;
       for (int64_t i = co->n - 1; i >= 0; --i) {
         if (nxt >= 0) {
-          const double z_min = co->h_z[i] + p.pi_min;
+          const double z_min = co->h_z[i] + window_lo(p);
           if (!(co->h_z[nxt] + L * 0 < z_min)) af[i] |= 1;
           if (!(co->h_z[nxt] + L * -1 < z_min)) af[i] |= 2;
         }
@@ -1066,8 +1073,11 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     K.pmax = p.pi_max;
     K.psize = dv.psize;
     K.rmax = p.rp_max;
+    K.wlo = window_lo(p);
+    K.whi = window_hi(p);
+    K.rmu = p.r_mu ? 1 : 0;
     K.h = G.hx;
-    K.zmargin = 1e-7 * (p.box_size + std::max(std::fabs(p.pi_min), std::fabs(p.pi_max)));
+    K.zmargin = 1e-7 * (p.box_size + std::max(std::fabs(K.wlo), std::fabs(K.whi)));
     K.nc = G.ncx;
     K.n_rp = p.n_rp;
     K.n_pi = p.n_pi;
@@ -1254,7 +1264,7 @@ Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
     const Plan &P = *Pp;
     if (P.autop || P.co->n == 0 || P.ci->n == 0) continue;
     SparseArgs sa{P.co->d_z, P.co->d_inc, P.co->n, P.ci->z_sorted ? P.ci->d_z : P.ci->d_zsorted, P.ci->n,
-                  P.p.box_size, P.p.pi_min, P.p.pi_max, P.p.include_prob_max};
+                  P.p.box_size, window_lo(P.p), window_hi(P.p), P.p.include_prob_max};
     check(launch_sparse(sa, ctx->accn.p + P.off_c + 2, s), st, "sparse");
   }
   check(hipStreamSynchronize(s), st, "sync");
@@ -1819,9 +1829,11 @@ cpc_status cpc_box_rr(const cpc_params *p, double *wt_num) {
   if (st != CPC_OK) return st;
   const double total_volume = std::pow(p->box_size, 3);
   for (int i = 0; i < p->n_rp; ++i) {
-    const double area = M_PI * (re[i + 1] * re[i + 1] - re[i] * re[i]);
+    // coordinates 6: a ring times the pi width; coordinates 7: a spherical shell, half of it per unit mu
+    const double area = p->r_mu ? 4. / 3. * M_PI * (re[i + 1] * re[i + 1] * re[i + 1] - re[i] * re[i] * re[i])
+                                : M_PI * (re[i + 1] * re[i + 1] - re[i] * re[i]);
     for (int k = 0; k < p->n_pi; ++k) {
-      const double volume = area * (pe[k + 1] - pe[k]);
+      const double volume = p->r_mu ? area / 2 * (pe[k + 1] - pe[k]) : area * (pe[k + 1] - pe[k]);
       wt_num[(size_t)i * p->n_pi + k] = volume / total_volume;
     }
   }

@@ -193,6 +193,8 @@ struct HotArgs {
   float gz, pioff32, s_out_lo, s_out_hi, s_cert_lo, linv, lrmin, la, lb, gA, gB, gC, gD, inv_psize32;
   int n_pi, n_rp, lin, gws, auto_mode;
   int lbase, lcell;  // first LDS cell (row0 * n_pi) and LDS cells per copy
+  double wlo, whi;   // z window offsets of an outer galaxy
+  int rmu;           // r-mu bins (coordinates 7)
 };
 
 // rp bin of a squared separation against the FP64 thresholds thr[0..n_rp]: bin k iff
@@ -208,22 +210,28 @@ __device__ __forceinline__ int rp_bin64(const HotArgs &P, const double *thr, dou
 }
 
 // The reference's candidate test and binning for outer (x1,y1,z1), inner (x2,y2,z2) seen through
-// image (PBx, PBy, PBz): do_pair_sorted (corr.cpp:67-86) then corel_bin (corels.cpp:21-46).
-// Returns the histogram cell n1*n_pi + n2, or -1.
+// image (PBx, PBy, PBz): do_pair_sorted (corr.cpp:67-86) then corel_bin (corels.cpp:21-46), with
+// r_mu_calc (calcs.cpp:135-140) for r-mu bins.  Returns the histogram cell n1*n_pi + n2, or -1.
 __device__ __forceinline__ int exact_cell(const HotArgs &P, const double *thr, double x1, double y1, double z1,
                                           double x2, double y2, double z2, int PBx, int PBy, int PBz) {
   const double L = P.L;
   const double zz = z2 + L * PBz;
-  const double zmax = z1 + P.pmax, zmin = z1 + P.pmin;
+  const double zmax = z1 + P.whi, zmin = z1 + P.wlo;
   if (zz > zmax || zz < zmin) return -1;
   const double yy = y2 + L * PBy;
   if (yy > y1 + P.rmax || yy < y1 - P.rmax) return -1;
   const double xx = x2 + L * PBx;
   if (xx < x1 - P.rmax || xx > x1 + P.rmax) return -1;
-  const double q2 = floor(((zz - z1) - P.pmin) / P.psize);
-  if (!(q2 >= 0.0 && q2 < (double)P.n_pi)) return -1;
   const double t0 = x1 - xx, t1 = y1 - yy;
-  const double s = t0 * t0 + t1 * t1;
+  double d_P = zz - z1, s = t0 * t0 + t1 * t1;
+  if (P.rmu) {
+    // rp rounded first, then r = sqrt(rp*rp + pi*pi); the thresholds bin sqrt of their argument
+    const double rp = sqrt(s);
+    s = rp * rp + d_P * d_P;
+    d_P /= sqrt(s);  // NaN for r = 0: no bin, as the reference's int conversion gives INT_MIN
+  }
+  const double q2 = floor((d_P - P.pmin) / P.psize);
+  if (!(q2 >= 0.0 && q2 < (double)P.n_pi)) return -1;
   const int k = rp_bin64(P, thr, s);
   if (k < 0) return -1;
   return k * P.n_pi + (int)q2;

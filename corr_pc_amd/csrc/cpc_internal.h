@@ -73,6 +73,9 @@ struct KArgs {
   const PRec *trec;    // tile-side records and their z
   const double *tz;
   double L, pmin, pmax, psize, rmax;
+  // z window of an outer galaxy: [z + wlo, z + whi] = pi range (rp-pi) or +- rmax (r-mu, coordinates 7)
+  double wlo, whi;
+  int rmu;  // r-mu bins: r = sqrt(rp*rp + pi*pi) against the thresholds, mu = pi / r in the pi bins
   double h;            // column width
   double zmargin;      // slack of the tile z-range search
   int nc;              // columns per side

@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
   // reloading them inside the loop, where each reload would also wait for the LDS atomics in flight
   const HotArgs H{P.L,         P.pmin, P.pmax,  P.rmax, P.psize, P.gz, P.pioff32, P.s_out_lo, P.s_out_hi,
                   P.s_cert_lo, P.linv, P.lrmin, P.la, P.lb, P.gA, P.gB, P.gC, P.gD, P.inv_psize32, P.n_pi, P.n_rp,
-                  P.lin, P.gws, P.auto_mode, P.row0 * P.n_pi, (P.n_rp - P.row0) * P.n_pi};
+                  P.lin, P.gws, P.auto_mode, P.row0 * P.n_pi, (P.n_rp - P.row0) * P.n_pi, P.wlo, P.whi, P.rmu};
   const PRec *__restrict__ brec = P.brec;
   const double *__restrict__ bz = P.bz;
   const PRec *__restrict__ trec = P.trec;
@@ -149,11 +149,11 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
           for (int sz = -1; sz <= 1; ++sz) {
             double lo, hi;
             if (!SWAP) {
-              lo = wzlo + H.pmin - H.L * sz - P.zmargin;
-              hi = wzhi + H.pmax - H.L * sz + P.zmargin;
+              lo = wzlo + H.wlo - H.L * sz - P.zmargin;
+              hi = wzhi + H.whi - H.L * sz + P.zmargin;
             } else {
-              lo = wzlo - H.pmax - H.L * sz - P.zmargin;
-              hi = wzhi - H.pmin - H.L * sz + P.zmargin;
+              lo = wzlo - H.whi - H.L * sz - P.zmargin;
+              hi = wzhi - H.wlo - H.L * sz + P.zmargin;
             }
             if (hi < tg.zlo || lo > tg.zhi) continue;
             const int j0 = zsearch<false>(tz, tg.start, tg.end, lo);
@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
                 }
                 int cell = -1;
                 bool unc = ok;
-                if (SPLIT && ok) {
+                if (SPLIT && ok && !H.rmu) {  // r-mu candidates all take the FP64 path
                   const float dx = rl_f(cq.x, l3) + ax;
                   const float dy = rl_f(cq.y, l3) + ay;
                   const float dz = rl_f(cq.z, l3) + az;

@@ -85,14 +85,16 @@ Bins load_region(const cpc_params &p, const cpc_hist *h, int r) {
   return b;
 }
 
-// RR_PB_calc (PB_calcs.cpp:63-85), coordinates 6: the expected fraction of pairs per (rp, pi) bin.
+// RR_PB_calc (PB_calcs.cpp:63-85): the expected fraction of pairs per bin, a ring times the pi width
+// (coordinates 6) or half a spherical shell per unit mu (coordinates 7, the reference's formula).
 void rr_periodic(Bins &rr, const cpc_params &p) {
   const double total_volume = std::pow(p.box_size, 3);
   for (Bin &b : rr) {
-    const double area = M_PI * (b.b_max * b.b_max - b.b_min * b.b_min);
+    const double area = p.r_mu ? 4. / 3. * M_PI * (b.b_max * b.b_max * b.b_max - b.b_min * b.b_min * b.b_min)
+                               : M_PI * (b.b_max * b.b_max - b.b_min * b.b_min);
     b.num = 1;
     for (PBin &q : b.p) {
-      const double volume = area * (q.p_max - q.p_min);
+      const double volume = p.r_mu ? area / 2 * (q.p_max - q.p_min) : area * (q.p_max - q.p_min);
       q.wt_num = volume / total_volume;
       q.num = 1;
     }

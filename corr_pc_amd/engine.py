@@ -49,7 +49,8 @@ class Params(ctypes.Structure):
                 ("rp_max", c_double), ("lin_bin", c_int), ("n_pi", c_int), ("pi_min", c_double),
                 ("pi_max", c_double), ("do_jk", c_int), ("n_jk", c_int), ("include_prob_max", c_double),
                 ("precision", c_int), ("jk_mode", c_int), ("sky_H", _dp), ("sky_dA", _dp), ("sky_n", c_int64),
-                ("sky_dz", c_double), ("z_max", c_double), ("z_sep_max", c_double), ("auto_pairs", c_int)]
+                ("sky_dz", c_double), ("z_max", c_double), ("z_sep_max", c_double), ("auto_pairs", c_int),
+                ("r_mu", c_int)]
 
     def attach_table(self, H, dA, dz, z_max, z_sep_max):
         """Sky geometry: keep references to the distance table arrays and point the struct at them."""
@@ -164,12 +165,14 @@ class Catalog:
 
 def make_params(box_size, n_rp, rp_min, rp_max, n_pi, pi_min, pi_max, kind=KIND_DD, g_with_shape=2,
                 estimator=0, which_corr=None, lin_bin=0, do_jk=0, n_jk=0, include_prob_max=1.0,
-                precision=PREC_SPLIT, jk_mode=JK_LEAVE_ONE_OUT):
+                precision=PREC_SPLIT, jk_mode=JK_LEAVE_ONE_OUT, r_mu=0):
     if which_corr is None:
         which_corr = {KIND_DD: 0, KIND_SHAPE_DENSITY: 1, KIND_SHAPE_SHAPE: 2, KIND_DENSITY_CONVERGENCE: 3,
                       KIND_SHAPE_CONVERGENCE: 4}[kind]
-    return Params(GEOM_BOX, box_size, kind, g_with_shape, estimator, which_corr, n_rp, rp_min, rp_max, lin_bin,
-                  n_pi, pi_min, pi_max, do_jk, n_jk, include_prob_max, precision, jk_mode)
+    p = Params(GEOM_BOX, box_size, kind, g_with_shape, estimator, which_corr, n_rp, rp_min, rp_max, lin_bin,
+               n_pi, pi_min, pi_max, do_jk, n_jk, include_prob_max, precision, jk_mode)
+    p.r_mu = r_mu
+    return p
 
 
 def make_sky_params(table, n_rp, rp_min, rp_max, n_pi, pi_min, pi_max, **kw):

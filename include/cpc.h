@@ -113,6 +113,11 @@ typedef struct cpc_params {
    * the lower galaxy i through images with PBz <= 0 (scan from j = i+1) or from any galaxy through
    * PBz = +1 (scan from j = 0), each scan ending at its first included galaxy outside the z window. */
   int auto_pairs;
+  /* 0: rp-pi bins (coordinates 6).  1: r-mu bins of a periodic box (coordinates 7): the z window is
+   * z_i +- rp_max (lim_calc_xyz, PB_calcs.cpp:14-23), r = sqrt(rp*rp + pi*pi) with rp = sqrt(t0*t0 +
+   * t1*t1) and mu = pi / r (r_mu_calc, calcs.cpp:135-140); rp_* bin r and pi_* bin mu.  The sky
+   * geometry supports 0 only. */
+  int r_mu;
 } cpc_params;
 
 /* Output histograms, caller-allocated host memory, overwritten by cpc_count.

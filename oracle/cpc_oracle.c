@@ -9,7 +9,7 @@
  *                                                      galaxy outside the z window
  *   per-outer window and image flags (box)             PB_calcs.cpp:12-61 (lim_calc_xyz, lim_calc_PB)
  *   per-outer window (sky)                             sky_calcs.cpp:33-81, 105-190 (lim_calc_sky)
- *   pi and rp of a pair                                PB_calcs.cpp:87-98; calcs.cpp:126-132;
+ *   pi and rp of a pair (r and mu, coordinates 7)      PB_calcs.cpp:87-98; calcs.cpp:126-140;
  *                                                      sky_calcs.cpp:24-31, 91-96 (dA_calc, rp_calc_sky)
  *   bin indices, pair weight                           corels.cpp:21-46 (corel_bin)
  *   the pair kinds                                     corels.cpp:48-77, 79-120, 122-204, 206-244
@@ -113,6 +113,7 @@ typedef struct {
   double L, pmin, pmax, psize, rmin, rmax, rsize;
   int n_rp, n_pi, lin, kind, gws, est, which_corr, jk_on;
   int autop; /* corels1 auto pass */
+  int rmu;   /* r-mu bins (coordinates 7) */
   double inc_max;
   /* sky */
   const double *H, *dA;
@@ -249,7 +250,8 @@ static inline void accumulate(const oparm *P, ohist *h, int n1, int n2, double w
   }
 }
 
-/* Box pair that passed the window predicates (pi_calc_PB, rp_calc_PB, corel, corel_jk). */
+/* Box pair that passed the window predicates (pi_calc_PB, rp_calc_PB, r_mu_calc for coordinates 7,
+ * corel, corel_jk). */
 static inline int corel_box(const oparm *P, const cpc_catalog_desc *g1, int64_t i, const cpc_catalog_desc *g2,
                             int64_t j, int PBx, int PBy, int PBz, ohist *h) {
   const double L = P->L;
@@ -257,6 +259,10 @@ static inline int corel_box(const oparm *P, const cpc_catalog_desc *g1, int64_t 
   double t0 = g1->x[i] - (g2->x[j] + L * PBx);
   double t1 = g1->y[i] - (g2->y[j] + L * PBy);
   double da_R = sqrt(t0 * t0 + t1 * t1);
+  if (P->rmu) { /* r_mu_calc, calcs.cpp:135-140 */
+    da_R = sqrt(da_R * da_R + d_P * d_P);
+    d_P /= da_R;
+  }
   int n1, n2;
   if (!bin_pair(P, d_P, da_R, &n1, &n2)) return 0;
   double wt = cw(g1, i) * cw(g2, j);
@@ -365,7 +371,9 @@ static int make_params(const cpc_params *p, oparm *P) {
   P->z_max = p->z_max;
   P->z_sep_max = p->z_sep_max;
   P->autop = p->auto_pairs != 0;
+  P->rmu = p->r_mu != 0;
   if (P->geom == CPC_GEOM_SKY && (!P->H || !P->dA || P->ntab < 2 || !(P->dz > 0))) return CPC_ERR_INVALID;
+  if (P->geom == CPC_GEOM_SKY && P->rmu) return CPC_ERR_UNSUPPORTED;
   return CPC_OK;
 }
 
@@ -453,8 +461,8 @@ int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_
           }
           continue;
         }
-        /* lim_calc_xyz + lim_calc_PB, PB_calcs.cpp:12-61 */
-        double z_max = g1->z[i] + P.pmax, z_min = g1->z[i] + P.pmin;
+        /* lim_calc_xyz + lim_calc_PB, PB_calcs.cpp:12-61: z +- rp_max for r-mu (coordinates 7) */
+        double z_max = g1->z[i] + (P.rmu ? P.rmax : P.pmax), z_min = g1->z[i] + (P.rmu ? -P.rmax : P.pmin);
         double ra_max = g1->x[i] + P.rmax, ra_min = g1->x[i] - P.rmax;
         double dec_max = g1->y[i] + P.rmax, dec_min = g1->y[i] - P.rmax;
         int PBx = 0, PBy = 0, PBz = 0;
