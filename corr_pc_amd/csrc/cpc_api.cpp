@@ -289,11 +289,11 @@ bool sky_window(const SkyTable &T, double pmin, double pmax, double rmax, double
 struct cpc_catalog {
   int64_t n = 0;
   double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_w = nullptr, *d_e1 = nullptr, *d_e2 = nullptr,
-         *d_phi = nullptr, *d_jkp = nullptr, *d_inc = nullptr;
+         *d_phi = nullptr, *d_jkp = nullptr, *d_inc = nullptr, *d_e3 = nullptr;
   int32_t *d_jk = nullptr;
   double *d_zsorted = nullptr;  // every galaxy's z ascending (only when the upload order is not sorted)
   SRec *d_sr = nullptr;         // sky quantities per galaxy (upload order), when DC/DA were given
-  bool z_sorted = true, has_e = false, has_jk = false, has_sky = false;
+  bool z_sorted = true, has_e = false, has_jk = false, has_sky = false, has_e3 = false;
   double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   int32_t jk_lo = 0, jk_hi = -1;
   // host copies: z and include_prob (first-included-galaxy rule); for sky catalogs also what the
@@ -455,7 +455,9 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, b
       check(S.idx.ensure(ni + kRecPad), st, "alloc") ||
       check(S.d_groups.ensure(S.groups.size()), st, "alloc") || check(S.d_colbeg.ensure(S.colbeg.size()), st, "alloc"))
     return st;
-  GatherArgs ga{c->d_x, c->d_y, c->d_z, c->d_w, c->d_e1, c->d_e2, c->d_phi, ni, G.ncx, G.ncy, G.x0, G.y0, G.hx, G.hy,
+  // ED passes carry the third axis component in the record's phi slot (ED never reads phi)
+  const double *phi_src = p.kind == CPC_KIND_ED ? c->d_e3 : c->d_phi;
+  GatherArgs ga{c->d_x, c->d_y, c->d_z, c->d_w, c->d_e1, c->d_e2, phi_src, ni, G.ncx, G.ncy, G.x0, G.y0, G.hx, G.hy,
                 p.box_size, window_lo(p), window_hi(p), p.rp_max, zfirst_plus_L, has_zfirst ? 1 : 0, aflags};
   check(launch_gather(ga, ctx->vals_b.p, S.rec.p, S.zs.p, S.idx.p, s), st, "gather");
   if (sky) {
@@ -695,11 +697,11 @@ struct Plan {
 
 // Pair-kind code of the kernels and the data[] fields each kind fills (corels.cpp:48-244):
 // 0 DD, 1 shape-density, 2 shape-shape estimator 0, 3 shape-shape other estimators,
-// 4 density-convergence, 5 shape-convergence.  kNE: how many of the fields (the first ones) also
+// 4 density-convergence, 5 shape-convergence, 6 ED.  kNE: how many of the fields (the first ones) also
 // carry their err sums.
-const int kNV[6] = {0, 3, 5, 6, 1, 3};
-const int kNE[6] = {0, 3, 5, 6, 0, 0};
-const int kValField[6][6] = {{0}, {1, 2, 6}, {1, 3, 4, 5, 6}, {1, 2, 3, 4, 5, 6}, {1}, {1, 3, 5}};
+const int kNV[7] = {0, 3, 5, 6, 1, 3, 1};
+const int kNE[7] = {0, 3, 5, 6, 0, 0, 1};
+const int kValField[7][6] = {{0}, {1, 2, 6}, {1, 3, 4, 5, 6}, {1, 2, 3, 4, 5, 6}, {1}, {1, 3, 5}, {6}};
 
 // The reference's own preconditions and the engine's, checked before any device work.
 Status validate(const cpc_job &job, Plan &P) {
@@ -723,9 +725,16 @@ Status validate(const cpc_job &job, Plan &P) {
   if (sky && p.r_mu) return {CPC_ERR_UNSUPPORTED, "r-mu bins are implemented for the periodic box only"};
   if (!sky && (2 * p.rp_max >= p.box_size || window_hi(p) - window_lo(p) >= p.box_size))
     return {CPC_ERR_UNSUPPORTED, "the rp and pi windows must be narrower than the box"};
-  if (p.kind < CPC_KIND_DD || p.kind > CPC_KIND_SHAPE_CONVERGENCE) return {CPC_ERR_INVALID, "unknown kind"};
+  if (p.kind < CPC_KIND_DD || p.kind > CPC_KIND_ED) return {CPC_ERR_INVALID, "unknown kind"};
+  if (p.kind == CPC_KIND_ED) {
+    if (sky) return {CPC_ERR_UNSUPPORTED, "the ED kind is defined for the periodic box only (ED_calc_PB)"};
+    const cpc_catalog *ws = p.g_with_shape == 2 ? ci : co;  // g2 if g_with_shape is 2, else g1
+    if (ws->n && !ws->has_e3)
+      return {CPC_ERR_INVALID, "the ED kind needs e1, e2 and e3 on the catalog whose axis it reads (the "
+                               "reference would dereference an unallocated e pointer, PB_calcs.cpp:116-120)"};
+  }
   const bool conv = p.kind == CPC_KIND_DENSITY_CONVERGENCE || p.kind == CPC_KIND_SHAPE_CONVERGENCE;
-  if (p.kind != CPC_KIND_DD && !conv && p.g_with_shape != 1 && p.g_with_shape != 2)
+  if (p.kind != CPC_KIND_DD && p.kind != CPC_KIND_ED && !conv && p.g_with_shape != 1 && p.g_with_shape != 2)
     return {CPC_ERR_INVALID, "g_with_shape must be 1 or 2 for shape kinds"};
   if (conv && (p.g_with_shape < 0 || p.g_with_shape > 2))
     return {CPC_ERR_INVALID, "g_with_shape must be 0, 1 or 2 for the convergence kinds"};
@@ -782,6 +791,8 @@ Status validate(const cpc_job &job, Plan &P) {
     P.kind_code = 1;
   else if (p.kind == CPC_KIND_SHAPE_SHAPE)
     P.kind_code = p.estimator == 0 ? 2 : 3;
+  else if (p.kind == CPC_KIND_ED)
+    P.kind_code = 6;
   else
     P.kind_code = p.kind == CPC_KIND_DENSITY_CONVERGENCE ? 4 : 5;
   P.NV = kNV[P.kind_code];
@@ -1668,6 +1679,7 @@ cpc_status cpc_catalog_upload(cpc_ctx *ctx, const cpc_catalog_desc *d, cpc_catal
   if (d->n > 0 && (!d->x || !d->y || !d->z)) return fail(ctx, CPC_ERR_INVALID, "x, y and z are required");
   if ((d->e1 == nullptr) != (d->e2 == nullptr)) return fail(ctx, CPC_ERR_INVALID, "e1 and e2 must both be given");
   if ((d->dc == nullptr) != (d->da == nullptr)) return fail(ctx, CPC_ERR_INVALID, "dc and da must both be given");
+  if (d->e3 && !d->e1) return fail(ctx, CPC_ERR_INVALID, "e3 needs e1 and e2");
   if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, CPC_ERR_DEVICE, "hipSetDevice failed");
   const int64_t n = d->n;
   cpc_catalog *c = new cpc_catalog();
@@ -1675,6 +1687,7 @@ cpc_status cpc_catalog_upload(cpc_ctx *ctx, const cpc_catalog_desc *d, cpc_catal
   c->has_e = d->e1 != nullptr;
   c->has_jk = d->jk != nullptr;
   c->has_sky = d->dc != nullptr;
+  c->has_e3 = d->e3 != nullptr;
   Status st;
   auto up = [&](const double *src, double **dst) {
     if (!src || n == 0) return;
@@ -1688,6 +1701,7 @@ cpc_status cpc_catalog_upload(cpc_ctx *ctx, const cpc_catalog_desc *d, cpc_catal
   up(d->e1, &c->d_e1);
   up(d->e2, &c->d_e2);
   up(d->phi, &c->d_phi);
+  up(d->e3, &c->d_e3);
   up(d->jk_prob, &c->d_jkp);
   up(d->include_prob, &c->d_inc);
   if (d->jk && n > 0) {
@@ -1767,7 +1781,8 @@ cpc_status cpc_catalog_upload(cpc_ctx *ctx, const cpc_catalog_desc *d, cpc_catal
 void cpc_catalog_free(cpc_ctx *ctx, cpc_catalog *c) {
   if (!c) return;
   if (ctx) (void)hipSetDevice(ctx->device);
-  double *bufs[] = {c->d_x, c->d_y, c->d_z, c->d_w, c->d_e1, c->d_e2, c->d_phi, c->d_jkp, c->d_inc, c->d_zsorted};
+  double *bufs[] = {c->d_x,   c->d_y,   c->d_z,   c->d_w,       c->d_e1, c->d_e2,
+                    c->d_phi, c->d_jkp, c->d_inc, c->d_zsorted, c->d_e3};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (c->d_jk) (void)hipFree(c->d_jk);

@@ -8,7 +8,7 @@
 
 namespace cpc {
 
-enum { kDD = 0, kSD = 1, kSS0 = 2, kSS1 = 3, kDK = 4, kSK = 5 };
+enum { kDD = 0, kSD = 1, kSS0 = 2, kSS1 = 3, kDK = 4, kSK = 5, kED = 6 };
 
 // Per pair kind: NV data[] values accumulated, and NE of them with their err sums (the first NE).
 template <int K>
@@ -25,6 +25,23 @@ template <>
 struct KindNV<kDK> { static constexpr int NV = 1, NE = 0; };  // data[1]; no err (corels.cpp:79-91)
 template <>
 struct KindNV<kSK> { static constexpr int NV = 3, NE = 0; };  // data[1], data[3], data[5]; no err (:93-120)
+template <>
+struct KindNV<kED> { static constexpr int NV = 1, NE = 1; };  // data[6] (corels.cpp:246-262)
+
+// ED_calc_PB (PB_calcs.cpp:101-134): cos^2 of the angle between the imaged separation
+// (sx, sy, sz) = g1 - (g2 + L PB) and the 3-D axis (e1, e2, e3) of g2 if g_with_shape is 2, else of g1;
+// the axis travels with e3 in the record's phi slot.  q = [wt, cos^2, cos^4].
+__device__ __forceinline__ void ed_values(double w1, double w2, double ax, double ay, double az, double sx,
+                                          double sy, double sz, double *q) {
+  const double wt = w1 * w2;
+  const double norm = sqrt(sx * sx + sy * sy + sz * sz);
+  double c = ax * sx + ay * sy + az * sz;
+  c /= norm;
+  const double t = c * c;
+  q[0] = wt;
+  q[1] = t;
+  q[2] = t * t;
+}
 
 // 1/x from the hardware reciprocal estimate and one Newton step: the relative error squares, to
 // below 1e-13 even from a 2^-22 estimate (x finite, != 0).  Used only where the result feeds a

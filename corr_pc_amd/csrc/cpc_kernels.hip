@@ -257,10 +257,20 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
                     ++run_n;
                     const Gal gme{me.x, me.y, me.w, me.e1, me.e2, me.phi};
                     double qv[NQ];
-                    if (!SWAP)
+                    if constexpr (K == kED) {
+                      // g1 = outer, g2 = inner in the reference's roles; PB is g2's image
+                      const double tzv = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(cz), t),
+                                                          __builtin_amdgcn_readlane(__double2loint(cz), t));
+                      const Gal &g1 = SWAP ? gt : gme, &g2 = SWAP ? gme : gt;
+                      const double z1 = SWAP ? tzv : mz, z2 = SWAP ? mz : tzv;
+                      const Gal &ax = H.gws == 2 ? g2 : g1;
+                      ed_values(g1.w, g2.w, ax.e1, ax.e2, ax.phi, g1.x - (g2.x + H.L * PBx),
+                                g1.y - (g2.y + H.L * PBy), z1 - (z2 + H.L * PBz), qv);
+                    } else if (!SWAP) {
                       pair_values<K, SPLIT>(gme, gt, H.gws, qv);
-                    else
+                    } else {
                       pair_values<K, SPLIT>(gt, gme, H.gws, qv);
+                    }
 #pragma unroll
                     for (int s = 0; s < NS; ++s) run[s] += qv[s];
                   }
@@ -474,6 +484,7 @@ hipError_t launch_pair(int kind, bool split, bool swap, const KArgs *jobs, const
   CPC_DISPATCH(kSS1)
   CPC_DISPATCH(kDK)
   CPC_DISPATCH(kSK)
+  CPC_DISPATCH(kED)
 #undef CPC_DISPATCH
   return hipErrorInvalidValue;
 }

@@ -6,7 +6,7 @@
 // bins2D_*.dat / *.temp file of do_bining, and inp2.info.  Host side only; the pair loops run on the
 // GPU inside cpc_count_batch.
 //
-// Restates, for which_corr 0-4 on a periodic box (periodic_box 1, coordinates 6) or a survey
+// Restates, for which_corr 0-5 on a periodic box (periodic_box 1, coordinates 6 or 7) or a survey
 // (periodic_box 0, coordinates 0), data_sorted 1 (or 0 without auto pairs), do_jk 0 or 1:
 //   main                                   main.cpp:27-82
 //   data_ini: the positional parser and    initialization.cpp:62-350
@@ -163,18 +163,19 @@ bool data_ini(DataInfo &d, const std::string &name) {
   if (in.fail()) return fail("data_ini:: could not read every parameter of " + name);
 
   // modes this engine runs
-  if (d.which_corr < 0 || d.which_corr > 4)
+  if (d.which_corr < 0 || d.which_corr > 5)
     return fail("which_corr " + std::to_string(d.which_corr) +
-                " is not supported (0-2: density and IA pairs, 3-4: convergence pairs)");
+                " is not supported (0-2: density and IA pairs, 3-4: convergence pairs, 5: ED)");
   if (d.periodic_box && d.coordinates != 6 && d.coordinates != 7) {
     std::cout << " wrong coordinates choice for periodic box:" << d.periodic_box_size << " " << d.coordinates
               << std::endl;
     return false;
   }
-  if ((d.periodic_box && d.coordinates != 6) || (!d.periodic_box && d.coordinates != 0))
+  if ((d.periodic_box && d.coordinates != 6 && d.coordinates != 7) || (!d.periodic_box && d.coordinates != 0))
     return fail("coordinates " + std::to_string(d.coordinates) +
                 " is not supported with periodic_box " + std::to_string((int)d.periodic_box) +
-                " (supported: 6 with a periodic box, 0 without)");
+                " (supported: 6 and 7 with a periodic box, 0 without)");
+  if (d.which_corr == 5 && !d.periodic_box) return fail("which_corr 5 (ED) is defined for the periodic box only");
   if (d.periodic_box && d.data_sorted == 2) return fail("data_sorted==2 doesnot work with periodic box.. use 1");
   if (d.data_sorted != 0 && d.data_sorted != 1) return fail("data_sorted must be 0 or 1");
   if (d.do_jk != 0 && d.do_jk != 1) return fail("do_jk must be 0 or 1");
@@ -239,9 +240,12 @@ bool data_ini(DataInfo &d, const std::string &name) {
     if (z_idx >= d.table_rows) return fail("da_Z:: distances file (dA vs Z) ended unexpectedly");
     d.z_sep_max = d.p_max * d.H_z[z_idx] / 300000.0 + 0.001;
     d.z_sep_min = d.p_min * d.H_z[z_idx] / 300000.0 - 0.001;
-  } else {  // coordinates 6
+  } else if (d.coordinates == 6) {
     d.z_sep_max = d.p_max + 1;
     d.z_sep_min = d.p_min - 1;
+  } else {  // coordinates 7
+    d.z_sep_max = d.bin_r_max + 0.1;
+    d.z_sep_min = d.z_sep_max * -1;
   }
   if (d.bin_r_max == 0 || d.bin_r_min > d.bin_r_max) return fail("Bad Bin limits");
   d.p_size = (d.p_max - d.p_min) / (d.n_p_bin);
@@ -320,10 +324,11 @@ bool outp_data_info(const DataInfo &d, const std::string &name) {
 struct Cat {
   std::string name;
   bool read = false;  // false: never read (an empty catalog for every pass)
-  std::vector<double> x, y, z, w, e1, e2, phi, jkp, inc, dc, da;
+  std::vector<double> x, y, z, w, e1, e2, e3, phi, jkp, inc, dc, da;
   std::vector<int32_t> jk;
   bool has_e = false;  // e1 (and e2) carry values: shapes, or a convergence in e1
   bool kappa = false;  // e1 is a convergence kappa read from a one-column file, e2 is zero
+  bool has_e3 = false; // (e1, e2, e3) is a unit 3-D axis (which_corr 5)
   int64_t n() const { return (int64_t)x.size(); }
 };
 
@@ -344,13 +349,14 @@ double elip_angle(double e0, double e1_in) {
   return phi;
 }
 
-// read_data (read_dat.cpp:18-347) for coordinates 0 and 6, which_corr 0-4.
+// read_data (read_dat.cpp:18-347) for coordinates 0, 6 and 7, which_corr 0-5.
 bool read_data(DataInfo &d, Role role, Cat &c) {
   std::string p_f, z_f, e_f, w_f, jk_f;
   int n_gal = 0;
   VarSt *wt = nullptr;
   // what the e file holds, per which_corr (read_dat.cpp:28-68): shapes (e1 e2), or a convergence
-  bool read_e = true, read_kappa = false, read_phi = false, cross_shape_shape = false, read_jk = true;
+  bool read_e = true, read_kappa = false, read_phi = false, read_e3 = false, cross_shape_shape = false,
+       read_jk = true;
   switch (d.which_corr) {
     case 0:
       read_e = false;
@@ -365,6 +371,11 @@ bool read_data(DataInfo &d, Role role, Cat &c) {
     case 4:
       read_e = role == kShape;  // the shape sample has e, the density sample a convergence
       read_kappa = role != kShape;
+      cross_shape_shape = true;
+      break;
+    case 5:  // the shape sample reads 3-D axes; the case falls into case 8 (read_dat.cpp:47-58)
+      read_e = false;
+      read_e3 = role == kShape;
       cross_shape_shape = true;
       break;
     default:
@@ -383,6 +394,7 @@ bool read_data(DataInfo &d, Role role, Cat &c) {
         e_f = d.density_e_file;
       else
         read_e = read_kappa = false;
+      read_e3 = false;
       break;
     case kShapeRand:
       p_f = d.shape_rand_pos_file, z_f = d.shape_rand_z_file, w_f = d.shape_rand_wt_file,
@@ -414,9 +426,12 @@ bool read_data(DataInfo &d, Role role, Cat &c) {
   if (!data_p.is_open()) return fail("initialisation::pos File not open: " + p_f);
   if (!data_z.is_open()) return fail("initialisation::z File not open:" + z_f);
   if (d.do_jk && !data_jk.is_open() && read_jk) return fail("initialisation::jk File not open: " + jk_f);
-  if (read_e || read_kappa) {
+  if (is_rand) read_e3 = false;
+  if (read_e || read_kappa || read_e3) {
     data_e.open(e_f.c_str());
     if (!data_e.is_open()) return fail("initialisation:: e File not open:" + e_f);
+    if (read_e3 && (e_f.find("phi") != std::string::npos || e_f.find("kappa") != std::string::npos))
+      return fail("e file " + e_f + ": a one-column phi / kappa name with the 3-D axes of which_corr 5");
     // the file's name can switch it to one column of phi or of kappa (read_dat.cpp:189-204)
     if (e_f.find("phi") != std::string::npos) read_phi = true, read_e = read_kappa = false;
     if (e_f.find("kappa") != std::string::npos || read_kappa) read_kappa = true, read_e = read_phi = false;
@@ -427,8 +442,9 @@ bool read_data(DataInfo &d, Role role, Cat &c) {
       return fail("e file " + e_f + ": read as one column of kappa, which leaves the shapes of which_corr " +
                   std::to_string(d.which_corr) + " unset");
   }
-  c.has_e = read_e || read_kappa;
+  c.has_e = read_e || read_kappa || read_e3;
   c.kappa = read_kappa;
+  c.has_e3 = read_e3;
   c.x.reserve(n_gal), c.y.reserve(n_gal), c.z.reserve(n_gal), c.w.reserve(n_gal), c.jkp.reserve(n_gal);
   int i = 0;
   while (data_p.good() && data_z.good() && (data_jk.good() || !read_jk) && (data_wt.good() || w_f == "0") &&
@@ -472,6 +488,19 @@ bool read_data(DataInfo &d, Role role, Cat &c) {
       data_e >> k;
       c.e1.push_back(k);
       c.e2.push_back(0.0);
+    } else if (read_e3) {
+      // three components, scaled to unit length when not zero (read_dat.cpp:283-293)
+      double a = 0, b = 0, e = 0;
+      data_e >> a >> b >> e;
+      const double temp_en = std::sqrt(a * a + b * b + e * e);
+      if (temp_en > 0) {
+        a /= temp_en;
+        b /= temp_en;
+        e /= temp_en;
+      }
+      c.e1.push_back(a);
+      c.e2.push_back(b);
+      c.e3.push_back(e);
     }
     double include_prob = 0;
     if (is_rand) include_prob = rand() / (RAND_MAX + 1.);
@@ -526,6 +555,8 @@ Setup setup_pass(const DataInfo &d, const Pass &ps) {
     s.kind = f == "SS" ? CPC_KIND_SHAPE_SHAPE : CPC_KIND_SHAPE_DENSITY;
   } else if (d.which_corr == 2) {
     s.kind = (f == "SD" || f == "SS") ? CPC_KIND_SHAPE_SHAPE : CPC_KIND_SHAPE_DENSITY;
+  } else if (d.which_corr == 5) {
+    s.kind = CPC_KIND_ED;
   } else {
     s.kind = d.which_corr == 3 ? CPC_KIND_DENSITY_CONVERGENCE : CPC_KIND_SHAPE_CONVERGENCE;
   }
@@ -560,6 +591,12 @@ Setup setup_pass(const DataInfo &d, const Pass &ps) {
       s.kind = CPC_KIND_DD;
       s.shapes_undefined = true;
     }
+  }
+  if (s.kind == CPC_KIND_ED && !(s.gws == 2 ? s.inner : s.outer)->has_e3) {
+    // ED_calc_PB reads the axis of g2 (g_with_shape 2) or else of g1; a catalog without axes there
+    // is a reference read of an unset pointer: density pairs instead
+    s.kind = CPC_KIND_DD;
+    s.shapes_undefined = true;
   }
   if ((s.kind == CPC_KIND_DENSITY_CONVERGENCE || s.kind == CPC_KIND_SHAPE_CONVERGENCE) && s.gws != 0) {
     // kappa from the g_with_shape galaxy (density-convergence) or from the other one (shape-
@@ -617,6 +654,7 @@ cpc_params base_params(const DataInfo &d, int precision) {
   p.include_prob_max = 1;
   p.precision = precision;
   p.jk_mode = CPC_JK_LEAVE_ONE_OUT;
+  p.r_mu = d.coordinates == 7 ? 1 : 0;
   if (!d.periodic_box) {
     p.sky_H = d.H_z.data();
     p.sky_dA = d.dA_z.data();
@@ -643,6 +681,7 @@ cpc_catalog_desc describe(const Cat &c) {
   s.include_prob = c.inc.data();
   s.dc = c.dc.data();
   s.da = c.da.data();
+  s.e3 = c.has_e3 ? c.e3.data() : nullptr;
   return s;
 }
 
@@ -652,7 +691,8 @@ const char *kind_name(int k) {
     case CPC_KIND_SHAPE_DENSITY: return "shape_density";
     case CPC_KIND_SHAPE_SHAPE: return "shape_shape";
     case CPC_KIND_DENSITY_CONVERGENCE: return "density_convergence";
-    default: return "shape_convergence";
+    case CPC_KIND_SHAPE_CONVERGENCE: return "shape_convergence";
+    default: return "ed";
   }
 }
 

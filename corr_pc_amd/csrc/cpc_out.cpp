@@ -1,6 +1,6 @@
 // cpc_out.cpp -- the output side of corr_pc (do_bining), on the histograms cpc_count returns.
 //
-// Restates, for which_corr 0-4 and estimators 0-4:
+// Restates, for which_corr 0-5 and estimators 0-4:
 //   final_calc / final_calc_bins / int_calc    bins_calcs.cpp:30-52, 94-256, 331-383
 //   final_calc_auto / re_bin                   bins_calcs.cpp:54-91, 465-527
 //   final_calc_mean_projected (estimator 4)    bins_calcs.cpp:258-329
@@ -105,7 +105,7 @@ struct Norm {
   Wt SD, SR, DR, RR;
 };
 
-// final_calc_bins (bins_calcs.cpp:94-256) for the kinds of which_corr 0-4.
+// final_calc_bins (bins_calcs.cpp:94-256) for the kinds of which_corr 0-5.
 void final_calc_bins(Bins &sd, Bins &sr, Bins &dr, Bins &rr, Bins &fin, const Norm &N, const cpc_params &p,
                      bool periodic) {
   const double SD_wt = N.SD.val, SR_wt = N.SR.val, DR_wt = N.DR.val, RR_wt = N.RR.val;
@@ -134,8 +134,15 @@ void final_calc_bins(Bins &sd, Bins &sr, Bins &dr, Bins &rr, Bins &fin, const No
       for (int j = 1; j < kF; ++j) {
         const double sd_val = sd[i].p[k].val[j];
         const double sr_val = sr[i].p[k].val[j];
-        const double sd_val_norm = (sd_val / rr_wt_num) * (RR_wt / SD_wt);
-        const double sr_val_norm = (sr_val / rr_wt_num) * (RR_wt / SR_wt);
+        double sd_val_norm = 0, sr_val_norm = 0;
+        if (p.which_corr == 5 && j == 6) {
+          // ED: the mean cos^2 over the SD pairs minus its isotropic value 1/3 (bins_calcs.cpp:190-196)
+          if (sd_num > 0) sd_val_norm = (sd_val / sd_wt_num);
+          sd_val_norm -= 1. / 3;
+        } else {
+          sd_val_norm = (sd_val / rr_wt_num) * (RR_wt / SD_wt);
+          sr_val_norm = (sr_val / rr_wt_num) * (RR_wt / SR_wt);
+        }
         double val_f = sd_val_norm - sr_val_norm;
         if (sd_num == 0 || sr_num == 0 || rr_num == 0) {
           val_f = 0;
@@ -503,7 +510,7 @@ void cross_branch(Writer &W, const cpc_params &p, const cpc_norm &n, const cpc_b
 
 const char *check_output_params(const cpc_params &p) {
   if (p.estimator < 0 || p.estimator > 4) return "estimator must be 0-4";
-  if (p.which_corr < 0 || p.which_corr > 4) return "which_corr must be 0-4";
+  if (p.which_corr < 0 || p.which_corr > 5) return "which_corr must be 0-5";
   if (p.n_rp <= 0 || p.n_pi <= 0 || !(p.pi_max > p.pi_min)) return "bad binning";
   if (p.do_jk != 0 && p.do_jk != 1) return "do_jk must be 0 or 1";
   if (p.do_jk == 1 && p.n_jk > 0 && p.jk_mode != CPC_JK_LEAVE_ONE_OUT) return "histograms must be leave-one-out";
@@ -532,7 +539,7 @@ cpc_status cpc_do_bining(const cpc_params *pp, const cpc_norm *norm, const cpc_b
   if (n_files) *n_files = 0;
   if (!pp || !norm || !sets || !out_file) return CPC_ERR_INVALID;
   const cpc_params &p = *pp;
-  if (p.estimator < 0 || p.estimator > 4 || p.which_corr < 0 || p.which_corr > 4) return CPC_ERR_UNSUPPORTED;
+  if (p.estimator < 0 || p.estimator > 4 || p.which_corr < 0 || p.which_corr > 5) return CPC_ERR_UNSUPPORTED;
   if (check_output_params(p)) return CPC_ERR_INVALID;
   const cpc_bining &b = *sets;
   Writer W;
@@ -553,7 +560,7 @@ cpc_status cpc_write_cross(const cpc_params *pp, const cpc_norm *norm, const cpc
   if (!pp || !norm || !sd || !sr || !dr || !rr || !out_file) return CPC_ERR_INVALID;
   const cpc_params &p = *pp;
   if (p.estimator != 0 && p.estimator != 4) return CPC_ERR_UNSUPPORTED;
-  if (p.which_corr < 0 || p.which_corr > 4) return CPC_ERR_UNSUPPORTED;
+  if (p.which_corr < 0 || p.which_corr > 5) return CPC_ERR_UNSUPPORTED;
   if (check_output_params(p)) return CPC_ERR_INVALID;
   cpc_bining b{};
   b.do_cross_corr = 1;

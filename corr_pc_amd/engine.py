@@ -27,6 +27,7 @@ KIND_SHAPE_DENSITY = 1
 KIND_SHAPE_SHAPE = 2
 KIND_DENSITY_CONVERGENCE = 3
 KIND_SHAPE_CONVERGENCE = 4
+KIND_ED = 5
 PREC_SPLIT = 0
 PREC_FP64 = 1
 JK_LEAVE_ONE_OUT = 0
@@ -40,7 +41,8 @@ _lp = POINTER(c_int64)
 
 class CatalogDesc(ctypes.Structure):
     _fields_ = [("n", c_int64), ("x", _dp), ("y", _dp), ("z", _dp), ("w", _dp), ("e1", _dp), ("e2", _dp),
-                ("phi", _dp), ("jk", _ip), ("jk_prob", _dp), ("include_prob", _dp), ("dc", _dp), ("da", _dp)]
+                ("phi", _dp), ("jk", _ip), ("jk_prob", _dp), ("include_prob", _dp), ("dc", _dp), ("da", _dp),
+                ("e3", _dp)]
 
 
 class Params(ctypes.Structure):
@@ -128,17 +130,18 @@ class Catalog:
     include_prob: Optional[np.ndarray] = None
     dc: Optional[np.ndarray] = None
     da: Optional[np.ndarray] = None
+    e3: Optional[np.ndarray] = None  # third component of a 3-D axis (which_corr 5, ED)
     _keep: list = field(default_factory=list, repr=False)
 
     def __post_init__(self):
-        for name in ("x", "y", "z", "w", "e1", "e2", "phi", "jk_prob", "include_prob", "dc", "da"):
+        for name in ("x", "y", "z", "w", "e1", "e2", "phi", "jk_prob", "include_prob", "dc", "da", "e3"):
             v = getattr(self, name)
             if v is not None:
                 setattr(self, name, np.ascontiguousarray(v, dtype=np.float64))
         if self.jk is not None:
             self.jk = np.ascontiguousarray(self.jk, dtype=np.int32)
         n = len(self.x)
-        for name in ("y", "z", "w", "e1", "e2", "phi", "jk", "jk_prob", "include_prob", "dc", "da"):
+        for name in ("y", "z", "w", "e1", "e2", "phi", "jk", "jk_prob", "include_prob", "dc", "da", "e3"):
             v = getattr(self, name)
             if v is not None and len(v) != n:
                 raise ValueError(f"catalog field {name} has length {len(v)} != {n}")
@@ -150,17 +153,25 @@ class Catalog:
     def subset(self, idx):
         """Catalog restricted to rows idx (keeps order)."""
         kw = {}
-        for name in ("x", "y", "z", "w", "e1", "e2", "phi", "jk", "jk_prob", "include_prob", "dc", "da"):
+        for name in ("x", "y", "z", "w", "e1", "e2", "phi", "jk", "jk_prob", "include_prob", "dc", "da", "e3"):
             v = getattr(self, name)
             kw[name] = None if v is None else v[idx]
         return Catalog(**kw)
 
     def desc(self):
+        # fields may have been replaced after construction: pass contiguous buffers of the C types,
+        # kept alive with the catalog
+        self._keep = []
+
         def p(a, t=_dp):
-            return None if a is None else a.ctypes.data_as(t)
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.int32 if t is _ip else np.float64)
+            self._keep.append(a)
+            return a.ctypes.data_as(t)
         return CatalogDesc(self.n, p(self.x), p(self.y), p(self.z), p(self.w), p(self.e1), p(self.e2),
                            p(self.phi), p(self.jk, _ip), p(self.jk_prob), p(self.include_prob), p(self.dc),
-                           p(self.da))
+                           p(self.da), p(self.e3))
 
 
 def make_params(box_size, n_rp, rp_min, rp_max, n_pi, pi_min, pi_max, kind=KIND_DD, g_with_shape=2,
@@ -168,7 +179,7 @@ def make_params(box_size, n_rp, rp_min, rp_max, n_pi, pi_min, pi_max, kind=KIND_
                 precision=PREC_SPLIT, jk_mode=JK_LEAVE_ONE_OUT, r_mu=0):
     if which_corr is None:
         which_corr = {KIND_DD: 0, KIND_SHAPE_DENSITY: 1, KIND_SHAPE_SHAPE: 2, KIND_DENSITY_CONVERGENCE: 3,
-                      KIND_SHAPE_CONVERGENCE: 4}[kind]
+                      KIND_SHAPE_CONVERGENCE: 4, KIND_ED: 5}[kind]
     p = Params(GEOM_BOX, box_size, kind, g_with_shape, estimator, which_corr, n_rp, rp_min, rp_max, lin_bin,
                n_pi, pi_min, pi_max, do_jk, n_jk, include_prob_max, precision, jk_mode)
     p.r_mu = r_mu

@@ -62,16 +62,17 @@ def write_inp(path, **params):
     return path
 
 
-def write_catalog(prefix, x, y, z, e1=None, e2=None, w=None, jk=None, kappa=None):
-    """Write <prefix>_pos.dat, _z.dat and, when given, _e.dat (e1 e2) or _kappa.dat (one column of
-    convergence, which_corr 3-4), _wt.dat, _jk.dat (write_data's layout).  Returns the .inp file
-    entries (pos, z, e, wt, jk), "0" for the files not written."""
+def write_catalog(prefix, x, y, z, e1=None, e2=None, w=None, jk=None, kappa=None, e3=None):
+    """Write <prefix>_pos.dat, _z.dat and, when given, _e.dat (e1 e2, or e1 e2 e3 for the 3-D axes
+    of which_corr 5) or _kappa.dat (one column of convergence, which_corr 3-4), _wt.dat, _jk.dat
+    (write_data's layout).  Returns the .inp file entries (pos, z, e, wt, jk), "0" for the files not
+    written."""
     os.makedirs(os.path.dirname(prefix) or ".", exist_ok=True)
     np.savetxt(prefix + "_pos.dat", np.column_stack((x, y)))
     np.savetxt(prefix + "_z.dat", np.asarray(z))
     files = [prefix + "_pos.dat", prefix + "_z.dat", "0", "0", "0"]
     if e1 is not None:
-        np.savetxt(prefix + "_e.dat", np.column_stack((e1, e2)))
+        np.savetxt(prefix + "_e.dat", np.column_stack((e1, e2) if e3 is None else (e1, e2, e3)))
         files[2] = prefix + "_e.dat"
     elif kappa is not None:
         np.savetxt(prefix + "_kappa.dat", np.asarray(kappa))

@@ -8,8 +8,8 @@ the engine-level (outer, inner, params) of such a call, ready for Engine.count.
 """
 from dataclasses import dataclass
 
-from .engine import (JK_LEAVE_ONE_OUT, KIND_DD, KIND_DENSITY_CONVERGENCE, KIND_SHAPE_CONVERGENCE, KIND_SHAPE_DENSITY,
-                     KIND_SHAPE_SHAPE, PREC_SPLIT, make_params)
+from .engine import (JK_LEAVE_ONE_OUT, KIND_DD, KIND_DENSITY_CONVERGENCE, KIND_ED, KIND_SHAPE_CONVERGENCE,
+                     KIND_SHAPE_DENSITY, KIND_SHAPE_SHAPE, PREC_SPLIT, make_params)
 
 PASSES = ("SD", "SRd", "RsRd", "SRs", "DRs", "RsRs", "SS")
 
@@ -30,6 +30,7 @@ class DataInfo:
     do_jk: int = 1
     n_jk: int = 0
     include_prob_RR: float = 1.0
+    r_mu: int = 0  # coordinates 7: r-mu bins
 
 
 def pass_kind(which_corr, pass_name):
@@ -46,6 +47,8 @@ def pass_kind(which_corr, pass_name):
         return KIND_DENSITY_CONVERGENCE
     if which_corr == 4:
         return KIND_SHAPE_CONVERGENCE
+    if which_corr == 5:
+        return KIND_ED
     raise ValueError(f"which_corr={which_corr} is outside the pair kinds this engine runs")
 
 
@@ -65,7 +68,7 @@ def setup_pass(pass_name, g1, g2, di: DataInfo, precision=PREC_SPLIT, jk_mode=JK
         return make_params(di.box_size, di.n_rp, di.rp_min, di.rp_max, di.n_pi, di.pi_min, di.pi_max, kind=kind,
                            g_with_shape=gws, estimator=di.estimator, which_corr=di.which_corr,
                            lin_bin=di.lin_bin, do_jk=di.do_jk, n_jk=di.n_jk, include_prob_max=inc_max,
-                           precision=precision, jk_mode=jk_mode)
+                           precision=precision, jk_mode=jk_mode, r_mu=di.r_mu)
 
     if auto:  # corels1: one catalog, g_with_shape = 1 (corr.cpp:111)
         if g1.n == 0:
@@ -88,6 +91,8 @@ def setup_pass(pass_name, g1, g2, di: DataInfo, precision=PREC_SPLIT, jk_mode=JK
         gws = 1
         if outer.e1 is None:
             kind = KIND_DD
+    if kind == KIND_ED and (inner if gws == 2 else outer).e3 is None:
+        kind = KIND_DD  # ED_calc_PB reads the axis of g2 (g_with_shape 2) or g1: none there
     if kind in (KIND_DENSITY_CONVERGENCE, KIND_SHAPE_CONVERGENCE):
         if gws == 0:
             return outer, inner, params(kind, 0), swapped  # assigns nothing: zero values (corels.cpp:79-120)

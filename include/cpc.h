@@ -50,6 +50,10 @@ typedef enum cpc_status {
  * reference then assigns nothing and adds the zero values its per-pass state starts with. */
 #define CPC_KIND_DENSITY_CONVERGENCE 3 /* density_convergence_corel, corels.cpp:79-91 */
 #define CPC_KIND_SHAPE_CONVERGENCE 4   /* shape_convergence_corel,   corels.cpp:93-120 */
+/* ED, which_corr 5 (periodic box only): cos^2 of the angle between the pair's imaged 3-D separation
+ * and the 3-D axis (e1, e2, e3) of the g_with_shape galaxy (of g1 unless g_with_shape is 2), into
+ * data[6] with its square as err (ED_corel / ED_calc_PB, corels.cpp:246-262, PB_calcs.cpp:101-134) */
+#define CPC_KIND_ED 5
 
 /* arithmetic of the pair kernel */
 #define CPC_PREC_SPLIT 0 /* FP32 classification, FP64 re-evaluation inside a guard band */
@@ -79,6 +83,9 @@ typedef struct cpc_catalog_desc {
    * reader looked up, gal.DC = dC_z[floor(z/dz)] and gal.DA = dA_z[floor(z/dz)] (read_dat.cpp:249-251) */
   const double *dc;
   const double *da;
+  /* which_corr 5 (ED) only: the third component gal.e[2] of a 3-D axis whose first two are e1 and e2,
+   * normalised to unit length as read_data does (read_dat.cpp:283-293); NULL otherwise */
+  const double *e3;
 } cpc_catalog_desc;
 
 /* The data_info fields the pair loop reads (data_def.h:46-96) plus engine choices. */

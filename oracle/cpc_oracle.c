@@ -12,7 +12,8 @@
  *   pi and rp of a pair (r and mu, coordinates 7)      PB_calcs.cpp:87-98; calcs.cpp:126-140;
  *                                                      sky_calcs.cpp:24-31, 91-96 (dA_calc, rp_calc_sky)
  *   bin indices, pair weight                           corels.cpp:21-46 (corel_bin)
- *   the pair kinds                                     corels.cpp:48-77, 79-120, 122-204, 206-244
+ *   the pair kinds                                     corels.cpp:48-77, 79-120, 122-204, 206-244, 246-262
+ *   ED: cos^2 of axis and separation                   PB_calcs.cpp:101-134
  *   orientation angle, rotation, ephi                  calcs.cpp:54-64, 97-105; sky_calcs.cpp:13-22
  *   jackknife touch slots and leave-one-out            jk.cpp:53-81, 138-171, 201-223; outp.cpp:242-288
  * in FP64 with the host libm (log10, sqrt, atan, sin, cos, acos, floor) and no FMA contraction, like
@@ -267,7 +268,22 @@ static inline int corel_box(const oparm *P, const cpc_catalog_desc *g1, int64_t 
   if (!bin_pair(P, d_P, da_R, &n1, &n2)) return 0;
   double wt = cw(g1, i) * cw(g2, j);
   double v[NF] = {0}, e[NF] = {0};
-  if (P->kind != CPC_KIND_DD) {
+  if (P->kind == CPC_KIND_ED) {
+    /* ED_corel / ED_calc_PB (corels.cpp:246-262, PB_calcs.cpp:101-134) */
+    double xyz[3];
+    xyz[0] = g1->x[i] - (g2->x[j] + L * PBx);
+    xyz[1] = g1->y[i] - (g2->y[j] + L * PBy);
+    xyz[2] = g1->z[i] - (g2->z[j] + L * PBz);
+    double xyz_norm = sqrt(xyz[0] * xyz[0] + xyz[1] * xyz[1] + xyz[2] * xyz[2]);
+    double cos_ang;
+    if (P->gws == 2)
+      cos_ang = g2->e1[j] * xyz[0] + g2->e2[j] * xyz[1] + g2->e3[j] * xyz[2];
+    else
+      cos_ang = g1->e1[i] * xyz[0] + g1->e2[i] * xyz[1] + g1->e3[i] * xyz[2];
+    cos_ang /= xyz_norm;
+    v[6] = cos_ang * cos_ang;
+    e[6] = pow(v[6], 2.0);
+  } else if (P->kind != CPC_KIND_DD) {
     double theta = atan((g2->y[j] - g1->y[i]) / (g2->x[j] - g1->x[i])); /* calcs.cpp:54-57 */
     kind_values(P, g1, i, g2, j, wt, theta, v, e);
   }
@@ -406,6 +422,11 @@ int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_
     if (s->n && (!s->e1 || !s->e2)) return CPC_ERR_INVALID;
   }
   if (P.kind == CPC_KIND_SHAPE_SHAPE && ((n1 && !g1->e1) || (n2 && !g2->e1))) return CPC_ERR_INVALID;
+  if (P.kind == CPC_KIND_ED) {
+    const cpc_catalog_desc *ws = P.gws == 2 ? g2 : g1;
+    if (sky) return CPC_ERR_UNSUPPORTED; /* ED_calc_PB: the periodic box */
+    if (ws->n && (!ws->e1 || !ws->e2 || !ws->e3)) return CPC_ERR_INVALID;
+  }
   if ((P.kind == CPC_KIND_DENSITY_CONVERGENCE || P.kind == CPC_KIND_SHAPE_CONVERGENCE) && P.gws != 0) {
     const cpc_catalog_desc *ws = P.gws == 2 ? g2 : g1, *ot = P.gws == 2 ? g1 : g2;
     const cpc_catalog_desc *kap = P.kind == CPC_KIND_DENSITY_CONVERGENCE ? ws : ot;

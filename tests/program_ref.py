@@ -71,9 +71,16 @@ def build_catalogs(inp, cols, table=None):
             jk_prob[:] = 0
         inc = draws[:, -1].copy() if role.endswith("rand") else None
         kap = (which == 3 and role == "shape") or (which == 4 and role == "density")
-        with_e = not kap and ((role == "shape" and which != 0) or (role == "density" and which == 2))
+        axis = which == 5 and role == "shape"  # 3-D axes, scaled to unit length (read_dat.cpp:283-293)
+        with_e = not kap and not axis and ((role == "shape" and which != 0) or (role == "density" and which == 2))
         e1 = c["e1"][:n] if with_e else (c["kappa"][:n] if kap else None)
         e2 = c["e2"][:n] if with_e else (np.zeros(n) if kap else None)
+        e3 = None
+        if axis:
+            a, b, e = c["e1"][:n], c["e2"][:n], c["e3"][:n]
+            norm = np.sqrt(a * a + b * b + e * e)
+            pos = norm > 0
+            e1, e2, e3 = (np.where(pos, v / np.where(pos, norm, 1.0), v) for v in (a, b, e))
         kw = {}
         if not box:
             it = np.floor(c["z"][:n] / table["dz"]).astype(np.int64)
@@ -81,7 +88,7 @@ def build_catalogs(inp, cols, table=None):
         jk = c["jk"][:n].astype(np.int32) if (do_jk and c.get("jk") is not None) else np.full(n, -1, np.int32)
         cat = Catalog(x=c["x"][:n], y=c["y"][:n], z=c["z"][:n], w=None if c.get("w") is None else c["w"][:n],
                       e1=e1, e2=e2, phi=elip_angle(e1, e2) if with_e else None, jk=jk, jk_prob=jk_prob,
-                      include_prob=inc, **kw)
+                      include_prob=inc, e3=e3, **kw)
         if int(inp.get("data_sorted", 1)) == 0:
             # the unsorted scan counts every pair inside the window, whatever the order; the oracle
             # scans in z order, so hand it the same galaxies sorted by z
@@ -100,7 +107,7 @@ def run(inp, cols, count, out_prefix, table=None, precision=0):
                   box_size=float(inp["periodic_box_size"]), n_rp=int(inp["n_bins"]), rp_min=float(inp["bin_r_min"]),
                   rp_max=float(inp["bin_r_max"]), n_pi=int(inp["n_p_bin"]), pi_min=float(inp["p_min"]),
                   pi_max=float(inp["p_max"]), lin_bin=int(inp["lin_bin"]), do_jk=int(inp["do_jk"]), n_jk=n_jk,
-                  include_prob_RR=flags["include_prob_RR"])
+                  include_prob_RR=flags["include_prob_RR"], r_mu=int(int(inp.get("coordinates", 6)) == 7))
     empty = Catalog(x=np.zeros(0), y=np.zeros(0), z=np.zeros(0))
     S = cats.get("shape", empty)
     Rs = cats.get("shape_rand", empty)
@@ -164,7 +171,7 @@ def run(inp, cols, count, out_prefix, table=None, precision=0):
     norm = Norm(S_w, D_w, Rs_w, Rd_w, flags["include_prob_RR"])
     po = make_params(di.box_size, di.n_rp, di.rp_min, di.rp_max, di.n_pi, di.pi_min, di.pi_max,
                      estimator=di.estimator, which_corr=di.which_corr, lin_bin=di.lin_bin, do_jk=di.do_jk,
-                     n_jk=di.n_jk)
+                     n_jk=di.n_jk, r_mu=di.r_mu)
     if not box:
         po.geometry = GEOM_SKY
     n_files, pn = do_bining(po, norm, {k: v for k, v in bining.items() if v is not None}, out_prefix,

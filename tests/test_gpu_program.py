@@ -67,6 +67,10 @@ CASES = [
     ("box", 4, 0, PREC_SPLIT, {"n_shape_rand": 0, "n_density_rand": 0}),
     ("sky", 3, 0, PREC_SPLIT, {}),
     ("sky", 4, 0, PREC_FP64, {}),
+    # the ED notebook's box run: which_corr 5 with r-mu bins (coordinates 7), one mu bin in [-1, 1)
+    ("box", 5, 0, PREC_SPLIT, {"coordinates": 7, "n_p_bin": 1, "p_min": -1.0, "p_max": 1.0, "bin_r_max": 10.0}),
+    # r-mu bins of the density and IA kinds
+    ("box", 2, 0, PREC_SPLIT, {"coordinates": 7, "n_p_bin": 4, "p_min": -1.0, "p_max": 1.0}),
 ]
 
 

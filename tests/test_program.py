@@ -71,11 +71,13 @@ def box_case(tmp_path, which=2, estimator=0, n_s=1500, n_d=1200, n_rs=2500, n_rd
              "w": rng.lognormal(0, 0.3, n) if with_w else None}
         if with_e and kappa_role(which, role):
             c["kappa"] = rng.normal(0, 0.05, n)
+        elif with_e and which == 5 and role == "shape":
+            c["e1"], c["e2"], c["e3"] = rng.normal(0, 1, n), rng.normal(0, 1, n), rng.normal(0, 1, n)
         elif with_e:
             c["e1"], c["e2"] = rng.normal(0, 0.25, n), rng.normal(0, 0.25, n)
         cols[role] = c
         files[role] = inputs.write_catalog(str(tmp_path / role), c["x"], c["y"], c["z"], c.get("e1"), c.get("e2"),
-                                           c["w"], jk, kappa=c.get("kappa"))
+                                           c["w"], jk, kappa=c.get("kappa"), e3=c.get("e3"))
     inp = dict(which_corr=which, coordinates=6, estimator=estimator, do_jk=do_jk, out_file=str(tmp_path / "out_"),
                n_jk_regions=jk_side ** 3, bin_r_min=0.5, bin_r_max=12.0, n_bins=6, lin_bin=0, n_p_bin=8, p_min=-16.0,
                p_max=16.0, z_min=0, z_max=L, periodic_box=1, periodic_box_size=L)
@@ -220,9 +222,9 @@ def test_sky_plan(tmp_path, estimator):
 
 def test_errors(tmp_path):
     path, inp, cols = box_case(tmp_path, which=0)
-    bad = inputs.write_inp(str(tmp_path / "w5.inp"), **dict(inp, which_corr=5))
+    bad = inputs.write_inp(str(tmp_path / "w7.inp"), **dict(inp, which_corr=7))
     r = run_prog(bad, "--plan")
-    assert r.returncode == 1 and "which_corr 5 is not supported" in r.stdout
+    assert r.returncode == 1 and "which_corr 7 is not supported" in r.stdout
     lines = open(path).read().splitlines()
     short = tmp_path / "short.inp"
     short.write_text("\n".join(lines[:-3]) + "\n")
