@@ -142,9 +142,28 @@ struct Derived {
 double window_lo(const cpc_params &p) { return p.r_mu ? -p.rp_max : p.pi_min; }
 double window_hi(const cpc_params &p) { return p.r_mu ? p.rp_max : p.pi_max; }
 
+// The sky's pi and rp formulas (choose_rp_pi_func, initialization.cpp:545-571): pi 0 pi_calc, 1
+// pi_calc_projected (0), 2 pi_calc_phi (ephi); rp 0 rp_calc_sky, 1 rp_calc_sky_projected, 2
+// rp_calc_sky_angular; coordinates 1 bins r and mu (r_mu_calc).
+struct SkyMode {
+  int pim, rpm;
+  bool rmu;
+};
+SkyMode sky_mode(const cpc_params &p) {
+  const int c = p.coordinates;
+  const bool proj = p.pi_max > 600 || c == 3 || c == 5;
+  return {(c == 2 || c == 4) ? 2 : (proj ? 1 : 0), (c == 4 || c == 5) ? 2 : (proj ? 1 : 0), c == 1};
+}
+
+// shape_shape_corel's estimator-0 branch (corels.cpp:139): estimator 0, or the sky's coordinates 2 and 4
+bool ss_shape_side_only(const cpc_params &p) {
+  return p.estimator == 0 || (p.geometry == CPC_GEOM_SKY && (p.coordinates == 2 || p.coordinates == 4));
+}
+
 Derived derive(const cpc_params &p) {
   Derived d;
   d.psize = (p.pi_max - p.pi_min) / (p.n_pi);  // initialization.cpp:305
+  if (p.geometry == CPC_GEOM_SKY && (p.coordinates == 3 || p.coordinates == 5)) d.psize = 1;  // :307-308
   if (p.lin_bin)
     d.rsize = (p.rp_max - p.rp_min) / p.n_rp;  // initialization.cpp:310
   else
@@ -256,22 +275,50 @@ double tab_interp(const double *tab, double dz, double z) {
   return tab[i_t] + ((tab[i_t + 1] - tab[i_t]) * (z - dz * i_t) / dz);
 }
 
-// Window of an outer galaxy for coordinates 0; false where the reference exits (z_min > z_max of
-// the data, sky_calcs.cpp:109-113) or would read outside its table.
-bool sky_window(const SkyTable &T, double pmin, double pmax, double rmax, double ra, double dec, double z, Win &w) {
-  double z_max = z + T.z_sep_max;
-  if (!tab_ok(T, z_max) || !tab_ok(T, z)) return false;
-  z_max = z + pmax * tab_interp(T.H, T.dz, z_max) / 300000.0;  // p_to_dz(p_max, z + z_sep_max)
-  double z_min = z + pmin * tab_interp(T.H, T.dz, z) / 300000.0;  // p_to_dz(p_min, z)
-  if (z_min < 0.001) z_min = 0.001;
-  if (z_min > T.z_max || !tab_ok(T, z_min)) return false;
-  const double da_max = rmax / tab_interp(T.dA, T.dz, z_min);  // dAR_to_dA(bin_r_max, z_min)
+// Window of an outer galaxy; false where the reference exits (z_min > z_max of the data,
+// sky_calcs.cpp:109-113) or would read outside its table.  Coordinates 4 and 5 make no z test
+// (corr.cpp:69): their window is unbounded in z.
+bool sky_window(const SkyTable &T, const cpc_params &p, double ra, double dec, double z, Win &w) {
+  const double rmax = p.rp_max;
+  double z_max = z + T.z_sep_max, z_min, da_max;
+  if (p.coordinates >= 2) {
+    // cases 2-5 (sky_calcs.cpp:158-175): the RA/Dec reach at z_min = z, clamped; then the z window
+    z_min = z;
+    if (z_min < 0.001) z_min = 0.001;
+    if (z_min > T.z_max) return false;
+    if (p.coordinates >= 4) {
+      da_max = rmax / kR2D;
+    } else {
+      if (!tab_ok(T, z_min)) return false;
+      da_max = rmax / tab_interp(T.dA, T.dz, z_min);
+    }
+  } else {
+    if (!tab_ok(T, z_max) || !tab_ok(T, z)) return false;
+    if (p.coordinates == 1) {
+      z_max = z + rmax * tab_interp(T.H, T.dz, z_max) / 300000.0;  // case 1: p_to_dz(bin_r_max, z + z_sep_max)
+      z_min = z * 2 - z_max;
+    } else {
+      z_max = z + p.pi_max * tab_interp(T.H, T.dz, z_max) / 300000.0;  // p_to_dz(p_max, z + z_sep_max)
+      z_min = z + p.pi_min * tab_interp(T.H, T.dz, z) / 300000.0;      // p_to_dz(p_min, z)
+    }
+    if (z_min < 0.001) z_min = 0.001;
+    if (z_min > T.z_max || !tab_ok(T, z_min)) return false;
+    da_max = rmax / tab_interp(T.dA, T.dz, z_min);  // dAR_to_dA(bin_r_max, z_min)
+  }
   const double dec_rad = dec / kR2D;
   const double sd = std::sin(dec_rad), cd = std::cos(dec_rad);
   const double ra_lim = std::acos((std::cos(da_max) - sd * sd) / (cd * cd)) * 1.02 * kR2D;
   double ra_max = ra + ra_lim, ra_min = ra - ra_lim;
   if (ra_max > 360) ra_min = 0;
   if (ra_min < 0) ra_max = 360;
+  if (p.coordinates >= 2) {
+    z_min = p.z_sep_min == 0 ? p.z_min : z + p.z_sep_min;
+    z_max = T.z_sep_max == 0 ? T.z_max : z + T.z_sep_max;
+  }
+  if (p.coordinates >= 4) {
+    z_min = -INFINITY;
+    z_max = INFINITY;
+  }
   w.zmin = z_min;
   w.zmax = z_max;
   w.ramin = ra_min;
@@ -549,12 +596,31 @@ double h_or(const std::vector<double> &v, int64_t i, double d) { return v.empty(
 // sky_calcs.cpp:13-31,91-96, jk.cpp:53-57,138-171,201-223.
 void host_sky_pair(const cpc_params &p, const Derived &dv, const cpc_catalog *g1, int64_t i, const cpc_catalog *g2,
                    int64_t j, bool jk_on, HostHist &H) {
-  const double d_P = g2->h_dc[j] - g1->h_dc[i];
+  const SkyMode m = sky_mode(p);
   const double dec1 = g1->h_y[i] / kR2D, dec2 = g2->h_y[j] / kR2D;
   const double ra1 = g1->h_x[i] / kR2D, ra2 = g2->h_x[j] / kR2D;
+  // angle_orient_axis_sky (sky_calcs.cpp:13-22): pi_calc_phi and the shape kinds
+  double theta = 0;
+  if (p.kind != CPC_KIND_DD || m.pim == 2)
+    theta = std::atan((g2->h_y[j] - g1->h_y[i]) / ((g2->h_x[j] - g1->h_x[i]) * std::cos((dec1 + dec2) / 2.0)));
+  auto ephi = [&](double phi) {  // ephi_calc, calcs.cpp:97-105
+    double t = phi - theta;
+    t = t < 0 ? -1 * t : t;
+    if (t > M_PI / 2.0) t = M_PI - t;
+    return t;
+  };
+  double d_P = 0;  // pi_calc_projected
+  if (m.pim == 0)
+    d_P = g2->h_dc[j] - g1->h_dc[i];
+  else if (m.pim == 2)
+    d_P = ephi(p.g_with_shape == 2 ? h_or(g2->h_phi, j, -10.0) : h_or(g1->h_phi, i, -10.0));
   const double sum = std::sin(dec1) * std::sin(dec2) + std::cos(dec1) * std::cos(dec2) * std::cos((ra1 - ra2));
   const double da = (sum > 0.9999999999) ? std::acos(1.0) : std::acos(sum);
-  const double da_R = da * (g1->h_da[i] + g2->h_da[j]) / 2.;
+  double da_R = m.rpm == 0 ? da * (g1->h_da[i] + g2->h_da[j]) / 2. : (m.rpm == 1 ? da * g1->h_da[i] : da * kR2D);
+  if (m.rmu) {  // r_mu_calc, calcs.cpp:135-140
+    da_R = std::sqrt(da_R * da_R + d_P * d_P);
+    d_P /= da_R;
+  }
   const double q2 = std::floor((d_P - p.pi_min) / dv.psize);
   if (!(q2 >= 0 && q2 < p.n_pi)) return;
   const double q1 = p.lin_bin ? std::floor((da_R - p.rp_min) / dv.rsize)
@@ -564,17 +630,9 @@ void host_sky_pair(const cpc_params &p, const Derived &dv, const cpc_catalog *g1
   const double wt = h_or(g1->h_w, i, 1.0) * h_or(g2->h_w, j, 1.0);
   double v[CPC_NFIELD] = {0}, e[CPC_NFIELD] = {0};
   if (p.kind != CPC_KIND_DD) {
-    const double theta =
-        std::atan((g2->h_y[j] - g1->h_y[i]) / ((g2->h_x[j] - g1->h_x[i]) * std::cos((dec1 + dec2) / 2.0)));
     auto rot = [&](double e1, double e2, double &ep, double &ex) {
       ep = std::cos(2.0 * theta) * e1 - std::sin(2.0 * theta) * e2;
       ex = std::sin(2.0 * theta) * e1 + std::cos(2.0 * theta) * e2;
-    };
-    auto ephi = [&](double phi) {
-      double t = phi - theta;
-      t = t < 0 ? -1 * t : t;
-      if (t > M_PI / 2.0) t = M_PI - t;
-      return t;
     };
     if (p.kind == CPC_KIND_DENSITY_CONVERGENCE) {
       // density_convergence_corel (corels.cpp:79-91): kappa in e[0] of the g_with_shape galaxy
@@ -609,7 +667,7 @@ void host_sky_pair(const cpc_params &p, const Derived &dv, const cpc_catalog *g1
       double epi, exi, epj, exj;
       rot(g1->h_e1[i], g1->h_e2[i], epi, exi);
       rot(g2->h_e1[j], g2->h_e2[j], epj, exj);
-      if (p.estimator == 0) {
+      if (ss_shape_side_only(p)) {
         const double ep = p.g_with_shape == 2 ? epj : epi;
         v[1] = 1.0 * wt * ep;
         e[1] = (wt * ep) * (wt * ep);
@@ -722,7 +780,10 @@ Status validate(const cpc_job &job, Plan &P) {
   if (!(p.pi_max > p.pi_min)) return {CPC_ERR_INVALID, "pi_max must exceed pi_min"};
   if (!p.lin_bin && !(p.rp_min > 0)) return {CPC_ERR_INVALID, "log rp bins need rp_min > 0"};
   if (p.r_mu != 0 && p.r_mu != 1) return {CPC_ERR_INVALID, "r_mu must be 0 or 1"};
-  if (sky && p.r_mu) return {CPC_ERR_UNSUPPORTED, "r-mu bins are implemented for the periodic box only"};
+  if (sky && p.r_mu) return {CPC_ERR_INVALID, "on the sky r-mu bins are coordinates 1; r_mu must be 0"};
+  if (!sky && p.coordinates != 0) return {CPC_ERR_INVALID, "coordinates selects the sky formulas: 0 in the box"};
+  if (sky && (p.coordinates < 0 || p.coordinates > 5))
+    return {CPC_ERR_UNSUPPORTED, "sky coordinates must be 0-5 (6 and 7 are the periodic box)"};
   if (!sky && (2 * p.rp_max >= p.box_size || window_hi(p) - window_lo(p) >= p.box_size))
     return {CPC_ERR_UNSUPPORTED, "the rp and pi windows must be narrower than the box"};
   if (p.kind < CPC_KIND_DD || p.kind > CPC_KIND_ED) return {CPC_ERR_INVALID, "unknown kind"};
@@ -790,7 +851,7 @@ Status validate(const cpc_job &job, Plan &P) {
   else if (p.kind == CPC_KIND_SHAPE_DENSITY)
     P.kind_code = 1;
   else if (p.kind == CPC_KIND_SHAPE_SHAPE)
-    P.kind_code = p.estimator == 0 ? 2 : 3;
+    P.kind_code = ss_shape_side_only(p) ? 2 : 3;
   else if (p.kind == CPC_KIND_ED)
     P.kind_code = 6;
   else
@@ -940,7 +1001,7 @@ Status setup(cpc_ctx *ctx, Plan &P) {
         w = Win{1, 0, 0, 0, 0, 0, 0, 0};  // never used: the galaxy is not in the pass
         continue;
       }
-      if (!sky_window(T, p.pi_min, p.pi_max, p.rp_max, co->h_x[i], co->h_y[i], co->h_z[i], w))
+      if (!sky_window(T, p, co->h_x[i], co->h_y[i], co->h_z[i], w))
         return {CPC_ERR_INVALID, "galaxy " + std::to_string(i) +
                                      ": its redshift window leaves the distance table or exceeds z_max "
                                      "(the reference exits here, sky_calcs.cpp:109-113)"};
@@ -1161,6 +1222,11 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     A.nra = G.ncx;
     A.ndec = G.ncy;
     A.dsum = 4e-15;
+    const SkyMode sm = sky_mode(p);
+    A.pim = sm.pim;
+    A.rpm = sm.rpm;
+    A.rmu = sm.rmu ? 1 : 0;
+    A.dphi = 1e-13;
     A.auto_mode = autop ? 1 : 0;
     // Split classification.  The estimate rp^2 = theta^2 dbar^2 (theta^2 = 4 asin^2(c/2) from the
     // chord c of FP64 unit vectors, a series with relative error below 1e-12) differs from the
@@ -1169,7 +1235,7 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     // a bin.  Rounding s to FP32 adds 2^-24.  In bin units: la / ln 2 per unit relative error, plus
     // v_log_f32 (2^-21 max(1, |log2 s|)) times la, the rounding of la and lb, and a margin of 2.
     A.split_on = 0;
-    if (P.split && !p.lin_bin && p.rp_min > 0) {
+    if (P.split && !p.lin_bin && p.rp_min > 0 && sm.pim == 0 && sm.rpm == 0 && !sm.rmu) {
       double da_max = 0;
       for (const cpc_catalog *c : {co, ci})
         for (double v : c->h_da) da_max = std::max(da_max, v);

@@ -140,6 +140,10 @@ struct SkyArgs {
   int reach_nb;
   double reach_z0, reach_inv;
   double dsum;             // bound on the device/host difference of the great-circle cosine sum
+  // the reference's pi and rp of a sky pair (choose_rp_pi_func): pim 0 DC_j - DC_i, 1 zero, 2 ephi of
+  // the g_with_shape galaxy; rpm 0 dA (DA_i + DA_j)/2, 1 dA DA_i, 2 dA in degrees; rmu bins r and mu
+  int pim, rpm, rmu;
+  double dphi;             // bound on the device/host difference of ephi
   // split classification (log rp bins): rp^2 from the chord of the unit vectors, its FP32 log bin
   // estimate kf = log2(s32) la + lb, certain when frac(kf) is at least g from 0 and 1; s32 below
   // s_out_lo or from s_out_hi on is outside every bin; split_on = 0 sends every candidate to the

@@ -43,6 +43,9 @@ __device__ __forceinline__ double cos_poly(double x) {
   return fma(t, p, 1.0);
 }
 
+// r2d of sky_calcs.cpp:10
+constexpr double kSkyR2D = 180.0 / M_PI;
+
 __device__ __forceinline__ int rp_bin_sky(const SkyArgs &P, const double *thr, double rp, double band,
                                           bool *unc) {
   // bin k iff thr[k] <= rp < thr[k+1]; uncertain if rp +- band straddles an edge
@@ -193,7 +196,10 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
             ncand += ok;
             int cell = -1;
             if (ok) {
-              const double q2 = floor(((si.dc - so.dc) - P.pmin) / P.psize);  // pi_calc, corel_bin
+              // pi_calc or pi_calc_projected, corel_bin; ephi (pi_calc_phi) and r-mu bin pi late
+              double dP = P.pim == 0 ? si.dc - so.dc : 0.0;
+              double q2 = floor((dP - P.pmin) / P.psize);
+              const bool late = P.pim == 2 || P.rmu;
               bool exact = true;
               if (SPLIT && P.split_on && q2 >= 0.0 && q2 < (double)P.n_pi) {
                 // rp^2 from the chord between the unit vectors: theta^2 = 4 asin^2(c/2), with
@@ -226,23 +232,50 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
                   }
                 }
               }
-              if (exact && q2 >= 0.0 && q2 < (double)P.n_pi) {
-                // dA_calc: sin d1 sin d2 + cos d1 cos d2 cos(a1 - a2), clamp, acos
+              if (exact && (late || (q2 >= 0.0 && q2 < (double)P.n_pi))) {
+                // dA_calc: sin d1 sin d2 + cos d1 cos d2 cos(a1 - a2), clamp, acos; then rp_calc_sky,
+                // rp_calc_sky_projected (outer DA) or rp_calc_sky_angular (degrees)
                 const double sum = so.sd * si.sd + so.cd * si.cd * cos((so.ra_rad - si.ra_rad));
                 const double dA = (sum > 0.9999999999) ? acos(1.0) : acos(sum);
-                const double rp = dA * (so.da + si.da) / 2.;
+                const double dsc = P.rpm == 0 ? 0.5 * (so.da + si.da) : (P.rpm == 1 ? so.da : kSkyR2D);
+                const double rp = P.rpm == 0 ? dA * (so.da + si.da) / 2. : dA * dsc;
                 const double one_m = fmax(1.0 - sum * sum, 1e-300);
                 const double dda = P.dsum / sqrt(one_m) + 1e-15 * dA;
-                const double band = 1.5 * (0.5 * (so.da + si.da)) * dda + 4e-16 * rp;
-                bool unc;
-                const int k = rp_bin_sky(P, lthr, rp, band, &unc);
-                unc = unc || fabs(sum - 0.9999999999) <= P.dsum;
+                double band = 1.5 * dsc * dda + 4e-16 * rp;
+                double r = rp;
+                bool unc = false;
+                if (late) {
+                  double pb = 0.0;  // bound on the device/host difference of the binned pi
+                  if (P.pim == 2) {
+                    // ephi_calc of the g_with_shape galaxy against angle_orient_axis_sky
+                    const double th = atan((pi.y - po.y) / ((pi.x - po.x) * cos((so.dec_rad + si.dec_rad) / 2.0)));
+                    double t = (P.gws == 2 ? pi.phi : po.phi) - th;
+                    t = t < 0 ? -1 * t : t;
+                    if (t > M_PI / 2.0) t = M_PI - t;
+                    dP = t;
+                    pb = P.dphi;
+                  }
+                  if (P.rmu) {
+                    // r_mu_calc: r = sqrt(rp^2 + pi^2) moves by at most rp's band, mu = pi / r by
+                    // |pi| band / r^2 plus its rounding
+                    r = sqrt(rp * rp + dP * dP);
+                    band += 4e-16 * r;
+                    const double mu = dP / r;
+                    pb = fabs(dP) * band / (r * r) + 4e-16;
+                    dP = mu;
+                  }
+                  q2 = floor((dP - P.pmin) / P.psize);
+                  unc = floor((dP - pb - P.pmin) / P.psize) != floor((dP + pb - P.pmin) / P.psize);
+                }
+                bool unc_r;
+                const int k = rp_bin_sky(P, lthr, r, band, &unc_r);
+                unc = unc || unc_r || fabs(sum - 0.9999999999) <= P.dsum;
                 if (unc) {
                   if (!SWAP) {
                     const unsigned long long slot = atomicAdd(&P.counters[3], 1ull);
                     if (slot < (unsigned long long)P.hq_cap) P.hq[slot] = make_uint2(B.idx[p], T.idx[j]);
                   }
-                } else if (k >= 0) {
+                } else if (k >= 0 && q2 >= 0.0 && q2 < (double)P.n_pi) {
                   cell = k * P.n_pi + (int)q2;
                 }
               }

@@ -52,7 +52,7 @@ class Params(ctypes.Structure):
                 ("pi_max", c_double), ("do_jk", c_int), ("n_jk", c_int), ("include_prob_max", c_double),
                 ("precision", c_int), ("jk_mode", c_int), ("sky_H", _dp), ("sky_dA", _dp), ("sky_n", c_int64),
                 ("sky_dz", c_double), ("z_max", c_double), ("z_sep_max", c_double), ("auto_pairs", c_int),
-                ("r_mu", c_int)]
+                ("r_mu", c_int), ("coordinates", c_int), ("z_min", c_double), ("z_sep_min", c_double)]
 
     def attach_table(self, H, dA, dz, z_max, z_sep_max):
         """Sky geometry: keep references to the distance table arrays and point the struct at them."""
@@ -186,13 +186,16 @@ def make_params(box_size, n_rp, rp_min, rp_max, n_pi, pi_min, pi_max, kind=KIND_
     return p
 
 
-def make_sky_params(table, n_rp, rp_min, rp_max, n_pi, pi_min, pi_max, **kw):
-    """Sky geometry (coordinates 0: RA, Dec in degrees and redshift): `table` is
-    (H, dA, dz, z_max, z_sep_max) as the reference's data_ini derives them (initialization.cpp:223,
-    272-276); the remaining keywords are those of make_params."""
+def make_sky_params(table, n_rp, rp_min, rp_max, n_pi, pi_min, pi_max, coordinates=0, z_min=0.0, z_sep_min=0.0,
+                    **kw):
+    """Sky geometry (RA, Dec in degrees and redshift): `table` is (H, dA, dz, z_max, z_sep_max) as the
+    reference's data_ini derives them (initialization.cpp:223, 272-276); `coordinates` is the
+    reference's sky choice 0-5 (cpc_params.coordinates), `z_min` and `z_sep_min` the data_info values
+    that coordinates 2-5 read; the remaining keywords are those of make_params."""
     p = make_params(0.0, n_rp, rp_min, rp_max, n_pi, pi_min, pi_max, **kw)
     p.geometry = GEOM_SKY
     p.attach_table(*table)
+    p.coordinates, p.z_min, p.z_sep_min = coordinates, z_min, z_sep_min
     return p
 
 

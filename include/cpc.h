@@ -122,9 +122,25 @@ typedef struct cpc_params {
   int auto_pairs;
   /* 0: rp-pi bins (coordinates 6).  1: r-mu bins of a periodic box (coordinates 7): the z window is
    * z_i +- rp_max (lim_calc_xyz, PB_calcs.cpp:14-23), r = sqrt(rp*rp + pi*pi) with rp = sqrt(t0*t0 +
-   * t1*t1) and mu = pi / r (r_mu_calc, calcs.cpp:135-140); rp_* bin r and pi_* bin mu.  The sky
-   * geometry supports 0 only. */
+   * t1*t1) and mu = pi / r (r_mu_calc, calcs.cpp:135-140); rp_* bin r and pi_* bin mu.  Must be 0
+   * on the sky, where coordinates 1 selects r-mu. */
   int r_mu;
+  /* Sky geometry only (0 in the box): data_info.coordinates, which picks the window, pi and rp of a
+   * pair (lim_calc_sky, sky_calcs.cpp:105-190; choose_rp_pi_func, initialization.cpp:545-571):
+   *   0  rp-pi:     pi = DC_j - DC_i, rp = dA (DA_i + DA_j)/2, z window from p_to_dz(p_min/p_max);
+   *   1  r-mu:      as 0, then r = sqrt(rp^2 + pi^2) and mu = pi / r are binned (r_mu_calc), z window
+   *                 z_i +- p_to_dz(rp_max) as lim_calc_sky's case 1 writes it;
+   *   2  rp-ephi:   pi = ephi of the g_with_shape galaxy (pi_calc_phi, calcs.cpp:115-123);
+   *   3  projected: pi = 0 (pi_calc_projected), rp = dA DA_i of the outer galaxy, pi bins of width 1;
+   *   4  angular-ephi: rp = dA in degrees (rp_calc_sky_angular), pi = ephi, no z test (corr.cpp:69);
+   *   5  angular projected: rp in degrees, pi = 0, pi bins of width 1, no z test.
+   * With 2-5 the z window is [z_min or z_i + z_sep_min, z_i + z_sep_max] (sky_calcs.cpp:158-175);
+   * pi_max > 600 with 0 or 2 takes the projected rp, and with 0 also pi = 0 (initialization.cpp:550-554).
+   * With 2 and 4 the shape_shape kind takes its estimator-0 branch whatever `estimator` says
+   * (corels.cpp:139). */
+  int coordinates;
+  double z_min;     /* data_info.z_min: the window's lower edge for coordinates 2-5 when z_sep_min is 0 */
+  double z_sep_min; /* data_info.z_sep_min (coordinates 2-5) */
 } cpc_params;
 
 /* Output histograms, caller-allocated host memory, overwritten by cpc_count.

@@ -114,12 +114,15 @@ typedef struct {
   double L, pmin, pmax, psize, rmin, rmax, rsize;
   int n_rp, n_pi, lin, kind, gws, est, which_corr, jk_on;
   int autop; /* corels1 auto pass */
-  int rmu;   /* r-mu bins (coordinates 7) */
+  int rmu;   /* r-mu bins (coordinates 7; on the sky coordinates 1) */
   double inc_max;
   /* sky */
   const double *H, *dA;
   int64_t ntab;
-  double dz, z_max, z_sep_max;
+  double dz, z_max, z_sep_max, z_min, z_sep_min;
+  int coord; /* data_info.coordinates on the sky, 0-5 */
+  int rpm;   /* rp: 0 rp_calc_sky, 1 rp_calc_sky_projected, 2 rp_calc_sky_angular */
+  int pim;   /* pi: 0 pi_calc, 1 pi_calc_projected, 2 pi_calc_phi */
 } oparm;
 
 /* Shape quantities of a binned pair (corels.cpp:122-244), theta = the orientation angle.
@@ -167,7 +170,7 @@ static inline void kind_values(const oparm *P, const cpc_catalog_desc *g1, int64
     double epi, exi, epj, exj;
     rotate_(g1->e1[i], g1->e2[i], theta, &epi, &exi);
     rotate_(g2->e1[j], g2->e2[j], theta, &epj, &exj);
-    if (P->est == 0) {
+    if (P->est == 0 || P->coord == 2 || P->coord == 4) { /* corels.cpp:139 */
       if (P->gws == 2) {
         v[1] = 1.0 * wt * (epj);
         e[1] = (wt * epj) * (wt * epj);
@@ -294,20 +297,43 @@ static inline int corel_box(const oparm *P, const cpc_catalog_desc *g1, int64_t 
 /* Sky pair that passed the window predicates (pi_calc, rp_calc_sky/dA_calc, corel, corel_jk). */
 static inline int corel_sky(const oparm *P, const cpc_catalog_desc *g1, int64_t i, const cpc_catalog_desc *g2,
                             int64_t j, ohist *h) {
-  double d_P = g2->dc[j] - g1->dc[i];
   const double dec1 = g1->y[i] / R2D, dec2 = g2->y[j] / R2D, ra1 = g1->x[i] / R2D, ra2 = g2->x[j] / R2D;
+  /* angle_orient_axis_sky, sky_calcs.cpp:13-22 (pi_calc_phi and the shape kinds) */
+  double theta = 0;
+  if (P->kind != CPC_KIND_DD || P->pim == 2)
+    theta = atan((g2->y[j] - g1->y[i]) / ((g2->x[j] - g1->x[i]) * cos((dec1 + dec2) / 2.0)));
+  /* ct.pi_calc (calcs.cpp:108-132) */
+  double d_P;
+  if (P->pim == 0) {
+    d_P = g2->dc[j] - g1->dc[i];
+  } else if (P->pim == 1) {
+    d_P = 0;
+  } else {
+    /* ephi_calc of the g_with_shape galaxy (g1 unless it is 2), calcs.cpp:97-105 */
+    double t = (P->gws == 2 ? cphi(g2, j) : cphi(g1, i)) - theta;
+    t = t < 0 ? -1 * t : t;
+    if (t > M_PI / 2.0) t = M_PI - t;
+    d_P = t;
+  }
+  /* ct.rp_calc: dA_calc, then the distance (sky_calcs.cpp:24-31, 84-102) */
   double sum = sin(dec1) * sin(dec2) + cos(dec1) * cos(dec2) * cos((ra1 - ra2));
   double da = (sum > 0.9999999999) ? acos(1.0) : acos(sum);
-  double da_R = da * (g1->da[i] + g2->da[j]) / 2.;
+  double da_R;
+  if (P->rpm == 0)
+    da_R = da * (g1->da[i] + g2->da[j]) / 2.;
+  else if (P->rpm == 1)
+    da_R = da * g1->da[i];
+  else
+    da_R = da * R2D;
+  if (P->rmu) { /* r_mu_calc, calcs.cpp:135-140 */
+    da_R = sqrt(da_R * da_R + d_P * d_P);
+    d_P /= da_R;
+  }
   int n1, n2;
   if (!bin_pair(P, d_P, da_R, &n1, &n2)) return 0;
   double wt = cw(g1, i) * cw(g2, j);
   double v[NF] = {0}, e[NF] = {0};
-  if (P->kind != CPC_KIND_DD) {
-    /* angle_orient_axis_sky, sky_calcs.cpp:13-22 */
-    double theta = atan((g2->y[j] - g1->y[i]) / ((g2->x[j] - g1->x[i]) * cos((dec1 + dec2) / 2.0)));
-    kind_values(P, g1, i, g2, j, wt, theta, v, e);
-  }
+  if (P->kind != CPC_KIND_DD) kind_values(P, g1, i, g2, j, wt, theta, v, e);
   accumulate(P, h, n1, n2, wt, v, e, P->jk_on ? g1->jk[i] : 0, P->jk_on ? g2->jk[j] : 0, cjp(g1, i), cjp(g2, j));
   return 1;
 }
@@ -323,23 +349,50 @@ static inline int tab_ok(const oparm *P, double z) {
   return f >= 0 && f + 1 < (double)P->ntab;
 }
 
-/* lim_calc_sky + lim_calc_ra_dec for coordinates 0 (sky_calcs.cpp:105-190).  Returns 0 on the
- * conditions where the reference exits or reads outside its table. */
+/* lim_calc_sky + lim_calc_ra_dec (sky_calcs.cpp:105-190).  Returns 0 on the conditions where the
+ * reference exits or reads outside its table.  Coordinates 4 and 5 make no z test (corr.cpp:69):
+ * their window is unbounded in z. */
 static int sky_window(const oparm *P, double ra, double dec, double z, double *w) {
-  double z_max = z + P->z_sep_max;
-  if (!tab_ok(P, z_max) || !tab_ok(P, z)) return 0;
-  z_max = z + P->pmax * tab_interp(P->H, P->dz, z_max) / 300000.0;
-  double z_min = z + P->pmin * tab_interp(P->H, P->dz, z) / 300000.0;
-  if (z_min < 0.001) z_min = 0.001;
-  if (z_min > P->z_max) return 0; /* the reference calls exit(1) */
-  if (!tab_ok(P, z_min)) return 0;
-  double da_max = P->rmax / tab_interp(P->dA, P->dz, z_min);
+  double z_max = z + P->z_sep_max, z_min, da_max;
+  if (P->coord >= 2) {
+    /* cases 2-5: the RA/Dec reach is taken at z_min = z (clamped), then the z window */
+    z_min = z;
+    if (z_min < 0.001) z_min = 0.001;
+    if (z_min > P->z_max) return 0;
+    if (P->coord >= 4) {
+      da_max = P->rmax / R2D;
+    } else {
+      if (!tab_ok(P, z_min)) return 0;
+      da_max = P->rmax / tab_interp(P->dA, P->dz, z_min);
+    }
+  } else {
+    if (!tab_ok(P, z_max) || !tab_ok(P, z)) return 0;
+    if (P->coord == 1) { /* case 1 */
+      z_max = z + P->rmax * tab_interp(P->H, P->dz, z_max) / 300000.0;
+      z_min = z * 2 - z_max;
+    } else {
+      z_max = z + P->pmax * tab_interp(P->H, P->dz, z_max) / 300000.0;
+      z_min = z + P->pmin * tab_interp(P->H, P->dz, z) / 300000.0;
+    }
+    if (z_min < 0.001) z_min = 0.001;
+    if (z_min > P->z_max) return 0; /* the reference calls exit(1) */
+    if (!tab_ok(P, z_min)) return 0;
+    da_max = P->rmax / tab_interp(P->dA, P->dz, z_min);
+  }
   double dec_rad = dec / R2D;
   double sd = sin(dec_rad), cd = cos(dec_rad);
   double ra_lim = acos((cos(da_max) - sd * sd) / (cd * cd)) * 1.02 * R2D;
   double ra_max = ra + ra_lim, ra_min = ra - ra_lim;
   if (ra_max > 360) ra_min = 0;
   if (ra_min < 0) ra_max = 360;
+  if (P->coord >= 2) {
+    z_min = P->z_sep_min == 0 ? P->z_min : z + P->z_sep_min;
+    z_max = P->z_sep_max == 0 ? P->z_max : z + P->z_sep_max;
+  }
+  if (P->coord >= 4) {
+    z_min = -INFINITY;
+    z_max = INFINITY;
+  }
   w[0] = z_min;
   w[1] = z_max;
   w[2] = ra_min;
@@ -386,10 +439,21 @@ static int make_params(const cpc_params *p, oparm *P) {
   P->dz = p->sky_dz;
   P->z_max = p->z_max;
   P->z_sep_max = p->z_sep_max;
+  P->z_min = p->z_min;
+  P->z_sep_min = p->z_sep_min;
   P->autop = p->auto_pairs != 0;
   P->rmu = p->r_mu != 0;
+  P->coord = p->coordinates;
   if (P->geom == CPC_GEOM_SKY && (!P->H || !P->dA || P->ntab < 2 || !(P->dz > 0))) return CPC_ERR_INVALID;
-  if (P->geom == CPC_GEOM_SKY && P->rmu) return CPC_ERR_UNSUPPORTED;
+  if (P->geom == CPC_GEOM_SKY && P->rmu) return CPC_ERR_INVALID; /* the sky's r-mu is coordinates 1 */
+  if (P->geom == CPC_GEOM_BOX && P->coord != 0) return CPC_ERR_INVALID;
+  if (P->coord < 0 || P->coord > 5) return CPC_ERR_UNSUPPORTED;
+  /* choose_rp_pi_func (initialization.cpp:545-571), data_ini's p_size (initialization.cpp:305-308) */
+  const int proj = P->pmax > 600 || P->coord == 3 || P->coord == 5;
+  P->rpm = (P->coord == 4 || P->coord == 5) ? 2 : (proj ? 1 : 0);
+  P->pim = (P->coord == 2 || P->coord == 4) ? 2 : (proj ? 1 : 0);
+  if (P->coord == 1) P->rmu = 1;
+  if (P->coord == 3 || P->coord == 5) P->psize = 1;
   return CPC_OK;
 }
 
