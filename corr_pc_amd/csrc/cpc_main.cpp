@@ -171,10 +171,9 @@ bool data_ini(DataInfo &d, const std::string &name) {
               << std::endl;
     return false;
   }
-  if ((d.periodic_box && d.coordinates != 6 && d.coordinates != 7) || (!d.periodic_box && d.coordinates != 0))
+  if (!d.periodic_box && (d.coordinates < 0 || d.coordinates > 5))
     return fail("coordinates " + std::to_string(d.coordinates) +
-                " is not supported with periodic_box " + std::to_string((int)d.periodic_box) +
-                " (supported: 6 and 7 with a periodic box, 0 without)");
+                " is not supported without a periodic box (supported: 0-5 on the sky, 6 and 7 in a box)");
   if (d.which_corr == 5 && !d.periodic_box) return fail("which_corr 5 (ED) is defined for the periodic box only");
   if (d.periodic_box && d.data_sorted == 2) return fail("data_sorted==2 doesnot work with periodic box.. use 1");
   if (d.data_sorted != 0 && d.data_sorted != 1) return fail("data_sorted must be 0 or 1");
@@ -235,20 +234,26 @@ bool data_ini(DataInfo &d, const std::string &name) {
   if (d.z_sep_max == 0) d.z_sep_max = 9;
   if (d.dz == 0 && !d.periodic_box) return fail("dz=0");
   if (!d.periodic_box && !da_Z(d)) return false;
-  if (d.coordinates == 0) {
+  if (d.coordinates == 0 || d.coordinates == 1) {
     const int z_idx = (int)std::ceil(d.z_max / d.dz);
     if (z_idx >= d.table_rows) return fail("da_Z:: distances file (dA vs Z) ended unexpectedly");
-    d.z_sep_max = d.p_max * d.H_z[z_idx] / 300000.0 + 0.001;
-    d.z_sep_min = d.p_min * d.H_z[z_idx] / 300000.0 - 0.001;
+    if (d.coordinates == 0) {
+      d.z_sep_max = d.p_max * d.H_z[z_idx] / 300000.0 + 0.001;
+      d.z_sep_min = d.p_min * d.H_z[z_idx] / 300000.0 - 0.001;
+    } else {
+      d.z_sep_max = d.bin_r_max * d.H_z[z_idx] / 300000.0 + 0.001;
+      d.z_sep_min = -1 * d.z_sep_max;
+    }
   } else if (d.coordinates == 6) {
     d.z_sep_max = d.p_max + 1;
     d.z_sep_min = d.p_min - 1;
-  } else {  // coordinates 7
+  } else if (d.coordinates == 7) {
     d.z_sep_max = d.bin_r_max + 0.1;
     d.z_sep_min = d.z_sep_max * -1;
-  }
+  }  // coordinates 2-5 keep the file's z_sep_min and z_sep_max
   if (d.bin_r_max == 0 || d.bin_r_min > d.bin_r_max) return fail("Bad Bin limits");
   d.p_size = (d.p_max - d.p_min) / (d.n_p_bin);
+  if (d.coordinates == 3 || d.coordinates == 5) d.p_size = 1;  // initialization.cpp:307-308
   if (d.lin_bin)
     d.r_size = (d.bin_r_max - d.bin_r_min) / d.n_bins;
   else
@@ -349,7 +354,10 @@ double elip_angle(double e0, double e1_in) {
   return phi;
 }
 
-// read_data (read_dat.cpp:18-347) for coordinates 0, 6 and 7, which_corr 0-5.
+// Sky coordinates whose pi is the ephi angle of the g_with_shape galaxy (pi_calc_phi).
+bool phi_bins(const DataInfo &d) { return !d.periodic_box && (d.coordinates == 2 || d.coordinates == 4); }
+
+// read_data (read_dat.cpp:18-347) for coordinates 0-7, which_corr 0-5.
 bool read_data(DataInfo &d, Role role, Cat &c) {
   std::string p_f, z_f, e_f, w_f, jk_f;
   int n_gal = 0;
@@ -382,6 +390,11 @@ bool read_data(DataInfo &d, Role role, Cat &c) {
       break;
   }
   const bool is_rand = role == kShapeRand || role == kDensityRand;
+  // coordinates 4 and 5 read no redshift (read_dat.cpp:27-28); with 2 and 4 the shape sample reads
+  // its e file for the angle phi, and random catalogs draw phi (read_dat.cpp:91-92, 127-131, 146-149)
+  const bool read_z = !(d.coordinates == 4 || d.coordinates == 5);
+  const bool rand_phi = phi_bins(d) && (role == kShapeRand || (role == kDensityRand && d.which_corr == 2));
+  if (role == kShape && phi_bins(d)) read_e = true;
   switch (role) {
     case kShape:
       p_f = d.shape_pos_file, z_f = d.shape_z_file, e_f = d.shape_e_file, w_f = d.shape_wt_file,
@@ -410,7 +423,8 @@ bool read_data(DataInfo &d, Role role, Cat &c) {
     std::cout << "n gal zero. Nothing to read: " << p_f << std::endl;
     return true;
   }
-  std::ifstream data_p(p_f.c_str()), data_z(z_f.c_str()), data_e, data_wt, data_jk;
+  std::ifstream data_p(p_f.c_str()), data_z, data_e, data_wt, data_jk;
+  if (read_z) data_z.open(z_f.c_str());
   if (jk_f != "0") {
     data_jk.open(jk_f.c_str());
   } else {
@@ -424,7 +438,7 @@ bool read_data(DataInfo &d, Role role, Cat &c) {
     if (!data_wt.is_open()) return fail("wt file not open: " + w_f);
   }
   if (!data_p.is_open()) return fail("initialisation::pos File not open: " + p_f);
-  if (!data_z.is_open()) return fail("initialisation::z File not open:" + z_f);
+  if (read_z && !data_z.is_open()) return fail("initialisation::z File not open:" + z_f);
   if (d.do_jk && !data_jk.is_open() && read_jk) return fail("initialisation::jk File not open: " + jk_f);
   if (is_rand) read_e3 = false;
   if (read_e || read_kappa || read_e3) {
@@ -447,19 +461,21 @@ bool read_data(DataInfo &d, Role role, Cat &c) {
   c.has_e3 = read_e3;
   c.x.reserve(n_gal), c.y.reserve(n_gal), c.z.reserve(n_gal), c.w.reserve(n_gal), c.jkp.reserve(n_gal);
   int i = 0;
-  while (data_p.good() && data_z.good() && (data_jk.good() || !read_jk) && (data_wt.good() || w_f == "0") &&
-         i < n_gal) {
+  while (data_p.good() && (data_z.good() || !read_z) && (data_jk.good() || !read_jk) &&
+         (data_wt.good() || w_f == "0") && i < n_gal) {
     double ra = 0, dec = 0, redshift = 0;
     data_p >> ra >> dec;
-    data_z >> redshift;
-    if (redshift < 0) {
-      redshift = 0;
-      std::cout << "gal redshift <0. changed to z=0" << std::endl;
-    }
-    if (redshift > d.z_max) {
-      std::cout << "initialisation:: wrong z at data " << i << "     z==" << redshift << "  vs  " << d.z_max
-                << std::endl;
-      return false;
+    if (read_z) {
+      data_z >> redshift;
+      if (redshift < 0) {
+        redshift = 0;
+        std::cout << "gal redshift <0. changed to z=0" << std::endl;
+      }
+      if (redshift > d.z_max) {
+        std::cout << "initialisation:: wrong z at data " << i << "     z==" << redshift << "  vs  " << d.z_max
+                  << std::endl;
+        return false;
+      }
     }
     double w = 1, jk_prob = 1, DC = 0, DA = 0;
     if (d.periodic_box) {
@@ -502,6 +518,7 @@ bool read_data(DataInfo &d, Role role, Cat &c) {
       c.e2.push_back(b);
       c.e3.push_back(e);
     }
+    if (rand_phi) phi = rand() / (RAND_MAX + 1.) * M_PI / 2 - M_PI / 2;
     double include_prob = 0;
     if (is_rand) include_prob = rand() / (RAND_MAX + 1.);
     c.x.push_back(ra);
@@ -555,6 +572,7 @@ Setup setup_pass(const DataInfo &d, const Pass &ps) {
     s.kind = f == "SS" ? CPC_KIND_SHAPE_SHAPE : CPC_KIND_SHAPE_DENSITY;
   } else if (d.which_corr == 2) {
     s.kind = (f == "SD" || f == "SS") ? CPC_KIND_SHAPE_SHAPE : CPC_KIND_SHAPE_DENSITY;
+    if (phi_bins(d)) s.kind = (f == "DRs" || f == "SD") ? CPC_KIND_SHAPE_SHAPE : CPC_KIND_DD;
   } else if (d.which_corr == 5) {
     s.kind = CPC_KIND_ED;
   } else {
@@ -568,9 +586,10 @@ Setup setup_pass(const DataInfo &d, const Pass &ps) {
     return s;
   }
   int n1 = 0, n2 = 0;  // sizes from the role flags, as corels2 takes them
-  if (f == "DRs") n2 = d.n_density, n1 = d.n_shape_rand, s.gws = 0;
-  else if (f == "RsRd") n2 = d.n_shape_rand, n1 = d.n_density_rand, s.gws = 0;
-  else if (f == "RsRs") n1 = n2 = d.n_shape_rand, s.gws = 0;
+  // coordinates 2 and 4 give the random passes a g_with_shape for pi_calc_phi (corr.cpp:226-249)
+  if (f == "DRs") n2 = d.n_density, n1 = d.n_shape_rand, s.gws = phi_bins(d) ? 1 : 0;
+  else if (f == "RsRd") n2 = d.n_shape_rand, n1 = d.n_density_rand, s.gws = phi_bins(d) ? 2 : 0;
+  else if (f == "RsRs") n1 = n2 = d.n_shape_rand, s.gws = phi_bins(d) ? 1 : 0;
   else if (f == "SRd") n2 = d.n_shape, n1 = d.n_density_rand, s.gws = 2;
   else if (f == "SD") n2 = d.n_shape, n1 = d.n_density, s.gws = 2;
   else if (f == "SRs") n2 = d.n_shape, n1 = d.n_shape_rand, s.gws = 2;
@@ -591,6 +610,13 @@ Setup setup_pass(const DataInfo &d, const Pass &ps) {
       s.kind = CPC_KIND_DD;
       s.shapes_undefined = true;
     }
+  }
+  if ((s.kind == CPC_KIND_SHAPE_DENSITY && !(s.gws == 2 ? s.inner : s.outer)->has_e) ||
+      (s.kind == CPC_KIND_SHAPE_SHAPE && (!s.outer->has_e || !s.inner->has_e))) {
+    // a shape kind on a catalog without shapes (coordinates 2 and 4 give such roles to random
+    // catalogs): the reference reads an unset pointer; density pairs instead
+    s.kind = CPC_KIND_DD;
+    s.shapes_undefined = true;
   }
   if (s.kind == CPC_KIND_ED && !(s.gws == 2 ? s.inner : s.outer)->has_e3) {
     // ED_calc_PB reads the axis of g2 (g_with_shape 2) or else of g1; a catalog without axes there
@@ -662,6 +688,9 @@ cpc_params base_params(const DataInfo &d, int precision) {
     p.sky_dz = d.dz;
     p.z_max = d.z_max;
     p.z_sep_max = d.z_sep_max;
+    p.coordinates = d.coordinates;
+    p.z_min = d.z_min;
+    p.z_sep_min = d.z_sep_min;
   }
   return p;
 }
@@ -854,6 +883,17 @@ int run(int argc, char **argv) {
       std::cout << "  " << passes[k].name << ": the shape side carries no shapes; counted as density pairs"
                 << std::endl;
     if (s.empty) continue;  // corels returns 0 pairs (corr.cpp:273-277)
+    if (!d.periodic_box && d.coordinates >= 4 && d.data_sorted == 1 && !passes[k].auto_pass) {
+      // no z test with coordinates 4 and 5, but corels2's sorted start pointer still skips inner
+      // galaxies outside the window; every z is 0, so it passes all of them unless 0 lies inside
+      // (corr.cpp:361-369, sky_calcs.cpp:158-175)
+      const double zlo = d.z_sep_min == 0 ? d.z_min : d.z_sep_min, zhi = d.z_sep_max == 0 ? d.z_max : d.z_sep_max;
+      if (!(0 <= zhi && 0 >= zlo)) {
+        std::cout << "  " << passes[k].name << ": z = 0 lies outside the window [" << zlo << ", " << zhi
+                  << "]: the sorted scan finds no pair" << std::endl;
+        continue;
+      }
+    }
     cpc_params &p = params[k];
     p = base_params(d, opt.precision);
     p.kind = s.kind;

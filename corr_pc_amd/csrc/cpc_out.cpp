@@ -24,6 +24,13 @@ namespace {
 
 constexpr int kF = 7;  // data[] fields: 0 = W, 1 = e+, 2 = ex, 3 = e++, 4 = exx, 5 = e+x, 6 = theta
 
+// data_info.p_size: the pi range over its bins, 1 for the sky's projected coordinates 3 and 5
+// (initialization.cpp:305-308)
+double pi_size(const cpc_params &p) {
+  if (p.geometry == CPC_GEOM_SKY && (p.coordinates == 3 || p.coordinates == 5)) return 1;
+  return (p.pi_max - p.pi_min) / p.n_pi;
+}
+
 struct PBin {
   double p_min = 0, p_max = 0;
   long num = 0;
@@ -162,7 +169,7 @@ void final_calc_bins(Bins &sd, Bins &sr, Bins &dr, Bins &rr, Bins &fin, const No
 
 // int_calc (bins_calcs.cpp:331-358): pi integration of one bin array.
 void int_calc(Bins &dat, const cpc_params &p) {
-  const double dC = (p.pi_max - p.pi_min) / p.n_pi;  // data_inf.p_size
+  const double dC = pi_size(p);  // data_inf.p_size
   for (Bin &b : dat) {
     for (const PBin &q : b.p) {
       b.wt_num += q.wt_num;
@@ -236,7 +243,7 @@ void jk_final(Bins &out, const Bins &fin, const std::vector<Bins> &jk, const cpc
 // of bins whose edges do not mirror to within p_size/100 is reported and left alone, as there.
 void re_bin(Bins &ss, const cpc_params &p) {
   if (p.n_pi <= 1) return;
-  const double p_size = (p.pi_max - p.pi_min) / p.n_pi;
+  const double p_size = pi_size(p);
   for (int i = 0; i < p.n_rp; ++i) {
     std::vector<PBin> &q = ss[i].p;
     for (int k = 0; k < p.n_pi; ++k) {

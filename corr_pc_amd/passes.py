@@ -31,10 +31,16 @@ class DataInfo:
     n_jk: int = 0
     include_prob_RR: float = 1.0
     r_mu: int = 0  # coordinates 7: r-mu bins
+    coordinates: int = 0  # the sky's coordinates 0-5 (0 in the box)
+
+    @property
+    def phi_bins(self):
+        """Coordinates 2 and 4: pi is the ephi angle of the g_with_shape galaxy (pi_calc_phi)."""
+        return self.coordinates in (2, 4)
 
 
-def pass_kind(which_corr, pass_name):
-    """choose_corel_func, initialization.cpp:490-543 (coordinates 0 and 6); pass_name is the do_* flag."""
+def pass_kind(which_corr, pass_name, coordinates=0):
+    """choose_corel_func, initialization.cpp:490-543; pass_name is the do_* flag."""
     if pass_name in ("RsRs", "RsRd"):
         return KIND_DD
     if which_corr == 0:
@@ -42,6 +48,8 @@ def pass_kind(which_corr, pass_name):
     if which_corr == 1:
         return KIND_SHAPE_SHAPE if pass_name == "SS" else KIND_SHAPE_DENSITY
     if which_corr == 2:
+        if coordinates in (2, 4):
+            return KIND_SHAPE_SHAPE if pass_name in ("DRs", "SD") else KIND_DD
         return KIND_SHAPE_SHAPE if pass_name in ("SD", "SS") else KIND_SHAPE_DENSITY
     if which_corr == 3:
         return KIND_DENSITY_CONVERGENCE
@@ -61,7 +69,7 @@ def setup_pass(pass_name, g1, g2, di: DataInfo, precision=PREC_SPLIT, jk_mode=JK
     grand_shape), :115)."""
     if pass_name not in PASSES:
         raise ValueError(pass_name)
-    kind = pass_kind(di.which_corr, pass_name)
+    kind = pass_kind(di.which_corr, pass_name, di.coordinates)
     inc_max = di.include_prob_RR if pass_name in ("RsRs", "RsRd") else 1.0  # initialization.cpp:597-599
 
     def params(kind, gws):
@@ -76,8 +84,11 @@ def setup_pass(pass_name, g1, g2, di: DataInfo, precision=PREC_SPLIT, jk_mode=JK
         p = params(kind, 1)
         p.auto_pairs = 1
         return g1, g1, p, False
-    # operand roles, corr.cpp:226-271: g_with_shape before the swap
+    # operand roles, corr.cpp:226-271: g_with_shape before the swap; coordinates 2 and 4 give the
+    # random passes one for pi_calc_phi
     gws = 2 if pass_name in ("SRd", "SD", "SRs") else 0
+    if di.phi_bins:
+        gws = {"DRs": 1, "RsRd": 2, "RsRs": 1}.get(pass_name, gws)
     n1, n2 = g1.n, g2.n
     if n1 == 0 or n2 == 0:
         return None  # corels2 returns 0 pairs (corr.cpp:273-277)
@@ -91,6 +102,9 @@ def setup_pass(pass_name, g1, g2, di: DataInfo, precision=PREC_SPLIT, jk_mode=JK
         gws = 1
         if outer.e1 is None:
             kind = KIND_DD
+    if (kind == KIND_SHAPE_DENSITY and (inner if gws == 2 else outer).e1 is None) or \
+            (kind == KIND_SHAPE_SHAPE and (outer.e1 is None or inner.e1 is None)):
+        kind = KIND_DD  # a shape kind on a catalog without shapes: a reference read of an unset pointer
     if kind == KIND_ED and (inner if gws == 2 else outer).e3 is None:
         kind = KIND_DD  # ED_calc_PB reads the axis of g2 (g_with_shape 2) or g1: none there
     if kind in (KIND_DENSITY_CONVERGENCE, KIND_SHAPE_CONVERGENCE):

@@ -57,7 +57,12 @@ def build_catalogs(inp, cols, table=None):
     counts = {"shape": int(inp["n_shape"]), "density": int(inp["n_density"]),
               "shape_rand": int(inp["n_shape_rand"]), "density_rand": flags["n_density_rand"]}
     order = read_order(inp, flags)
-    per_gal = {r: (1 if box else 0) + (1 if r.endswith("rand") else 0) for r in order}
+    coords = int(inp.get("coordinates", 6 if box else 0))
+    phi_bins = not box and coords in (2, 4)
+    # random catalogs draw phi with coordinates 2 and 4 (shape randoms always, density randoms for
+    # which_corr 2) before include_prob (read_dat.cpp:127-131, 146-149, 305-318)
+    rand_phi = {r: phi_bins and (r == "shape_rand" or (r == "density_rand" and which == 2)) for r in order}
+    per_gal = {r: (1 if box else 0) + (1 if rand_phi[r] else 0) + (1 if r.endswith("rand") else 0) for r in order}
     total = sum(counts[r] * per_gal[r] for r in order)
     stream = rand_stream(total, seed=BOX_SEED if box else DEFAULT_SEED) if total else np.zeros(0)
     cats, k = {}, 0
@@ -72,7 +77,8 @@ def build_catalogs(inp, cols, table=None):
         inc = draws[:, -1].copy() if role.endswith("rand") else None
         kap = (which == 3 and role == "shape") or (which == 4 and role == "density")
         axis = which == 5 and role == "shape"  # 3-D axes, scaled to unit length (read_dat.cpp:283-293)
-        with_e = not kap and not axis and ((role == "shape" and which != 0) or (role == "density" and which == 2))
+        with_e = not kap and not axis and ((role == "shape" and (which != 0 or phi_bins)) or
+                                           (role == "density" and which == 2))
         e1 = c["e1"][:n] if with_e else (c["kappa"][:n] if kap else None)
         e2 = c["e2"][:n] if with_e else (np.zeros(n) if kap else None)
         e3 = None
@@ -82,12 +88,16 @@ def build_catalogs(inp, cols, table=None):
             pos = norm > 0
             e1, e2, e3 = (np.where(pos, v / np.where(pos, norm, 1.0), v) for v in (a, b, e))
         kw = {}
+        z = c["z"][:n] if coords not in (4, 5) else np.zeros(n)  # 4 and 5 read no redshift
         if not box:
-            it = np.floor(c["z"][:n] / table["dz"]).astype(np.int64)
+            it = np.floor(z / table["dz"]).astype(np.int64)
             kw = {"dc": table["dC"][it], "da": table["dA"][it]}
+        phi = elip_angle(e1, e2) if with_e else None
+        if rand_phi[role]:
+            phi = draws[:, (1 if box else 0)] * np.pi / 2 - np.pi / 2
         jk = c["jk"][:n].astype(np.int32) if (do_jk and c.get("jk") is not None) else np.full(n, -1, np.int32)
-        cat = Catalog(x=c["x"][:n], y=c["y"][:n], z=c["z"][:n], w=None if c.get("w") is None else c["w"][:n],
-                      e1=e1, e2=e2, phi=elip_angle(e1, e2) if with_e else None, jk=jk, jk_prob=jk_prob,
+        cat = Catalog(x=c["x"][:n], y=c["y"][:n], z=z, w=None if c.get("w") is None else c["w"][:n],
+                      e1=e1, e2=e2, phi=phi, jk=jk, jk_prob=jk_prob,
                       include_prob=inc, e3=e3, **kw)
         if int(inp.get("data_sorted", 1)) == 0:
             # the unsorted scan counts every pair inside the window, whatever the order; the oracle
@@ -107,7 +117,8 @@ def run(inp, cols, count, out_prefix, table=None, precision=0):
                   box_size=float(inp["periodic_box_size"]), n_rp=int(inp["n_bins"]), rp_min=float(inp["bin_r_min"]),
                   rp_max=float(inp["bin_r_max"]), n_pi=int(inp["n_p_bin"]), pi_min=float(inp["p_min"]),
                   pi_max=float(inp["p_max"]), lin_bin=int(inp["lin_bin"]), do_jk=int(inp["do_jk"]), n_jk=n_jk,
-                  include_prob_RR=flags["include_prob_RR"], r_mu=int(int(inp.get("coordinates", 6)) == 7))
+                  include_prob_RR=flags["include_prob_RR"], r_mu=int(int(inp.get("coordinates", 6)) == 7),
+                  coordinates=0 if box else int(inp.get("coordinates", 0)))
     empty = Catalog(x=np.zeros(0), y=np.zeros(0), z=np.zeros(0))
     S = cats.get("shape", empty)
     Rs = cats.get("shape_rand", empty)
@@ -130,7 +141,15 @@ def run(inp, cols, count, out_prefix, table=None, precision=0):
     if not box:
         dz, z_max = table["dz"], float(inp["z_max"])
         rows = int(min(len(table["z"]), np.ceil((z_max + 1) / dz) + 1))
-        z_sep_max = float(inp["p_max"]) * table["H"][int(np.ceil(z_max / dz))] / 300000.0 + 0.001
+        # data_ini (initialization.cpp:241-281): from p_max (0) or rp_max (1); the file's value, 9
+        # when 0, otherwise
+        h_top = table["H"][int(np.ceil(z_max / dz))]
+        z_sep_max = float(inp.get("z_sep_max", 0)) or 9.0
+        z_sep_min = float(inp.get("z_sep_min", 0))
+        if di.coordinates == 0:
+            z_sep_max = float(inp["p_max"]) * h_top / 300000.0 + 0.001
+        elif di.coordinates == 1:
+            z_sep_max = float(inp["bin_r_max"]) * h_top / 300000.0 + 0.001
     hists = {}
     shape = None
     for name, flag, g1, g2, auto in passes:
@@ -142,6 +161,13 @@ def run(inp, cols, count, out_prefix, table=None, precision=0):
         if not box:
             p.geometry = GEOM_SKY
             p.attach_table(table["H"][:rows], table["dA"][:rows], dz, z_max, z_sep_max)
+            p.coordinates, p.z_min, p.z_sep_min = di.coordinates, float(inp.get("z_min", 0)), z_sep_min
+            if di.coordinates >= 4 and int(inp.get("data_sorted", 1)) == 1 and not auto:
+                # corels2's sorted start pointer skips inner galaxies (all at z = 0) outside the window
+                zlo = z_sep_min if z_sep_min != 0 else float(inp.get("z_min", 0))
+                if not zlo <= 0 <= z_sep_max:
+                    hists[name] = None
+                    continue
         h, _ = count(outer, inner, p)
         hists[name] = h
         shape = h.num.shape
@@ -174,6 +200,7 @@ def run(inp, cols, count, out_prefix, table=None, precision=0):
                      n_jk=di.n_jk, r_mu=di.r_mu)
     if not box:
         po.geometry = GEOM_SKY
+        po.coordinates = di.coordinates
     n_files, pn = do_bining(po, norm, {k: v for k, v in bining.items() if v is not None}, out_prefix,
                             do_auto_corr=int(flags["auto"]), do_cross_corr=int(flags["cross"]),
                             do_auto_rebin=int(flags["rebin"]), rsrd_same=int(flags["rsrd_same"]))

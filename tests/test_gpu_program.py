@@ -71,6 +71,16 @@ CASES = [
     ("box", 5, 0, PREC_SPLIT, {"coordinates": 7, "n_p_bin": 1, "p_min": -1.0, "p_max": 1.0, "bin_r_max": 10.0}),
     # r-mu bins of the density and IA kinds
     ("box", 2, 0, PREC_SPLIT, {"coordinates": 7, "n_p_bin": 4, "p_min": -1.0, "p_max": 1.0}),
+    # the other sky coordinates: r-mu, rp-ephi (with a z window), projected, angular-ephi, angular
+    ("sky", 1, 0, PREC_SPLIT, {"coordinates": 1, "n_p_bin": 4, "p_min": -1.0, "p_max": 1.0}),
+    ("sky", 2, 0, PREC_SPLIT, {"coordinates": 2, "n_p_bin": 4, "p_min": 0.0, "p_max": 1.6,
+                               "z_sep_min": -0.05, "z_sep_max": 0.05}),
+    ("sky", 0, 3, PREC_SPLIT, {"coordinates": 3, "n_p_bin": 1, "p_min": 0.0, "p_max": 1.0,
+                               "z_sep_max": 0.1}),
+    ("sky", 1, 0, PREC_FP64, {"coordinates": 4, "n_p_bin": 4, "p_min": 0.0, "p_max": 1.6, "bin_r_min": 0.02,
+                              "bin_r_max": 1.5}),
+    ("sky", 2, 4, PREC_SPLIT, {"coordinates": 5, "n_p_bin": 1, "p_min": 0.0, "p_max": 1.0, "bin_r_min": 0.02,
+                               "bin_r_max": 1.5}),
 ]
 
 

@@ -220,6 +220,40 @@ def test_sky_plan(tmp_path, estimator):
         assert cats["D"]["read"] == "0"
 
 
+@pytest.mark.parametrize("coordinates", [2, 4])
+def test_sky_plan_angle_bins(tmp_path, coordinates):
+    """Coordinates 2 and 4 (read_dat.cpp:27-28, 91-92, 127-131, 146-149; corr.cpp:226-249;
+    initialization.cpp:510-520): random catalogs draw phi before include_prob, coordinates 4 reads
+    no redshift (table row 0), the random passes get a g_with_shape, and which_corr 2 runs
+    shape_shape for SD and DRs only (DRs falls back to density pairs: the shape randoms carry no
+    shapes) and density pairs otherwise."""
+    path, inp, cols, table = sky_case(tmp_path, which=2, estimator=0, n_rd=900, coordinates=coordinates,
+                                      p_min=0.0, p_max=1.6, n_p_bin=4)
+    r = run_prog(path, "--plan")
+    assert r.returncode == 0, r.stdout + r.stderr
+    cats, passes, info = parse_plan(r.stdout)
+    nRs, nRd = 2400, 900
+    s = rand_stream(2 * nRs + 2 * nRd, seed=program_ref.DEFAULT_SEED)
+    # read order S, Rs, D, Rd: (phi, include_prob) per random galaxy
+    assert float(cats["Rs"]["phi0"]) == s[0] * np.pi / 2 - np.pi / 2
+    assert float(cats["Rs"]["include_prob0"]) == s[1]
+    assert float(cats["Rd"]["phi0"]) == s[2 * nRs] * np.pi / 2 - np.pi / 2
+    assert float(cats["Rd"]["include_prob0"]) == s[2 * nRs + 1]
+    assert cats["S"]["has_e"] == "1" and cats["D"]["has_e"] == "1" and cats["Rs"]["has_e"] == "0"
+    if coordinates == 4:
+        assert float(cats["S"]["dc0"]) == table["dC"][0] and float(cats["S"]["da0"]) == table["dA"][0]
+    else:
+        it = int(np.floor(cols["shape"]["z"][0] / 1e-4))
+        assert float(cats["S"]["dc0"]) == table["dC"][it]
+    by = {p["name"]: p for p in passes}
+    assert by["SD"]["kind"] == "shape_shape"
+    assert by["DRs"]["kind"] == "dd" and by["SRd"]["kind"] == "dd" and by["RsRd"]["kind"] == "dd"
+    # DRs: corels2(grand_shape, gdensity) with g_with_shape 1 (the outer Rs, 2400 > 1000 galaxies)
+    assert by["DRs"]["outer"] == "Rs" and by["DRs"]["gws"] == "1"
+    # RsRd: corels2(grand_density, grand_shape), g_with_shape 2, swapped (Rs is larger): 1
+    assert by["RsRd"]["swapped"] == "1" and by["RsRd"]["gws"] == "1"
+
+
 def test_errors(tmp_path):
     path, inp, cols = box_case(tmp_path, which=0)
     bad = inputs.write_inp(str(tmp_path / "w7.inp"), **dict(inp, which_corr=7))
