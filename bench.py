@@ -235,7 +235,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-stride", type=int, default=250,
+    ap.add_argument("--cpu-stride", type=int, default=40,
                     help="CPU baseline: every n-th outer galaxy of the same pass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -318,6 +318,22 @@ def main():
                 "ms_primary": float(np.mean([s["ms_primary"] for s in st64])),
                 "counts_identical": bool(np.array_equal(h64.num, h_split.num))}
 
+    # the same SD pass as which_corr = 1 runs it (shape_density_corel: wg+ alone), one step per precision
+    wg_only = None
+    if not args.no_fp64:
+        import dataclasses
+        di1 = dataclasses.replace(di, which_corr=1)
+        _, _, q_split, _ = setup_pass("SD", cats["D"], cats["S"], di1, precision=PREC_SPLIT)
+        _, _, q_fp64, _ = setup_pass("SD", cats["D"], cats["S"], di1, precision=PREC_FP64)
+        run(q_split, 1, "warmup wg+ split")
+        ds, sts, hs = run(q_split, 1, "wg+ split")
+        run(q_fp64, 1, "warmup wg+ fp64")
+        d64, st64b, h64b = run(q_fp64, 1, "wg+ fp64")
+        wg_only = {"kind": "shape_density (which_corr 1)", "split_ms_per_step": ds * 1e3,
+                   "fp64_ms_per_step": d64 * 1e3, "split_ms_primary": sts[-1]["ms_primary"],
+                   "fp64_ms_primary": st64b[-1]["ms_primary"], "split_vs_fp64": d64 / ds,
+                   "n_binned": sts[-1]["n_binned"], "counts_identical": bool(np.array_equal(hs.num, h64b.num))}
+
     achieved = FLOPS_PER_PAIR["SS"] * n_binned / (ms_primary * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_TFLOPS, "traffic": measured_traffic(),
@@ -359,6 +375,7 @@ def main():
         "n_fp64_reeval": st_split[-1]["n_fp64_reeval"],
         "fp64": fp64,
         "split_vs_fp64": (value / fp64["value"]) if fp64 else None,
+        "wg_plus_only": wg_only,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
