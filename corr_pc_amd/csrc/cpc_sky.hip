@@ -57,7 +57,9 @@ __device__ __forceinline__ int rp_bin_sky(const SkyArgs &P, const double *thr, d
   return (k >= 0 && k < P.n_rp) ? k : -1;
 }
 
-template <int K, bool SPLIT, bool SWAP>
+// GEN compiles in the other coordinates' pi and rp (SkyArgs pim, rpm, rmu); without it the kernel is
+// that of coordinates 0, whose registers allow two waves per SIMD.
+template <int K, bool SPLIT, bool SWAP, bool GEN>
 __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide T) {
   const PRec *__restrict__ brec = B.rec;
   const SRec *__restrict__ bsr = B.sr;
@@ -197,9 +199,9 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
             int cell = -1;
             if (ok) {
               // pi_calc or pi_calc_projected, corel_bin; ephi (pi_calc_phi) and r-mu bin pi late
-              double dP = P.pim == 0 ? si.dc - so.dc : 0.0;
+              double dP = (!GEN || P.pim == 0) ? si.dc - so.dc : 0.0;
               double q2 = floor((dP - P.pmin) / P.psize);
-              const bool late = P.pim == 2 || P.rmu;
+              const bool late = GEN && (P.pim == 2 || P.rmu);
               bool exact = true;
               if (SPLIT && P.split_on && q2 >= 0.0 && q2 < (double)P.n_pi) {
                 // rp^2 from the chord between the unit vectors: theta^2 = 4 asin^2(c/2), with
@@ -237,8 +239,9 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
                 // rp_calc_sky_projected (outer DA) or rp_calc_sky_angular (degrees)
                 const double sum = so.sd * si.sd + so.cd * si.cd * cos((so.ra_rad - si.ra_rad));
                 const double dA = (sum > 0.9999999999) ? acos(1.0) : acos(sum);
-                const double dsc = P.rpm == 0 ? 0.5 * (so.da + si.da) : (P.rpm == 1 ? so.da : kSkyR2D);
-                const double rp = P.rpm == 0 ? dA * (so.da + si.da) / 2. : dA * dsc;
+                const int rpm = GEN ? P.rpm : 0;
+                const double dsc = rpm == 0 ? 0.5 * (so.da + si.da) : (rpm == 1 ? so.da : kSkyR2D);
+                const double rp = rpm == 0 ? dA * (so.da + si.da) / 2. : dA * dsc;
                 const double one_m = fmax(1.0 - sum * sum, 1e-300);
                 const double dda = P.dsum / sqrt(one_m) + 1e-15 * dA;
                 double band = 1.5 * dsc * dda + 4e-16 * rp;
@@ -349,10 +352,10 @@ hipError_t launch_gather_sky(const int32_t *sidx, int64_t n, const SRec *src_sr,
   return hipGetLastError();
 }
 
-template <int K, bool SPLIT, bool SWAP>
+template <int K, bool SPLIT, bool SWAP, bool GEN = false>
 static hipError_t launch_sky_t(const SkyArgs &P, int threads, size_t lds, const SkySide &B, const SkySide &T,
                                hipStream_t st) {
-  auto fn = sky_kernel<K, SPLIT, SWAP>;
+  auto fn = sky_kernel<K, SPLIT, SWAP, GEN>;
   hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   fn<<<P.n_items, threads, lds, st>>>(P, B, T);
@@ -363,8 +366,12 @@ hipError_t launch_sky(int kind, bool split, bool swap, const SkyArgs &P, int thr
                       const SkySide &T, hipStream_t st) {
   if (P.n_items == 0) return hipSuccess;
   const size_t lds = pair_lds_bytes(P.ncell - P.row0 * P.n_pi, P.nslots, P.n_rp, P.ncopy);
+  // the other coordinates classify every candidate in FP64 (split_on is 0 there): one kernel for both
+  const bool gen = P.pim != 0 || P.rpm != 0 || P.rmu != 0;
 #define CPC_SKY(KK)                                                                                     \
   if (kind == KK) {                                                                                     \
+    if (gen && swap) return launch_sky_t<KK, false, true, true>(P, threads, lds, B, T, st); \
+    if (gen) return launch_sky_t<KK, false, false, true>(P, threads, lds, B, T, st);        \
     if (split && swap) return launch_sky_t<KK, true, true>(P, threads, lds, B, T, st); \
     if (split) return launch_sky_t<KK, true, false>(P, threads, lds, B, T, st);        \
     if (swap) return launch_sky_t<KK, false, true>(P, threads, lds, B, T, st);         \
