@@ -102,3 +102,21 @@ def test_sky_coordinates_rejected(engine):
     p.r_mu = 1
     with pytest.raises(EngineError):
         engine.count(A, B, p)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_sky_coordinates_variants(engine, oracle_lib, prec):
+    """Linear bins without jackknife and with subsampled randoms (coordinates 1); the projected rp
+    with ephi bins (coordinates 2 with p_max > 600); shape_shape's estimator-1 branch with the
+    projected coordinates 3."""
+    A, B, n_jk, _ = sky_catalogs(3000, 5000, 75, shapes_in=True, inc=True, table=_TABLE)
+    A.e1, A.e2, A.phi = B.e1[: A.n].copy(), B.e2[: A.n].copy(), B.phi[: A.n].copy()
+    p = make_sky_params(_table(50.0 * _H_ZMAX / 300000.0 + 0.001), 10, 0.0, 50.0, 8, -1.0, 1.0, coordinates=1,
+                        kind=KIND_DD, lin_bin=1, do_jk=0, include_prob_max=0.6, precision=prec)
+    _run(engine, oracle_lib, A, B, p, f"sky r-mu linear, no jk, include_prob prec={prec}")
+    p = make_sky_params(_table(0.05), 12, 0.5, 60.0, 4, 0.0, 700.0, coordinates=2, kind=KIND_SHAPE_DENSITY,
+                        g_with_shape=2, which_corr=1, do_jk=1, n_jk=n_jk, precision=prec)
+    _run(engine, oracle_lib, A, B, p, f"sky rp-ephi projected prec={prec}")
+    p = make_sky_params(_table(0.04), 12, 0.5, 60.0, 2, -1.0, 5.0, coordinates=3, kind=KIND_SHAPE_SHAPE,
+                        g_with_shape=2, estimator=1, which_corr=2, do_jk=1, n_jk=n_jk, z_min=0.2, precision=prec)
+    _run(engine, oracle_lib, A, B, p, f"sky projected shape_shape est 1 prec={prec}")
