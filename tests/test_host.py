@@ -78,9 +78,10 @@ def test_pi_bin_edges(oracle_lib):
     assert oracle_lib.pi_bin(p, 100.0 - 1e-13) == 39
 
 
-def _cat(n):
+def _cat(n, shapes=False):
     z = np.sort(np.random.default_rng(n).uniform(0, 10, n))
-    return Catalog(x=np.zeros(n) + 1, y=np.zeros(n) + 1, z=z)
+    e = np.zeros(n) if shapes else None
+    return Catalog(x=np.zeros(n) + 1, y=np.zeros(n) + 1, z=z, e1=e, e2=e)
 
 
 def test_pass_roles_and_swap():
@@ -88,9 +89,11 @@ def test_pass_roles_and_swap():
     (initialization.cpp:490-543)."""
     di = DataInfo(which_corr=2, estimator=0, box_size=100.0, n_rp=5, rp_min=0.1, rp_max=10.0, n_pi=4, pi_min=-8.0,
                   pi_max=8.0, do_jk=1, n_jk=8, include_prob_RR=0.3)
-    D, S, R = _cat(100), _cat(100), _cat(300)
+    D, S, R = _cat(100, shapes=True), _cat(100, shapes=True), _cat(300)
     o, i, p, sw = setup_pass("SD", D, S, di)  # corels2(gdensity, gshape), equal sizes: no swap
     assert (o is D, i is S, sw, p.g_with_shape, p.kind) == (True, True, False, 2, KIND_SHAPE_SHAPE)
+    # a shape kind on a catalog without shapes is a reference read of an unset pointer: density pairs
+    assert setup_pass("SD", _cat(100), S, di)[2].kind == KIND_DD
     o, i, p, sw = setup_pass("SRd", R, S, di)  # randoms larger: stay outer
     assert (o is R, i is S, sw, p.g_with_shape, p.kind) == (True, True, False, 2, KIND_SHAPE_DENSITY)
     R2 = _cat(50)
