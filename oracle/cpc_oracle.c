@@ -8,9 +8,13 @@
  *                                                      for PBz > 0, each ending at the first included
  *                                                      galaxy outside the z window
  *   per-outer window and image flags (box)             PB_calcs.cpp:12-61 (lim_calc_xyz, lim_calc_PB)
- *   per-outer window (sky)                             sky_calcs.cpp:33-81, 105-190 (lim_calc_sky)
- *   pi and rp of a pair (r and mu, coordinates 7)      PB_calcs.cpp:87-98; calcs.cpp:126-140;
- *                                                      sky_calcs.cpp:24-31, 91-96 (dA_calc, rp_calc_sky)
+ *   per-outer window (sky, coordinates 0-5)            sky_calcs.cpp:33-81, 105-190 (lim_calc_sky,
+ *                                                      lim_calc_ra_dec); no z test for 4-5, corr.cpp:69
+ *   pi and rp of a pair (r and mu, coordinates 1, 7)   PB_calcs.cpp:87-98; calcs.cpp:108-140
+ *                                                      (pi_calc, pi_calc_projected, pi_calc_phi,
+ *                                                      r_mu_calc); sky_calcs.cpp:24-31, 84-102
+ *                                                      (dA_calc, rp_calc_sky, _projected, _angular);
+ *                                                      choose_rp_pi_func, initialization.cpp:545-571
  *   bin indices, pair weight                           corels.cpp:21-46 (corel_bin)
  *   the pair kinds                                     corels.cpp:48-77, 79-120, 122-204, 206-244, 246-262
  *   ED: cos^2 of axis and separation                   PB_calcs.cpp:101-134
