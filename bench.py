@@ -259,8 +259,9 @@ def main():
         print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 3}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is described "
-              f"in DESIGN.md and is not launched.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): more GPUs would each take a share "
+              f"of the work items and sum their histograms in one reduction, which is not launched here.",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "mocks":
