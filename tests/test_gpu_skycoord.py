@@ -120,3 +120,20 @@ def test_sky_coordinates_variants(engine, oracle_lib, prec):
     p = make_sky_params(_table(0.04), 12, 0.5, 60.0, 2, -1.0, 5.0, coordinates=3, kind=KIND_SHAPE_SHAPE,
                         g_with_shape=2, estimator=1, which_corr=2, do_jk=1, n_jk=n_jk, z_min=0.2, precision=prec)
     _run(engine, oracle_lib, A, B, p, f"sky projected shape_shape est 1 prec={prec}")
+
+
+def test_angular_ephi_sums_to_angular_projected_at_scale(engine):
+    """A size-independent check on 2e5 x 3e5 galaxies: coordinates 4 bins the same pairs as
+    coordinates 5 (same rp in degrees, no z test), spread over ephi bins covering [0, pi/2]; summed
+    over pi, its counts equal coordinates 5's one pi bin exactly, whichever pairs the guard bands
+    sent to the host."""
+    A, B, n_jk, _ = sky_catalogs(200_000, 300_000, 76, shapes_in=True, table=_TABLE)
+    A, B = _zero_z(A), _zero_z(B)
+    p4 = _params(4, KIND_DD, 2, 0, n_jk, PREC_SPLIT)
+    p5 = _params(5, KIND_DD, 2, 0, n_jk, PREC_SPLIT)
+    print("[case] sky coordinates 4 vs 5 at 2e5 x 3e5", flush=True)
+    h4, st4 = engine.count(A, B, p4)
+    h5, st5 = engine.count(A, B, p5)
+    assert st4["n_binned"] == st5["n_binned"] > 1e8
+    assert np.array_equal(h4.num.sum(axis=2), h5.num[..., 0])
+    np.testing.assert_allclose(h4.wt_num.sum(axis=2), h5.wt_num[..., 0], rtol=1e-10)
