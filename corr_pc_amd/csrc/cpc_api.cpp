@@ -777,7 +777,8 @@ Status validate(const cpc_job &job, Plan &P) {
   if (!sky && !(p.box_size > 0)) return {CPC_ERR_INVALID, "box_size must be > 0"};
   if (p.n_rp <= 0 || p.n_pi <= 0) return {CPC_ERR_INVALID, "n_rp and n_pi must be > 0"};
   if (!(p.rp_max > 0) || p.rp_min > p.rp_max) return {CPC_ERR_INVALID, "Bad Bin limits"};  // initialization.cpp:299-303
-  if (!(p.pi_max > p.pi_min)) return {CPC_ERR_INVALID, "pi_max must exceed pi_min"};
+  const bool unit_pi = sky && (p.coordinates == 3 || p.coordinates == 5);  // pi bins of width 1 (pi = 0)
+  if (!unit_pi && !(p.pi_max > p.pi_min)) return {CPC_ERR_INVALID, "pi_max must exceed pi_min"};
   if (!p.lin_bin && !(p.rp_min > 0)) return {CPC_ERR_INVALID, "log rp bins need rp_min > 0"};
   if (p.r_mu != 0 && p.r_mu != 1) return {CPC_ERR_INVALID, "r_mu must be 0 or 1"};
   if (sky && p.r_mu) return {CPC_ERR_INVALID, "on the sky r-mu bins are coordinates 1; r_mu must be 0"};
@@ -1891,8 +1892,8 @@ cpc_status cpc_bin_edges(const cpc_params *p, double *rp_edges, double *pi_edges
     }
   }
   if (pi_edges) {
-    // P_bin_ini: p_size = float((p_max - p_min) / n_p_bin), running FP32 sum from p_min
-    const float p_size = (float)((p->pi_max - p->pi_min) / (p->n_pi));
+    // P_bin_ini: p_size = float(data_info.p_size), running FP32 sum from p_min
+    const float p_size = (float)derive(*p).psize;
     float temp = (float)p->pi_min;
     pi_edges[0] = temp;
     for (int i = 0; i < p->n_pi; ++i) {

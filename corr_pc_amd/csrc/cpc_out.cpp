@@ -518,7 +518,8 @@ void cross_branch(Writer &W, const cpc_params &p, const cpc_norm &n, const cpc_b
 const char *check_output_params(const cpc_params &p) {
   if (p.estimator < 0 || p.estimator > 4) return "estimator must be 0-4";
   if (p.which_corr < 0 || p.which_corr > 5) return "which_corr must be 0-5";
-  if (p.n_rp <= 0 || p.n_pi <= 0 || !(p.pi_max > p.pi_min)) return "bad binning";
+  const bool unit_pi = p.geometry == CPC_GEOM_SKY && (p.coordinates == 3 || p.coordinates == 5);
+  if (p.n_rp <= 0 || p.n_pi <= 0 || (!unit_pi && !(p.pi_max > p.pi_min))) return "bad binning";
   if (p.do_jk != 0 && p.do_jk != 1) return "do_jk must be 0 or 1";
   if (p.do_jk == 1 && p.n_jk > 0 && p.jk_mode != CPC_JK_LEAVE_ONE_OUT) return "histograms must be leave-one-out";
   return nullptr;

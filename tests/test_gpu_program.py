@@ -81,6 +81,11 @@ CASES = [
                               "bin_r_max": 1.5}),
     ("sky", 2, 4, PREC_SPLIT, {"coordinates": 5, "n_p_bin": 1, "p_min": 0.0, "p_max": 1.0, "bin_r_min": 0.02,
                                "bin_r_max": 1.5}),
+    # the HSC notebook's run: projected coordinates 3, estimator 4, no randoms, pmin = pmax = 0 (pi
+    # bins of width 1), z_sep -100..100, 10 log rp bins in [0.1, 10)
+    ("sky", 1, 4, PREC_SPLIT, {"coordinates": 3, "n_p_bin": 1, "p_min": 0, "p_max": 0, "z_sep_min": -100,
+                               "z_sep_max": 100, "bin_r_min": 0.1, "bin_r_max": 10.0, "n_bins": 10,
+                               "n_shape_rand": 0, "n_density_rand": 0}),
 ]
 
 
