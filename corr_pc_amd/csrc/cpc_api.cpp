@@ -785,8 +785,12 @@ Status validate(const cpc_job &job, Plan &P) {
   if (!sky && p.coordinates != 0) return {CPC_ERR_INVALID, "coordinates selects the sky formulas: 0 in the box"};
   if (sky && (p.coordinates < 0 || p.coordinates > 5))
     return {CPC_ERR_UNSUPPORTED, "sky coordinates must be 0-5 (6 and 7 are the periodic box)"};
-  if (!sky && (2 * p.rp_max >= p.box_size || window_hi(p) - window_lo(p) >= p.box_size))
-    return {CPC_ERR_UNSUPPORTED, "the rp and pi windows must be narrower than the box"};
+  // rp windows narrower than half the box (each x-y neighbour column is met through one image); the
+  // z window may be as wide as the reference allows: it tests the images -1, 0, +1 of lim_calc_PB
+  // (PB_calcs.cpp:30-61) one by one, and so does the engine (the generator notebook's box of 75
+  // with |pi| < 75 counts every pair under two z images)
+  if (!sky && 2 * p.rp_max >= p.box_size)
+    return {CPC_ERR_UNSUPPORTED, "rp_max must be below half the box size"};
   if (p.kind < CPC_KIND_DD || p.kind > CPC_KIND_ED) return {CPC_ERR_INVALID, "unknown kind"};
   if (p.kind == CPC_KIND_ED) {
     if (sky) return {CPC_ERR_UNSUPPORTED, "the ED kind is defined for the periodic box only (ED_calc_PB)"};

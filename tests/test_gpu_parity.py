@@ -174,3 +174,17 @@ def test_split_matches_fp64_full_config1(engine):
     # rp-level counts are the pi sums
     assert np.all(h_split.num[1:] <= h_split.num[0][None])
     assert np.array_equal(h_split.num_rp, h_split.num.sum(axis=2))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_pi_window_wider_than_the_box(engine, oracle_lib, prec):
+    """The generator notebook's box (L = 75, |pi| < 75, 20 pi bins, rp in [0.1, 10)): the z window
+    is twice the box, so every pair is met under two z images; cross and auto passes."""
+    L = 75.0
+    A = small_catalog(3000, L, 94, clumps=30)
+    B = small_catalog(2500, L, 95, clumps=30)
+    p = make_params(L, 10, 0.1, 10.0, 20, -75.0, 75.0, kind=KIND_SHAPE_SHAPE, estimator=0, g_with_shape=2, do_jk=1,
+                    n_jk=8, precision=prec)
+    _run(engine, oracle_lib, A, B, p, f"wide pi window cross prec={prec}")
+    p.auto_pairs = 1
+    _run(engine, oracle_lib, A, A, p, f"wide pi window auto prec={prec}")

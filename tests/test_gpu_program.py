@@ -69,6 +69,9 @@ CASES = [
     ("sky", 4, 0, PREC_FP64, {}),
     # the ED notebook's box run: which_corr 5 with r-mu bins (coordinates 7), one mu bin in [-1, 1)
     ("box", 5, 0, PREC_SPLIT, {"coordinates": 7, "n_p_bin": 1, "p_min": -1.0, "p_max": 1.0, "bin_r_max": 10.0}),
+    # the generator notebook's box: L = 75, |pi| < 75 (a z window twice the box), 20 pi bins
+    ("box", 2, 0, PREC_SPLIT, {"L": 75.0, "p_min": -75.0, "p_max": 75.0, "n_p_bin": 20, "bin_r_min": 0.1,
+                               "bin_r_max": 10.0, "n_bins": 10}),
     # r-mu bins of the density and IA kinds
     ("box", 2, 0, PREC_SPLIT, {"coordinates": 7, "n_p_bin": 4, "p_min": -1.0, "p_max": 1.0}),
     # the other sky coordinates: r-mu, rp-ephi (with a z window), projected, angular-ephi, angular

@@ -80,3 +80,4 @@ def test_poisson_uniform_box(oracle_lib, tmp_path):
                 str(tmp_path) + "/rm_")
     w = np.loadtxt(str(tmp_path) + "/rm_bins2D_cross_final.dat", comments="#")[:, 3].reshape(p.n_rp, p.n_pi)
     assert np.all(np.abs(w[ok]) * np.sqrt(expect[ok]) < 5)
+

@@ -55,11 +55,10 @@ def kappa_role(which, role):
 
 
 def box_case(tmp_path, which=2, estimator=0, n_s=1500, n_d=1200, n_rs=2500, n_rd=2000, do_jk=1, jk_side=2, seed=3,
-             sorted_z=True, **extra):
+             sorted_z=True, L=60.0, **extra):
     """A small periodic box in the reference's file format: shape and density samples with shapes,
     weights and jk files, shape and density randoms."""
     rng = np.random.default_rng(seed)
-    L = 60.0
     cols, files = {}, {}
     for role, n, with_e, with_w in (("shape", n_s, True, True), ("density", n_d, True, True),
                                     ("shape_rand", n_rs, False, False), ("density_rand", n_rd, False, False)):
