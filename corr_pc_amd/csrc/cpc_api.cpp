@@ -808,7 +808,8 @@ Status validate(const cpc_job &job, Plan &P) {
   if (p.auto_pairs != 0 && p.auto_pairs != 1) return {CPC_ERR_INVALID, "auto_pairs must be 0 or 1"};
   P.autop = p.auto_pairs == 1;
   if (P.autop && co != ci) return {CPC_ERR_INVALID, "the auto pass (corels1) takes one catalog on both sides"};
-  if (P.autop && co->n > 0 && !co->z_sorted)
+  if (p.unsorted != 0 && p.unsorted != 1) return {CPC_ERR_INVALID, "unsorted must be 0 or 1"};
+  if (P.autop && !p.unsorted && co->n > 0 && !co->z_sorted)
     return {CPC_ERR_INVALID, "the auto pass needs the catalog sorted by z (data_sorted=1, corr.cpp:129-132)"};
   if (p.precision != CPC_PREC_SPLIT && p.precision != CPC_PREC_FP64) return {CPC_ERR_INVALID, "unknown precision"};
   P.jk_on = p.do_jk == 1 && p.n_jk > 0;
@@ -958,10 +959,11 @@ Status setup(cpc_ctx *ctx, Plan &P) {
       const double inc = ci->h_inc.empty() ? 0.0 : ci->h_inc[j];
       if (inc > p.include_prob_max) continue;
       has_zfirst = true;
-      zfirst_plus_L = ci->h_z[j] + L * 1;
+      // an unsorted scan (do_pair_unsorted) does not stop at that first galaxy
+      zfirst_plus_L = p.unsorted ? INFINITY : ci->h_z[j] + L * 1;
       break;
     }
-    if (autop && co->n > 0) {
+    if (autop && !p.unsorted && co->n > 0) {
       // corels1 scans for PBz = 0 and PBz = -1 start at j = i+1 and stop at once when that next
       // included galaxy lies below the window (do_pair_sorted, corr.cpp:71-73): bit 0 / bit 1
       std::vector<uint8_t> &af = P.aflags;
@@ -1054,7 +1056,7 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     if (check(B.reach.ensure(nb), st, "alloc")) return st;
     check(hipMemcpyAsync(B.reach.p, reach.data(), nb * sizeof(double2), hipMemcpyHostToDevice, s), st, "copy");
     SA.reach_tab = B.reach.p;
-    if (autop) {
+    if (autop && !p.unsorted) {
       // corels1's scan from j = i+1 stops at once when that next included galaxy lies below the
       // window (corr.cpp:168-186): such outer galaxies take no pairs
       int64_t nxt = -1;
@@ -1077,7 +1079,7 @@ Status setup(cpc_ctx *ctx, Plan &P) {
   if ((double)G.ncx * G.ncy * nreg * 2.0 >= 4294967295.0)
     return {CPC_ERR_UNSUPPORTED, "too many cells x jackknife regions for 32-bit sort keys"};
   st = build_side(ctx, co, G, nreg, jk_on, p, has_zfirst, zfirst_plus_L, sky, sky ? B.win_raw.p : nullptr,
-                  (autop && !sky) ? B.aflags.p : nullptr, B.so);
+                  (autop && !sky && !p.unsorted) ? B.aflags.p : nullptr, B.so);
   if (st.code) return st;
   const bool same = (co == ci);
   if (!same) {
@@ -1178,7 +1180,7 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     K.gC = gr.gC;
     K.gD = gr.gD;
     K.nbr = B.nbr.p;
-    K.auto_mode = autop ? 1 : 0;
+    K.auto_mode = autop ? (p.unsorted ? 2 : 1) : 0;
     K.chunks = B.chunks.p;
     // primary pass: blocks = outer galaxies, tiles = inner galaxies
     P.kp = K;
@@ -1232,7 +1234,7 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     A.rpm = sm.rpm;
     A.rmu = sm.rmu ? 1 : 0;
     A.dphi = 1e-13;
-    A.auto_mode = autop ? 1 : 0;
+    A.auto_mode = autop ? (p.unsorted ? 2 : 1) : 0;
     // Split classification.  The estimate rp^2 = theta^2 dbar^2 (theta^2 = 4 asin^2(c/2) from the
     // chord c of FP64 unit vectors, a series with relative error below 1e-12) differs from the
     // reference's acos(sum) dbar, squared, by at most 2 (5e-16 / theta^2) relative (the rounding of

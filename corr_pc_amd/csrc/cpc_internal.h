@@ -86,8 +86,9 @@ struct KArgs {
   int gws, n_nbr;
   int phase_mode;      // 0 primary pass without jackknife, 1 primary with jackknife, 2 touch pass
   int nslots;          // f64 slots accumulated by this pass
-  int auto_mode;       // corels1 pass: pairs through PBz <= 0 images only with the inner galaxy after
-                       // the outer one (z, then catalog index)
+  int auto_mode;       // corels1 pass: 1 (sorted) pairs through PBz <= 0 images only with the inner
+                       // galaxy after the outer one (z, then catalog index); 2 (unsorted) through every
+                       // image only with the inner galaxy after the outer one in catalog order
   const int32_t *bidx, *tidx;  // catalog index of the block-side and tile-side records
   int lin;
   // split-precision classification
@@ -150,7 +151,8 @@ struct SkyArgs {
   // reference's formula
   int split_on;
   float la, lb, g, s_out_lo, s_out_hi;
-  int auto_mode;           // corels1 pass: pairs only with the inner galaxy after the outer one
+  int auto_mode;           // corels1 pass: pairs only with the inner galaxy after the outer one, in
+                           // (z, catalog index) order (1) or catalog order (2, unsorted)
   int row0, ncopy;         // LDS holds rp rows row0.. in ncopy lane-selected copies (7+ slots per cell)
   const double *thr;       // n_rp+1 thresholds on rp (host libm log10)
   const Item *items;

@@ -162,7 +162,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
             // image of the inner galaxy in the reference's terms
             const int PBx = SWAP ? -sx : sx, PBy = SWAP ? -sy : sy, PBz = SWAP ? -sz : sz;
             const bool lane_ok = valid && (SWAP || image_allowed(me.meta, PBx, PBy, PBz));
-            const bool ordered = H.auto_mode && PBz <= 0;  // the inner galaxy must come after the outer
+            // the inner galaxy must come after the outer (sorted: images with PBz <= 0; unsorted: all)
+            const bool ordered = H.auto_mode == 2 || (H.auto_mode == 1 && PBz <= 0);
             // FP32 offsets of this lane to the tile column: d = tile local + offset
             const float ax = Ox - me.xl, ay = Oy - me.yl, az = (float)(H.L * sz) - me.zf;
             const float pinv = SWAP ? -H.inv_psize32 : H.inv_psize32;  // pi = -dz in the touch pass
@@ -181,7 +182,11 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
                 const int l3 = 4 * t + 3;  // lane of the quarter {xl, yl, zf, meta}
                 bool ok = lane_ok;
                 if (SWAP) ok = ok && image_allowed(rl_i(cq.w, l3), PBx, PBy, PBz);
-                if (ordered) {
+                if (ordered && H.auto_mode == 2) {
+                  // unsorted corels1: j > i in catalog order
+                  const int tix = P.tidx[jb + t];
+                  ok = ok && (SWAP ? midx > tix : tix > midx);
+                } else if (ordered) {
                   // corels1 scans j > i in catalog order, which is (z, index) order for its z-sorted input
                   const double tzo = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(cz), t),
                                                       __builtin_amdgcn_readlane(__double2loint(cz), t));

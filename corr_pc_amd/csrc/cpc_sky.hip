@@ -190,10 +190,11 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
             bool ok = valid && !(piz > w.zmax || piz < w.zmin) && !(pi.y > w.decmax || pi.y < w.decmin) &&
                       !(pi.x < w.ramin || pi.x > w.ramax);
             if (P.auto_mode) {
-              // corels1 pairs: the inner galaxy after the outer one in catalog ((z, index)) order
+              // corels1 pairs: the inner galaxy after the outer one in catalog order, which is (z,
+              // index) order for z-sorted input (1) and the index alone for unsorted input (2)
               const double zo = SWAP ? tzj : mez;
               const int io = SWAP ? T.idx[j] : B.idx[p], ii = SWAP ? B.idx[p] : T.idx[j];
-              ok = ok && (piz > zo || (piz == zo && ii > io));
+              ok = ok && (P.auto_mode == 2 ? ii > io : (piz > zo || (piz == zo && ii > io)));
             }
             ncand += ok;
             int cell = -1;

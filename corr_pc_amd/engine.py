@@ -52,7 +52,8 @@ class Params(ctypes.Structure):
                 ("pi_max", c_double), ("do_jk", c_int), ("n_jk", c_int), ("include_prob_max", c_double),
                 ("precision", c_int), ("jk_mode", c_int), ("sky_H", _dp), ("sky_dA", _dp), ("sky_n", c_int64),
                 ("sky_dz", c_double), ("z_max", c_double), ("z_sep_max", c_double), ("auto_pairs", c_int),
-                ("r_mu", c_int), ("coordinates", c_int), ("z_min", c_double), ("z_sep_min", c_double)]
+                ("r_mu", c_int), ("coordinates", c_int), ("z_min", c_double), ("z_sep_min", c_double),
+                ("unsorted", c_int)]
 
     def attach_table(self, H, dA, dz, z_max, z_sep_max):
         """Sky geometry: keep references to the distance table arrays and point the struct at them."""

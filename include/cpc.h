@@ -141,6 +141,11 @@ typedef struct cpc_params {
   int coordinates;
   double z_min;     /* data_info.z_min: the window's lower edge for coordinates 2-5 when z_sep_min is 0 */
   double z_sep_min; /* data_info.z_sep_min (coordinates 2-5) */
+  /* data_sorted 0: do_pair_unsorted (corr.cpp:36-52) ends no scan, so every image's scan meets every
+   * inner galaxy: the PBz = +1 image takes no first-galaxy rule, and the auto pass (corels1) pairs
+   * the galaxies after the outer one in catalog order under every image (corr.cpp:167-182).  The
+   * catalogs need not be sorted.  0: data_sorted 1. */
+  int unsorted;
 } cpc_params;
 
 /* Output histograms, caller-allocated host memory, overwritten by cpc_count.

@@ -118,6 +118,7 @@ typedef struct {
   double L, pmin, pmax, psize, rmin, rmax, rsize;
   int n_rp, n_pi, lin, kind, gws, est, which_corr, jk_on;
   int autop; /* corels1 auto pass */
+  int unsorted; /* data_sorted 0: no scan stops (do_pair_unsorted); auto pairs j > i under every image */
   int rmu;   /* r-mu bins (coordinates 7; on the sky coordinates 1) */
   double inc_max;
   /* sky */
@@ -446,6 +447,7 @@ static int make_params(const cpc_params *p, oparm *P) {
   P->z_min = p->z_min;
   P->z_sep_min = p->z_sep_min;
   P->autop = p->auto_pairs != 0;
+  P->unsorted = p->unsorted != 0;
   P->rmu = p->r_mu != 0;
   P->coord = p->coordinates;
   if (P->geom == CPC_GEOM_SKY && (!P->H || !P->dA || P->ntab < 2 || !(P->dz > 0))) return CPC_ERR_INVALID;
@@ -535,6 +537,8 @@ int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_
           int64_t j;
           if (P.autop) {
             j = i + 1; /* corels1: scan from j = i+1 (corr.cpp:168) */
+          } else if (P.unsorted) {
+            j = 0; /* do_pair_unsorted scans every inner galaxy (corr.cpp:36-52) */
           } else {
             j = lower_bound_(g2->z, n2, 0.0, w[0]);
             if (j < n2 && g2->z[j] > w[1]) sparse++;
@@ -542,7 +546,10 @@ int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_
           for (; j < n2; j++) {
             if (cinc(g2, j) > P.inc_max) continue;
             double zz = g2->z[j];
-            if (zz > w[1] || zz < w[0]) break;
+            if (zz > w[1] || zz < w[0]) {
+              if (P.unsorted) continue;
+              break;
+            }
             if (g2->y[j] > w[5] || g2->y[j] < w[4]) continue;
             if (g2->x[j] < w[2] || g2->x[j] > w[3]) continue;
             cand++;
@@ -571,7 +578,9 @@ int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_
             for (int zi = 0; zi <= abs(zt); zi++, cz--) {
               const double sz = L * cz, sy = L * cy, sx = L * cx;
               int64_t j;
-              if (cz > 0) {
+              if (P.unsorted) {
+                j = P.autop ? i + 1 : 0; /* corels1 / corels2 with data_sorted 0 (corr.cpp:167-182, 380-423) */
+              } else if (cz > 0) {
                 j = 0;
               } else if (P.autop) {
                 j = i + 1; /* corels1: PBz <= 0 scans from j = i+1 (corr.cpp:168-176) */
@@ -582,7 +591,10 @@ int orc_count(const cpc_catalog_desc *g1, const cpc_catalog_desc *g2, const cpc_
               for (; j < n2; j++) {
                 if (cinc(g2, j) > P.inc_max) continue;
                 double zz = g2->z[j] + sz;
-                if (zz > z_max || zz < z_min) break; /* do_pair_sorted, corr.cpp:71-73 */
+                if (zz > z_max || zz < z_min) {
+                  if (P.unsorted) continue; /* do_pair_unsorted, corr.cpp:38-43 */
+                  break;                    /* do_pair_sorted, corr.cpp:71-73 */
+                }
                 double yy = g2->y[j] + sy;
                 if (yy > dec_max || yy < dec_min) continue;
                 double xx = g2->x[j] + sx;

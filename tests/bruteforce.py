@@ -90,6 +90,9 @@ def count(outer, inner, p):
     after i for PBz <= 0, the first of the catalog for PBz > 0) lies at or above the window's lower
     edge (corr.cpp:159-190 with do_pair_sorted's break, corr.cpp:71-73)."""
     auto = bool(getattr(p, "auto_pairs", 0))
+    # data_sorted 0 (do_pair_unsorted, corr.cpp:36-52): no scan stops, and corels1 takes j > i under
+    # every image (corr.cpp:167-182)
+    unsorted = bool(getattr(p, "unsorted", 0))
     if auto:
         inner = outer
     L = p.box_size
@@ -119,14 +122,14 @@ def count(outer, inner, p):
         zmin, zmax = zi + pmin, zi + pmax
         nxt = next((j for j in range(i + 1, inner.n) if not inc_i[j] > p.include_prob_max), None) if auto else None
         for (PBx, PBy, PBz) in images(zi, xi, yi, L, pmin, pmax, rmax):
-            if PBz == 1 and (first is None or float(inner.z[first]) + L * 1 < zmin):
+            if not unsorted and PBz == 1 and (first is None or float(inner.z[first]) + L * 1 < zmin):
                 continue
-            if auto and PBz <= 0 and (nxt is None or float(inner.z[nxt]) + L * PBz < zmin):
+            if not unsorted and auto and PBz <= 0 and (nxt is None or float(inner.z[nxt]) + L * PBz < zmin):
                 continue
             for j in range(inner.n):
                 if inc_i[j] > p.include_prob_max:
                     continue
-                if auto and PBz <= 0 and j <= i:
+                if auto and (PBz <= 0 or unsorted) and j <= i:
                     continue
                 xj, yj, zj = float(inner.x[j]), float(inner.y[j]), float(inner.z[j])
                 zz = zj + L * PBz
@@ -215,6 +218,7 @@ def count_sky(outer, inner, p):
     """Every ordered (outer, inner) pair of the sky pass, no z-sorted sweep; with p.auto_pairs, the
     corels1 pairs (i, j > i) of `outer`, for i whose next included galaxy is not below its window."""
     auto = bool(getattr(p, "auto_pairs", 0))
+    unsorted = bool(getattr(p, "unsorted", 0))
     if auto:
         inner = outer
     pmin, pmax, rmin, rmax = p.pi_min, p.pi_max, p.rp_min, p.rp_max
@@ -236,7 +240,7 @@ def count_sky(outer, inner, p):
             continue
         rai, deci, zi = float(outer.x[i]), float(outer.y[i]), float(outer.z[i])
         zmin, zmax, ramin, ramax, decmin, decmax = sky_window(p, rai, deci, zi)
-        if auto:
+        if auto and not unsorted:
             nxt = next((j for j in range(i + 1, inner.n) if not g(inner, "include_prob", j, 0.0) > p.include_prob_max),
                        None)
             if nxt is None or float(inner.z[nxt]) < zmin:
