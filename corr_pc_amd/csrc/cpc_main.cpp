@@ -213,12 +213,8 @@ bool data_ini(DataInfo &d, const std::string &name) {
       d.do_auto_rebin = false;
       break;
   }
-  // The unsorted scan (do_pair_unsorted, corr.cpp:36-52) visits every inner galaxy, so a cross pass
-  // counts every pair inside the window whatever the order, as the engine does.  The auto pass of
-  // unsorted mode starts every image at j = i+1 (corr.cpp:167-182), which gives the pairs across the
-  // z boundary the other pi sign than sorted mode; the engine implements sorted mode only.
-  if (d.data_sorted == 0 && d.do_auto_corr)
-    return fail("estimators 1-3 (auto pairs) need data_sorted 1 and catalogs sorted by redshift");
+  // data_sorted 0 reaches the engine as cpc_params.unsorted: do_pair_unsorted (corr.cpp:36-52) ends
+  // no scan, and the auto pass starts every image's scan at j = i+1 (corr.cpp:167-182).
   d.RsRd_same = false;
   if (d.n_density_rand == 0 && d.which_corr <= 2) {
     d.RsRd_same = true;
@@ -692,6 +688,7 @@ cpc_params base_params(const DataInfo &d, int precision) {
     p.z_min = d.z_min;
     p.z_sep_min = d.z_sep_min;
   }
+  p.unsorted = d.data_sorted == 0 ? 1 : 0;
   return p;
 }
 

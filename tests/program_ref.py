@@ -99,10 +99,6 @@ def build_catalogs(inp, cols, table=None):
         cat = Catalog(x=c["x"][:n], y=c["y"][:n], z=z, w=None if c.get("w") is None else c["w"][:n],
                       e1=e1, e2=e2, phi=phi, jk=jk, jk_prob=jk_prob,
                       include_prob=inc, e3=e3, **kw)
-        if int(inp.get("data_sorted", 1)) == 0:
-            # the unsorted scan counts every pair inside the window, whatever the order; the oracle
-            # scans in z order, so hand it the same galaxies sorted by z
-            cat = cat.subset(np.argsort(cat.z, kind="stable"))
         cats[role] = cat
     return cats, flags
 
@@ -158,6 +154,7 @@ def run(inp, cols, count, out_prefix, table=None, precision=0):
             hists[name] = None
             continue
         outer, inner, p, _ = st
+        p.unsorted = int(int(inp.get("data_sorted", 1)) == 0)
         if not box:
             p.geometry = GEOM_SKY
             p.attach_table(table["H"][:rows], table["dA"][:rows], dz, z_max, z_sep_max)

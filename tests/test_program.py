@@ -87,7 +87,7 @@ def box_case(tmp_path, which=2, estimator=0, n_s=1500, n_d=1200, n_rs=2500, n_rd
     return path, inp, cols
 
 
-def sky_case(tmp_path, which=1, estimator=0, n_s=1200, n_d=1000, n_rs=2400, n_rd=0, seed=5, **extra):
+def sky_case(tmp_path, which=1, estimator=0, n_s=1200, n_d=1000, n_rs=2400, n_rd=0, seed=5, sorted_z=True, **extra):
     """A small survey field: RA 10-14 deg, Dec -2..2 deg, z in (0.2, 0.5), with a flat LCDM table."""
     rng = np.random.default_rng(seed)
     table = synth.distance_table(0.3, 1e-4, 0.5)
@@ -96,7 +96,9 @@ def sky_case(tmp_path, which=1, estimator=0, n_s=1200, n_d=1000, n_rs=2400, n_rd
                                     ("shape_rand", n_rs, False, False), ("density_rand", n_rd, False, False)):
         ra = rng.uniform(10.0, 14.0, n)
         dec = rng.uniform(-2.0, 2.0, n)
-        z = np.sort(rng.uniform(0.2, 0.5, n))
+        z = rng.uniform(0.2, 0.5, n)
+        if sorted_z:
+            z = np.sort(z)
         jk = (np.minimum(((ra - 10.0) / 2.0).astype(int), 1) * 2 + np.minimum(((dec + 2.0) / 2.0).astype(int), 1))
         c = {"x": ra, "y": dec, "z": z, "jk": jk, "w": rng.lognormal(0, 0.3, n) if with_w else None}
         if with_e and kappa_role(which, role):
@@ -276,14 +278,17 @@ def test_unsorted_catalog_rejected(tmp_path):
     assert r.returncode == 1 and "not sorted by redshift" in r.stdout
 
 
-def test_unsorted_input_cross_only(tmp_path):
-    """data_sorted 0: cross passes take catalogs in any order; the auto estimators refuse it."""
+def test_unsorted_input(tmp_path):
+    """data_sorted 0: every pass takes catalogs in any order, the auto estimators included (their
+    corels1 scans start at j = i+1 under every image, corr.cpp:167-182)."""
     path, inp, cols = box_case(tmp_path, which=0, sorted_z=False, data_sorted=0)
     r = run_prog(path, "--plan")
     assert r.returncode == 0, r.stdout
     inputs.write_inp(path, **dict(inp, estimator=1))
     r = run_prog(path, "--plan")
-    assert r.returncode == 1 and "need data_sorted 1" in r.stdout
+    assert r.returncode == 0, r.stdout
+    cats, passes, info = parse_plan(r.stdout)
+    assert [(p["name"], p["auto"]) for p in passes][0] == ("SS", "1")
 
 
 def test_rand_subsample_include_prob(tmp_path):
