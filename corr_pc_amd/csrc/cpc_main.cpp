@@ -176,7 +176,7 @@ bool data_ini(DataInfo &d, const std::string &name) {
                 " is not supported without a periodic box (supported: 0-5 on the sky, 6 and 7 in a box)");
   if (d.which_corr == 5 && !d.periodic_box) return fail("which_corr 5 (ED) is defined for the periodic box only");
   if (d.periodic_box && d.data_sorted == 2) return fail("data_sorted==2 doesnot work with periodic box.. use 1");
-  if (d.data_sorted != 0 && d.data_sorted != 1) return fail("data_sorted must be 0 or 1");
+  if (d.data_sorted < 0 || d.data_sorted > 2) return fail("data_sorted must be 0, 1 or 2");
   if (d.do_jk != 0 && d.do_jk != 1) return fail("do_jk must be 0 or 1");
   if (d.estimator < 0 || d.estimator > 4) return fail("estimator must be 0-4");
 
@@ -214,7 +214,11 @@ bool data_ini(DataInfo &d, const std::string &name) {
       break;
   }
   // data_sorted 0 reaches the engine as cpc_params.unsorted: do_pair_unsorted (corr.cpp:36-52) ends
-  // no scan, and the auto pass starts every image's scan at j = i+1 (corr.cpp:167-182).
+  // no scan, and the auto pass starts every image's scan at j = i+1 (corr.cpp:167-182).  data_sorted
+  // 2 (the sky only) sorts by Dec: the z test then ends no scan (ret_z = 1) and the Dec test ends a
+  // scan only above the window, which a scan from j = i+1 or from the window's first galaxy reaches
+  // last (ret_dec, initialization.cpp:607-612; the Dec window is two-sided), so its pairs are the
+  // unsorted ones with catalog order = Dec order.
   d.RsRd_same = false;
   if (d.n_density_rand == 0 && d.which_corr <= 2) {
     d.RsRd_same = true;
@@ -538,6 +542,9 @@ bool read_data(DataInfo &d, Role role, Cat &c) {
   for (size_t k = 1; d.data_sorted == 1 && k < c.z.size(); ++k)
     if (c.z[k] < c.z[k - 1])
       return fail("data_sorted=1 but " + z_f + " is not sorted by redshift (row " + std::to_string(k) + ")");
+  for (size_t k = 1; d.data_sorted == 2 && k < c.y.size(); ++k)
+    if (c.y[k] < c.y[k - 1])
+      return fail("data_sorted=2 but " + p_f + " is not sorted by Dec (row " + std::to_string(k) + ")");
   return true;
 }
 
@@ -688,7 +695,7 @@ cpc_params base_params(const DataInfo &d, int precision) {
     p.z_min = d.z_min;
     p.z_sep_min = d.z_sep_min;
   }
-  p.unsorted = d.data_sorted == 0 ? 1 : 0;
+  p.unsorted = d.data_sorted == 1 ? 0 : 1;
   return p;
 }
 

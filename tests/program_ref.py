@@ -154,7 +154,9 @@ def run(inp, cols, count, out_prefix, table=None, precision=0):
             hists[name] = None
             continue
         outer, inner, p, _ = st
-        p.unsorted = int(int(inp.get("data_sorted", 1)) == 0)
+        # data_sorted 0, and 2 on the sky (Dec order: the z test ends no scan, the Dec test only
+        # past the top of its two-sided window), scan like do_pair_unsorted in catalog order
+        p.unsorted = int(int(inp.get("data_sorted", 1)) != 1)
         if not box:
             p.geometry = GEOM_SKY
             p.attach_table(table["H"][:rows], table["dA"][:rows], dz, z_max, z_sep_max)

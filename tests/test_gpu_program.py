@@ -59,6 +59,8 @@ CASES = [
     # unsorted input with the auto estimators (corels1's unsorted scans), box and sky
     ("box", 2, 3, PREC_SPLIT, {"data_sorted": 0}),
     ("sky", 0, 2, PREC_FP64, {"data_sorted": 0}),
+    # Dec-sorted sky input (data_sorted 2) with the auto and cross estimators
+    ("sky", 1, 3, PREC_SPLIT, {"data_sorted": 2}),
     # the generator notebook's box run: shape and density samples only, no randoms
     ("box", 2, 0, PREC_SPLIT, {"n_shape_rand": 0, "n_density_rand": 0}),
     ("sky", 1, 0, PREC_SPLIT, {}),
@@ -106,7 +108,8 @@ def test_program_matches_restatement(tmp_path, oracle_lib, geom, which, estimato
         # which_corr <= 2, initialization.cpp:189-192): give them density randoms
         n_rd = 1800 if (estimator == 4 or extra.get("rand_subsample") or which >= 3) else 0
         path, inp, cols, table = sky_case(tmp_path, which=which, estimator=estimator, n_rd=n_rd,
-                                          sorted_z=extra.get("data_sorted", 1) == 1, **extra)
+                                          sorted_z=extra.get("data_sorted", 1) == 1,
+                                          sort_dec=extra.get("data_sorted", 1) == 2, **extra)
     _run_program(path, precision)
     ref_prefix = str(tmp_path / "ref" / "out_")
     os.makedirs(tmp_path / "ref")

@@ -87,7 +87,8 @@ def box_case(tmp_path, which=2, estimator=0, n_s=1500, n_d=1200, n_rs=2500, n_rd
     return path, inp, cols
 
 
-def sky_case(tmp_path, which=1, estimator=0, n_s=1200, n_d=1000, n_rs=2400, n_rd=0, seed=5, sorted_z=True, **extra):
+def sky_case(tmp_path, which=1, estimator=0, n_s=1200, n_d=1000, n_rs=2400, n_rd=0, seed=5, sorted_z=True,
+             sort_dec=False, **extra):
     """A small survey field: RA 10-14 deg, Dec -2..2 deg, z in (0.2, 0.5), with a flat LCDM table."""
     rng = np.random.default_rng(seed)
     table = synth.distance_table(0.3, 1e-4, 0.5)
@@ -99,6 +100,9 @@ def sky_case(tmp_path, which=1, estimator=0, n_s=1200, n_d=1000, n_rs=2400, n_rd
         z = rng.uniform(0.2, 0.5, n)
         if sorted_z:
             z = np.sort(z)
+        if sort_dec:
+            o = np.argsort(dec, kind="stable")
+            ra, dec, z = ra[o], dec[o], z[o]
         jk = (np.minimum(((ra - 10.0) / 2.0).astype(int), 1) * 2 + np.minimum(((dec + 2.0) / 2.0).astype(int), 1))
         c = {"x": ra, "y": dec, "z": z, "jk": jk, "w": rng.lognormal(0, 0.3, n) if with_w else None}
         if with_e and kappa_role(which, role):
