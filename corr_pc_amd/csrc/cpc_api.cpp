@@ -1143,7 +1143,8 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     const double pabs = std::max(std::fabs(p.pi_min), std::fabs(p.pi_max));
     gz = eps_z / dv.psize + 8.0 * (p.n_pi + 2 * pabs / dv.psize + 2) * two24;
   }
-  P.use_split = P.split && (sky || gz < 0.25);
+  // r-mu passes classify every candidate in FP64: the FP64 kernel with the r-mu code compiled in
+  P.use_split = P.split && (sky || gz < 0.25) && (sky || !p.r_mu);
 
   if (!sky) {
     KArgs K{};
@@ -1301,7 +1302,10 @@ Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
     size_t lds, first, n;
   };
   std::map<std::pair<int, int>, std::vector<size_t>> groups;  // (kind, split) -> plans
-  for (size_t i = 0; i < plans.size(); ++i) groups[{plans[i]->kind_code, plans[i]->use_split ? 1 : 0}].push_back(i);
+  // (kind, split) -> plans; r-mu box passes run their own instantiation (kind + kRmuKind)
+  for (size_t i = 0; i < plans.size(); ++i)
+    groups[{plans[i]->kind_code + (!plans[i]->sky && plans[i]->p.r_mu ? kRmuKind : 0), plans[i]->use_split ? 1 : 0}]
+        .push_back(i);
   std::vector<Item> all;
   std::vector<Launch> launches;
   for (int swap = 0; swap < 2; ++swap)

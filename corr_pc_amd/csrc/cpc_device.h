@@ -229,6 +229,7 @@ __device__ __forceinline__ int rp_bin64(const HotArgs &P, const double *thr, dou
 // The reference's candidate test and binning for outer (x1,y1,z1), inner (x2,y2,z2) seen through
 // image (PBx, PBy, PBz): do_pair_sorted (corr.cpp:67-86) then corel_bin (corels.cpp:21-46), with
 // r_mu_calc (calcs.cpp:135-140) for r-mu bins.  Returns the histogram cell n1*n_pi + n2, or -1.
+template <bool RMU>
 __device__ __forceinline__ int exact_cell(const HotArgs &P, const double *thr, double x1, double y1, double z1,
                                           double x2, double y2, double z2, int PBx, int PBy, int PBz) {
   const double L = P.L;
@@ -241,7 +242,7 @@ __device__ __forceinline__ int exact_cell(const HotArgs &P, const double *thr, d
   if (xx < x1 - P.rmax || xx > x1 + P.rmax) return -1;
   const double t0 = x1 - xx, t1 = y1 - yy;
   double d_P = zz - z1, s = t0 * t0 + t1 * t1;
-  if (P.rmu) {
+  if constexpr (RMU) {
     // rp rounded first, then r = sqrt(rp*rp + pi*pi); the thresholds bin sqrt of their argument
     const double rp = sqrt(s);
     s = rp * rp + d_P * d_P;

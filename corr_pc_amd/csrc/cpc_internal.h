@@ -213,6 +213,8 @@ hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *ko
 size_t pair_lds_bytes(int ncell, int nslots, int n_rp, int ncopy = 1);
 // One launch over `n_items` items whose KArgs live in the device table `jobs` (all with the same
 // kind, precision and pass direction); `lds` covers the largest histogram among them.
+// kind + kRmuKind selects the r-mu instantiation (FP64 only) of the kind.
+constexpr int kRmuKind = 16;
 hipError_t launch_pair(int kind, bool split, bool swap, const KArgs *jobs, const Item *items, int n_items,
                        size_t lds, int threads, hipStream_t st);
 // Record arrays of one side of a sky pass.

@@ -47,7 +47,9 @@ __device__ __forceinline__ unsigned long long popc_ballot(bool b) { return (unsi
 
 using CKArgs = __attribute__((address_space(4))) const KArgs;
 
-template <int K, bool SPLIT, bool SWAP>
+// RMU compiles in the r-mu binning of coordinates 7 (with SPLIT false only: r-mu candidates all take
+// the FP64 path); without it the rp-pi kernels keep their registers and schedule.
+template <int K, bool SPLIT, bool SWAP, bool RMU>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT ? 4 : 2))) pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
   constexpr int NV = KindNV<K>::NV;
   constexpr int NE = KindNV<K>::NE;
@@ -201,7 +203,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
                 }
                 int cell = -1;
                 bool unc = ok;
-                if (SPLIT && ok && !H.rmu) {  // r-mu candidates all take the FP64 path
+                if (SPLIT && ok) {
                   const float dx = rl_f(cq.x, l3) + ax;
                   const float dy = rl_f(cq.y, l3) + ay;
                   const float dz = rl_f(cq.z, l3) + az;
@@ -237,9 +239,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
                                                       __builtin_amdgcn_readlane(__double2loint(cz), t));
                   if (unc) {
                     if (!SWAP)
-                      cell = exact_cell(H, lthr, me.x, me.y, mz, tx, ty, tzz, PBx, PBy, PBz);
+                      cell = exact_cell<RMU>(H, lthr, me.x, me.y, mz, tx, ty, tzz, PBx, PBy, PBz);
                     else
-                      cell = exact_cell(H, lthr, tx, ty, tzz, me.x, me.y, mz, PBx, PBy, PBz);
+                      cell = exact_cell<RMU>(H, lthr, tx, ty, tzz, me.x, me.y, mz, PBx, PBy, PBz);
                   }
                 }
                 if (__ballot(cell >= 0) != 0ull) {
@@ -458,10 +460,10 @@ hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *ko
   return sort_pairs_impl<double>(tmp, tmp_bytes, kin, kout, vin, vout, n, 64, st);
 }
 
-template <int K, bool SPLIT, bool SWAP>
+template <int K, bool SPLIT, bool SWAP, bool RMU = false>
 static hipError_t launch_pair_t(const KArgs *jobs, const Item *items, int n_items, int threads, size_t lds,
                                 hipStream_t st) {
-  auto fn = pair_kernel<K, SPLIT, SWAP>;
+  auto fn = pair_kernel<K, SPLIT, SWAP, RMU>;
   hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   fn<<<n_items, threads, lds, st>>>(jobs, items);
@@ -476,8 +478,12 @@ size_t pair_lds_bytes(int ncell, int nslots, int n_rp, int ncopy) {
 hipError_t launch_pair(int kind, bool split, bool swap, const KArgs *jobs, const Item *items, int n_items,
                        size_t lds, int threads, hipStream_t st) {
   if (n_items == 0) return hipSuccess;
+  const bool rmu = kind >= kRmuKind;
+  if (rmu) kind -= kRmuKind;
 #define CPC_DISPATCH(KK)                                                                          \
   if (kind == KK) {                                                                               \
+    if (rmu && swap) return launch_pair_t<KK, false, true, true>(jobs, items, n_items, threads, lds, st); \
+    if (rmu) return launch_pair_t<KK, false, false, true>(jobs, items, n_items, threads, lds, st);        \
     if (split && swap) return launch_pair_t<KK, true, true>(jobs, items, n_items, threads, lds, st); \
     if (split) return launch_pair_t<KK, true, false>(jobs, items, n_items, threads, lds, st);        \
     if (swap) return launch_pair_t<KK, false, true>(jobs, items, n_items, threads, lds, st);         \
