@@ -166,6 +166,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
             const bool lane_ok = valid && (SWAP || image_allowed(me.meta, PBx, PBy, PBz));
             // the inner galaxy must come after the outer (sorted: images with PBz <= 0; unsorted: all)
             const bool ordered = H.auto_mode == 2 || (H.auto_mode == 1 && PBz <= 0);
+            const bool unsorted_off = H.auto_mode != 2;
             // FP32 offsets of this lane to the tile column: d = tile local + offset
             const float ax = Ox - me.xl, ay = Oy - me.yl, az = (float)(H.L * sz) - me.zf;
             const float pinv = SWAP ? -H.inv_psize32 : H.inv_psize32;  // pi = -dz in the touch pass
@@ -184,17 +185,14 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
                 const int l3 = 4 * t + 3;  // lane of the quarter {xl, yl, zf, meta}
                 bool ok = lane_ok;
                 if (SWAP) ok = ok && image_allowed(rl_i(cq.w, l3), PBx, PBy, PBz);
-                if (ordered && H.auto_mode == 2) {
-                  // unsorted corels1: j > i in catalog order
-                  const int tix = P.tidx[jb + t];
-                  ok = ok && (SWAP ? midx > tix : tix > midx);
-                } else if (ordered) {
-                  // corels1 scans j > i in catalog order, which is (z, index) order for its z-sorted input
+                if (ordered) {
+                  // corels1 scans j > i in catalog order: (z, index) order for its z-sorted input, the
+                  // index alone for unsorted input (every pair then goes to the index comparison)
                   const double tzo = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(cz), t),
                                                       __builtin_amdgcn_readlane(__double2loint(cz), t));
                   const double zo = SWAP ? tzo : mz, zi = SWAP ? mz : tzo;
-                  bool after = zi > zo;
-                  const bool tie = zi == zo;
+                  bool after = zi > zo && unsorted_off;
+                  const bool tie = zi == zo || !unsorted_off;
                   if (__ballot(ok && tie) != 0ull) {
                     const int tix = P.tidx[jb + t];
                     after = after || (tie && (SWAP ? midx > tix : tix > midx));
