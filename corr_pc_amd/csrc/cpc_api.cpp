@@ -360,6 +360,7 @@ struct JobBufs {
   DevBuf<Win> win_raw;
   DevBuf<double2> reach;
   DevBuf<uint2> hq;
+  DevBuf<int32_t> red_beg, red_lst;  // box: target lists of the device-side reduction
   void release() {
     so.release();
     si.release();
@@ -370,6 +371,8 @@ struct JobBufs {
     win_raw.release();
     reach.release();
     hq.release();
+    red_beg.release();
+    red_lst.release();
   }
 };
 
@@ -394,6 +397,9 @@ struct cpc_ctx {
   DevBuf<KArgs> jobs;
   DevBuf<double> acc;
   DevBuf<unsigned long long> accn;
+  // raw pass targets that are reduced on the device before the download (box passes with jackknife)
+  DevBuf<double> acc_raw;
+  DevBuf<unsigned long long> accn_raw;
   std::vector<std::unique_ptr<JobBufs>> pool;
   std::map<std::tuple<double, double, int, int, int>, std::vector<double>> thr_cache;
 };
@@ -573,6 +579,100 @@ void make_items(const Side &B, bool touch, bool jk_on, int per_item, Cost nbr_co
   }
 }
 
+// Work items of a box pass, with the jackknife decomposed by the tile side so that every pair is
+// evaluated once.  A pair of an outer galaxy (region a, coin ci) and an inner galaxy (region b, coin
+// cj) touches region a when b == a or ci >= 0.5, and region b when b != a and cj >= 0.5
+// (corel_jk, jk.cpp:201-223).  Each item takes block chunks of one (a, ci) and the tile groups of one
+// filter, which fixes both flags, and counts into one target:
+//   ci = 1: tiles of region a or coin 0      -> target 2a+1      (touches a only)
+//           tiles of region b != a, coin 1   -> target Y(a, 1, b) (touches a and b)
+//   ci = 0: tiles of region a                -> target 2a+1      (touches a only)
+//           tiles of other regions, coin 0   -> target 2a        (touches nothing)
+//           tiles of region b != a, coin 1   -> target Y(a, 0, b) (touches b only)
+// Y targets follow the 2 nreg others in first-use order; `ytar` receives (a, x, b) of each, x the
+// flag of touching a.  Without the jackknife every item takes every tile group into target 1.
+// Chunks of one (target, filter) are sorted by cost and packed `per_item` at a time; items run
+// most expensive first.  The cost of a chunk is its galaxies times the tile galaxies its filter
+// takes in the neighbour columns.
+void make_items_box(const Side &B, const Side &T, bool jk_on, int nreg, int per_item, int nc,
+                    const std::vector<int2> &nbr, std::vector<Item> &items, std::vector<double> &costs,
+                    std::vector<Chunk> &chunks, std::vector<int3> &ytar) {
+  struct Key {
+    int target, mode, reg;
+    bool operator<(const Key &o) const { return std::tie(target, mode, reg) < std::tie(o.target, o.mode, o.reg); }
+  };
+  std::map<Key, std::vector<std::pair<double, Chunk>>> by_key;
+  std::map<std::tuple<int, int, int>, int> yidx;
+  ytar.clear();
+  std::map<Key, double> reach;
+  for (size_t g = 0; g < B.groups.size(); ++g) {
+    const Group &G = B.groups[g];
+    const int a = G.region, ci = G.coin;
+    const int bcx = G.col % nc, bcy = G.col / nc;
+    reach.clear();
+    for (const int2 &o : nbr) {
+      const int ux = ((bcx + o.x) % nc + nc) % nc, uy = ((bcy + o.y) % nc + nc) % nc;
+      const int col = uy * nc + ux;
+      for (int tg = T.colbeg[col]; tg < T.colbeg[col + 1]; ++tg) {
+        const Group &H = T.groups[tg];
+        const double cnt = (double)(H.end - H.start);
+        Key k{};
+        if (!jk_on) {
+          k = Key{1, kTileAll, 0};
+        } else if (H.region == a || H.coin == 0) {
+          if (ci)
+            k = Key{2 * a + 1, kTileSameOrC0, 0};
+          else
+            k = H.region == a ? Key{2 * a + 1, kTileSame, 0} : Key{2 * a, kTileOtherC0, 0};
+        } else {
+          const auto yk = std::make_tuple(a, ci, H.region);
+          auto it = yidx.find(yk);
+          if (it == yidx.end()) {
+            it = yidx.emplace(yk, (int)ytar.size()).first;
+            ytar.push_back(make_int3(a, ci, H.region));
+          }
+          k = Key{2 * nreg + it->second, kTileRegionC1, H.region};
+        }
+        reach[k] += cnt;
+      }
+    }
+    for (const auto &kv : reach) {
+      auto &v = by_key[kv.first];
+      for (int f = G.start; f < G.end; f += 64) {
+        const Chunk c{(int32_t)g, f, std::min(64, G.end - f), 0};
+        v.push_back({(double)c.count * kv.second, c});
+      }
+    }
+  }
+  struct Packed {
+    double cost;
+    Key key;
+    std::vector<Chunk> cs;
+  };
+  std::vector<Packed> packed;
+  for (auto &kv : by_key) {
+    auto &v = kv.second;
+    std::stable_sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+    for (size_t a = 0; a < v.size(); a += per_item) {
+      Packed pk{0.0, kv.first, {}};
+      for (size_t b = a; b < std::min(v.size(), a + per_item); ++b) {
+        pk.cost += v[b].first;
+        pk.cs.push_back(v[b].second);
+      }
+      packed.push_back(std::move(pk));
+    }
+  }
+  std::stable_sort(packed.begin(), packed.end(), [](const Packed &x, const Packed &y) { return x.cost > y.cost; });
+  items.clear();
+  costs.clear();
+  for (const Packed &pk : packed) {
+    items.push_back(Item{(int32_t)chunks.size(), (int32_t)pk.cs.size(), pk.key.target, pk.cs[0].group, 0,
+                         pk.key.mode, pk.key.reg, 0});
+    costs.push_back(pk.cost);
+    chunks.insert(chunks.end(), pk.cs.begin(), pk.cs.end());
+  }
+}
+
 // Contributions binned on the host, in the reference layout: region 0 the full sample (all
 // fields), regions 1..n_jk the touch sums (num, wt_num, data[].val).
 struct HostHist {
@@ -740,6 +840,12 @@ struct Plan {
   std::vector<double2> reach;
   // work
   std::vector<Item> items_p, items_t;
+  std::vector<int3> ytar;  // box: (outer region, touches-outer flag, inner region) of targets 2 nreg + k
+  // box with jackknife: the nP raw targets stay on the device and are reduced there into
+  // 1 + n_jk downloaded histograms, the full sample and touch_r (lists rbeg/rlst)
+  bool reduce = false;
+  std::vector<int32_t> rbeg, rlst;
+  size_t off_raw = 0, off_rawn = 0;
   std::vector<double> cost_p, cost_t;
   KArgs kp{}, kt{};  // box: primary and touch pass (accumulator pointers set at launch)
   SkyArgs sa{};      // sky: arguments shared by both passes
@@ -925,7 +1031,9 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     choose(P.NSP, P.row0_p, P.ncopy_p);
     choose(P.NST, P.row0_t, P.ncopy_t);
   }
-  const int item_size = ctx->item_size ? ctx->item_size : 512;
+  // block galaxies per work item: four chunks per wave of the group (waves claim chunks as they
+  // finish, so a few chunks each keep the group's waves busy until close to its end)
+  const int item_size = ctx->item_size ? ctx->item_size : (sky ? 512 : 4 * P.threads);
   const int64_t nbig = std::max(co->n, ci->n);
 
   // ---- geometry-specific set-up: grid, neighbour cells, windows
@@ -1094,21 +1202,33 @@ Status setup(cpc_ctx *ctx, Plan &P) {
   // ---- work items (cost: block galaxies x tile galaxies in reach)
   const int per_item = std::max(1, item_size / 64);
   if (!sky) {
-    const int nc = G.ncx;
-    const std::vector<int2> &nbr = P.nbr;
-    auto cost_of = [&](const Side &T) {
-      return [&T, &nbr, nc](const Group &g) {
-        const int bcx = g.col % nc, bcy = g.col / nc;
-        double c = 0;
-        for (const int2 &o : nbr) {
-          const int ux = ((bcx + o.x) % nc + nc) % nc, uy = ((bcy + o.y) % nc + nc) % nc;
-          c += (double)T.colcount[(size_t)uy * nc + ux];
-        }
-        return c;
-      };
-    };
-    make_items(SO, false, jk_on, per_item, cost_of(SI), P.items_p, P.cost_p, P.chunks);
-    if (jk_on) make_items(SI, true, jk_on, per_item, cost_of(SO), P.items_t, P.cost_t, P.chunks);
+    make_items_box(SO, SI, jk_on, nreg, per_item, G.ncx, P.nbr, P.items_p, P.cost_p, P.chunks, P.ytar);
+    P.nP = (size_t)2 * nreg + P.ytar.size();
+    P.nT = 0;
+    if (jk_on) {
+      // output 0: every target; output r+1: the targets whose pairs touch region r (2r+1 and the
+      // Y targets through the outer galaxy, x = 1, or the inner one, b = r)
+      std::vector<std::vector<int32_t>> lists(1 + nreg);
+      for (int t = 0; t < (int)P.nP; ++t) lists[0].push_back(t);
+      for (int r = 0; r < nreg; ++r) lists[1 + r].push_back(2 * r + 1);
+      for (size_t k = 0; k < P.ytar.size(); ++k) {
+        const int3 y = P.ytar[k];
+        if (y.y) lists[1 + y.x].push_back(2 * nreg + (int)k);
+        lists[1 + y.z].push_back(2 * nreg + (int)k);
+      }
+      P.rbeg.assign(1, 0);
+      P.rlst.clear();
+      for (const auto &l : lists) {
+        P.rlst.insert(P.rlst.end(), l.begin(), l.end());
+        P.rbeg.push_back((int32_t)P.rlst.size());
+      }
+      if (check(B.red_beg.ensure(P.rbeg.size()), st, "alloc") || check(B.red_lst.ensure(P.rlst.size()), st, "alloc"))
+        return st;
+      check(hipMemcpyAsync(B.red_beg.p, P.rbeg.data(), P.rbeg.size() * 4, hipMemcpyHostToDevice, s), st, "copy");
+      check(hipMemcpyAsync(B.red_lst.p, P.rlst.data(), P.rlst.size() * 4, hipMemcpyHostToDevice, s), st, "copy");
+      if (st.code) return st;
+      P.reduce = true;
+    }
   } else {
     auto flat = [](const Group &) { return 1.0; };
     make_items(SO, false, jk_on, per_item, flat, P.items_p, P.cost_p, P.chunks);
@@ -1131,6 +1251,7 @@ Status setup(cpc_ctx *ctx, Plan &P) {
   const double two24 = std::ldexp(1.0, -24);
   double gz = 1.0;
   Guard gr{};
+  float cull_r2 = INFINITY, cull_d = 0.0f;
   if (!sky) {
     // guard bands of the FP32 classification from bounds on the FP32 error of dx, dy (column-local
     // coordinates plus a rounded column offset) and of dz (absolute coordinates)
@@ -1139,6 +1260,13 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     const double omax = (Rn + 1) * h + 1e-9 * L;
     const double eps_xy = 2.0 * (6 * amax + 2 * omax) * two24;
     gr = rp_guard(p, dv, thr, eps_xy);
+    // tile cull radius: the neighbour-column cut (no pair beyond it reaches a bin), widened by the
+    // FP32 rounding of the squares and their sum; cull_d covers the rounding of the tile's and the
+    // lanes' column-relative coordinates and of the axis distances (four roundings of magnitude
+    // at most omax + amax), doubled
+    const double rcut_c = std::sqrt(thr[p.n_rp]) * (1 + 1e-9) + 1e-9 * L;
+    cull_r2 = round_up_f(rcut_c * rcut_c * (1 + 1e-6));
+    cull_d = round_up_f(2.0 * 4 * (omax + amax) * two24);
     const double eps_z = 2.0 * 5 * L * two24;
     const double pabs = std::max(std::fabs(p.pi_min), std::fabs(p.pi_max));
     gz = eps_z / dv.psize + 8.0 * (p.n_pi + 2 * pabs / dv.psize + 2) * two24;
@@ -1180,6 +1308,8 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     K.gB = gr.gB;
     K.gC = gr.gC;
     K.gD = gr.gD;
+    K.cull_r2 = cull_r2;
+    K.cull_d = cull_d;
     K.nbr = B.nbr.p;
     K.auto_mode = autop ? (p.unsorted ? 2 : 1) : 0;
     K.chunks = B.chunks.p;
@@ -1286,8 +1416,8 @@ Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
     Plan &P = *plans[i];
     KArgs &a = table[2 * i];
     a = P.kp;
-    a.acc = ctx->acc.p + P.off_p;
-    a.accnum = ctx->accn.p + P.off_pn;
+    a.acc = P.reduce ? ctx->acc_raw.p + P.off_raw : ctx->acc.p + P.off_p;
+    a.accnum = P.reduce ? ctx->accn_raw.p + P.off_rawn : ctx->accn.p + P.off_pn;
     a.counters = ctx->accn.p + P.off_c;
     KArgs &b = table[2 * i + 1];
     b = P.kt;
@@ -1348,6 +1478,13 @@ Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
       check(launch_pair(l.kind, l.split, true, ctx->jobs.p, ctx->items.p + l.first, (int)l.n, l.lds, l.threads, s),
             st, "touch kernel");
   check(hipEventRecord(ctx->ev1, s), st, "event");
+  for (Plan *Pp : plans) {  // raw targets into the full sample and touch histograms
+    const Plan &P = *Pp;
+    if (!P.reduce) continue;
+    ReduceArgs ra{ctx->acc_raw.p + P.off_raw, ctx->accn_raw.p + P.off_rawn, ctx->acc.p + P.off_p,
+                  ctx->accn.p + P.off_pn, P.B->red_beg.p, P.B->red_lst.p, 1 + P.nreg, P.NSP, P.ncell};
+    check(launch_reduce(ra, s), st, "reduce");
+  }
   for (Plan *Pp : plans) {  // sparse-window diagnostic (corels2's starts pointer)
     const Plan &P = *Pp;
     if (P.autop || P.co->n == 0 || P.ci->n == 0) continue;
@@ -1492,8 +1629,9 @@ int64_t assemble(const Plan &P, const std::vector<double> &acc, const std::vecto
   std::fill(out->val, out->val + cells_all * CPC_NFIELD, 0.0);
   std::fill(out->err, out->err + cells_all * CPC_NFIELD, 0.0);
   std::fill(out->num_rp, out->num_rp + (size_t)R * p.n_rp, 0);
-  // full sample = every primary target (+ host-binned pairs)
-  for (int t = 0; t < 2 * nreg; ++t)
+  // full sample = every primary target (+ host-binned pairs); reduced on the device into target 0
+  const int n_full = P.reduce ? 1 : (int)P.nP;
+  for (int t = 0; t < n_full; ++t)
     for (int c = 0; c < ncell; ++c) {
       const unsigned long long n = accpn[(size_t)t * ncell + c];
       if (!n) continue;
@@ -1517,16 +1655,27 @@ int64_t assemble(const Plan &P, const std::vector<double> &acc, const std::vecto
   if (P.jk_on) {
     std::vector<double> tv((size_t)ncell * CPC_NFIELD), tw(ncell);
     std::vector<int64_t> tn(ncell);
+    // the targets that touch each region: primary target 2r+1, the box's Y targets whose pairs
+    // touch r (through the outer galaxy, x = 1, or the inner one, b = r), the sky's touch target r
+    std::vector<std::vector<std::pair<int, int>>> touch_of(p.n_jk);  // (arena 0 primary / 1 touch, target)
+    for (int r = 0; r < p.n_jk; ++r) {
+      if (P.reduce) {
+        touch_of[r].push_back({0, 1 + r});  // reduced on the device
+        continue;
+      }
+      touch_of[r].push_back({0, 2 * r + 1});
+      if (P.nT) touch_of[r].push_back({1, r});
+    }
     for (int r = 0; r < p.n_jk; ++r) {
       // touch_r (jk.cpp:138-171): num, wt_num and data[].val; shape vals only when which_corr != 0
       std::fill(tv.begin(), tv.end(), 0.0);
       std::fill(tw.begin(), tw.end(), 0.0);
       std::fill(tn.begin(), tn.end(), 0);
-      for (int src = 0; src < 2; ++src) {
+      for (const auto &src_t : touch_of[r]) {
+        const int src = src_t.first, t = src_t.second;
         const double *a = src == 0 ? accp : acct;
         const unsigned long long *num = src == 0 ? accpn : acctn;
         const int NS = src == 0 ? NSP : NST;
-        const int t = src == 0 ? 2 * r + 1 : r;
         for (int c = 0; c < ncell; ++c) {
           const unsigned long long n = num[(size_t)t * ncell + c];
           if (!n) continue;
@@ -1611,14 +1760,21 @@ cpc_status run_jobs(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *to
     if (st.code) return fail(ctx, tag(i, st));
   }
   // accumulator arenas: per plan [primary f64 | touch f64] and [primary counts | touch counts | 4 counters]
-  size_t nd = 0, nu = 0;
+  size_t nd = 0, nu = 0, nr = 0, nrn = 0;
   for (Plan &P : plans) {
+    const size_t nP_dl = P.reduce ? (size_t)(1 + P.nreg) : P.nP;  // targets downloaded
+    if (P.reduce) {
+      P.off_raw = nr;
+      nr += P.nP * P.NSP * P.ncell;
+      P.off_rawn = nrn;
+      nrn += P.nP * P.ncell;
+    }
     P.off_p = nd;
-    nd += P.nP * P.NSP * P.ncell;
+    nd += nP_dl * P.NSP * P.ncell;
     P.off_t = nd;
     nd += P.nT * P.NST * P.ncell;
     P.off_pn = nu;
-    nu += P.nP * P.ncell;
+    nu += nP_dl * P.ncell;
     P.off_tn = nu;
     nu += P.nT * P.ncell;
     P.off_c = nu;
@@ -1631,6 +1787,12 @@ cpc_status run_jobs(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *to
     return fail(ctx, st);
   if (nd) check(hipMemsetAsync(ctx->acc.p, 0, nd * 8, s), st, "memset");
   if (nu) check(hipMemsetAsync(ctx->accn.p, 0, nu * 8, s), st, "memset");
+  if (nr) {
+    if (check(ctx->acc_raw.ensure(nr), st, "alloc") || check(ctx->accn_raw.ensure(nrn), st, "alloc"))
+      return fail(ctx, st);
+    check(hipMemsetAsync(ctx->acc_raw.p, 0, nr * 8, s), st, "memset");
+    check(hipMemsetAsync(ctx->accn_raw.p, 0, nrn * 8, s), st, "memset");
+  }
   if (st.code) return fail(ctx, st);
   std::vector<Plan *> box;
   for (Plan &P : plans)
@@ -1960,7 +2122,7 @@ cpc_status cpc_autotune(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_tunin
   // candidate values per knob; the sky kernel runs at most 256 threads per group
   const std::vector<double> fracs = any_sky && !any_box ? std::vector<double>{0.0, 0.35, 0.75}
                                                         : std::vector<double>{0.0, 0.35, 0.5};
-  const std::vector<int> items = {0, 256, 1024};
+  const std::vector<int> items = any_sky && !any_box ? std::vector<int>{0, 256, 1024} : std::vector<int>{0, 512, 4096};
   const std::vector<int> threads = any_sky ? std::vector<int>{0, 128} : std::vector<int>{0, 256};
   cpc_tuning cur{0.0, 0, 0};
   double cur_ms = 0;
@@ -2007,6 +2169,26 @@ cpc_status cpc_autotune(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_tunin
   *best = cur;
   if (tried) *tried = runs;
   return rc;
+}
+
+cpc_status cpc_fp32_error(cpc_ctx *ctx, float s_lo, float s_hi, double out[2]) {
+  if (!ctx || !out) return fail(ctx, CPC_ERR_INVALID, "null argument");
+  if (!(s_lo > 0) || !(s_hi >= s_lo) || !std::isfinite(s_hi))
+    return fail(ctx, CPC_ERR_INVALID, "cpc_fp32_error needs 0 < s_lo <= s_hi < inf");
+  Status st;
+  uint32_t b0, b1;
+  std::memcpy(&b0, &s_lo, 4);
+  std::memcpy(&b1, &s_hi, 4);
+  DevBuf<unsigned long long> d;
+  if (check(d.ensure(2), st, "alloc")) return fail(ctx, st);
+  unsigned long long h[2] = {0, 0};
+  check(hipMemcpyAsync(d.p, h, sizeof(h), hipMemcpyHostToDevice, ctx->stream), st, "copy");
+  check(launch_fp32_error(b0, b1 - b0 + 1, d.p, ctx->stream), st, "fp32 error kernel");
+  check(hipMemcpyAsync(h, d.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream), st, "copy");
+  check(hipStreamSynchronize(ctx->stream), st, "sync");
+  if (st.code) return fail(ctx, st);
+  for (int k = 0; k < 2; ++k) std::memcpy(&out[k], &h[k], 8);
+  return CPC_OK;
 }
 
 }  // extern "C"

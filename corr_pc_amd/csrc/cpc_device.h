@@ -212,6 +212,7 @@ struct HotArgs {
   int lbase, lcell;  // first LDS cell (row0 * n_pi) and LDS cells per copy
   double wlo, whi;   // z window offsets of an outer galaxy
   int rmu;           // r-mu bins (coordinates 7)
+  float cull_r2, cull_d;  // tile cull against the wave's bounding box
 };
 
 // rp bin of a squared separation against the FP64 thresholds thr[0..n_rp]: bin k iff

@@ -56,9 +56,20 @@ struct alignas(16) Chunk {
 // A unit of work (one workgroup): chunks [begin, begin + n) of its job's chunk table, all from
 // groups of the same jackknife region and coin, so that they share one histogram target; group0 is
 // one of those groups.  `job` selects the pass arguments (KArgs) in the launch's job table.
+// Box items also select the tile groups they meet (`tmode`, `treg`, see TileFilter), so that every
+// pair an item counts belongs to its one target.
 struct alignas(16) Item {
   int32_t begin, n, target, group0;
-  int32_t job, pad0, pad1, pad2;
+  int32_t job, tmode, treg, pad;
+};
+
+// Tile-group filter of a box item whose block galaxies lie in region a:
+enum TileFilter : int32_t {
+  kTileAll = 0,         // every tile group (no jackknife)
+  kTileSame = 1,        // region a
+  kTileOtherC0 = 2,     // another region, coin < 0.5
+  kTileSameOrC0 = 3,    // region a, or coin < 0.5
+  kTileRegionC1 = 4,    // region treg (!= a), coin >= 0.5
 };
 
 // Histogram slots per pair kind.  The primary pass accumulates [wt, vals..., errs...], the touch
@@ -100,6 +111,10 @@ struct KArgs {
   float s_out_lo, s_out_hi;  // FP32 s below s_out_lo or from s_out_hi on is outside every rp bin
   float s_cert_lo;           // FP32 s below this is never classified in FP32
   float gA, gB, gC, gD;      // guard band of the rp bin index: g = gA + (gB + gC/r)/r + gD r, r = sqrt(s)
+  // tile cull (both precisions): a tile whose FP32 column-relative position lies farther than
+  // sqrt(cull_r2) from the bounding box of a wave's block galaxies, after taking cull_d off each axis
+  // distance, cannot pair with any of them; cull_d bounds the FP32 error of those distances
+  float cull_r2, cull_d;
   const int2 *nbr;     // neighbour column offsets
   const Chunk *chunks;
   const Group *bgrp;   // groups of the block side
@@ -201,11 +216,24 @@ struct SparseArgs {
   double L, pmin, pmax, inc_max;
 };
 
+// Sums of raw pass targets into output histograms: output o is the sum of targets
+// lst[beg[o] .. beg[o+1]) in list order, for the counts and every f64 slot.
+struct ReduceArgs {
+  const double *acc;                // raw targets [n_target][ns][ncell]
+  const unsigned long long *accn;   // raw counts [n_target][ncell]
+  double *out;                      // [n_out][ns][ncell]
+  unsigned long long *outn;         // [n_out][ncell]
+  const int32_t *beg, *lst;
+  int n_out, ns, ncell;
+};
+
 // Launch helpers implemented in cpc_kernels.hip.
 hipError_t launch_keys(const KeyArgs &A, const int32_t *order, uint32_t *keys, int32_t *vals, hipStream_t st);
 hipError_t launch_gather(const GatherArgs &A, const int32_t *sidx, PRec *rec, double *zs, int32_t *idx,
                          hipStream_t st);
 hipError_t launch_sparse(const SparseArgs &A, unsigned long long *count, hipStream_t st);
+hipError_t launch_fp32_error(uint32_t b0, uint32_t n, unsigned long long *out, hipStream_t st);
+hipError_t launch_reduce(const ReduceArgs &A, hipStream_t st);
 hipError_t sort_u32(void *&tmp, size_t &tmp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
                     int32_t *vout, int64_t n, hipStream_t st);
 hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *kout, const int32_t *vin,

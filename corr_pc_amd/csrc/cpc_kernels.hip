@@ -44,6 +44,16 @@ __device__ __forceinline__ double rl_d(unsigned lo, unsigned hi, int l) {
   return __hiloint2double((int)__builtin_amdgcn_readlane(hi, l), (int)__builtin_amdgcn_readlane(lo, l));
 }
 __device__ __forceinline__ unsigned long long popc_ballot(bool b) { return (unsigned long long)__popcll(__ballot(b)); }
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
 
 using CKArgs = __attribute__((address_space(4))) const KArgs;
 
@@ -64,7 +74,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
   // reloading them inside the loop, where each reload would also wait for the LDS atomics in flight
   const HotArgs H{P.L,         P.pmin, P.pmax,  P.rmax, P.psize, P.gz, P.pioff32, P.s_out_lo, P.s_out_hi,
                   P.s_cert_lo, P.linv, P.lrmin, P.la, P.lb, P.gA, P.gB, P.gC, P.gD, P.inv_psize32, P.n_pi, P.n_rp,
-                  P.lin, P.gws, P.auto_mode, P.row0 * P.n_pi, (P.n_rp - P.row0) * P.n_pi, P.wlo, P.whi, P.rmu};
+                  P.lin, P.gws, P.auto_mode, P.row0 * P.n_pi, (P.n_rp - P.row0) * P.n_pi, P.wlo, P.whi, P.rmu,
+                  P.cull_r2, P.cull_d};
   const PRec *__restrict__ brec = P.brec;
   const double *__restrict__ bz = P.bz;
   const PRec *__restrict__ trec = P.trec;
@@ -91,27 +102,27 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
   for (int k = tid; k <= H.n_rp; k += blockDim.x) lthr[k] = P.thr[k];
 
   const Group bg = P.bgrp[it.group0];  // region and coin of every chunk of the item
-  // phase_mode: 0 primary without jackknife, 1 primary with jackknife, 2 touch pass
-  const bool two_phase = (P.phase_mode == 1) && (bg.coin == 0);
-  const int nphase = two_phase ? 2 : 1;
   const uint4 *tq = reinterpret_cast<const uint4 *>(trec);  // records as 16-byte quarters
+  const int tmode = it.tmode, treg = it.treg;
 
   unsigned long long ncand = 0, nre = 0;  // wave totals
+  __shared__ int next_chunk;  // next chunk of the item no wave has taken
+  int next_nx = 0;
 
-  for (int ph = 0; ph < nphase; ++ph) {
-    // filter on tile region: 0 all, 1 same region as block, 2 other regions
-    const int filt = (P.phase_mode == 2) ? 2 : (two_phase ? (ph == 0 ? 1 : 2) : 0);
-    const int tgt = (P.phase_mode == 2) ? it.target : (two_phase ? it.target + (ph == 0 ? 1 : 0) : it.target);
-    unsigned long long *const gnum = P.accnum + (size_t)tgt * ncell;
-    double *const gacc = P.acc + (size_t)tgt * NS * ncell;
+  {
+    unsigned long long *const gnum = P.accnum + (size_t)it.target * ncell;
+    double *const gacc = P.acc + (size_t)it.target * NS * ncell;
     for (int c = tid; c < ncopy * lcell; c += blockDim.x) {
       lnum[c] = 0;
 #pragma unroll
       for (int s = 0; s < NS; ++s) lacc[s * ncopy * lcell + c] = 0.0;
     }
+    if (tid == 0) next_chunk = nwave;
     __syncthreads();
 
-    for (int ch = wave; ch < it.n; ch += nwave) {
+    // chunks come sorted by decreasing cost; each wave starts on its own and then takes the next
+    // unclaimed one, so the waves of the group finish close together
+    for (int ch = wave; ch < it.n; ch = uni(next_nx)) {
       const Chunk ck = P.chunks[it.begin + ch];
       const int col = uni(P.bgrp[ck.group].col);
       const int bcx = col % P.nc, bcy = col / P.nc;
@@ -124,6 +135,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
       const int midx = H.auto_mode ? P.bidx[p] : 0;
       const double wzlo = bz[base];
       const double wzhi = bz[base + cnt - 1];
+      // bounding box of the wave's block galaxies in column-local FP32 coordinates (tile cull)
+      const float bx0 = wave_min(valid ? me.xl : INFINITY), bx1 = wave_max(valid ? me.xl : -INFINITY);
+      const float by0 = wave_min(valid ? me.yl : INFINITY), by1 = wave_max(valid ? me.yl : -INFINITY);
 
       // A lane keeps the sums of its current histogram cell in registers and adds them to LDS only
       // when its cell changes: lanes of a wave hold neighbouring galaxies, so their pairs crowd into
@@ -146,8 +160,13 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
         const float Oy = (float)((double)(tcy - bcy) * P.h + H.L * sy);
         for (int g = g0; g < g1; ++g) {
           const Group tg = P.tgrp[g];
-          if (filt == 1 && tg.region != bg.region) continue;
-          if (filt == 2 && tg.region == bg.region) continue;
+          // the item's tile filter: every pair it meets belongs to its target
+          const bool same = tg.region == bg.region;
+          const bool take = tmode == kTileAll || (tmode == kTileSame && same) ||
+                            (tmode == kTileOtherC0 && !same && tg.coin == 0) ||
+                            (tmode == kTileSameOrC0 && (same || tg.coin == 0)) ||
+                            (tmode == kTileRegionC1 && tg.region == treg && tg.coin == 1);
+          if (!take) continue;
           for (int sz = -1; sz <= 1; ++sz) {
             double lo, hi;
             if (!SWAP) {
@@ -181,7 +200,17 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
                 nz = tz[jb + kBatch + (lane & (kBatch - 1))];
               }
               const int nt = min(kBatch, j1 - jb);
-              for (int t = 0; t < nt; ++t) {
+              // tiles within reach of the wave's bounding box: lane 4t+3 holds tile t's column-local
+              // {xl, yl}; the others cannot bin with any lane and are skipped by the whole wave
+              unsigned long long tmask;
+              {
+                const float X = __uint_as_float(cq.x) + Ox, Y = __uint_as_float(cq.y) + Oy;
+                const float ex = fmaxf(fmaxf(bx0 - X, X - bx1) - H.cull_d, 0.0f);
+                const float ey = fmaxf(fmaxf(by0 - Y, Y - by1) - H.cull_d, 0.0f);
+                tmask = __ballot((lane & 3) == 3 && (lane >> 2) < nt && fmaf(ex, ex, ey * ey) <= H.cull_r2);
+              }
+              for (; tmask; tmask &= tmask - 1) {
+                const int t = uni(__builtin_ctzll(tmask) >> 2);
                 const int l3 = 4 * t + 3;  // lane of the quarter {xl, yl, zf, meta}
                 bool ok = lane_ok;
                 if (SWAP) ok = ok && image_allowed(rl_i(cq.w, l3), PBx, PBy, PBz);
@@ -293,6 +322,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
         else
           lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
       }
+      int nx = 0;
+      if (lane == 0) nx = atomicAdd(&next_chunk, 1);
+      next_nx = __builtin_amdgcn_readfirstlane(nx);
     }
     __syncthreads();
     for (int c = tid; c < lcell; c += blockDim.x) {
@@ -405,8 +437,65 @@ __global__ void k_sparse(SparseArgs A, unsigned long long *count) {
   if (c) atomicAdd(count, (unsigned long long)c);
 }
 
+// The split kernels' FP32 approximations against FP64 over the FP32 values b0 .. b0+n-1 (as bit
+// patterns): out[0] the scaled log2 error, out[1] the rsq error, each a maximum of non-negative
+// doubles kept as their bit patterns (which order like the values).
+__global__ void k_fp32_error(uint32_t b0, uint32_t n, unsigned long long *out) {
+  double e_log = 0.0, e_rsq = 0.0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float s = __uint_as_float(b0 + i);
+    const double lg = log2((double)s);
+    e_log = fmax(e_log, fabs((double)__log2f(s) - lg) / fmax(1.0, fabs(lg)));
+    e_rsq = fmax(e_rsq, fabs((double)__builtin_amdgcn_rsqf(s) * sqrt((double)s) - 1.0));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    e_log = fmax(e_log, __shfl_xor(e_log, o));
+    e_rsq = fmax(e_rsq, __shfl_xor(e_rsq, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&out[0], (unsigned long long)__double_as_longlong(e_log));
+    atomicMax(&out[1], (unsigned long long)__double_as_longlong(e_rsq));
+  }
+}
+
+// One thread per (output, slot, cell); slot -1 is the count.  Summation follows the list order, so
+// the result does not depend on the launch.
+__global__ void k_reduce(ReduceArgs A) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per_out = (int64_t)(A.ns + 1) * A.ncell;
+  if (i >= (int64_t)A.n_out * per_out) return;
+  const int o = (int)(i / per_out);
+  const int s = (int)((i % per_out) / A.ncell) - 1;
+  const int c = (int)(i % A.ncell);
+  const int b = A.beg[o], e = A.beg[o + 1];
+  if (s < 0) {
+    unsigned long long n = 0;
+    for (int k = b; k < e; ++k) n += A.accn[(size_t)A.lst[k] * A.ncell + c];
+    A.outn[(size_t)o * A.ncell + c] = n;
+  } else {
+    double v = 0.0;
+    for (int k = b; k < e; ++k) v += A.acc[((size_t)A.lst[k] * A.ns + s) * A.ncell + c];
+    A.out[((size_t)o * A.ns + s) * A.ncell + c] = v;
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Host-side launch helpers (called from cpc_api.cpp)
+
+hipError_t launch_reduce(const ReduceArgs &A, hipStream_t st) {
+  const int64_t n = (int64_t)A.n_out * (A.ns + 1) * A.ncell;
+  if (n == 0) return hipSuccess;
+  const int bs = 256;
+  k_reduce<<<(unsigned)((n + bs - 1) / bs), bs, 0, st>>>(A);
+  return hipGetLastError();
+}
+
+hipError_t launch_fp32_error(uint32_t b0, uint32_t n, unsigned long long *out, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  k_fp32_error<<<2048, 256, 0, st>>>(b0, n, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_keys(const KeyArgs &A, const int32_t *order, uint32_t *keys, int32_t *vals, hipStream_t st) {
   if (A.n == 0) return hipSuccess;

@@ -268,6 +268,8 @@ def load_library(path=LIB_PATH):
     lib.cpc_do_bining.argtypes = [POINTER(Params), POINTER(Norm), POINTER(Bining), c_char_p, POINTER(PairNorm),
                                   POINTER(c_int)]
     lib.cpc_do_bining.restype = c_int
+    lib.cpc_fp32_error.argtypes = [c_void_p, ctypes.c_float, ctypes.c_float, _dp]
+    lib.cpc_fp32_error.restype = c_int
     _lib = lib
     return lib
 
@@ -275,7 +277,7 @@ def load_library(path=LIB_PATH):
 EXPORTED_SYMBOLS = ["cpc_init", "cpc_destroy", "cpc_last_error", "cpc_version", "cpc_catalog_upload",
                     "cpc_catalog_free", "cpc_count", "cpc_count_batch", "cpc_rp_thresholds", "cpc_rand_stream",
                     "cpc_set_tuning", "cpc_autotune", "cpc_bin_edges", "cpc_box_rr", "cpc_weight_sum",
-                    "cpc_write_cross", "cpc_do_bining"]
+                    "cpc_write_cross", "cpc_do_bining", "cpc_fp32_error"]
 
 
 REFERENCE_SEED = 124693517  # seed(), jk.cpp:40-43
@@ -452,6 +454,14 @@ class Engine:
         self._check(self.lib.cpc_autotune(self.ctx, n, arr, ctypes.byref(best), ctypes.byref(tried)), "autotune")
         tun = {"col_frac": best.col_frac, "item_size": best.item_size, "block_threads": best.block_threads}
         return tun, tried.value, [(h, st.as_dict()) for (_, h, _, st) in keep]
+
+    def fp32_error(self, s_lo, s_hi):
+        """(log2 error, rsq error) of the device's FP32 approximations over every FP32 value in
+        [s_lo, s_hi] (cpc_fp32_error)."""
+        out = np.zeros(2, np.float64)
+        self._check(self.lib.cpc_fp32_error(self.ctx, float(s_lo), float(s_hi), out.ctypes.data_as(_dp)),
+                    "cpc_fp32_error")
+        return float(out[0]), float(out[1])
 
     def close(self):
         if self.ctx is not None:

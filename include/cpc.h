@@ -322,6 +322,15 @@ typedef struct cpc_tuning {
  * the number of batches run. */
 cpc_status cpc_autotune(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_tuning *best, int *tried);
 
+/* Device check of the FP32 hardware approximations the split kernels classify with (no counterpart
+ * in the reference, which bins in FP64 with the host libm, corels.cpp:31-34).  For every FP32 value s
+ * in [s_lo, s_hi] (both positive and finite) the device evaluates the same instructions as the pair
+ * kernels: out[0] = max |log2~(s) - log2(s)| / max(1, |log2(s)|) for the hardware logarithm
+ * (v_log_f32), out[1] = max |rsq~(s) sqrt(s) - 1| for the hardware reciprocal square root
+ * (v_rsq_f32), against FP64 references.  The split guard bands take both as at most 2^-21 (about
+ * 4.8e-7); a larger value means the guard bands do not cover this device. */
+cpc_status cpc_fp32_error(cpc_ctx *ctx, float s_lo, float s_hi, double out[2]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from corr_pc_amd import synth
+from corr_pc_amd.compare import field_errors, kind_fields
 from corr_pc_amd.engine import (JK_LEAVE_ONE_OUT, JK_TOUCH, KIND_DD, KIND_SHAPE_DENSITY, KIND_SHAPE_SHAPE, PREC_FP64,
                                 PREC_SPLIT, make_params)
 from corr_pc_amd.passes import config_datainfo, setup_pass
@@ -35,6 +36,11 @@ def _run(engine, oracle_lib, outer, inner, p, what):
             engine.set_tuning()
         assert_hist_match(got, ref, what=f"{what} tuning={tun}")
         assert st["n_binned"] == rst["n_binned"]
+        # the stated error of the weighted bins against the oracle (corr_pc's CPU arithmetic)
+        fe = field_errors(got, ref, kind_fields(p.kind, p.estimator))
+        rel = max((v["max_rel"] for v in fe.values() if isinstance(v, dict)), default=0.0)
+        print(f"[err] {what} tuning={tun}: max |d|/|ref| {rel:.2e}, max |d|/magnitude {fe['max_rel_mag_all']:.2e}",
+              flush=True)
     return st
 
 
@@ -169,6 +175,9 @@ def test_split_matches_fp64_full_config1(engine):
     p.precision = PREC_FP64
     h_64, st_d = engine.count(outer, inner, p)
     assert_hist_match(h_split, h_64, what="config1 full SD split vs fp64")
+    fe = field_errors(h_split, h_64, kind_fields(p.kind, p.estimator))
+    print("[err] config1 full SD split vs fp64: " + ", ".join(
+        f"{k} {v['max_rel']:.2e}/{v['max_rel_mag']:.2e}" for k, v in fe.items() if isinstance(v, dict)), flush=True)
     assert st_s["n_binned"] == st_d["n_binned"] > 4e10
     # invariants of the jackknife layout: leave-one-out counts never exceed the full sample and the
     # rp-level counts are the pi sums
