@@ -1271,8 +1271,9 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     const double pabs = std::max(std::fabs(p.pi_min), std::fabs(p.pi_max));
     gz = eps_z / dv.psize + 8.0 * (p.n_pi + 2 * pabs / dv.psize + 2) * two24;
   }
-  // r-mu passes classify every candidate in FP64: the FP64 kernel with the r-mu code compiled in
-  P.use_split = P.split && (sky || gz < 0.25) && (sky || !p.r_mu);
+  // r-mu passes classify every candidate in FP64 (the FP64 kernel with the r-mu code compiled in),
+  // and so do box passes with linear rp bins: the box split classification is for log bins
+  P.use_split = P.split && (sky || gz < 0.25) && (sky || (!p.r_mu && !p.lin_bin));
 
   if (!sky) {
     KArgs K{};
@@ -1432,13 +1433,16 @@ Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
     size_t lds, first, n;
   };
   std::map<std::pair<int, int>, std::vector<size_t>> groups;  // (kind, split) -> plans
-  // (kind, split) -> plans; r-mu box passes run their own instantiation (kind + kRmuKind)
+  // (kind, split, auto) -> plans; r-mu box passes run their own instantiation (kind + kRmuKind),
+  // auto passes (corels1) theirs (+ kAutoKey)
+  constexpr int kAutoKey = 64;
   for (size_t i = 0; i < plans.size(); ++i)
-    groups[{plans[i]->kind_code + (!plans[i]->sky && plans[i]->p.r_mu ? kRmuKind : 0), plans[i]->use_split ? 1 : 0}]
+    groups[{plans[i]->kind_code + (!plans[i]->sky && plans[i]->p.r_mu ? kRmuKind : 0) + (plans[i]->autop ? kAutoKey : 0),
+            plans[i]->use_split ? 1 : 0}]
         .push_back(i);
   std::vector<Item> all;
   std::vector<Launch> launches;
-  for (int swap = 0; swap < 2; ++swap)
+  for (int swap = 0; swap < 1; ++swap)  // box passes have no touch launch (make_items_box)
     for (const auto &g : groups) {
       std::vector<std::pair<double, Item>> v;
       size_t lds = 0;
@@ -1469,14 +1473,10 @@ Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
     check(hipMemcpyAsync(ctx->items.p, all.data(), all.size() * sizeof(Item), hipMemcpyHostToDevice, s), st, "copy");
   check(hipEventRecord(ctx->ev0, s), st, "event");
   for (const Launch &l : launches)
-    if (!l.swap)
-      check(launch_pair(l.kind, l.split, false, ctx->jobs.p, ctx->items.p + l.first, (int)l.n, l.lds, l.threads, s),
-            st, "pair kernel");
+    check(launch_pair(l.kind % kAutoKey, l.split, l.kind >= kAutoKey, ctx->jobs.p, ctx->items.p + l.first, (int)l.n,
+                      l.lds, l.threads, s),
+          st, "pair kernel");
   check(hipEventRecord(ctx->ev2, s), st, "event");
-  for (const Launch &l : launches)
-    if (l.swap)
-      check(launch_pair(l.kind, l.split, true, ctx->jobs.p, ctx->items.p + l.first, (int)l.n, l.lds, l.threads, s),
-            st, "touch kernel");
   check(hipEventRecord(ctx->ev1, s), st, "event");
   for (Plan *Pp : plans) {  // raw targets into the full sample and touch histograms
     const Plan &P = *Pp;

@@ -55,12 +55,14 @@ struct Gal {
   double x, y, w, e1, e2, phi;
 };
 
-// atan(y/x) to within 1e-14 absolute, in registers only.  With a = min(|x|,|y|), b = max, the ratio
-// r = a/b is reduced about the nearest quarter c = k/4 (k from an FP32 estimate):
+// atan(y/x) to within 1e-14 absolute, in registers only and without branches.  With
+// a = min(|x|,|y|), b = max, the ratio r = a/b is reduced about the nearest quarter c = k/4 (k from an
+// FP32 estimate):
 //   atan(r) = atan(c) + atan(d),  d = (a - c b)/(b + c a),  |d| <= 0.126,
-// atan(d) is its odd Taylor series to d^13 (truncation below |d|^15/15 < 3e-15) and atan(k/4) is one
-// of five correctly rounded constants.  |y| > |x| uses pi/2 - atan(x/y); the sign is that of the IEEE
-// quotient y/x; x = y = 0 gives NaN like atan(0/0).
+// atan(d) is its odd Taylor series to d^13 (truncation below |d|^15/15 < 3e-15) and atan(k/4) is the
+// quartic in k through the five values atan(0), atan(1/4), .., atan(1) (coefficients solved in
+// 50-digit arithmetic; within 6e-17 of each value at k = 0..4).  |y| > |x| uses
+// pi/2 - atan(x/y); the sign is that of the IEEE quotient y/x; x = y = 0 gives NaN like atan(0/0).
 __device__ __forceinline__ double atan_ratio(double y, double x) {
   const double ax = fabs(x), ay = fabs(y);
   const bool sw = ay > ax;
@@ -75,13 +77,12 @@ __device__ __forceinline__ double atan_ratio(double y, double x) {
   p = fma(d2, p, -1.0 / 7.0);
   p = fma(d2, p, 1.0 / 5.0);
   p = fma(d2, p, -1.0 / 3.0);
-  const double base = k == 0 ? 0.0
-                      : k == 1 ? 0x1.f5b75f92c80ddp-3   // atan(1/4)
-                      : k == 2 ? 0x1.dac670561bb4fp-2   // atan(1/2)
-                      : k == 3 ? 0x1.4978fa3269ee1p-1   // atan(3/4)
-                               : 0x1.921fb54442d18p-1;  // atan(1)
-  double t = base + fma(d * d2, p, d);
-  if (sw) t = M_PI_2 - t;
+  const double kd = (double)k;
+  const double base =
+      kd * fma(kd, fma(kd, fma(kd, 0x1.23f4fc5bb1b6dp-11, -0x1.63904aa83c890p-8), -0x1.971b236888072p-11),
+               0x1.00a384078b643p-2);
+  const double t0 = base + fma(d * d2, p, d);
+  const double t = sw ? M_PI_2 - t0 : t0;
   return (__builtin_signbit(x) != __builtin_signbit(y)) ? -t : t;
 }
 
@@ -217,7 +218,8 @@ struct HotArgs {
 
 // rp bin of a squared separation against the FP64 thresholds thr[0..n_rp]: bin k iff
 // thr[k] <= s < thr[k+1].  An FP32 logarithm gives the starting guess.
-__device__ __forceinline__ int rp_bin64(const HotArgs &P, const double *thr, double s) {
+template <typename A>
+__device__ __forceinline__ int rp_bin64(const A &P, const double *thr, double s) {
   const float sf = (float)s;
   float kf = P.lin ? (sqrtf(sf) - P.lrmin) * P.linv : __log2f(sf) * P.la + P.lb;
   kf = fminf(fmaxf(kf, -1.0f), (float)P.n_rp);
@@ -230,8 +232,10 @@ __device__ __forceinline__ int rp_bin64(const HotArgs &P, const double *thr, dou
 // The reference's candidate test and binning for outer (x1,y1,z1), inner (x2,y2,z2) seen through
 // image (PBx, PBy, PBz): do_pair_sorted (corr.cpp:67-86) then corel_bin (corels.cpp:21-46), with
 // r_mu_calc (calcs.cpp:135-140) for r-mu bins.  Returns the histogram cell n1*n_pi + n2, or -1.
-template <bool RMU>
-__device__ __forceinline__ int exact_cell(const HotArgs &P, const double *thr, double x1, double y1, double z1,
+// `A` is HotArgs (arguments held in registers: the FP64 kernel, where this is the hot path) or the
+// job's KArgs in constant memory (the split kernel, where only guard-band candidates come here).
+template <bool RMU, typename A>
+__device__ __forceinline__ int exact_cell(const A &P, const double *thr, double x1, double y1, double z1,
                                           double x2, double y2, double z2, int PBx, int PBy, int PBz) {
   const double L = P.L;
   const double zz = z2 + L * PBz;

@@ -243,7 +243,7 @@ size_t pair_lds_bytes(int ncell, int nslots, int n_rp, int ncopy = 1);
 // kind, precision and pass direction); `lds` covers the largest histogram among them.
 // kind + kRmuKind selects the r-mu instantiation (FP64 only) of the kind.
 constexpr int kRmuKind = 16;
-hipError_t launch_pair(int kind, bool split, bool swap, const KArgs *jobs, const Item *items, int n_items,
+hipError_t launch_pair(int kind, bool split, bool autop, const KArgs *jobs, const Item *items, int n_items,
                        size_t lds, int threads, hipStream_t st);
 // Record arrays of one side of a sky pass.
 struct SkySide {

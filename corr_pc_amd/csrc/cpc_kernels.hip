@@ -57,15 +57,38 @@ __device__ __forceinline__ float wave_max(float v) {
 
 using CKArgs = __attribute__((address_space(4))) const KArgs;
 
-// RMU compiles in the r-mu binning of coordinates 7 (with SPLIT false only: r-mu candidates all take
-// the FP64 path); without it the rp-pi kernels keep their registers and schedule.
-template <int K, bool SPLIT, bool SWAP, bool RMU>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT ? 4 : 2))) pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
+// A closed run of one lane into the cell-major LDS histogram copy (lacc[cell][NS]: one address and
+// immediate slot offsets), or, for cells below the LDS rows (CUT only), straight into the target's
+// global histogram ([NS][ncell]).
+template <int NS, bool CUT>
+__device__ __forceinline__ void flush_cm(lds_u32 *mynum, lds_f64 *myacc, int lbase, unsigned long long *gnum,
+                                         double *gacc, int ncell, int cell, unsigned n, const double *v) {
+  const int lc = cell - lbase;
+  if (!CUT || lc >= 0) {
+    __hip_atomic_fetch_add(&mynum[lc], n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    lds_f64 *const a = myacc + lc * NS;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) __hip_atomic_fetch_add(&a[s], v[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  } else {
+    atomicAdd(&gnum[cell], (unsigned long long)n);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) unsafeAtomicAdd(&gacc[(size_t)s * ncell + cell], v[s]);
+  }
+}
+
+// The box pair kernel.  SPLIT classifies candidates in FP32 (log rp bins) and re-evaluates the
+// guard band in FP64; without it every candidate takes the FP64 path.  AUTO compiles in corels1's
+// order predicate (auto_mode 1 and 2).  RMU compiles in the r-mu binning of coordinates 7 (with
+// SPLIT false only: r-mu candidates all take the FP64 path).  Kinds, AUTO and RMU are separate
+// instantiations so that each keeps only the registers and branches it needs.
+template <int K, bool SPLIT, bool AUTO, bool RMU>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT ? 4 : 2)))
+pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
   constexpr int NV = KindNV<K>::NV;
   constexpr int NE = KindNV<K>::NE;
-  constexpr int NS = SWAP ? 1 + NV : 1 + NV + NE;  // f64 slots accumulated
-  constexpr int NQ = 1 + NV + NE;                  // quantities computed
+  constexpr int NS = 1 + NV + NE;  // f64 slots accumulated per cell: wt, vals, errs
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int next_chunk;  // next chunk of the item that no wave has taken
   const Item it = items[blockIdx.x];
   // the pass arguments of the item's job, read as constant memory: wave-uniform scalar loads the
   // compiler may repeat instead of holding them in registers, as it does with kernel arguments
@@ -78,10 +101,10 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
                   P.cull_r2, P.cull_d};
   const PRec *__restrict__ brec = P.brec;
   const double *__restrict__ bz = P.bz;
-  const PRec *__restrict__ trec = P.trec;
   const double *__restrict__ tz = P.tz;
+  const uint4 *__restrict__ tq = reinterpret_cast<const uint4 *>(P.trec);  // records as 16-byte quarters
   // LDS: ncopy histogram copies of the cells from lbase on, lnum[ncopy][lcell] and
-  // lacc[ncopy][NS][lcell], then the thresholds.  Neighbouring lanes often close runs of the same cell
+  // lacc[ncopy][lcell][NS], then the thresholds.  Neighbouring lanes often close runs of the same cell
   // in the same instruction, and LDS atomics to one address serialise (about 3 cycles per lane,
   // against about 6 cycles for a whole instruction without collisions); copies spread those lanes.
   // Row cut and copies are compiled in only where the flush dominates: 7 or more slots per cell.
@@ -89,7 +112,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
   const int ncell = P.ncell, ncopy = CUT ? P.ncopy : 1, lcell = CUT ? H.lcell : P.ncell, lbase = CUT ? H.lbase : 0;
   unsigned *lnum = reinterpret_cast<unsigned *>(smem);
   double *lacc = reinterpret_cast<double *>(smem + ((ncopy * lcell * 4 + 15) & ~15));
-  double *lthr = lacc + (size_t)ncopy * NS * lcell;
+  double *lthr = lacc + (size_t)ncopy * lcell * NS;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -97,250 +120,215 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT 
   const int nwave = blockDim.x >> 6;
   const int cp = lane % ncopy;
   lds_u32 *const mynum = (lds_u32 *)(lnum + cp * lcell);
-  lds_f64 *const myacc = (lds_f64 *)(lacc + (size_t)cp * NS * lcell);
+  lds_f64 *const myacc = (lds_f64 *)(lacc + (size_t)cp * lcell * NS);
 
   for (int k = tid; k <= H.n_rp; k += blockDim.x) lthr[k] = P.thr[k];
+  for (int c = tid; c < ncopy * lcell; c += blockDim.x) {
+    lnum[c] = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) lacc[(size_t)c * NS + s] = 0.0;
+  }
+  if (tid == 0) next_chunk = nwave;
+  __syncthreads();
 
   const Group bg = P.bgrp[it.group0];  // region and coin of every chunk of the item
-  const uint4 *tq = reinterpret_cast<const uint4 *>(trec);  // records as 16-byte quarters
   const int tmode = it.tmode, treg = it.treg;
-
+  unsigned long long *const gnum = P.accnum + (size_t)it.target * ncell;
+  double *const gacc = P.acc + (size_t)it.target * NS * ncell;
   unsigned long long ncand = 0, nre = 0;  // wave totals
-  __shared__ int next_chunk;  // next chunk of the item no wave has taken
   int next_nx = 0;
 
-  {
-    unsigned long long *const gnum = P.accnum + (size_t)it.target * ncell;
-    double *const gacc = P.acc + (size_t)it.target * NS * ncell;
-    for (int c = tid; c < ncopy * lcell; c += blockDim.x) {
-      lnum[c] = 0;
+  // chunks come sorted by decreasing cost; each wave starts on its own and then takes the next
+  // unclaimed one, so the waves of the group finish close together
+  for (int ch = wave; ch < it.n; ch = uni(next_nx)) {
+    const Chunk ck = P.chunks[it.begin + ch];
+    const int col = uni(P.bgrp[ck.group].col);
+    const int bcx = col % P.nc, bcy = col / P.nc;
+    const int base = uni(ck.first);
+    const int cnt = uni(ck.count);
+    const bool valid = lane < cnt;
+    const int p = base + (valid ? lane : 0);
+    const PRec me = brec[p];
+    const double mz = bz[p];
+    const int midx = AUTO ? P.bidx[p] : 0;
+    const double wzlo = bz[base];
+    const double wzhi = bz[base + cnt - 1];
+    // bounding box of the wave's block galaxies in column-local FP32 coordinates (tile cull)
+    const float bx0 = wave_min(valid ? me.xl : INFINITY), bx1 = wave_max(valid ? me.xl : -INFINITY);
+    const float by0 = wave_min(valid ? me.yl : INFINITY), by1 = wave_max(valid ? me.yl : -INFINITY);
+
+    // A lane keeps the sums of its current histogram cell in registers and adds them to LDS only
+    // when its cell changes: lanes of a wave hold neighbouring galaxies, so their pairs crowd into
+    // few cells, and flushing per run instead of per pair divides the same-address LDS atomics.
+    int run_cell = -1;
+    unsigned run_n = 0;
+    double run[NS];
 #pragma unroll
-      for (int s = 0; s < NS; ++s) lacc[s * ncopy * lcell + c] = 0.0;
-    }
-    if (tid == 0) next_chunk = nwave;
-    __syncthreads();
+    for (int s = 0; s < NS; ++s) run[s] = 0.0;
 
-    // chunks come sorted by decreasing cost; each wave starts on its own and then takes the next
-    // unclaimed one, so the waves of the group finish close together
-    for (int ch = wave; ch < it.n; ch = uni(next_nx)) {
-      const Chunk ck = P.chunks[it.begin + ch];
-      const int col = uni(P.bgrp[ck.group].col);
-      const int bcx = col % P.nc, bcy = col / P.nc;
-      const int base = uni(ck.first);
-      const int cnt = uni(ck.count);
-      const bool valid = lane < cnt;
-      const int p = base + (valid ? lane : 0);
-      const PRec me = brec[p];
-      const double mz = bz[p];
-      const int midx = H.auto_mode ? P.bidx[p] : 0;
-      const double wzlo = bz[base];
-      const double wzhi = bz[base + cnt - 1];
-      // bounding box of the wave's block galaxies in column-local FP32 coordinates (tile cull)
-      const float bx0 = wave_min(valid ? me.xl : INFINITY), bx1 = wave_max(valid ? me.xl : -INFINITY);
-      const float by0 = wave_min(valid ? me.yl : INFINITY), by1 = wave_max(valid ? me.yl : -INFINITY);
+    for (int q = 0; q < P.n_nbr; ++q) {
+      const int2 o = P.nbr[q];
+      const int ux = bcx + o.x, uy = bcy + o.y;
+      const int sx = ux >= 0 ? ux / P.nc : -((-ux + P.nc - 1) / P.nc);
+      const int sy = uy >= 0 ? uy / P.nc : -((-uy + P.nc - 1) / P.nc);
+      const int tcx = ux - sx * P.nc, tcy = uy - sy * P.nc;
+      const int tcoln = tcy * P.nc + tcx;
+      const int g0 = P.tcol[tcoln], g1 = P.tcol[tcoln + 1];
+      const float Ox = (float)((double)(tcx - bcx) * P.h + H.L * sx);
+      const float Oy = (float)((double)(tcy - bcy) * P.h + H.L * sy);
+      for (int g = g0; g < g1; ++g) {
+        const Group tg = P.tgrp[g];
+        // the item's tile filter: every pair it meets belongs to its target
+        const bool same = tg.region == bg.region;
+        const bool take = tmode == kTileAll || (tmode == kTileSame && same) ||
+                          (tmode == kTileOtherC0 && !same && tg.coin == 0) ||
+                          (tmode == kTileSameOrC0 && (same || tg.coin == 0)) ||
+                          (tmode == kTileRegionC1 && tg.region == treg && tg.coin == 1);
+        if (!take) continue;
+        for (int sz = -1; sz <= 1; ++sz) {
+          const double lo = wzlo + H.wlo - H.L * sz - P.zmargin;
+          const double hi = wzhi + H.whi - H.L * sz + P.zmargin;
+          if (hi < tg.zlo || lo > tg.zhi) continue;
+          const int j0 = zsearch<false>(tz, tg.start, tg.end, lo);
+          const int j1 = zsearch<true>(tz, j0, tg.end, hi);
+          if (j0 >= j1) continue;
+          // (sx, sy, sz) is the inner galaxy's image in the reference's terms
+          const bool lane_ok = valid && image_allowed(me.meta, sx, sy, sz);
+          // corels1: the inner galaxy must come after the outer one (sorted: images with PBz <= 0,
+          // in (z, index) order; unsorted: every image, in index order)
+          const bool ordered = AUTO && (H.auto_mode == 2 || (H.auto_mode == 1 && sz <= 0));
+          const bool unsorted_off = H.auto_mode != 2;
+          // FP32 offsets of this lane to the tile column: d = tile local + offset
+          const float ax = Ox - me.xl, ay = Oy - me.yl, az = (float)(H.L * sz) - me.zf;
 
-      // A lane keeps the sums of its current histogram cell in registers and adds them to LDS only
-      // when its cell changes: lanes of a wave hold neighbouring galaxies, so their pairs crowd into
-      // few cells, and flushing per run instead of per pair divides the same-address LDS atomics.
-      int run_cell = -1;
-      unsigned run_n = 0;
-      double run[NS];
-#pragma unroll
-      for (int s = 0; s < NS; ++s) run[s] = 0.0;
-
-      for (int q = 0; q < P.n_nbr; ++q) {
-        const int2 o = P.nbr[q];
-        const int ux = bcx + o.x, uy = bcy + o.y;
-        const int sx = ux >= 0 ? ux / P.nc : -((-ux + P.nc - 1) / P.nc);
-        const int sy = uy >= 0 ? uy / P.nc : -((-uy + P.nc - 1) / P.nc);
-        const int tcx = ux - sx * P.nc, tcy = uy - sy * P.nc;
-        const int tcoln = tcy * P.nc + tcx;
-        const int g0 = P.tcol[tcoln], g1 = P.tcol[tcoln + 1];
-        const float Ox = (float)((double)(tcx - bcx) * P.h + H.L * sx);
-        const float Oy = (float)((double)(tcy - bcy) * P.h + H.L * sy);
-        for (int g = g0; g < g1; ++g) {
-          const Group tg = P.tgrp[g];
-          // the item's tile filter: every pair it meets belongs to its target
-          const bool same = tg.region == bg.region;
-          const bool take = tmode == kTileAll || (tmode == kTileSame && same) ||
-                            (tmode == kTileOtherC0 && !same && tg.coin == 0) ||
-                            (tmode == kTileSameOrC0 && (same || tg.coin == 0)) ||
-                            (tmode == kTileRegionC1 && tg.region == treg && tg.coin == 1);
-          if (!take) continue;
-          for (int sz = -1; sz <= 1; ++sz) {
-            double lo, hi;
-            if (!SWAP) {
-              lo = wzlo + H.wlo - H.L * sz - P.zmargin;
-              hi = wzhi + H.whi - H.L * sz + P.zmargin;
-            } else {
-              lo = wzlo - H.whi - H.L * sz - P.zmargin;
-              hi = wzhi - H.wlo - H.L * sz + P.zmargin;
+          uint4 cq = tq[(size_t)j0 * 4 + lane];
+          double cz = tz[j0 + (lane & (kBatch - 1))];  // z of tile t in lane t
+          for (int jb = j0; jb < j1; jb += kBatch) {
+            uint4 nq = cq;
+            double nz = cz;
+            if (jb + kBatch < j1) {
+              nq = tq[(size_t)(jb + kBatch) * 4 + lane];
+              nz = tz[jb + kBatch + (lane & (kBatch - 1))];
             }
-            if (hi < tg.zlo || lo > tg.zhi) continue;
-            const int j0 = zsearch<false>(tz, tg.start, tg.end, lo);
-            const int j1 = zsearch<true>(tz, j0, tg.end, hi);
-            if (j0 >= j1) continue;
-            // image of the inner galaxy in the reference's terms
-            const int PBx = SWAP ? -sx : sx, PBy = SWAP ? -sy : sy, PBz = SWAP ? -sz : sz;
-            const bool lane_ok = valid && (SWAP || image_allowed(me.meta, PBx, PBy, PBz));
-            // the inner galaxy must come after the outer (sorted: images with PBz <= 0; unsorted: all)
-            const bool ordered = H.auto_mode == 2 || (H.auto_mode == 1 && PBz <= 0);
-            const bool unsorted_off = H.auto_mode != 2;
-            // FP32 offsets of this lane to the tile column: d = tile local + offset
-            const float ax = Ox - me.xl, ay = Oy - me.yl, az = (float)(H.L * sz) - me.zf;
-            const float pinv = SWAP ? -H.inv_psize32 : H.inv_psize32;  // pi = -dz in the touch pass
-
-            uint4 cq = tq[(size_t)j0 * 4 + lane];
-            double cz = tz[j0 + (lane & (kBatch - 1))];  // z of tile t in lane t
-            for (int jb = j0; jb < j1; jb += kBatch) {
-              uint4 nq = cq;
-              double nz = cz;
-              if (jb + kBatch < j1) {
-                nq = tq[(size_t)(jb + kBatch) * 4 + lane];
-                nz = tz[jb + kBatch + (lane & (kBatch - 1))];
-              }
-              const int nt = min(kBatch, j1 - jb);
-              // tiles within reach of the wave's bounding box: lane 4t+3 holds tile t's column-local
-              // {xl, yl}; the others cannot bin with any lane and are skipped by the whole wave
-              unsigned long long tmask;
-              {
-                const float X = __uint_as_float(cq.x) + Ox, Y = __uint_as_float(cq.y) + Oy;
-                const float ex = fmaxf(fmaxf(bx0 - X, X - bx1) - H.cull_d, 0.0f);
-                const float ey = fmaxf(fmaxf(by0 - Y, Y - by1) - H.cull_d, 0.0f);
-                tmask = __ballot((lane & 3) == 3 && (lane >> 2) < nt && fmaf(ex, ex, ey * ey) <= H.cull_r2);
-              }
-              for (; tmask; tmask &= tmask - 1) {
-                const int t = uni(__builtin_ctzll(tmask) >> 2);
-                const int l3 = 4 * t + 3;  // lane of the quarter {xl, yl, zf, meta}
-                bool ok = lane_ok;
-                if (SWAP) ok = ok && image_allowed(rl_i(cq.w, l3), PBx, PBy, PBz);
-                if (ordered) {
-                  // corels1 scans j > i in catalog order: (z, index) order for its z-sorted input, the
-                  // index alone for unsorted input (every pair then goes to the index comparison)
-                  const double tzo = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(cz), t),
-                                                      __builtin_amdgcn_readlane(__double2loint(cz), t));
-                  const double zo = SWAP ? tzo : mz, zi = SWAP ? mz : tzo;
-                  bool after = zi > zo && unsorted_off;
-                  const bool tie = zi == zo || !unsorted_off;
-                  if (__ballot(ok && tie) != 0ull) {
-                    const int tix = P.tidx[jb + t];
-                    after = after || (tie && (SWAP ? midx > tix : tix > midx));
-                  }
-                  ok = ok && after;
-                }
-                int cell = -1;
-                bool unc = ok;
-                if (SPLIT && ok) {
-                  const float dx = rl_f(cq.x, l3) + ax;
-                  const float dy = rl_f(cq.y, l3) + ay;
-                  const float dz = rl_f(cq.z, l3) + az;
-                  const float s32 = fmaf(dx, dx, dy * dy);
-                  const float uu = fmaf(dz, pinv, H.pioff32);
-                  const float fu = floorf(uu);
-                  const float fr = uu - fu;
-                  const bool near_pi = (fr < H.gz) || (fr > 1.0f - H.gz);
-                  const bool pi_in = (fu >= 0.0f) && (fu < (float)H.n_pi);
-                  const bool rp_out = (s32 < H.s_out_lo) || (s32 >= H.s_out_hi);
-                  if ((!near_pi && !pi_in) || rp_out) {
-                    unc = false;
-                  } else if (!near_pi && s32 >= H.s_cert_lo) {
-                    // rp bin index in FP32 and its guard band; outside the band the bin is that of
-                    // the FP64 squared separation
-                    // (s32 is finite and at least s_cert_lo here, so kf is finite)
-                    const float rinv = __builtin_amdgcn_rsqf(s32);
-                    const float kf = H.lin ? (s32 * rinv - H.lrmin) * H.linv : __log2f(s32) * H.la + H.lb;
-                    const float fk = floorf(kf);
-                    const float frk = kf - fk;
-                    const float gk = fmaf(H.gD, s32 * rinv, fmaf(fmaf(H.gC, rinv, H.gB), rinv, H.gA));
-                    if (frk >= gk && frk <= 1.0f - gk) {
-                      unc = false;
-                      if (fk >= 0.0f && fk < (float)H.n_rp) cell = (int)fmaf(fk, (float)H.n_pi, fu);
-                    }
-                  }
-                }
-                ncand += popc_ballot(ok);
-                if (__ballot(unc) != 0ull) {
-                  nre += SPLIT ? popc_ballot(unc) : 0ull;
-                  const double tx = rl_d(cq.x, cq.y, 4 * t), ty = rl_d(cq.z, cq.w, 4 * t);
-                  const double tzz = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(cz), t),
-                                                      __builtin_amdgcn_readlane(__double2loint(cz), t));
-                  if (unc) {
-                    if (!SWAP)
-                      cell = exact_cell<RMU>(H, lthr, me.x, me.y, mz, tx, ty, tzz, PBx, PBy, PBz);
-                    else
-                      cell = exact_cell<RMU>(H, lthr, tx, ty, tzz, me.x, me.y, mz, PBx, PBy, PBz);
-                  }
-                }
-                if (__ballot(cell >= 0) != 0ull) {
-                  const Gal gt{rl_d(cq.x, cq.y, 4 * t),     rl_d(cq.z, cq.w, 4 * t),
-                               rl_d(cq.x, cq.y, 4 * t + 1), rl_d(cq.z, cq.w, 4 * t + 1),
-                               rl_d(cq.x, cq.y, 4 * t + 2), rl_d(cq.z, cq.w, 4 * t + 2)};
-                  if (cell >= 0) {
-                    if (cell != run_cell) {
-                      if (run_cell >= 0) {
-                        if constexpr (CUT)
-                          flush_run<NS>(mynum, myacc, lcell, lbase, gnum, gacc, ncell, run_cell, run_n, run);
-                        else
-                          lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
-                      }
-                      run_cell = cell;
-                      run_n = 0;
-#pragma unroll
-                      for (int s = 0; s < NS; ++s) run[s] = 0.0;
-                    }
-                    ++run_n;
-                    const Gal gme{me.x, me.y, me.w, me.e1, me.e2, me.phi};
-                    double qv[NQ];
-                    if constexpr (K == kED) {
-                      // g1 = outer, g2 = inner in the reference's roles; PB is g2's image
-                      const double tzv = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(cz), t),
-                                                          __builtin_amdgcn_readlane(__double2loint(cz), t));
-                      const Gal &g1 = SWAP ? gt : gme, &g2 = SWAP ? gme : gt;
-                      const double z1 = SWAP ? tzv : mz, z2 = SWAP ? mz : tzv;
-                      const Gal &ax = H.gws == 2 ? g2 : g1;
-                      ed_values(g1.w, g2.w, ax.e1, ax.e2, ax.phi, g1.x - (g2.x + H.L * PBx),
-                                g1.y - (g2.y + H.L * PBy), z1 - (z2 + H.L * PBz), qv);
-                    } else if (!SWAP) {
-                      pair_values<K, SPLIT>(gme, gt, H.gws, qv);
-                    } else {
-                      pair_values<K, SPLIT>(gt, gme, H.gws, qv);
-                    }
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) run[s] += qv[s];
-                  }
-                }
-              }
-              cq = nq;
-              cz = nz;
+            const int nt = min(kBatch, j1 - jb);
+            // tiles within reach of the wave's bounding box: lane 4t+3 holds tile t's column-local
+            // {xl, yl}; the others cannot bin with any lane and are skipped by the whole wave
+            unsigned long long tmask;
+            {
+              const float X = __uint_as_float(cq.x) + Ox, Y = __uint_as_float(cq.y) + Oy;
+              const float ex = fmaxf(fmaxf(bx0 - X, X - bx1) - H.cull_d, 0.0f);
+              const float ey = fmaxf(fmaxf(by0 - Y, Y - by1) - H.cull_d, 0.0f);
+              tmask = __ballot((lane & 3) == 3 && (lane >> 2) < nt && fmaf(ex, ex, ey * ey) <= H.cull_r2);
             }
+            for (; tmask; tmask &= tmask - 1) {
+              const int t = uni(__builtin_ctzll(tmask) >> 2);
+              const int l3 = 4 * t + 3;  // lane of the quarter {xl, yl, zf, meta}
+              bool ok = lane_ok;
+              if (AUTO && ordered) {
+                const double zi = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(cz), t),
+                                                   __builtin_amdgcn_readlane(__double2loint(cz), t));
+                bool after = zi > mz && unsorted_off;
+                const bool tie = zi == mz || !unsorted_off;
+                if (__ballot(ok && tie) != 0ull) {
+                  const int tix = P.tidx[jb + t];
+                  after = after || (tie && tix > midx);
+                }
+                ok = ok && after;
+              }
+              int cell = -1;
+              bool unc = ok;
+              if (SPLIT) {
+                // FP32 classification, branch-free: a candidate is certainly outside every bin
+                // (`out`), certainly in the bin (fk, fu) (`cert`: pi and rp bin fractions outside
+                // their guard bands), or uncertain and re-evaluated in FP64
+                const float dx = rl_f(cq.x, l3) + ax;
+                const float dy = rl_f(cq.y, l3) + ay;
+                const float dz = rl_f(cq.z, l3) + az;
+                const float s32 = fmaf(dx, dx, dy * dy);
+                const float uu = fmaf(dz, H.inv_psize32, H.pioff32);
+                const float fu = floorf(uu);
+                const float fr = uu - fu;
+                const bool near_pi = (fr < H.gz) || (fr > 1.0f - H.gz);
+                const bool pi_in = (fu >= 0.0f) && (fu < (float)H.n_pi);
+                const bool out = (!near_pi && !pi_in) || (s32 < H.s_out_lo) || (s32 >= H.s_out_hi);
+                // s32 >= s_cert_lo > 0 wherever cert can hold, so kf and gk are finite there
+                const float rinv = __builtin_amdgcn_rsqf(s32);
+                const float kf = __log2f(s32) * H.la + H.lb;
+                const float fk = floorf(kf);
+                const float frk = kf - fk;
+                const float gk = fmaf(fmaf(H.gC, rinv, H.gB), rinv, H.gA);
+                const bool cert = !near_pi && s32 >= H.s_cert_lo && frk >= gk && frk <= 1.0f - gk;
+                unc = ok && !out && !cert;
+                if (ok && !out && cert && fk >= 0.0f && fk < (float)H.n_rp)
+                  cell = (int)fmaf(fk, (float)H.n_pi, fu);
+              }
+              ncand += popc_ballot(ok);
+              if (__ballot(unc) != 0ull) {
+                nre += SPLIT ? popc_ballot(unc) : 0ull;
+                const double tx = rl_d(cq.x, cq.y, 4 * t), ty = rl_d(cq.z, cq.w, 4 * t);
+                const double tzz = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(cz), t),
+                                                    __builtin_amdgcn_readlane(__double2loint(cz), t));
+                if (unc) {
+                  if constexpr (SPLIT)
+                    cell = exact_cell<RMU>(P, lthr, me.x, me.y, mz, tx, ty, tzz, sx, sy, sz);
+                  else
+                    cell = exact_cell<RMU>(H, lthr, me.x, me.y, mz, tx, ty, tzz, sx, sy, sz);
+                }
+              }
+              if (__ballot(cell >= 0) != 0ull) {
+                const Gal gt{rl_d(cq.x, cq.y, 4 * t),     rl_d(cq.z, cq.w, 4 * t),
+                             rl_d(cq.x, cq.y, 4 * t + 1), rl_d(cq.z, cq.w, 4 * t + 1),
+                             rl_d(cq.x, cq.y, 4 * t + 2), rl_d(cq.z, cq.w, 4 * t + 2)};
+                if (cell >= 0) {
+                  if (cell != run_cell) {
+                    if (run_cell >= 0) flush_cm<NS, CUT>(mynum, myacc, lbase, gnum, gacc, ncell, run_cell, run_n, run);
+                    run_cell = cell;
+                    run_n = 0;
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) run[s] = 0.0;
+                  }
+                  ++run_n;
+                  const Gal gme{me.x, me.y, me.w, me.e1, me.e2, me.phi};
+                  double qv[NS];
+                  if constexpr (K == kED) {
+                    // g1 = outer, g2 = inner in the reference's roles; (sx, sy, sz) is g2's image
+                    const double tzv = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(cz), t),
+                                                        __builtin_amdgcn_readlane(__double2loint(cz), t));
+                    const Gal &ax3 = H.gws == 2 ? gt : gme;
+                    ed_values(gme.w, gt.w, ax3.e1, ax3.e2, ax3.phi, gme.x - (gt.x + H.L * sx),
+                              gme.y - (gt.y + H.L * sy), mz - (tzv + H.L * sz), qv);
+                  } else {
+                    pair_values<K, SPLIT>(gme, gt, H.gws, qv);
+                  }
+#pragma unroll
+                  for (int s = 0; s < NS; ++s) run[s] += qv[s];
+                }
+              }
+            }
+            cq = nq;
+            cz = nz;
           }
         }
       }
-      if (run_cell >= 0) {
-        if constexpr (CUT)
-          flush_run<NS>(mynum, myacc, lcell, lbase, gnum, gacc, ncell, run_cell, run_n, run);
-        else
-          lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
-      }
-      int nx = 0;
-      if (lane == 0) nx = atomicAdd(&next_chunk, 1);
-      next_nx = __builtin_amdgcn_readfirstlane(nx);
     }
-    __syncthreads();
-    for (int c = tid; c < lcell; c += blockDim.x) {
-      unsigned n = 0;
-      for (int k = 0; k < ncopy; ++k) n += lnum[k * lcell + c];
-      if (n) {
-        atomicAdd(&gnum[lbase + c], (unsigned long long)n);
+    if (run_cell >= 0) flush_cm<NS, CUT>(mynum, myacc, lbase, gnum, gacc, ncell, run_cell, run_n, run);
+    int nx = 0;
+    if (lane == 0) nx = atomicAdd(&next_chunk, 1);
+    next_nx = __builtin_amdgcn_readfirstlane(nx);
+  }
+  __syncthreads();
+  for (int c = tid; c < lcell; c += blockDim.x) {
+    unsigned n = 0;
+    for (int k = 0; k < ncopy; ++k) n += lnum[k * lcell + c];
+    if (n) {
+      atomicAdd(&gnum[lbase + c], (unsigned long long)n);
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-          double v = 0.0;
-          for (int k = 0; k < ncopy; ++k) v += lacc[((size_t)k * NS + s) * lcell + c];
-          unsafeAtomicAdd(&gacc[(size_t)s * ncell + lbase + c], v);
-        }
+      for (int s = 0; s < NS; ++s) {
+        double v = 0.0;
+        for (int k = 0; k < ncopy; ++k) v += lacc[((size_t)k * lcell + c) * NS + s];
+        unsafeAtomicAdd(&gacc[(size_t)s * ncell + lbase + c], v);
       }
     }
-    __syncthreads();
   }
   if (lane == 0) {
     if (ncand) atomicAdd(&P.counters[0], ncand);
@@ -547,10 +535,10 @@ hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *ko
   return sort_pairs_impl<double>(tmp, tmp_bytes, kin, kout, vin, vout, n, 64, st);
 }
 
-template <int K, bool SPLIT, bool SWAP, bool RMU = false>
+template <int K, bool SPLIT, bool AUTO, bool RMU>
 static hipError_t launch_pair_t(const KArgs *jobs, const Item *items, int n_items, int threads, size_t lds,
                                 hipStream_t st) {
-  auto fn = pair_kernel<K, SPLIT, SWAP, RMU>;
+  auto fn = pair_kernel<K, SPLIT, AUTO, RMU>;
   hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   fn<<<n_items, threads, lds, st>>>(jobs, items);
@@ -562,19 +550,19 @@ size_t pair_lds_bytes(int ncell, int nslots, int n_rp, int ncopy) {
   return (size_t)((ncopy * ncell * 4 + 15) & ~15) + (size_t)ncopy * nslots * ncell * 8 + (size_t)(n_rp + 1) * 8;
 }
 
-hipError_t launch_pair(int kind, bool split, bool swap, const KArgs *jobs, const Item *items, int n_items,
+hipError_t launch_pair(int kind, bool split, bool autop, const KArgs *jobs, const Item *items, int n_items,
                        size_t lds, int threads, hipStream_t st) {
   if (n_items == 0) return hipSuccess;
   const bool rmu = kind >= kRmuKind;
   if (rmu) kind -= kRmuKind;
-#define CPC_DISPATCH(KK)                                                                          \
-  if (kind == KK) {                                                                               \
-    if (rmu && swap) return launch_pair_t<KK, false, true, true>(jobs, items, n_items, threads, lds, st); \
-    if (rmu) return launch_pair_t<KK, false, false, true>(jobs, items, n_items, threads, lds, st);        \
-    if (split && swap) return launch_pair_t<KK, true, true>(jobs, items, n_items, threads, lds, st); \
-    if (split) return launch_pair_t<KK, true, false>(jobs, items, n_items, threads, lds, st);        \
-    if (swap) return launch_pair_t<KK, false, true>(jobs, items, n_items, threads, lds, st);         \
-    return launch_pair_t<KK, false, false>(jobs, items, n_items, threads, lds, st);                  \
+#define CPC_DISPATCH(KK)                                                                                  \
+  if (kind == KK) {                                                                                       \
+    if (rmu && autop) return launch_pair_t<KK, false, true, true>(jobs, items, n_items, threads, lds, st);  \
+    if (rmu) return launch_pair_t<KK, false, false, true>(jobs, items, n_items, threads, lds, st);          \
+    if (split && autop) return launch_pair_t<KK, true, true, false>(jobs, items, n_items, threads, lds, st); \
+    if (split) return launch_pair_t<KK, true, false, false>(jobs, items, n_items, threads, lds, st);        \
+    if (autop) return launch_pair_t<KK, false, true, false>(jobs, items, n_items, threads, lds, st);        \
+    return launch_pair_t<KK, false, false, false>(jobs, items, n_items, threads, lds, st);                  \
   }
   CPC_DISPATCH(kDD)
   CPC_DISPATCH(kSD)

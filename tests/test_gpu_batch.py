@@ -125,7 +125,7 @@ def test_autotune_keeps_valid_results(engine):
     try:
         tun, tried, results = engine.autotune(jobs)
         assert tried >= 6
-        assert tun["col_frac"] in (0.0, 0.35, 0.5) and tun["item_size"] in (0, 256, 1024)
+        assert tun["col_frac"] in (0.0, 0.35, 0.5) and tun["item_size"] in (0, 512, 4096)
         assert tun["block_threads"] in (0, 256)
         for k, ((outer, inner, pp), (h, st)) in enumerate(zip(jobs, results)):
             single, _ = engine.count(outer, inner, pp)
