@@ -132,3 +132,35 @@ def test_autotune_keeps_valid_results(engine):
             assert_hist_match(h, single, what=f"autotuned pass {k}")
     finally:
         engine.set_tuning()
+
+
+def test_config3_mocks_batch_against_oracle(engine, oracle_lib):
+    """The batched-mocks workload's shape at sizes the oracle checks in seconds: 16 independent
+    periodic boxes of side 250 with data sizes log-uniform in [1e4, 3e4] (clustered like the wg+ box),
+    2x randoms, 2x2x2 jackknife, 10 log rp bins in [0.1, 30) and 20 pi bins in [-60, 60); each mock's
+    SD, SRd and RsRd passes (do_corr.cpp:110-131), all 48 in one cpc_count_batch call with the
+    autotuned decomposition, every job against the oracle."""
+    from corr_pc_amd.passes import config_datainfo, setup_pass
+    sizes = synth.mock_sizes(16, n_min=10_000, n_max=30_000)
+    di = config_datainfo(synth.MOCK_BINNING)
+    jobs = []
+    for k, n in enumerate(sizes):
+        c = synth.mock_catalogs(k, int(n))
+        for name, g1, g2 in (("SD", c["D"], c["S"]), ("SRd", c["Rd"], c["S"]), ("RsRd", c["Rd"], c["Rs"])):
+            outer, inner, p, _ = setup_pass(name, g1, g2, di, precision=PREC_SPLIT)
+            jobs.append((outer, inner, p))
+    print(f"[case] config-3 mocks: {len(sizes)} boxes, {len(jobs)} passes, data sizes {sizes.min()}..{sizes.max()}",
+          flush=True)
+    try:
+        tun, tried, _ = engine.autotune(jobs)
+        print(f"[case] autotuned: {tun} after {tried} batches", flush=True)
+        results, tot = engine.count_batch(jobs)
+    finally:
+        engine.set_tuning()
+    nb = 0
+    for k, ((outer, inner, p), (h, st)) in enumerate(zip(jobs, results)):
+        ref, rst = oracle_lib.count(outer, inner, p)
+        assert_hist_match(h, ref, what=f"mock {k // 3} pass {('SD', 'SRd', 'RsRd')[k % 3]} batch vs oracle")
+        assert st["n_binned"] == rst["n_binned"] > 0
+        nb += st["n_binned"]
+    assert tot["n_binned"] == nb

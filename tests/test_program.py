@@ -337,3 +337,34 @@ def test_convergence_plan(tmp_path, which):
         assert kinds == {"SRd": "density_convergence", "RsRd": "dd", "SD": "density_convergence"}
     else:
         assert kinds == {"SRd": "dd", "RsRd": "dd", "SD": "shape_convergence"}
+
+
+def test_more_error_paths(tmp_path):
+    """The reference's own input checks, each with its message and exit status 1:
+    a redshift above z_max (read_dat.cpp:234), bad bin limits (initialization.cpp:299-303), dz = 0 on
+    the sky (:249-253), a distance table shorter than z_max / dz (:48-52), data_sorted 2 in a box
+    (:158-160)."""
+    spath, sinp, scols, table = sky_case(tmp_path, which=0)
+    # a galaxy above z_max: z_max below the catalogs' largest redshift
+    p = inputs.write_inp(str(tmp_path / "zmax.inp"), **dict(sinp, z_max=0.3))
+    r = run_prog(p, "--plan")
+    assert r.returncode == 1 and "wrong z at data" in r.stdout, r.stdout
+    p = inputs.write_inp(str(tmp_path / "dz.inp"), **dict(sinp, dz=0))
+    r = run_prog(p, "--plan")
+    assert r.returncode == 1 and "dz=0" in r.stdout, r.stdout
+    short = tmp_path / "short_dist.dat"
+    rows = open(sinp["distance_file"]).read().splitlines()
+    need = int(np.ceil(sinp["z_max"] / sinp["dz"])) + 1  # rows da_Z must read (initialization.cpp:48)
+    assert len(rows) >= need
+    short.write_text("\n".join(rows[: need - 1000]) + "\n")
+    p = inputs.write_inp(str(tmp_path / "table.inp"), **dict(sinp, distance_file=str(short)))
+    r = run_prog(p, "--plan")
+    assert r.returncode == 1 and "ended unexpectedly" in r.stdout, r.stdout
+    bpath, binp, bcols = box_case(tmp_path, which=0)
+    for bad in (dict(bin_r_min=5.0, bin_r_max=2.0), dict(bin_r_max=0)):
+        p = inputs.write_inp(str(tmp_path / "bins.inp"), **dict(binp, **bad))
+        r = run_prog(p, "--plan")
+        assert r.returncode == 1 and "Bad Bin limits" in r.stdout, (bad, r.stdout)
+    p = inputs.write_inp(str(tmp_path / "ds2.inp"), **dict(binp, data_sorted=2))
+    r = run_prog(p, "--plan")
+    assert r.returncode == 1 and "data_sorted==2 doesnot work with periodic box" in r.stdout, r.stdout
