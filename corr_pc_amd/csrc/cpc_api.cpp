@@ -29,24 +29,56 @@ namespace {
 const char *kVersion = "corr_pc_amd 0.2 (gfx950)";
 const double kR2D = 180.0 / M_PI;  // r2d of calcs.cpp:19
 
+// Device memory of one cpc_count_noalloc call: consecutive 256-byte aligned pieces of the library's
+// static scratch array, handed out in order and all returned when the call ends.
+struct Arena {
+  char *base = nullptr;
+  size_t cap = 0, off = 0;
+  bool overflow = false;
+  void *take(size_t bytes) {
+    const size_t at = (off + 255) & ~(size_t)255;
+    if (at + bytes > cap) {
+      overflow = true;
+      return nullptr;
+    }
+    off = at + bytes;
+    return base + at;
+  }
+};
+// Set while a cpc_count_noalloc call runs on this thread: device buffers then come from it.
+thread_local Arena *t_arena = nullptr;
+
 template <typename T>
 struct DevBuf {
   T *p = nullptr;
-  size_t cap = 0;  // elements
+  size_t cap = 0;         // elements
+  bool borrowed = false;  // p lies in the no-alloc scratch (never freed)
   hipError_t ensure(size_t n) {
     if (n <= cap && p) return hipSuccess;
-    if (p) (void)hipFree(p);
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    if (t_arena) {
+      // a no-alloc call only ever grows buffers that it borrowed itself
+      if (p && !borrowed) return hipErrorInvalidValue;
+      void *q = t_arena->take(bytes);
+      if (!q) return hipErrorOutOfMemory;
+      p = (T *)q;
+      cap = std::max<size_t>(n, 1);
+      borrowed = true;
+      return hipSuccess;
+    }
+    if (p && !borrowed) (void)hipFree(p);
     p = nullptr;
     cap = 0;
-    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    borrowed = false;
     hipError_t e = hipMalloc((void **)&p, bytes);
     if (e == hipSuccess) cap = std::max<size_t>(n, 1);
     return e;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !borrowed) (void)hipFree(p);
     p = nullptr;
     cap = 0;
+    borrowed = false;
   }
 };
 
@@ -378,6 +410,7 @@ struct JobBufs {
 
 struct cpc_ctx {
   int device = 0;
+  char *scratch = nullptr;  // this device's copy of the no-alloc scratch array (hipGetSymbolAddress)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   std::string err;
@@ -402,6 +435,55 @@ struct cpc_ctx {
   DevBuf<unsigned long long> accn_raw;
   std::vector<std::unique_ptr<JobBufs>> pool;
   std::map<std::tuple<double, double, int, int, int>, std::vector<double>> thr_cache;
+  // every device buffer a pass uses, exchanged wholesale around a no-alloc call
+  struct Swap {
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    DevBuf<uint32_t> keys_a, keys_b;
+    DevBuf<int32_t> vals_a, vals_b, order;
+    DevBuf<double> zk_b;
+    DevBuf<Item> items;
+    DevBuf<KArgs> jobs;
+    DevBuf<double> acc, acc_raw;
+    DevBuf<unsigned long long> accn, accn_raw;
+    std::vector<std::unique_ptr<JobBufs>> pool;
+    // borrowed buffers hold no memory of their own: forget them
+    void drop() {
+      sort_tmp = nullptr;
+      sort_tmp_bytes = 0;
+      keys_a = {};
+      keys_b = {};
+      vals_a = {};
+      vals_b = {};
+      order = {};
+      zk_b = {};
+      items = {};
+      jobs = {};
+      acc = {};
+      acc_raw = {};
+      accn = {};
+      accn_raw = {};
+      for (auto &b : pool) *b = JobBufs();
+      pool.clear();
+    }
+  };
+  void swap_buffers(Swap &o) {
+    std::swap(sort_tmp, o.sort_tmp);
+    std::swap(sort_tmp_bytes, o.sort_tmp_bytes);
+    std::swap(keys_a, o.keys_a);
+    std::swap(keys_b, o.keys_b);
+    std::swap(vals_a, o.vals_a);
+    std::swap(vals_b, o.vals_b);
+    std::swap(order, o.order);
+    std::swap(zk_b, o.zk_b);
+    std::swap(items, o.items);
+    std::swap(jobs, o.jobs);
+    std::swap(acc, o.acc);
+    std::swap(acc_raw, o.acc_raw);
+    std::swap(accn, o.accn);
+    std::swap(accn_raw, o.accn_raw);
+    std::swap(pool, o.pool);
+  }
 };
 
 namespace {
@@ -460,7 +542,9 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, b
     std::vector<int32_t> iota(n);
     std::iota(iota.begin(), iota.end(), 0);
     check(hipMemcpyAsync(ctx->vals_a.p, iota.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s), st, "copy");
-    check(sort_f64(ctx->sort_tmp, ctx->sort_tmp_bytes, c->d_z, ctx->zk_b.p, ctx->vals_a.p, ctx->order.p, n, s), st,
+    check(sort_f64(ctx->sort_tmp, ctx->sort_tmp_bytes, c->d_z, ctx->zk_b.p, ctx->vals_a.p, ctx->order.p, n, s,
+                   t_arena == nullptr),
+          st,
           "sort z");
     check(hipStreamSynchronize(s), st, "sync");
     if (st.code) return st;
@@ -469,7 +553,8 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, b
   KeyArgs ka{c->d_x, c->d_y, c->d_inc, c->d_jk, jk_on ? c->d_jkp : nullptr, n, G.ncx, G.ncy,
              G.x0, G.y0, G.hx, G.hy, nreg, p.include_prob_max};
   check(launch_keys(ka, order, ctx->keys_a.p, ctx->vals_a.p, s), st, "keys");
-  check(sort_u32(ctx->sort_tmp, ctx->sort_tmp_bytes, ctx->keys_a.p, ctx->keys_b.p, ctx->vals_a.p, ctx->vals_b.p, n, s),
+  check(sort_u32(ctx->sort_tmp, ctx->sort_tmp_bytes, ctx->keys_a.p, ctx->keys_b.p, ctx->vals_a.p, ctx->vals_b.p, n, s,
+                 t_arena == nullptr),
         st, "sort keys");
   if (st.code) return st;
   std::vector<uint32_t> hkeys(n);
@@ -1844,6 +1929,11 @@ cpc_status run_jobs(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *to
 
 }  // namespace
 
+// The no-alloc scratch (cpc_count_noalloc): every device the library runs on holds this array
+// until the process ends.  Host code reaches it only through cpc_ctx::scratch.
+constexpr size_t kScratchBytes = (size_t)CPC_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 extern "C" {
 
 const char *cpc_version(void) { return kVersion; }
@@ -1871,6 +1961,14 @@ cpc_status cpc_init(cpc_ctx **out, int device) {
     delete ctx;
     return CPC_ERR_DEVICE;
   }
+  // the no-alloc scratch: in host code `scratch` names a host-side stand-in, so its device address
+  // on this device (each device holds its own copy) is looked up once, with the device current
+  // (a failed lookup leaves it null: cpc_count_noalloc then reports the error, the rest works)
+  void *sp = nullptr;
+  if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) == hipSuccess && sp)
+    ctx->scratch = (char *)sp;
+  else
+    (void)hipGetLastError();
   *out = ctx;
   return CPC_OK;
 }
@@ -1889,6 +1987,8 @@ void cpc_destroy(cpc_ctx *ctx) {
   ctx->jobs.release();
   ctx->acc.release();
   ctx->accn.release();
+  ctx->acc_raw.release();
+  ctx->accn_raw.release();
   for (auto &b : ctx->pool) b->release();
   ctx->pool.clear();
   if (ctx->sort_tmp) (void)hipFree(ctx->sort_tmp);
@@ -2107,6 +2207,42 @@ cpc_status cpc_count(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci,
                      cpc_stats *stats) {
   const cpc_job job{co, ci, pp, out, stats};
   return run_jobs(ctx, 1, &job, nullptr, false);
+}
+
+cpc_status cpc_count_noalloc(cpc_ctx *ctx, const cpc_catalog *co, const cpc_catalog *ci, const cpc_params *pp,
+                             cpc_hist *out, cpc_stats *stats) {
+  if (!ctx) return CPC_ERR_INVALID;
+  if (!ctx->scratch) return fail(ctx, CPC_ERR_DEVICE, "no-alloc scratch unavailable on this device");
+  if (t_arena) return fail(ctx, CPC_ERR_INVALID, "cpc_count_noalloc called from inside another no-alloc call");
+  Arena A;
+  A.base = ctx->scratch;
+  A.cap = kScratchBytes;
+  // the context's own buffers step aside for the call: every device buffer of the pass is then
+  // borrowed from the scratch, and none of them outlives the call
+  cpc_ctx::Swap saved;
+  ctx->swap_buffers(saved);
+  const int64_t nmax = std::max(co ? co->n : 0, ci ? ci->n : 0);
+  size_t sort_need = 0;
+  cpc_status rc = CPC_OK;
+  if (sort_temp_bytes(std::max<int64_t>(nmax, 1), &sort_need) != hipSuccess) {
+    rc = fail(ctx, CPC_ERR_DEVICE, "sort size query failed");
+  } else {
+    ctx->sort_tmp = A.take(sort_need);
+    ctx->sort_tmp_bytes = ctx->sort_tmp ? sort_need : 0;
+    t_arena = &A;
+    const cpc_job job{co, ci, pp, out, stats};
+    rc = run_jobs(ctx, 1, &job, nullptr, false);
+    t_arena = nullptr;
+  }
+  ctx->swap_buffers(saved);  // the borrowed buffers go back with `saved` and are dropped
+  saved.drop();
+  if (A.overflow) {
+    (void)hipStreamSynchronize(ctx->stream);  // work launched before the shortfall ends first
+    return fail(ctx, CPC_ERR_NOMEM,
+                "the pass needs more device memory than the no-alloc scratch holds (" +
+                    std::to_string(kScratchBytes >> 20) + " MiB); use cpc_count");
+  }
+  return rc;
 }
 
 cpc_status cpc_count_batch(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *total) {

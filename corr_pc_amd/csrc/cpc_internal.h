@@ -235,9 +235,10 @@ hipError_t launch_sparse(const SparseArgs &A, unsigned long long *count, hipStre
 hipError_t launch_fp32_error(uint32_t b0, uint32_t n, unsigned long long *out, hipStream_t st);
 hipError_t launch_reduce(const ReduceArgs &A, hipStream_t st);
 hipError_t sort_u32(void *&tmp, size_t &tmp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
-                    int32_t *vout, int64_t n, hipStream_t st);
+                    int32_t *vout, int64_t n, hipStream_t st, bool may_alloc);
 hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *kout, const int32_t *vin,
-                    int32_t *vout, int64_t n, hipStream_t st);
+                    int32_t *vout, int64_t n, hipStream_t st, bool may_alloc);
+hipError_t sort_temp_bytes(int64_t n, size_t *bytes);
 size_t pair_lds_bytes(int ncell, int nslots, int n_rp, int ncopy = 1);
 // One launch over `n_items` items whose KArgs live in the device table `jobs` (all with the same
 // kind, precision and pass direction); `lds` covers the largest histogram among them.

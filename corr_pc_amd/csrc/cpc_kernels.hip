@@ -507,13 +507,15 @@ hipError_t launch_sparse(const SparseArgs &A, unsigned long long *count, hipStre
   return hipGetLastError();
 }
 
-// Stable sort of (key, value) pairs; temporary storage grows as needed.
+// Stable sort of (key, value) pairs; temporary storage grows as needed, unless `may_alloc` is false
+// (then a too small `tmp` is an error).
 template <typename KeyT>
 static hipError_t sort_pairs_impl(void *&tmp, size_t &tmp_bytes, const KeyT *kin, KeyT *kout, const int32_t *vin,
-                                  int32_t *vout, int64_t n, int end_bit, hipStream_t st) {
+                                  int32_t *vout, int64_t n, int end_bit, hipStream_t st, bool may_alloc) {
   size_t need = 0;
   hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, need, kin, kout, vin, vout, (int)n, 0, end_bit, st);
   if (e != hipSuccess) return e;
+  if (need > tmp_bytes && !may_alloc) return hipErrorOutOfMemory;
   if (need > tmp_bytes) {
     if (tmp) (void)hipFree(tmp);
     tmp = nullptr;
@@ -526,13 +528,25 @@ static hipError_t sort_pairs_impl(void *&tmp, size_t &tmp_bytes, const KeyT *kin
 }
 
 hipError_t sort_u32(void *&tmp, size_t &tmp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
-                    int32_t *vout, int64_t n, hipStream_t st) {
-  return sort_pairs_impl<uint32_t>(tmp, tmp_bytes, kin, kout, vin, vout, n, 32, st);
+                    int32_t *vout, int64_t n, hipStream_t st, bool may_alloc) {
+  return sort_pairs_impl<uint32_t>(tmp, tmp_bytes, kin, kout, vin, vout, n, 32, st, may_alloc);
 }
 
 hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *kout, const int32_t *vin,
-                    int32_t *vout, int64_t n, hipStream_t st) {
-  return sort_pairs_impl<double>(tmp, tmp_bytes, kin, kout, vin, vout, n, 64, st);
+                    int32_t *vout, int64_t n, hipStream_t st, bool may_alloc) {
+  return sort_pairs_impl<double>(tmp, tmp_bytes, kin, kout, vin, vout, n, 64, st, may_alloc);
+}
+
+// Temporary storage both sorts need for n pairs (a query: hipcub allocates nothing without storage).
+hipError_t sort_temp_bytes(int64_t n, size_t *bytes) {
+  size_t a = 0, b = 0;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, a, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                                    (const int32_t *)nullptr, (int32_t *)nullptr, (int)n, 0, 32);
+  if (e != hipSuccess) return e;
+  e = hipcub::DeviceRadixSort::SortPairs(nullptr, b, (const double *)nullptr, (double *)nullptr,
+                                         (const int32_t *)nullptr, (int32_t *)nullptr, (int)n, 0, 64);
+  *bytes = a > b ? a : b;
+  return e;
 }
 
 template <int K, bool SPLIT, bool AUTO, bool RMU>

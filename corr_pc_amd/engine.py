@@ -268,6 +268,8 @@ def load_library(path=LIB_PATH):
     lib.cpc_do_bining.argtypes = [POINTER(Params), POINTER(Norm), POINTER(Bining), c_char_p, POINTER(PairNorm),
                                   POINTER(c_int)]
     lib.cpc_do_bining.restype = c_int
+    lib.cpc_count_noalloc.argtypes = [c_void_p, c_void_p, c_void_p, POINTER(Params), POINTER(HistC), POINTER(Stats)]
+    lib.cpc_count_noalloc.restype = c_int
     lib.cpc_fp32_error.argtypes = [c_void_p, ctypes.c_float, ctypes.c_float, _dp]
     lib.cpc_fp32_error.restype = c_int
     _lib = lib
@@ -277,7 +279,7 @@ def load_library(path=LIB_PATH):
 EXPORTED_SYMBOLS = ["cpc_init", "cpc_destroy", "cpc_last_error", "cpc_version", "cpc_catalog_upload",
                     "cpc_catalog_free", "cpc_count", "cpc_count_batch", "cpc_rp_thresholds", "cpc_rand_stream",
                     "cpc_set_tuning", "cpc_autotune", "cpc_bin_edges", "cpc_box_rr", "cpc_weight_sum",
-                    "cpc_write_cross", "cpc_do_bining", "cpc_fp32_error"]
+                    "cpc_write_cross", "cpc_do_bining", "cpc_fp32_error", "cpc_count_noalloc"]
 
 
 REFERENCE_SEED = 124693517  # seed(), jk.cpp:40-43
@@ -421,6 +423,19 @@ class Engine:
         st = Stats()
         self._check(self.lib.cpc_count(self.ctx, ho, hi, ctypes.byref(params), ctypes.byref(hc), ctypes.byref(st)),
                     "count")
+        return hist, st.as_dict()
+
+    def count_noalloc(self, outer: Catalog, inner: Catalog, params: Params, hist: Histogram = None):
+        """The same pass through cpc_count_noalloc: no device allocation during the call (catalogs are
+        uploaded first), every device buffer from the library's static scratch array."""
+        ho = self.upload(outer)
+        hi = ho if inner is outer else self.upload(inner)
+        if hist is None:
+            hist = Histogram.for_params(params)
+        hc = hist.c_struct()
+        st = Stats()
+        self._check(self.lib.cpc_count_noalloc(self.ctx, ho, hi, ctypes.byref(params), ctypes.byref(hc),
+                                               ctypes.byref(st)), "count_noalloc")
         return hist, st.as_dict()
 
     def _jobs(self, jobs):

@@ -204,6 +204,21 @@ typedef struct cpc_job {
   cpc_stats *stats; /* may be NULL */
 } cpc_job;
 
+/* Size of the library's static device scratch array used by cpc_count_noalloc, in MiB. */
+#define CPC_NOALLOC_SCRATCH_MIB 256
+
+/* cpc_count without device allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory during the call (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Arguments, results and errors are those of cpc_count; the
+ * catalogs are uploaded beforehand (cpc_catalog_upload allocates).  Every device buffer of the pass
+ * (cell lists, sort space, work items, accumulators) is carved from one static device array of
+ * CPC_NOALLOC_SCRATCH_MIB MiB that each device the library runs on holds until the process ends; a
+ * pass that needs more fails with CPC_ERR_NOMEM and a message (use cpc_count then).  The call
+ * allocates and frees no device memory.  All calls on one device share that array: calls that overlap
+ * on one device (two contexts, or two threads) race and must be serialised by the caller. */
+cpc_status cpc_count_noalloc(cpc_ctx *ctx, const cpc_catalog *outer, const cpc_catalog *inner,
+                             const cpc_params *params, cpc_hist *out, cpc_stats *stats);
+
 /* Many passes at once: every job's `out` and `stats` as cpc_count(job) would write them (integer
  * counts identical; FP64 sums equal up to summation order).  The reference runs its passes one after
  * another (do_corr.cpp:63-131, once per catalog pair); here the work items of all box jobs are merged
