@@ -999,7 +999,9 @@ Status validate(const cpc_job &job, Plan &P) {
   if (p.auto_pairs != 0 && p.auto_pairs != 1) return {CPC_ERR_INVALID, "auto_pairs must be 0 or 1"};
   P.autop = p.auto_pairs == 1;
   if (P.autop && co != ci) return {CPC_ERR_INVALID, "the auto pass (corels1) takes one catalog on both sides"};
-  if (p.unsorted != 0 && p.unsorted != 1) return {CPC_ERR_INVALID, "unsorted must be 0 or 1"};
+  if (p.unsorted < 0 || p.unsorted > 2) return {CPC_ERR_INVALID, "unsorted must be 0, 1 or 2"};
+  if (p.unsorted == 2 && !sky)
+    return {CPC_ERR_INVALID, "unsorted = 2 (data_sorted 2, Dec order) is for the sky (initialization.cpp:158-160)"};
   if (P.autop && !p.unsorted && co->n > 0 && !co->z_sorted)
     return {CPC_ERR_INVALID, "the auto pass needs the catalog sorted by z (data_sorted=1, corr.cpp:129-132)"};
   if (p.precision != CPC_PREC_SPLIT && p.precision != CPC_PREC_FP64) return {CPC_ERR_INVALID, "unknown precision"};
@@ -1665,11 +1667,15 @@ Status run_sky(cpc_ctx *ctx, Plan &P) {
     }
     break;
   }
+  // the pass is deterministic, so the rerun queue holds every pair; a still larger count would mean
+  // pairs lost, never a partial copy
+  if (counters[3] > (unsigned long long)P.hq_cap)
+    return {CPC_ERR_DEVICE, "guard-band queue still overflowing after a rerun with room for every pair"};
   (void)hipEventElapsedTime(&P.kms, ctx->ev0, ctx->ev1);
   (void)hipEventElapsedTime(&P.kms_p, ctx->ev0, ctx->ev2);
   (void)hipEventElapsedTime(&P.kms_t, ctx->ev2, ctx->ev1);
   if (counters[3] > 0) {
-    P.hq_pairs.resize(counters[3]);
+    P.hq_pairs.resize(std::min<unsigned long long>(counters[3], (unsigned long long)P.hq_cap));
     check(hipMemcpyAsync(P.hq_pairs.data(), B.hq.p, P.hq_pairs.size() * sizeof(uint2), hipMemcpyDeviceToHost, s), st,
           "copy");
     check(hipStreamSynchronize(s), st, "sync");
@@ -1699,6 +1705,25 @@ int64_t assemble(const Plan &P, const std::vector<double> &acc, const std::vecto
     for (const uint2 &q : P.hq_pairs) host_sky_pair(p, P.dv, co, (int64_t)q.x, ci, (int64_t)q.y, P.jk_on, HH);
   }
   int64_t sparse = (int64_t)counters[2];
+  if (P.sky && p.unsorted == 2 && !P.autop && ci->n > 0) {
+    // data_sorted 2: corels2 starts each outer galaxy's scan at a per-thread pointer into the
+    // Dec-ordered inner catalog that only moves forward, past galaxies outside the Dec window
+    // (corr.cpp:356-362), and a scan ends at its first galaxy above the window (ret_dec,
+    // corr.cpp:74-76).  The engine counts every pair of the window instead.  Replayed here in
+    // catalog order, as one thread runs it: an outer galaxy whose window begins below the pointer
+    // (windows narrow and widen with redshift) or after the pointer ran off an empty window loses
+    // the galaxies in between; those outer galaxies are counted.
+    const std::vector<double> &dec = ci->h_y;
+    int64_t starts = 0;
+    for (int64_t i = 0; i < co->n; ++i) {
+      if (!co->h_inc.empty() && co->h_inc[i] > p.include_prob_max) continue;
+      const Win &w = P.wins[i];
+      while (starts < ci->n && (dec[starts] > w.decmax || dec[starts] < w.decmin)) ++starts;
+      const int64_t lo = std::lower_bound(dec.begin(), dec.end(), w.decmin) - dec.begin();
+      const int64_t hi = std::upper_bound(dec.begin(), dec.end(), w.decmax) - dec.begin();
+      if (lo < hi && starts > lo) ++sparse;
+    }
+  }
   if (P.sky && ci->z_sorted && ci->n > 0) {  // sparse-window diagnostic on the host
     for (int64_t i = 0; i < co->n; ++i) {
       const double inc = co->h_inc.empty() ? 0.0 : co->h_inc[i];

@@ -213,12 +213,15 @@ bool data_ini(DataInfo &d, const std::string &name) {
       d.do_auto_rebin = false;
       break;
   }
-  // data_sorted 0 reaches the engine as cpc_params.unsorted: do_pair_unsorted (corr.cpp:36-52) ends
-  // no scan, and the auto pass starts every image's scan at j = i+1 (corr.cpp:167-182).  data_sorted
-  // 2 (the sky only) sorts by Dec: the z test then ends no scan (ret_z = 1) and the Dec test ends a
-  // scan only above the window, which a scan from j = i+1 or from the window's first galaxy reaches
-  // last (ret_dec, initialization.cpp:607-612; the Dec window is two-sided), so its pairs are the
-  // unsorted ones with catalog order = Dec order.
+  // data_sorted 0 reaches the engine as cpc_params.unsorted = 1: do_pair_unsorted (corr.cpp:36-52)
+  // ends no scan, and the auto pass starts every image's scan at j = i+1 (corr.cpp:167-182).
+  // data_sorted 2 (the sky only, Dec order) reaches it as unsorted = 2: the z test ends no scan
+  // (ret_z = 1) and the Dec test ends a scan at its first galaxy above the window (ret_dec,
+  // initialization.cpp:607-612), so a scan from the window's first galaxy finds every pair of the
+  // window, which the engine counts; but corels2 starts scans at a forward-only pointer
+  // (corr.cpp:356-362), which skips galaxies when Dec windows are not ordered like the galaxies (their
+  // half-width changes with redshift) or runs off an empty window.  The engine reports the outer
+  // galaxies where one thread's pointer would do so (n_sparse_windows), and the program warns.
   d.RsRd_same = false;
   if (d.n_density_rand == 0 && d.which_corr <= 2) {
     d.RsRd_same = true;
@@ -695,7 +698,7 @@ cpc_params base_params(const DataInfo &d, int precision) {
     p.z_min = d.z_min;
     p.z_sep_min = d.z_sep_min;
   }
-  p.unsorted = d.data_sorted == 1 ? 0 : 1;
+  p.unsorted = d.data_sorted == 1 ? 0 : (d.data_sorted == 2 ? 2 : 1);
   return p;
 }
 
@@ -916,6 +919,11 @@ int run(int argc, char **argv) {
     for (size_t j = 0; j < jobs.size(); ++j) {
       const size_t k = job_pass[j];
       std::cout << passes[k].name << ": pairs in= " << stats[k].n_binned << std::endl;
+      if (stats[k].n_sparse_windows > 0)
+        std::cout << "  warning: " << passes[k].name << ": " << stats[k].n_sparse_windows
+                  << " outer galaxies have windows that corr_pc's forward-only start pointer would cut short "
+                     "(its count then depends on its thread schedule); every pair of those windows is counted"
+                  << std::endl;
     }
     std::cout << "pair counting: " << total.ms_total << " ms (" << total.ms_kernel << " ms on the GPU)" << std::endl;
   }

@@ -144,7 +144,12 @@ typedef struct cpc_params {
   /* data_sorted 0: do_pair_unsorted (corr.cpp:36-52) ends no scan, so every image's scan meets every
    * inner galaxy: the PBz = +1 image takes no first-galaxy rule, and the auto pass (corels1) pairs
    * the galaxies after the outer one in catalog order under every image (corr.cpp:167-182).  The
-   * catalogs need not be sorted.  0: data_sorted 1. */
+   * catalogs need not be sorted.  0: data_sorted 1.  2: data_sorted 2 (the sky only, catalogs in
+   * Dec order): the pairs of 1, that is every pair of each outer galaxy's window; corels2's Dec start
+   * pointer only moves forward (corr.cpp:356-362), so where windows are not ordered like the
+   * galaxies (their Dec half-width changes with redshift) or a window is empty, the reference skips
+   * some of those pairs, depending on its OpenMP schedule: cpc_stats.n_sparse_windows counts the
+   * outer galaxies whose window one thread's pointer would cut short. */
   int unsorted;
 } cpc_params;
 

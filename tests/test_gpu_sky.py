@@ -67,3 +67,29 @@ def test_sky_large_split(engine, oracle_lib):
     p = _params(KIND_SHAPE_DENSITY, n_jk, g_with_shape=1, which_corr=1, precision=PREC_SPLIT)
     st = _run(engine, oracle_lib, A, B, p, "sky large split")
     assert st["n_host_pairs"] < 1e-4 * st["n_binned"] + 100
+
+
+def _dec_sorted(cat):
+    o = np.argsort(cat.y, kind="stable")
+    return cat.subset(o)
+
+
+def test_sky_dec_sorted_input_counts_every_window_pair_and_flags_the_pointer(engine, oracle_lib):
+    """data_sorted 2 (unsorted = 2): catalogs in Dec order.  The counts are every pair of each
+    outer galaxy's window (the oracle's unsorted scans).  corels2's forward-only Dec start pointer
+    (corr.cpp:356-362) would cut windows short where a later outer galaxy's window begins lower
+    (its redshift, hence its Dec half-width, differs): with redshifts spread over 0.2-1 that happens
+    and n_sparse_windows says so; with every outer galaxy at one redshift the windows are ordered
+    like the galaxies and nothing is flagged."""
+    A, B, n_jk, _ = sky_catalogs(3000, 5000, 18, table=_TABLE)
+    A, B = _dec_sorted(A), _dec_sorted(B)
+    p = _params(KIND_DD, n_jk, precision=PREC_SPLIT)
+    p.unsorted = 2
+    st = _run(engine, oracle_lib, A, B, p, "sky dec-sorted mixed redshifts")
+    assert st["n_sparse_windows"] > 0
+    # the same outer galaxies all at the redshift of the first one: ordered windows
+    from corr_pc_amd.engine import Catalog
+    z0 = np.full(A.n, A.z[0])
+    C = Catalog(x=A.x, y=A.y, z=z0, jk=A.jk, jk_prob=A.jk_prob, dc=np.full(A.n, A.dc[0]), da=np.full(A.n, A.da[0]))
+    st = _run(engine, oracle_lib, C, B, p, "sky dec-sorted one redshift")
+    assert st["n_sparse_windows"] == 0
