@@ -3,12 +3,18 @@ FP64 kernel, on the 1M-galaxy wg+/w++ periodic box (L = 500 Mpc/h; 70% uniform +
 clumps; shapes and lognormal weights; 20 log rp bins in [0.1, 100); 40 pi bins in [-100, 100);
 27 jackknife regions).  One step is one full DD pass of corr_pc's which_corr=2 run (the SD pass:
 shape_shape_corel, 1M outer x 1M inner), from catalogs resident on the device to the finished
-histogram in host memory: cell lists, both pair kernels (full sample + jackknife touch) and the
+histogram in host memory: cell lists, the pair kernel, the device-side target reduction and the
 assembly of the jackknife layout.
+
+The line also carries the measured error of the weighted bins (split against FP64 on the full pass,
+and against the CPU oracle on the cpu_baseline sample), and, under "secondary", the other workloads
+at their real shapes: the 2e6-galaxy sky survey (split and FP64), the 256 batched mocks in one call
+against sequential calls, and the 2e7-galaxy box on every 8th outer galaxy.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -22,28 +28,58 @@ sys.path.insert(0, ROOT)
 FLOPS_PER_PAIR = {"DD": 12, "SD": 33, "SS": 53}
 PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak
 
-
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "pair_kernel_ss_split_primary_box1m_traffic.md")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "pair_kernel_ss_split_box1m_traffic.md")
+KERNEL_SOURCES = [os.path.join(ROOT, "corr_pc_amd", "csrc", f) for f in ("cpc_kernels.hip", "cpc_device.h",
+                                                                          "cpc_internal.h")]
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def kernel_source_sha():
+    """First 16 hex digits of the SHA-256 of the box kernel's sources, as the traffic summary records
+    the sources it was measured on."""
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic():
-    """HBM bytes per launch of the primary split kernel, from the committed rocprofv3 FETCH_SIZE and
-    WRITE_SIZE summary of this workload (None when the summary is absent)."""
+    """(HBM bytes per launch of the split kernel, note): from the committed rocprofv3 FETCH_SIZE and
+    WRITE_SIZE summary of this workload, only when it was measured on the kernel sources built now."""
     try:
+        vals = {}
         with open(TRAFFIC_SUMMARY) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+                if ":" in line and not line.startswith(("|", "#", " ", "`")):
+                    k, v = line.split(":", 1)
+                    vals[k.strip()] = v.strip()
+    except OSError:
+        return None, "no traffic summary under profiles/"
+    if vals.get("kernel_source_sha") != kernel_source_sha():
+        return None, (f"the traffic summary was measured on kernel sources {vals.get('kernel_source_sha')}, "
+                      f"not the built {kernel_source_sha()}")
+    try:
+        return float(vals["traffic_bytes_per_launch"]), "measured on these kernel sources"
+    except (KeyError, ValueError):
+        return None, "traffic summary without traffic_bytes_per_launch"
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
     return None
 
 
-def bench_mocks(args):
+def mocks_workload(args, steps=1, warmup=0):
     """The batched-mocks workload: independent wgg boxes (L = 250, data sizes log-uniform in
     [1e4, 5e5], clustered, 2x randoms, 2x2x2 jackknife; 10 log rp bins in [0.1, 30), 20 pi bins in
     [-60, 60)), each with its DD, DR and RR passes (SD, SRd, RsRd), all in one cpc_count_batch call
@@ -67,9 +103,7 @@ def bench_mocks(args):
         jobs += mine
         if k % args.tune_stride == 0:
             tune_jobs += mine
-        if (k + 1) % 32 == 0:
-            log(f"mocks generated: {k + 1}/{len(sizes)} ({time.perf_counter() - t:.1f}s)")
-    log(f"{len(sizes)} mocks, {int(sizes.sum())} data galaxies, {len(jobs)} passes ready in "
+    log(f"mocks: {len(sizes)} boxes, {int(sizes.sum())} data galaxies, {len(jobs)} passes ready in "
         f"{time.perf_counter() - t:.1f}s")
     eng = Engine(0)
     for o, i, _ in jobs:
@@ -77,20 +111,16 @@ def bench_mocks(args):
         eng.upload(i)
     t0 = time.perf_counter()
     tun, tried, _ = eng.autotune(tune_jobs)
-    log(f"autotune: {tun} after {tried} batches of {len(tune_jobs)} passes in {time.perf_counter() - t0:.1f}s")
-
-    for k in range(args.warmup):
+    log(f"mocks autotune: {tun} after {tried} batches of {len(tune_jobs)} passes in {time.perf_counter() - t0:.1f}s")
+    for k in range(warmup):
         _, tot = eng.count_batch(jobs)
-        log(f"warmup batch {k}: {tot['ms_total']:.0f} ms (device {tot['ms_kernel']:.0f} ms)")
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         res, tot = eng.count_batch(jobs)
-        log(f"batch step {k}: {tot['ms_total']:.0f} ms (device {tot['ms_kernel']:.0f} ms, build "
+        log(f"mocks batch step {k}: {tot['ms_total']:.0f} ms (device {tot['ms_kernel']:.0f} ms, build "
             f"{tot['ms_build']:.0f} ms), binned {tot['n_binned']}")
-    dt = (time.perf_counter() - t0) / args.steps
+    dt = (time.perf_counter() - t0) / steps
     n_binned = tot["n_binned"]
-
-    # the same passes one call at a time
     t0 = time.perf_counter()
     seq_dev = 0.0
     identical = True
@@ -99,42 +129,24 @@ def bench_mocks(args):
         seq_dev += st["ms_kernel"]
         identical = identical and bool(np.array_equal(hs.num, h.num))
     dt_seq = time.perf_counter() - t0
-    log(f"sequential: {dt_seq * 1e3:.0f} ms (device {seq_dev:.0f} ms), counts identical: {identical}")
-
-    out = {
-        "metric": "pair evals/sec: batched wgg mocks in one call vs sequential calls",
-        "value": n_binned / dt,
-        "unit": "pair evals/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": dt * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+f64",
-        "data": "synthetic",
-        "config": {"workload": f"{len(sizes)} wgg boxes L=250, data log-uniform in [1e4, 5e5] (clustered), 2x "
-                               f"randoms, 8 jackknife regions, DD+DR+RR passes, 10 log rp bins [0.1,30) x 20 pi "
-                               f"bins [-60,60)",
-                   "global_batch": int(sizes.sum()), "parallelism": "single GPU"},
-        "n_binned": n_binned,
-        "n_passes": len(jobs),
-        "tuning": tun,
-        "batch_device_ms": tot["ms_kernel"],
+    log(f"mocks sequential: {dt_seq * 1e3:.0f} ms (device {seq_dev:.0f} ms), counts identical: {identical}")
+    eng.close()
+    return {
+        "workload": f"{len(sizes)} wgg boxes L=250, data log-uniform in [1e4, 5e5] (clustered), 2x randoms, 8 "
+                    f"jackknife regions, DD+DR+RR passes, 10 log rp bins [0.1,30) x 20 pi bins [-60,60)",
+        "value": n_binned / dt, "unit": "pair evals/s", "ms_per_step": dt * 1e3, "n_binned": n_binned,
+        "n_passes": len(jobs), "tuning": tun, "batch_device_ms": tot["ms_kernel"],
+        "batch_host_ms": dt * 1e3 - tot["ms_kernel"],
         "sequential": {"value": n_binned / dt_seq, "ms": dt_seq * 1e3, "device_ms": seq_dev,
                        "counts_identical": identical},
         "batch_vs_sequential": dt_seq / dt,
     }
-    print(json.dumps(out), flush=True)
-    eng.close()
 
 
-def bench_large(args):
+def large_workload(args, steps=1, warmup=0):
     """The large box at its real shape: L = 1000, 2e7 galaxies (30% in 40,000 clumps) with shapes and
     weights, 25 log rp bins in [0.1, 200), 60 pi bins in [-150, 150), 64 jackknife regions; the SD
-    pass of every `--large-stride`-th outer galaxy against all 2e7 inner galaxies (1/8 of the pass's
-    work by default)."""
+    pass of every `--large-stride`-th outer galaxy against all 2e7 inner galaxies."""
     import numpy as np
 
     from corr_pc_amd import synth
@@ -146,43 +158,36 @@ def bench_large(args):
     di = config_datainfo(cats["config"])
     outer, inner, p, _ = setup_pass("SD", cats["D"], cats["S"], di, precision=PREC_SPLIT)
     outer = outer.subset(np.arange(0, outer.n, args.large_stride))
-    log(f"catalogs ready in {time.perf_counter() - t:.1f}s: outer {outer.n}, inner {inner.n}")
+    log(f"large: catalogs ready in {time.perf_counter() - t:.1f}s: outer {outer.n}, inner {inner.n}")
     eng = Engine(0)
     eng.upload(outer)
     eng.upload(inner)
-    for k in range(args.warmup):
-        _, st = eng.count(outer, inner, p)
-        log(f"warmup {k}: {st['ms_total']:.0f} ms")
+    for k in range(warmup):
+        eng.count(outer, inner, p)
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         _, st = eng.count(outer, inner, p)
-        log(f"step {k}: {st['ms_total']:.0f} ms (primary {st['ms_primary']:.0f} ms, touch {st['ms_touch']:.0f} ms, "
-            f"build {st['ms_build']:.0f} ms), binned {st['n_binned']}")
-    dt = (time.perf_counter() - t0) / args.steps
-    achieved = FLOPS_PER_PAIR["SS"] * st["n_binned"] / (st["ms_primary"] * 1e-3) / 1e12
-    out = {
-        "metric": "pair evals/sec: split-FP32 kernel, large wg+/w++ box at real shape",
-        "value": st["n_binned"] / dt, "unit": "pair evals/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": f"wg+/w++ periodic box L=1000, 2e7 galaxies (40,000 clumps), SD pass, every "
-                               f"{args.large_stride}th outer galaxy x all inner, 25 log rp bins [0.1,200) x 60 pi "
-                               f"bins [-150,150), 64 jackknife regions",
-                   "global_batch": outer.n, "parallelism": "single GPU"},
-        "n_binned": st["n_binned"], "n_candidates": st["n_candidates"], "n_fp64_reeval": st["n_fp64_reeval"],
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_TFLOPS, "traffic": None,
-                     "kernel": "pair_kernel<shape_shape, split, primary>", "ms_per_launch": st["ms_primary"]},
-    }
-    print(json.dumps(out), flush=True)
+        log(f"large step {k}: {st['ms_total']:.0f} ms (kernel {st['ms_primary']:.0f} ms, build {st['ms_build']:.0f} "
+            f"ms), binned {st['n_binned']}")
+    dt = (time.perf_counter() - t0) / steps
     eng.close()
+    achieved = FLOPS_PER_PAIR["SS"] * st["n_binned"] / (st["ms_primary"] * 1e-3) / 1e12
+    return {
+        "workload": f"wg+/w++ periodic box L=1000, 2e7 galaxies (40,000 clumps), SD pass, every {args.large_stride}th "
+                    f"outer galaxy x all inner, 25 log rp bins [0.1,200) x 60 pi bins [-150,150), 64 jackknife regions",
+        "value": st["n_binned"] / dt, "unit": "pair evals/s", "ms_per_step": dt * 1e3, "n_binned": st["n_binned"],
+        "n_candidates": st["n_candidates"], "n_fp64_reeval": st["n_fp64_reeval"],
+        "kernel_ms": st["ms_primary"], "frac_fp32_valu": achieved / PEAK_TFLOPS,
+    }
 
 
-def bench_sky(args):
+def sky_workload(args, steps=1, warmup=1):
     """The sky survey at its real shape: 2e6 galaxies in six RA/Dec fields (about 150 deg^2),
     0.2 < z < 1 with n(z) ~ z^2 exp(-(z/0.4)^1.5), flat LCDM distances, 100 jackknife regions, 20 log
     rp bins in [0.1, 100), 40 pi bins in [-100, 100); the SD pass of which_corr = 1 (shape_density,
     wg+), 2e6 x 2e6, split precision and plain FP64."""
+    import numpy as np
+
     from corr_pc_amd import synth
     from corr_pc_amd.engine import KIND_SHAPE_DENSITY, PREC_FP64, PREC_SPLIT, Catalog, Engine, make_sky_params
 
@@ -194,40 +199,43 @@ def bench_sky(args):
     params = {prec: make_sky_params(pt, 20, 0.1, 100.0, 40, -100.0, 100.0, kind=KIND_SHAPE_DENSITY, g_with_shape=2,
                                     which_corr=1, do_jk=1, n_jk=n_jk, precision=prec)
               for prec in (PREC_SPLIT, PREC_FP64)}
-    log(f"sky catalogs ready in {time.perf_counter() - t:.1f}s: {S.n} galaxies, {n_jk} jackknife regions")
+    log(f"sky: catalogs ready in {time.perf_counter() - t:.1f}s: {S.n} galaxies, {n_jk} jackknife regions")
     eng = Engine(0)
     eng.upload(D)
     eng.upload(S)
     res = {}
     for prec, label in ((PREC_SPLIT, "split"), (PREC_FP64, "fp64")):
-        for k in range(args.warmup):
+        for k in range(warmup):
             eng.count(D, S, params[prec])
         t0 = time.perf_counter()
-        for k in range(args.steps):
+        for k in range(steps):
             h, st = eng.count(D, S, params[prec])
-            log(f"{label} step {k}: {st['ms_total']:.0f} ms (primary {st['ms_primary']:.0f} ms, touch {st['ms_touch']:.0f} "
-                f"ms, build {st['ms_build']:.0f} ms), binned {st['n_binned']}, host-binned {st['n_host_pairs']}")
-        res[label] = ((time.perf_counter() - t0) / args.steps, st, h)
+            log(f"sky {label} step {k}: {st['ms_total']:.0f} ms (primary {st['ms_primary']:.0f} ms, touch "
+                f"{st['ms_touch']:.0f} ms, build {st['ms_build']:.0f} ms), binned {st['n_binned']}, host-binned "
+                f"{st['n_host_pairs']}")
+        res[label] = ((time.perf_counter() - t0) / steps, st, h)
+    eng.close()
     dt, st, h = res["split"]
     dt64, st64, h64 = res["fp64"]
-    import numpy as np
-    out = {
-        "metric": "pair evals/sec: split-FP32 vs FP64 sky kernel, wg+ 2M-galaxy survey",
-        "value": st["n_binned"] / dt, "unit": "pair evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": f"sky survey, 2e6 galaxies in 6 RA/Dec fields (~150 deg^2), 0.2<z<1, SD pass "
-                               f"(shape_density) 2e6 x 2e6, 20 log rp bins [0.1,100) x 40 pi bins [-100,100), "
-                               f"{n_jk} jackknife regions",
-                   "global_batch": S.n, "parallelism": "single GPU"},
-        "n_binned": st["n_binned"], "n_candidates": st["n_candidates"], "n_host_pairs": st["n_host_pairs"],
-        "ms_primary": st["ms_primary"], "ms_touch": st["ms_touch"],
-        "fp64": {"value": st64["n_binned"] / dt64, "ms_per_step": dt64 * 1e3, "ms_primary": st64["ms_primary"],
+    return {
+        "workload": f"sky survey, 2e6 galaxies in 6 RA/Dec fields (~150 deg^2), 0.2<z<1, SD pass (shape_density) "
+                    f"2e6 x 2e6, 20 log rp bins [0.1,100) x 40 pi bins [-100,100), {n_jk} jackknife regions",
+        "value": st["n_binned"] / dt, "unit": "pair evals/s", "ms_per_step": dt * 1e3, "n_binned": st["n_binned"],
+        "n_host_pairs": st["n_host_pairs"], "ms_primary": st["ms_primary"], "ms_touch": st["ms_touch"],
+        "fp64": {"value": st64["n_binned"] / dt64, "ms_per_step": dt64 * 1e3,
                  "counts_identical": bool(np.array_equal(h.num, h64.num))},
         "split_vs_fp64": dt64 / dt,
     }
+
+
+def standalone(name, fn, args):
+    """--workload mocks|large|sky: one workload as its own JSON line."""
+    r = fn(args, steps=args.steps, warmup=args.warmup)
+    out = {"metric": f"pair evals/sec: {name}", "value": r["value"], "unit": "pair evals/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": r["workload"], "parallelism": "single GPU"}, "detail": r}
     print(json.dumps(out), flush=True)
-    eng.close()
 
 
 def main():
@@ -239,9 +247,10 @@ def main():
                     help="CPU baseline: every n-th outer galaxy of the same pass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the sky, mocks and large-box workloads")
     ap.add_argument("--workload", choices=("wgplus", "mocks", "large", "sky"), default="wgplus",
-                    help="wgplus: the 1M-galaxy wg+/w++ box (default); mocks: batched wgg mocks; large: the "
-                         "2e7-galaxy box on a sample of outer galaxies; sky: the 2e6-galaxy survey")
+                    help="wgplus: the 1M-galaxy wg+/w++ box (default, with the others as secondary); mocks, "
+                         "large, sky: one of the secondary workloads alone")
     ap.add_argument("--large-stride", type=int, default=8, help="large workload: every n-th outer galaxy")
     ap.add_argument("--mocks", type=int, default=256, help="number of mocks of the mocks workload")
     ap.add_argument("--tune-stride", type=int, default=16, help="mocks workload: autotune on every n-th mock")
@@ -265,18 +274,19 @@ def main():
         sys.exit(2)
 
     if args.workload == "mocks":
-        bench_mocks(args)
+        standalone("batched wgg mocks in one call vs sequential calls", mocks_workload, args)
         return
     if args.workload == "large":
-        bench_large(args)
+        standalone("split-FP32 kernel, large wg+/w++ box at real shape", large_workload, args)
         return
     if args.workload == "sky":
-        bench_sky(args)
+        standalone("split-FP32 vs FP64 sky kernel, wg+ 2M-galaxy survey", sky_workload, args)
         return
 
     import numpy as np
 
     from corr_pc_amd import synth
+    from corr_pc_amd.compare import field_errors, kind_fields
     from corr_pc_amd.engine import PREC_FP64, PREC_SPLIT, Engine
     from corr_pc_amd.passes import config_datainfo, setup_pass
 
@@ -286,6 +296,7 @@ def main():
     di = config_datainfo(cfg)
     outer, inner, p_split, _ = setup_pass("SD", cats["D"], cats["S"], di, precision=PREC_SPLIT)
     _, _, p_fp64, _ = setup_pass("SD", cats["D"], cats["S"], di, precision=PREC_FP64)
+    fields = kind_fields(p_split.kind, p_split.estimator)
     log(f"catalogs ready in {time.perf_counter() - t:.1f}s: outer {outer.n}, inner {inner.n}")
 
     eng = Engine(0)
@@ -299,8 +310,8 @@ def main():
         for k in range(steps):
             h, st = eng.count(outer, inner, p)
             stats.append(st)
-            log(f"{label} step {k}: {st['ms_total']:.1f} ms (primary {st['ms_primary']:.1f} ms, touch "
-                f"{st['ms_touch']:.1f} ms, build {st['ms_build']:.1f} ms), binned {st['n_binned']}")
+            log(f"{label} step {k}: {st['ms_total']:.1f} ms (kernel {st['ms_primary']:.1f} ms, build "
+                f"{st['ms_build']:.1f} ms), binned {st['n_binned']}")
         return time.perf_counter() - t0, stats, h
 
     run(p_split, args.warmup, "warmup split")
@@ -308,16 +319,20 @@ def main():
     ms_step = dt / args.steps * 1e3
     n_binned = st_split[-1]["n_binned"]
     value = n_binned / (dt / args.steps)
-    ms_primary = float(np.mean([s["ms_primary"] for s in st_split]))
+    ms_kernel = float(np.mean([s["ms_primary"] for s in st_split]))
 
     fp64 = None
+    errors = {}
     if not args.no_fp64:
         run(p_fp64, 1, "warmup fp64")
         dt64, st64, h64 = run(p_fp64, args.steps, "fp64")
         v64 = st64[-1]["n_binned"] / (dt64 / args.steps)
         fp64 = {"value": v64, "ms_per_step": dt64 / args.steps * 1e3,
-                "ms_primary": float(np.mean([s["ms_primary"] for s in st64])),
+                "ms_kernel": float(np.mean([s["ms_primary"] for s in st64])),
                 "counts_identical": bool(np.array_equal(h64.num, h_split.num))}
+        # the stated error of every weighted bin (wgg wt_num; wg+ and w++ val and err sums) of the full
+        # pass: split against the FP64 kernel, per field, max over regions and cells
+        errors["split_vs_fp64_full_pass"] = field_errors(h_split, h64, fields)
 
     # the same SD pass as which_corr = 1 runs it (shape_density_corel: wg+ alone), one step per precision
     wg_only = None
@@ -331,15 +346,16 @@ def main():
         run(q_fp64, 1, "warmup wg+ fp64")
         d64, st64b, h64b = run(q_fp64, 1, "wg+ fp64")
         wg_only = {"kind": "shape_density (which_corr 1)", "split_ms_per_step": ds * 1e3,
-                   "fp64_ms_per_step": d64 * 1e3, "split_ms_primary": sts[-1]["ms_primary"],
-                   "fp64_ms_primary": st64b[-1]["ms_primary"], "split_vs_fp64": d64 / ds,
+                   "fp64_ms_per_step": d64 * 1e3, "split_ms_kernel": sts[-1]["ms_primary"],
+                   "fp64_ms_kernel": st64b[-1]["ms_primary"], "split_vs_fp64": d64 / ds,
                    "n_binned": sts[-1]["n_binned"], "counts_identical": bool(np.array_equal(hs.num, h64b.num))}
 
-    achieved = FLOPS_PER_PAIR["SS"] * n_binned / (ms_primary * 1e-3) / 1e12
+    traffic, traffic_note = measured_traffic()
+    achieved = FLOPS_PER_PAIR["SS"] * n_binned / (ms_kernel * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_TFLOPS, "traffic": measured_traffic(),
-                "kernel": "pair_kernel<shape_shape, split, primary>",
-                "flops_per_pair": FLOPS_PER_PAIR["SS"], "ms_per_launch": ms_primary}
+                "frac": achieved / PEAK_TFLOPS, "traffic": traffic, "traffic_note": traffic_note,
+                "kernel": "pair_kernel<shape_shape, split>", "flops_per_pair": FLOPS_PER_PAIR["SS"],
+                "ms_per_launch": ms_kernel, "kernel_source_sha": kernel_source_sha()}
 
     cpu = None
     if not args.no_cpu:
@@ -347,12 +363,25 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         sub = outer.subset(np.arange(0, outer.n, args.cpu_stride))
         t0 = time.perf_counter()
-        _, cst = oracle.count(sub, inner, p_split, n_threads=threads)
+        href, cst = oracle.count(sub, inner, p_split, n_threads=threads)
         cdt = time.perf_counter() - t0
         cpu = {"value": cst["n_binned"] / cdt, "unit": "pair evals/s", "cores": threads, "kind": "port",
+               "cpu_model": cpu_model(),
                "sample": f"same SD pass, every {args.cpu_stride}th outer galaxy ({sub.n} of {outer.n}) against all "
                          f"{inner.n} inner galaxies: {cst['n_binned']} binned pairs in {cdt:.1f} s"}
         log(f"cpu baseline: {cpu['value']:.3e} pairs/s on {threads} threads")
+        # the split kernel on the same sample against the oracle (corr_pc's CPU arithmetic)
+        hsub, _ = eng.count(sub, inner, p_split)
+        errors["split_vs_oracle_cpu_sample"] = field_errors(hsub, href, fields)
+    eng.close()
+
+    secondary = None
+    if not args.no_secondary:
+        secondary = {}
+        for name, fn in (("sky", sky_workload), ("mocks", mocks_workload), ("large", large_workload)):
+            t0 = time.perf_counter()
+            secondary[name] = fn(args)
+            log(f"secondary {name}: {time.perf_counter() - t0:.1f} s")
 
     out = {
         "metric": "pair evals/sec: split-FP32 vs FP64 GPU kernel, wg+ 1M-gal box, bit-exact bins",
@@ -377,11 +406,12 @@ def main():
         "fp64": fp64,
         "split_vs_fp64": (value / fp64["value"]) if fp64 else None,
         "wg_plus_only": wg_only,
+        "weighted_error": errors or None,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "secondary": secondary,
     }
     print(json.dumps(out), flush=True)
-    eng.close()
 
 
 if __name__ == "__main__":
