@@ -18,6 +18,10 @@
 #include <numeric>
 #include <string>
 #include <vector>
+#include <functional>
+#include <type_traits>
+#include <thread>
+#include <atomic>
 
 #include "../../include/cpc.h"
 #include "cpc_internal.h"
@@ -106,7 +110,15 @@ struct Side {
   std::vector<Group> groups;
   std::vector<int> colbeg;        // [ncells+1]
   std::vector<int64_t> colcount;  // galaxies per cell
+  // group boundaries found on the device, read back after every side of a call has been launched
+  DevBuf<int4> d_bounds;
+  std::vector<int4> h_bounds;
+  bool pending = false;
+  uint64_t nkeys = 0;
+  int nreg = 1;
+  const cpc_catalog *cat = nullptr;
   void release() {
+    d_bounds.release();
     rec.release();
     zs.release();
     idx.release();
@@ -424,6 +436,7 @@ struct cpc_ctx {
   DevBuf<uint32_t> keys_a, keys_b;
   DevBuf<int32_t> vals_a, vals_b, order;
   DevBuf<double> zk_b;
+  DevBuf<char> tables;  // every plan's small tables (stage_tables)
   // per call: work items of every launch, box pass arguments (two per job), and the accumulator
   // arenas (f64 histogram slots; counts and counters)
   DevBuf<Item> items;
@@ -442,6 +455,7 @@ struct cpc_ctx {
     DevBuf<uint32_t> keys_a, keys_b;
     DevBuf<int32_t> vals_a, vals_b, order;
     DevBuf<double> zk_b;
+    DevBuf<char> tables;
     DevBuf<Item> items;
     DevBuf<KArgs> jobs;
     DevBuf<double> acc, acc_raw;
@@ -457,6 +471,7 @@ struct cpc_ctx {
       vals_b = {};
       order = {};
       zk_b = {};
+      tables = {};
       items = {};
       jobs = {};
       acc = {};
@@ -476,6 +491,7 @@ struct cpc_ctx {
     std::swap(vals_b, o.vals_b);
     std::swap(order, o.order);
     std::swap(zk_b, o.zk_b);
+    std::swap(tables, o.tables);
     std::swap(items, o.items);
     std::swap(jobs, o.jobs);
     std::swap(acc, o.acc);
@@ -508,6 +524,8 @@ const std::vector<double> *cached_thresholds(cpc_ctx *ctx, const cpc_params &p, 
 
 // Build one side: keys, stable sort, gather, group tables.  `zfirst_plus_L` feeds the PBz>0 flag
 // (box only).  For the sky, `win_raw` (per original galaxy, outer side) is gathered as well.
+Status side_tables(cpc_ctx *ctx, Side &S, size_t ncells);
+
 Status build_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, bool jk_on, const cpc_params &p,
                   bool has_zfirst, double zfirst_plus_L, bool sky, const Win *win_raw, const uint8_t *aflags,
                   Side &S) {
@@ -519,17 +537,13 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, b
   S.colbeg.assign(ncells + 1, 0);
   S.colcount.assign(ncells, 0);
   S.n_incl = 0;
+  S.pending = false;
   if (n == 0) {
-    check(S.d_colbeg.ensure(S.colbeg.size()), st, "alloc");
-    check(hipMemcpyAsync(S.d_colbeg.p, S.colbeg.data(), S.colbeg.size() * sizeof(int), hipMemcpyHostToDevice, s),
-          st, "copy");
-    check(S.d_groups.ensure(1), st, "alloc");
     check(S.rec.ensure(kRecPad), st, "alloc");
     check(S.zs.ensure(kRecPad), st, "alloc");
     check(S.idx.ensure(kRecPad), st, "alloc");
     check(S.sr.ensure(1), st, "alloc");
     check(S.win.ensure(1), st, "alloc");
-    check(hipStreamSynchronize(s), st, "sync");
     return st;
   }
   if (check(ctx->keys_a.ensure(n), st, "alloc") || check(ctx->keys_b.ensure(n), st, "alloc") ||
@@ -557,58 +571,99 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, b
                  t_arena == nullptr),
         st, "sort keys");
   if (st.code) return st;
-  std::vector<uint32_t> hkeys(n);
-  std::vector<int32_t> hidx(n);
-  check(hipMemcpyAsync(hkeys.data(), ctx->keys_b.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s), st, "copy");
-  check(hipMemcpyAsync(hidx.data(), ctx->vals_b.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, s), st, "copy");
-  check(hipStreamSynchronize(s), st, "sync");
-  if (st.code) return st;
-  const int64_t ni = std::lower_bound(hkeys.begin(), hkeys.end(), 0xffffffffu) - hkeys.begin();
-  S.n_incl = ni;
-  for (int64_t a = 0; a < ni;) {  // groups: runs of equal keys
-    int64_t b = a + 1;
-    while (b < ni && hkeys[b] == hkeys[a]) ++b;
-    const uint32_t key = hkeys[a];
-    Group g{};
-    g.start = (int32_t)a;
-    g.end = (int32_t)b;
-    g.coin = key & 1u;
-    g.region = (int32_t)((key >> 1) % (uint32_t)nreg);
-    g.col = (int32_t)((key >> 1) / (uint32_t)nreg);
-    g.zlo = c->h_z[hidx[a]];
-    g.zhi = c->h_z[hidx[b - 1]];
-    S.groups.push_back(g);
-    S.colcount[g.col] += b - a;
-    a = b;
-  }
-  {
-    size_t gi = 0;
-    for (size_t col = 0; col < ncells; ++col) {
-      S.colbeg[col] = (int)gi;
-      while (gi < S.groups.size() && (size_t)S.groups[gi].col == col) ++gi;
+  // groups: runs of equal keys.  Found on the device by one binary search per possible key; the
+  // boundaries are read back once every side of the call has been launched (finish_side), and the
+  // records of every galaxy are gathered now (the excluded ones, last in key order, lie past the
+  // groups and are never read).  With very many possible keys the sorted keys are scanned here.
+  const uint64_t nkeys = (uint64_t)ncells * (uint64_t)nreg * 2u;
+  int64_t ng = 0;  // records gathered
+  S.pending = false;
+  if (nkeys <= (1u << 23)) {
+    if (check(S.d_bounds.ensure(nkeys + 1), st, "alloc")) return st;
+    check(launch_group_bounds(ctx->keys_b.p, ctx->vals_b.p, n, (uint32_t)nkeys, S.d_bounds.p, s), st, "groups");
+    S.pending = true;
+    S.nkeys = nkeys;
+    S.nreg = nreg;
+    S.cat = c;
+    ng = n;
+  } else {
+    std::vector<uint32_t> hkeys(n);
+    std::vector<int32_t> hidx(n);
+    check(hipMemcpyAsync(hkeys.data(), ctx->keys_b.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s), st, "copy");
+    check(hipMemcpyAsync(hidx.data(), ctx->vals_b.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, s), st, "copy");
+    check(hipStreamSynchronize(s), st, "sync");
+    if (st.code) return st;
+    const int64_t ni = std::lower_bound(hkeys.begin(), hkeys.end(), 0xffffffffu) - hkeys.begin();
+    for (int64_t a = 0; a < ni;) {
+      int64_t b = a + 1;
+      while (b < ni && hkeys[b] == hkeys[a]) ++b;
+      const uint32_t key = hkeys[a];
+      Group g{};
+      g.start = (int32_t)a;
+      g.end = (int32_t)b;
+      g.coin = key & 1u;
+      g.region = (int32_t)((key >> 1) % (uint32_t)nreg);
+      g.col = (int32_t)((key >> 1) / (uint32_t)nreg);
+      g.zlo = c->h_z[hidx[a]];
+      g.zhi = c->h_z[hidx[b - 1]];
+      S.groups.push_back(g);
+      S.colcount[g.col] += b - a;
+      a = b;
     }
-    S.colbeg[ncells] = (int)S.groups.size();
+    S.n_incl = ni;
+    ng = ni;
   }
-  if (check(S.rec.ensure(ni + kRecPad), st, "alloc") || check(S.zs.ensure(ni + kRecPad), st, "alloc") ||
-      check(S.idx.ensure(ni + kRecPad), st, "alloc") ||
-      check(S.d_groups.ensure(S.groups.size()), st, "alloc") || check(S.d_colbeg.ensure(S.colbeg.size()), st, "alloc"))
+  if (check(S.rec.ensure(ng + kRecPad), st, "alloc") || check(S.zs.ensure(ng + kRecPad), st, "alloc") ||
+      check(S.idx.ensure(ng + kRecPad), st, "alloc"))
     return st;
   // ED passes carry the third axis component in the record's phi slot (ED never reads phi)
   const double *phi_src = p.kind == CPC_KIND_ED ? c->d_e3 : c->d_phi;
-  GatherArgs ga{c->d_x, c->d_y, c->d_z, c->d_w, c->d_e1, c->d_e2, phi_src, ni, G.ncx, G.ncy, G.x0, G.y0, G.hx, G.hy,
+  GatherArgs ga{c->d_x, c->d_y, c->d_z, c->d_w, c->d_e1, c->d_e2, phi_src, ng, G.ncx, G.ncy, G.x0, G.y0, G.hx, G.hy,
                 p.box_size, window_lo(p), window_hi(p), p.rp_max, zfirst_plus_L, has_zfirst ? 1 : 0, aflags};
   check(launch_gather(ga, ctx->vals_b.p, S.rec.p, S.zs.p, S.idx.p, s), st, "gather");
   if (sky) {
-    if (check(S.sr.ensure(ni), st, "alloc") || check(S.win.ensure(ni), st, "alloc")) return st;
-    check(launch_gather_sky(ctx->vals_b.p, ni, c->d_sr, win_raw, S.sr.p, S.win.p, s), st, "gather sky");
+    if (check(S.sr.ensure(std::max<int64_t>(ng, 1)), st, "alloc") || check(S.win.ensure(std::max<int64_t>(ng, 1)), st, "alloc"))
+      return st;
+    check(launch_gather_sky(ctx->vals_b.p, ng, c->d_sr, win_raw, S.sr.p, S.win.p, s), st, "gather sky");
   }
-  if (!S.groups.empty())
-    check(hipMemcpyAsync(S.d_groups.p, S.groups.data(), S.groups.size() * sizeof(Group), hipMemcpyHostToDevice, s),
-          st, "copy");
-  check(hipMemcpyAsync(S.d_colbeg.p, S.colbeg.data(), S.colbeg.size() * sizeof(int), hipMemcpyHostToDevice, s), st,
-        "copy");
-  check(hipStreamSynchronize(s), st, "sync");
+  if (!S.pending) st = side_tables(ctx, S, ncells);
   return st;
+}
+
+// The group table of a side whose boundaries came back from the device (build_side).
+Status finish_side(cpc_ctx *ctx, Side &S, size_t ncells) {
+  if (!S.pending) return {};
+  S.pending = false;
+  const std::vector<int4> &hb = S.h_bounds;
+  const cpc_catalog *c = S.cat;
+  for (uint64_t key = 0; key < S.nkeys; ++key) {
+    const int4 q = hb[key];
+    if (q.x >= q.y) continue;
+    Group g{};
+    g.start = q.x;
+    g.end = q.y;
+    g.coin = (int32_t)(key & 1u);
+    g.region = (int32_t)((key >> 1) % (uint64_t)S.nreg);
+    g.col = (int32_t)((key >> 1) / (uint64_t)S.nreg);
+    g.zlo = c->h_z[q.z];
+    g.zhi = c->h_z[q.w];
+    S.groups.push_back(g);
+    S.colcount[g.col] += q.y - q.x;
+  }
+  S.n_incl = hb[S.nkeys].x;
+  return side_tables(ctx, S, ncells);
+}
+
+// Column starts of a side's groups (host; the tables reach the device with the call's other small
+// tables, stage_tables).
+Status side_tables(cpc_ctx *, Side &S, size_t ncells) {
+  size_t gi = 0;
+  for (size_t col = 0; col < ncells; ++col) {
+    S.colbeg[col] = (int)gi;
+    while (gi < S.groups.size() && (size_t)S.groups[gi].col == col) ++gi;
+  }
+  S.colbeg[ncells] = (int)S.groups.size();
+  return {};
 }
 
 // Work items over the block side: every group is cut into chunks of up to 64 galaxies (one wave
@@ -687,38 +742,56 @@ void make_items_box(const Side &B, const Side &T, bool jk_on, int nreg, int per_
     bool operator<(const Key &o) const { return std::tie(target, mode, reg) < std::tie(o.target, o.mode, o.reg); }
   };
   std::map<Key, std::vector<std::pair<double, Chunk>>> by_key;
-  std::map<std::tuple<int, int, int>, int> yidx;
+  std::vector<int> yidx((size_t)nreg * 2 * nreg, -1);  // (a, x, b) -> Y target number
   ytar.clear();
-  std::map<Key, double> reach;
+  // tile galaxies in reach of each column, by tile (region, coin): [col][2 * region + coin], filled
+  // on first use (block groups of one column share it)
+  const int nr2 = 2 * nreg;
+  std::vector<double> colsum;
+  std::vector<char> have((size_t)nc * nc, 0);
+  colsum.assign((size_t)nc * nc * nr2, 0.0);
+  std::vector<std::pair<Key, double>> reach;
   for (size_t g = 0; g < B.groups.size(); ++g) {
     const Group &G = B.groups[g];
     const int a = G.region, ci = G.coin;
-    const int bcx = G.col % nc, bcy = G.col / nc;
-    reach.clear();
-    for (const int2 &o : nbr) {
-      const int ux = ((bcx + o.x) % nc + nc) % nc, uy = ((bcy + o.y) % nc + nc) % nc;
-      const int col = uy * nc + ux;
-      for (int tg = T.colbeg[col]; tg < T.colbeg[col + 1]; ++tg) {
-        const Group &H = T.groups[tg];
-        const double cnt = (double)(H.end - H.start);
-        Key k{};
-        if (!jk_on) {
-          k = Key{1, kTileAll, 0};
-        } else if (H.region == a || H.coin == 0) {
-          if (ci)
-            k = Key{2 * a + 1, kTileSameOrC0, 0};
-          else
-            k = H.region == a ? Key{2 * a + 1, kTileSame, 0} : Key{2 * a, kTileOtherC0, 0};
-        } else {
-          const auto yk = std::make_tuple(a, ci, H.region);
-          auto it = yidx.find(yk);
-          if (it == yidx.end()) {
-            it = yidx.emplace(yk, (int)ytar.size()).first;
-            ytar.push_back(make_int3(a, ci, H.region));
-          }
-          k = Key{2 * nreg + it->second, kTileRegionC1, H.region};
+    double *tc = &colsum[(size_t)G.col * nr2];
+    if (!have[G.col]) {
+      have[G.col] = 1;
+      const int bcx = G.col % nc, bcy = G.col / nc;
+      for (const int2 &o : nbr) {
+        const int ux = ((bcx + o.x) % nc + nc) % nc, uy = ((bcy + o.y) % nc + nc) % nc;
+        const int col = uy * nc + ux;
+        for (int tg = T.colbeg[col]; tg < T.colbeg[col + 1]; ++tg) {
+          const Group &H = T.groups[tg];
+          tc[2 * H.region + H.coin] += (double)(H.end - H.start);
         }
-        reach[k] += cnt;
+      }
+    }
+    reach.clear();
+    if (!jk_on) {
+      double c = 0;
+      for (int k = 0; k < nr2; ++k) c += tc[k];
+      if (c > 0) reach.push_back({Key{1, kTileAll, 0}, c});
+    } else {
+      // tiles that touch region a only: region a, and (ci = 1) other regions' coin-0 tiles
+      double touch_a = tc[2 * a] + tc[2 * a + 1], other0 = 0;
+      for (int b = 0; b < nreg; ++b)
+        if (b != a) other0 += tc[2 * b];
+      if (ci) {
+        touch_a += other0;
+        if (touch_a > 0) reach.push_back({Key{2 * a + 1, kTileSameOrC0, 0}, touch_a});
+      } else {
+        if (touch_a > 0) reach.push_back({Key{2 * a + 1, kTileSame, 0}, touch_a});
+        if (other0 > 0) reach.push_back({Key{2 * a, kTileOtherC0, 0}, other0});
+      }
+      for (int b = 0; b < nreg; ++b) {
+        if (b == a || !(tc[2 * b + 1] > 0)) continue;
+        int &y = yidx[((size_t)a * 2 + ci) * nreg + b];
+        if (y < 0) {
+          y = (int)ytar.size();
+          ytar.push_back(make_int3(a, ci, b));
+        }
+        reach.push_back({Key{2 * nreg + y, kTileRegionC1, b}, tc[2 * b + 1]});
       }
     }
     for (const auto &kv : reach) {
@@ -926,6 +999,9 @@ struct Plan {
   // work
   std::vector<Item> items_p, items_t;
   std::vector<int3> ytar;  // box: (outer region, touches-outer flag, inner region) of targets 2 nreg + k
+  Grid G{};                // cell grid
+  int Rn = 0;              // box: neighbour reach in columns
+  int per_item = 8;        // chunks per work item
   // box with jackknife: the nP raw targets stay on the device and are reduced there into
   // 1 + n_jk downloaded histograms, the full sample and touch_r (lists rbeg/rlst)
   bool reduce = false;
@@ -1283,13 +1359,23 @@ Status setup(cpc_ctx *ctx, Plan &P) {
   }
   P.SO = &B.so;
   P.SI = same ? &B.so : &B.si;
-  Side &SO = *P.SO, &SI = *P.SI;
   P.ms_build = ms_since(t_build);
 
-  // ---- work items (cost: block galaxies x tile galaxies in reach)
-  const int per_item = std::max(1, item_size / 64);
+  P.G = G;
+  P.Rn = Rn;
+  P.sa = SA;
+  P.per_item = std::max(1, item_size / 64);
+  return st;
+}
+
+// The plan's work items, from its sides' group tables: host work only, so the plans of a batch
+// build theirs concurrently.
+void setup_items(Plan &P) {
+  const bool sky = P.sky, jk_on = P.jk_on;
+  const int nreg = P.nreg, per_item = P.per_item;
+  const Side &SO = *P.SO, &SI = *P.SI;
   if (!sky) {
-    make_items_box(SO, SI, jk_on, nreg, per_item, G.ncx, P.nbr, P.items_p, P.cost_p, P.chunks, P.ytar);
+    make_items_box(SO, SI, jk_on, nreg, per_item, P.G.ncx, P.nbr, P.items_p, P.cost_p, P.chunks, P.ytar);
     P.nP = (size_t)2 * nreg + P.ytar.size();
     P.nT = 0;
     if (jk_on) {
@@ -1309,11 +1395,6 @@ Status setup(cpc_ctx *ctx, Plan &P) {
         P.rlst.insert(P.rlst.end(), l.begin(), l.end());
         P.rbeg.push_back((int32_t)P.rlst.size());
       }
-      if (check(B.red_beg.ensure(P.rbeg.size()), st, "alloc") || check(B.red_lst.ensure(P.rlst.size()), st, "alloc"))
-        return st;
-      check(hipMemcpyAsync(B.red_beg.p, P.rbeg.data(), P.rbeg.size() * 4, hipMemcpyHostToDevice, s), st, "copy");
-      check(hipMemcpyAsync(B.red_lst.p, P.rlst.data(), P.rlst.size() * 4, hipMemcpyHostToDevice, s), st, "copy");
-      if (st.code) return st;
       P.reduce = true;
     }
   } else {
@@ -1321,20 +1402,28 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     make_items(SO, false, jk_on, per_item, flat, P.items_p, P.cost_p, P.chunks);
     if (jk_on) make_items(SI, true, jk_on, per_item, flat, P.items_t, P.cost_t, P.chunks);
   }
+}
 
-  // ---- device tables
-  if (check(B.thr.ensure(p.n_rp + 1), st, "alloc") ||
-      check(B.nbr.ensure(std::max<size_t>(P.nbr.size(), 1)), st, "alloc") ||
-      check(B.chunks.ensure(P.chunks.size()), st, "alloc"))
-    return st;
-  check(hipMemcpyAsync(B.thr.p, thr.data(), thr.size() * 8, hipMemcpyHostToDevice, s), st, "copy");
-  if (!P.nbr.empty())
-    check(hipMemcpyAsync(B.nbr.p, P.nbr.data(), P.nbr.size() * sizeof(int2), hipMemcpyHostToDevice, s), st, "copy");
-  if (!P.chunks.empty())
-    check(hipMemcpyAsync(B.chunks.p, P.chunks.data(), P.chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, s),
-          st, "copy");
-  if (st.code) return st;
-
+// Device tables of the plan (thresholds, neighbour offsets, chunks, reduction lists), its guard
+// bands and its kernel arguments.
+Status setup_tables(cpc_ctx *ctx, Plan &P) {
+  const cpc_params &p = P.p;
+  const cpc_catalog *co = P.co, *ci = P.ci;
+  const bool sky = P.sky, jk_on = P.jk_on, autop = P.autop;
+  const int ncell = P.ncell;
+  const Derived &dv = P.dv;
+  JobBufs &B = *P.B;
+  Status st;
+  hipStream_t s = ctx->stream;
+  const Grid &G = P.G;
+  const int Rn = P.Rn;
+  SkyArgs &SA = P.sa;
+  const std::vector<double> &thr = *P.thr;
+  Side &SO = *P.SO, &SI = *P.SI;
+  const int64_t nbig = std::max(co->n, ci->n);
+  (void)jk_on;
+  (void)s;
+  // (the device tables themselves were placed by stage_tables)
   const double two24 = std::ldexp(1.0, -24);
   double gz = 1.0;
   Guard gr{};
@@ -1488,6 +1577,60 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     // host-queue capacity for sky guard-band pairs; grown and the pass rerun if ever exceeded
     P.hq_cap = (int)std::min<int64_t>(std::max<int64_t>(1 << 16, nbig), 1 << 24);
   }
+  return st;
+}
+
+// The small device tables of every plan of a call (group tables and column starts of both sides,
+// reduction lists, thresholds, neighbour offsets, chunks) in one device region, filled by one copy
+// from a host staging buffer: a batch of many passes would otherwise pay thousands of small copies.
+// The tables' DevBufs then point into the region (borrowed: the region owns the memory).
+Status stage_tables(cpc_ctx *ctx, std::vector<Plan> &plans) {
+  Status st;
+  size_t total = 0;
+  struct Copy {
+    size_t at;
+    const void *src;
+    size_t bytes;
+  };
+  std::vector<Copy> copies;
+  std::vector<std::function<void(char *)>> binders;
+  auto add = [&](auto &buf, const void *src, size_t n) {
+    using T = typename std::remove_pointer<decltype(buf.p)>::type;
+    const size_t cap = std::max<size_t>(n, 1);
+    const size_t at = (total + 255) & ~(size_t)255;
+    total = at + cap * sizeof(T);
+    if (n) copies.push_back({at, src, n * sizeof(T)});
+    binders.push_back([&buf, at, cap](char *base) {
+      if (buf.p && !buf.borrowed && !t_arena) (void)hipFree(buf.p);
+      buf.p = reinterpret_cast<T *>(base + at);
+      buf.cap = cap;
+      buf.borrowed = true;
+    });
+  };
+  for (Plan &P : plans) {
+    JobBufs &B = *P.B;
+    // the outer side, and the inner side when it is another catalog (an auto pass has one side)
+    add(P.SO->d_groups, P.SO->groups.data(), P.SO->groups.size());
+    add(P.SO->d_colbeg, P.SO->colbeg.data(), P.SO->colbeg.size());
+    if (P.SI != P.SO) {
+      add(P.SI->d_groups, P.SI->groups.data(), P.SI->groups.size());
+      add(P.SI->d_colbeg, P.SI->colbeg.data(), P.SI->colbeg.size());
+    }
+    if (P.reduce) {
+      add(B.red_beg, P.rbeg.data(), P.rbeg.size());
+      add(B.red_lst, P.rlst.data(), P.rlst.size());
+    }
+    add(B.thr, P.thr->data(), P.thr->size());
+    add(B.nbr, P.nbr.data(), P.nbr.size());
+    add(B.chunks, P.chunks.data(), P.chunks.size());
+  }
+  if (check(ctx->tables.ensure(std::max<size_t>(total, 1)), st, "alloc")) return st;
+  std::vector<char> h(total);
+  for (const Copy &c : copies) std::memcpy(h.data() + c.at, c.src, c.bytes);
+  if (total) check(hipMemcpyAsync(ctx->tables.p, h.data(), total, hipMemcpyHostToDevice, ctx->stream), st, "copy");
+  // the copy reads `h` before it returns (pageable memory)
+  check(hipStreamSynchronize(ctx->stream), st, "sync");
+  for (auto &b : binders) b(ctx->tables.p);
   return st;
 }
 
@@ -1869,6 +2012,51 @@ cpc_status run_jobs(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *to
     Status st = setup(ctx, plans[i]);
     if (st.code) return fail(ctx, tag(i, st));
   }
+  {
+    // every side's group boundaries, once all sides are launched: one wait, then small copies
+    Status st;
+    check(hipStreamSynchronize(ctx->stream), st, "sync");
+    for (Plan &P : plans)
+      for (Side *S : {P.SO, P.SI})
+        if (S->pending) {
+          S->h_bounds.resize(S->nkeys + 1);
+          check(hipMemcpyAsync(S->h_bounds.data(), S->d_bounds.p, (S->nkeys + 1) * sizeof(int4), hipMemcpyDeviceToHost,
+                               ctx->stream),
+                st, "copy");
+        }
+    check(hipStreamSynchronize(ctx->stream), st, "sync");
+    if (st.code) return fail(ctx, st);
+    for (Plan &P : plans) {
+      const size_t ncells = (size_t)P.G.ncx * P.G.ncy;
+      st = finish_side(ctx, *P.SO, ncells);
+      if (!st.code && P.SI != P.SO) st = finish_side(ctx, *P.SI, ncells);
+      if (st.code) return fail(ctx, st);
+    }
+  }
+  // work items of every plan: host work, spread over threads when the batch is large
+  {
+    const int nthreads = (int)std::min<int64_t>(
+        {(int64_t)n_jobs, (int64_t)std::max(1u, std::thread::hardware_concurrency()), (int64_t)16});
+    if (nthreads <= 1) {
+      for (Plan &P : plans) setup_items(P);
+    } else {
+      std::atomic<int> next{0};
+      std::vector<std::thread> pool;
+      for (int t = 0; t < nthreads; ++t)
+        pool.emplace_back([&] {
+          for (int i = next++; i < n_jobs; i = next++) setup_items(plans[i]);
+        });
+      for (auto &th : pool) th.join();
+    }
+  }
+  {
+    Status st = stage_tables(ctx, plans);
+    if (st.code) return fail(ctx, st);
+  }
+  for (int i = 0; i < n_jobs; ++i) {
+    Status st = setup_tables(ctx, plans[i]);
+    if (st.code) return fail(ctx, tag(i, st));
+  }
   // accumulator arenas: per plan [primary f64 | touch f64] and [primary counts | touch counts | 4 counters]
   size_t nd = 0, nu = 0, nr = 0, nrn = 0;
   for (Plan &P : plans) {
@@ -2008,6 +2196,7 @@ void cpc_destroy(cpc_ctx *ctx) {
   ctx->vals_b.release();
   ctx->order.release();
   ctx->zk_b.release();
+  ctx->tables.release();
   ctx->items.release();
   ctx->jobs.release();
   ctx->acc.release();

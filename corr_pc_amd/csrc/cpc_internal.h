@@ -232,6 +232,8 @@ hipError_t launch_keys(const KeyArgs &A, const int32_t *order, uint32_t *keys, i
 hipError_t launch_gather(const GatherArgs &A, const int32_t *sidx, PRec *rec, double *zs, int32_t *idx,
                          hipStream_t st);
 hipError_t launch_sparse(const SparseArgs &A, unsigned long long *count, hipStream_t st);
+hipError_t launch_group_bounds(const uint32_t *keys, const int32_t *vals, int64_t n, uint32_t nkeys, int4 *out,
+                               hipStream_t st);
 hipError_t launch_fp32_error(uint32_t b0, uint32_t n, unsigned long long *out, hipStream_t st);
 hipError_t launch_reduce(const ReduceArgs &A, hipStream_t st);
 hipError_t sort_u32(void *&tmp, size_t &tmp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,

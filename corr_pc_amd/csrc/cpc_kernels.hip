@@ -400,6 +400,30 @@ __global__ void k_gather(GatherArgs A, const int32_t *__restrict__ sidx, PRec *_
   idx[p] = i;
 }
 
+// Group boundaries of a side from its sorted keys: for every possible key k < nkeys the records of
+// group k are [out[k].x, out[k].y), with the catalog indices of its first and last galaxy in .z and
+// .w (-1 when the group is empty); out[nkeys].x is the number of included records (excluded
+// galaxies carry key 0xffffffff >= nkeys).
+__global__ void k_group_bounds(const uint32_t *__restrict__ keys, const int32_t *__restrict__ vals, int64_t n,
+                               uint32_t nkeys, int4 *__restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > (int64_t)nkeys) return;
+  auto first_at_least = [&](uint64_t v) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+      const int64_t mid = lo + ((hi - lo) >> 1);
+      if ((uint64_t)keys[mid] < v)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    return lo;
+  };
+  const int64_t b = first_at_least((uint64_t)k);
+  const int64_t e = k < (int64_t)nkeys ? first_at_least((uint64_t)k + 1) : b;
+  out[k] = make_int4((int)b, (int)e, b < e ? vals[b] : -1, b < e ? vals[e - 1] : -1);
+}
+
 // Outer galaxies whose PBz = 0 or PBz = -1 window holds no inner galaxy while later ones exist: the
 // reference's forward-only start pointer runs off in that case (corr.cpp:361-369, 395-404).
 
@@ -497,6 +521,13 @@ hipError_t launch_gather(const GatherArgs &A, const int32_t *sidx, PRec *rec, do
   if (A.n == 0) return hipSuccess;
   const int bs = 256;
   k_gather<<<(unsigned)((A.n + bs - 1) / bs), bs, 0, st>>>(A, sidx, rec, zs, idx);
+  return hipGetLastError();
+}
+
+hipError_t launch_group_bounds(const uint32_t *keys, const int32_t *vals, int64_t n, uint32_t nkeys, int4 *out,
+                               hipStream_t st) {
+  const int bs = 256;
+  k_group_bounds<<<(unsigned)(((int64_t)nkeys + 1 + bs - 1) / bs), bs, 0, st>>>(keys, vals, n, nkeys, out);
   return hipGetLastError();
 }
 
