@@ -402,9 +402,8 @@ struct JobBufs {
   DevBuf<Chunk> chunks;
   DevBuf<uint8_t> aflags;
   DevBuf<Win> win_raw;
-  DevBuf<double2> reach;
   DevBuf<uint2> hq;
-  DevBuf<int32_t> red_beg, red_lst;  // box: target lists of the device-side reduction
+  DevBuf<int32_t> red_beg, red_lst;  // target lists of the device-side jackknife reduction
   void release() {
     so.release();
     si.release();
@@ -413,7 +412,6 @@ struct JobBufs {
     chunks.release();
     aflags.release();
     win_raw.release();
-    reach.release();
     hq.release();
     red_beg.release();
     red_lst.release();
@@ -666,64 +664,13 @@ Status side_tables(cpc_ctx *, Side &S, size_t ncells) {
   return {};
 }
 
-// Work items over the block side: every group is cut into chunks of up to 64 galaxies (one wave
-// each); chunks sharing a histogram target are sorted by cost and packed `per_item` at a time, and
-// the items run most expensive first.  `nbr_cost` gives, for a group, the number of tile galaxies
-// it may meet.  Chunks are appended to `chunks` in item order; `costs` receives each item's cost.
-template <typename Cost>
-void make_items(const Side &B, bool touch, bool jk_on, int per_item, Cost nbr_cost, std::vector<Item> &items,
-                std::vector<double> &costs, std::vector<Chunk> &chunks) {
-  std::map<int, std::vector<std::pair<double, Chunk>>> by_target;
-  for (size_t g = 0; g < B.groups.size(); ++g) {
-    const Group &G = B.groups[g];
-    if (touch && G.coin == 0) continue;
-    const double ncol = nbr_cost(G);
-    int target;
-    if (touch)
-      target = G.region;
-    else if (jk_on)
-      target = 2 * G.region + (G.coin ? 1 : 0);
-    else
-      target = 1;
-    auto &v = by_target[target];
-    for (int f = G.start; f < G.end; f += 64) {
-      const Chunk c{(int32_t)g, f, std::min(64, G.end - f), 0};
-      v.push_back({(double)c.count * ncol, c});
-    }
-  }
-  struct Packed {
-    double cost;
-    int target;
-    std::vector<Chunk> cs;
-  };
-  std::vector<Packed> packed;
-  for (auto &kv : by_target) {
-    auto &v = kv.second;
-    std::stable_sort(v.begin(), v.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
-    for (size_t a = 0; a < v.size(); a += per_item) {
-      Packed pk{0.0, kv.first, {}};
-      for (size_t b = a; b < std::min(v.size(), a + per_item); ++b) {
-        pk.cost += v[b].first;
-        pk.cs.push_back(v[b].second);
-      }
-      packed.push_back(std::move(pk));
-    }
-  }
-  std::stable_sort(packed.begin(), packed.end(), [](const Packed &a, const Packed &b) { return a.cost > b.cost; });
-  items.clear();
-  costs.clear();
-  for (const Packed &pk : packed) {
-    items.push_back(Item{(int32_t)chunks.size(), (int32_t)pk.cs.size(), pk.target, pk.cs[0].group, 0, 0, 0, 0});
-    costs.push_back(pk.cost);
-    chunks.insert(chunks.end(), pk.cs.begin(), pk.cs.end());
-  }
-}
-
-// Work items of a box pass, with the jackknife decomposed by the tile side so that every pair is
+// Work items of a pass, with the jackknife decomposed by the tile side so that every pair is
 // evaluated once.  A pair of an outer galaxy (region a, coin ci) and an inner galaxy (region b, coin
-// cj) touches region a when b == a or ci >= 0.5, and region b when b != a and cj >= 0.5
-// (corel_jk, jk.cpp:201-223).  Each item takes block chunks of one (a, ci) and the tile groups of one
-// filter, which fixes both flags, and counts into one target:
+// cj) touches region a when ci >= 0.5, or in the box also when b == a (`same_touches`: jk_check_PB
+// against jk_check_sky, jk.cpp:53-71), and region b when b != a and cj >= 0.5 (corel_jk,
+// jk.cpp:201-223).  Each item takes block chunks of one (a, ci) and the tile groups of one filter,
+// which fixes both flags, and counts into one target (sky with ci = 0: region a and coin-0 tiles ->
+// target 2a):
 //   ci = 1: tiles of region a or coin 0      -> target 2a+1      (touches a only)
 //           tiles of region b != a, coin 1   -> target Y(a, 1, b) (touches a and b)
 //   ci = 0: tiles of region a                -> target 2a+1      (touches a only)
@@ -734,9 +681,10 @@ void make_items(const Side &B, bool touch, bool jk_on, int per_item, Cost nbr_co
 // Chunks of one (target, filter) are sorted by cost and packed `per_item` at a time; items run
 // most expensive first.  The cost of a chunk is its galaxies times the tile galaxies its filter
 // takes in the neighbour columns.
-void make_items_box(const Side &B, const Side &T, bool jk_on, int nreg, int per_item, int nc,
-                    const std::vector<int2> &nbr, std::vector<Item> &items, std::vector<double> &costs,
-                    std::vector<Chunk> &chunks, std::vector<int3> &ytar) {
+template <typename ForNbr>
+void make_items_jk(const Side &B, const Side &T, bool jk_on, bool same_touches, int nreg, int per_item,
+                   size_t ncols, ForNbr for_nbr, std::vector<Item> &items, std::vector<double> &costs,
+                   std::vector<Chunk> &chunks, std::vector<int3> &ytar) {
   struct Key {
     int target, mode, reg;
     bool operator<(const Key &o) const { return std::tie(target, mode, reg) < std::tie(o.target, o.mode, o.reg); }
@@ -748,8 +696,8 @@ void make_items_box(const Side &B, const Side &T, bool jk_on, int nreg, int per_
   // on first use (block groups of one column share it)
   const int nr2 = 2 * nreg;
   std::vector<double> colsum;
-  std::vector<char> have((size_t)nc * nc, 0);
-  colsum.assign((size_t)nc * nc * nr2, 0.0);
+  std::vector<char> have(ncols, 0);
+  colsum.assign(ncols * nr2, 0.0);
   std::vector<std::pair<Key, double>> reach;
   for (size_t g = 0; g < B.groups.size(); ++g) {
     const Group &G = B.groups[g];
@@ -757,15 +705,12 @@ void make_items_box(const Side &B, const Side &T, bool jk_on, int nreg, int per_
     double *tc = &colsum[(size_t)G.col * nr2];
     if (!have[G.col]) {
       have[G.col] = 1;
-      const int bcx = G.col % nc, bcy = G.col / nc;
-      for (const int2 &o : nbr) {
-        const int ux = ((bcx + o.x) % nc + nc) % nc, uy = ((bcy + o.y) % nc + nc) % nc;
-        const int col = uy * nc + ux;
+      for_nbr(G.col, [&](int col) {
         for (int tg = T.colbeg[col]; tg < T.colbeg[col + 1]; ++tg) {
           const Group &H = T.groups[tg];
           tc[2 * H.region + H.coin] += (double)(H.end - H.start);
         }
-      }
+      });
     }
     reach.clear();
     if (!jk_on) {
@@ -780,6 +725,10 @@ void make_items_box(const Side &B, const Side &T, bool jk_on, int nreg, int per_
       if (ci) {
         touch_a += other0;
         if (touch_a > 0) reach.push_back({Key{2 * a + 1, kTileSameOrC0, 0}, touch_a});
+      } else if (!same_touches) {
+        // jk_check_sky: with coin 0 the outer galaxy touches nothing, its own region included
+        touch_a += other0;
+        if (touch_a > 0) reach.push_back({Key{2 * a, kTileSameOrC0, 0}, touch_a});
       } else {
         if (touch_a > 0) reach.push_back({Key{2 * a + 1, kTileSame, 0}, touch_a});
         if (other0 > 0) reach.push_back({Key{2 * a, kTileOtherC0, 0}, other0});
@@ -984,8 +933,8 @@ struct Plan {
   cpc_stats *stats = nullptr;
   // derived
   bool sky = false, jk_on = false, autop = false, split = false, use_split = false;
-  int R = 1, nreg = 1, ncell = 0, kind_code = 0, NV = 0, NE = 0, NSP = 1, NST = 1, threads = 256;
-  int row0_p = 0, ncopy_p = 1, row0_t = 0, ncopy_t = 1;  // LDS rows and copies per pass
+  int R = 1, nreg = 1, ncell = 0, kind_code = 0, NV = 0, NE = 0, NSP = 1, threads = 256;
+  int row0_p = 0, ncopy_p = 1;  // LDS rows and copies
   Derived dv{};
   const std::vector<double> *thr = nullptr;
   // device state and the host arrays copied into it (alive until the call's last synchronisation)
@@ -995,28 +944,28 @@ struct Plan {
   std::vector<Chunk> chunks;
   std::vector<uint8_t> aflags;
   std::vector<Win> wins;
-  std::vector<double2> reach;
+  std::vector<double2> reach;  // sky: RA/Dec half-widths of the outer windows per redshift bin
   // work
-  std::vector<Item> items_p, items_t;
-  std::vector<int3> ytar;  // box: (outer region, touches-outer flag, inner region) of targets 2 nreg + k
+  std::vector<Item> items_p;
+  std::vector<int3> ytar;  // (outer region, touches-outer flag, inner region) of targets 2 nreg + k
   Grid G{};                // cell grid
   int Rn = 0;              // box: neighbour reach in columns
   int per_item = 8;        // chunks per work item
-  // box with jackknife: the nP raw targets stay on the device and are reduced there into
+  // with the jackknife: the nP raw targets stay on the device and are reduced there into
   // 1 + n_jk downloaded histograms, the full sample and touch_r (lists rbeg/rlst)
   bool reduce = false;
   std::vector<int32_t> rbeg, rlst;
   size_t off_raw = 0, off_rawn = 0;
-  std::vector<double> cost_p, cost_t;
-  KArgs kp{}, kt{};  // box: primary and touch pass (accumulator pointers set at launch)
-  SkyArgs sa{};      // sky: arguments shared by both passes
+  std::vector<double> cost_p;
+  KArgs kp{};    // box: pass arguments (accumulator pointers set at launch)
+  SkyArgs sa{};  // sky: pass arguments
   int hq_cap = 1;
   // accumulators: element offsets into the context's arenas
-  size_t nP = 0, nT = 0;
-  size_t off_p = 0, off_t = 0, off_pn = 0, off_tn = 0, off_c = 0;
+  size_t nP = 0;
+  size_t off_p = 0, off_pn = 0, off_c = 0;
   // results
   double ms_build = 0;
-  float kms = 0, kms_p = 0, kms_t = 0;
+  float kms = 0, kms_p = 0, kms_r = 0;  // pass, pair kernel, reduction
   std::vector<uint2> hq_pairs;
 };
 
@@ -1134,9 +1083,7 @@ Status validate(const cpc_job &job, Plan &P) {
   P.NV = kNV[P.kind_code];
   P.NE = kNE[P.kind_code];
   P.NSP = 1 + P.NV + P.NE;
-  P.NST = 1 + P.NV;
   P.nP = (size_t)2 * P.nreg;
-  P.nT = (size_t)P.nreg;
   P.split = p.precision == CPC_PREC_SPLIT;
   return {};
 }
@@ -1164,7 +1111,7 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     return {CPC_ERR_UNSUPPORTED, "histogram of " + std::to_string(ncell) + " cells does not fit in LDS (" +
                                      std::to_string(lds_p) + " bytes)"};
   {
-    // LDS histogram per pass: passes with 7 or more slots per cell (shape kinds), whose run flushes
+    // LDS histogram: passes with 7 or more slots per cell (shape kinds), whose run flushes
     // dominate, get as many copies (at most 4) of the top rp rows as fit in the group's share of the
     // CU's LDS, keeping in LDS every row from rp >= rp_max / 20 up (rows below hold about 0.25% of
     // the pairs of a homogeneous sample and go to the global histogram directly: device-wide atomics
@@ -1192,7 +1139,6 @@ Status setup(cpc_ctx *ctx, Plan &P) {
       }
     };
     choose(P.NSP, P.row0_p, P.ncopy_p);
-    choose(P.NST, P.row0_t, P.ncopy_t);
   }
   // block galaxies per work item: four chunks per wave of the group (waves claim chunks as they
   // finish, so a few chunks each keep the group's waves busy until close to its end)
@@ -1257,7 +1203,6 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     const SkyTable T{p.sky_H, p.sky_dA, p.sky_n, p.sky_dz, p.z_max, p.z_sep_max};
     std::vector<Win> &wins = P.wins;
     wins.resize(co->n);
-    double reach_zup = 0, reach_zdn = 0;
     std::vector<double> dec_half;
     // RA/Dec reach per bin of the outer redshift
     double ozlo = INFINITY, ozhi = -INFINITY;
@@ -1292,8 +1237,6 @@ Status setup(cpc_ctx *ctx, Plan &P) {
       const double dh = std::max(co->h_y[i] - w.decmin, w.decmax - co->h_y[i]);
       rb.y = std::max(rb.y, dh * (1 + 1e-12) + 1e-9);
       dec_half.push_back(dh);
-      reach_zup = std::max(reach_zup, w.zmax - co->h_z[i]);
-      reach_zdn = std::max(reach_zdn, co->h_z[i] - w.zmin);
     }
     // RA/Dec grid over both catalogs, cells about half the median window
     double ralo = 360, rahi = 0, declo = 90, dechi = -90;
@@ -1319,14 +1262,6 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     G.hy = span_dec / G.ncy * (1 + 1e-12);
     G.x0 = ralo;
     G.y0 = declo;
-    SA.reach_zup = reach_zup * (1 + 1e-12) + 1e-12;
-    SA.reach_zdn = reach_zdn * (1 + 1e-12) + 1e-12;
-    SA.reach_nb = nb;
-    SA.reach_z0 = rz0;
-    SA.reach_inv = rinv;
-    if (check(B.reach.ensure(nb), st, "alloc")) return st;
-    check(hipMemcpyAsync(B.reach.p, reach.data(), nb * sizeof(double2), hipMemcpyHostToDevice, s), st, "copy");
-    SA.reach_tab = B.reach.p;
     if (autop && !p.unsorted) {
       // corels1's scan from j = i+1 stops at once when that next included galaxy lies below the
       // window (corr.cpp:168-186): such outer galaxies take no pairs
@@ -1374,10 +1309,38 @@ void setup_items(Plan &P) {
   const bool sky = P.sky, jk_on = P.jk_on;
   const int nreg = P.nreg, per_item = P.per_item;
   const Side &SO = *P.SO, &SI = *P.SI;
-  if (!sky) {
-    make_items_box(SO, SI, jk_on, nreg, per_item, P.G.ncx, P.nbr, P.items_p, P.cost_p, P.chunks, P.ytar);
+  {
+    const int ncx = P.G.ncx, ncy = P.G.ncy;
+    const size_t ncols = (size_t)ncx * ncy;
+    if (!sky) {
+      // periodic neighbour columns of the box
+      const std::vector<int2> &nbr = P.nbr;
+      auto for_nbr = [&](int col, auto &&fn) {
+        const int bcx = col % ncx, bcy = col / ncx;
+        for (const int2 &o : nbr) {
+          const int ux = ((bcx + o.x) % ncx + ncx) % ncx, uy = ((bcy + o.y) % ncy + ncy) % ncy;
+          fn(uy * ncx + ux);
+        }
+      };
+      make_items_jk(SO, SI, jk_on, true, nreg, per_item, ncols, for_nbr, P.items_p, P.cost_p, P.chunks, P.ytar);
+    } else {
+      // the sky: cells within the widest window reach of any outer galaxy (every RA cell when some
+      // window covers the full circle)
+      double rra = 0, rdec = 0;
+      for (const double2 &r : P.reach) {
+        rra = std::max(rra, r.x);
+        rdec = std::max(rdec, r.y);
+      }
+      const int dx = std::isfinite(rra) ? (int)std::min<double>(ncx, std::ceil(rra / P.G.hx) + 1) : ncx;
+      const int dy = (int)std::min<double>(ncy, std::ceil(rdec / P.G.hy) + 1);
+      auto for_nbr = [&](int col, auto &&fn) {
+        const int bcx = col % ncx, bcy = col / ncx;
+        for (int y = std::max(0, bcy - dy); y <= std::min(ncy - 1, bcy + dy); ++y)
+          for (int x = std::max(0, bcx - dx); x <= std::min(ncx - 1, bcx + dx); ++x) fn(y * ncx + x);
+      };
+      make_items_jk(SO, SI, jk_on, false, nreg, per_item, ncols, for_nbr, P.items_p, P.cost_p, P.chunks, P.ytar);
+    }
     P.nP = (size_t)2 * nreg + P.ytar.size();
-    P.nT = 0;
     if (jk_on) {
       // output 0: every target; output r+1: the targets whose pairs touch region r (2r+1 and the
       // Y targets through the outer galaxy, x = 1, or the inner one, b = r)
@@ -1397,10 +1360,6 @@ void setup_items(Plan &P) {
       }
       P.reduce = true;
     }
-  } else {
-    auto flat = [](const Group &) { return 1.0; };
-    make_items(SO, false, jk_on, per_item, flat, P.items_p, P.cost_p, P.chunks);
-    if (jk_on) make_items(SI, true, jk_on, per_item, flat, P.items_t, P.cost_t, P.chunks);
   }
 }
 
@@ -1496,7 +1455,6 @@ Status setup_tables(cpc_ctx *ctx, Plan &P) {
     P.kp.bz = SO.zs.p;
     P.kp.trec = SI.rec.p;
     P.kp.tz = SI.zs.p;
-    P.kp.phase_mode = jk_on ? 1 : 0;
     P.kp.nslots = P.NSP;
     P.kp.row0 = P.row0_p;
     P.kp.ncopy = P.ncopy_p;
@@ -1505,21 +1463,6 @@ Status setup_tables(cpc_ctx *ctx, Plan &P) {
     P.kp.tcol = SI.d_colbeg.p;
     P.kp.bidx = SO.idx.p;
     P.kp.tidx = SI.idx.p;
-    // touch pass: blocks = inner galaxies with coin >= 0.5, tiles = outer galaxies
-    P.kt = K;
-    P.kt.brec = SI.rec.p;
-    P.kt.bz = SI.zs.p;
-    P.kt.trec = SO.rec.p;
-    P.kt.tz = SO.zs.p;
-    P.kt.phase_mode = 2;
-    P.kt.nslots = P.NST;
-    P.kt.row0 = P.row0_t;
-    P.kt.ncopy = P.ncopy_t;
-    P.kt.bgrp = SI.d_groups.p;
-    P.kt.tgrp = SO.d_groups.p;
-    P.kt.tcol = SO.d_colbeg.p;
-    P.kt.bidx = SI.idx.p;
-    P.kt.tidx = SO.idx.p;
   } else {
     SkyArgs &A = SA;
     A.pmin = p.pi_min;
@@ -1634,31 +1577,27 @@ Status stage_tables(cpc_ctx *ctx, std::vector<Plan> &plans) {
   return st;
 }
 
-// All box plans: one launch per (kind, precision, pass direction), each over the items of every
-// plan it serves ordered by decreasing cost, so that the device starts the longest items first
-// whichever job they belong to.  Kernel times are those of the whole group of launches.
+// All box plans: one launch per (kind, precision, auto), each over the items of every plan it
+// serves ordered by decreasing cost, so that the device starts the longest items first whichever
+// job they belong to; then the jackknife reductions.  Kernel times are those of the whole group of
+// launches.
 Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
   Status st;
   if (plans.empty()) return st;
   hipStream_t s = ctx->stream;
-  // job table: entry 2i is plan i's primary pass, 2i+1 its touch pass
-  std::vector<KArgs> table(2 * plans.size());
+  // job table: entry i is plan i's pass
+  std::vector<KArgs> table(plans.size());
   for (size_t i = 0; i < plans.size(); ++i) {
     Plan &P = *plans[i];
-    KArgs &a = table[2 * i];
+    KArgs &a = table[i];
     a = P.kp;
     a.acc = P.reduce ? ctx->acc_raw.p + P.off_raw : ctx->acc.p + P.off_p;
     a.accnum = P.reduce ? ctx->accn_raw.p + P.off_rawn : ctx->accn.p + P.off_pn;
     a.counters = ctx->accn.p + P.off_c;
-    KArgs &b = table[2 * i + 1];
-    b = P.kt;
-    b.acc = ctx->acc.p + P.off_t;
-    b.accnum = ctx->accn.p + P.off_tn;
-    b.counters = ctx->accn.p + P.off_c;
   }
   struct Launch {
     int kind;
-    bool split, swap;
+    bool split;
     int threads;
     size_t lds, first, n;
   };
@@ -1672,29 +1611,25 @@ Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
         .push_back(i);
   std::vector<Item> all;
   std::vector<Launch> launches;
-  for (int swap = 0; swap < 1; ++swap)  // box passes have no touch launch (make_items_box)
-    for (const auto &g : groups) {
-      std::vector<std::pair<double, Item>> v;
-      size_t lds = 0;
-      int threads = 0;
-      for (size_t i : g.second) {
-        const Plan &P = *plans[i];
-        const std::vector<Item> &its = swap ? P.items_t : P.items_p;
-        const std::vector<double> &cs = swap ? P.cost_t : P.cost_p;
-        for (size_t k = 0; k < its.size(); ++k) {
-          Item it = its[k];
-          it.job = (int32_t)(2 * i + swap);
-          v.push_back({cs[k], it});
-        }
-        const int row0 = swap ? P.row0_t : P.row0_p, ncopy = swap ? P.ncopy_t : P.ncopy_p;
-        lds = std::max(lds, pair_lds_bytes((P.p.n_rp - row0) * P.p.n_pi, swap ? P.NST : P.NSP, P.p.n_rp, ncopy));
-        threads = P.threads;
+  for (const auto &g : groups) {
+    std::vector<std::pair<double, Item>> v;
+    size_t lds = 0;
+    int threads = 0;
+    for (size_t i : g.second) {
+      const Plan &P = *plans[i];
+      for (size_t k = 0; k < P.items_p.size(); ++k) {
+        Item it = P.items_p[k];
+        it.job = (int32_t)i;
+        v.push_back({P.cost_p[k], it});
       }
-      if (v.empty()) continue;
-      std::stable_sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
-      launches.push_back(Launch{g.first.first, g.first.second != 0, swap != 0, threads, lds, all.size(), v.size()});
-      for (const auto &x : v) all.push_back(x.second);
+      lds = std::max(lds, pair_lds_bytes((P.p.n_rp - P.row0_p) * P.p.n_pi, P.NSP, P.p.n_rp, P.ncopy_p));
+      threads = P.threads;
     }
+    if (v.empty()) continue;
+    std::stable_sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+    launches.push_back(Launch{g.first.first, g.first.second != 0, threads, lds, all.size(), v.size()});
+    for (const auto &x : v) all.push_back(x.second);
+  }
   if (check(ctx->jobs.ensure(table.size()), st, "alloc") || check(ctx->items.ensure(all.size()), st, "alloc"))
     return st;
   check(hipMemcpyAsync(ctx->jobs.p, table.data(), table.size() * sizeof(KArgs), hipMemcpyHostToDevice, s), st,
@@ -1707,7 +1642,6 @@ Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
                       l.lds, l.threads, s),
           st, "pair kernel");
   check(hipEventRecord(ctx->ev2, s), st, "event");
-  check(hipEventRecord(ctx->ev1, s), st, "event");
   for (Plan *Pp : plans) {  // raw targets into the full sample and touch histograms
     const Plan &P = *Pp;
     if (!P.reduce) continue;
@@ -1715,6 +1649,7 @@ Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
                   ctx->accn.p + P.off_pn, P.B->red_beg.p, P.B->red_lst.p, 1 + P.nreg, P.NSP, P.ncell};
     check(launch_reduce(ra, s), st, "reduce");
   }
+  check(hipEventRecord(ctx->ev1, s), st, "event");
   for (Plan *Pp : plans) {  // sparse-window diagnostic (corels2's starts pointer)
     const Plan &P = *Pp;
     if (P.autop || P.co->n == 0 || P.ci->n == 0) continue;
@@ -1724,41 +1659,36 @@ Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
   }
   check(hipStreamSynchronize(s), st, "sync");
   if (st.code) return st;
-  float kms = 0, kms_p = 0, kms_t = 0;
+  float kms = 0, kms_p = 0, kms_r = 0;
   (void)hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1);
   (void)hipEventElapsedTime(&kms_p, ctx->ev0, ctx->ev2);
-  (void)hipEventElapsedTime(&kms_t, ctx->ev2, ctx->ev1);
+  (void)hipEventElapsedTime(&kms_r, ctx->ev2, ctx->ev1);
   for (Plan *P : plans) {
     P->kms = kms;
     P->kms_p = kms_p;
-    P->kms_t = kms_t;
+    P->kms_r = kms_r;
   }
   return st;
 }
 
-// One sky plan: primary and touch launches, rerun with a larger host queue if guard-band pairs
-// overflowed it; the queued pairs are copied back for host binning.
+// One sky plan: the pair kernel, rerun with a larger host queue if guard-band pairs overflowed it,
+// and the jackknife reduction; the queued pairs are copied back for host binning.
 Status run_sky(cpc_ctx *ctx, Plan &P) {
   Status st;
   hipStream_t s = ctx->stream;
   JobBufs &B = *P.B;
-  const size_t np = P.nP * P.NSP * P.ncell, nt = P.nT * P.NST * P.ncell;
-  const size_t npn = P.nP * P.ncell, ntn = P.nT * P.ncell;
-  if (check(ctx->items.ensure(P.items_p.size() + P.items_t.size()), st, "alloc")) return st;
+  const size_t np = P.nP * P.NSP * P.ncell, npn = P.nP * P.ncell;
+  if (check(ctx->items.ensure(P.items_p.size()), st, "alloc")) return st;
   if (!P.items_p.empty())
     check(hipMemcpyAsync(ctx->items.p, P.items_p.data(), P.items_p.size() * sizeof(Item), hipMemcpyHostToDevice, s),
-          st, "copy");
-  if (!P.items_t.empty())
-    check(hipMemcpyAsync(ctx->items.p + P.items_p.size(), P.items_t.data(), P.items_t.size() * sizeof(Item),
-                         hipMemcpyHostToDevice, s),
           st, "copy");
   unsigned long long counters[4] = {0, 0, 0, 0};
   for (int attempt = 0; attempt < 2; ++attempt) {
     if (check(B.hq.ensure(P.hq_cap), st, "alloc")) return st;
-    check(hipMemsetAsync(ctx->acc.p + P.off_p, 0, np * 8, s), st, "memset");
-    check(hipMemsetAsync(ctx->acc.p + P.off_t, 0, nt * 8, s), st, "memset");
-    check(hipMemsetAsync(ctx->accn.p + P.off_pn, 0, npn * 8, s), st, "memset");
-    check(hipMemsetAsync(ctx->accn.p + P.off_tn, 0, ntn * 8, s), st, "memset");
+    double *const accp = P.reduce ? ctx->acc_raw.p + P.off_raw : ctx->acc.p + P.off_p;
+    unsigned long long *const accpn = P.reduce ? ctx->accn_raw.p + P.off_rawn : ctx->accn.p + P.off_pn;
+    check(hipMemsetAsync(accp, 0, np * 8, s), st, "memset");
+    check(hipMemsetAsync(accpn, 0, npn * 8, s), st, "memset");
     check(hipMemsetAsync(ctx->accn.p + P.off_c, 0, 4 * 8, s), st, "memset");
     SkyArgs A = P.sa;
     A.counters = ctx->accn.p + P.off_c;
@@ -1768,37 +1698,19 @@ Status run_sky(cpc_ctx *ctx, Plan &P) {
     if (!P.items_p.empty()) {
       SkyArgs C = A;
       C.n_items = (int)P.items_p.size();
-      C.phase_mode = P.jk_on ? 1 : 0;
       C.nslots = P.NSP;
       C.items = ctx->items.p;
       C.bgrp = P.SO->d_groups.p;
       C.tgrp = P.SI->d_groups.p;
       C.tcol = P.SI->d_colbeg.p;
-      C.acc = ctx->acc.p + P.off_p;
-      C.accnum = ctx->accn.p + P.off_pn;
+      C.acc = accp;
+      C.accnum = accpn;
       C.row0 = P.row0_p;
       C.ncopy = P.ncopy_p;
-      check(launch_sky(P.kind_code, P.use_split, false, C, P.threads, sky_side(*P.SO), sky_side(*P.SI), s), st,
+      check(launch_sky(P.kind_code, P.use_split, C, P.threads, sky_side(*P.SO), sky_side(*P.SI), s), st,
             "sky pair kernel");
     }
     check(hipEventRecord(ctx->ev2, s), st, "event");
-    if (!P.items_t.empty()) {
-      SkyArgs C = A;
-      C.n_items = (int)P.items_t.size();
-      C.phase_mode = 2;
-      C.nslots = P.NST;
-      C.items = ctx->items.p + P.items_p.size();
-      C.bgrp = P.SI->d_groups.p;
-      C.tgrp = P.SO->d_groups.p;
-      C.tcol = P.SO->d_colbeg.p;
-      C.acc = ctx->acc.p + P.off_t;
-      C.accnum = ctx->accn.p + P.off_tn;
-      C.row0 = P.row0_t;
-      C.ncopy = P.ncopy_t;
-      check(launch_sky(P.kind_code, P.use_split, true, C, P.threads, sky_side(*P.SI), sky_side(*P.SO), s), st,
-            "sky touch kernel");
-    }
-    check(hipEventRecord(ctx->ev1, s), st, "event");
     if (st.code) return st;
     check(hipMemcpyAsync(counters, ctx->accn.p + P.off_c, 4 * 8, hipMemcpyDeviceToHost, s), st, "copy");
     check(hipStreamSynchronize(s), st, "sync");
@@ -1810,13 +1722,21 @@ Status run_sky(cpc_ctx *ctx, Plan &P) {
     }
     break;
   }
+  if (P.reduce) {  // raw targets into the full sample and touch histograms
+    ReduceArgs ra{ctx->acc_raw.p + P.off_raw, ctx->accn_raw.p + P.off_rawn, ctx->acc.p + P.off_p,
+                  ctx->accn.p + P.off_pn, P.B->red_beg.p, P.B->red_lst.p, 1 + P.nreg, P.NSP, P.ncell};
+    check(launch_reduce(ra, s), st, "reduce");
+  }
+  check(hipEventRecord(ctx->ev1, s), st, "event");
+  check(hipStreamSynchronize(s), st, "sync");
+  if (st.code) return st;
   // the pass is deterministic, so the rerun queue holds every pair; a still larger count would mean
   // pairs lost, never a partial copy
   if (counters[3] > (unsigned long long)P.hq_cap)
     return {CPC_ERR_DEVICE, "guard-band queue still overflowing after a rerun with room for every pair"};
   (void)hipEventElapsedTime(&P.kms, ctx->ev0, ctx->ev1);
   (void)hipEventElapsedTime(&P.kms_p, ctx->ev0, ctx->ev2);
-  (void)hipEventElapsedTime(&P.kms_t, ctx->ev2, ctx->ev1);
+  (void)hipEventElapsedTime(&P.kms_r, ctx->ev2, ctx->ev1);
   if (counters[3] > 0) {
     P.hq_pairs.resize(std::min<unsigned long long>(counters[3], (unsigned long long)P.hq_cap));
     check(hipMemcpyAsync(P.hq_pairs.data(), B.hq.p, P.hq_pairs.size() * sizeof(uint2), hipMemcpyDeviceToHost, s), st,
@@ -1832,10 +1752,9 @@ Status run_sky(cpc_ctx *ctx, Plan &P) {
 int64_t assemble(const Plan &P, const std::vector<double> &acc, const std::vector<unsigned long long> &accn,
                  double ms_total) {
   const cpc_params &p = P.p;
-  const int ncell = P.ncell, nreg = P.nreg, R = P.R, NV = P.NV, NE = P.NE, NSP = P.NSP, NST = P.NST,
-            kc = P.kind_code;
-  const double *accp = acc.data() + P.off_p, *acct = acc.data() + P.off_t;
-  const unsigned long long *accpn = accn.data() + P.off_pn, *acctn = accn.data() + P.off_tn;
+  const int ncell = P.ncell, nreg = P.nreg, R = P.R, NV = P.NV, NE = P.NE, NSP = P.NSP, kc = P.kind_code;
+  const double *accp = acc.data() + P.off_p;
+  const unsigned long long *accpn = accn.data() + P.off_pn;
   const unsigned long long *counters = accn.data() + P.off_c;
   cpc_hist *out = P.out;
   const cpc_catalog *co = P.co, *ci = P.ci;
@@ -1908,35 +1827,20 @@ int64_t assemble(const Plan &P, const std::vector<double> &acc, const std::vecto
   if (P.jk_on) {
     std::vector<double> tv((size_t)ncell * CPC_NFIELD), tw(ncell);
     std::vector<int64_t> tn(ncell);
-    // the targets that touch each region: primary target 2r+1, the box's Y targets whose pairs
-    // touch r (through the outer galaxy, x = 1, or the inner one, b = r), the sky's touch target r
-    std::vector<std::vector<std::pair<int, int>>> touch_of(p.n_jk);  // (arena 0 primary / 1 touch, target)
     for (int r = 0; r < p.n_jk; ++r) {
-      if (P.reduce) {
-        touch_of[r].push_back({0, 1 + r});  // reduced on the device
-        continue;
-      }
-      touch_of[r].push_back({0, 2 * r + 1});
-      if (P.nT) touch_of[r].push_back({1, r});
-    }
-    for (int r = 0; r < p.n_jk; ++r) {
-      // touch_r (jk.cpp:138-171): num, wt_num and data[].val; shape vals only when which_corr != 0
+      // touch_r (jk.cpp:138-171): num, wt_num and data[].val; shape vals only when which_corr != 0.
+      // The device reduced every target whose pairs touch r into downloaded histogram 1 + r.
       std::fill(tv.begin(), tv.end(), 0.0);
       std::fill(tw.begin(), tw.end(), 0.0);
       std::fill(tn.begin(), tn.end(), 0);
-      for (const auto &src_t : touch_of[r]) {
-        const int src = src_t.first, t = src_t.second;
-        const double *a = src == 0 ? accp : acct;
-        const unsigned long long *num = src == 0 ? accpn : acctn;
-        const int NS = src == 0 ? NSP : NST;
-        for (int c = 0; c < ncell; ++c) {
-          const unsigned long long n = num[(size_t)t * ncell + c];
-          if (!n) continue;
-          tn[c] += (int64_t)n;
-          tw[c] += a[((size_t)t * NS + 0) * ncell + c];
-          for (int v = 0; v < NV; ++v)
-            tv[(size_t)c * CPC_NFIELD + kValField[kc][v]] += a[((size_t)t * NS + 1 + v) * ncell + c];
-        }
+      const int t = 1 + r;
+      for (int c = 0; c < ncell; ++c) {
+        const unsigned long long n = accpn[(size_t)t * ncell + c];
+        if (!n) continue;
+        tn[c] += (int64_t)n;
+        tw[c] += accp[((size_t)t * NSP + 0) * ncell + c];
+        for (int v = 0; v < NV; ++v)
+          tv[(size_t)c * CPC_NFIELD + kValField[kc][v]] += accp[((size_t)t * NSP + 1 + v) * ncell + c];
       }
       if (host_pairs)
         for (int c = 0; c < ncell; ++c) {
@@ -1981,7 +1885,7 @@ int64_t assemble(const Plan &P, const std::vector<double> &acc, const std::vecto
     st->n_sparse_windows = sparse;
     st->ms_kernel = P.kms;
     st->ms_primary = P.kms_p;
-    st->ms_touch = P.kms_t;
+    st->ms_reduce = P.kms_r;
     st->ms_build = P.ms_build;
     st->n_items_primary = (int64_t)P.items_p.size();
     st->n_host_pairs = (int64_t)P.hq_pairs.size();
@@ -2057,7 +1961,8 @@ cpc_status run_jobs(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *to
     Status st = setup_tables(ctx, plans[i]);
     if (st.code) return fail(ctx, tag(i, st));
   }
-  // accumulator arenas: per plan [primary f64 | touch f64] and [primary counts | touch counts | 4 counters]
+  // accumulator arenas: per plan the downloaded f64 histograms and [counts | 4 counters]; with the
+  // jackknife the raw targets in arenas of their own
   size_t nd = 0, nu = 0, nr = 0, nrn = 0;
   for (Plan &P : plans) {
     const size_t nP_dl = P.reduce ? (size_t)(1 + P.nreg) : P.nP;  // targets downloaded
@@ -2069,12 +1974,8 @@ cpc_status run_jobs(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *to
     }
     P.off_p = nd;
     nd += nP_dl * P.NSP * P.ncell;
-    P.off_t = nd;
-    nd += P.nT * P.NST * P.ncell;
     P.off_pn = nu;
     nu += nP_dl * P.ncell;
-    P.off_tn = nu;
-    nu += P.nT * P.ncell;
     P.off_c = nu;
     nu += 4;
   }
@@ -2124,13 +2025,13 @@ cpc_status run_jobs(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *to
       if (P.sky) {
         t.ms_kernel += P.kms;
         t.ms_primary += P.kms_p;
-        t.ms_touch += P.kms_t;
+        t.ms_reduce += P.kms_r;
       }
     }
     if (!box.empty()) {  // box plans share their launches
       t.ms_kernel += box[0]->kms;
       t.ms_primary += box[0]->kms_p;
-      t.ms_touch += box[0]->kms_t;
+      t.ms_reduce += box[0]->kms_r;
     }
     t.n_sparse_windows = sparse;
     t.ms_total = ms_since(t_start);

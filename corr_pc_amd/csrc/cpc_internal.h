@@ -56,7 +56,7 @@ struct alignas(16) Chunk {
 // A unit of work (one workgroup): chunks [begin, begin + n) of its job's chunk table, all from
 // groups of the same jackknife region and coin, so that they share one histogram target; group0 is
 // one of those groups.  `job` selects the pass arguments (KArgs) in the launch's job table.
-// Box items also select the tile groups they meet (`tmode`, `treg`, see TileFilter), so that every
+// Items also select the tile groups they meet (`tmode`, `treg`, see TileFilter), so that every
 // pair an item counts belongs to its one target.
 struct alignas(16) Item {
   int32_t begin, n, target, group0;
@@ -95,7 +95,6 @@ struct KArgs {
   // copies (lane l accumulates into copy l % ncopy); cells below go to the global histogram directly
   int row0, ncopy;
   int gws, n_nbr;
-  int phase_mode;      // 0 primary pass without jackknife, 1 primary with jackknife, 2 touch pass
   int nslots;          // f64 slots accumulated by this pass
   int auto_mode;       // corels1 pass: 1 (sorted) pairs through PBz <= 0 images only with the inner
                        // galaxy after the outer one (z, then catalog index); 2 (unsorted) through every
@@ -144,17 +143,9 @@ struct alignas(64) Win {
 struct SkyArgs {
   double pmin, pmax, psize;
   int n_rp, n_pi, ncell, gws, n_items, nslots, lin;
-  int phase_mode;          // 0 no jackknife, 1 jackknife (primary pass), 2 touch pass
   // cell grid over RA/Dec (degrees)
   double ra0, dec0, hra, hdec;
   int nra, ndec;
-  // touch pass: how far an outer window can reach from its galaxy: in z over all outer galaxies, in
-  // RA and Dec per bin of the outer galaxy's z ({ra, dec} half-widths, ra = inf for a full-circle
-  // window), bin b = floor((z - reach_z0) * reach_inv) clamped to [0, reach_nb)
-  double reach_zup, reach_zdn;
-  const double2 *reach_tab;
-  int reach_nb;
-  double reach_z0, reach_inv;
   double dsum;             // bound on the device/host difference of the great-circle cosine sum
   // the reference's pi and rp of a sky pair (choose_rp_pi_func): pim 0 DC_j - DC_i, 1 zero, 2 ephi of
   // the g_with_shape galaxy; rpm 0 dA (DA_i + DA_j)/2, 1 dA DA_i, 2 dA in degrees; rmu bins r and mu
@@ -256,7 +247,7 @@ struct SkySide {
   const SRec *sr;
   const Win *win;  // outer side only
 };
-hipError_t launch_sky(int kind, bool split, bool swap, const SkyArgs &P, int threads, const SkySide &B,
+hipError_t launch_sky(int kind, bool split, const SkyArgs &P, int threads, const SkySide &B,
                       const SkySide &T, hipStream_t st);
 hipError_t launch_gather_sky(const int32_t *sidx, int64_t n, const SRec *src_sr, const Win *src_win, SRec *sr,
                              Win *win, hipStream_t st);

@@ -15,14 +15,15 @@
 // The split path classifies in FP32 on column-local coordinates and sends every candidate that lies
 // within a proven error bound of a bin edge or window edge to the FP64 path.
 //
-// Jackknife.  Every pair is accumulated once, into a histogram selected per (item, phase):
-//   primary pass (block = outer catalog):
-//     target 2a+1: pairs that touch region a through the outer galaxy (same region, or coin >= 0.5)
-//     target 2a  : pairs that do not (different region and outer coin < 0.5)
-//   touch pass (block = inner galaxies of region b with coin >= 0.5, tiles = outer galaxies of other
-//     regions): target b.
-// The host rebuilds full = sum of primary targets and touch_r = target 2r+1 + touch-pass target r
-// (jk.cpp:201-223), then leave-one-out jk_r = full - touch_r (jk.cpp:74-81).
+// Jackknife.  Every pair is evaluated once (block = outer catalog) and accumulated into the target
+// of its item.  An item's block galaxies share a region a and coin, and its tile filter (Item tmode,
+// treg) takes only tile groups whose pairs touch the same regions, so the target fixes them:
+//   target 2a+1   : pairs that touch region a only (through the outer galaxy)
+//   target 2a     : pairs that touch no region
+//   target Y(a,x,b): pairs with an inner galaxy of region b != a and coin >= 0.5, which touch b, and
+//                    a as well when x = 1
+// (jk.cpp:201-223).  k_reduce then adds every target into the full sample and every target that
+// touches r into touch_r; the host forms leave-one-out jk_r = full - touch_r (jk.cpp:74-81).
 
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>

@@ -10,7 +10,10 @@
 //     pair whose rp lies within a proven bound of a bin edge (or whose sum lies near the 0.9999999999
 //     clamp) is handed to the host, which bins it with the reference's libm;
 //   * jackknife: the outer galaxy's region is touched iff its jk_prob >= 0.5 whatever the inner region
-//     (jk_check_sky, jk.cpp:53-57), so there is a single phase.
+//     (jk_check_sky, jk.cpp:53-57), the inner galaxy's iff its jk_prob >= 0.5 and the regions differ
+//     (corel_jk, jk.cpp:201-223).  As in the box kernel, each item's tile filter (tmode, treg) takes
+//     only tile groups whose pairs share one set of touched regions, and the item counts into that
+//     set's target; a reduction adds the targets into the full sample and the touch histograms.
 
 #include <hip/hip_runtime.h>
 
@@ -59,18 +62,16 @@ __device__ __forceinline__ int rp_bin_sky(const SkyArgs &P, const double *thr, d
 
 // GEN compiles in the other coordinates' pi and rp (SkyArgs pim, rpm, rmu); without it the kernel is
 // that of coordinates 0, whose registers allow two waves per SIMD.
-template <int K, bool SPLIT, bool SWAP, bool GEN>
+template <int K, bool SPLIT, bool GEN>
 __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide T) {
   const PRec *__restrict__ brec = B.rec;
   const SRec *__restrict__ bsr = B.sr;
   const Win *__restrict__ bwin = B.win;
   const PRec *__restrict__ trec = T.rec;
   const SRec *__restrict__ tsr = T.sr;
-  const Win *__restrict__ twin = T.win;
   constexpr int NV = KindNV<K>::NV;
   constexpr int NE = KindNV<K>::NE;
-  constexpr int NS = SWAP ? 1 + NV : 1 + NV + NE;
-  constexpr int NQ = 1 + NV + NE;
+  constexpr int NS = 1 + NV + NE;
   extern __shared__ __align__(16) unsigned char smem[];
   // LDS as in the box kernel: ncopy lane-selected copies of the rp rows from row0 up, lnum[ncopy][lcell]
   // and lacc[ncopy][NS][lcell], then the thresholds; runs of the rows below go to the global histogram.
@@ -92,8 +93,8 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
   for (int k = tid; k <= P.n_rp; k += blockDim.x) lthr[k] = P.thr[k];
 
   const Item it = P.items[blockIdx.x];
-  const Group bg = P.bgrp[it.group0];  // region of every chunk of the item
-  const int tgt = it.target;
+  const Group bg = P.bgrp[it.group0];  // region and coin of every chunk of the item
+  const int tgt = it.target, tmode = it.tmode, treg = it.treg;
   unsigned long long *const gnum = P.accnum + (size_t)tgt * ncell;
   double *const gacc = P.acc + (size_t)tgt * NS * ncell;
   unsigned long long ncand = 0;
@@ -113,38 +114,15 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
     const PRec me = brec[p];
     const double mez = B.z[p];
     const SRec ms = bsr[p];
-    Win mw{};
-    if (!SWAP) mw = bwin[p];
-    // search rectangle of the wave
-    double zlo, zhi, ralo, rahi, declo, dechi;
-    bool ra_all;
-    if (!SWAP) {
-      zlo = wave_min(valid ? mw.zmin : INFINITY);
-      zhi = wave_max(valid ? mw.zmax : -INFINITY);
-      declo = wave_min(valid ? mw.decmin : INFINITY);
-      dechi = wave_max(valid ? mw.decmax : -INFINITY);
-      const bool nan_ra = valid && !(mw.ramin <= mw.ramax);
-      ra_all = __ballot(nan_ra) != 0;
-      ralo = wave_min(valid ? mw.ramin : INFINITY);
-      rahi = wave_max(valid ? mw.ramax : -INFINITY);
-    } else {
-      zlo = wave_min(valid ? mez : INFINITY) - P.reach_zup;
-      zhi = wave_max(valid ? mez : -INFINITY) + P.reach_zdn;
-      // the widest RA/Dec reach of outer galaxies whose redshift lies in [zlo, zhi]
-      const int b0 = uni((int)fmax(0.0, fmin((double)(P.reach_nb - 1), floor((zlo - P.reach_z0) * P.reach_inv))));
-      const int b1 = uni((int)fmax(0.0, fmin((double)(P.reach_nb - 1), floor((zhi - P.reach_z0) * P.reach_inv))));
-      double rra = 0.0, rdec = 0.0;
-      for (int b = b0; b <= b1; ++b) {
-        const double2 t = P.reach_tab[b];
-        rra = fmax(rra, t.x);
-        rdec = fmax(rdec, t.y);
-      }
-      declo = wave_min(valid ? me.y : INFINITY) - rdec;
-      dechi = wave_max(valid ? me.y : -INFINITY) + rdec;
-      ra_all = __builtin_isinf(rra);
-      ralo = wave_min(valid ? me.x : INFINITY) - rra;
-      rahi = wave_max(valid ? me.x : -INFINITY) + rra;
-    }
+    const Win mw = bwin[p];
+    // search rectangle of the wave: the union of its galaxies' windows
+    const double zlo = wave_min(valid ? mw.zmin : INFINITY);
+    const double zhi = wave_max(valid ? mw.zmax : -INFINITY);
+    const double declo = wave_min(valid ? mw.decmin : INFINITY);
+    const double dechi = wave_max(valid ? mw.decmax : -INFINITY);
+    const bool ra_all = __ballot(valid && !(mw.ramin <= mw.ramax)) != 0;
+    const double ralo = wave_min(valid ? mw.ramin : INFINITY);
+    const double rahi = wave_max(valid ? mw.ramax : -INFINITY);
     const double eps = 1e-9;
     int cx0 = 0, cx1 = P.nra - 1;
     if (!ra_all) {
@@ -167,7 +145,13 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
         const int g0 = P.tcol[col], g1 = P.tcol[col + 1];
         for (int g = g0; g < g1; ++g) {
           const Group tg = P.tgrp[g];
-          if (P.phase_mode == 2 && tg.region == bg.region) continue;
+          // the item's tile filter: every pair it meets belongs to its target
+          const bool same = tg.region == bg.region;
+          const bool take = tmode == kTileAll || (tmode == kTileSame && same) ||
+                            (tmode == kTileOtherC0 && !same && tg.coin == 0) ||
+                            (tmode == kTileSameOrC0 && (same || tg.coin == 0)) ||
+                            (tmode == kTileRegionC1 && tg.region == treg && tg.coin == 1);
+          if (!take) continue;
           if (zsh < tg.zlo || zsl > tg.zhi) continue;
           const int j0 = zsearch<false>(T.z, tg.start, tg.end, zsl);
           const int j1 = zsearch<true>(T.z, j0, tg.end, zsh);
@@ -175,26 +159,21 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
             const PRec t = trec[j];
             const double tzj = T.z[j];
             const SRec ts = tsr[j];
-            // outer (o) and inner (i) roles of the pair
-            const PRec &po = SWAP ? t : me;
-            const PRec &pi = SWAP ? me : t;
-            const SRec &so = SWAP ? ts : ms;
-            const SRec &si = SWAP ? ms : ts;
-            Win w;
-            if (SWAP)
-              w = twin[j];
-            else
-              w = mw;
+            // outer (o: block lane) and inner (i: tile) roles of the pair
+            const PRec &po = me;
+            const PRec &pi = t;
+            const SRec &so = ms;
+            const SRec &si = ts;
+            const Win &w = mw;
             // do_pair_sorted's tests against the outer galaxy's window (PBz = PBy = PBx = 0)
-            const double piz = SWAP ? mez : tzj;
+            const double piz = tzj;
             bool ok = valid && !(piz > w.zmax || piz < w.zmin) && !(pi.y > w.decmax || pi.y < w.decmin) &&
                       !(pi.x < w.ramin || pi.x > w.ramax);
             if (P.auto_mode) {
               // corels1 pairs: the inner galaxy after the outer one in catalog order, which is (z,
               // index) order for z-sorted input (1) and the index alone for unsorted input (2)
-              const double zo = SWAP ? tzj : mez;
-              const int io = SWAP ? T.idx[j] : B.idx[p], ii = SWAP ? B.idx[p] : T.idx[j];
-              ok = ok && (P.auto_mode == 2 ? ii > io : (piz > zo || (piz == zo && ii > io)));
+              const int io = B.idx[p], ii = T.idx[j];
+              ok = ok && (P.auto_mode == 2 ? ii > io : (piz > mez || (piz == mez && ii > io)));
             }
             ncand += ok;
             int cell = -1;
@@ -275,10 +254,8 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
                 const int k = rp_bin_sky(P, lthr, r, band, &unc_r);
                 unc = unc || unc_r || fabs(sum - 0.9999999999) <= P.dsum;
                 if (unc) {
-                  if (!SWAP) {
-                    const unsigned long long slot = atomicAdd(&P.counters[3], 1ull);
-                    if (slot < (unsigned long long)P.hq_cap) P.hq[slot] = make_uint2(B.idx[p], T.idx[j]);
-                  }
+                  const unsigned long long slot = atomicAdd(&P.counters[3], 1ull);
+                  if (slot < (unsigned long long)P.hq_cap) P.hq[slot] = make_uint2(B.idx[p], T.idx[j]);
                 } else if (k >= 0 && q2 >= 0.0 && q2 < (double)P.n_pi) {
                   cell = k * P.n_pi + (int)q2;
                 }
@@ -304,7 +281,7 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
               const double X = (pi.x - po.x) * (SPLIT ? cos_poly((so.dec_rad + si.dec_rad) / 2.0)
                                                       : cos((so.dec_rad + si.dec_rad) / 2.0));
               const double Y = pi.y - po.y;
-              double qv[NQ];
+              double qv[NS];
               pair_values_v<K, SPLIT>(go, gi, P.gws, X, Y, qv);
 #pragma unroll
               for (int s = 0; s < NS; ++s) run[s] += qv[s];
@@ -353,30 +330,27 @@ hipError_t launch_gather_sky(const int32_t *sidx, int64_t n, const SRec *src_sr,
   return hipGetLastError();
 }
 
-template <int K, bool SPLIT, bool SWAP, bool GEN = false>
+template <int K, bool SPLIT, bool GEN = false>
 static hipError_t launch_sky_t(const SkyArgs &P, int threads, size_t lds, const SkySide &B, const SkySide &T,
                                hipStream_t st) {
-  auto fn = sky_kernel<K, SPLIT, SWAP, GEN>;
+  auto fn = sky_kernel<K, SPLIT, GEN>;
   hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   fn<<<P.n_items, threads, lds, st>>>(P, B, T);
   return hipGetLastError();
 }
 
-hipError_t launch_sky(int kind, bool split, bool swap, const SkyArgs &P, int threads, const SkySide &B,
-                      const SkySide &T, hipStream_t st) {
+hipError_t launch_sky(int kind, bool split, const SkyArgs &P, int threads, const SkySide &B, const SkySide &T,
+                      hipStream_t st) {
   if (P.n_items == 0) return hipSuccess;
   const size_t lds = pair_lds_bytes(P.ncell - P.row0 * P.n_pi, P.nslots, P.n_rp, P.ncopy);
   // the other coordinates classify every candidate in FP64 (split_on is 0 there): one kernel for both
   const bool gen = P.pim != 0 || P.rpm != 0 || P.rmu != 0;
-#define CPC_SKY(KK)                                                                                     \
-  if (kind == KK) {                                                                                     \
-    if (gen && swap) return launch_sky_t<KK, false, true, true>(P, threads, lds, B, T, st); \
-    if (gen) return launch_sky_t<KK, false, false, true>(P, threads, lds, B, T, st);        \
-    if (split && swap) return launch_sky_t<KK, true, true>(P, threads, lds, B, T, st); \
-    if (split) return launch_sky_t<KK, true, false>(P, threads, lds, B, T, st);        \
-    if (swap) return launch_sky_t<KK, false, true>(P, threads, lds, B, T, st);         \
-    return launch_sky_t<KK, false, false>(P, threads, lds, B, T, st);                  \
+#define CPC_SKY(KK)                                                            \
+  if (kind == KK) {                                                            \
+    if (gen) return launch_sky_t<KK, false, true>(P, threads, lds, B, T, st);  \
+    if (split) return launch_sky_t<KK, true>(P, threads, lds, B, T, st);       \
+    return launch_sky_t<KK, false>(P, threads, lds, B, T, st);                 \
   }
   CPC_SKY(kDD)
   CPC_SKY(kSD)

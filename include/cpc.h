@@ -171,8 +171,9 @@ typedef struct cpc_stats {
   int64_t n_sparse_windows; /* outer galaxies whose z window held no inner galaxy while later ones exist */
   double ms_total;          /* wall time of cpc_count */
   double ms_kernel;         /* device time of the pair kernels (HIP events on the engine's stream) */
-  double ms_primary;        /* device time of the primary pair kernel (block side = outer catalog) */
-  double ms_touch;          /* device time of the jackknife touch kernel (block side = inner catalog) */
+  double ms_primary;        /* device time of the pair kernel (block side = outer catalog) */
+  double ms_reduce;         /* device time of the jackknife reduction (raw targets into the full sample
+                               and the touch histograms) */
   double ms_build;          /* wall time of the cell-list construction */
   int64_t n_items_primary;  /* work items (workgroups) of the primary kernel */
   int64_t n_host_pairs;     /* sky: guard-band pairs binned on the host with the host libm */
@@ -227,8 +228,8 @@ cpc_status cpc_count_noalloc(cpc_ctx *ctx, const cpc_catalog *outer, const cpc_c
 /* Many passes at once: every job's `out` and `stats` as cpc_count(job) would write them (integer
  * counts identical; FP64 sums equal up to summation order).  The reference runs its passes one after
  * another (do_corr.cpp:63-131, once per catalog pair); here the work items of all box jobs are merged
- * into one launch per pass direction (the primary pass, then the jackknife touch pass; one more
- * launch per further pair kind or precision in the batch), ordered by cost across jobs, so small
+ * into one launch (one more per further pair kind, precision or auto pass in the batch) and then
+ * one jackknife reduction per job, ordered by cost across jobs, so small
  * catalogs fill the device together and only one tail remains.  Sky jobs run one after another
  * within the call.  Jobs are validated before any work starts: on failure no output is written and
  * the message names the job.  The records of all jobs are resident on the device together.
