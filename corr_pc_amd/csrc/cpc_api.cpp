@@ -19,6 +19,7 @@
 #include <string>
 #include <vector>
 #include <functional>
+#include <limits>
 #include <type_traits>
 #include <thread>
 #include <atomic>
@@ -86,6 +87,48 @@ struct DevBuf {
   }
 };
 
+// Page-locked host memory kept by the context between calls: copies to and from it are DMA
+// transfers that the host does not wait for.  Not used by no-alloc calls.
+struct HostBuf {
+  char *p = nullptr;
+  size_t cap = 0;  // bytes
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    release();
+    hipError_t e = hipHostMalloc((void **)&p, std::max<size_t>(bytes, 1), hipHostMallocDefault);
+    if (e == hipSuccess)
+      cap = std::max<size_t>(bytes, 1);
+    else
+      p = nullptr;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// fn(i) for i in [0, n) on up to `nthreads` host threads (the calling one among them)
+template <typename Fn>
+void parallel_for(int64_t n, int nthreads, Fn fn) {
+  nthreads = (int)std::min<int64_t>(std::max(1, nthreads), n);
+  if (nthreads <= 1) {
+    for (int64_t i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  std::atomic<int64_t> next{0};
+  auto work = [&] {
+    for (int64_t i = next++; i < n; i = next++) fn(i);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nthreads; ++t) pool.emplace_back(work);
+  work();
+  for (auto &th : pool) th.join();
+}
+
+int host_threads() { return (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency())); }
+
 struct Status {
   cpc_status code = CPC_OK;
   std::string msg;
@@ -112,7 +155,8 @@ struct Side {
   std::vector<int64_t> colcount;  // galaxies per cell
   // group boundaries found on the device, read back after every side of a call has been launched
   DevBuf<int4> d_bounds;
-  std::vector<int4> h_bounds;
+  std::vector<int4> h_bounds;  // host copy when no page-locked buffer is used
+  const int4 *hb = nullptr;    // the host copy read by finish_side
   bool pending = false;
   uint64_t nkeys = 0;
   int nreg = 1;
@@ -435,6 +479,17 @@ struct cpc_ctx {
   DevBuf<int32_t> vals_a, vals_b, order;
   DevBuf<double> zk_b;
   DevBuf<char> tables;  // every plan's small tables (stage_tables)
+  DevBuf<char> aux;     // per call: the tables of the batched reduction and sparse-window launches
+  HostBuf hb_bounds, hb_stage;  // page-locked group-boundary readback and table staging
+  // batched cell-list build: the sides queued by setup and their concatenated sort arrays, group
+  // boundaries and table (never used by a no-alloc call)
+  std::vector<SideBuild> side_q;
+  std::vector<Side *> side_q_side;
+  int64_t side_q_total = 0;
+  DevBuf<uint64_t> k64_a, k64_b;
+  DevBuf<int32_t> sv_a, sv_b;
+  DevBuf<int4> bounds_all;
+  DevBuf<char> side_tab;
   // per call: work items of every launch, box pass arguments (two per job), and the accumulator
   // arenas (f64 histogram slots; counts and counters)
   DevBuf<Item> items;
@@ -453,7 +508,7 @@ struct cpc_ctx {
     DevBuf<uint32_t> keys_a, keys_b;
     DevBuf<int32_t> vals_a, vals_b, order;
     DevBuf<double> zk_b;
-    DevBuf<char> tables;
+    DevBuf<char> tables, aux;
     DevBuf<Item> items;
     DevBuf<KArgs> jobs;
     DevBuf<double> acc, acc_raw;
@@ -470,6 +525,7 @@ struct cpc_ctx {
       order = {};
       zk_b = {};
       tables = {};
+      aux = {};
       items = {};
       jobs = {};
       acc = {};
@@ -490,6 +546,7 @@ struct cpc_ctx {
     std::swap(order, o.order);
     std::swap(zk_b, o.zk_b);
     std::swap(tables, o.tables);
+    std::swap(aux, o.aux);
     std::swap(items, o.items);
     std::swap(jobs, o.jobs);
     std::swap(acc, o.acc);
@@ -583,6 +640,7 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, b
     S.nkeys = nkeys;
     S.nreg = nreg;
     S.cat = c;
+    S.hb = nullptr;
     ng = n;
   } else {
     std::vector<uint32_t> hkeys(n);
@@ -632,7 +690,7 @@ Status build_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, b
 Status finish_side(cpc_ctx *ctx, Side &S, size_t ncells) {
   if (!S.pending) return {};
   S.pending = false;
-  const std::vector<int4> &hb = S.h_bounds;
+  const int4 *hb = S.hb;
   const cpc_catalog *c = S.cat;
   for (uint64_t key = 0; key < S.nkeys; ++key) {
     const int4 q = hb[key];
@@ -650,6 +708,101 @@ Status finish_side(cpc_ctx *ctx, Side &S, size_t ncells) {
   }
   S.n_incl = hb[S.nkeys].x;
   return side_tables(ctx, S, ncells);
+}
+
+// A side of the call: queued for the batched build (build_queued_sides) when its catalog is in z
+// order and the call may allocate, else built now (build_side).
+Status place_side(cpc_ctx *ctx, const cpc_catalog *c, const Grid &G, int nreg, bool jk_on, const cpc_params &p,
+                  bool has_zfirst, double zfirst_plus_L, bool sky, const Win *win_raw, const uint8_t *aflags,
+                  Side &S) {
+  const int64_t n = c->n;
+  const size_t ncells = (size_t)G.ncx * G.ncy;
+  const uint64_t nkeys = (uint64_t)ncells * (uint64_t)nreg * 2u;
+  const bool batched = !t_arena && n > 0 && c->z_sorted && nkeys <= (1u << 23) &&
+                       ctx->side_q_total + n < (int64_t)INT32_MAX && ctx->side_q.size() < (1u << 20);
+  if (!batched) return build_side(ctx, c, G, nreg, jk_on, p, has_zfirst, zfirst_plus_L, sky, win_raw, aflags, S);
+  Status st;
+  S.groups.clear();
+  S.colbeg.assign(ncells + 1, 0);
+  S.colcount.assign(ncells, 0);
+  S.n_incl = 0;
+  if (check(S.rec.ensure(n + kRecPad), st, "alloc") || check(S.zs.ensure(n + kRecPad), st, "alloc") ||
+      check(S.idx.ensure(n + kRecPad), st, "alloc"))
+    return st;
+  if (sky && (check(S.sr.ensure(n), st, "alloc") || check(S.win.ensure(n), st, "alloc"))) return st;
+  SideBuild b{};
+  b.ka = KeyArgs{c->d_x, c->d_y, c->d_inc, c->d_jk, jk_on ? c->d_jkp : nullptr, n, G.ncx, G.ncy,
+                 G.x0, G.y0, G.hx, G.hy, nreg, p.include_prob_max};
+  // ED passes carry the third axis component in the record's phi slot (ED never reads phi)
+  const double *phi_src = p.kind == CPC_KIND_ED ? c->d_e3 : c->d_phi;
+  b.ga = GatherArgs{c->d_x, c->d_y, c->d_z, c->d_w, c->d_e1, c->d_e2, phi_src, n, G.ncx, G.ncy, G.x0, G.y0, G.hx, G.hy,
+                    p.box_size, window_lo(p), window_hi(p), p.rp_max, zfirst_plus_L, has_zfirst ? 1 : 0, aflags};
+  b.rec = S.rec.p;
+  b.zs = S.zs.p;
+  b.idx = S.idx.p;
+  if (sky) {
+    b.src_sr = c->d_sr;
+    b.src_win = win_raw;
+    b.sr = S.sr.p;
+    b.win = S.win.p;
+  }
+  b.nkeys = (uint32_t)nkeys;
+  ctx->side_q.push_back(b);
+  ctx->side_q_side.push_back(&S);
+  ctx->side_q_total += n;
+  S.pending = true;
+  S.nkeys = nkeys;
+  S.nreg = nreg;
+  S.cat = c;
+  S.hb = nullptr;
+  return st;
+}
+
+// Every queued side in one build: keys, one sort, group boundaries and records (build_sides_many),
+// then the boundaries copied back behind it; each side's `hb` points at its part of the copy.
+Status build_queued_sides(cpc_ctx *ctx) {
+  Status st;
+  std::vector<SideBuild> &q = ctx->side_q;
+  if (q.empty()) return st;
+  const int n = (int)q.size();
+  std::vector<int64_t> blk_el{0}, blk_key{0};
+  int64_t total = 0, nb = 0;
+  for (SideBuild &b : q) {
+    b.off = total;
+    total += b.ka.n;
+    b.boff = nb;
+    nb += (int64_t)b.nkeys + 1;
+    blk_el.push_back(blk_el.back() + (b.ka.n + 255) / 256);
+    blk_key.push_back(blk_key.back() + ((int64_t)b.nkeys + 1 + 255) / 256);
+  }
+  const size_t o_el = q.size() * sizeof(SideBuild), o_key = o_el + blk_el.size() * sizeof(int64_t),
+               bytes = o_key + blk_key.size() * sizeof(int64_t);
+  static_assert(sizeof(SideBuild) % 8 == 0, "table of 8-byte aligned records");
+  std::vector<char> h(bytes);
+  std::memcpy(h.data(), q.data(), o_el);
+  std::memcpy(h.data() + o_el, blk_el.data(), blk_el.size() * sizeof(int64_t));
+  std::memcpy(h.data() + o_key, blk_key.data(), blk_key.size() * sizeof(int64_t));
+  hipStream_t s = ctx->stream;
+  if (check(ctx->k64_a.ensure(total), st, "alloc") || check(ctx->k64_b.ensure(total), st, "alloc") ||
+      check(ctx->sv_a.ensure(total), st, "alloc") || check(ctx->sv_b.ensure(total), st, "alloc") ||
+      check(ctx->bounds_all.ensure(nb), st, "alloc") || check(ctx->side_tab.ensure(bytes), st, "alloc"))
+    return st;
+  check(hipMemcpyAsync(ctx->side_tab.p, h.data(), bytes, hipMemcpyHostToDevice, s), st, "copy");
+  const char *tab = ctx->side_tab.p;
+  check(build_sides_many(reinterpret_cast<const SideBuild *>(tab), n, reinterpret_cast<const int64_t *>(tab + o_el),
+                         blk_el.back(), reinterpret_cast<const int64_t *>(tab + o_key), blk_key.back(), ctx->k64_a.p,
+                         ctx->k64_b.p, ctx->sv_a.p, ctx->sv_b.p, total, ctx->bounds_all.p, ctx->sort_tmp,
+                         ctx->sort_tmp_bytes, s),
+        st, "side build");
+  if (st.code) return st;
+  if (check(ctx->hb_bounds.ensure(nb * sizeof(int4)), st, "host alloc")) return st;
+  const int4 *hb = reinterpret_cast<const int4 *>(ctx->hb_bounds.p);
+  check(hipMemcpyAsync(ctx->hb_bounds.p, ctx->bounds_all.p, nb * sizeof(int4), hipMemcpyDeviceToHost, s), st, "copy");
+  for (int k = 0; k < n; ++k) ctx->side_q_side[k]->hb = hb + q[k].boff;
+  q.clear();
+  ctx->side_q_side.clear();
+  ctx->side_q_total = 0;
+  return st;
 }
 
 // Column starts of a side's groups (host; the tables reach the device with the call's other small
@@ -798,11 +951,19 @@ struct HostHist {
 
 double h_or(const std::vector<double> &v, int64_t i, double d) { return v.empty() ? d : v[i]; }
 
+// One sky pair binned on the host: its cell, the regions it touches (-1: none) and its sums.
+struct HostPair {
+  int c;
+  int reg[2];
+  double wt;
+  double v[CPC_NFIELD], e[CPC_NFIELD];
+};
+
 // The reference's sky pair with the host libm (pi_calc, rp_calc_sky/dA_calc, corel_bin, the kind,
 // corel_jk with jk_check_sky): corels.cpp:21-244, calcs.cpp:60-64,97-105,126-132,
-// sky_calcs.cpp:13-31,91-96, jk.cpp:53-57,138-171,201-223.
-void host_sky_pair(const cpc_params &p, const Derived &dv, const cpc_catalog *g1, int64_t i, const cpc_catalog *g2,
-                   int64_t j, bool jk_on, HostHist &H) {
+// sky_calcs.cpp:13-31,91-96, jk.cpp:53-57,138-171,201-223.  False when the pair lands in no bin.
+bool host_sky_pair(const cpc_params &p, const Derived &dv, const cpc_catalog *g1, int64_t i, const cpc_catalog *g2,
+                   int64_t j, bool jk_on, HostPair &out) {
   const SkyMode m = sky_mode(p);
   const double dec1 = g1->h_y[i] / kR2D, dec2 = g2->h_y[j] / kR2D;
   const double ra1 = g1->h_x[i] / kR2D, ra2 = g2->h_x[j] / kR2D;
@@ -829,13 +990,15 @@ void host_sky_pair(const cpc_params &p, const Derived &dv, const cpc_catalog *g1
     d_P /= da_R;
   }
   const double q2 = std::floor((d_P - p.pi_min) / dv.psize);
-  if (!(q2 >= 0 && q2 < p.n_pi)) return;
+  if (!(q2 >= 0 && q2 < p.n_pi)) return false;
   const double q1 = p.lin_bin ? std::floor((da_R - p.rp_min) / dv.rsize)
                               : std::floor(std::log10(da_R / p.rp_min) / dv.rsize);
-  if (!(q1 >= 0 && q1 < p.n_rp)) return;
+  if (!(q1 >= 0 && q1 < p.n_rp)) return false;
   const int c = (int)q1 * p.n_pi + (int)q2;
   const double wt = h_or(g1->h_w, i, 1.0) * h_or(g2->h_w, j, 1.0);
-  double v[CPC_NFIELD] = {0}, e[CPC_NFIELD] = {0};
+  double *v = out.v, *e = out.e;
+  std::fill(v, v + CPC_NFIELD, 0.0);
+  std::fill(e, e + CPC_NFIELD, 0.0);
   if (p.kind != CPC_KIND_DD) {
     auto rot = [&](double e1, double e2, double &ep, double &ex) {
       ep = std::cos(2.0 * theta) * e1 - std::sin(2.0 * theta) * e2;
@@ -898,23 +1061,34 @@ void host_sky_pair(const cpc_params &p, const Derived &dv, const cpc_catalog *g1
       e[5] = a * a + b * b;
     }
   }
-  H.num[c]++;
-  H.wt[c] += wt;
-  for (int f = 1; f < CPC_NFIELD; ++f) {
-    H.val[(size_t)c * CPC_NFIELD + f] += v[f];
-    H.err[(size_t)c * CPC_NFIELD + f] += e[f];
-  }
-  if (!jk_on) return;
+  out.c = c;
+  out.wt = wt;
+  out.reg[0] = out.reg[1] = -1;
+  if (!jk_on) return true;
   int gjk[2] = {g1->h_jk[i], g2->h_jk[j]};
   double jp[2] = {h_or(g1->h_jkp, i, 0.0), h_or(g2->h_jkp, j, 0.0)};
   if (gjk[0] == gjk[1]) jp[1] = 0;
+  for (int s = 0; s < 2; ++s)
+    if (jp[s] >= 0.5) out.reg[s] = gjk[s];
+  return true;
+}
+
+// Adds a host-binned pair to the full sample and to the touch sums of its regions.
+void host_add(const cpc_params &p, const HostPair &q, HostHist &H) {
+  const int c = q.c;
+  H.num[c]++;
+  H.wt[c] += q.wt;
+  for (int f = 1; f < CPC_NFIELD; ++f) {
+    H.val[(size_t)c * CPC_NFIELD + f] += q.v[f];
+    H.err[(size_t)c * CPC_NFIELD + f] += q.e[f];
+  }
   for (int s = 0; s < 2; ++s) {
-    if (jp[s] < 0.5) continue;
-    const size_t o = (size_t)(1 + gjk[s]) * H.ncell + c;
+    if (q.reg[s] < 0) continue;
+    const size_t o = (size_t)(1 + q.reg[s]) * H.ncell + c;
     H.num[o]++;
-    H.wt[o] += wt;
+    H.wt[o] += q.wt;
     if (p.which_corr != 0)
-      for (int f = 1; f < CPC_NFIELD; ++f) H.val[o * CPC_NFIELD + f] += v[f];
+      for (int f = 1; f < CPC_NFIELD; ++f) H.val[o * CPC_NFIELD + f] += q.v[f];
   }
 }
 
@@ -1284,12 +1458,12 @@ Status setup(cpc_ctx *ctx, Plan &P) {
   // ---- sides (sort keys are (cell, region, coin) packed in 32 bits, all ones meaning excluded)
   if ((double)G.ncx * G.ncy * nreg * 2.0 >= 4294967295.0)
     return {CPC_ERR_UNSUPPORTED, "too many cells x jackknife regions for 32-bit sort keys"};
-  st = build_side(ctx, co, G, nreg, jk_on, p, has_zfirst, zfirst_plus_L, sky, sky ? B.win_raw.p : nullptr,
+  st = place_side(ctx, co, G, nreg, jk_on, p, has_zfirst, zfirst_plus_L, sky, sky ? B.win_raw.p : nullptr,
                   (autop && !sky && !p.unsorted) ? B.aflags.p : nullptr, B.so);
   if (st.code) return st;
   const bool same = (co == ci);
   if (!same) {
-    st = build_side(ctx, ci, G, nreg, jk_on, p, has_zfirst, zfirst_plus_L, sky, nullptr, nullptr, B.si);
+    st = place_side(ctx, ci, G, nreg, jk_on, p, has_zfirst, zfirst_plus_L, sky, nullptr, nullptr, B.si);
     if (st.code) return st;
   }
   P.SO = &B.so;
@@ -1479,7 +1653,10 @@ Status setup_tables(cpc_ctx *ctx, Plan &P) {
     A.hdec = G.hy;
     A.nra = G.ncx;
     A.ndec = G.ncy;
-    A.dsum = 4e-15;
+    // sum = fl(fl(sd sd) + fl(fl(cd cd) c)) with sd, cd from the host: only c = cos(a1 - a2) differs
+    // between device and host, by at most kSkyCosUlp ulps (at most 2^-53 each, c <= 1); the product
+    // and the sum can then round apart by one more ulp each: |dsum| <= (k + 2) 2^-53, doubled
+    A.dsum = 2.0 * (kSkyCosUlp + 2.0) * std::ldexp(1.0, -53);
     const SkyMode sm = sky_mode(p);
     A.pim = sm.pim;
     A.rpm = sm.rpm;
@@ -1568,10 +1745,20 @@ Status stage_tables(cpc_ctx *ctx, std::vector<Plan> &plans) {
     add(B.chunks, P.chunks.data(), P.chunks.size());
   }
   if (check(ctx->tables.ensure(std::max<size_t>(total, 1)), st, "alloc")) return st;
-  std::vector<char> h(total);
-  for (const Copy &c : copies) std::memcpy(h.data() + c.at, c.src, c.bytes);
-  if (total) check(hipMemcpyAsync(ctx->tables.p, h.data(), total, hipMemcpyHostToDevice, ctx->stream), st, "copy");
-  // the copy reads `h` before it returns (pageable memory)
+  // staged in page-locked memory unless the call may not allocate; the gaps between tables are
+  // never read
+  std::vector<char> hv;
+  char *h;
+  if (!t_arena && ctx->hb_stage.ensure(total) == hipSuccess) {
+    h = ctx->hb_stage.p;
+  } else {
+    hv.resize(total);
+    h = hv.data();
+  }
+  parallel_for((int64_t)copies.size(), copies.size() > 256 ? host_threads() : 1,
+               [&](int64_t k) { std::memcpy(h + copies[k].at, copies[k].src, copies[k].bytes); });
+  if (total) check(hipMemcpyAsync(ctx->tables.p, h, total, hipMemcpyHostToDevice, ctx->stream), st, "copy");
+  // the staging memory is reused by the next call: wait for the copy
   check(hipStreamSynchronize(ctx->stream), st, "sync");
   for (auto &b : binders) b(ctx->tables.p);
   return st;
@@ -1630,33 +1817,54 @@ Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
     launches.push_back(Launch{g.first.first, g.first.second != 0, threads, lds, all.size(), v.size()});
     for (const auto &x : v) all.push_back(x.second);
   }
-  if (check(ctx->jobs.ensure(table.size()), st, "alloc") || check(ctx->items.ensure(all.size()), st, "alloc"))
+  // every plan's jackknife reduction and sparse-window count, one launch each: the tables and their
+  // block offsets in one device region
+  std::vector<ReduceArgs> rtab;
+  std::vector<SparseArgs> stab;
+  std::vector<int64_t> roff{0}, soff{0};
+  for (Plan *Pp : plans) {
+    const Plan &P = *Pp;
+    if (P.reduce) {  // raw targets into the full sample and touch histograms
+      rtab.push_back(ReduceArgs{ctx->acc_raw.p + P.off_raw, ctx->accn_raw.p + P.off_rawn, ctx->acc.p + P.off_p,
+                                ctx->accn.p + P.off_pn, P.B->red_beg.p, P.B->red_lst.p, 1 + P.nreg, P.NSP, P.ncell});
+      roff.push_back(roff.back() + reduce_blocks(rtab.back()));
+    }
+    if (!P.autop && P.co->n > 0 && P.ci->n > 0) {  // sparse-window diagnostic (corels2's starts pointer)
+      stab.push_back(SparseArgs{P.co->d_z, P.co->d_inc, P.co->n, P.ci->z_sorted ? P.ci->d_z : P.ci->d_zsorted,
+                                P.ci->n, P.p.box_size, window_lo(P.p), window_hi(P.p), P.p.include_prob_max,
+                                ctx->accn.p + P.off_c + 2});
+      soff.push_back(soff.back() + sparse_blocks(stab.back()));
+    }
+  }
+  const size_t o_roff = rtab.size() * sizeof(ReduceArgs), o_stab = o_roff + roff.size() * sizeof(int64_t),
+               o_soff = o_stab + stab.size() * sizeof(SparseArgs), aux_bytes = o_soff + soff.size() * sizeof(int64_t);
+  std::vector<char> aux(aux_bytes);
+  std::memcpy(aux.data(), rtab.data(), rtab.size() * sizeof(ReduceArgs));
+  std::memcpy(aux.data() + o_roff, roff.data(), roff.size() * sizeof(int64_t));
+  std::memcpy(aux.data() + o_stab, stab.data(), stab.size() * sizeof(SparseArgs));
+  std::memcpy(aux.data() + o_soff, soff.data(), soff.size() * sizeof(int64_t));
+  static_assert(sizeof(ReduceArgs) % 8 == 0 && sizeof(SparseArgs) % 8 == 0, "tables of 8-byte aligned records");
+  if (check(ctx->jobs.ensure(table.size()), st, "alloc") || check(ctx->items.ensure(all.size()), st, "alloc") ||
+      check(ctx->aux.ensure(aux_bytes), st, "alloc"))
     return st;
   check(hipMemcpyAsync(ctx->jobs.p, table.data(), table.size() * sizeof(KArgs), hipMemcpyHostToDevice, s), st,
         "copy");
   if (!all.empty())
     check(hipMemcpyAsync(ctx->items.p, all.data(), all.size() * sizeof(Item), hipMemcpyHostToDevice, s), st, "copy");
+  check(hipMemcpyAsync(ctx->aux.p, aux.data(), aux_bytes, hipMemcpyHostToDevice, s), st, "copy");
   check(hipEventRecord(ctx->ev0, s), st, "event");
   for (const Launch &l : launches)
     check(launch_pair(l.kind % kAutoKey, l.split, l.kind >= kAutoKey, ctx->jobs.p, ctx->items.p + l.first, (int)l.n,
                       l.lds, l.threads, s),
           st, "pair kernel");
   check(hipEventRecord(ctx->ev2, s), st, "event");
-  for (Plan *Pp : plans) {  // raw targets into the full sample and touch histograms
-    const Plan &P = *Pp;
-    if (!P.reduce) continue;
-    ReduceArgs ra{ctx->acc_raw.p + P.off_raw, ctx->accn_raw.p + P.off_rawn, ctx->acc.p + P.off_p,
-                  ctx->accn.p + P.off_pn, P.B->red_beg.p, P.B->red_lst.p, 1 + P.nreg, P.NSP, P.ncell};
-    check(launch_reduce(ra, s), st, "reduce");
-  }
+  check(launch_reduce_many(reinterpret_cast<const ReduceArgs *>(ctx->aux.p),
+                           reinterpret_cast<const int64_t *>(ctx->aux.p + o_roff), (int)rtab.size(), roff.back(), s),
+        st, "reduce");
   check(hipEventRecord(ctx->ev1, s), st, "event");
-  for (Plan *Pp : plans) {  // sparse-window diagnostic (corels2's starts pointer)
-    const Plan &P = *Pp;
-    if (P.autop || P.co->n == 0 || P.ci->n == 0) continue;
-    SparseArgs sa{P.co->d_z, P.co->d_inc, P.co->n, P.ci->z_sorted ? P.ci->d_z : P.ci->d_zsorted, P.ci->n,
-                  P.p.box_size, window_lo(P.p), window_hi(P.p), P.p.include_prob_max};
-    check(launch_sparse(sa, ctx->accn.p + P.off_c + 2, s), st, "sparse");
-  }
+  check(launch_sparse_many(reinterpret_cast<const SparseArgs *>(ctx->aux.p + o_stab),
+                           reinterpret_cast<const int64_t *>(ctx->aux.p + o_soff), (int)stab.size(), soff.back(), s),
+        st, "sparse");
   check(hipStreamSynchronize(s), st, "sync");
   if (st.code) return st;
   float kms = 0, kms_p = 0, kms_r = 0;
@@ -1750,7 +1958,7 @@ Status run_sky(cpc_ctx *ctx, Plan &P) {
 // sample, the touch or leave-one-out jackknife regions, num_rp, and the plan's stats.
 // Returns the plan's sparse-window count.
 int64_t assemble(const Plan &P, const std::vector<double> &acc, const std::vector<unsigned long long> &accn,
-                 double ms_total) {
+                 double ms_total, int threads) {
   const cpc_params &p = P.p;
   const int ncell = P.ncell, nreg = P.nreg, R = P.R, NV = P.NV, NE = P.NE, NSP = P.NSP, kc = P.kind_code;
   const double *accp = acc.data() + P.off_p;
@@ -1763,8 +1971,20 @@ int64_t assemble(const Plan &P, const std::vector<double> &acc, const std::vecto
   HostHist HH;
   const bool host_pairs = !P.hq_pairs.empty();
   if (host_pairs) {
+    // pairs evaluated on host threads a block at a time, then added in queue order
     HH.init(R, ncell);
-    for (const uint2 &q : P.hq_pairs) host_sky_pair(p, P.dv, co, (int64_t)q.x, ci, (int64_t)q.y, P.jk_on, HH);
+    const int64_t nq = (int64_t)P.hq_pairs.size(), blk = 1 << 16;
+    std::vector<HostPair> hp((size_t)std::min(nq, blk));
+    std::vector<char> ok(hp.size());
+    for (int64_t b0 = 0; b0 < nq; b0 += blk) {
+      const int64_t nb = std::min(blk, nq - b0);
+      parallel_for(nb, nb >= 4096 ? threads : 1, [&](int64_t k) {
+        const uint2 q = P.hq_pairs[b0 + k];
+        ok[k] = host_sky_pair(p, P.dv, co, (int64_t)q.x, ci, (int64_t)q.y, P.jk_on, hp[k]);
+      });
+      for (int64_t k = 0; k < nb; ++k)
+        if (ok[k]) host_add(p, hp[k], HH);
+    }
   }
   int64_t sparse = (int64_t)counters[2];
   if (P.sky && p.unsorted == 2 && !P.autop && ci->n > 0) {
@@ -1911,48 +2131,48 @@ cpc_status run_jobs(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *to
   }
   if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, CPC_ERR_DEVICE, "hipSetDevice failed");
   while (ctx->pool.size() < (size_t)n_jobs) ctx->pool.push_back(std::unique_ptr<JobBufs>(new JobBufs()));
+  ctx->side_q.clear();
+  ctx->side_q_side.clear();
+  ctx->side_q_total = 0;
   for (int i = 0; i < n_jobs; ++i) {
     plans[i].B = ctx->pool[i].get();
     Status st = setup(ctx, plans[i]);
     if (st.code) return fail(ctx, tag(i, st));
   }
+  if (n_jobs > 0) {
+    auto t_b = std::chrono::steady_clock::now();
+    Status st = build_queued_sides(ctx);
+    if (st.code) return fail(ctx, st);
+    plans[0].ms_build += ms_since(t_b);
+  }
   {
-    // every side's group boundaries, once all sides are launched: one wait, then small copies
+    // every side's group boundaries, once all sides are launched: the batched build copied its
+    // sides' back already; the sides built one at a time copy theirs, then one wait and the group
+    // tables of all sides on host threads
     Status st;
-    check(hipStreamSynchronize(ctx->stream), st, "sync");
+    std::vector<std::pair<Side *, size_t>> pend;  // side, its grid's cells
     for (Plan &P : plans)
       for (Side *S : {P.SO, P.SI})
-        if (S->pending) {
-          S->h_bounds.resize(S->nkeys + 1);
-          check(hipMemcpyAsync(S->h_bounds.data(), S->d_bounds.p, (S->nkeys + 1) * sizeof(int4), hipMemcpyDeviceToHost,
-                               ctx->stream),
-                st, "copy");
-        }
+        if (S->pending && (pend.empty() || pend.back().first != S)) pend.push_back({S, (size_t)P.G.ncx * P.G.ncy});
+    for (const auto &q : pend) {
+      Side *S = q.first;
+      if (S->hb) continue;
+      S->h_bounds.resize(S->nkeys + 1);
+      S->hb = S->h_bounds.data();
+      check(hipMemcpyAsync(S->h_bounds.data(), S->d_bounds.p, (S->nkeys + 1) * sizeof(int4), hipMemcpyDeviceToHost,
+                           ctx->stream),
+            st, "copy");
+    }
     check(hipStreamSynchronize(ctx->stream), st, "sync");
     if (st.code) return fail(ctx, st);
-    for (Plan &P : plans) {
-      const size_t ncells = (size_t)P.G.ncx * P.G.ncy;
-      st = finish_side(ctx, *P.SO, ncells);
-      if (!st.code && P.SI != P.SO) st = finish_side(ctx, *P.SI, ncells);
-      if (st.code) return fail(ctx, st);
-    }
+    std::vector<Status> sts(pend.size());
+    parallel_for((int64_t)pend.size(), host_threads(),
+                 [&](int64_t k) { sts[k] = finish_side(ctx, *pend[k].first, pend[k].second); });
+    for (const Status &x : sts)
+      if (x.code) return fail(ctx, x);
   }
   // work items of every plan: host work, spread over threads when the batch is large
-  {
-    const int nthreads = (int)std::min<int64_t>(
-        {(int64_t)n_jobs, (int64_t)std::max(1u, std::thread::hardware_concurrency()), (int64_t)16});
-    if (nthreads <= 1) {
-      for (Plan &P : plans) setup_items(P);
-    } else {
-      std::atomic<int> next{0};
-      std::vector<std::thread> pool;
-      for (int t = 0; t < nthreads; ++t)
-        pool.emplace_back([&] {
-          for (int i = next++; i < n_jobs; i = next++) setup_items(plans[i]);
-        });
-      for (auto &th : pool) th.join();
-    }
-  }
+  parallel_for(n_jobs, host_threads(), [&](int64_t i) { setup_items(plans[i]); });
   {
     Status st = stage_tables(ctx, plans);
     if (st.code) return fail(ctx, st);
@@ -2010,8 +2230,17 @@ cpc_status run_jobs(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *to
   check(hipStreamSynchronize(s), st, "sync");
   if (st.code) return fail(ctx, st);
   const double ms_total = ms_since(t_start);
+  std::vector<int64_t> sparse_of(n_jobs, 0);
+  // plans on host threads in a large batch, else each plan's host-binned pairs on them
+  const bool outer_par = n_jobs > 16;
+  parallel_for(n_jobs, outer_par ? host_threads() : 1, [&](int64_t i) {
+    sparse_of[i] = assemble(plans[i], acc, accn, ms_total, outer_par ? 1 : host_threads());
+  });
   int64_t sparse = 0;
-  for (const Plan &P : plans) sparse += assemble(P, acc, accn, ms_total);
+  for (int64_t v : sparse_of) sparse += v;
+  const double ms_all = ms_since(t_start);  // the call's wall time, host binning and assembly included
+  for (Plan &P : plans)
+    if (P.stats) P.stats->ms_total = ms_all;
   if (total) {
     cpc_stats t{};
     for (const Plan &P : plans) {
@@ -2098,6 +2327,15 @@ void cpc_destroy(cpc_ctx *ctx) {
   ctx->order.release();
   ctx->zk_b.release();
   ctx->tables.release();
+  ctx->aux.release();
+  ctx->k64_a.release();
+  ctx->k64_b.release();
+  ctx->sv_a.release();
+  ctx->sv_b.release();
+  ctx->bounds_all.release();
+  ctx->side_tab.release();
+  ctx->hb_bounds.release();
+  ctx->hb_stage.release();
   ctx->items.release();
   ctx->jobs.release();
   ctx->acc.release();
@@ -2422,6 +2660,39 @@ cpc_status cpc_autotune(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_tunin
   return rc;
 }
 
+cpc_status cpc_libm_error(cpc_ctx *ctx, int fn, double lo, double hi, int64_t n, double *max_ulp) {
+  if (!ctx || !max_ulp) return fail(ctx, CPC_ERR_INVALID, "null argument");
+  if ((fn != 0 && fn != 1) || !(lo <= hi) || !std::isfinite(lo) || !std::isfinite(hi) || n < 1 || n > (1 << 26))
+    return fail(ctx, CPC_ERR_INVALID, "cpc_libm_error needs fn 0 or 1, finite lo <= hi and 1 <= n <= 2^26");
+  if (fn == 1 && (lo < -1 || hi > 1)) return fail(ctx, CPC_ERR_INVALID, "acos arguments lie in [-1, 1]");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, CPC_ERR_DEVICE, "hipSetDevice failed");
+  std::vector<double> x(n), y(n);
+  for (int64_t k = 0; k < n; ++k) x[k] = n == 1 ? lo : lo + (hi - lo) * ((double)k / (double)(n - 1));
+  Status st;
+  DevBuf<double> dx, dy;
+  if (check(dx.ensure(n), st, "alloc") || check(dy.ensure(n), st, "alloc")) {
+    dx.release();
+    dy.release();
+    return fail(ctx, st);
+  }
+  check(hipMemcpyAsync(dx.p, x.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream), st, "copy");
+  check(launch_libm(fn, dx.p, n, dy.p, ctx->stream), st, "libm kernel");
+  check(hipMemcpyAsync(y.data(), dy.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), st, "copy");
+  check(hipStreamSynchronize(ctx->stream), st, "sync");
+  dx.release();
+  dy.release();
+  if (st.code) return fail(ctx, st);
+  double worst = 0;
+  for (int64_t k = 0; k < n; ++k) {
+    const double h = fn == 0 ? std::cos(x[k]) : std::acos(x[k]);
+    const double a = std::fabs(h);
+    const double ulp = std::nextafter(a, INFINITY) - a;  // spacing above |h|
+    worst = std::max(worst, std::fabs(y[k] - h) / (ulp > 0 ? ulp : std::numeric_limits<double>::denorm_min()));
+  }
+  *max_ulp = worst;
+  return CPC_OK;
+}
+
 cpc_status cpc_fp32_error(cpc_ctx *ctx, float s_lo, float s_hi, double out[2]) {
   if (!ctx || !out) return fail(ctx, CPC_ERR_INVALID, "null argument");
   if (!(s_lo > 0) || !(s_hi >= s_lo) || !std::isfinite(s_hi))
@@ -2437,6 +2708,7 @@ cpc_status cpc_fp32_error(cpc_ctx *ctx, float s_lo, float s_hi, double out[2]) {
   check(launch_fp32_error(b0, b1 - b0 + 1, d.p, ctx->stream), st, "fp32 error kernel");
   check(hipMemcpyAsync(h, d.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream), st, "copy");
   check(hipStreamSynchronize(ctx->stream), st, "sync");
+  d.release();
   if (st.code) return fail(ctx, st);
   for (int k = 0; k < 2; ++k) std::memcpy(&out[k], &h[k], 8);
   return CPC_OK;

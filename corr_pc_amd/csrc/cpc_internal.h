@@ -205,6 +205,7 @@ struct SparseArgs {
   const double *zi;         // inner z, ascending, every galaxy
   int64_t ni;
   double L, pmin, pmax, inc_max;
+  unsigned long long *count;  // outer galaxies found
 };
 
 // Sums of raw pass targets into output histograms: output o is the sum of targets
@@ -218,15 +219,49 @@ struct ReduceArgs {
   int n_out, ns, ncell;
 };
 
+// One side of a batched cell-list build (every eligible side of a call is sorted in one launch):
+// its galaxies occupy [off, off + ka.n) of the concatenated key and index arrays, sort key
+// (entry << 32 | cell key) with entry its place in the table, and its group boundaries occupy
+// [boff, boff + nkeys] of the bounds array (as k_group_bounds writes them, positions relative to
+// the side).  The records of its ga.n = ka.n galaxies go to rec, zs, idx (and, for a sky side,
+// src_sr / src_win gathered into sr / win; src_sr null otherwise).
+struct SideBuild {
+  KeyArgs ka;
+  GatherArgs ga;
+  PRec *rec;
+  double *zs;
+  int32_t *idx;
+  const SRec *src_sr;
+  const Win *src_win;
+  SRec *sr;
+  Win *win;
+  int64_t off, boff;
+  uint32_t nkeys, pad;
+};
+
 // Launch helpers implemented in cpc_kernels.hip.
 hipError_t launch_keys(const KeyArgs &A, const int32_t *order, uint32_t *keys, int32_t *vals, hipStream_t st);
 hipError_t launch_gather(const GatherArgs &A, const int32_t *sidx, PRec *rec, double *zs, int32_t *idx,
                          hipStream_t st);
-hipError_t launch_sparse(const SparseArgs &A, unsigned long long *count, hipStream_t st);
+// Many reductions (sparse-window counts) in one launch: entry e of the device table `tab` is served
+// by blocks [off[e], off[e+1]) of 256 threads, off[e+1] - off[e] = reduce_blocks (sparse_blocks) of
+// it; `blocks` = off[n].
+int64_t reduce_blocks(const ReduceArgs &A);
+int64_t sparse_blocks(const SparseArgs &A);
+hipError_t launch_reduce_many(const ReduceArgs *tab, const int64_t *off, int n, int64_t blocks, hipStream_t st);
+hipError_t launch_sparse_many(const SparseArgs *tab, const int64_t *off, int n, int64_t blocks, hipStream_t st);
 hipError_t launch_group_bounds(const uint32_t *keys, const int32_t *vals, int64_t n, uint32_t nkeys, int4 *out,
                                hipStream_t st);
 hipError_t launch_fp32_error(uint32_t b0, uint32_t n, unsigned long long *out, hipStream_t st);
 hipError_t launch_reduce(const ReduceArgs &A, hipStream_t st);
+// The batched build of the n sides of the device table `tab`: keys, one stable radix sort of all
+// `total` galaxies (entries above cell keys, so sides stay apart and each keeps its z order),
+// group boundaries and records.  blk_el / blk_key: block offsets of each entry for the per-galaxy
+// and per-key kernels (256 threads a block).  Grows `tmp` as the sort needs.
+hipError_t build_sides_many(const SideBuild *tab, int n, const int64_t *blk_el, int64_t nblk_el,
+                            const int64_t *blk_key, int64_t nblk_key, uint64_t *keys_a, uint64_t *keys_b,
+                            int32_t *vals_a, int32_t *vals_b, int64_t total, int4 *bounds, void *&tmp,
+                            size_t &tmp_bytes, hipStream_t st);
 hipError_t sort_u32(void *&tmp, size_t &tmp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
                     int32_t *vout, int64_t n, hipStream_t st, bool may_alloc);
 hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *kout, const int32_t *vin,
@@ -249,6 +284,11 @@ struct SkySide {
 };
 hipError_t launch_sky(int kind, bool split, const SkyArgs &P, int threads, const SkySide &B,
                       const SkySide &T, hipStream_t st);
+// y = cos(x) (fn 0) or acos(x) (fn 1) with the device functions of the sky kernels
+hipError_t launch_libm(int fn, const double *x, int64_t n, double *y, hipStream_t st);
+// ulps by which the device cos of the sky kernels may differ from the host's (cpc_libm_error checks
+// it on the device); the great-circle sum bound dsum follows from it
+constexpr double kSkyCosUlp = 2.0;
 hipError_t launch_gather_sky(const int32_t *sidx, int64_t n, const SRec *src_sr, const Win *src_win, SRec *sr,
                              Win *win, hipStream_t st);
 

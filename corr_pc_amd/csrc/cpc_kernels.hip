@@ -343,11 +343,7 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
 
 // key = ((column * nreg + region) * 2 + coin), or 0xffffffff for galaxies the pass skips
 // (include_prob > include_prob_max, corr.cpp:352,414)
-__global__ void k_keys(KeyArgs A, const int32_t *__restrict__ order, uint32_t *__restrict__ keys,
-                       int32_t *__restrict__ vals) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= A.n) return;
-  const int32_t i = order ? order[p] : (int32_t)p;
+__device__ __forceinline__ uint32_t cell_key(const KeyArgs &A, int32_t i) {
   uint32_t key = 0xffffffffu;
   const double inc = A.inc ? A.inc[i] : 0.0;
   if (!(inc > A.inc_max)) {
@@ -358,16 +354,23 @@ __global__ void k_keys(KeyArgs A, const int32_t *__restrict__ order, uint32_t *_
     const int coin = A.jkp ? (A.jkp[i] >= 0.5) : 0;
     key = ((uint32_t)(cy * A.ncx + cx) * (uint32_t)A.nreg + (uint32_t)reg) * 2u + (uint32_t)coin;
   }
-  keys[p] = key;
+  return key;
+}
+
+__global__ void k_keys(KeyArgs A, const int32_t *__restrict__ order, uint32_t *__restrict__ keys,
+                       int32_t *__restrict__ vals) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= A.n) return;
+  const int32_t i = order ? order[p] : (int32_t)p;
+  keys[p] = cell_key(A, i);
   vals[p] = i;
 }
 
 
-__global__ void k_gather(GatherArgs A, const int32_t *__restrict__ sidx, PRec *__restrict__ rec,
-                         double *__restrict__ zs, int32_t *__restrict__ idx) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= A.n) return;
-  const int32_t i = sidx[p];
+// Record p of a side from catalog galaxy i (position, weight, shape, image flags, column-local
+// FP32 coordinates)
+__device__ __forceinline__ void gather_rec(const GatherArgs &A, int32_t i, int64_t p, PRec *__restrict__ rec,
+                                           double *__restrict__ zs, int32_t *__restrict__ idx) {
   PRec r;
   r.x = A.x[i];
   r.y = A.y[i];
@@ -401,19 +404,26 @@ __global__ void k_gather(GatherArgs A, const int32_t *__restrict__ sidx, PRec *_
   idx[p] = i;
 }
 
+__global__ void k_gather(GatherArgs A, const int32_t *__restrict__ sidx, PRec *__restrict__ rec,
+                         double *__restrict__ zs, int32_t *__restrict__ idx) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= A.n) return;
+  gather_rec(A, sidx[p], p, rec, zs, idx);
+}
+
 // Group boundaries of a side from its sorted keys: for every possible key k < nkeys the records of
 // group k are [out[k].x, out[k].y), with the catalog indices of its first and last galaxy in .z and
 // .w (-1 when the group is empty); out[nkeys].x is the number of included records (excluded
 // galaxies carry key 0xffffffff >= nkeys).
-__global__ void k_group_bounds(const uint32_t *__restrict__ keys, const int32_t *__restrict__ vals, int64_t n,
-                               uint32_t nkeys, int4 *__restrict__ out) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k > (int64_t)nkeys) return;
+// Boundaries of group k over n sorted keys whose low 32 bits are the cell keys (all equal above)
+template <typename KeyT>
+__device__ __forceinline__ int4 group_bounds_at(const KeyT *__restrict__ keys, const int32_t *__restrict__ vals,
+                                                int64_t n, uint32_t nkeys, int64_t k) {
   auto first_at_least = [&](uint64_t v) {
     int64_t lo = 0, hi = n;
     while (lo < hi) {
       const int64_t mid = lo + ((hi - lo) >> 1);
-      if ((uint64_t)keys[mid] < v)
+      if ((uint64_t)(uint32_t)keys[mid] < v)
         lo = mid + 1;
       else
         hi = mid;
@@ -422,14 +432,20 @@ __global__ void k_group_bounds(const uint32_t *__restrict__ keys, const int32_t 
   };
   const int64_t b = first_at_least((uint64_t)k);
   const int64_t e = k < (int64_t)nkeys ? first_at_least((uint64_t)k + 1) : b;
-  out[k] = make_int4((int)b, (int)e, b < e ? vals[b] : -1, b < e ? vals[e - 1] : -1);
+  return make_int4((int)b, (int)e, b < e ? vals[b] : -1, b < e ? vals[e - 1] : -1);
+}
+
+__global__ void k_group_bounds(const uint32_t *__restrict__ keys, const int32_t *__restrict__ vals, int64_t n,
+                               uint32_t nkeys, int4 *__restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > (int64_t)nkeys) return;
+  out[k] = group_bounds_at(keys, vals, n, nkeys, k);
 }
 
 // Outer galaxies whose PBz = 0 or PBz = -1 window holds no inner galaxy while later ones exist: the
 // reference's forward-only start pointer runs off in that case (corr.cpp:361-369, 395-404).
 
-__global__ void k_sparse(SparseArgs A, unsigned long long *count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void sparse_at(const SparseArgs &A, int64_t i) {
   unsigned c = 0;
   if (i < A.no && !((A.inco ? A.inco[i] : 0.0) > A.inc_max)) {
     const double z_max = A.zo[i] + A.pmax, z_min = A.zo[i] + A.pmin;
@@ -447,7 +463,63 @@ __global__ void k_sparse(SparseArgs A, unsigned long long *count) {
       if (lo < A.ni && A.zi[lo] + sh > z_max) ++c;
     }
   }
-  if (c) atomicAdd(count, (unsigned long long)c);
+  if (c) atomicAdd(A.count, (unsigned long long)c);
+}
+
+// Entry e of a table of launches, served by blocks [off[e], off[e+1]) (the block's own entry is
+// found by binary search; off is ascending with off[0] = 0)
+__device__ __forceinline__ int table_entry(const int64_t *__restrict__ off, int n, int64_t b) {
+  int lo = 0, hi = n;  // off[lo] <= b < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_sparse_many(const SparseArgs *__restrict__ tab, const int64_t *__restrict__ off, int n) {
+  const int e = table_entry(off, n, blockIdx.x);
+  sparse_at(tab[e], (int64_t)(blockIdx.x - off[e]) * blockDim.x + threadIdx.x);
+}
+
+// Batched cell-list build (SideBuild): every side's sort keys, entry number above the cell key
+__global__ void k_keys_many(const SideBuild *__restrict__ tab, const int64_t *__restrict__ blk, int n,
+                            uint64_t *__restrict__ keys, int32_t *__restrict__ vals) {
+  const int e = table_entry(blk, n, blockIdx.x);
+  const SideBuild &S = tab[e];
+  const int64_t p = (int64_t)(blockIdx.x - blk[e]) * blockDim.x + threadIdx.x;
+  if (p >= S.ka.n) return;
+  keys[S.off + p] = ((uint64_t)e << 32) | cell_key(S.ka, (int32_t)p);
+  vals[S.off + p] = (int32_t)p;
+}
+
+// ... each side's group boundaries within its run of the sorted arrays
+__global__ void k_bounds_many(const SideBuild *__restrict__ tab, const int64_t *__restrict__ blk, int n,
+                              const uint64_t *__restrict__ keys, const int32_t *__restrict__ vals,
+                              int4 *__restrict__ bounds) {
+  const int e = table_entry(blk, n, blockIdx.x);
+  const SideBuild &S = tab[e];
+  const int64_t k = (int64_t)(blockIdx.x - blk[e]) * blockDim.x + threadIdx.x;
+  if (k > (int64_t)S.nkeys) return;
+  bounds[S.boff + k] = group_bounds_at(keys + S.off, vals + S.off, S.ka.n, S.nkeys, k);
+}
+
+// ... and each side's records in sorted order (the sky's SRec and windows too)
+__global__ void k_gather_many(const SideBuild *__restrict__ tab, const int64_t *__restrict__ blk, int n,
+                              const int32_t *__restrict__ vals) {
+  const int e = table_entry(blk, n, blockIdx.x);
+  const SideBuild &S = tab[e];
+  const int64_t p = (int64_t)(blockIdx.x - blk[e]) * blockDim.x + threadIdx.x;
+  if (p >= S.ga.n) return;
+  const int32_t i = vals[S.off + p];
+  gather_rec(S.ga, i, p, S.rec, S.zs, S.idx);
+  if (S.src_sr) {
+    S.sr[p] = S.src_sr[i];
+    if (S.src_win) S.win[p] = S.src_win[i];
+  }
 }
 
 // The split kernels' FP32 approximations against FP64 over the FP32 values b0 .. b0+n-1 (as bit
@@ -474,8 +546,7 @@ __global__ void k_fp32_error(uint32_t b0, uint32_t n, unsigned long long *out) {
 
 // One thread per (output, slot, cell); slot -1 is the count.  Summation follows the list order, so
 // the result does not depend on the launch.
-__global__ void k_reduce(ReduceArgs A) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void reduce_at(const ReduceArgs &A, int64_t i) {
   const int64_t per_out = (int64_t)(A.ns + 1) * A.ncell;
   if (i >= (int64_t)A.n_out * per_out) return;
   const int o = (int)(i / per_out);
@@ -491,6 +562,13 @@ __global__ void k_reduce(ReduceArgs A) {
     for (int k = b; k < e; ++k) v += A.acc[((size_t)A.lst[k] * A.ns + s) * A.ncell + c];
     A.out[((size_t)o * A.ns + s) * A.ncell + c] = v;
   }
+}
+
+__global__ void k_reduce(ReduceArgs A) { reduce_at(A, (int64_t)blockIdx.x * blockDim.x + threadIdx.x); }
+
+__global__ void k_reduce_many(const ReduceArgs *__restrict__ tab, const int64_t *__restrict__ off, int n) {
+  const int e = table_entry(off, n, blockIdx.x);
+  reduce_at(tab[e], (int64_t)(blockIdx.x - off[e]) * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -532,10 +610,19 @@ hipError_t launch_group_bounds(const uint32_t *keys, const int32_t *vals, int64_
   return hipGetLastError();
 }
 
-hipError_t launch_sparse(const SparseArgs &A, unsigned long long *count, hipStream_t st) {
-  if (A.no == 0 || A.ni == 0) return hipSuccess;
-  const int bs = 256;
-  k_sparse<<<(unsigned)((A.no + bs - 1) / bs), bs, 0, st>>>(A, count);
+int64_t reduce_blocks(const ReduceArgs &A) { return ((int64_t)A.n_out * (A.ns + 1) * A.ncell + 255) / 256; }
+
+int64_t sparse_blocks(const SparseArgs &A) { return (A.no == 0 || A.ni == 0) ? 0 : (A.no + 255) / 256; }
+
+hipError_t launch_reduce_many(const ReduceArgs *tab, const int64_t *off, int n, int64_t blocks, hipStream_t st) {
+  if (n == 0 || blocks == 0) return hipSuccess;
+  k_reduce_many<<<(unsigned)blocks, 256, 0, st>>>(tab, off, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_sparse_many(const SparseArgs *tab, const int64_t *off, int n, int64_t blocks, hipStream_t st) {
+  if (n == 0 || blocks == 0) return hipSuccess;
+  k_sparse_many<<<(unsigned)blocks, 256, 0, st>>>(tab, off, n);
   return hipGetLastError();
 }
 
@@ -567,6 +654,26 @@ hipError_t sort_u32(void *&tmp, size_t &tmp_bytes, const uint32_t *kin, uint32_t
 hipError_t sort_f64(void *&tmp, size_t &tmp_bytes, const double *kin, double *kout, const int32_t *vin,
                     int32_t *vout, int64_t n, hipStream_t st, bool may_alloc) {
   return sort_pairs_impl<double>(tmp, tmp_bytes, kin, kout, vin, vout, n, 64, st, may_alloc);
+}
+
+hipError_t build_sides_many(const SideBuild *tab, int n, const int64_t *blk_el, int64_t nblk_el,
+                            const int64_t *blk_key, int64_t nblk_key, uint64_t *keys_a, uint64_t *keys_b,
+                            int32_t *vals_a, int32_t *vals_b, int64_t total, int4 *bounds, void *&tmp,
+                            size_t &tmp_bytes, hipStream_t st) {
+  if (n == 0 || total == 0) return hipSuccess;
+  k_keys_many<<<(unsigned)nblk_el, 256, 0, st>>>(tab, blk_el, n, keys_a, vals_a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // entry numbers take the bits above the 32-bit cell key
+  int end_bit = 32;
+  while (end_bit < 64 && ((uint64_t)(n - 1) >> (end_bit - 32)) != 0) ++end_bit;
+  e = sort_pairs_impl<uint64_t>(tmp, tmp_bytes, keys_a, keys_b, vals_a, vals_b, total, end_bit, st, true);
+  if (e != hipSuccess) return e;
+  k_bounds_many<<<(unsigned)nblk_key, 256, 0, st>>>(tab, blk_key, n, keys_b, vals_b, bounds);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_gather_many<<<(unsigned)nblk_el, 256, 0, st>>>(tab, blk_el, n, vals_b);
+  return hipGetLastError();
 }
 
 // Temporary storage both sorts need for n pairs (a query: hipcub allocates nothing without storage).

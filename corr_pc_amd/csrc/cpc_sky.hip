@@ -218,12 +218,16 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
                 // dA_calc: sin d1 sin d2 + cos d1 cos d2 cos(a1 - a2), clamp, acos; then rp_calc_sky,
                 // rp_calc_sky_projected (outer DA) or rp_calc_sky_angular (degrees)
                 const double sum = so.sd * si.sd + so.cd * si.cd * cos((so.ra_rad - si.ra_rad));
-                const double dA = (sum > 0.9999999999) ? acos(1.0) : acos(sum);
+                const bool clamped = sum > 0.9999999999;
+                const double dA = clamped ? acos(1.0) : acos(sum);
                 const int rpm = GEN ? P.rpm : 0;
                 const double dsc = rpm == 0 ? 0.5 * (so.da + si.da) : (rpm == 1 ? so.da : kSkyR2D);
                 const double rp = rpm == 0 ? dA * (so.da + si.da) / 2. : dA * dsc;
+                // the host's dA moves by dsum / sqrt(1 - sum^2) from the sum's difference, and by the
+                // acos's own rounding; a clamped sum (more than dsum above the clamp, checked below)
+                // is clamped on the host too and gives dA = acos(1) there as well, exactly
                 const double one_m = fmax(1.0 - sum * sum, 1e-300);
-                const double dda = P.dsum / sqrt(one_m) + 1e-15 * dA;
+                const double dda = clamped ? 0.0 : P.dsum / sqrt(one_m) + 1e-15 * dA;
                 double band = 1.5 * dsc * dda + 4e-16 * rp;
                 double r = rp;
                 bool unc = false;
@@ -321,6 +325,19 @@ __global__ void k_gather_sky(const int32_t *__restrict__ sidx, int64_t n, const 
   const int32_t i = sidx[p];
   sr[p] = src_sr[i];
   if (src_win) win[p] = src_win[i];
+}
+
+// The device cos (fn 0) or acos (fn 1) of the sky kernels, for checks against the host C library
+__global__ void k_libm(int fn, const double *__restrict__ x, int64_t n, double *__restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  y[i] = fn == 0 ? cos(x[i]) : acos(x[i]);
+}
+
+hipError_t launch_libm(int fn, const double *x, int64_t n, double *y, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  k_libm<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(fn, x, n, y);
+  return hipGetLastError();
 }
 
 hipError_t launch_gather_sky(const int32_t *sidx, int64_t n, const SRec *src_sr, const Win *src_win, SRec *sr,

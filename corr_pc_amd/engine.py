@@ -272,6 +272,8 @@ def load_library(path=LIB_PATH):
     lib.cpc_count_noalloc.restype = c_int
     lib.cpc_fp32_error.argtypes = [c_void_p, ctypes.c_float, ctypes.c_float, _dp]
     lib.cpc_fp32_error.restype = c_int
+    lib.cpc_libm_error.argtypes = [c_void_p, c_int, c_double, c_double, ctypes.c_int64, _dp]
+    lib.cpc_libm_error.restype = c_int
     _lib = lib
     return lib
 
@@ -279,7 +281,7 @@ def load_library(path=LIB_PATH):
 EXPORTED_SYMBOLS = ["cpc_init", "cpc_destroy", "cpc_last_error", "cpc_version", "cpc_catalog_upload",
                     "cpc_catalog_free", "cpc_count", "cpc_count_batch", "cpc_rp_thresholds", "cpc_rand_stream",
                     "cpc_set_tuning", "cpc_autotune", "cpc_bin_edges", "cpc_box_rr", "cpc_weight_sum",
-                    "cpc_write_cross", "cpc_do_bining", "cpc_fp32_error", "cpc_count_noalloc"]
+                    "cpc_write_cross", "cpc_do_bining", "cpc_fp32_error", "cpc_count_noalloc", "cpc_libm_error"]
 
 
 REFERENCE_SEED = 124693517  # seed(), jk.cpp:40-43
@@ -477,6 +479,15 @@ class Engine:
         self._check(self.lib.cpc_fp32_error(self.ctx, float(s_lo), float(s_hi), out.ctypes.data_as(_dp)),
                     "cpc_fp32_error")
         return float(out[0]), float(out[1])
+
+    def libm_error(self, fn, lo, hi, n):
+        """Max ulps between the sky kernels' device cos (fn "cos") or acos (fn "acos") and the host C
+        library's on n points spread evenly over [lo, hi] (cpc_libm_error)."""
+        out = np.zeros(1, np.float64)
+        code = {"cos": 0, "acos": 1}[fn]
+        self._check(self.lib.cpc_libm_error(self.ctx, code, float(lo), float(hi), int(n), out.ctypes.data_as(_dp)),
+                    "cpc_libm_error")
+        return float(out[0])
 
     def close(self):
         if self.ctx is not None:

@@ -227,11 +227,13 @@ cpc_status cpc_count_noalloc(cpc_ctx *ctx, const cpc_catalog *outer, const cpc_c
 
 /* Many passes at once: every job's `out` and `stats` as cpc_count(job) would write them (integer
  * counts identical; FP64 sums equal up to summation order).  The reference runs its passes one after
- * another (do_corr.cpp:63-131, once per catalog pair); here the work items of all box jobs are merged
- * into one launch (one more per further pair kind, precision or auto pass in the batch) and then
- * one jackknife reduction per job, ordered by cost across jobs, so small
- * catalogs fill the device together and only one tail remains.  Sky jobs run one after another
- * within the call.  Jobs are validated before any work starts: on failure no output is written and
+ * another (do_corr.cpp:63-131, once per catalog pair); here the cell lists of all jobs are built
+ * by one sort, and the work items of all box jobs are merged into one launch (one more per further
+ * pair kind, precision or auto pass in the batch), ordered by cost across jobs, so small catalogs
+ * fill the device together and only one tail remains; one more launch reduces every job's
+ * jackknife targets.  Sky jobs run one after another
+ * within the call.  Jobs write their outputs concurrently, so no two jobs may share an `out` or
+ * `stats`.  Jobs are validated before any work starts: on failure no output is written and
  * the message names the job.  The records of all jobs are resident on the device together.
  * `total`, if non-NULL, receives summed counters; its times are those of the whole call.  In each
  * job's stats the kernel times are those of the launches that served it (shared by the batch). */
@@ -351,6 +353,14 @@ cpc_status cpc_autotune(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_tunin
  * (v_rsq_f32), against FP64 references.  The split guard bands take both as at most 2^-21 (about
  * 4.8e-7); a larger value means the guard bands do not cover this device. */
 cpc_status cpc_fp32_error(cpc_ctx *ctx, float s_lo, float s_hi, double out[2]);
+
+/* No counterpart in the reference: a check of the device functions the sky kernels bin with.  The
+ * device cos (fn 0) or acos (fn 1) of the sky kernels against the host C library's on n points
+ * spread evenly over [lo, hi] (n <= 2^26): *max_ulp = max |device - host| in units of the last
+ * place of the host value.  The sky guard band takes the cosine's as at most 2 ulps (its bound on
+ * the device/host difference of the great-circle sum follows from that) and the arc cosine's as
+ * within 1e-15 relative. */
+cpc_status cpc_libm_error(cpc_ctx *ctx, int fn, double lo, double hi, int64_t n, double *max_ulp);
 
 #ifdef __cplusplus
 }

@@ -62,3 +62,30 @@ def test_fp32_error_rejects_bad_range(engine):
     with pytest.raises(EngineError):
         engine.fp32_error(2.0, 1.0)
     assert np.isfinite(engine.fp32_error(1.0, 1.0)[0])
+
+
+# The sky kernels' device cos and acos against the host C library.  The sky guard band bounds the
+# device/host difference of the great-circle sum sin d1 sin d2 + cos d1 cos d2 cos(a1 - a2) from a
+# cosine within 2 ulps of the host's (sky_calcs.cpp:24-31), and takes the arc cosine within 1e-15
+# relative (about 4.5 ulps).  Ranges: RA differences of one field up to the full circle, and the
+# sums of separations up to a few degrees, where acos is steepest.
+@pytest.mark.parametrize("lo,hi", [(0.0, 0.05), (0.0, 1.6), (-7.0, 7.0)])
+def test_sky_cos_within_bound(engine, lo, hi):
+    ulp = engine.libm_error("cos", lo, hi, 1 << 22)
+    print(f"[libm] cos on [{lo}, {hi}]: {ulp:.3f} ulp", flush=True)
+    assert ulp <= 2.0
+
+
+@pytest.mark.parametrize("lo,hi", [(0.99, 1.0), (-1.0, 1.0)])
+def test_sky_acos_within_bound(engine, lo, hi):
+    ulp = engine.libm_error("acos", lo, hi, 1 << 22)
+    print(f"[libm] acos on [{lo}, {hi}]: {ulp:.3f} ulp", flush=True)
+    assert ulp <= 4.0
+
+
+def test_libm_error_rejects_bad_arguments(engine):
+    from corr_pc_amd.engine import EngineError
+    with pytest.raises(EngineError):
+        engine.libm_error("acos", 0.5, 1.5, 16)
+    with pytest.raises(EngineError):
+        engine.libm_error("cos", 1.0, 0.0, 16)
