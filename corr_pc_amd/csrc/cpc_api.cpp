@@ -1391,26 +1391,47 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     std::vector<double2> &reach = P.reach;
     reach.assign(nb, make_double2(0.0, 0.0));
     dec_half.reserve(co->n);
-    for (int64_t i = 0; i < co->n; ++i) {
-      const double inc = co->h_inc.empty() ? 0.0 : co->h_inc[i];
-      Win &w = wins[i];
-      if (inc > p.include_prob_max) {
-        w = Win{1, 0, 0, 0, 0, 0, 0, 0};  // never used: the galaxy is not in the pass
-        continue;
+    // on host threads, a fixed range of galaxies a chunk, each with its own reach table and
+    // half-widths, merged in chunk order
+    const int64_t nchunk = std::min<int64_t>(64, std::max<int64_t>(1, co->n / 4096));
+    std::vector<std::vector<double2>> creach(nchunk, std::vector<double2>(nb, make_double2(0.0, 0.0)));
+    std::vector<std::vector<double>> chalf(nchunk);
+    std::vector<int64_t> cbad(nchunk, -1);  // first galaxy whose window fails
+    parallel_for(nchunk, host_threads(), [&](int64_t k) {
+      const int64_t i0 = co->n * k / nchunk, i1 = co->n * (k + 1) / nchunk;
+      std::vector<double2> &rr = creach[k];
+      for (int64_t i = i0; i < i1; ++i) {
+        const double inc = co->h_inc.empty() ? 0.0 : co->h_inc[i];
+        Win &w = wins[i];
+        if (inc > p.include_prob_max) {
+          w = Win{1, 0, 0, 0, 0, 0, 0, 0};  // never used: the galaxy is not in the pass
+          continue;
+        }
+        if (!sky_window(T, p, co->h_x[i], co->h_y[i], co->h_z[i], w)) {
+          cbad[k] = i;
+          return;
+        }
+        const int b = (int)std::fmax(0.0, std::fmin((double)(nb - 1), std::floor((co->h_z[i] - rz0) * rinv)));
+        double2 &rb = rr[b];
+        if (!(w.ramin <= w.ramax) || w.ramin <= 0 || w.ramax >= 360)
+          rb.x = INFINITY;
+        else
+          rb.x = std::max(rb.x, std::max(co->h_x[i] - w.ramin, w.ramax - co->h_x[i]) * (1 + 1e-12) + 1e-9);
+        const double dh = std::max(co->h_y[i] - w.decmin, w.decmax - co->h_y[i]);
+        rb.y = std::max(rb.y, dh * (1 + 1e-12) + 1e-9);
+        chalf[k].push_back(dh);
       }
-      if (!sky_window(T, p, co->h_x[i], co->h_y[i], co->h_z[i], w))
-        return {CPC_ERR_INVALID, "galaxy " + std::to_string(i) +
+    });
+    for (int64_t k = 0; k < nchunk; ++k) {
+      if (cbad[k] >= 0)
+        return {CPC_ERR_INVALID, "galaxy " + std::to_string(cbad[k]) +
                                      ": its redshift window leaves the distance table or exceeds z_max "
                                      "(the reference exits here, sky_calcs.cpp:109-113)"};
-      const int b = (int)std::fmax(0.0, std::fmin((double)(nb - 1), std::floor((co->h_z[i] - rz0) * rinv)));
-      double2 &rb = reach[b];
-      if (!(w.ramin <= w.ramax) || w.ramin <= 0 || w.ramax >= 360)
-        rb.x = INFINITY;
-      else
-        rb.x = std::max(rb.x, std::max(co->h_x[i] - w.ramin, w.ramax - co->h_x[i]) * (1 + 1e-12) + 1e-9);
-      const double dh = std::max(co->h_y[i] - w.decmin, w.decmax - co->h_y[i]);
-      rb.y = std::max(rb.y, dh * (1 + 1e-12) + 1e-9);
-      dec_half.push_back(dh);
+      for (int b = 0; b < nb; ++b) {
+        reach[b].x = std::max(reach[b].x, creach[k][b].x);
+        reach[b].y = std::max(reach[b].y, creach[k][b].y);
+      }
+      dec_half.insert(dec_half.end(), chalf[k].begin(), chalf[k].end());
     }
     // RA/Dec grid over both catalogs, cells about half the median window
     double ralo = 360, rahi = 0, declo = 90, dechi = -90;
