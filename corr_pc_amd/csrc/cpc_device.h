@@ -64,6 +64,7 @@ struct Gal {
 // 50-digit arithmetic; within 6e-17 of each value at k = 0..4).  |y| > |x| uses
 // pi/2 - atan(x/y); the sign is that of the IEEE quotient y/x; x = y = 0 gives NaN like atan(0/0).
 __device__ __forceinline__ double atan_ratio(double y, double x) {
+#pragma clang fp contract(fast)
   const double ax = fabs(x), ay = fabs(y);
   const bool sw = ay > ax;
   const double a = sw ? ax : ay, b = sw ? ay : ax;
@@ -103,6 +104,8 @@ __device__ __forceinline__ double ephi(double phi, double theta) {
 template <int K, bool SPLIT>
 __device__ __forceinline__ void pair_values_v(const Gal &g1, const Gal &g2, int gws, double dxr, double dyr,
                                               double *q) {
+  // weighted sums only (tolerance 1e-10 relative): products and sums may fuse into FMAs
+#pragma clang fp contract(fast)
   const double wt = g1.w * g2.w;
   q[0] = wt;
   if constexpr (K == kDD) {

@@ -161,11 +161,18 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
     // A lane keeps the sums of its current histogram cell in registers and adds them to LDS only
     // when its cell changes: lanes of a wave hold neighbouring galaxies, so their pairs crowd into
     // few cells, and flushing per run instead of per pair divides the same-address LDS atomics.
+    // (the sums share a struct with the count: an array of doubles alone would be promoted to one
+    // vector register tuple, which every join of the loop's control flow copies whole)
     int run_cell = -1;
-    unsigned run_n = 0;
-    double run[NS];
+    struct {
+      double v[NS];
+      unsigned n;
+    } R;
+    R.n = 0;
 #pragma unroll
-    for (int s = 0; s < NS; ++s) run[s] = 0.0;
+    for (int s = 0; s < NS; ++s) R.v[s] = 0.0;
+    double *const run = R.v;
+    unsigned &run_n = R.n;
 
     for (int q = 0; q < P.n_nbr; ++q) {
       const int2 o = P.nbr[q];
@@ -281,14 +288,6 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
                              rl_d(cq.x, cq.y, 4 * t + 1), rl_d(cq.z, cq.w, 4 * t + 1),
                              rl_d(cq.x, cq.y, 4 * t + 2), rl_d(cq.z, cq.w, 4 * t + 2)};
                 if (cell >= 0) {
-                  if (cell != run_cell) {
-                    if (run_cell >= 0) flush_cm<NS, CUT>(mynum, myacc, lbase, gnum, gacc, ncell, run_cell, run_n, run);
-                    run_cell = cell;
-                    run_n = 0;
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) run[s] = 0.0;
-                  }
-                  ++run_n;
                   const Gal gme{me.x, me.y, me.w, me.e1, me.e2, me.phi};
                   double qv[NS];
                   if constexpr (K == kED) {
@@ -301,8 +300,18 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
                   } else {
                     pair_values<K, SPLIT>(gme, gt, H.gws, qv);
                   }
+                  if (cell != run_cell) {
+                    if (run_cell >= 0) flush_cm<NS, CUT>(mynum, myacc, lbase, gnum, gacc, ncell, run_cell, run_n, run);
+                    // the new run starts from this pair's values
 #pragma unroll
-                  for (int s = 0; s < NS; ++s) run[s] += qv[s];
+                    for (int s = 0; s < NS; ++s) run[s] = qv[s];
+                    run_n = 1;
+                    run_cell = cell;
+                  } else {
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) run[s] += qv[s];
+                    ++run_n;
+                  }
                 }
               }
             }
