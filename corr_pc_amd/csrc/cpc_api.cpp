@@ -448,7 +448,9 @@ struct JobBufs {
   DevBuf<Win> win_raw;
   DevBuf<uint2> hq;
   DevBuf<int32_t> red_beg, red_lst;  // target lists of the device-side jackknife reduction
+  HostBuf win_host;                   // sky windows, page-locked for the copy (not in no-alloc calls)
   void release() {
+    win_host.release();
     so.release();
     si.release();
     thr.release();
@@ -909,19 +911,25 @@ void make_items_jk(const Side &B, const Side &T, bool jk_on, bool same_touches, 
     Key key;
     std::vector<Chunk> cs;
   };
-  std::vector<Packed> packed;
-  for (auto &kv : by_key) {
-    auto &v = kv.second;
+  // each key's chunks sorted by cost and packed into items on host threads, concatenated in key order
+  std::vector<std::pair<const Key, std::vector<std::pair<double, Chunk>>> *> keys;
+  for (auto &kv : by_key) keys.push_back(&kv);
+  std::vector<std::vector<Packed>> per_key(keys.size());
+  parallel_for((int64_t)keys.size(), host_threads(), [&](int64_t k) {
+    auto &v = keys[k]->second;
     std::stable_sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
     for (size_t a = 0; a < v.size(); a += per_item) {
-      Packed pk{0.0, kv.first, {}};
+      Packed pk{0.0, keys[k]->first, {}};
       for (size_t b = a; b < std::min(v.size(), a + per_item); ++b) {
         pk.cost += v[b].first;
         pk.cs.push_back(v[b].second);
       }
-      packed.push_back(std::move(pk));
+      per_key[k].push_back(std::move(pk));
     }
-  }
+  });
+  std::vector<Packed> packed;
+  for (auto &v : per_key)
+    for (auto &pk : v) packed.push_back(std::move(pk));
   std::stable_sort(packed.begin(), packed.end(), [](const Packed &x, const Packed &y) { return x.cost > y.cost; });
   items.clear();
   costs.clear();
@@ -1117,7 +1125,8 @@ struct Plan {
   std::vector<int2> nbr;
   std::vector<Chunk> chunks;
   std::vector<uint8_t> aflags;
-  std::vector<Win> wins;
+  Win *wins = nullptr;  // sky windows of the outer galaxies: in JobBufs::win_host, or wins_vec
+  std::vector<Win> wins_vec;
   std::vector<double2> reach;  // sky: RA/Dec half-widths of the outer windows per redshift bin
   // work
   std::vector<Item> items_p;
@@ -1375,8 +1384,13 @@ Status setup(cpc_ctx *ctx, Plan &P) {
   } else {
     // per-galaxy windows of the outer catalog (lim_calc_sky), with the host libm
     const SkyTable T{p.sky_H, p.sky_dA, p.sky_n, p.sky_dz, p.z_max, p.z_sep_max};
-    std::vector<Win> &wins = P.wins;
-    wins.resize(co->n);
+    if (t_arena || B.win_host.ensure(std::max<int64_t>(co->n, 1) * sizeof(Win)) != hipSuccess) {
+      P.wins_vec.resize(std::max<int64_t>(co->n, 1));
+      P.wins = P.wins_vec.data();
+    } else {
+      P.wins = reinterpret_cast<Win *>(B.win_host.p);
+    }
+    Win *const wins = P.wins;
     std::vector<double> dec_half;
     // RA/Dec reach per bin of the outer redshift
     double ozlo = INFINITY, ozhi = -INFINITY;
@@ -1444,8 +1458,14 @@ Status setup(cpc_ctx *ctx, Plan &P) {
       }
     double hd = 1.0;
     if (!dec_half.empty()) {
-      std::nth_element(dec_half.begin(), dec_half.begin() + dec_half.size() / 2, dec_half.end());
-      hd = dec_half[dec_half.size() / 2] * (ctx->col_frac > 0 ? ctx->col_frac : 0.5);
+      // the median Dec half-width sets the cell size only (never which pairs are counted): from at
+      // most 65536 evenly strided galaxies
+      const size_t stride = dec_half.size() / 65536 + 1;
+      std::vector<double> smp;
+      smp.reserve(dec_half.size() / stride + 1);
+      for (size_t i = 0; i < dec_half.size(); i += stride) smp.push_back(dec_half[i]);
+      std::nth_element(smp.begin(), smp.begin() + smp.size() / 2, smp.end());
+      hd = smp[smp.size() / 2] * (ctx->col_frac > 0 ? ctx->col_frac : 0.5);
     }
     const double span_ra = std::max(rahi - ralo, 1e-6), span_dec = std::max(dechi - declo, 1e-6);
     hd = std::max(hd, span_dec / 2048);
@@ -1472,7 +1492,7 @@ Status setup(cpc_ctx *ctx, Plan &P) {
     }
     if (check(B.win_raw.ensure(std::max<int64_t>(co->n, 1)), st, "alloc")) return st;
     if (co->n)
-      check(hipMemcpyAsync(B.win_raw.p, wins.data(), co->n * sizeof(Win), hipMemcpyHostToDevice, s), st, "copy");
+      check(hipMemcpyAsync(B.win_raw.p, wins, co->n * sizeof(Win), hipMemcpyHostToDevice, s), st, "copy");
     if (st.code) return st;
   }
 
@@ -2027,13 +2047,18 @@ int64_t assemble(const Plan &P, const std::vector<double> &acc, const std::vecto
       if (lo < hi && starts > lo) ++sparse;
     }
   }
-  if (P.sky && ci->z_sorted && ci->n > 0) {  // sparse-window diagnostic on the host
-    for (int64_t i = 0; i < co->n; ++i) {
-      const double inc = co->h_inc.empty() ? 0.0 : co->h_inc[i];
-      if (inc > p.include_prob_max) continue;
-      auto it = std::lower_bound(ci->h_z.begin(), ci->h_z.end(), P.wins[i].zmin);
-      if (it != ci->h_z.end() && *it > P.wins[i].zmax) ++sparse;
-    }
+  if (P.sky && ci->z_sorted && ci->n > 0) {  // sparse-window diagnostic on host threads
+    const int64_t blk = 1 << 16, nblk = (co->n + blk - 1) / blk;
+    std::vector<int64_t> cnt(nblk, 0);
+    parallel_for(nblk, threads, [&](int64_t b) {
+      for (int64_t i = b * blk; i < std::min(co->n, (b + 1) * blk); ++i) {
+        const double inc = co->h_inc.empty() ? 0.0 : co->h_inc[i];
+        if (inc > p.include_prob_max) continue;
+        auto it = std::lower_bound(ci->h_z.begin(), ci->h_z.end(), P.wins[i].zmin);
+        if (it != ci->h_z.end() && *it > P.wins[i].zmax) ++cnt[b];
+      }
+    });
+    for (int64_t c : cnt) sparse += c;
   }
 
   const size_t cells_all = (size_t)R * ncell;

@@ -133,11 +133,18 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
     const int cy1 = uni((int)fmax(0.0, fmin((double)(P.ndec - 1), floor((dechi + eps - P.dec0) / P.hdec))));
     const double zsl = zlo - 1e-12, zsh = zhi + 1e-12;
 
+    // (the sums share a struct with the count: an array of doubles alone would be promoted to one
+    // vector register tuple, which every join of the loop's control flow copies whole)
     int run_cell = -1;
-    unsigned run_n = 0;
-    double run[NS];
+    struct {
+      double v[NS];
+      unsigned n;
+    } R;
+    R.n = 0;
 #pragma unroll
-    for (int s = 0; s < NS; ++s) run[s] = 0.0;
+    for (int s = 0; s < NS; ++s) R.v[s] = 0.0;
+    double *const run = R.v;
+    unsigned &run_n = R.n;
 
     for (int cy = cy0; cy <= cy1; ++cy) {
       for (int cx = cx0; cx <= cx1; ++cx) {
@@ -266,19 +273,6 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
               }
             }
             if (cell >= 0) {
-              if (cell != run_cell) {
-                if (run_cell >= 0) {
-                  if constexpr (CUT)
-                    flush_run<NS>(mynum, myacc, lcell, lbase, gnum, gacc, ncell, run_cell, run_n, run);
-                  else
-                    lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
-                }
-                run_cell = cell;
-                run_n = 0;
-#pragma unroll
-                for (int s = 0; s < NS; ++s) run[s] = 0.0;
-              }
-              ++run_n;
               const Gal go{po.x, po.y, po.w, po.e1, po.e2, po.phi};
               const Gal gi{pi.x, pi.y, pi.w, pi.e1, pi.e2, pi.phi};
               // angle_orient_axis_sky: atan((dec_i - dec_o) / ((ra_i - ra_o) cos((dec_o + dec_i)/2)))
@@ -287,8 +281,23 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
               const double Y = pi.y - po.y;
               double qv[NS];
               pair_values_v<K, SPLIT>(go, gi, P.gws, X, Y, qv);
+              if (cell != run_cell) {
+                if (run_cell >= 0) {
+                  if constexpr (CUT)
+                    flush_run<NS>(mynum, myacc, lcell, lbase, gnum, gacc, ncell, run_cell, run_n, run);
+                  else
+                    lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
+                }
+                // the new run starts from this pair's values
 #pragma unroll
-              for (int s = 0; s < NS; ++s) run[s] += qv[s];
+                for (int s = 0; s < NS; ++s) run[s] = qv[s];
+                run_n = 1;
+                run_cell = cell;
+              } else {
+#pragma unroll
+                for (int s = 0; s < NS; ++s) run[s] += qv[s];
+                ++run_n;
+              }
             }
           }
         }
