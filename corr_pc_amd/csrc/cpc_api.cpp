@@ -839,7 +839,7 @@ Status side_tables(cpc_ctx *, Side &S, size_t ncells) {
 template <typename ForNbr>
 void make_items_jk(const Side &B, const Side &T, bool jk_on, bool same_touches, int nreg, int per_item,
                    size_t ncols, ForNbr for_nbr, std::vector<Item> &items, std::vector<double> &costs,
-                   std::vector<Chunk> &chunks, std::vector<int3> &ytar) {
+                   std::vector<Chunk> &chunks, std::vector<int3> &ytar, int threads) {
   struct Key {
     int target, mode, reg;
     bool operator<(const Key &o) const { return std::tie(target, mode, reg) < std::tie(o.target, o.mode, o.reg); }
@@ -915,7 +915,7 @@ void make_items_jk(const Side &B, const Side &T, bool jk_on, bool same_touches, 
   std::vector<std::pair<const Key, std::vector<std::pair<double, Chunk>>> *> keys;
   for (auto &kv : by_key) keys.push_back(&kv);
   std::vector<std::vector<Packed>> per_key(keys.size());
-  parallel_for((int64_t)keys.size(), host_threads(), [&](int64_t k) {
+  parallel_for((int64_t)keys.size(), threads, [&](int64_t k) {
     auto &v = keys[k]->second;
     std::stable_sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
     for (size_t a = 0; a < v.size(); a += per_item) {
@@ -1520,7 +1520,8 @@ Status setup(cpc_ctx *ctx, Plan &P) {
 
 // The plan's work items, from its sides' group tables: host work only, so the plans of a batch
 // build theirs concurrently.
-void setup_items(Plan &P) {
+// (threads: host threads for this plan's own work; 1 when plans are set up on threads already)
+void setup_items(Plan &P, int threads) {
   const bool sky = P.sky, jk_on = P.jk_on;
   const int nreg = P.nreg, per_item = P.per_item;
   const Side &SO = *P.SO, &SI = *P.SI;
@@ -1537,7 +1538,7 @@ void setup_items(Plan &P) {
           fn(uy * ncx + ux);
         }
       };
-      make_items_jk(SO, SI, jk_on, true, nreg, per_item, ncols, for_nbr, P.items_p, P.cost_p, P.chunks, P.ytar);
+      make_items_jk(SO, SI, jk_on, true, nreg, per_item, ncols, for_nbr, P.items_p, P.cost_p, P.chunks, P.ytar, threads);
     } else {
       // the sky: cells within the widest window reach of any outer galaxy (every RA cell when some
       // window covers the full circle)
@@ -1553,7 +1554,7 @@ void setup_items(Plan &P) {
         for (int y = std::max(0, bcy - dy); y <= std::min(ncy - 1, bcy + dy); ++y)
           for (int x = std::max(0, bcx - dx); x <= std::min(ncx - 1, bcx + dx); ++x) fn(y * ncx + x);
       };
-      make_items_jk(SO, SI, jk_on, false, nreg, per_item, ncols, for_nbr, P.items_p, P.cost_p, P.chunks, P.ytar);
+      make_items_jk(SO, SI, jk_on, false, nreg, per_item, ncols, for_nbr, P.items_p, P.cost_p, P.chunks, P.ytar, threads);
     }
     P.nP = (size_t)2 * nreg + P.ytar.size();
     if (jk_on) {
@@ -1841,8 +1842,10 @@ Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
   std::vector<Launch> launches;
   for (const auto &g : groups) {
     std::vector<std::pair<double, Item>> v;
-    size_t lds = 0;
+    size_t lds = 0, nv = 0;
     int threads = 0;
+    for (size_t i : g.second) nv += plans[i]->items_p.size();
+    v.reserve(nv);
     for (size_t i : g.second) {
       const Plan &P = *plans[i];
       for (size_t k = 0; k < P.items_p.size(); ++k) {
@@ -1854,9 +1857,23 @@ Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
       threads = P.threads;
     }
     if (v.empty()) continue;
-    std::stable_sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
     launches.push_back(Launch{g.first.first, g.first.second != 0, threads, lds, all.size(), v.size()});
-    for (const auto &x : v) all.push_back(x.second);
+    if (g.second.size() == 1) {
+      for (const auto &x : v) all.push_back(x.second);  // one plan: its items are in cost order already
+      continue;
+    }
+    // items of several plans, most expensive first (the grid runs in that order): a stable counting
+    // sort on the cost's binary logarithm in 1/16 steps, ties in plan order
+    auto bucket = [](double c) {
+      const double l = c > 0 ? std::log2(c) * 16.0 : 0.0;
+      return (int)std::min(4095.0, std::max(0.0, l));
+    };
+    std::vector<int64_t> cnt(4097, 0);
+    for (const auto &x : v) ++cnt[4095 - bucket(x.first) + 1];
+    for (int b = 1; b <= 4096; ++b) cnt[b] += cnt[b - 1];
+    const size_t base = all.size();
+    all.resize(base + v.size());
+    for (const auto &x : v) all[base + cnt[4095 - bucket(x.first)]++] = x.second;
   }
   // every plan's jackknife reduction and sparse-window count, one launch each: the tables and their
   // block offsets in one device region
@@ -2218,7 +2235,7 @@ cpc_status run_jobs(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *to
       if (x.code) return fail(ctx, x);
   }
   // work items of every plan: host work, spread over threads when the batch is large
-  parallel_for(n_jobs, host_threads(), [&](int64_t i) { setup_items(plans[i]); });
+  parallel_for(n_jobs, host_threads(), [&](int64_t i) { setup_items(plans[i], n_jobs > 1 ? 1 : host_threads()); });
   {
     Status st = stage_tables(ctx, plans);
     if (st.code) return fail(ctx, st);
