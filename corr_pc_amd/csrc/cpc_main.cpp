@@ -179,6 +179,11 @@ bool data_ini(DataInfo &d, const std::string &name) {
   if (d.data_sorted < 0 || d.data_sorted > 2) return fail("data_sorted must be 0, 1 or 2");
   if (d.do_jk != 0 && d.do_jk != 1) return fail("do_jk must be 0 or 1");
   if (d.estimator < 0 || d.estimator > 4) return fail("estimator must be 0-4");
+  // sig_crit > 0 weights every pair by Sigma_crit^-2 and scales its shape sums by Sigma_crit
+  // (corels.cpp:52-61, calcs.cpp:142-176), the weak-lensing weighting; counting without it would
+  // silently give other numbers
+  if (d.sig_crit != 0)
+    return fail("sig_crit " + std::to_string(d.sig_crit) + " is not supported (the weak-lensing Sigma_crit weights)");
 
   d.n_bin_var = 7;
   if (d.periodic_box) {

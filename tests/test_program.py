@@ -368,3 +368,7 @@ def test_more_error_paths(tmp_path):
     p = inputs.write_inp(str(tmp_path / "ds2.inp"), **dict(binp, data_sorted=2))
     r = run_prog(p, "--plan")
     assert r.returncode == 1 and "data_sorted==2 doesnot work with periodic box" in r.stdout, r.stdout
+    # sig_crit weighting (corels.cpp:52-61) is refused rather than silently left out
+    p = inputs.write_inp(str(tmp_path / "sc.inp"), **dict(binp, sig_crit=1))
+    r = run_prog(p, "--plan")
+    assert r.returncode == 1 and "sig_crit 1 is not supported" in r.stdout, r.stdout
