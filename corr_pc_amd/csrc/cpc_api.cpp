@@ -1711,6 +1711,11 @@ Status setup_tables(cpc_ctx *ctx, Plan &P) {
     // sum near 1, about 4 ulp, propagated through acos) plus 1e-12; theta >= rp_min / DA_max for pairs that can reach
     // a bin.  Rounding s to FP32 adds 2^-24.  In bin units: la / ln 2 per unit relative error, plus
     // v_log_f32 (2^-21 max(1, |log2 s|)) times la, the rounding of la and lb, and a margin of 2.
+    // pi bin by product: (dP - pmin) * fl(1/psize) is within 2^-51 |u| of the quotient the reference
+    // rounds (one rounding of 1/psize, one of the product, half an ulp of the IEEE quotient); with
+    // |u| below 2^20 (the kernel checks it) that is under 2^-31, so a fraction 1e-9 from an edge is certain
+    A.inv_psize = 1.0 / dv.psize;
+    A.pguard = 1e-9;
     A.split_on = 0;
     if (P.split && !p.lin_bin && p.rp_min > 0 && sm.pim == 0 && sm.rpm == 0 && !sm.rmu) {
       double da_max = 0;

@@ -289,36 +289,26 @@ __device__ __forceinline__ int zsearch(const double *__restrict__ zs, int lo, in
   return lo;
 }
 
-template <int NS>
-__device__ __forceinline__ void lds_add(unsigned *lnum, double *lacc, int ncell, int cell, unsigned n,
-                                        const double *v) {
-  atomicAdd(&lnum[cell], n);
-#pragma unroll
-  for (int s = 0; s < NS; ++s) atomicAdd(&lacc[s * ncell + cell], v[s]);
-}
-
-// A closed run of one lane: into its LDS histogram copy when the cell is held there, else straight
-// into the pass target's global histogram (gnum, gacc: target base, [NS][ncell] slots).  The LDS
-// side uses explicit LDS pointers, so the two sides stay distinct instructions.
 using lds_u32 = __attribute__((address_space(3))) unsigned;
 using lds_f64 = __attribute__((address_space(3))) double;
-template <int NS>
-__device__ __forceinline__ void flush_run(lds_u32 *mynum, lds_f64 *myacc, int lcell, int lbase,
-                                          unsigned long long *gnum, double *gacc, int ncell, int cell, unsigned n,
-                                          const double *v) {
+
+// A closed run of one lane into the cell-major LDS histogram copy (lacc[cell][NS]: one address and
+// immediate slot offsets), or, for cells below the LDS rows (CUT only), straight into the target's
+// global histogram ([NS][ncell]).
+template <int NS, bool CUT>
+__device__ __forceinline__ void flush_cm(lds_u32 *mynum, lds_f64 *myacc, int lbase, unsigned long long *gnum,
+                                         double *gacc, int ncell, int cell, unsigned n, const double *v) {
   const int lc = cell - lbase;
-  if (lc >= 0) {
+  if (!CUT || lc >= 0) {
     __hip_atomic_fetch_add(&mynum[lc], n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    lds_f64 *const a = myacc + lc * NS;
 #pragma unroll
-    for (int s = 0; s < NS; ++s)
-      __hip_atomic_fetch_add(&myacc[s * lcell + lc], v[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (int s = 0; s < NS; ++s) __hip_atomic_fetch_add(&a[s], v[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   } else {
     atomicAdd(&gnum[cell], (unsigned long long)n);
 #pragma unroll
     for (int s = 0; s < NS; ++s) unsafeAtomicAdd(&gacc[(size_t)s * ncell + cell], v[s]);
   }
 }
-
-
 
 }  // namespace cpc

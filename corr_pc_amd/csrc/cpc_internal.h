@@ -142,6 +142,7 @@ struct alignas(64) Win {
 
 struct SkyArgs {
   double pmin, pmax, psize;
+  double inv_psize, pguard;  // split kernels: pi bin from (dP - pmin) * inv_psize, certain pguard from an edge
   int n_rp, n_pi, ncell, gws, n_items, nslots, lin;
   // cell grid over RA/Dec (degrees)
   double ra0, dec0, hra, hdec;

@@ -58,25 +58,6 @@ __device__ __forceinline__ float wave_max(float v) {
 
 using CKArgs = __attribute__((address_space(4))) const KArgs;
 
-// A closed run of one lane into the cell-major LDS histogram copy (lacc[cell][NS]: one address and
-// immediate slot offsets), or, for cells below the LDS rows (CUT only), straight into the target's
-// global histogram ([NS][ncell]).
-template <int NS, bool CUT>
-__device__ __forceinline__ void flush_cm(lds_u32 *mynum, lds_f64 *myacc, int lbase, unsigned long long *gnum,
-                                         double *gacc, int ncell, int cell, unsigned n, const double *v) {
-  const int lc = cell - lbase;
-  if (!CUT || lc >= 0) {
-    __hip_atomic_fetch_add(&mynum[lc], n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    lds_f64 *const a = myacc + lc * NS;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) __hip_atomic_fetch_add(&a[s], v[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-  } else {
-    atomicAdd(&gnum[cell], (unsigned long long)n);
-#pragma unroll
-    for (int s = 0; s < NS; ++s) unsafeAtomicAdd(&gacc[(size_t)s * ncell + cell], v[s]);
-  }
-}
-
 // The box pair kernel.  SPLIT classifies candidates in FP32 (log rp bins) and re-evaluates the
 // guard band in FP64; without it every candidate takes the FP64 path.  AUTO compiles in corels1's
 // order predicate (auto_mode 1 and 2).  RMU compiles in the r-mu binning of coordinates 7 (with

@@ -73,15 +73,16 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
   constexpr int NE = KindNV<K>::NE;
   constexpr int NS = 1 + NV + NE;
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int next_chunk;  // next chunk of the item that no wave has taken
   // LDS as in the box kernel: ncopy lane-selected copies of the rp rows from row0 up, lnum[ncopy][lcell]
-  // and lacc[ncopy][NS][lcell], then the thresholds; runs of the rows below go to the global histogram.
-  // Compiled in where the flush dominates (7 or more slots per cell).
+  // and cell-major lacc[ncopy][lcell][NS], then the thresholds; runs of the rows below go to the global
+  // histogram.  Compiled in where the flush dominates (7 or more slots per cell).
   constexpr bool CUT = NS >= 7;
   const int ncell = P.ncell, ncopy = CUT ? P.ncopy : 1;
   const int lbase = CUT ? P.row0 * P.n_pi : 0, lcell = CUT ? ncell - lbase : ncell;
   unsigned *lnum = reinterpret_cast<unsigned *>(smem);
   double *lacc = reinterpret_cast<double *>(smem + ((ncopy * lcell * 4 + 15) & ~15));
-  double *lthr = lacc + (size_t)ncopy * NS * lcell;
+  double *lthr = lacc + (size_t)ncopy * lcell * NS;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -89,7 +90,7 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
   const int nwave = blockDim.x >> 6;
   const int cp = lane % ncopy;
   lds_u32 *const mynum = (lds_u32 *)(lnum + cp * lcell);
-  lds_f64 *const myacc = (lds_f64 *)(lacc + (size_t)cp * NS * lcell);
+  lds_f64 *const myacc = (lds_f64 *)(lacc + (size_t)cp * lcell * NS);
   for (int k = tid; k <= P.n_rp; k += blockDim.x) lthr[k] = P.thr[k];
 
   const Item it = P.items[blockIdx.x];
@@ -101,11 +102,15 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
   for (int c = tid; c < ncopy * lcell; c += blockDim.x) {
     lnum[c] = 0;
 #pragma unroll
-    for (int s = 0; s < NS; ++s) lacc[s * ncopy * lcell + c] = 0.0;
+    for (int s = 0; s < NS; ++s) lacc[(size_t)c * NS + s] = 0.0;
   }
+  if (tid == 0) next_chunk = nwave;
   __syncthreads();
 
-  for (int ch = wave; ch < it.n; ch += nwave) {
+  // chunks come sorted by decreasing cost; each wave starts on its own and then takes the next
+  // unclaimed one, so the waves of the group finish close together
+  int next_nx = 0;
+  for (int ch = wave; ch < it.n; ch = uni(next_nx)) {
     const Chunk ck = P.chunks[it.begin + ch];
     const int base = uni(ck.first);
     const int cnt = uni(ck.count);
@@ -187,7 +192,19 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
             if (ok) {
               // pi_calc or pi_calc_projected, corel_bin; ephi (pi_calc_phi) and r-mu bin pi late
               double dP = (!GEN || P.pim == 0) ? si.dc - so.dc : 0.0;
-              double q2 = floor((dP - P.pmin) / P.psize);
+              double q2;
+              if constexpr (SPLIT && !GEN) {
+                // the pi bin from a product with 1/psize: it differs from the IEEE quotient by about
+                // one ulp, so where its fraction lies at least pguard from both edges its floor is
+                // the reference's; nearer an edge the quotient itself decides
+                const double u = (dP - P.pmin) * P.inv_psize;
+                q2 = floor(u);
+                const double fr = u - q2;
+                if (!(fr >= P.pguard && fr <= 1.0 - P.pguard && fabs(u) < 1048576.0))
+                  q2 = floor((dP - P.pmin) / P.psize);
+              } else {
+                q2 = floor((dP - P.pmin) / P.psize);
+              }
               const bool late = GEN && (P.pim == 2 || P.rmu);
               bool exact = true;
               if (SPLIT && P.split_on && q2 >= 0.0 && q2 < (double)P.n_pi) {
@@ -282,12 +299,7 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
               double qv[NS];
               pair_values_v<K, SPLIT>(go, gi, P.gws, X, Y, qv);
               if (cell != run_cell) {
-                if (run_cell >= 0) {
-                  if constexpr (CUT)
-                    flush_run<NS>(mynum, myacc, lcell, lbase, gnum, gacc, ncell, run_cell, run_n, run);
-                  else
-                    lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
-                }
+                if (run_cell >= 0) flush_cm<NS, CUT>(mynum, myacc, lbase, gnum, gacc, ncell, run_cell, run_n, run);
                 // the new run starts from this pair's values
 #pragma unroll
                 for (int s = 0; s < NS; ++s) run[s] = qv[s];
@@ -303,12 +315,10 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
         }
       }
     }
-    if (run_cell >= 0) {
-      if constexpr (CUT)
-        flush_run<NS>(mynum, myacc, lcell, lbase, gnum, gacc, ncell, run_cell, run_n, run);
-      else
-        lds_add<NS>(lnum, lacc, ncell, run_cell, run_n, run);
-    }
+    if (run_cell >= 0) flush_cm<NS, CUT>(mynum, myacc, lbase, gnum, gacc, ncell, run_cell, run_n, run);
+    int nx = 0;
+    if (lane == 0) nx = atomicAdd(&next_chunk, 1);
+    next_nx = __builtin_amdgcn_readfirstlane(nx);
   }
   __syncthreads();
   for (int c = tid; c < lcell; c += blockDim.x) {
@@ -319,7 +329,7 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
         double v = 0.0;
-        for (int k = 0; k < ncopy; ++k) v += lacc[((size_t)k * NS + s) * lcell + c];
+        for (int k = 0; k < ncopy; ++k) v += lacc[((size_t)k * lcell + c) * NS + s];
         unsafeAtomicAdd(&gacc[(size_t)s * ncell + lbase + c], v);
       }
     }
