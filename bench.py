@@ -191,8 +191,10 @@ def wgg_workload(args, steps=1, warmup=1):
 
 def large_workload(args, steps=1, warmup=0):
     """The large box at its real shape: L = 1000, 2e7 galaxies (30% in 40,000 clumps) with shapes and
-    weights, 25 log rp bins in [0.1, 200), 60 pi bins in [-150, 150), 64 jackknife regions; the SD
-    pass of every `--large-stride`-th outer galaxy against all 2e7 inner galaxies."""
+    weights, 4e7 randoms, 25 log rp bins in [0.1, 200), 60 pi bins in [-150, 150), 64 jackknife
+    regions; the SD pass (or with `--large-pass SRd` the randoms against the shapes) of every
+    `--large-stride`-th outer galaxy against all inner galaxies.  With `--large-full` every residue
+    class of the stride runs in turn (one call each), which together are the whole pass."""
     import numpy as np
 
     from corr_pc_amd import synth
@@ -200,30 +202,48 @@ def large_workload(args, steps=1, warmup=0):
     from corr_pc_amd.passes import config_datainfo, setup_pass
 
     t = time.perf_counter()
-    cats = synth.catalogs(4, n_rand=10)
+    srd = args.large_pass == "SRd"
+    cats = synth.catalogs(4, n_rand=None if srd else 10)  # the SD pass reads no randoms
     di = config_datainfo(cats["config"])
-    outer, inner, p, _ = setup_pass("SD", cats["D"], cats["S"], di, precision=PREC_SPLIT)
-    outer = outer.subset(np.arange(0, outer.n, args.large_stride))
-    log(f"large: catalogs ready in {time.perf_counter() - t:.1f}s: outer {outer.n}, inner {inner.n}")
+    if srd:  # the density randoms against the shape sample (do_corr.cpp:125)
+        outer0, inner, p, _ = setup_pass("SRd", cats["Rd"], cats["S"], di, precision=PREC_SPLIT)
+    else:
+        outer0, inner, p, _ = setup_pass("SD", cats["D"], cats["S"], di, precision=PREC_SPLIT)
+    offsets = range(args.large_stride) if args.large_full else range(1)
+    parts = [outer0.subset(np.arange(o, outer0.n, args.large_stride)) for o in offsets]
+    log(f"large: catalogs ready in {time.perf_counter() - t:.1f}s: outer {sum(q.n for q in parts)} of {outer0.n} "
+        f"in {len(parts)} call(s), inner {inner.n}")
     eng = Engine(0)
-    eng.upload(outer)
     eng.upload(inner)
+    for q in parts:
+        eng.upload(q)
     for k in range(warmup):
-        eng.count(outer, inner, p)
+        eng.count(parts[0], inner, p)
+    n_binned = n_cand = n_re = 0
+    ms_kernel = 0.0
     t0 = time.perf_counter()
     for k in range(steps):
-        _, st = eng.count(outer, inner, p)
-        log(f"large step {k}: {st['ms_total']:.0f} ms (kernel {st['ms_primary']:.0f} ms, build {st['ms_build']:.0f} "
-            f"ms), binned {st['n_binned']}")
+        n_binned = n_cand = n_re = 0
+        ms_kernel = 0.0
+        for i, q in enumerate(parts):
+            _, st = eng.count(q, inner, p)
+            n_binned += st["n_binned"]
+            n_cand += st["n_candidates"]
+            n_re += st["n_fp64_reeval"]
+            ms_kernel += st["ms_primary"]
+            log(f"large step {k} part {i}: {st['ms_total']:.0f} ms (kernel {st['ms_primary']:.0f} ms, build "
+                f"{st['ms_build']:.0f} ms), binned {st['n_binned']}")
     dt = (time.perf_counter() - t0) / steps
     eng.close()
-    achieved = FLOPS_PER_PAIR["SS"] * st["n_binned"] / (st["ms_primary"] * 1e-3) / 1e12
+    achieved = FLOPS_PER_PAIR["SD" if srd else "SS"] * n_binned / (ms_kernel * 1e-3) / 1e12
+    outer_n = sum(q.n for q in parts)
     return {
-        "workload": f"wg+/w++ periodic box L=1000, 2e7 galaxies (40,000 clumps), SD pass, every {args.large_stride}th "
-                    f"outer galaxy x all inner, 25 log rp bins [0.1,200) x 60 pi bins [-150,150), 64 jackknife regions",
-        "value": st["n_binned"] / dt, "unit": "pair evals/s", "ms_per_step": dt * 1e3, "n_binned": st["n_binned"],
-        "n_candidates": st["n_candidates"], "n_fp64_reeval": st["n_fp64_reeval"],
-        "kernel_ms": st["ms_primary"], "frac_fp32_valu": achieved / PEAK_TFLOPS,
+        "workload": f"wg+/w++ periodic box L=1000, 2e7 galaxies (40,000 clumps), 4e7 randoms, {args.large_pass} pass, "
+                    f"{outer_n} of {outer0.n} outer galaxies x all {inner.n} inner, 25 log rp bins [0.1,200) x 60 pi "
+                    f"bins [-150,150), 64 jackknife regions",
+        "value": n_binned / dt, "unit": "pair evals/s", "ms_per_step": dt * 1e3, "n_binned": n_binned,
+        "n_candidates": n_cand, "n_fp64_reeval": n_re, "n_calls": len(parts),
+        "kernel_ms": ms_kernel, "frac_fp32_valu": achieved / PEAK_TFLOPS,
     }
 
 
@@ -299,6 +319,10 @@ def main():
                     help="wgplus: the 1M-galaxy wg+/w++ box (default, with the others as secondary); mocks, "
                          "large, sky, wgg: one of the secondary workloads alone")
     ap.add_argument("--large-stride", type=int, default=8, help="large workload: every n-th outer galaxy")
+    ap.add_argument("--large-full", action="store_true",
+                    help="large workload: every residue class of the stride in turn, i.e. the whole pass")
+    ap.add_argument("--large-pass", choices=("SD", "SRd"), default="SD",
+                    help="large workload: the SD pass (default) or the SRd pass (4e7 randoms x 2e7 shapes)")
     ap.add_argument("--mocks", type=int, default=256, help="number of mocks of the mocks workload")
     ap.add_argument("--tune-stride", type=int, default=16, help="mocks workload: autotune on every n-th mock")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

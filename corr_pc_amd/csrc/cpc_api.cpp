@@ -1665,7 +1665,7 @@ Status setup_tables(cpc_ctx *ctx, Plan &P) {
     K.nbr = B.nbr.p;
     K.auto_mode = autop ? (p.unsorted ? 2 : 1) : 0;
     K.chunks = B.chunks.p;
-    // primary pass: blocks = outer galaxies, tiles = inner galaxies
+    // the pass: blocks = outer galaxies, tiles = inner galaxies
     P.kp = K;
     P.kp.brec = SO.rec.p;
     P.kp.bz = SO.zs.p;

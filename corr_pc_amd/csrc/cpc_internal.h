@@ -72,8 +72,8 @@ enum TileFilter : int32_t {
   kTileRegionC1 = 4,    // region treg (!= a), coin >= 0.5
 };
 
-// Histogram slots per pair kind.  The primary pass accumulates [wt, vals..., errs...], the touch
-// pass [wt, vals...].  Quantity codes: 0 = wt, 1..6 = data[f].val, 7..12 = data[f-6].err.
+// Histogram slots per pair kind: every target accumulates [wt, vals..., errs...].  Quantity codes:
+// 0 = wt, 1..6 = data[f].val, 7..12 = data[f-6].err.
 constexpr int kMaxSlots = 13;
 
 // Arguments of one pass of one job.  A launch carries a device table of them and every work item
