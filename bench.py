@@ -9,7 +9,9 @@ assembly of the jackknife layout.
 The line also carries the measured error of the weighted bins (split against FP64 on the full pass,
 and against the CPU oracle on the cpu_baseline sample), and, under "secondary", the other workloads
 at their real shapes: the 2e6-galaxy sky survey (split and FP64), the 256 batched mocks in one call
-against sequential calls, and the 2e7-galaxy box on every 8th outer galaxy.
+against sequential calls, and the 2e7-galaxy box on every 8th outer galaxy.  `--workload` runs one
+of these alone, or a full-size run no default line includes: `large --large-full` (the large box's
+whole SD or SRd pass) and `skyfull` (the sky survey's four passes with its 2e7 randoms).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -295,8 +297,95 @@ def sky_workload(args, steps=1, warmup=1):
     }
 
 
+def sky_passes_workload(args, steps=1, warmup=0):
+    """The sky survey's whole pass set at its real shape: 2e6 galaxies and 2e7 randoms in the six
+    fields, which_corr = 1 (wg+) with one random catalog for both roles, as do_corr runs it
+    (do_corr.cpp:66-78, 110-131 with RsRd_same): SD = corels2(D, S) (shape_density, 2e6 x 2e6),
+    SRs = corels2(Rs, S) (randoms against the shapes, 2e7 x 2e6), DRs = corels2(Rs, D) (density
+    pairs, 2e7 x 2e6) and RsRd = corels2(Rs, Rs) (density pairs, 2e7 x 2e7).  Each pass runs in
+    `--sky-stride` calls, one per residue class of its outer galaxies, in split precision and, with
+    `--sky-fp64`, in plain FP64 as well (counts compared call by call)."""
+    import numpy as np
+
+    from corr_pc_amd import synth
+    from corr_pc_amd.engine import PREC_FP64, PREC_SPLIT, Catalog, Engine, make_sky_params
+    from corr_pc_amd.passes import DataInfo, setup_pass
+
+    t = time.perf_counter()
+    table = synth.distance_table(0.3, 1e-4, 1.0)
+    S, n_jk, _ = synth.sky_catalog(2, "data", table=table)
+    R, n_jk_r, _ = synth.sky_catalog(2, "rand", table=table)
+    if n_jk_r != n_jk:
+        raise RuntimeError(f"the randoms span {n_jk_r} jackknife regions, the data {n_jk}")
+    D = Catalog(x=S.x, y=S.y, z=S.z, w=S.w, jk=S.jk, jk_prob=S.jk_prob, dc=S.dc, da=S.da)
+    pt = synth.sky_params_table(2, table)
+    # the role logic of corels2 (larger catalog outer, g_with_shape flip, pair kind per do_* flag);
+    # box_size is not read by it
+    di = DataInfo(which_corr=1, estimator=0, box_size=1.0, n_rp=20, rp_min=0.1, rp_max=100.0, n_pi=40,
+                  pi_min=-100.0, pi_max=100.0, do_jk=1, n_jk=n_jk)
+    passes = [("SD", D, S), ("SRs", R, S), ("DRs", R, D), ("RsRd", R, R)]
+    log(f"skyfull: catalogs ready in {time.perf_counter() - t:.1f}s: {S.n} galaxies, {R.n} randoms, {n_jk} "
+        f"jackknife regions")
+    eng = Engine(0)
+    for c in (D, S, R):
+        eng.upload(c)
+    precs = ((PREC_SPLIT, "split"), (PREC_FP64, "fp64")) if args.sky_fp64 else ((PREC_SPLIT, "split"),)
+    out = {}
+    for name, g1, g2 in passes:
+        outer, inner, pb, _ = setup_pass(name, g1, g2, di)
+        prm = {prec: make_sky_params(pt, di.n_rp, di.rp_min, di.rp_max, di.n_pi, di.pi_min, di.pi_max, kind=pb.kind,
+                                     g_with_shape=pb.g_with_shape, which_corr=1, do_jk=1, n_jk=n_jk,
+                                     include_prob_max=pb.include_prob_max, precision=prec)
+               for prec, _ in precs}
+        stride = max(1, args.sky_stride)
+        parts = [outer if stride == 1 else outer.subset(np.arange(o, outer.n, stride)) for o in range(stride)]
+        for q in parts:
+            if q is not outer:
+                eng.upload(q)
+        rec = {"outer": outer.n, "inner": inner.n, "kind": int(pb.kind), "g_with_shape": int(pb.g_with_shape),
+               "n_calls": len(parts)}
+        for k in range(warmup):
+            eng.count(parts[0], inner, prm[PREC_SPLIT])
+        for prec, label in precs:
+            n_binned = n_cand = n_host = 0
+            ms_kernel = 0.0
+            hists = []
+            t0 = time.perf_counter()
+            for i, q in enumerate(parts):
+                h, st = eng.count(q, inner, prm[prec])
+                hists.append(h.num.copy())
+                n_binned += st["n_binned"]
+                n_cand += st["n_candidates"]
+                n_host += st["n_host_pairs"]
+                ms_kernel += st["ms_primary"]
+                log(f"skyfull {name} {label} part {i}: {st['ms_total']:.0f} ms (kernel {st['ms_primary']:.0f} ms, "
+                    f"build {st['ms_build']:.0f} ms), binned {st['n_binned']}, host-binned {st['n_host_pairs']}")
+            dt = time.perf_counter() - t0
+            rec[label] = {"ms": dt * 1e3, "kernel_ms": ms_kernel, "n_binned": n_binned, "n_candidates": n_cand,
+                          "n_host_pairs": n_host, "value": n_binned / dt}
+            if label == "split":
+                split_hists = hists
+            else:
+                rec["counts_identical"] = all(np.array_equal(a, b) for a, b in zip(split_hists, hists))
+                rec["split_vs_fp64"] = rec["fp64"]["ms"] / rec["split"]["ms"]
+        out[name] = rec
+        for q in parts:
+            if q is not outer:
+                eng.free(q)
+    eng.close()
+    n_all = sum(r["split"]["n_binned"] for r in out.values())
+    ms_all = sum(r["split"]["ms"] for r in out.values())
+    return {
+        "workload": f"sky survey, 2e6 galaxies and 2e7 randoms in 6 RA/Dec fields (~150 deg^2), 0.2<z<1, which_corr 1: "
+                    f"passes SD, SRs, DRs, RsRd (DD, RD, DR, RR), 20 log rp bins [0.1,100) x 40 pi bins [-100,100), "
+                    f"{n_jk} jackknife regions",
+        "value": n_all / (ms_all * 1e-3), "unit": "pair evals/s", "ms_per_step": ms_all, "n_binned": n_all,
+        "passes": out,
+    }
+
+
 def standalone(name, fn, args):
-    """--workload mocks|large|sky: one workload as its own JSON line."""
+    """--workload mocks|large|sky|wgg|skyfull: one workload as its own JSON line."""
     r = fn(args, steps=args.steps, warmup=args.warmup)
     out = {"metric": f"pair evals/sec: {name}", "value": r["value"], "unit": "pair evals/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
@@ -315,9 +404,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="skip the sky, mocks and large-box workloads")
-    ap.add_argument("--workload", choices=("wgplus", "mocks", "large", "sky", "wgg"), default="wgplus",
+    ap.add_argument("--workload", choices=("wgplus", "mocks", "large", "sky", "wgg", "skyfull"), default="wgplus",
                     help="wgplus: the 1M-galaxy wg+/w++ box (default, with the others as secondary); mocks, "
-                         "large, sky, wgg: one of the secondary workloads alone")
+                         "large, sky, wgg: one of the secondary workloads alone; skyfull: the sky survey's four "
+                         "passes with its 2e7 randoms")
+    ap.add_argument("--sky-stride", type=int, default=1,
+                    help="skyfull workload: calls per pass, one per residue class of the outer galaxies")
+    ap.add_argument("--sky-fp64", action="store_true", help="skyfull workload: also run every call in plain FP64")
     ap.add_argument("--large-stride", type=int, default=8, help="large workload: every n-th outer galaxy")
     ap.add_argument("--large-full", action="store_true",
                     help="large workload: every residue class of the stride in turn, i.e. the whole pass")
@@ -352,6 +445,10 @@ def main():
         return
     if args.workload == "sky":
         standalone("split-FP32 vs FP64 sky kernel, wg+ 2M-galaxy survey", sky_workload, args)
+        return
+    if args.workload == "skyfull":
+        standalone("split-FP32 sky kernel, the 2M-galaxy survey's SD, SRs, DRs and RsRd passes", sky_passes_workload,
+                   args)
         return
     if args.workload == "wgg":
         standalone("split-FP32 vs FP64 density kernel, wgg 1M-gal box", wgg_workload, args)
