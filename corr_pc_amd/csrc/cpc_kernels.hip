@@ -81,6 +81,11 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
                   P.s_cert_lo, P.linv, P.lrmin, P.la, P.lb, P.gA, P.gB, P.gC, P.gD, P.inv_psize32, P.n_pi, P.n_rp,
                   P.lin, P.gws, P.auto_mode, P.row0 * P.n_pi, (P.n_rp - P.row0) * P.n_pi, P.wlo, P.whi, P.rmu,
                   P.cull_r2, P.cull_d};
+  // three addends of the FP32 classification, held in vector registers where the kind has
+  // registers to spare (fewer than 7 slots): an FMA reads at most one scalar register, so with two
+  // scalar operands the loop copies one to a VGPR on every tile
+  float pioff_v = H.pioff32, gB_v = H.gB, lb_v = H.lb;
+  if constexpr (NS < 7) asm volatile("" : "+v"(pioff_v), "+v"(gB_v), "+v"(lb_v));
   const PRec *__restrict__ brec = P.brec;
   const double *__restrict__ bz = P.bz;
   const double *__restrict__ tz = P.tz;
@@ -117,7 +122,8 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
   const int tmode = it.tmode, treg = it.treg;
   unsigned long long *const gnum = P.accnum + (size_t)it.target * ncell;
   double *const gacc = P.acc + (size_t)it.target * NS * ncell;
-  unsigned long long ncand = 0, nre = 0;  // wave totals
+  unsigned long long ncand = 0;  // wave total
+  unsigned nre_l = 0;            // this lane's FP64 re-evaluations
   int next_nx = 0;
 
   // chunks come sorted by decreasing cost; each wave starts on its own and then takes the next
@@ -187,6 +193,7 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
           // in (z, index) order; unsorted: every image, in index order)
           const bool ordered = AUTO && (H.auto_mode == 2 || (H.auto_mode == 1 && sz <= 0));
           const bool unsorted_off = H.auto_mode != 2;
+          const unsigned long long ok_lanes = AUTO ? 0ull : popc_ballot(lane_ok);
           // FP32 offsets of this lane to the tile column: d = tile local + offset
           const float ax = Ox - me.xl, ay = Oy - me.yl, az = (float)(H.L * sz) - me.zf;
 
@@ -209,6 +216,8 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
               const float ey = fmaxf(fmaxf(by0 - Y, Y - by1) - H.cull_d, 0.0f);
               tmask = __ballot((lane & 3) == 3 && (lane >> 2) < nt && fmaf(ex, ex, ey * ey) <= H.cull_r2);
             }
+            // without the order predicate every tile of the batch meets the same candidate lanes
+            if (!AUTO) ncand += ok_lanes * (unsigned long long)__popcll(tmask);
             for (; tmask; tmask &= tmask - 1) {
               const int t = uni(__builtin_ctzll(tmask) >> 2);
               const int l3 = 4 * t + 3;  // lane of the quarter {xl, yl, zf, meta}
@@ -234,7 +243,7 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
                 const float dy = rl_f(cq.y, l3) + ay;
                 const float dz = rl_f(cq.z, l3) + az;
                 const float s32 = fmaf(dx, dx, dy * dy);
-                const float uu = fmaf(dz, H.inv_psize32, H.pioff32);
+                const float uu = fmaf(dz, H.inv_psize32, pioff_v);
                 const float fu = floorf(uu);
                 const float fr = uu - fu;
                 const bool near_pi = (fr < H.gz) || (fr > 1.0f - H.gz);
@@ -242,27 +251,27 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
                 const bool out = (!near_pi && !pi_in) || (s32 < H.s_out_lo) || (s32 >= H.s_out_hi);
                 // s32 >= s_cert_lo > 0 wherever cert can hold, so kf and gk are finite there
                 const float rinv = __builtin_amdgcn_rsqf(s32);
-                const float kf = __log2f(s32) * H.la + H.lb;
+                const float kf = fmaf(__log2f(s32), H.la, lb_v);  // one rounding fewer than the guard band allows for
                 const float fk = floorf(kf);
                 const float frk = kf - fk;
-                const float gk = fmaf(fmaf(H.gC, rinv, H.gB), rinv, H.gA);
+                const float gk = fmaf(fmaf(H.gC, rinv, gB_v), rinv, H.gA);
                 const bool cert = !near_pi && s32 >= H.s_cert_lo && frk >= gk && frk <= 1.0f - gk;
                 unc = ok && !out && !cert;
                 if (ok && !out && cert && fk >= 0.0f && fk < (float)H.n_rp)
                   cell = (int)fmaf(fk, (float)H.n_pi, fu);
               }
-              ncand += popc_ballot(ok);
-              if (__ballot(unc) != 0ull) {
-                nre += SPLIT ? popc_ballot(unc) : 0ull;
+              if (AUTO) ncand += popc_ballot(ok);
+              // a plain branch: the wave skips it when no lane is uncertain, and counts per lane
+              // (a ballot of a lane mask would be rebuilt from a 0/1 vector on every tile)
+              if (unc) {
+                if (SPLIT) ++nre_l;
                 const double tx = rl_d(cq.x, cq.y, 4 * t), ty = rl_d(cq.z, cq.w, 4 * t);
                 const double tzz = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(cz), t),
                                                     __builtin_amdgcn_readlane(__double2loint(cz), t));
-                if (unc) {
-                  if constexpr (SPLIT)
-                    cell = exact_cell<RMU>(P, lthr, me.x, me.y, mz, tx, ty, tzz, sx, sy, sz);
-                  else
-                    cell = exact_cell<RMU>(H, lthr, me.x, me.y, mz, tx, ty, tzz, sx, sy, sz);
-                }
+                if constexpr (SPLIT)
+                  cell = exact_cell<RMU>(P, lthr, me.x, me.y, mz, tx, ty, tzz, sx, sy, sz);
+                else
+                  cell = exact_cell<RMU>(H, lthr, me.x, me.y, mz, tx, ty, tzz, sx, sy, sz);
               }
               if (__ballot(cell >= 0) != 0ull) {
                 const Gal gt{rl_d(cq.x, cq.y, 4 * t),     rl_d(cq.z, cq.w, 4 * t),
@@ -281,7 +290,15 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
                   } else {
                     pair_values<K, SPLIT>(gme, gt, H.gws, qv);
                   }
-                  if (cell != run_cell) {
+                  if constexpr (NS == 1) {
+                    // one slot (density pairs): the run restarts through selects, and only the
+                    // flush branches (no copies of the run at the join of two branches)
+                    const bool nw = cell != run_cell;
+                    if (nw && run_cell >= 0) flush_cm<NS, CUT>(mynum, myacc, lbase, gnum, gacc, ncell, run_cell, run_n, run);
+                    run[0] = (nw ? 0.0 : run[0]) + qv[0];
+                    run_n = (nw ? 0u : run_n) + 1u;
+                    run_cell = cell;
+                  } else if (cell != run_cell) {
                     if (run_cell >= 0) flush_cm<NS, CUT>(mynum, myacc, lbase, gnum, gacc, ncell, run_cell, run_n, run);
                     // the new run starts from this pair's values
 #pragma unroll
@@ -321,6 +338,9 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
       }
     }
   }
+  unsigned long long nre = nre_l;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) nre += __shfl_xor(nre, o);
   if (lane == 0) {
     if (ncand) atomicAdd(&P.counters[0], ncand);
     if (nre) atomicAdd(&P.counters[1], nre);

@@ -298,7 +298,15 @@ __global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide 
               const double Y = pi.y - po.y;
               double qv[NS];
               pair_values_v<K, SPLIT>(go, gi, P.gws, X, Y, qv);
-              if (cell != run_cell) {
+              if constexpr (NS == 1) {
+                // one slot (density pairs): the run restarts through selects, and only the flush
+                // branches (no copies of the run at the join of two branches)
+                const bool nw = cell != run_cell;
+                if (nw && run_cell >= 0) flush_cm<NS, CUT>(mynum, myacc, lbase, gnum, gacc, ncell, run_cell, run_n, run);
+                run[0] = (nw ? 0.0 : run[0]) + qv[0];
+                run_n = (nw ? 0u : run_n) + 1u;
+                run_cell = cell;
+              } else if (cell != run_cell) {
                 if (run_cell >= 0) flush_cm<NS, CUT>(mynum, myacc, lbase, gnum, gacc, ncell, run_cell, run_n, run);
                 // the new run starts from this pair's values
 #pragma unroll
