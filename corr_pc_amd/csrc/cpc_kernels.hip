@@ -122,8 +122,7 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
   const int tmode = it.tmode, treg = it.treg;
   unsigned long long *const gnum = P.accnum + (size_t)it.target * ncell;
   double *const gacc = P.acc + (size_t)it.target * NS * ncell;
-  unsigned long long ncand = 0;  // wave total
-  unsigned nre_l = 0;            // this lane's FP64 re-evaluations
+  unsigned long long ncand = 0, nre = 0;  // wave totals
   int next_nx = 0;
 
   // chunks come sorted by decreasing cost; each wave starts on its own and then takes the next
@@ -251,7 +250,8 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
                 const bool out = (!near_pi && !pi_in) || (s32 < H.s_out_lo) || (s32 >= H.s_out_hi);
                 // s32 >= s_cert_lo > 0 wherever cert can hold, so kf and gk are finite there
                 const float rinv = __builtin_amdgcn_rsqf(s32);
-                const float kf = fmaf(__log2f(s32), H.la, lb_v);  // one rounding fewer than the guard band allows for
+                // one FMA where lb sits in a VGPR (one rounding fewer than the guard band allows for)
+                const float kf = NS < 7 ? fmaf(__log2f(s32), H.la, lb_v) : __log2f(s32) * H.la + H.lb;
                 const float fk = floorf(kf);
                 const float frk = kf - fk;
                 const float gk = fmaf(fmaf(H.gC, rinv, gB_v), rinv, H.gA);
@@ -261,17 +261,22 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
                   cell = (int)fmaf(fk, (float)H.n_pi, fu);
               }
               if (AUTO) ncand += popc_ballot(ok);
-              // a plain branch: the wave skips it when no lane is uncertain, and counts per lane
-              // (a ballot of a lane mask would be rebuilt from a 0/1 vector on every tile)
-              if (unc) {
-                if (SPLIT) ++nre_l;
+              // the small kinds take a plain branch, which the wave skips when no lane is uncertain,
+              // and count the lanes inside from the execution mask (a ballot of a lane mask is
+              // rebuilt from a 0/1 vector on every tile); the shape kinds keep the wave-uniform
+              // test, with which their register allocation spills nothing inside the tile loops
+              constexpr bool kPlain = NS < 7;
+              if (kPlain ? unc : (__ballot(unc) != 0ull)) {
+                if (SPLIT) nre += kPlain ? (unsigned long long)__popcll(__activemask()) : popc_ballot(unc);
                 const double tx = rl_d(cq.x, cq.y, 4 * t), ty = rl_d(cq.z, cq.w, 4 * t);
                 const double tzz = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(cz), t),
                                                     __builtin_amdgcn_readlane(__double2loint(cz), t));
-                if constexpr (SPLIT)
-                  cell = exact_cell<RMU>(P, lthr, me.x, me.y, mz, tx, ty, tzz, sx, sy, sz);
-                else
-                  cell = exact_cell<RMU>(H, lthr, me.x, me.y, mz, tx, ty, tzz, sx, sy, sz);
+                if (unc) {
+                  if constexpr (SPLIT)
+                    cell = exact_cell<RMU>(P, lthr, me.x, me.y, mz, tx, ty, tzz, sx, sy, sz);
+                  else
+                    cell = exact_cell<RMU>(H, lthr, me.x, me.y, mz, tx, ty, tzz, sx, sy, sz);
+                }
               }
               if (__ballot(cell >= 0) != 0ull) {
                 const Gal gt{rl_d(cq.x, cq.y, 4 * t),     rl_d(cq.z, cq.w, 4 * t),
@@ -338,9 +343,6 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
       }
     }
   }
-  unsigned long long nre = nre_l;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) nre += __shfl_xor(nre, o);
   if (lane == 0) {
     if (ncand) atomicAdd(&P.counters[0], ncand);
     if (nre) atomicAdd(&P.counters[1], nre);
