@@ -62,9 +62,11 @@ using CKArgs = __attribute__((address_space(4))) const KArgs;
 // guard band in FP64; without it every candidate takes the FP64 path.  AUTO compiles in corels1's
 // order predicate (auto_mode 1 and 2).  RMU compiles in the r-mu binning of coordinates 7 (with
 // SPLIT false only: r-mu candidates all take the FP64 path).  Kinds, AUTO and RMU are separate
-// instantiations so that each keeps only the registers and branches it needs.
+// instantiations so that each keeps only the registers and branches it needs.  Occupancy: the split
+// kernels keep 4 waves per SIMD (128 registers), the split density kernel 5 (96 registers: with
+// 256-thread groups a CU then holds 5 groups), the FP64 kernels 2.
 template <int K, bool SPLIT, bool AUTO, bool RMU>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT ? 4 : 2)))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SPLIT ? (KindNV<K>::NV == 0 ? 5 : 4) : 2)))
 pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
   constexpr int NV = KindNV<K>::NV;
   constexpr int NE = KindNV<K>::NE;
