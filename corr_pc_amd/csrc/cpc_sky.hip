@@ -61,9 +61,14 @@ __device__ __forceinline__ int rp_bin_sky(const SkyArgs &P, const double *thr, d
 }
 
 // GEN compiles in the other coordinates' pi and rp (SkyArgs pim, rpm, rmu); without it the kernel is
-// that of coordinates 0, whose registers allow two waves per SIMD.
+// that of coordinates 0, whose registers allow two waves per SIMD (three for density pairs).  The
+// split density kernel is held to 128 registers for four waves per SIMD: its few spills sit in the
+// FP64 path of the uncertain candidates (device cos and acos), and it runs 10% faster than at 168
+// registers and three waves; the shape kinds spill inside the tile loop at three waves and run 27%
+// slower, so they keep the compiler's choice.
 template <int K, bool SPLIT, bool GEN>
-__global__ void __launch_bounds__(256) sky_kernel(SkyArgs P, SkySide B, SkySide T) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((SPLIT && !GEN && K == kDD) ? 4 : 1)))
+sky_kernel(SkyArgs P, SkySide B, SkySide T) {
   const PRec *__restrict__ brec = B.rec;
   const SRec *__restrict__ bsr = B.sr;
   const Win *__restrict__ bwin = B.win;
