@@ -819,6 +819,31 @@ Status side_tables(cpc_ctx *, Side &S, size_t ncells) {
   return {};
 }
 
+// Descending order of cost in 1/16 steps of its binary logarithm, ties in input order: a stable
+// counting sort over the steps the vector uses (the order run_box gives the items of several plans).
+int cost_step(double c) {
+  const double l = c > 0 ? std::log2(c) * 16.0 : 0.0;
+  return (int)std::min(4095.0, std::max(0.0, l));
+}
+
+template <typename T>
+void sort_by_cost_desc(std::vector<std::pair<double, T>> &v) {
+  if (v.size() < 2) return;
+  std::vector<int> st(v.size());
+  int lo = 4095, hi = 0;
+  for (size_t i = 0; i < v.size(); ++i) {
+    st[i] = cost_step(v[i].first);
+    lo = std::min(lo, st[i]);
+    hi = std::max(hi, st[i]);
+  }
+  std::vector<size_t> pos(hi - lo + 2, 0);  // pos[hi - s + 1]: items of steps above s
+  for (int x : st) ++pos[hi - x + 1];
+  for (size_t b = 1; b < pos.size(); ++b) pos[b] += pos[b - 1];
+  std::vector<std::pair<double, T>> out(v.size());
+  for (size_t i = 0; i < v.size(); ++i) out[pos[hi - st[i]]++] = v[i];
+  v.swap(out);
+}
+
 // Work items of a pass, with the jackknife decomposed by the tile side so that every pair is
 // evaluated once.  A pair of an outer galaxy (region a, coin ci) and an inner galaxy (region b, coin
 // cj) touches region a when ci >= 0.5, or in the box also when b == a (`same_touches`: jk_check_PB
@@ -833,8 +858,8 @@ Status side_tables(cpc_ctx *, Side &S, size_t ncells) {
 //           tiles of region b != a, coin 1   -> target Y(a, 0, b) (touches b only)
 // Y targets follow the 2 nreg others in first-use order; `ytar` receives (a, x, b) of each, x the
 // flag of touching a.  Without the jackknife every item takes every tile group into target 1.
-// Chunks of one (target, filter) are sorted by cost and packed `per_item` at a time; items run
-// most expensive first.  The cost of a chunk is its galaxies times the tile galaxies its filter
+// Chunks of one (target, filter) are ordered by cost (sort_by_cost_desc) and packed `per_item` at a
+// time; items run most expensive first.  The cost of a chunk is its galaxies times the tile galaxies its filter
 // takes in the neighbour columns.
 template <typename ForNbr>
 void make_items_jk(const Side &B, const Side &T, bool jk_on, bool same_touches, int nreg, int per_item,
@@ -906,38 +931,42 @@ void make_items_jk(const Side &B, const Side &T, bool jk_on, bool same_touches, 
       }
     }
   }
-  struct Packed {
-    double cost;
-    Key key;
-    std::vector<Chunk> cs;
-  };
-  // each key's chunks sorted by cost and packed into items on host threads, concatenated in key order
+  // each key's chunks ordered by cost and packed into items on host threads; then every item, most
+  // expensive first
   std::vector<std::pair<const Key, std::vector<std::pair<double, Chunk>>> *> keys;
   for (auto &kv : by_key) keys.push_back(&kv);
-  std::vector<std::vector<Packed>> per_key(keys.size());
+  std::vector<std::vector<Chunk>> kchunks(keys.size());
+  std::vector<std::vector<double>> kcost(keys.size());  // per item of the key
   parallel_for((int64_t)keys.size(), threads, [&](int64_t k) {
     auto &v = keys[k]->second;
-    std::stable_sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+    sort_by_cost_desc(v);
+    std::vector<Chunk> &cs = kchunks[k];
+    cs.reserve(v.size());
     for (size_t a = 0; a < v.size(); a += per_item) {
-      Packed pk{0.0, keys[k]->first, {}};
+      double c = 0.0;
       for (size_t b = a; b < std::min(v.size(), a + per_item); ++b) {
-        pk.cost += v[b].first;
-        pk.cs.push_back(v[b].second);
+        c += v[b].first;
+        cs.push_back(v[b].second);
       }
-      per_key[k].push_back(std::move(pk));
+      kcost[k].push_back(c);
     }
   });
-  std::vector<Packed> packed;
-  for (auto &v : per_key)
-    for (auto &pk : v) packed.push_back(std::move(pk));
-  std::stable_sort(packed.begin(), packed.end(), [](const Packed &x, const Packed &y) { return x.cost > y.cost; });
+  std::vector<std::pair<double, int2>> order;  // (cost, (key, item of the key))
+  for (size_t k = 0; k < keys.size(); ++k)
+    for (size_t j = 0; j < kcost[k].size(); ++j) order.push_back({kcost[k][j], make_int2((int)k, (int)j)});
+  sort_by_cost_desc(order);
   items.clear();
   costs.clear();
-  for (const Packed &pk : packed) {
-    items.push_back(Item{(int32_t)chunks.size(), (int32_t)pk.cs.size(), pk.key.target, pk.cs[0].group, 0,
-                         pk.key.mode, pk.key.reg, 0});
-    costs.push_back(pk.cost);
-    chunks.insert(chunks.end(), pk.cs.begin(), pk.cs.end());
+  items.reserve(order.size());
+  costs.reserve(order.size());
+  for (const auto &o : order) {
+    const int k = o.second.x, j = o.second.y;
+    const std::vector<Chunk> &cs = kchunks[k];
+    const size_t a = (size_t)j * per_item, e = std::min(cs.size(), a + per_item);
+    const Key &key = keys[k]->first;
+    items.push_back(Item{(int32_t)chunks.size(), (int32_t)(e - a), key.target, cs[a].group, 0, key.mode, key.reg, 0});
+    costs.push_back(o.first);
+    chunks.insert(chunks.end(), cs.begin() + a, cs.begin() + e);
   }
 }
 
@@ -1869,16 +1898,8 @@ Status run_box(cpc_ctx *ctx, const std::vector<Plan *> &plans) {
     }
     // items of several plans, most expensive first (the grid runs in that order): a stable counting
     // sort on the cost's binary logarithm in 1/16 steps, ties in plan order
-    auto bucket = [](double c) {
-      const double l = c > 0 ? std::log2(c) * 16.0 : 0.0;
-      return (int)std::min(4095.0, std::max(0.0, l));
-    };
-    std::vector<int64_t> cnt(4097, 0);
-    for (const auto &x : v) ++cnt[4095 - bucket(x.first) + 1];
-    for (int b = 1; b <= 4096; ++b) cnt[b] += cnt[b - 1];
-    const size_t base = all.size();
-    all.resize(base + v.size());
-    for (const auto &x : v) all[base + cnt[4095 - bucket(x.first)]++] = x.second;
+    sort_by_cost_desc(v);
+    for (const auto &x : v) all.push_back(x.second);
   }
   // every plan's jackknife reduction and sparse-window count, one launch each: the tables and their
   // block offsets in one device region

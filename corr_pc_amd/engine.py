@@ -216,7 +216,35 @@ class Histogram:
         R = 1 + p.n_jk if (p.do_jk == 1 and p.n_jk > 0) else 1
         return cls(R, p.n_rp, p.n_pi)
 
+    @classmethod
+    def many(cls, params_list):
+        """Histograms of many passes as views of two zeroed buffers (one int64, one float64), with
+        their addresses kept for c_struct: one allocation for a whole batch."""
+        shapes = [(1 + p.n_jk if (p.do_jk == 1 and p.n_jk > 0) else 1, p.n_rp, p.n_pi) for p in params_list]
+        ibuf = np.zeros(sum(R * r * q + R * r for R, r, q in shapes), np.int64)
+        fbuf = np.zeros(sum(R * r * q * (1 + 2 * NFIELD) for R, r, q in shapes), np.float64)
+        ib, fb = ibuf.ctypes.data, fbuf.ctypes.data
+        out, io, fo = [], 0, 0
+        for R, r, q in shapes:
+            h = cls.__new__(cls)
+            h.R, h.n_rp, h.n_pi = R, r, q
+            n = R * r * q
+            h.num = ibuf[io:io + n].reshape(R, r, q)
+            h.num_rp = ibuf[io + n:io + n + R * r].reshape(R, r)
+            h.wt_num = fbuf[fo:fo + n].reshape(R, r, q)
+            h.val = fbuf[fo + n:fo + n * (1 + NFIELD)].reshape(R, r, q, NFIELD)
+            h.err = fbuf[fo + n * (1 + NFIELD):fo + n * (1 + 2 * NFIELD)].reshape(R, r, q, NFIELD)
+            h._addr = (ib + 8 * io, fb + 8 * fo, fb + 8 * (fo + n), fb + 8 * (fo + n * (1 + NFIELD)), ib + 8 * (io + n))
+            io += n + R * r
+            fo += n * (1 + 2 * NFIELD)
+            out.append(h)
+        return out
+
     def c_struct(self):
+        a = getattr(self, "_addr", None)
+        if a is not None:
+            return HistC(ctypes.cast(a[0], _lp), ctypes.cast(a[1], _dp), ctypes.cast(a[2], _dp), ctypes.cast(a[3], _dp),
+                         ctypes.cast(a[4], _lp))
         return HistC(self.num.ctypes.data_as(_lp), self.wt_num.ctypes.data_as(_dp), self.val.ctypes.data_as(_dp),
                      self.err.ctypes.data_as(_dp), self.num_rp.ctypes.data_as(_lp))
 
@@ -444,10 +472,11 @@ class Engine:
         n = len(jobs)
         arr = (JobC * max(n, 1))()
         keep = []
+        hists = Histogram.many([params for _, _, params in jobs])
         for k, (outer, inner, params) in enumerate(jobs):
             ho = self.upload(outer)
             hi = ho if inner is outer else self.upload(inner)
-            hist = Histogram.for_params(params)
+            hist = hists[k]
             hc = hist.c_struct()
             st = Stats()
             keep.append((params, hist, hc, st))

@@ -183,3 +183,27 @@ def test_box_rr_analytic():
     assert rr[k, q] == math.pi * (re_[k + 1] ** 2 - re_[k] ** 2) * (pe[q + 1] - pe[q]) / L ** 3
     total = math.pi * (re_[-1] ** 2 - re_[0] ** 2) * (pe[-1] - pe[0]) / L ** 3
     assert rr.sum() == pytest.approx(total, rel=1e-12)
+
+
+def test_batch_histograms_share_one_buffer_without_overlap():
+    """Histogram.many: per pass the shapes of Histogram.for_params, C-contiguous views whose address
+    ranges (the pointers c_struct hands to the library) are disjoint and zeroed."""
+    from corr_pc_amd.engine import Histogram
+    ps = [make_params(100.0, 5, 0.1, 10.0, 4, -4.0, 4.0, do_jk=1, n_jk=8),
+          make_params(100.0, 3, 0.1, 10.0, 6, -6.0, 6.0),
+          make_params(100.0, 7, 0.1, 10.0, 2, -2.0, 2.0, do_jk=1, n_jk=27)]
+    hs = Histogram.many(ps)
+    spans = []
+    for p, h in zip(ps, hs):
+        ref = Histogram.for_params(p)
+        for name in ("num", "wt_num", "val", "err", "num_rp"):
+            a, b = getattr(h, name), getattr(ref, name)
+            assert a.shape == b.shape and a.dtype == b.dtype and a.flags["C_CONTIGUOUS"]
+            assert not a.any()
+            spans.append((a.ctypes.data, a.ctypes.data + a.nbytes))
+        c = h.c_struct()
+        assert ctypes.cast(c.num, ctypes.c_void_p).value == h.num.ctypes.data
+        assert ctypes.cast(c.err, ctypes.c_void_p).value == h.err.ctypes.data
+        assert ctypes.cast(c.num_rp, ctypes.c_void_p).value == h.num_rp.ctypes.data
+    spans.sort()
+    assert all(e <= s for (_, e), (s, _) in zip(spans, spans[1:]))
