@@ -31,8 +31,10 @@ FLOPS_PER_PAIR = {"DD": 12, "SD": 33, "SS": 53}
 PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "pair_kernel_ss_split_box1m_traffic.md")
+# the box kernel's sources and the host code that decomposes the pass into work items (their order
+# changes which tile records the caches hold, hence the traffic)
 KERNEL_SOURCES = [os.path.join(ROOT, "corr_pc_amd", "csrc", f) for f in ("cpc_kernels.hip", "cpc_device.h",
-                                                                          "cpc_internal.h")]
+                                                                          "cpc_internal.h", "cpc_api.cpp")]
 
 
 def log(msg):
@@ -40,8 +42,8 @@ def log(msg):
 
 
 def kernel_source_sha():
-    """First 16 hex digits of the SHA-256 of the box kernel's sources, as the traffic summary records
-    the sources it was measured on."""
+    """First 16 hex digits of the SHA-256 of the box kernel's sources and of the host code that
+    decomposes the pass, as the traffic summary records the sources it was measured on."""
     h = hashlib.sha256()
     for f in KERNEL_SOURCES:
         with open(f, "rb") as fh:
