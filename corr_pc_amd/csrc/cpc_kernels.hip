@@ -125,6 +125,7 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
   unsigned long long *const gnum = P.accnum + (size_t)it.target * ncell;
   double *const gacc = P.acc + (size_t)it.target * NS * ncell;
   unsigned long long ncand = 0, nre = 0;  // wave totals
+  unsigned nre_l = 0;  // small kinds: this lane's FP64 re-evaluations (summed over the wave at the end)
   int next_nx = 0;
 
   // chunks come sorted by decreasing cost; each wave starts on its own and then takes the next
@@ -264,12 +265,18 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
               }
               if (AUTO) ncand += popc_ballot(ok);
               // the small kinds take a plain branch, which the wave skips when no lane is uncertain,
-              // and count the lanes inside from the execution mask (a ballot of a lane mask is
-              // rebuilt from a 0/1 vector on every tile); the shape kinds keep the wave-uniform
-              // test, with which their register allocation spills nothing inside the tile loops
+              // and count per lane (a ballot of a lane mask is rebuilt from a 0/1 vector on every
+              // tile; a wave total updated inside the branch would become a per-lane value); the
+              // shape kinds keep the wave-uniform test, with which their register allocation spills
+              // nothing inside the tile loops
               constexpr bool kPlain = NS < 7;
               if (kPlain ? unc : (__ballot(unc) != 0ull)) {
-                if (SPLIT) nre += kPlain ? (unsigned long long)__popcll(__activemask()) : popc_ballot(unc);
+                if (SPLIT) {
+                  if (kPlain)
+                    ++nre_l;
+                  else
+                    nre += popc_ballot(unc);
+                }
                 const double tx = rl_d(cq.x, cq.y, 4 * t), ty = rl_d(cq.z, cq.w, 4 * t);
                 const double tzz = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(cz), t),
                                                     __builtin_amdgcn_readlane(__double2loint(cz), t));
@@ -344,6 +351,12 @@ pair_kernel(const KArgs *__restrict__ jobs, const Item *__restrict__ items) {
         unsafeAtomicAdd(&gacc[(size_t)s * ncell + lbase + c], v);
       }
     }
+  }
+  if constexpr (NS < 7) {
+    unsigned long long t = nre_l;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+    nre += t;
   }
   if (lane == 0) {
     if (ncand) atomicAdd(&P.counters[0], ncand);

@@ -197,3 +197,31 @@ def test_pi_window_wider_than_the_box(engine, oracle_lib, prec):
     _run(engine, oracle_lib, A, B, p, f"wide pi window cross prec={prec}")
     p.auto_pairs = 1
     _run(engine, oracle_lib, A, A, p, f"wide pi window auto prec={prec}")
+
+
+def test_candidate_counters_do_not_depend_on_the_kind(engine):
+    """The split classification (candidate test, guard bands) is the same for every pair kind, so one
+    pass counted as density, shape-density and shape-shape pairs must report the same candidates and
+    binned pairs, and the same FP64 re-evaluations up to the rounding of the log-bin estimate (one
+    FMA in the density kernel, a product and a sum in the shape kinds: a candidate at the guard
+    band's edge may fall on either side); the FP64 kernel re-evaluates nothing it counts there.  Also
+    a regression test for a re-evaluation count that was kept per wave inside a divergent branch
+    (it read a small fraction of the true count)."""
+    L = 120.0
+    A = small_catalog(20000, L, 21, clumps=80)
+    B = small_catalog(20000, L, 22, clumps=80)
+    seen = {}
+    for kind in (KIND_DD, KIND_SHAPE_DENSITY, KIND_SHAPE_SHAPE):
+        p = make_params(L, 12, 0.05, 20.0, 16, -20.0, 20.0, kind=kind, g_with_shape=2, do_jk=1, n_jk=8,
+                        precision=PREC_SPLIT)
+        _, st = engine.count(A, B, p)
+        seen[kind] = (st["n_candidates"], st["n_fp64_reeval"], st["n_binned"])
+        print(f"[counters] kind={kind}: {seen[kind]}", flush=True)
+    assert len({(c, b) for c, _, b in seen.values()}) == 1, seen
+    assert seen[KIND_SHAPE_DENSITY][1] == seen[KIND_SHAPE_SHAPE][1], seen  # the same arithmetic
+    re_dd, re_ss = seen[KIND_DD][1], seen[KIND_SHAPE_SHAPE][1]
+    assert re_ss > 0  # the guard band is met at this size
+    assert abs(re_dd - re_ss) <= max(2, 2e-3 * re_ss), seen
+    p = make_params(L, 12, 0.05, 20.0, 16, -20.0, 20.0, kind=KIND_DD, do_jk=1, n_jk=8, precision=PREC_FP64)
+    _, st = engine.count(A, B, p)
+    assert st["n_candidates"] == seen[KIND_DD][0] and st["n_fp64_reeval"] == 0
