@@ -1,4 +1,4 @@
-"""The GPU box's own arithmetic that bin assignment rests on.
+"""The arithmetic of the machine that runs the kernels, on which bin assignment rests.
 
 * The host C library of the machine that drives the GPU: its log10 decides the rp thresholds
   (corels.cpp:34) and its rand() the jackknife coins (jk.cpp:40-43, read_dat.cpp:243); both are
