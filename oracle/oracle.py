@@ -12,7 +12,8 @@ import numpy as np
 from corr_pc_amd.engine import CPC_OK, CatalogDesc, EngineError, Histogram, HistC, Params, Stats
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcpc_oracle.so")
+# CPC_ORACLE_LIB selects another build of the same source (the sanitizer build of oracle/Makefile)
+LIB_PATH = os.environ.get("CPC_ORACLE_LIB") or os.path.join(HERE, "libcpc_oracle.so")
 
 _lib = None
 
