@@ -74,5 +74,23 @@ def build_program(verbose=False):
     return PROGRAM
 
 
+def build_program_sanitized(verbose=False):
+    """The corr_pc program built with AddressSanitizer and UndefinedBehaviorSanitizer (host code
+    only; the library it calls is the ordinary build) as corr_pc_amd/_asan/corr_pc, for the CPU tests
+    of its reader and planner.  Not part of build()."""
+    out_dir = os.path.join(HERE, "_asan")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "corr_pc")
+    tmp = out + ".tmp"
+    cmd = [CXX, "-O1", "-g", "-std=c++17", "-Wall", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+           "-fno-sanitize-recover=undefined", "-o", tmp, PROGRAM_SOURCE, "-L" + HERE, "-lcpc",
+           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

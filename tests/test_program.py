@@ -24,7 +24,9 @@ from corr_pc_amd.engine import KIND_DD, KIND_SHAPE_DENSITY, KIND_SHAPE_SHAPE, ra
 from corr_pc_amd.passes import DataInfo, setup_pass
 import program_ref
 
-PROG = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "corr_pc_amd", "corr_pc")
+# CPC_PROG selects another build of the program (the sanitizer build of corr_pc_amd/build.py)
+PROG = os.environ.get("CPC_PROG") or os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                                  "corr_pc_amd", "corr_pc")
 KINDS = {"dd": KIND_DD, "shape_density": KIND_SHAPE_DENSITY, "shape_shape": KIND_SHAPE_SHAPE}
 
 

@@ -1,9 +1,14 @@
-"""The oracle's C restatement under AddressSanitizer and UndefinedBehaviorSanitizer.
+"""Host code under AddressSanitizer and UndefinedBehaviorSanitizer.
 
-oracle/Makefile's `asan` target builds the same source with both sanitizers (host code only); the
-oracle's own test files then run in a child process that loads that build (CPC_ORACLE_LIB) with the
-ASan runtime preloaded into the Python interpreter.  Any out-of-bounds access, use after free or
-undefined behaviour aborts the child, and its report fails this test.
+* The oracle's C restatement: oracle/Makefile's `asan` target builds the same source with both
+  sanitizers; the oracle's own test files then run in a child process that loads that build
+  (CPC_ORACLE_LIB) with the ASan runtime preloaded into the Python interpreter.
+* The corr_pc program's reader and planner (cpc_main.cpp, the code that parses .inp and catalog
+  files): corr_pc_amd/build.py builds it with both sanitizers, and the program's CPU tests
+  (tests/test_program.py, `--plan`, which stops before the GPU) run against that build (CPC_PROG).
+
+Any out-of-bounds access, use after free or undefined behaviour aborts the child, and its report
+fails the test.
 """
 import os
 import subprocess
@@ -56,6 +61,24 @@ def test_oracle_under_address_and_ub_sanitizers():
                    if f.startswith("test_oracle") and f.endswith(".py"))
     r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider", "-m", "not gpu"] + files,
                        capture_output=True, text=True, env=env, cwd=ROOT, timeout=900)
+    out = r.stdout + r.stderr
+    assert "AddressSanitizer" not in out and "runtime error" not in out, out[-4000:]
+    assert r.returncode == 0, out[-4000:]
+    print(out.strip().splitlines()[-1])
+
+
+def test_program_reader_under_address_and_ub_sanitizers():
+    if _asan_runtime() is None:
+        pytest.skip("gcc has no AddressSanitizer runtime here")
+    from corr_pc_amd.build import LIB, build_program_sanitized
+    if not os.path.exists(LIB):
+        pytest.skip("the library is not built")
+    prog = build_program_sanitized()
+    env = dict(os.environ, CPC_PROG=prog, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+    r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider", "-m", "not gpu",
+                        os.path.join("tests", "test_program.py")], capture_output=True, text=True, env=env, cwd=ROOT,
+                       timeout=900)
     out = r.stdout + r.stderr
     assert "AddressSanitizer" not in out and "runtime error" not in out, out[-4000:]
     assert r.returncode == 0, out[-4000:]
