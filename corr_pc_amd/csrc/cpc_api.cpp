@@ -469,6 +469,8 @@ struct cpc_ctx {
   char *scratch = nullptr;  // this device's copy of the no-alloc scratch array (hipGetSymbolAddress)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  hipEvent_t evc0 = nullptr, evc1 = nullptr;  // around the batched cell-list build (ms_cells)
+  bool cells_timed = false;
   std::string err;
   size_t max_lds = 65536;
   int n_cu = 256;
@@ -789,6 +791,7 @@ Status build_queued_sides(cpc_ctx *ctx) {
       check(ctx->sv_a.ensure(total), st, "alloc") || check(ctx->sv_b.ensure(total), st, "alloc") ||
       check(ctx->bounds_all.ensure(nb), st, "alloc") || check(ctx->side_tab.ensure(bytes), st, "alloc"))
     return st;
+  check(hipEventRecord(ctx->evc0, s), st, "event");
   check(hipMemcpyAsync(ctx->side_tab.p, h.data(), bytes, hipMemcpyHostToDevice, s), st, "copy");
   const char *tab = ctx->side_tab.p;
   check(build_sides_many(reinterpret_cast<const SideBuild *>(tab), n, reinterpret_cast<const int64_t *>(tab + o_el),
@@ -800,6 +803,8 @@ Status build_queued_sides(cpc_ctx *ctx) {
   if (check(ctx->hb_bounds.ensure(nb * sizeof(int4)), st, "host alloc")) return st;
   const int4 *hb = reinterpret_cast<const int4 *>(ctx->hb_bounds.p);
   check(hipMemcpyAsync(ctx->hb_bounds.p, ctx->bounds_all.p, nb * sizeof(int4), hipMemcpyDeviceToHost, s), st, "copy");
+  check(hipEventRecord(ctx->evc1, s), st, "event");
+  ctx->cells_timed = !st.code;
   for (int k = 0; k < n; ++k) ctx->side_q_side[k]->hb = hb + q[k].boff;
   q.clear();
   ctx->side_q_side.clear();
@@ -1186,6 +1191,7 @@ struct Plan {
   size_t off_p = 0, off_pn = 0, off_c = 0;
   // results
   double ms_build = 0;
+  float ms_cells = 0;                   // device time of the cell-list build (first plan of a call)
   float kms = 0, kms_p = 0, kms_r = 0;  // pass, pair kernel, reduction
   std::vector<uint2> hq_pairs;
 };
@@ -2207,6 +2213,7 @@ int64_t assemble(const Plan &P, const std::vector<double> &acc, const std::vecto
     st->ms_primary = P.kms_p;
     st->ms_reduce = P.kms_r;
     st->ms_build = P.ms_build;
+    st->ms_cells = P.ms_cells;
     st->n_items_primary = (int64_t)P.items_p.size();
     st->n_host_pairs = (int64_t)P.hq_pairs.size();
     st->ms_total = ms_total;
@@ -2239,6 +2246,7 @@ cpc_status run_jobs(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *to
     Status st = setup(ctx, plans[i]);
     if (st.code) return fail(ctx, tag(i, st));
   }
+  ctx->cells_timed = false;
   if (n_jobs > 0) {
     auto t_b = std::chrono::steady_clock::now();
     Status st = build_queued_sides(ctx);
@@ -2265,6 +2273,7 @@ cpc_status run_jobs(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *to
     }
     check(hipStreamSynchronize(ctx->stream), st, "sync");
     if (st.code) return fail(ctx, st);
+    if (ctx->cells_timed && n_jobs > 0) (void)hipEventElapsedTime(&plans[0].ms_cells, ctx->evc0, ctx->evc1);
     std::vector<Status> sts(pend.size());
     parallel_for((int64_t)pend.size(), host_threads(),
                  [&](int64_t k) { sts[k] = finish_side(ctx, *pend[k].first, pend[k].second); });
@@ -2351,6 +2360,7 @@ cpc_status run_jobs(cpc_ctx *ctx, int n_jobs, const cpc_job *jobs, cpc_stats *to
       t.n_items_primary += (int64_t)P.items_p.size();
       t.n_host_pairs += (int64_t)P.hq_pairs.size();
       t.ms_build += P.ms_build;
+      t.ms_cells += P.ms_cells;
       if (P.sky) {
         t.ms_kernel += P.kms;
         t.ms_primary += P.kms_p;
@@ -2400,7 +2410,8 @@ cpc_status cpc_init(cpc_ctx **out, int device) {
   }
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
-      hipEventCreate(&ctx->ev2) != hipSuccess) {
+      hipEventCreate(&ctx->ev2) != hipSuccess || hipEventCreate(&ctx->evc0) != hipSuccess ||
+      hipEventCreate(&ctx->evc1) != hipSuccess) {
     delete ctx;
     return CPC_ERR_DEVICE;
   }
@@ -2448,6 +2459,8 @@ void cpc_destroy(cpc_ctx *ctx) {
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
   (void)hipEventDestroy(ctx->ev2);
+  (void)hipEventDestroy(ctx->evc0);
+  (void)hipEventDestroy(ctx->evc1);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

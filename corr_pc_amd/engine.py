@@ -72,7 +72,7 @@ class Stats(ctypes.Structure):
     _fields_ = [("n_binned", c_int64), ("n_candidates", c_int64), ("n_fp64_reeval", c_int64),
                 ("n_sparse_windows", c_int64), ("ms_total", c_double), ("ms_kernel", c_double),
                 ("ms_primary", c_double), ("ms_reduce", c_double), ("ms_build", c_double),
-                ("n_items_primary", c_int64), ("n_host_pairs", c_int64)]
+                ("n_items_primary", c_int64), ("n_host_pairs", c_int64), ("ms_cells", c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

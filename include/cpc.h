@@ -177,6 +177,9 @@ typedef struct cpc_stats {
   double ms_build;          /* wall time of the cell-list construction */
   int64_t n_items_primary;  /* work items (workgroups) of the primary kernel */
   int64_t n_host_pairs;     /* sky: guard-band pairs binned on the host with the host libm */
+  double ms_cells;          /* device time of the cell-list construction (HIP events around its key, sort,
+                               bounds and gather kernels and the copies of its tables); in a batch the
+                               first job's stats and the total carry it */
 } cpc_stats;
 
 typedef struct cpc_ctx cpc_ctx;

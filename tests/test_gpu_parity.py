@@ -215,6 +215,7 @@ def test_candidate_counters_do_not_depend_on_the_kind(engine):
         p = make_params(L, 12, 0.05, 20.0, 16, -20.0, 20.0, kind=kind, g_with_shape=2, do_jk=1, n_jk=8,
                         precision=PREC_SPLIT)
         _, st = engine.count(A, B, p)
+        assert st["ms_cells"] > 0 and st["ms_kernel"] > 0  # the cell lists and the kernels are timed
         seen[kind] = (st["n_candidates"], st["n_fp64_reeval"], st["n_binned"])
         print(f"[counters] kind={kind}: {seen[kind]}", flush=True)
     assert len({(c, b) for c, _, b in seen.values()}) == 1, seen
