@@ -1330,9 +1330,12 @@ Status setup(cpc_ctx *ctx, Plan &P) {
   if (!P.thr) return st;
   const std::vector<double> &thr = *P.thr;
   // box kernel: up to 512 threads (the split kernel keeps 4 waves per SIMD with 8 waves sharing a
-  // histogram; the FP64 kernel needs more registers and runs 4 waves per group); sky kernel: up to 256
+  // histogram; the FP64 kernel needs more registers and runs 4 waves per group; the split density
+  // kernel runs 5 waves per SIMD, which 256-thread groups fill: 145 against 170 ms on the 1M-galaxy
+  // wgg pass); sky kernel: up to 256
+  const int box_threads = P.split ? (P.NSP == 1 ? 256 : 512) : 256;
   P.threads = sky ? std::min(ctx->block_threads ? ctx->block_threads : 256, 256)
-                  : (ctx->block_threads ? ctx->block_threads : (P.split ? 512 : 256));
+                  : (ctx->block_threads ? ctx->block_threads : box_threads);
   const size_t lds_p = pair_lds_bytes(ncell, P.NSP, p.n_rp);
   if (lds_p > ctx->max_lds)
     return {CPC_ERR_UNSUPPORTED, "histogram of " + std::to_string(ncell) + " cells does not fit in LDS (" +
