@@ -335,12 +335,14 @@ Guard rp_guard(const cpc_params &p, const Derived &dv, const std::vector<double>
   return g;
 }
 
-hipError_t check(hipError_t e, Status &st, const char *what) {
+// Records the first failing HIP call of a step in `st`; true when `e` is an error (so that steps
+// chain as `if (check(..) || check(..)) return st;` or run unconditionally and are tested once).
+bool check(hipError_t e, Status &st, const char *what) {
   if (e != hipSuccess && st.code == CPC_OK) {
     st.code = (e == hipErrorOutOfMemory) ? CPC_ERR_NOMEM : CPC_ERR_DEVICE;
     st.msg = std::string(what) + ": " + hipGetErrorString(e);
   }
-  return e;
+  return e != hipSuccess;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1321,7 +1323,6 @@ Status setup(cpc_ctx *ctx, Plan &P) {
   const cpc_catalog *co = P.co, *ci = P.ci;
   const bool sky = P.sky, jk_on = P.jk_on, autop = P.autop;
   const int nreg = P.nreg, ncell = P.ncell;
-  const Derived &dv = P.dv;
   JobBufs &B = *P.B;
   Status st;
   hipStream_t s = ctx->stream;
@@ -2064,7 +2065,7 @@ Status run_sky(cpc_ctx *ctx, Plan &P) {
 int64_t assemble(const Plan &P, const std::vector<double> &acc, const std::vector<unsigned long long> &accn,
                  double ms_total, int threads) {
   const cpc_params &p = P.p;
-  const int ncell = P.ncell, nreg = P.nreg, R = P.R, NV = P.NV, NE = P.NE, NSP = P.NSP, kc = P.kind_code;
+  const int ncell = P.ncell, R = P.R, NV = P.NV, NE = P.NE, NSP = P.NSP, kc = P.kind_code;
   const double *accp = acc.data() + P.off_p;
   const unsigned long long *accpn = accn.data() + P.off_pn;
   const unsigned long long *counters = accn.data() + P.off_c;

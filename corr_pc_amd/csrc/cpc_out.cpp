@@ -410,7 +410,6 @@ std::string jk_tag(int r) {
   return s.str();
 }
 
-Wt to_wt(const cpc_wsum &w) { return {w.val, w.err}; }
 cpc_wsum to_wsum(const Wt &w) { return {w.val, w.err}; }
 
 // The auto branch of do_bining (do_corr.cpp:220-324).
